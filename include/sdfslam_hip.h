@@ -1,0 +1,222 @@
+/*
+ * sdfslam_hip.h - C ABI of the MI355X (gfx950) backend for sdf-slam's joint
+ * Levenberg-Marquardt hot path (libsdfslam_hip.so).
+ *
+ * Plain C: pointers, sizes and status codes only. No exception crosses this
+ * boundary; the C++ shim (sdf-slam_amd/host/solver.cpp, SolveHip) maps the
+ * status codes back to the exception types the reference throws.
+ *
+ * Reference interfaces replaced (reference file:line):
+ *   sdfslam_hip_solve            <- SolveResult Solve(Problem&, const SolverOptions&)
+ *                                   src/solvers/solver.hpp:67 dispatching to
+ *                                   SolveLm src/solvers/lm_solver.cpp:201-434
+ *   sdfslam_problem_view         <- Problem getters src/problem/problem.hpp:104-142
+ *                                   (map(), poses(), point_specs(), eikonal_specs(),
+ *                                   odom_specs(), huber_delta(), smooth_gradient*(),
+ *                                   NodeColumn(), PoseColumn())
+ *   sdfslam_solver_options       <- SolverOptions src/solvers/solver.hpp:29-55
+ *   sdfslam_solve_result         <- SolveResult src/solvers/solver.hpp:57-64
+ *   stage entry points           <- the phases of one LM iteration,
+ *                                   src/solvers/lm_solver.cpp:232-364
+ */
+#ifndef SDFSLAM_HIP_H_
+#define SDFSLAM_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The C++ shim rethrows:
+ *   SDFSLAM_ERR_INVALID_ARGUMENT -> std::invalid_argument (problem.cpp:118-120)
+ *   SDFSLAM_ERR_NOT_SPD          -> std::runtime_error("sparse Cholesky factorization failed")
+ *                                   (lm_solver.cpp:354-355, 360-361)
+ *   everything else              -> std::runtime_error(sdfslam_hip_status_string(code)) */
+enum sdfslam_status {
+  SDFSLAM_OK = 0,
+  SDFSLAM_ERR_INVALID_ARGUMENT = 1,
+  SDFSLAM_ERR_NOT_SPD = 2,
+  SDFSLAM_ERR_HIP = 3,
+  SDFSLAM_ERR_OOM = 4,
+  SDFSLAM_ERR_NO_DEVICE = 5,
+  SDFSLAM_ERR_INTERNAL = 6
+};
+
+/* Grid geometry of GridMap (src/core/grid_map.hpp:11-33): node (w, h) has flat
+ * id h * nx + w; node position = (min_x + w * dx, min_y + h * dy). */
+typedef struct sdfslam_grid {
+  int32_t nx;
+  int32_t ny;
+  double min_x;
+  double min_y;
+  double dx;
+  double dy;
+} sdfslam_grid;
+
+/* Read-only view of a Problem (src/problem/problem.hpp:92-161) as structure of
+ * arrays. Rows keep the reference order: point rows (spec order), eikonal
+ * rows, then 3 rows per odometry/relation spec (lm_solver.cpp:209-211, 281).
+ * State layout: [active nodes in node-id order; 3 per pose for frames >= 1]
+ * (problem.hpp:125-139). */
+typedef struct sdfslam_problem_view {
+  sdfslam_grid grid;
+  int32_t num_poses;
+  int32_t num_active_nodes;
+  const int32_t* node_to_active; /* nx*ny entries; -1 = constant node (NodeColumn) */
+
+  int64_t num_points; /* PointSpec: problem.hpp:28-33 */
+  const int32_t* point_frame;
+  const double* point_x; /* point in the sensor frame */
+  const double* point_y;
+  const double* point_expected_sdf;
+  const double* point_sqrt_weight;
+
+  int64_t num_eikonal; /* EikonalSpec: problem.hpp:36-41 */
+  const int32_t* eik_w;
+  const int32_t* eik_h;
+  const double* eik_nx;
+  const double* eik_ny;
+  const double* eik_sqrt_weight;
+
+  int64_t num_odom; /* OdomSpec: problem.hpp:45-50 (odometry then relations) */
+  const int32_t* odom_i;
+  const int32_t* odom_j;
+  const double* odom_dx;
+  const double* odom_dy;
+  const double* odom_dtheta;
+  const double* odom_sqrt_weight;
+
+  double huber_delta;          /* problem.hpp:70, 115-121 */
+  int32_t smooth_gradient;     /* problem.hpp:76 */
+  double smooth_gradient_step; /* problem.hpp:81 */
+} sdfslam_problem_view;
+
+/* SolverOptions (solver.hpp:29-55). num_threads and linear_solver are accepted
+ * for symmetry with the CPU backends and do not change the device path. */
+typedef struct sdfslam_solver_options {
+  int32_t max_iterations;
+  double step_tolerance;
+  double lambda_init;
+  double lambda_factor;
+  int32_t reject_worse_steps;
+  int32_t marquardt_scaling;
+  int32_t trust_region;
+  int32_t num_threads;
+  int32_t linear_solver;
+} sdfslam_solver_options;
+
+/* SolveResult (solver.hpp:57-64). cost_history is caller-owned storage of
+ * cost_history_capacity doubles; the backend writes iterations + 1 entries
+ * (truncated to the capacity) and the full count to cost_history_length. */
+typedef struct sdfslam_solve_result {
+  int32_t converged;
+  int32_t iterations;
+  double initial_cost;
+  double final_cost;
+  double duration_seconds;
+  double* cost_history;
+  int32_t cost_history_capacity;
+  int32_t cost_history_length;
+} sdfslam_solve_result;
+
+/* Human-readable text for a status code (static storage). */
+const char* sdfslam_hip_status_string(int status);
+
+/* Message of the last error on the calling thread (static per thread). */
+const char* sdfslam_hip_last_error(void);
+
+/* Minimizes the problem in place (Solve/SolveLm, solver.hpp:67,
+ * lm_solver.cpp:201-434). map_values: nx*ny doubles (only active nodes
+ * change); poses: 3*num_poses doubles (x, y, theta; frame 0 is fixed). */
+int sdfslam_hip_solve(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
+                      double* map_values, double* poses, sdfslam_solve_result* result);
+
+/* ---- Stage entry points (device-resident context) --------------------------
+ * A context holds the problem, its state, the symbolic analysis and the
+ * workspaces on the current HIP device. The stage calls expose the phases of
+ * one LM iteration so each can be compared against the CPU path. */
+typedef struct sdfslam_hip_context sdfslam_hip_context;
+
+int sdfslam_hip_context_create(const sdfslam_problem_view* problem,
+                               const sdfslam_solver_options* options, const double* map_values,
+                               const double* poses, sdfslam_hip_context** out);
+int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx);
+int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, const double* poses);
+int sdfslam_hip_get_state(sdfslam_hip_context* ctx, double* map_values, double* poses);
+
+/* Sizes of the analysis: num_parameters, num_residuals, nnz of the lower
+ * triangle of the fixed J^T J pattern, supernodes, nnz of the supernodal factor
+ * (incl. explicit zeros), and the factorization flop count sum_j (c_j^2 + 3 c_j). */
+typedef struct sdfslam_hip_analysis_info {
+  int64_t num_parameters;
+  int64_t num_residuals;
+  int64_t pattern_nnz_lower;
+  int64_t num_supernodes;
+  int64_t factor_nnz;
+  double factor_flops;
+  int64_t update_doubles;
+  int32_t tree_levels;
+  int32_t pattern_epochs;
+} sdfslam_hip_analysis_info;
+int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* info);
+
+/* Residual vector and Jacobian rows at the current state
+ * (AssemblePointRows/AssembleEikonalRows/odometry tail, lm_solver.cpp:30-82,
+ * 275-296). Rows are in reference order; each row has 7 slots of
+ * (column, value) with column -1 for an empty slot, in the reference's triplet
+ * order within the row. residuals: num_residuals; j_cols/j_vals: 7*num_residuals. */
+int sdfslam_hip_assemble(sdfslam_hip_context* ctx, double* residuals, int32_t* j_cols,
+                         double* j_vals);
+
+/* Damped normal equations at the current state (ParallelAtA + damping +
+ * ParallelNegAtb, lm_solver.cpp:113-197, 319-331), lower triangle in the
+ * original column order as CSC over the fixed pattern: col_ptr (n+1), row_idx
+ * and values (pattern_nnz_lower), structural (pattern_nnz_lower bytes: 1 where
+ * the reference's J^T J has a structural entry, i.e. some row couples the two
+ * columns, or on the diagonal), rhs = -J^T r (n). */
+int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int32_t* col_ptr,
+                                 int32_t* row_idx, double* values, uint8_t* structural,
+                                 double* rhs);
+
+/* Solution of (J^T J + damping) step = -J^T r at the current state
+ * (lm_solver.cpp:351-364). Returns SDFSLAM_ERR_NOT_SPD on a zero pivot. */
+int sdfslam_hip_linear_step(sdfslam_hip_context* ctx, double lambda, double* step);
+
+/* Total weighted squared error at the current state (Problem::Cost,
+ * problem.cpp:132-156). */
+int sdfslam_hip_cost(sdfslam_hip_context* ctx, double* cost);
+
+/* Runs the LM loop on the context's device state (SolveLm,
+ * lm_solver.cpp:201-434); the state stays on the device. */
+int sdfslam_hip_context_solve(sdfslam_hip_context* ctx, const sdfslam_solver_options* options,
+                              sdfslam_solve_result* result);
+
+/* Runs exactly `iterations` LM iterations from the current device state,
+ * carrying the LM state (lambda, previous cost) in the context and ignoring
+ * max_iterations; used to time iterations. With collect_phase_times != 0
+ * every iteration is bracketed by HIP events per phase (and synchronized),
+ * otherwise iterations are enqueued back to back. */
+int sdfslam_hip_context_iterate(sdfslam_hip_context* ctx, int32_t iterations,
+                                int32_t collect_phase_times);
+
+/* Average device time in milliseconds per iteration of each phase over the
+ * last context_iterate with collect_phase_times (HIP events on the context's
+ * stream): assembly, normal equations, factorization, triangular solves,
+ * step+cost+control, their total, and the point-row assembly kernel alone. */
+typedef struct sdfslam_hip_phase_times {
+  double assembly_ms;
+  double normal_ms;
+  double factor_ms;
+  double solve_ms;
+  double update_ms;
+  double total_ms;
+  double point_assembly_kernel_ms;
+} sdfslam_hip_phase_times;
+int sdfslam_hip_phase_times_get(sdfslam_hip_context* ctx, sdfslam_hip_phase_times* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDFSLAM_HIP_H_ */

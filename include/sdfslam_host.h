@@ -1,0 +1,104 @@
+/*
+ * sdfslam_host.h - C API of the host library (libsdfslam_host.so): the
+ * mirror of the reference's Problem builder and Solve entry point, plus the
+ * seeded synthetic workloads. It is how Python tests and the benchmark build
+ * problems and reach Solve(problem, {backend: hip}).
+ *
+ * Reference interfaces mirrored (reference file:line):
+ *   sdfslam_host_problem_create  <- Problem::Problem, src/problem/problem.cpp:11-115
+ *   sdfslam_host_solve           <- Solve, src/solvers/solver.hpp:67 (+ kHip)
+ *   sdfslam_host_parse_solver_yaml <- LoadConfig's solver block, src/pipeline/config.cpp:170-190
+ */
+#ifndef SDFSLAM_HOST_H_
+#define SDFSLAM_HOST_H_
+
+#include <stdint.h>
+
+#include "sdfslam_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ProblemOptions (src/problem/problem.hpp:14-87). normals_method: 0 pca,
+ * 1 hybrid, 2 weighted. */
+typedef struct sdfslam_problem_options {
+  double weight_scan;
+  double weight_hallucination;
+  double weight_eikonal;
+  double weight_odometry;
+  double weight_relation;
+  int32_t hallucination_points_per_scan_point;
+  double hallucination_step_size;
+  int32_t hallucination_both_directions;
+  int32_t normals_method;
+  int32_t normals_k_neighbors;
+  int32_t normals_per_scan;
+  double normals_corner_threshold;
+  double huber_delta;
+  int32_t smooth_gradient;
+  double smooth_gradient_step;
+  int32_t active_region;
+  int32_t active_margin;
+} sdfslam_problem_options;
+
+typedef struct sdfslam_host_problem sdfslam_host_problem;
+
+/* Defaults of ProblemOptions / SolverOptions as the reference declares them. */
+void sdfslam_host_default_problem_options(sdfslam_problem_options* out);
+void sdfslam_host_default_solver_options(sdfslam_solver_options* out);
+
+/* Message of the last failed call on this thread. */
+const char* sdfslam_host_last_error(void);
+
+/* Builds a Problem. scans: num_frames scans, scan k has points
+ * scan_xy[2*scan_offsets[k] .. 2*scan_offsets[k+1]) in the sensor frame;
+ * poses: 3*num_frames; odometry: 3*(num_frames-1) or NULL; relations:
+ * rel_frames (2 per relation), rel_meas (3 per relation), rel_sqrt_weight.
+ * map_values: nx*ny initial values (NULL = 0). */
+int sdfslam_host_problem_create(const sdfslam_grid* grid, const double* map_values,
+                                int32_t num_frames, const double* poses,
+                                const int64_t* scan_offsets, const double* scan_xy,
+                                const double* odometry, int32_t num_relations,
+                                const int32_t* rel_frames, const double* rel_meas,
+                                const double* rel_sqrt_weight,
+                                const sdfslam_problem_options* options,
+                                sdfslam_host_problem** out);
+
+/* Builds the Problem of a seeded synthetic workload at its initial state:
+ * "room", "room_small", "office150", "office200", "office_small", "corner". */
+int sdfslam_host_workload_create(const char* name, uint64_t seed, sdfslam_host_problem** out);
+
+void sdfslam_host_problem_destroy(sdfslam_host_problem* problem);
+
+/* View of the specs; pointers stay valid while the problem lives. */
+int sdfslam_host_problem_view(const sdfslam_host_problem* problem, sdfslam_problem_view* out);
+int sdfslam_host_problem_get_state(const sdfslam_host_problem* problem, double* map_values,
+                                   double* poses);
+int sdfslam_host_problem_set_state(sdfslam_host_problem* problem, const double* map_values,
+                                   const double* poses);
+
+/* Workload metadata: the solver options of the workload, whether it is an
+ * incremental run, its iterations per increment, and ground-truth poses
+ * (3*num_poses, may be NULL). */
+int sdfslam_host_workload_info(const sdfslam_host_problem* problem,
+                               sdfslam_solver_options* solver, int32_t* incremental,
+                               int32_t* iterations_per_increment, double* ground_truth);
+
+/* Solve(problem, options) with backend "hip" (the only backend built here);
+ * any other name fails with the reference's logic_error text. Returns a
+ * status code; the exception message is in sdfslam_host_last_error(). */
+int sdfslam_host_solve(sdfslam_host_problem* problem, const char* backend,
+                       const sdfslam_solver_options* options, sdfslam_solve_result* result);
+
+/* Parses the `solver:` block of a run-config YAML file with the reference's
+ * strictness (unknown keys are errors, config.cpp:13-26, 170-190). backend
+ * receives "lm", "ceres" or "hip" (capacity >= 8). */
+int sdfslam_host_parse_solver_yaml(const char* path, sdfslam_solver_options* out, char* backend,
+                                   int32_t backend_capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDFSLAM_HOST_H_ */

@@ -1,0 +1,985 @@
+// Host driver of the device backend and the C ABI of libsdfslam_hip.so
+// (include/sdfslam_hip.h). Owns device memory, runs the symbolic analysis
+// per pattern epoch, enqueues LM iterations on one stream and reads back
+// only the small LM state between batches of iterations.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "analysis.hpp"
+#include "kernels.hpp"
+#include "sdfslam_hip.h"
+
+using namespace sdfslam;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure : std::runtime_error {
+  int code;
+  Failure(int c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+void HipCheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? SDFSLAM_ERR_OOM
+                                                                                 : SDFSLAM_ERR_HIP;
+    throw Failure(code, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+#define HIPC(x) HipCheck((x), #x)
+
+class DeviceArena {
+ public:
+  DeviceArena() = default;
+  DeviceArena(const DeviceArena&) = delete;
+  DeviceArena& operator=(const DeviceArena&) = delete;
+  ~DeviceArena() { Clear(); }
+  template <typename T>
+  T* Alloc(size_t n) {
+    void* p = nullptr;
+    HIPC(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    ptrs_.push_back(p);
+    bytes_ += std::max<size_t>(n, 1) * sizeof(T);
+    return static_cast<T*>(p);
+  }
+  template <typename T>
+  T* Upload(const std::vector<T>& v) {
+    T* p = Alloc<T>(v.size());
+    if (!v.empty()) HIPC(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  template <typename T>
+  T* Upload(const T* data, size_t n) {
+    T* p = Alloc<T>(n);
+    if (n > 0) HIPC(hipMemcpy(p, data, n * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  void Clear() {
+    for (void* p : ptrs_) (void)hipFree(p);
+    ptrs_.clear();
+    bytes_ = 0;
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  std::vector<void*> ptrs_;
+  size_t bytes_{0};
+};
+
+template <typename T>
+std::vector<T> CopyArray(const T* p, int64_t n) {
+  if (n <= 0) return {};
+  if (p == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null spec array");
+  return std::vector<T>(p, p + n);
+}
+
+/// Owned copy of a problem view.
+struct HostProblem {
+  sdfslam_problem_view view{};
+  std::vector<int32_t> node_to_active, pt_frame, eik_w, eik_h, od_i, od_j;
+  std::vector<double> pt_x, pt_y, pt_delta, pt_sw, eik_nx, eik_ny, eik_sw, od_dx, od_dy, od_dth,
+      od_sw;
+
+  explicit HostProblem(const sdfslam_problem_view& v) {
+    const sdfslam_grid& g = v.grid;
+    if (g.nx < 2 || g.ny < 2 || !(g.dx > 0.0) || !(g.dy > 0.0)) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "grid needs nx, ny >= 2 and dx, dy > 0");
+    }
+    if (v.num_poses < 1 || v.num_points < 0 || v.num_eikonal < 0 || v.num_odom < 0) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "negative sizes or no poses");
+    }
+    const int64_t nn = static_cast<int64_t>(g.nx) * g.ny;
+    if (nn > (1LL << 30)) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "grid too large");
+    node_to_active = CopyArray(v.node_to_active, nn);
+    int count = 0;
+    for (const int32_t c : node_to_active) {
+      if (c >= 0) ++count;
+      if (c >= v.num_active_nodes) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "node column out of range");
+    }
+    if (count != v.num_active_nodes) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "num_active_nodes does not match node_to_active");
+    }
+    pt_frame = CopyArray(v.point_frame, v.num_points);
+    pt_x = CopyArray(v.point_x, v.num_points);
+    pt_y = CopyArray(v.point_y, v.num_points);
+    pt_delta = CopyArray(v.point_expected_sdf, v.num_points);
+    pt_sw = CopyArray(v.point_sqrt_weight, v.num_points);
+    for (const int32_t f : pt_frame) {
+      if (f < 0 || f >= v.num_poses) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "point frame out of range");
+    }
+    eik_w = CopyArray(v.eik_w, v.num_eikonal);
+    eik_h = CopyArray(v.eik_h, v.num_eikonal);
+    eik_nx = CopyArray(v.eik_nx, v.num_eikonal);
+    eik_ny = CopyArray(v.eik_ny, v.num_eikonal);
+    eik_sw = CopyArray(v.eik_sqrt_weight, v.num_eikonal);
+    for (int64_t e = 0; e < v.num_eikonal; ++e) {
+      if (eik_w[e] < 0 || eik_h[e] < 0 || eik_w[e] + 1 >= g.nx || eik_h[e] + 1 >= g.ny) {
+        throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "eikonal spec outside the grid");
+      }
+    }
+    od_i = CopyArray(v.odom_i, v.num_odom);
+    od_j = CopyArray(v.odom_j, v.num_odom);
+    od_dx = CopyArray(v.odom_dx, v.num_odom);
+    od_dy = CopyArray(v.odom_dy, v.num_odom);
+    od_dth = CopyArray(v.odom_dtheta, v.num_odom);
+    od_sw = CopyArray(v.odom_sqrt_weight, v.num_odom);
+    for (int64_t o = 0; o < v.num_odom; ++o) {
+      if (od_i[o] < 0 || od_j[o] < 0 || od_i[o] >= v.num_poses || od_j[o] >= v.num_poses ||
+          od_i[o] == od_j[o]) {
+        throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "odometry frames out of range");
+      }
+    }
+    view = v;
+    view.node_to_active = node_to_active.data();
+    view.point_frame = pt_frame.data();
+    view.point_x = pt_x.data();
+    view.point_y = pt_y.data();
+    view.point_expected_sdf = pt_delta.data();
+    view.point_sqrt_weight = pt_sw.data();
+    view.eik_w = eik_w.data();
+    view.eik_h = eik_h.data();
+    view.eik_nx = eik_nx.data();
+    view.eik_ny = eik_ny.data();
+    view.eik_sqrt_weight = eik_sw.data();
+    view.odom_i = od_i.data();
+    view.odom_j = od_j.data();
+    view.odom_dx = od_dx.data();
+    view.odom_dy = od_dy.data();
+    view.odom_dtheta = od_dth.data();
+    view.odom_sqrt_weight = od_sw.data();
+  }
+  int n() const { return view.num_active_nodes + 3 * (view.num_poses - 1); }
+  int64_t rows() const { return view.num_points + view.num_eikonal + 3 * view.num_odom; }
+};
+
+struct ChildList {
+  int64_t offset;
+  int count;
+  int max_m;
+};
+
+constexpr int kPhases = 5;
+
+}  // namespace
+
+struct sdfslam_hip_context {
+  int device{0};
+  hipStream_t stream{nullptr};
+  std::unique_ptr<HostProblem> hp;
+  sdfslam_solver_options opts{};
+  AnalysisOptions aopt;
+  Analysis an;
+  int pattern_epochs{0};
+  DeviceArena base;
+  DeviceArena epoch;
+  DeviceProblem dp{};
+  DeviceAnalysis da{};
+  DeviceWork dw{};
+  int32_t* iota{nullptr};
+  uint32_t* sorted_key{nullptr};
+  int32_t* sorted_row{nullptr};
+  double* j_eik{nullptr};
+  double* scalar{nullptr};
+  void* sort_tmp{nullptr};
+  size_t sort_tmp_bytes{0};
+  unsigned sort_bits{1};
+  int history_capacity{0};
+  int32_t* level_sn{nullptr};
+  std::vector<int64_t> level_off;
+  int32_t* child_lists{nullptr};
+  std::vector<std::vector<ChildList>> level_children;
+  int max_f{0};
+  hipEvent_t events[kPhases + 3]{};  // phase marks, then the point-row kernel pair
+  double phase_ms[kPhases + 1]{};
+  int timed_iterations{0};
+  bool lm_ready{false};
+
+  ~sdfslam_hip_context() {
+    for (hipEvent_t& e : events) {
+      if (e != nullptr) (void)hipEventDestroy(e);
+    }
+    epoch.Clear();
+    base.Clear();
+    if (stream != nullptr) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+LmOptions ToLm(const sdfslam_solver_options& o, int capacity) {
+  LmOptions l{};
+  l.max_iterations = o.max_iterations;
+  l.history_capacity = capacity;
+  l.step_tolerance = o.step_tolerance;
+  l.lambda_factor = o.lambda_factor;
+  l.reject_worse_steps = o.reject_worse_steps;
+  l.marquardt_scaling = o.marquardt_scaling;
+  l.trust_region = o.trust_region;
+  return l;
+}
+
+void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double* poses) {
+  const sdfslam_problem_view& v = c.hp->view;
+  HostProblem& hp = *c.hp;
+  DeviceProblem& P = c.dp;
+  P.grid = {v.grid.nx, v.grid.ny, v.grid.min_x, v.grid.min_y, v.grid.dx, v.grid.dy};
+  P.num_cells = (v.grid.nx - 1) * (v.grid.ny - 1);
+  P.num_active = v.num_active_nodes;
+  P.num_poses = v.num_poses;
+  P.n = hp.n();
+  P.num_points = v.num_points;
+  P.num_eikonal = v.num_eikonal;
+  P.num_odom = v.num_odom;
+  std::vector<int32_t> active_to_node(static_cast<size_t>(v.num_active_nodes));
+  for (size_t g = 0; g < hp.node_to_active.size(); ++g) {
+    if (hp.node_to_active[g] >= 0) active_to_node[static_cast<size_t>(hp.node_to_active[g])] = static_cast<int32_t>(g);
+  }
+  DeviceArena& A = c.base;
+  P.node_to_active = A.Upload(hp.node_to_active);
+  P.active_to_node = A.Upload(active_to_node);
+  P.pt_frame = A.Upload(hp.pt_frame);
+  P.pt_x = A.Upload(hp.pt_x);
+  P.pt_y = A.Upload(hp.pt_y);
+  P.pt_delta = A.Upload(hp.pt_delta);
+  P.pt_sw = A.Upload(hp.pt_sw);
+  P.eik_w = A.Upload(hp.eik_w);
+  P.eik_h = A.Upload(hp.eik_h);
+  P.eik_nx = A.Upload(hp.eik_nx);
+  P.eik_ny = A.Upload(hp.eik_ny);
+  P.eik_sw = A.Upload(hp.eik_sw);
+  P.od_i = A.Upload(hp.od_i);
+  P.od_j = A.Upload(hp.od_j);
+  P.od_dx = A.Upload(hp.od_dx);
+  P.od_dy = A.Upload(hp.od_dy);
+  P.od_dth = A.Upload(hp.od_dth);
+  P.od_sw = A.Upload(hp.od_sw);
+  P.huber_delta = v.huber_delta;
+  P.smooth_gradient = v.smooth_gradient;
+  P.smooth_gradient_step = v.smooth_gradient_step;
+
+  DeviceWork& W = c.dw;
+  const size_t nmap = static_cast<size_t>(v.grid.nx) * v.grid.ny;
+  const size_t npose = 3 * static_cast<size_t>(v.num_poses);
+  W.map = A.Upload(map_values, nmap);
+  W.poses = A.Upload(poses, npose);
+  W.map_backup = A.Alloc<double>(nmap);
+  W.poses_backup = A.Alloc<double>(npose);
+  const auto np = static_cast<size_t>(v.num_points);
+  W.r_pt = A.Alloc<double>(np);
+  W.jw = A.Alloc<double>(4 * np);
+  W.jp = A.Alloc<double>(3 * np);
+  W.cellkey = A.Alloc<uint32_t>(np);
+  W.r_eik = A.Alloc<double>(static_cast<size_t>(v.num_eikonal));
+  c.j_eik = A.Alloc<double>(3 * static_cast<size_t>(v.num_eikonal));
+  W.od_r = A.Alloc<double>(3 * static_cast<size_t>(v.num_odom));
+  W.od_ji = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
+  W.od_jj = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
+  c.sorted_key = A.Alloc<uint32_t>(np);
+  c.sorted_row = A.Alloc<int32_t>(np);
+  W.sorted_key = c.sorted_key;
+  W.sorted_row = c.sorted_row;
+  std::vector<int32_t> iota(np);
+  for (size_t i = 0; i < np; ++i) iota[i] = static_cast<int32_t>(i);
+  c.iota = A.Upload(iota);
+  W.cell_begin = A.Alloc<int32_t>(static_cast<size_t>(P.num_cells) + 1);
+  W.cell_end = A.Alloc<int32_t>(static_cast<size_t>(P.num_cells) + 1);
+  W.nn = A.Alloc<double>(10 * static_cast<size_t>(P.num_cells));
+  W.rn = A.Alloc<double>(4 * static_cast<size_t>(P.num_cells));
+  W.rhs = A.Alloc<double>(static_cast<size_t>(P.n));
+  W.step = A.Alloc<double>(static_cast<size_t>(P.n));
+  W.xnew = A.Alloc<double>(static_cast<size_t>(P.n));
+  W.partials = A.Alloc<double>(3 * static_cast<size_t>(kPartialBlocks));
+  c.history_capacity = std::max(c.opts.max_iterations, 0) + 1;
+  W.cost_history = A.Alloc<double>(static_cast<size_t>(c.history_capacity));
+  W.state = A.Alloc<LmState>(1);
+  c.scalar = A.Alloc<double>(1);
+  LmState zero{};
+  HIPC(hipMemcpy(W.state, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  // Radix sort of the point rows by cell.
+  unsigned bits = 1;
+  while ((1ULL << bits) <= static_cast<unsigned long long>(P.num_cells)) ++bits;
+  c.sort_bits = bits;
+  if (np > 0) {
+    HIPC(rocprim::radix_sort_pairs(nullptr, c.sort_tmp_bytes, W.cellkey, c.sorted_key, c.iota,
+                                   c.sorted_row, np, 0, bits, c.stream));
+    c.sort_tmp = A.Alloc<unsigned char>(c.sort_tmp_bytes);
+  }
+}
+
+void ReadState(sdfslam_hip_context& c, LmState& st) {
+  HIPC(hipMemcpyAsync(&st, c.dw.state, sizeof(LmState), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+}
+
+void WriteState(sdfslam_hip_context& c, const LmState& st) {
+  HIPC(hipMemcpyAsync(c.dw.state, &st, sizeof(LmState), hipMemcpyHostToDevice, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+}
+
+void DownloadState(sdfslam_hip_context& c, std::vector<double>& map, std::vector<double>& poses) {
+  const sdfslam_problem_view& v = c.hp->view;
+  map.resize(static_cast<size_t>(v.grid.nx) * v.grid.ny);
+  poses.resize(3 * static_cast<size_t>(v.num_poses));
+  HIPC(hipMemcpyAsync(map.data(), c.dw.map, map.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipMemcpyAsync(poses.data(), c.dw.poses, poses.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+}
+
+/// Symbolic analysis at the current device state and the device buffers
+/// that depend on it (a new pattern epoch).
+void BuildEpoch(sdfslam_hip_context& c) {
+  std::vector<double> map;
+  std::vector<double> poses;
+  DownloadState(c, map, poses);
+  c.epoch.Clear();
+  c.an = Analyze(c.hp->view, map.data(), poses.data(), c.aopt);
+  ++c.pattern_epochs;
+  const Analysis& an = c.an;
+  if (an.max_points_per_pose > 4096) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "more than 4096 point rows in one frame");
+  }
+  DeviceArena& E = c.epoch;
+  DeviceAnalysis& D = c.da;
+  D.perm = E.Upload(an.perm);
+  D.iperm = E.Upload(an.iperm);
+  D.sn_first = E.Upload(an.sn_first);
+  D.sn_m = E.Upload(an.sn_m);
+  D.sn_f = E.Upload(an.sn_f);
+  D.sn_idx_ptr = E.Upload(an.sn_idx_ptr);
+  D.sn_idx = E.Upload(an.sn_idx);
+  D.panel_off = E.Upload(an.panel_off);
+  D.upd_off = E.Upload(an.upd_off);
+  D.sn_parent = E.Upload(an.sn_parent);
+  D.rel_off = E.Upload(an.rel_off);
+  D.rel = E.Upload(an.rel);
+  D.rel_split = E.Upload(an.rel_split);
+  D.nn_off = E.Upload(an.nn_off);
+  D.nn_eik = E.Upload(an.nn_eik);
+  D.eik_at = E.Upload(an.eik_at);
+  D.fp_ptr = E.Upload(an.fp_ptr);
+  D.fp_node = E.Upload(an.fp_node);
+  D.fp_off = E.Upload(an.fp_off);
+  D.fp_stride = E.Upload(an.fp_stride);
+  D.words_per_pose = an.words_per_pose;
+  D.fp_cellbits = E.Upload(an.fp_cellbits);
+  D.pr_ptr = E.Upload(an.pr_ptr);
+  D.pr_row = E.Upload(an.pr_row);
+  D.pd_off = E.Upload(an.pd_off);
+  D.po_ptr = E.Upload(an.po_ptr);
+  D.po_spec = E.Upload(an.po_spec);
+  D.num_pairs = static_cast<int>(an.op_pose.size() / 2);
+  D.op_pose = E.Upload(an.op_pose);
+  D.op_ptr = E.Upload(an.op_ptr);
+  D.op_spec = E.Upload(an.op_spec);
+  D.op_off = E.Upload(an.op_off);
+  std::vector<int64_t> cptr(static_cast<size_t>(an.num_sn) + 1, 0);
+  std::vector<int32_t> clist;
+  for (int s = 0; s < an.num_sn; ++s) {
+    for (const int ch : an.children[static_cast<size_t>(s)]) clist.push_back(ch);
+    cptr[static_cast<size_t>(s) + 1] = static_cast<int64_t>(clist.size());
+  }
+  D.children_ptr = E.Upload(cptr);
+  D.children = E.Upload(clist);
+  // Level lists and, per level and child rank, the children to extend-add.
+  std::vector<int32_t> lvl;
+  c.level_off.assign(static_cast<size_t>(an.num_levels) + 1, 0);
+  std::vector<int32_t> ranks;
+  c.level_children.assign(static_cast<size_t>(an.num_levels), {});
+  for (int L = 0; L < an.num_levels; ++L) {
+    for (const int s : an.level_sn[static_cast<size_t>(L)]) lvl.push_back(s);
+    c.level_off[static_cast<size_t>(L) + 1] = static_cast<int64_t>(lvl.size());
+    for (int r = 0; r < an.level_max_children[static_cast<size_t>(L)]; ++r) {
+      ChildList cl{static_cast<int64_t>(ranks.size()), 0, 0};
+      for (const int s : an.level_sn[static_cast<size_t>(L)]) {
+        const auto& ch = an.children[static_cast<size_t>(s)];
+        if (static_cast<int>(ch.size()) > r) {
+          const int child = ch[static_cast<size_t>(r)];
+          ranks.push_back(child);
+          ++cl.count;
+          cl.max_m = std::max(cl.max_m, an.sn_m[static_cast<size_t>(child)]);
+        }
+      }
+      c.level_children[static_cast<size_t>(L)].push_back(cl);
+    }
+  }
+  c.level_sn = E.Upload(lvl);
+  c.child_lists = E.Upload(ranks);
+  c.max_f = 0;
+  for (const int f : an.sn_f) c.max_f = std::max(c.max_f, f);
+  if (static_cast<size_t>(c.max_f) * sizeof(double) > 150 * 1024) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "front larger than the solve kernels' LDS budget");
+  }
+  c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
+  c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
+  c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
+}
+
+enum Phase { kAssembly = 0, kNormal = 1, kFactor = 2, kSolve = 3, kUpdate = 4 };
+
+void Mark(sdfslam_hip_context& c, bool timing, int k) {
+  if (timing) HIPC(hipEventRecord(c.events[k], c.stream));
+}
+
+void EnqueueAssemblyAndNormal(sdfslam_hip_context& c, const LmOptions& o, bool timing) {
+  hipStream_t s = c.stream;
+  LaunchAssemble(c.dp, c.da, c.dw, true, s, timing ? c.events[kPhases + 1] : nullptr,
+                 timing ? c.events[kPhases + 2] : nullptr);
+  if (c.dp.num_points > 0) {
+    HIPC(rocprim::radix_sort_pairs(c.sort_tmp, c.sort_tmp_bytes, c.dw.cellkey, c.sorted_key, c.iota,
+                                   c.sorted_row, static_cast<size_t>(c.dp.num_points), 0,
+                                   c.sort_bits, s));
+  }
+  Mark(c, timing, 1);
+  LaunchCellReduce(c.dp, c.dw, s);
+  LaunchFill(c.dw.panel, c.an.panel_total, 0.0, s);
+  LaunchNormalSlots(c.dp, c.da, c.dw, o, c.an.max_points_per_pose, s);
+}
+
+void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
+  hipStream_t s = c.stream;
+  const Analysis& an = c.an;
+  for (int L = 0; L < an.num_levels; ++L) {
+    const auto& cls = c.level_children[static_cast<size_t>(L)];
+    for (const ChildList& cl : cls) {
+      LaunchExtendAddPanel(c.da, c.dw, c.child_lists + cl.offset, cl.count, cl.max_m, s);
+    }
+    const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
+    const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
+    LaunchFactorLevel(c.da, c.dw, lvl, count, an.level_max_nb[static_cast<size_t>(L)],
+                      an.level_max_f[static_cast<size_t>(L)], an.level_max_m[static_cast<size_t>(L)], s);
+    for (const ChildList& cl : cls) {
+      LaunchExtendAddUpdate(c.da, c.dw, c.child_lists + cl.offset, cl.count, cl.max_m, s);
+    }
+  }
+  Mark(c, timing, 3);
+  for (int L = 0; L < an.num_levels; ++L) {
+    const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
+    const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
+    LaunchForwardLevel(c.da, c.dw, lvl, count, an.level_max_f[static_cast<size_t>(L)], s);
+  }
+  for (int L = an.num_levels - 1; L >= 0; --L) {
+    const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
+    const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
+    LaunchBackwardLevel(c.da, c.dw, lvl, count, an.level_max_f[static_cast<size_t>(L)], s);
+  }
+}
+
+void EnqueueIteration(sdfslam_hip_context& c, const LmOptions& o, bool timing) {
+  Mark(c, timing, 0);
+  EnqueueAssemblyAndNormal(c, o, timing);
+  Mark(c, timing, 2);
+  EnqueueFactorAndSolve(c, timing);
+  Mark(c, timing, 4);
+  LaunchStepAndControl(c.dp, c.dw, o, c.stream);
+  Mark(c, timing, 5);
+}
+
+void AccumulateTimes(sdfslam_hip_context& c) {
+  for (int k = 0; k < kPhases; ++k) {
+    float ms = 0.0f;
+    HIPC(hipEventElapsedTime(&ms, c.events[k], c.events[k + 1]));
+    c.phase_ms[k] += ms;
+  }
+  float ms = 0.0f;
+  HIPC(hipEventElapsedTime(&ms, c.events[kPhases + 1], c.events[kPhases + 2]));
+  c.phase_ms[kPhases] += ms;
+  ++c.timed_iterations;
+}
+
+void InitLm(sdfslam_hip_context& c, double lambda_init) {
+  LaunchCost(c.dp, c.dw, c.dw.map, c.dw.poses, c.stream);
+  LaunchLmInit(c.dw, lambda_init, c.stream);
+  c.lm_ready = true;
+}
+
+/// Clears the flags so stage calls run, optionally with a given lambda.
+void PrepareStage(sdfslam_hip_context& c, const double* lambda) {
+  LmState st{};
+  ReadState(c, st);
+  st.halt = 0;
+  st.overflow = 0;
+  st.not_spd = 0;
+  st.revert = 0;
+  if (lambda != nullptr) st.lambda = *lambda;
+  WriteState(c, st);
+}
+
+/// Runs `iterations` LM iterations, re-analysing when a point leaves its
+/// pose's footprint. Returns the final LM state.
+LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations, bool timing) {
+  LmState st{};
+  ReadState(c, st);
+  const int target = st.iter + iterations;
+  int last_overflow_iter = -1;
+  while (true) {
+    if (st.halt != 0 || st.iter >= target) break;
+    const int chunk = timing ? (target - st.iter) : std::min(target - st.iter, 8);
+    for (int k = 0; k < chunk; ++k) {
+      EnqueueIteration(c, o, timing);
+      if (timing) {
+        HIPC(hipStreamSynchronize(c.stream));
+        AccumulateTimes(c);
+      }
+    }
+    ReadState(c, st);
+    if (st.not_spd != 0) break;
+    if (st.overflow != 0) {
+      if (st.iter == last_overflow_iter) {
+        throw Failure(SDFSLAM_ERR_INTERNAL, "pattern re-analysis did not cover the point cells");
+      }
+      last_overflow_iter = st.iter;
+      BuildEpoch(c);
+      st.overflow = 0;
+      st.halt = 0;
+      WriteState(c, st);
+    }
+  }
+  return st;
+}
+
+int Guard(const std::function<int()>& fn) {
+  try {
+    return fn();
+  } catch (const Failure& f) {
+    g_last_error = f.what();
+    return f.code;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  } catch (const std::bad_alloc& e) {
+    g_last_error = e.what();
+    return SDFSLAM_ERR_OOM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return SDFSLAM_ERR_INTERNAL;
+  }
+}
+
+void CreateContext(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
+                   const double* map_values, const double* poses, sdfslam_hip_context** out) {
+  if (problem == nullptr || options == nullptr || map_values == nullptr || poses == nullptr ||
+      out == nullptr) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    throw Failure(SDFSLAM_ERR_NO_DEVICE, "no HIP device");
+  }
+  auto c = std::make_unique<sdfslam_hip_context>();
+  c->hp = std::make_unique<HostProblem>(*problem);
+  c->opts = *options;
+  HIPC(hipGetDevice(&c->device));
+  HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (hipEvent_t& e : c->events) HIPC(hipEventCreate(&e));
+  AllocateBase(*c, map_values, poses);
+  BuildEpoch(*c);
+  InitLm(*c, options->lambda_init);
+  HIPC(hipStreamSynchronize(c->stream));
+  *out = c.release();
+}
+
+void FillResult(sdfslam_hip_context& c, const LmState& st, sdfslam_solve_result* result) {
+  result->converged = st.converged;
+  result->iterations = st.iter;
+  result->initial_cost = st.initial_cost;
+  result->final_cost = st.iter == 0 ? st.initial_cost : st.final_cost;
+  const int len = st.iter + 1;
+  result->cost_history_length = len;
+  if (result->cost_history != nullptr && result->cost_history_capacity > 0) {
+    const int n = std::min({len, result->cost_history_capacity, c.history_capacity});
+    HIPC(hipMemcpy(result->cost_history, c.dw.cost_history, static_cast<size_t>(n) * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  }
+}
+
+int SolveContext(sdfslam_hip_context& c, const sdfslam_solver_options& options,
+                 sdfslam_solve_result* result) {
+  if (options.max_iterations + 1 > c.history_capacity) {
+    c.history_capacity = options.max_iterations + 1;
+    c.dw.cost_history = c.base.Alloc<double>(static_cast<size_t>(c.history_capacity));
+  }
+  const LmOptions o = ToLm(options, c.history_capacity);
+  InitLm(c, options.lambda_init);
+  const LmState st = RunIterations(c, o, std::max(options.max_iterations, 0), false);
+  if (st.not_spd != 0) {
+    g_last_error = "sparse Cholesky factorization failed";
+    return SDFSLAM_ERR_NOT_SPD;
+  }
+  if (result != nullptr) FillResult(c, st, result);
+  return SDFSLAM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sdfslam_hip_status_string(int status) {
+  switch (status) {
+    case SDFSLAM_OK: return "ok";
+    case SDFSLAM_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case SDFSLAM_ERR_NOT_SPD: return "sparse Cholesky factorization failed";
+    case SDFSLAM_ERR_HIP: return "HIP runtime error";
+    case SDFSLAM_ERR_OOM: return "out of device memory";
+    case SDFSLAM_ERR_NO_DEVICE: return "no HIP device";
+    case SDFSLAM_ERR_INTERNAL: return "internal error";
+    default: return "unknown status";
+  }
+}
+
+const char* sdfslam_hip_last_error(void) { return g_last_error.c_str(); }
+
+int sdfslam_hip_context_create(const sdfslam_problem_view* problem,
+                               const sdfslam_solver_options* options, const double* map_values,
+                               const double* poses, sdfslam_hip_context** out) {
+  return Guard([&]() -> int {
+    CreateContext(problem, options, map_values, poses, out);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx) {
+  delete ctx;
+  return SDFSLAM_OK;
+}
+
+int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, const double* poses) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
+    const sdfslam_problem_view& v = ctx->hp->view;
+    if (map_values != nullptr) {
+      HIPC(hipMemcpy(ctx->dw.map, map_values, sizeof(double) * v.grid.nx * v.grid.ny, hipMemcpyHostToDevice));
+    }
+    if (poses != nullptr) {
+      HIPC(hipMemcpy(ctx->dw.poses, poses, sizeof(double) * 3 * v.num_poses, hipMemcpyHostToDevice));
+    }
+    BuildEpoch(*ctx);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_get_state(sdfslam_hip_context* ctx, double* map_values, double* poses) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
+    std::vector<double> m, p;
+    DownloadState(*ctx, m, p);
+    if (map_values != nullptr) std::copy(m.begin(), m.end(), map_values);
+    if (poses != nullptr) std::copy(p.begin(), p.end(), poses);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* info) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || info == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    const Analysis& a = ctx->an;
+    info->num_parameters = a.n;
+    info->num_residuals = ctx->hp->rows();
+    info->pattern_nnz_lower = a.pattern_nnz_lower;
+    info->num_supernodes = a.num_sn;
+    info->factor_nnz = a.factor_nnz;
+    info->factor_flops = a.factor_flops;
+    info->update_doubles = a.upd_total;
+    info->tree_levels = a.num_levels;
+    info->pattern_epochs = ctx->pattern_epochs;
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_assemble(sdfslam_hip_context* ctx, double* residuals, int32_t* j_cols,
+                         double* j_vals) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || residuals == nullptr || j_cols == nullptr || j_vals == nullptr) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    sdfslam_hip_context& c = *ctx;
+    PrepareStage(c, nullptr);
+    LaunchAssemble(c.dp, c.da, c.dw, false, c.stream);
+    LaunchAssembleEikonalJ(c.dp, c.dw, c.j_eik, c.stream);
+    const sdfslam_problem_view& v = c.hp->view;
+    const auto np = static_cast<size_t>(v.num_points);
+    const auto ne = static_cast<size_t>(v.num_eikonal);
+    const auto no = static_cast<size_t>(v.num_odom);
+    std::vector<double> r(np), jw(4 * np), jp(3 * np), re(ne), je(3 * ne), odr(3 * no), oji(9 * no),
+        ojj(9 * no);
+    std::vector<uint32_t> key(np);
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+      if (bytes > 0) HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream));
+    };
+    d2h(r.data(), c.dw.r_pt, np * 8);
+    d2h(jw.data(), c.dw.jw, 4 * np * 8);
+    d2h(jp.data(), c.dw.jp, 3 * np * 8);
+    d2h(key.data(), c.dw.cellkey, np * 4);
+    d2h(re.data(), c.dw.r_eik, ne * 8);
+    d2h(je.data(), c.j_eik, 3 * ne * 8);
+    d2h(odr.data(), c.dw.od_r, 3 * no * 8);
+    d2h(oji.data(), c.dw.od_ji, 9 * no * 8);
+    d2h(ojj.data(), c.dw.od_jj, 9 * no * 8);
+    HIPC(hipStreamSynchronize(c.stream));
+    // Row-ELL in the reference's triplet order within each row.
+    const int nx = v.grid.nx;
+    const int ncx = nx - 1;
+    const int na = v.num_active_nodes;
+    const uint32_t invalid = static_cast<uint32_t>(ncx * (v.grid.ny - 1));
+    for (size_t s = 0; s < np; ++s) {
+      residuals[s] = r[s];
+      int32_t* cols = j_cols + 7 * s;
+      double* vals = j_vals + 7 * s;
+      int k = 0;
+      if (key[s] < invalid) {
+        const int cw = static_cast<int>(key[s]) % ncx;
+        const int ch = static_cast<int>(key[s]) / ncx;
+        const int ids[4] = {ch * nx + cw, ch * nx + cw + 1, (ch + 1) * nx + cw, (ch + 1) * nx + cw + 1};
+        for (int t = 0; t < 4; ++t) {
+          const int col = v.node_to_active[ids[t]];
+          if (col >= 0) {
+            cols[k] = col;
+            vals[k++] = jw[t * np + s];
+          }
+        }
+        const int f = v.point_frame[s];
+        if (f >= 1) {
+          for (int t = 0; t < 3; ++t) {
+            cols[k] = na + 3 * (f - 1) + t;
+            vals[k++] = jp[t * np + s];
+          }
+        }
+      }
+      for (; k < 7; ++k) {
+        cols[k] = -1;
+        vals[k] = 0.0;
+      }
+    }
+    for (size_t e = 0; e < ne; ++e) {
+      const size_t row = np + e;
+      residuals[row] = re[e];
+      int32_t* cols = j_cols + 7 * row;
+      double* vals = j_vals + 7 * row;
+      const int g = v.eik_h[e] * nx + v.eik_w[e];
+      const int ids[3] = {g, g + 1, g + nx};
+      int k = 0;
+      for (int t = 0; t < 3; ++t) {
+        const int col = v.node_to_active[ids[t]];
+        if (col >= 0) {
+          cols[k] = col;
+          vals[k++] = je[3 * e + t];
+        }
+      }
+      for (; k < 7; ++k) {
+        cols[k] = -1;
+        vals[k] = 0.0;
+      }
+    }
+    for (size_t o = 0; o < no; ++o) {
+      for (int rr = 0; rr < 3; ++rr) {
+        const size_t row = np + ne + 3 * o + rr;
+        residuals[row] = odr[3 * o + rr];
+        int32_t* cols = j_cols + 7 * row;
+        double* vals = j_vals + 7 * row;
+        int k = 0;
+        const int fi = v.odom_i[o];
+        const int fj = v.odom_j[o];
+        if (fi >= 1) {
+          for (int t = 0; t < 3; ++t) {
+            cols[k] = na + 3 * (fi - 1) + t;
+            vals[k++] = oji[9 * o + 3 * rr + t];
+          }
+        }
+        for (int t = 0; t < 3; ++t) {
+          cols[k] = na + 3 * (fj - 1) + t;
+          vals[k++] = ojj[9 * o + 3 * rr + t];
+        }
+        for (; k < 7; ++k) {
+          cols[k] = -1;
+          vals[k] = 0.0;
+        }
+      }
+    }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int32_t* col_ptr,
+                                 int32_t* row_idx, double* values, uint8_t* structural,
+                                 double* rhs) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
+    sdfslam_hip_context& c = *ctx;
+    const LmOptions o = ToLm(c.opts, c.history_capacity);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      PrepareStage(c, &lambda);
+      EnqueueAssemblyAndNormal(c, o, false);
+      LmState st{};
+      ReadState(c, st);
+      if (st.overflow == 0) break;
+      if (attempt == 1) throw Failure(SDFSLAM_ERR_INTERNAL, "pattern re-analysis failed");
+      BuildEpoch(c);
+    }
+    const Analysis& an = c.an;
+    const std::vector<PatternEntry> pat = LowerPattern(an, c.hp->view);
+    std::vector<double> panel(static_cast<size_t>(an.panel_total));
+    HIPC(hipMemcpy(panel.data(), c.dw.panel, panel.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (rhs != nullptr) {
+      HIPC(hipMemcpy(rhs, c.dw.rhs, sizeof(double) * an.n, hipMemcpyDeviceToHost));
+    }
+    if (col_ptr != nullptr) {
+      std::fill(col_ptr, col_ptr + an.n + 1, 0);
+      for (const PatternEntry& e : pat) ++col_ptr[e.col + 1];
+      for (int j = 0; j < an.n; ++j) col_ptr[j + 1] += col_ptr[j];
+    }
+    for (size_t k = 0; k < pat.size(); ++k) {
+      if (row_idx != nullptr) row_idx[k] = pat[k].row;
+      if (values != nullptr) values[k] = panel[static_cast<size_t>(pat[k].off)];
+    }
+    if (structural != nullptr) {
+      // Pairs of columns coupled by some Jacobian row of the device assembly.
+      const HostProblem& hp = *c.hp;
+      const int64_t rows = hp.rows();
+      std::vector<double> r(static_cast<size_t>(rows));
+      std::vector<int32_t> jc(7 * static_cast<size_t>(rows));
+      std::vector<double> jv(7 * static_cast<size_t>(rows));
+      const int st = sdfslam_hip_assemble(ctx, r.data(), jc.data(), jv.data());
+      if (st != SDFSLAM_OK) throw Failure(st, g_last_error);
+      std::vector<int64_t> keys;
+      for (int64_t row = 0; row < rows; ++row) {
+        const int32_t* cs = jc.data() + 7 * row;
+        for (int a = 0; a < 7 && cs[a] >= 0; ++a) {
+          for (int b = 0; b < 7 && cs[b] >= 0; ++b) {
+            if (cs[a] >= cs[b]) keys.push_back(static_cast<int64_t>(cs[a]) * an.n + cs[b]);
+          }
+        }
+      }
+      std::sort(keys.begin(), keys.end());
+      keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+      for (size_t k = 0; k < pat.size(); ++k) {
+        const int64_t key = static_cast<int64_t>(pat[k].row) * an.n + pat[k].col;
+        structural[k] = (pat[k].row == pat[k].col || std::binary_search(keys.begin(), keys.end(), key)) ? 1 : 0;
+      }
+    }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_linear_step(sdfslam_hip_context* ctx, double lambda, double* step) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || step == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    sdfslam_hip_context& c = *ctx;
+    const LmOptions o = ToLm(c.opts, c.history_capacity);
+    LmState st{};
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      PrepareStage(c, &lambda);
+      EnqueueAssemblyAndNormal(c, o, false);
+      EnqueueFactorAndSolve(c, false);
+      ReadState(c, st);
+      if (st.overflow == 0) break;
+      if (attempt == 1) throw Failure(SDFSLAM_ERR_INTERNAL, "pattern re-analysis failed");
+      BuildEpoch(c);
+    }
+    if (st.not_spd != 0) {
+      g_last_error = "sparse Cholesky factorization failed";
+      return SDFSLAM_ERR_NOT_SPD;
+    }
+    HIPC(hipMemcpy(step, c.dw.step, sizeof(double) * c.an.n, hipMemcpyDeviceToHost));
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_cost(sdfslam_hip_context* ctx, double* cost) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || cost == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    sdfslam_hip_context& c = *ctx;
+    LaunchCost(c.dp, c.dw, c.dw.map, c.dw.poses, c.stream);
+    LaunchSumPartials(c.dw, c.scalar, c.stream);
+    HIPC(hipMemcpyAsync(cost, c.scalar, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_context_solve(sdfslam_hip_context* ctx, const sdfslam_solver_options* options,
+                              sdfslam_solve_result* result) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || options == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    const auto t0 = std::chrono::steady_clock::now();
+    ctx->opts = *options;
+    const int st = SolveContext(*ctx, *options, result);
+    if (result != nullptr) {
+      result->duration_seconds =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return st;
+  });
+}
+
+int sdfslam_hip_context_iterate(sdfslam_hip_context* ctx, int32_t iterations,
+                                int32_t collect_phase_times) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || iterations < 0) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "bad argument");
+    sdfslam_hip_context& c = *ctx;
+    LmOptions o = ToLm(c.opts, c.history_capacity);
+    o.max_iterations = INT_MAX;
+    for (double& p : c.phase_ms) p = 0.0;
+    c.timed_iterations = 0;
+    LmState st{};
+    ReadState(c, st);
+    if (st.halt != 0) {
+      st.halt = 0;
+      WriteState(c, st);
+    }
+    st = RunIterations(c, o, iterations, collect_phase_times != 0);
+    if (st.not_spd != 0) {
+      g_last_error = "sparse Cholesky factorization failed";
+      return SDFSLAM_ERR_NOT_SPD;
+    }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_phase_times_get(sdfslam_hip_context* ctx, sdfslam_hip_phase_times* out) {
+  if (ctx == nullptr || out == nullptr) return SDFSLAM_ERR_INVALID_ARGUMENT;
+  const double n = std::max(1, ctx->timed_iterations);
+  out->assembly_ms = ctx->phase_ms[kAssembly] / n;
+  out->normal_ms = ctx->phase_ms[kNormal] / n;
+  out->factor_ms = ctx->phase_ms[kFactor] / n;
+  out->solve_ms = ctx->phase_ms[kSolve] / n;
+  out->update_ms = ctx->phase_ms[kUpdate] / n;
+  out->total_ms = (ctx->phase_ms[0] + ctx->phase_ms[1] + ctx->phase_ms[2] + ctx->phase_ms[3] +
+                   ctx->phase_ms[4]) / n;
+  out->point_assembly_kernel_ms = ctx->phase_ms[kPhases] / n;
+  return SDFSLAM_OK;
+}
+
+int sdfslam_hip_solve(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
+                      double* map_values, double* poses, sdfslam_solve_result* result) {
+  return Guard([&]() -> int {
+    const auto t0 = std::chrono::steady_clock::now();
+    sdfslam_hip_context* raw = nullptr;
+    CreateContext(problem, options, map_values, poses, &raw);
+    std::unique_ptr<sdfslam_hip_context> ctx(raw);
+    const int st = SolveContext(*ctx, *options, result);
+    if (st != SDFSLAM_OK) return st;
+    std::vector<double> m, p;
+    DownloadState(*ctx, m, p);
+    std::copy(m.begin(), m.end(), map_values);
+    std::copy(p.begin(), p.end(), poses);
+    if (result != nullptr) {
+      result->duration_seconds =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return SDFSLAM_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,1312 @@
+// Device kernels of the joint-LM backend (gfx950, FP64).
+//
+// Assembly follows the reference's residual model term by term
+// (src/problem/residual_math.hpp:27-205, src/core/grid_map.hpp:58-115,
+// src/solvers/lm_solver.cpp:30-82, 275-296). Normal equations are formed
+// into fixed factor slots with fixed-order reductions (no float atomics),
+// the numeric factorization is a level-scheduled multifrontal LDL^T, and the
+// LM control of lm_solver.cpp:366-425 runs in a one-block kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace sdfslam {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kNB = 32;     // block column width of the panel factorization
+constexpr int kTile = 64;   // GEMM tile edge
+constexpr int kTk = 16;     // GEMM k chunk
+
+__device__ __forceinline__ bool Halted(const LmState* st) { return st->halt != 0; }
+
+__device__ __forceinline__ double WrapAngle(double a) { return atan2(sin(a), cos(a)); }
+
+__device__ __forceinline__ double WaveSum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v += __shfl_xor(v, off, 64);
+  }
+  return v;
+}
+
+unsigned Blocks(int64_t n, int threads = kThreads, int64_t cap = 1 << 20) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return static_cast<unsigned>(b);
+}
+
+// ---------------------------------------------------------------- utilities
+
+__global__ void KFill(double* p, int64_t n, double value) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    p[i] = value;
+  }
+}
+
+__global__ void KFillI32(int32_t* p, int64_t n, int32_t value) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    p[i] = value;
+  }
+}
+
+// ---------------------------------------------------------------- assembly
+
+struct CellW {
+  int w, h;
+  double a, b;
+  double wt[4];
+};
+
+// GridMap::FillCell (grid_map.hpp:109-115)
+__device__ __forceinline__ void FillWeights(CellW& c) {
+  c.wt[0] = (1.0 - c.a) * (1.0 - c.b);
+  c.wt[1] = c.a * (1.0 - c.b);
+  c.wt[2] = (1.0 - c.a) * c.b;
+  c.wt[3] = c.a * c.b;
+}
+
+// GridMap::Interpolate (grid_map.hpp:86-93)
+__device__ __forceinline__ double InterpolateCell(const GridParams& g, const double* map,
+                                                  const CellW& c) {
+  const int n00 = c.h * g.nx + c.w;
+  double value = 0.0;
+  value += c.wt[0] * map[n00];
+  value += c.wt[1] * map[n00 + 1];
+  value += c.wt[2] * map[n00 + g.nx];
+  value += c.wt[3] * map[n00 + g.nx + 1];
+  return value;
+}
+
+// GridMap::LocateClamped (grid_map.hpp:75-83) + Interpolate
+__device__ double InterpolateClamped(const GridParams& g, const double* map, double x, double y) {
+  const double fx = (x - g.min_x) / g.dx;
+  const double fy = (y - g.min_y) / g.dy;
+  CellW c;
+  c.w = min(max(static_cast<int>(fx), 0), g.nx - 2);
+  c.h = min(max(static_cast<int>(fy), 0), g.ny - 2);
+  c.a = fx - c.w;
+  c.b = fy - c.h;
+  FillWeights(c);
+  return InterpolateCell(g, map, c);
+}
+
+// Point residual and Jacobian (residual_math.hpp:75-106) with the IRLS
+// scaling of AssemblePointRows (lm_solver.cpp:30-61). One thread per row.
+__global__ void KAssemblePoints(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int check) {
+  if (Halted(W.state)) return;
+  const int64_t s = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (s >= P.num_points) return;
+  const GridParams g = P.grid;
+  const int f = P.pt_frame[s];
+  const double px = P.pt_x[s];
+  const double py = P.pt_y[s];
+  const double delta = P.pt_delta[s];
+  const double sws = P.pt_sw[s];
+  const double tx = W.poses[3 * f];
+  const double ty = W.poses[3 * f + 1];
+  const double th = W.poses[3 * f + 2];
+  const double c = cos(th);
+  const double sn = sin(th);
+  const double gx = c * px + (-sn) * py + tx;
+  const double gy = sn * px + c * py + ty;
+  const double fx = (gx - g.min_x) / g.dx;
+  const double fy = (gy - g.min_y) / g.dy;
+  const bool valid = !(fx < 0.0 || fy < 0.0 || fx > g.nx - 1 || fy > g.ny - 1);
+  double residual;
+  double wt[4] = {0.0, 0.0, 0.0, 0.0};
+  double dp[3] = {0.0, 0.0, 0.0};
+  uint32_t key = static_cast<uint32_t>(P.num_cells);
+  if (!valid) {
+    residual = -delta;
+  } else {
+    CellW cell;
+    cell.w = min(static_cast<int>(fx), g.nx - 2);
+    cell.h = min(static_cast<int>(fy), g.ny - 2);
+    cell.a = fx - cell.w;
+    cell.b = fy - cell.h;
+    FillWeights(cell);
+    residual = InterpolateCell(g, W.map, cell) - delta;
+    double grx, gry;
+    if (P.smooth_gradient != 0 && P.smooth_gradient_step > 0.0) {
+      // SmoothGradientMeters (residual_math.hpp:63-73)
+      const double st = P.smooth_gradient_step;
+      grx = (InterpolateClamped(g, W.map, gx + st, gy) - InterpolateClamped(g, W.map, gx - st, gy)) /
+            (2.0 * st);
+      gry = (InterpolateClamped(g, W.map, gx, gy + st) - InterpolateClamped(g, W.map, gx, gy - st)) /
+            (2.0 * st);
+    } else if (P.smooth_gradient != 0) {
+      // SmoothGradient (residual_math.hpp:40-57)
+      double ngx[4], ngy[4];
+      const int ws[4] = {cell.w, cell.w + 1, cell.w, cell.w + 1};
+      const int hs[4] = {cell.h, cell.h, cell.h + 1, cell.h + 1};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int w = ws[k];
+        const int h = hs[k];
+        const int w_lo = max(w - 1, 0);
+        const int w_hi = min(w + 1, g.nx - 1);
+        const int h_lo = max(h - 1, 0);
+        const int h_hi = min(h + 1, g.ny - 1);
+        ngx[k] = (W.map[h * g.nx + w_hi] - W.map[h * g.nx + w_lo]) / ((w_hi - w_lo) * g.dx);
+        ngy[k] = (W.map[h_hi * g.nx + w] - W.map[h_lo * g.nx + w]) / ((h_hi - h_lo) * g.dy);
+      }
+      const double a = cell.a;
+      const double b = cell.b;
+      const double s00 = (1.0 - a) * (1.0 - b);
+      const double s10 = a * (1.0 - b);
+      const double s01 = (1.0 - a) * b;
+      const double s11 = a * b;
+      grx = s00 * ngx[0] + s10 * ngx[1] + s01 * ngx[2] + s11 * ngx[3];
+      gry = s00 * ngy[0] + s10 * ngy[1] + s01 * ngy[2] + s11 * ngy[3];
+    } else {
+      // GridMap::Gradient (grid_map.hpp:98-106)
+      const int n00 = cell.h * g.nx + cell.w;
+      const double d00 = W.map[n00];
+      const double d10 = W.map[n00 + 1];
+      const double d01 = W.map[n00 + g.nx];
+      const double d11 = W.map[n00 + g.nx + 1];
+      grx = ((1.0 - cell.b) * (d10 - d00) + cell.b * (d11 - d01)) / g.dx;
+      gry = ((1.0 - cell.a) * (d01 - d00) + cell.a * (d11 - d10)) / g.dy;
+    }
+    const double dxdt = -sn * px - c * py;
+    const double dydt = c * px - sn * py;
+    dp[0] = grx;
+    dp[1] = gry;
+    dp[2] = grx * dxdt + gry * dydt;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wt[k] = cell.wt[k];
+    key = static_cast<uint32_t>(cell.h * (g.nx - 1) + cell.w);
+  }
+  // Huber IRLS weight (problem.hpp:115-121, lm_solver.cpp:39-47).
+  const double weighted = sws * residual;
+  double hw = 1.0;
+  if (P.huber_delta > 0.0) {
+    const double ar = fabs(weighted);
+    hw = ar <= P.huber_delta ? 1.0 : P.huber_delta / ar;
+  }
+  const double swh = sqrt(hw);
+  W.r_pt[s] = swh * weighted;
+  const double scale = swh * sws;
+  const int64_t np = P.num_points;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) W.jw[k * np + s] = valid ? scale * wt[k] : 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) W.jp[k * np + s] = valid ? scale * dp[k] : 0.0;
+  W.cellkey[s] = key;
+  if (check != 0 && valid && f >= 1) {
+    const uint32_t word = A.fp_cellbits[static_cast<int64_t>(f) * A.words_per_pose + (key >> 5)];
+    if (((word >> (key & 31U)) & 1U) == 0U) {
+      W.state->overflow = 1;
+      W.state->halt = 1;
+    }
+  }
+}
+
+// EvalEikonalResidual (residual_math.hpp:158-171), AssembleEikonalRows
+// (lm_solver.cpp:67-82): residual and the three scaled Jacobian entries.
+__global__ void KAssembleEikonal(DeviceProblem P, DeviceWork W, double* j_eik) {
+  if (Halted(W.state)) return;
+  const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (e >= P.num_eikonal) return;
+  const GridParams g = P.grid;
+  const int w = P.eik_w[e];
+  const int h = P.eik_h[e];
+  const double nx = P.eik_nx[e];
+  const double ny = P.eik_ny[e];
+  const double sw = P.eik_sw[e];
+  const double d00 = W.map[h * g.nx + w];
+  const double d10 = W.map[h * g.nx + w + 1];
+  const double d01 = W.map[(h + 1) * g.nx + w];
+  const double grx = (d10 - d00) / g.dx;
+  const double gry = (d01 - d00) / g.dy;
+  W.r_eik[e] = sw * (1.0 - (nx * grx + ny * gry));
+  if (j_eik != nullptr) {
+    j_eik[3 * e] = sw * (nx / g.dx + ny / g.dy);
+    j_eik[3 * e + 1] = sw * (-nx / g.dx);
+    j_eik[3 * e + 2] = sw * (-ny / g.dy);
+  }
+}
+
+// EvalOdomResidual (residual_math.hpp:182-205), scaled as in the odometry
+// tail of SolveLm (lm_solver.cpp:275-296).
+__global__ void KAssembleOdom(DeviceProblem P, DeviceWork W) {
+  if (Halted(W.state)) return;
+  const int64_t o = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (o >= P.num_odom) return;
+  const int i = P.od_i[o];
+  const int j = P.od_j[o];
+  const double* pi = W.poses + 3 * i;
+  const double* pj = W.poses + 3 * j;
+  const double c = cos(pi[2]);
+  const double s = sin(pi[2]);
+  const double dtx = pj[0] - pi[0];
+  const double dty = pj[1] - pi[1];
+  const double relx = c * dtx + s * dty;
+  const double rely = (-s) * dtx + c * dty;
+  const double sw = P.od_sw[o];
+  W.od_r[3 * o] = sw * (relx - P.od_dx[o]);
+  W.od_r[3 * o + 1] = sw * (rely - P.od_dy[o]);
+  W.od_r[3 * o + 2] = sw * WrapAngle(pj[2] - pi[2] - P.od_dth[o]);
+  double* ji = W.od_ji + 9 * o;
+  double* jj = W.od_jj + 9 * o;
+  ji[0] = sw * (-c);
+  ji[1] = sw * (-s);
+  ji[2] = sw * ((-s) * dtx + c * dty);
+  ji[3] = sw * s;
+  ji[4] = sw * (-c);
+  ji[5] = sw * ((-c) * dtx + (-s) * dty);
+  ji[6] = sw * 0.0;
+  ji[7] = sw * 0.0;
+  ji[8] = sw * -1.0;
+  jj[0] = sw * c;
+  jj[1] = sw * s;
+  jj[2] = sw * 0.0;
+  jj[3] = sw * (-s);
+  jj[4] = sw * c;
+  jj[5] = sw * 0.0;
+  jj[6] = sw * 0.0;
+  jj[7] = sw * 0.0;
+  jj[8] = sw * 1.0;
+}
+
+// ---------------------------------------------------------------- J^T J
+
+// Range of each cell in the cell-sorted point order.
+__global__ void KCellRanges(int64_t np, const uint32_t* key, int32_t* begin, int32_t* end,
+                            const LmState* st) {
+  if (Halted(st)) return;
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t >= np) return;
+  const uint32_t k = key[t];
+  if (t == 0 || key[t - 1] != k) begin[k] = static_cast<int32_t>(t);
+  if (t == np - 1 || key[t + 1] != k) end[k] = static_cast<int32_t>(t + 1);
+}
+
+// Per cell: the 10 products of its node weights and the 4 weight x residual
+// sums over the cell's points (all frames), one wavefront per cell, lanes in
+// a fixed stride and a fixed butterfly.
+__global__ void KCellReduce(DeviceProblem P, DeviceWork W) {
+  if (Halted(W.state)) return;
+  const int lane = threadIdx.x & 63;
+  const int cell = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  if (cell >= P.num_cells) return;
+  const int b = W.cell_begin[cell];
+  const int e = W.cell_end[cell];
+  const int64_t np = P.num_points;
+  double acc[14];
+#pragma unroll
+  for (int k = 0; k < 14; ++k) acc[k] = 0.0;
+  for (int t = b + lane; t < e; t += 64) {
+    const int row = W.sorted_row[t];
+    const double w0 = W.jw[row];
+    const double w1 = W.jw[np + row];
+    const double w2 = W.jw[2 * np + row];
+    const double w3 = W.jw[3 * np + row];
+    const double r = W.r_pt[row];
+    acc[0] += w0 * w0;
+    acc[1] += w0 * w1;
+    acc[2] += w0 * w2;
+    acc[3] += w0 * w3;
+    acc[4] += w1 * w1;
+    acc[5] += w1 * w2;
+    acc[6] += w1 * w3;
+    acc[7] += w2 * w2;
+    acc[8] += w2 * w3;
+    acc[9] += w3 * w3;
+    acc[10] += w0 * r;
+    acc[11] += w1 * r;
+    acc[12] += w2 * r;
+    acc[13] += w3 * r;
+  }
+#pragma unroll
+  for (int k = 0; k < 14; ++k) acc[k] = WaveSum(acc[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 10; ++k) W.nn[static_cast<int64_t>(k) * P.num_cells + cell] = acc[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) W.rn[static_cast<int64_t>(k) * P.num_cells + cell] = acc[10 + k];
+  }
+}
+
+__device__ __forceinline__ double Damp(double diag, double lambda, int marquardt) {
+  // lm_solver.cpp:321-330
+  return marquardt != 0 ? diag + lambda * fmax(diag, 1e-12) : diag + lambda;
+}
+
+// Node-node slots (9-point stencil, lower triangle owned by the node with the
+// smaller grid id) and the node entries of -J^T r. One thread per grid node.
+__global__ void KNodeSlots(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int marquardt) {
+  if (Halted(W.state)) return;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = P.grid.nx;
+  const int ny = P.grid.ny;
+  if (g >= nx * ny) return;
+  const int cu = P.node_to_active[g];
+  if (cu < 0) return;
+  const int w = g % nx;
+  const int h = g / nx;
+  const int ncx = nx - 1;
+  const int ncy = ny - 1;
+  const int nc = P.num_cells;
+  const double lambda = W.state->lambda;
+  auto ok = [&](int cw, int ch) { return cw >= 0 && ch >= 0 && cw < ncx && ch < ncy; };
+  auto nn = [&](int cw, int ch, int t) { return W.nn[static_cast<int64_t>(t) * nc + ch * ncx + cw]; };
+  auto rn = [&](int cw, int ch, int t) { return W.rn[static_cast<int64_t>(t) * nc + ch * ncx + cw]; };
+  const int64_t* off = A.nn_off + 5 * static_cast<int64_t>(g);
+  const double* eik = A.nn_eik + 5 * static_cast<int64_t>(g);
+  // Upper-triangle index of corner pair (a <= b) in the 10 cell products.
+  // (0,0)0 (0,1)1 (0,2)2 (0,3)3 (1,1)4 (1,2)5 (1,3)6 (2,2)7 (2,3)8 (3,3)9
+  {
+    double v = 0.0;
+    if (ok(w - 1, h - 1)) v += nn(w - 1, h - 1, 9);
+    if (ok(w, h - 1)) v += nn(w, h - 1, 7);
+    if (ok(w - 1, h)) v += nn(w - 1, h, 4);
+    if (ok(w, h)) v += nn(w, h, 0);
+    v += eik[0];
+    W.panel[off[0]] = Damp(v, lambda, marquardt);
+  }
+  if (off[1] >= 0) {  // right neighbour (w+1, h)
+    double v = 0.0;
+    if (ok(w, h - 1)) v += nn(w, h - 1, 8);
+    if (ok(w, h)) v += nn(w, h, 1);
+    W.panel[off[1]] = v + eik[1];
+  }
+  if (off[2] >= 0) {  // up-left neighbour (w-1, h+1)
+    double v = 0.0;
+    if (ok(w - 1, h)) v += nn(w - 1, h, 5);
+    W.panel[off[2]] = v + eik[2];
+  }
+  if (off[3] >= 0) {  // up neighbour (w, h+1)
+    double v = 0.0;
+    if (ok(w - 1, h)) v += nn(w - 1, h, 6);
+    if (ok(w, h)) v += nn(w, h, 2);
+    W.panel[off[3]] = v + eik[3];
+  }
+  if (off[4] >= 0) {  // up-right neighbour (w+1, h+1)
+    double v = 0.0;
+    if (ok(w, h)) v += nn(w, h, 3);
+    W.panel[off[4]] = v + eik[4];
+  }
+  double gsum = 0.0;
+  if (ok(w - 1, h - 1)) gsum += rn(w - 1, h - 1, 3);
+  if (ok(w, h - 1)) gsum += rn(w, h - 1, 2);
+  if (ok(w - 1, h)) gsum += rn(w - 1, h, 1);
+  if (ok(w, h)) gsum += rn(w, h, 0);
+  const double dx = P.grid.dx;
+  const double dy = P.grid.dy;
+  int e = A.eik_at[g];
+  if (e >= 0) gsum += (P.eik_sw[e] * (P.eik_nx[e] / dx + P.eik_ny[e] / dy)) * W.r_eik[e];
+  if (w >= 1) {
+    e = A.eik_at[g - 1];
+    if (e >= 0) gsum += (P.eik_sw[e] * (-P.eik_nx[e] / dx)) * W.r_eik[e];
+  }
+  if (h >= 1) {
+    e = A.eik_at[g - nx];
+    if (e >= 0) gsum += (P.eik_sw[e] * (-P.eik_ny[e] / dy)) * W.r_eik[e];
+  }
+  W.rhs[cu] = -gsum;
+}
+
+// Pose rows: footprint (pose x node) entries, the pose's 3x3 diagonal block
+// and its -J^T r entries. One workgroup per pose; the pose's point rows are
+// sorted by cell in LDS (bitonic) so every entry sums its points in a fixed
+// order.
+__global__ void KPoseBlock(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int marquardt,
+                           int sort_size) {
+  if (Halted(W.state)) return;
+  const int pose = blockIdx.x + 1;
+  const int tid = threadIdx.x;
+  extern __shared__ unsigned char smem[];
+  uint64_t* keys = reinterpret_cast<uint64_t*>(smem);
+  int32_t* rows = reinterpret_cast<int32_t*>(keys + sort_size);
+  int32_t* ucell = rows + sort_size;
+  int32_t* ustart = ucell + sort_size;
+  __shared__ int32_t counts[kThreads + 1];
+  __shared__ double red[9][kThreads];
+  const int64_t rb = A.pr_ptr[pose];
+  const int m = static_cast<int>(A.pr_ptr[pose + 1] - rb);
+  const int64_t np = P.num_points;
+  const uint32_t invalid = static_cast<uint32_t>(P.num_cells);
+  for (int i = tid; i < sort_size; i += kThreads) {
+    if (i < m) {
+      const int32_t row = A.pr_row[rb + i];
+      rows[i] = row;
+      keys[i] = (static_cast<uint64_t>(W.cellkey[row]) << 32) | static_cast<uint32_t>(i);
+    } else {
+      keys[i] = ~0ULL;
+    }
+  }
+  __syncthreads();
+  // Bitonic sort of (cell, local row) keys.
+  for (int k = 2; k <= sort_size; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = tid; i < sort_size; i += kThreads) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint64_t a = keys[i];
+          const uint64_t b = keys[l];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // Unique valid cells and their start positions (fixed-order compaction).
+  const int per = (sort_size + kThreads - 1) / kThreads;
+  const int lo = tid * per;
+  const int hi = min(lo + per, sort_size);
+  int cnt = 0;
+  for (int t = lo; t < hi; ++t) {
+    const uint32_t c = static_cast<uint32_t>(keys[t] >> 32);
+    if (c < invalid && (t == 0 || static_cast<uint32_t>(keys[t - 1] >> 32) != c)) ++cnt;
+  }
+  counts[tid] = cnt;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int i = 0; i < kThreads; ++i) {
+      const int c = counts[i];
+      counts[i] = run;
+      run += c;
+    }
+    counts[kThreads] = run;
+  }
+  __syncthreads();
+  int pos = counts[tid];
+  int valid_end_local = -1;
+  for (int t = lo; t < hi; ++t) {
+    const uint32_t c = static_cast<uint32_t>(keys[t] >> 32);
+    if (c < invalid) {
+      if (t == 0 || static_cast<uint32_t>(keys[t - 1] >> 32) != c) {
+        ucell[pos] = static_cast<int32_t>(c);
+        ustart[pos] = t;
+        ++pos;
+      }
+      if (t + 1 == sort_size || static_cast<uint32_t>(keys[t + 1] >> 32) >= invalid) {
+        valid_end_local = t + 1;
+      }
+    }
+  }
+  const int num_u = counts[kThreads];
+  __syncthreads();
+  if (valid_end_local >= 0) ustart[num_u] = valid_end_local;
+  if (num_u == 0 && tid == 0) ustart[0] = 0;
+  __syncthreads();
+
+  // Footprint entries: sum over the (up to) four cells around the node.
+  const int nx = P.grid.nx;
+  const int ncx = nx - 1;
+  const int ncy = P.grid.ny - 1;
+  const double* jp0 = W.jp;
+  const double* jp1 = W.jp + np;
+  const double* jp2 = W.jp + 2 * np;
+  for (int64_t q = A.fp_ptr[pose] + tid; q < A.fp_ptr[pose + 1]; q += kThreads) {
+    const int g = A.fp_node[q];
+    const int w = g % nx;
+    const int h = g / nx;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    const int cws[4] = {w - 1, w, w - 1, w};
+    const int chs[4] = {h - 1, h - 1, h, h};
+    const int corner[4] = {3, 2, 1, 0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int cw = cws[k];
+      const int ch = chs[k];
+      if (cw < 0 || ch < 0 || cw >= ncx || ch >= ncy) continue;
+      const int cid = ch * ncx + cw;
+      int lo2 = 0, hi2 = num_u;
+      while (lo2 < hi2) {
+        const int mid = (lo2 + hi2) >> 1;
+        if (ucell[mid] < cid) lo2 = mid + 1; else hi2 = mid;
+      }
+      if (lo2 >= num_u || ucell[lo2] != cid) continue;
+      const double* jw = W.jw + static_cast<int64_t>(corner[k]) * np;
+      for (int t = ustart[lo2]; t < ustart[lo2 + 1]; ++t) {
+        const int row = rows[static_cast<uint32_t>(keys[t] & 0xffffffffULL)];
+        const double wv = jw[row];
+        v0 += jp0[row] * wv;
+        v1 += jp1[row] * wv;
+        v2 += jp2[row] * wv;
+      }
+    }
+    const int64_t o = A.fp_off[q];
+    const int64_t st = A.fp_stride[q];
+    W.panel[o] = v0;
+    W.panel[o + st] = v1;
+    W.panel[o + 2 * st] = v2;
+  }
+  // Diagonal block and -J^T r over the pose's rows (strided, then a fixed tree).
+  double acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  for (int i = tid; i < m; i += kThreads) {
+    const int row = rows[i];
+    if (W.cellkey[row] >= invalid) continue;
+    const double a0 = jp0[row];
+    const double a1 = jp1[row];
+    const double a2 = jp2[row];
+    const double r = W.r_pt[row];
+    acc[0] += a0 * a0;
+    acc[1] += a1 * a0;
+    acc[2] += a1 * a1;
+    acc[3] += a2 * a0;
+    acc[4] += a2 * a1;
+    acc[5] += a2 * a2;
+    acc[6] += a0 * r;
+    acc[7] += a1 * r;
+    acc[8] += a2 * r;
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) red[k][tid] = acc[k];
+  __syncthreads();
+  for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
+    if (tid < stride) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) red[k][tid] += red[k][tid + stride];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double d[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) d[k] = red[k][0];
+    // Odometry/relation rows touching this pose, in spec order.
+    for (int64_t q = A.po_ptr[pose]; q < A.po_ptr[pose + 1]; ++q) {
+      const int o = A.po_spec[q];
+      const double* J = (P.od_i[o] == pose) ? W.od_ji + 9 * static_cast<int64_t>(o)
+                                            : W.od_jj + 9 * static_cast<int64_t>(o);
+      const double* r = W.od_r + 3 * static_cast<int64_t>(o);
+      const int ij[6][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+      for (int k = 0; k < 6; ++k) {
+        const int a = ij[k][0];
+        const int b = ij[k][1];
+        d[k] += J[a] * J[b] + J[3 + a] * J[3 + b] + J[6 + a] * J[6 + b];
+      }
+      for (int a = 0; a < 3; ++a) {
+        d[6 + a] += J[a] * r[0] + J[3 + a] * r[1] + J[6 + a] * r[2];
+      }
+    }
+    const double lambda = W.state->lambda;
+    d[0] = Damp(d[0], lambda, marquardt);
+    d[2] = Damp(d[2], lambda, marquardt);
+    d[5] = Damp(d[5], lambda, marquardt);
+    const int64_t* po = A.pd_off + 6 * static_cast<int64_t>(pose);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) W.panel[po[k]] = d[k];
+    const int pc = P.num_active + 3 * (pose - 1);
+    W.rhs[pc] = -d[6];
+    W.rhs[pc + 1] = -d[7];
+    W.rhs[pc + 2] = -d[8];
+  }
+}
+
+// Off-diagonal pose-pose blocks from odometry/relation rows (spec order).
+__global__ void KOdomPairs(DeviceProblem P, DeviceAnalysis A, DeviceWork W) {
+  if (Halted(W.state)) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.num_pairs) return;
+  const int pp = A.op_pose[2 * k];
+  double v[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) v[t] = 0.0;
+  for (int64_t q = A.op_ptr[k]; q < A.op_ptr[k + 1]; ++q) {
+    const int o = A.op_spec[q];
+    const bool p_is_i = P.od_i[o] == pp;
+    const double* JP = p_is_i ? W.od_ji + 9 * static_cast<int64_t>(o) : W.od_jj + 9 * static_cast<int64_t>(o);
+    const double* JQ = p_is_i ? W.od_jj + 9 * static_cast<int64_t>(o) : W.od_ji + 9 * static_cast<int64_t>(o);
+    for (int a = 0; a < 3; ++a) {
+      for (int b = 0; b < 3; ++b) {
+        v[3 * a + b] += JP[a] * JQ[b] + JP[3 + a] * JQ[3 + b] + JP[6 + a] * JQ[6 + b];
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 9; ++t) W.panel[A.op_off[9 * static_cast<int64_t>(k) + t]] = v[t];
+}
+
+// ---------------------------------------------------------------- factorization
+
+// Extend-add of a child's update matrix: columns [0, split) land in the
+// parent's panel, columns [split, m_c) in the parent's update matrix.
+__global__ void KExtendAddPanel(DeviceAnalysis A, DeviceWork W, const int32_t* children) {
+  if (Halted(W.state)) return;
+  const int c = children[blockIdx.y];
+  const int b = blockIdx.x;
+  if (b >= A.rel_split[c]) return;
+  const int p = A.sn_parent[c];
+  const int mc = A.sn_m[c];
+  const double* uc = W.upd + A.upd_off[c];
+  const int32_t* rel = A.rel + A.rel_off[c];
+  const int fp = A.sn_f[p];
+  double* fpanel = W.panel + A.panel_off[p];
+  const int64_t col = static_cast<int64_t>(rel[b]) * fp;
+  for (int a = b + threadIdx.x; a < mc; a += blockDim.x) {
+    fpanel[col + rel[a]] += uc[static_cast<int64_t>(b) * mc + a];
+  }
+}
+
+__global__ void KExtendAddUpdate(DeviceAnalysis A, DeviceWork W, const int32_t* children) {
+  if (Halted(W.state)) return;
+  const int c = children[blockIdx.y];
+  const int mc = A.sn_m[c];
+  const int b = A.rel_split[c] + blockIdx.x;
+  if (b >= mc) return;
+  const int p = A.sn_parent[c];
+  const int np = A.sn_first[p + 1] - A.sn_first[p];
+  const int mp = A.sn_m[p];
+  const double* uc = W.upd + A.upd_off[c];
+  const int32_t* rel = A.rel + A.rel_off[c];
+  double* up = W.upd + A.upd_off[p];
+  const int64_t col = static_cast<int64_t>(rel[b] - np) * mp;
+  for (int a = b + threadIdx.x; a < mc; a += blockDim.x) {
+    up[col + (rel[a] - np)] += uc[static_cast<int64_t>(b) * mc + a];
+  }
+}
+
+// LDL^T of the kb-th diagonal block of each front at the level (in LDS).
+__global__ void KDiagFactor(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn, int kb) {
+  if (Halted(W.state)) return;
+  const int s = level_sn[blockIdx.x];
+  const int ns = A.sn_first[s + 1] - A.sn_first[s];
+  const int c0 = kb * kNB;
+  if (c0 >= ns) return;
+  const int nb = min(kNB, ns - c0);
+  const int f = A.sn_f[s];
+  double* F = W.panel + A.panel_off[s];
+  __shared__ double a[kNB][kNB + 1];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+    const int i = idx % nb;
+    const int j = idx / nb;
+    if (i >= j) a[i][j] = F[static_cast<int64_t>(c0 + j) * f + c0 + i];
+  }
+  __syncthreads();
+  bool zero_pivot = false;
+  for (int k = 0; k < nb; ++k) {
+    const double d = a[k][k];
+    if (d == 0.0) zero_pivot = true;
+    for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+      const int i = idx % nb;
+      const int j = idx / nb;
+      if (j > k && i >= j) a[i][j] = fma(-a[i][k], a[j][k] / d, a[i][j]);
+    }
+    __syncthreads();
+    for (int i = k + 1 + tid; i < nb; i += blockDim.x) a[i][k] = a[i][k] / d;
+    __syncthreads();
+  }
+  if (zero_pivot && tid == 0) {
+    W.state->not_spd = 1;
+    W.state->halt = 1;
+  }
+  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+    const int i = idx % nb;
+    const int j = idx / nb;
+    if (i >= j) F[static_cast<int64_t>(c0 + j) * f + c0 + i] = a[i][j];
+  }
+}
+
+// Rows below the kb-th diagonal block: L = F L_kk^-T D_k^-1 (one row per thread).
+__global__ void KPanelTrsm(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn, int kb) {
+  if (Halted(W.state)) return;
+  const int s = level_sn[blockIdx.y];
+  const int ns = A.sn_first[s + 1] - A.sn_first[s];
+  const int c0 = kb * kNB;
+  if (c0 >= ns) return;
+  const int nb = min(kNB, ns - c0);
+  const int f = A.sn_f[s];
+  const int r0 = c0 + nb + blockIdx.x * blockDim.x;
+  if (r0 >= f) return;
+  double* F = W.panel + A.panel_off[s];
+  __shared__ double L[kNB][kNB + 1];
+  for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+    const int i = idx % nb;
+    const int j = idx / nb;
+    if (i >= j) L[i][j] = F[static_cast<int64_t>(c0 + j) * f + c0 + i];
+  }
+  __syncthreads();
+  const int r = r0 + threadIdx.x;
+  if (r >= f) return;
+  double y[kNB];
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (j < nb) {
+      double x = F[static_cast<int64_t>(c0 + j) * f + r];
+#pragma unroll
+      for (int t = 0; t < kNB; ++t) {
+        if (t < j) x = fma(-y[t], L[j][t], x);
+      }
+      y[j] = x;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (j < nb) F[static_cast<int64_t>(c0 + j) * f + r] = y[j] / L[j][j];
+  }
+}
+
+// C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
+// mode 0: trailing panel update after block column kb (K = nb);
+// mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated.
+__global__ void __launch_bounds__(kThreads) KGemmLower(DeviceAnalysis A, DeviceWork W,
+                                                       const int32_t* level_sn, int kb, int mode) {
+  if (Halted(W.state)) return;
+  const int ti = blockIdx.x;
+  const int tj = blockIdx.y;
+  if (tj > ti) return;
+  const int s = level_sn[blockIdx.z];
+  const int ns = A.sn_first[s + 1] - A.sn_first[s];
+  const int f = A.sn_f[s];
+  const double* F = W.panel + A.panel_off[s];
+  int kbase, K, rbase, M, N;
+  double* C;
+  int64_t ldc;
+  if (mode == 0) {
+    const int c0 = kb * kNB;
+    if (c0 >= ns) return;
+    const int nb = min(kNB, ns - c0);
+    kbase = c0;
+    K = nb;
+    rbase = c0 + nb;  // panel row of tile row 0 and panel column of tile col 0
+    M = f - rbase;
+    N = ns - rbase;
+    C = W.panel + A.panel_off[s] + static_cast<int64_t>(rbase) * f + rbase;
+    ldc = f;
+  } else {
+    const int ms = A.sn_m[s];
+    kbase = 0;
+    K = ns;
+    rbase = ns;
+    M = ms;
+    N = ms;
+    C = W.upd + A.upd_off[s];
+    ldc = ms;
+  }
+  const int R0 = ti * kTile;
+  const int C0 = tj * kTile;
+  if (R0 >= M || C0 >= N) return;
+  __shared__ double As[kTk][kTile + 1];
+  __shared__ double Bs[kTk][kTile + 1];
+  const int tid = threadIdx.x;
+  const int tx = tid & 15;
+  const int ty = tid >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < K; k0 += kTk) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + kThreads * e;
+      const int rr = idx & 63;
+      const int kk = idx >> 6;
+      const int kcol = kbase + k0 + kk;
+      const bool kin = k0 + kk < K;
+      const int64_t colbase = static_cast<int64_t>(kcol) * f;
+      const double dk = kin ? F[colbase + kcol] : 0.0;
+      As[kk][rr] = (kin && R0 + rr < M) ? F[colbase + rbase + R0 + rr] : 0.0;
+      Bs[kk][rr] = (kin && C0 + rr < N) ? F[colbase + rbase + C0 + rr] * dk : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kTk; ++kk) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = R0 + ty + 16 * i;
+    if (r >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = C0 + tx + 16 * j;
+      if (c >= N || c > r) continue;
+      double* dst = C + static_cast<int64_t>(c) * ldc + r;
+      *dst = mode == 0 ? *dst - acc[i][j] : -acc[i][j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- solves
+
+// Forward: gather the children's update vectors, then L y = b over the front
+// (unit lower), leaving y in the own rows and the update for the parent in R.
+__global__ void KForward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn) {
+  if (Halted(W.state)) return;
+  const int s = level_sn[blockIdx.x];
+  const int first = A.sn_first[s];
+  const int n = A.sn_first[s + 1] - first;
+  const int f = A.sn_f[s];
+  const double* F = W.panel + A.panel_off[s];
+  extern __shared__ double v[];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < f; i += blockDim.x) {
+    v[i] = i < n ? W.rhs[A.iperm[first + i]] : 0.0;
+  }
+  __syncthreads();
+  for (int64_t ci = A.children_ptr[s]; ci < A.children_ptr[s + 1]; ++ci) {
+    const int c = A.children[ci];
+    const int mc = A.sn_m[c];
+    const int nc = A.sn_first[c + 1] - A.sn_first[c];
+    const int32_t* rel = A.rel + A.rel_off[c];
+    const double* u = W.vec + A.sn_idx_ptr[c] + nc;
+    for (int a = tid; a < mc; a += blockDim.x) v[rel[a]] += u[a];
+    __syncthreads();
+  }
+  for (int b0 = 0; b0 < n; b0 += 64) {
+    const int b1 = min(b0 + 64, n);
+    if (tid < 64) {
+      const int i = b0 + tid;
+      double vi = i < b1 ? v[i] : 0.0;
+      for (int k = b0; k < b1; ++k) {
+        const double yk = __shfl(vi, k - b0, 64);
+        if (i > k && i < b1) vi = fma(-F[static_cast<int64_t>(k) * f + i], yk, vi);
+      }
+      if (i < b1) v[i] = vi;
+    }
+    __syncthreads();
+    for (int i = b1 + tid; i < f; i += blockDim.x) {
+      double acc = v[i];
+      for (int k = b0; k < b1; ++k) acc = fma(-F[static_cast<int64_t>(k) * f + i], v[k], acc);
+      v[i] = acc;
+    }
+    __syncthreads();
+  }
+  double* out = W.vec + A.sn_idx_ptr[s];
+  for (int i = tid; i < f; i += blockDim.x) out[i] = v[i];
+}
+
+// Backward: x1 = L11^-T (D^-1 y1 - L21^T x2), x2 from the ancestors.
+__global__ void KBackward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn) {
+  if (Halted(W.state)) return;
+  const int s = level_sn[blockIdx.x];
+  const int first = A.sn_first[s];
+  const int n = A.sn_first[s + 1] - first;
+  const int m = A.sn_m[s];
+  const int f = n + m;
+  const double* F = W.panel + A.panel_off[s];
+  const double* y = W.vec + A.sn_idx_ptr[s];
+  const int32_t* idx = A.sn_idx + A.sn_idx_ptr[s];
+  extern __shared__ double t[];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int waves = blockDim.x >> 6;
+  for (int j = tid; j < n; j += blockDim.x) t[j] = y[j] / F[static_cast<int64_t>(j) * f + j];
+  for (int a = tid; a < m; a += blockDim.x) t[n + a] = W.xnew[idx[n + a]];
+  __syncthreads();
+  for (int j = wave; j < n; j += waves) {
+    double acc = 0.0;
+    for (int a = lane; a < m; a += 64) acc = fma(F[static_cast<int64_t>(j) * f + n + a], t[n + a], acc);
+    acc = WaveSum(acc);
+    if (lane == 0) t[j] -= acc;
+  }
+  __syncthreads();
+  for (int b1 = n; b1 > 0;) {
+    const int b0 = max(0, b1 - 64);
+    for (int j = b0 + wave; j < b1; j += waves) {
+      double acc = 0.0;
+      for (int i = b1 + lane; i < n; i += 64) acc = fma(F[static_cast<int64_t>(j) * f + i], t[i], acc);
+      acc = WaveSum(acc);
+      if (lane == 0) t[j] -= acc;
+    }
+    __syncthreads();
+    if (tid < 64) {
+      const int i = b0 + lane;
+      double tv = i < b1 ? t[i] : 0.0;
+      for (int j = b1 - 1; j >= b0; --j) {
+        const double xj = __shfl(tv, j - b0, 64);
+        if (i < j) tv = fma(-F[static_cast<int64_t>(i) * f + j], xj, tv);
+      }
+      if (i < b1) t[i] = tv;
+    }
+    __syncthreads();
+    b1 = b0;
+  }
+  for (int j = tid; j < n; j += blockDim.x) {
+    W.xnew[first + j] = t[j];
+    W.step[A.iperm[first + j]] = t[j];
+  }
+}
+
+// ---------------------------------------------------------------- LM control
+
+__device__ double BlockSum(double v, double* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int stride = blockDim.x / 2; stride > 0; stride >>= 1) {
+    if (tid < stride) red[tid] += red[tid + stride];
+    __syncthreads();
+  }
+  const double out = red[0];
+  __syncthreads();
+  return out;
+}
+
+// Problem::Cost terms (problem.cpp:132-156) into per-block partial sums.
+__global__ void KCost(DeviceProblem P, const double* map, const double* poses, double* partials,
+                      const LmState* st, int ignore_halt) {
+  if (ignore_halt == 0 && Halted(st)) return;
+  __shared__ double red[kThreads];
+  const GridParams g = P.grid;
+  const int64_t total = P.num_points + P.num_eikonal + P.num_odom;
+  double acc = 0.0;
+  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (idx < P.num_points) {
+      const int64_t s = idx;
+      const int f = P.pt_frame[s];
+      const double px = P.pt_x[s];
+      const double py = P.pt_y[s];
+      const double c = cos(poses[3 * f + 2]);
+      const double sn = sin(poses[3 * f + 2]);
+      const double gx = c * px + (-sn) * py + poses[3 * f];
+      const double gy = sn * px + c * py + poses[3 * f + 1];
+      const double fx = (gx - g.min_x) / g.dx;
+      const double fy = (gy - g.min_y) / g.dy;
+      double residual = -P.pt_delta[s];
+      if (!(fx < 0.0 || fy < 0.0 || fx > g.nx - 1 || fy > g.ny - 1)) {
+        CellW cell;
+        cell.w = min(static_cast<int>(fx), g.nx - 2);
+        cell.h = min(static_cast<int>(fy), g.ny - 2);
+        cell.a = fx - cell.w;
+        cell.b = fy - cell.h;
+        FillWeights(cell);
+        residual = InterpolateCell(g, map, cell) - P.pt_delta[s];
+      }
+      const double r = P.pt_sw[s] * residual;
+      if (P.huber_delta > 0.0 && fabs(r) > P.huber_delta) {
+        acc += P.huber_delta * (2.0 * fabs(r) - P.huber_delta);
+      } else {
+        acc += r * r;
+      }
+    } else if (idx < P.num_points + P.num_eikonal) {
+      const int64_t e = idx - P.num_points;
+      const int w = P.eik_w[e];
+      const int h = P.eik_h[e];
+      const double d00 = map[h * g.nx + w];
+      const double d10 = map[h * g.nx + w + 1];
+      const double d01 = map[(h + 1) * g.nx + w];
+      const double grx = (d10 - d00) / g.dx;
+      const double gry = (d01 - d00) / g.dy;
+      const double r = P.eik_sw[e] * (1.0 - (P.eik_nx[e] * grx + P.eik_ny[e] * gry));
+      acc += r * r;
+    } else {
+      const int64_t o = idx - P.num_points - P.num_eikonal;
+      const double* pi = poses + 3 * P.od_i[o];
+      const double* pj = poses + 3 * P.od_j[o];
+      const double c = cos(pi[2]);
+      const double s = sin(pi[2]);
+      const double dtx = pj[0] - pi[0];
+      const double dty = pj[1] - pi[1];
+      const double sw = P.od_sw[o];
+      const double a = sw * ((c * dtx + s * dty) - P.od_dx[o]);
+      const double b = sw * (((-s) * dtx + c * dty) - P.od_dy[o]);
+      const double cc = sw * WrapAngle(pj[2] - pi[2] - P.od_dth[o]);
+      acc += a * a + b * b + cc * cc;
+    }
+  }
+  const double total_sum = BlockSum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total_sum;
+}
+
+__global__ void KLmInit(DeviceWork W, double lambda_init) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kPartialBlocks; i += blockDim.x) acc += W.partials[i];
+  const double cost = BlockSum(acc, red);
+  if (threadIdx.x == 0) {
+    LmState* st = W.state;
+    st->lambda = lambda_init;
+    st->previous_cost = cost;
+    st->nu = 2.0;
+    st->initial_cost = cost;
+    st->final_cost = cost;
+    st->last_cost = cost;
+    st->step_norm = 0.0;
+    st->iter = 0;
+    st->halt = 0;
+    st->converged = 0;
+    st->overflow = 0;
+    st->not_spd = 0;
+    st->revert = 0;
+    W.cost_history[0] = cost;
+  }
+}
+
+// Step norm and the predicted reduction step . (lambda step + rhs).
+__global__ void KStepReduce(int n, const double* step, const double* rhs, double* partials,
+                            const LmState* st) {
+  if (Halted(st)) return;
+  __shared__ double red[kThreads];
+  const double lambda = st->lambda;
+  double a = 0.0;
+  double b = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double s = step[i];
+    a += s * s;
+    b += s * (lambda * s + rhs[i]);
+  }
+  const double sa = BlockSum(a, red);
+  const double sb = BlockSum(b, red);
+  if (threadIdx.x == 0) {
+    partials[kPartialBlocks + blockIdx.x] = sa;
+    partials[2 * kPartialBlocks + blockIdx.x] = sb;
+  }
+}
+
+__global__ void KBackup(int nmap, int npose, DeviceWork W) {
+  if (Halted(W.state)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nmap + npose; i += gridDim.x * blockDim.x) {
+    if (i < nmap) W.map_backup[i] = W.map[i];
+    else W.poses_backup[i - nmap] = W.poses[i - nmap];
+  }
+}
+
+// Problem::ApplyStep (problem.cpp:117-130)
+__global__ void KApplyStep(DeviceProblem P, DeviceWork W) {
+  if (Halted(W.state)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P.num_active) {
+    W.map[P.active_to_node[i]] += W.step[i];
+  } else if (i < P.num_active + P.num_poses - 1) {
+    const int pose = i - P.num_active + 1;
+    const int col = P.num_active + 3 * (pose - 1);
+    W.poses[3 * pose] += W.step[col];
+    W.poses[3 * pose + 1] += W.step[col + 1];
+    W.poses[3 * pose + 2] = WrapAngle(W.poses[3 * pose + 2] + W.step[col + 2]);
+  }
+}
+
+// Convergence test and damping update of SolveLm (lm_solver.cpp:376-425).
+__global__ void KControl(DeviceWork W, LmOptions o) {
+  if (Halted(W.state)) return;
+  __shared__ double red[kThreads];
+  double c = 0.0, a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < kPartialBlocks; i += blockDim.x) {
+    c += W.partials[i];
+    a += W.partials[kPartialBlocks + i];
+    b += W.partials[2 * kPartialBlocks + i];
+  }
+  const double cost = BlockSum(c, red);
+  const double norm_sq = BlockSum(a, red);
+  const double predicted = BlockSum(b, red);
+  if (threadIdx.x != 0) return;
+  LmState* st = W.state;
+  st->revert = 0;
+  st->last_cost = cost;
+  st->iter += 1;
+  const int iter = st->iter;
+  st->step_norm = sqrt(norm_sq);
+  if (sqrt(norm_sq) < o.step_tolerance) {
+    if (iter < o.history_capacity) W.cost_history[iter] = cost;
+    st->final_cost = cost;
+    st->converged = 1;
+    st->halt = 1;
+    return;
+  }
+  double current = cost;
+  if (o.trust_region != 0) {
+    const double rho = predicted > 0.0 ? (st->previous_cost - cost) / predicted : -1.0;
+    if (rho > 0.0) {
+      st->previous_cost = cost;
+      const double shrink = 1.0 - pow(2.0 * rho - 1.0, 3.0);
+      st->lambda = fmax(st->lambda * fmax(1.0 / 3.0, shrink), 1e-7);
+      st->nu = 2.0;
+    } else {
+      st->revert = 1;
+      current = st->previous_cost;
+      st->lambda *= st->nu;
+      st->nu *= 2.0;
+    }
+  } else if (cost < st->previous_cost) {
+    st->lambda /= o.lambda_factor;
+    st->previous_cost = cost;
+  } else {
+    st->lambda *= o.lambda_factor;
+    if (o.reject_worse_steps != 0) {
+      st->revert = 1;
+      current = st->previous_cost;
+    } else {
+      st->previous_cost = cost;
+    }
+  }
+  if (iter < o.history_capacity) W.cost_history[iter] = current;
+  st->final_cost = current;
+  if (iter >= o.max_iterations) st->halt = 1;
+}
+
+__global__ void KSumPartials(const double* partials, double* out) {
+  __shared__ double red[kThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kPartialBlocks; i += blockDim.x) acc += partials[i];
+  const double total = BlockSum(acc, red);
+  if (threadIdx.x == 0) *out = total;
+}
+
+__global__ void KRevert(int nmap, int npose, DeviceWork W) {
+  if (W.state->revert == 0) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nmap + npose; i += gridDim.x * blockDim.x) {
+    if (i < nmap) W.map[i] = W.map_backup[i];
+    else W.poses[i - nmap] = W.poses_backup[i - nmap];
+  }
+}
+
+}  // namespace
+
+void LaunchFill(double* p, int64_t n, double value, hipStream_t s) {
+  if (n <= 0) return;
+  KFill<<<Blocks(n, kThreads, 4096), kThreads, 0, s>>>(p, n, value);
+}
+
+void LaunchFillI32(int32_t* p, int64_t n, int32_t value, hipStream_t s) {
+  if (n <= 0) return;
+  KFillI32<<<Blocks(n, kThreads, 4096), kThreads, 0, s>>>(p, n, value);
+}
+
+void LaunchAssemble(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
+                    bool check_footprint, hipStream_t s, hipEvent_t before, hipEvent_t after) {
+  if (before != nullptr) (void)hipEventRecord(before, s);
+  if (P.num_points > 0) {
+    KAssemblePoints<<<Blocks(P.num_points), kThreads, 0, s>>>(P, A, W, check_footprint ? 1 : 0);
+  }
+  if (after != nullptr) (void)hipEventRecord(after, s);
+  if (P.num_eikonal > 0) {
+    KAssembleEikonal<<<Blocks(P.num_eikonal), kThreads, 0, s>>>(P, W, nullptr);
+  }
+  if (P.num_odom > 0) {
+    KAssembleOdom<<<Blocks(P.num_odom), kThreads, 0, s>>>(P, W);
+  }
+}
+
+void LaunchAssembleEikonalJ(const DeviceProblem& P, const DeviceWork& W, double* j_eik,
+                            hipStream_t s) {
+  if (P.num_eikonal > 0) {
+    KAssembleEikonal<<<Blocks(P.num_eikonal), kThreads, 0, s>>>(P, W, j_eik);
+  }
+}
+
+void LaunchCellReduce(const DeviceProblem& P, const DeviceWork& W, hipStream_t s) {
+  LaunchFillI32(W.cell_begin, P.num_cells + 1, 0, s);
+  LaunchFillI32(W.cell_end, P.num_cells + 1, 0, s);
+  if (P.num_points > 0) {
+    KCellRanges<<<Blocks(P.num_points), kThreads, 0, s>>>(P.num_points, W.sorted_key, W.cell_begin,
+                                                          W.cell_end, W.state);
+  }
+  KCellReduce<<<Blocks(P.num_cells, kThreads / 64), kThreads, 0, s>>>(P, W);
+}
+
+void LaunchNormalSlots(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
+                       const LmOptions& o, int max_points_per_pose, hipStream_t s) {
+  KNodeSlots<<<Blocks(static_cast<int64_t>(P.grid.nx) * P.grid.ny), kThreads, 0, s>>>(
+      P, A, W, o.marquardt_scaling);
+  if (P.num_poses > 1) {
+    int sort_size = 1;
+    while (sort_size < max_points_per_pose) sort_size <<= 1;
+    const size_t lds = static_cast<size_t>(sort_size) * (8 + 4 + 4 + 4) + 4;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KPoseBlock),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    KPoseBlock<<<P.num_poses - 1, kThreads, lds, s>>>(P, A, W, o.marquardt_scaling, sort_size);
+  }
+  if (A.num_pairs > 0) {
+    KOdomPairs<<<Blocks(A.num_pairs), kThreads, 0, s>>>(P, A, W);
+  }
+}
+
+void LaunchExtendAddPanel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
+                          int count, int max_cols, hipStream_t s) {
+  if (count <= 0 || max_cols <= 0) return;
+  KExtendAddPanel<<<dim3(max_cols, count), kThreads, 0, s>>>(A, W, children);
+}
+
+void LaunchExtendAddUpdate(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
+                           int count, int max_cols, hipStream_t s) {
+  if (count <= 0 || max_cols <= 0) return;
+  KExtendAddUpdate<<<dim3(max_cols, count), kThreads, 0, s>>>(A, W, children);
+}
+
+void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
+                       int count, int max_n, int max_f, int max_m, hipStream_t s) {
+  const int nblocks = (max_n + kNB - 1) / kNB;
+  for (int kb = 0; kb < nblocks; ++kb) {
+    KDiagFactor<<<count, kThreads, 0, s>>>(A, W, level_sn, kb);
+    const int rows = max_f - kb * kNB;
+    if (rows > 0) {
+      KPanelTrsm<<<dim3((rows + kThreads - 1) / kThreads, count), kThreads, 0, s>>>(A, W, level_sn, kb);
+    }
+    const int trail_rows = max_f - (kb + 1) * kNB;
+    const int trail_cols = max_n - (kb + 1) * kNB;
+    if (trail_rows > 0 && trail_cols > 0) {
+      const int tr = (trail_rows + kTile - 1) / kTile;
+      const int tc = (trail_cols + kTile - 1) / kTile;
+      KGemmLower<<<dim3(tr, tc, count), kThreads, 0, s>>>(A, W, level_sn, kb, 0);
+    }
+  }
+  if (max_m > 0) {
+    const int t = (max_m + kTile - 1) / kTile;
+    KGemmLower<<<dim3(t, t, count), kThreads, 0, s>>>(A, W, level_sn, 0, 1);
+  }
+}
+
+void LaunchForwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
+                        int count, int max_f, hipStream_t s) {
+  KForward<<<count, kThreads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn);
+}
+
+void LaunchBackwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
+                         int count, int max_f, hipStream_t s) {
+  KBackward<<<count, kThreads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn);
+}
+
+void LaunchCost(const DeviceProblem& P, const DeviceWork& W, const double* map,
+                const double* poses, hipStream_t s) {
+  KCost<<<kPartialBlocks, kThreads, 0, s>>>(P, map, poses, W.partials, W.state, 1);
+}
+
+void LaunchSumPartials(const DeviceWork& W, double* out, hipStream_t s) {
+  KSumPartials<<<1, kThreads, 0, s>>>(W.partials, out);
+}
+
+void LaunchLmInit(const DeviceWork& W, double lambda_init, hipStream_t s) {
+  KLmInit<<<1, kThreads, 0, s>>>(W, lambda_init);
+}
+
+void LaunchStepAndControl(const DeviceProblem& P, const DeviceWork& W, const LmOptions& o,
+                          hipStream_t s) {
+  const int nmap = P.grid.nx * P.grid.ny;
+  const int npose = 3 * P.num_poses;
+  KStepReduce<<<kPartialBlocks, kThreads, 0, s>>>(P.n, W.step, W.rhs, W.partials, W.state);
+  if (o.reject_worse_steps != 0 || o.trust_region != 0) {
+    KBackup<<<Blocks(nmap + npose, kThreads, 1024), kThreads, 0, s>>>(nmap, npose, W);
+  }
+  KApplyStep<<<Blocks(P.num_active + P.num_poses), kThreads, 0, s>>>(P, W);
+  KCost<<<kPartialBlocks, kThreads, 0, s>>>(P, W.map, W.poses, W.partials, W.state, 0);
+  KControl<<<1, kThreads, 0, s>>>(W, o);
+  if (o.reject_worse_steps != 0 || o.trust_region != 0) {
+    KRevert<<<Blocks(nmap + npose, kThreads, 1024), kThreads, 0, s>>>(nmap, npose, W);
+  }
+}
+
+}  // namespace sdfslam

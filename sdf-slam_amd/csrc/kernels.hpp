@@ -1,0 +1,184 @@
+// Device kernels of the joint-LM backend and their launch wrappers.
+// All wrappers enqueue on the given stream and never synchronize.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sdfslam {
+
+struct GridParams {
+  int nx;
+  int ny;
+  double min_x;
+  double min_y;
+  double dx;
+  double dy;
+};
+
+/// LM state kept on the device (SolveLm's locals, lm_solver.cpp:204-215).
+struct LmState {
+  double lambda;
+  double previous_cost;
+  double nu;
+  double initial_cost;
+  double final_cost;
+  double last_cost;
+  double step_norm;
+  int iter;          // iterations finished (result.iterations)
+  int halt;          // 1 once converged, out of iterations, or failed
+  int converged;
+  int overflow;      // a point left its pose's footprint: re-analyse
+  int not_spd;       // zero pivot
+  int revert;        // restore the backup after this iteration
+};
+
+struct LmOptions {
+  int max_iterations;
+  int history_capacity;  // entries of cost_history
+  double step_tolerance;
+  double lambda_factor;
+  int reject_worse_steps;
+  int marquardt_scaling;
+  int trust_region;
+};
+
+/// Point specs, eikonal specs, odometry specs (SoA, device pointers).
+struct DeviceProblem {
+  GridParams grid;
+  int num_cells;
+  int num_active;
+  int num_poses;
+  int n;
+  int64_t num_points;
+  int64_t num_eikonal;
+  int64_t num_odom;
+  const int32_t* node_to_active;
+  const int32_t* active_to_node;
+  const int32_t* pt_frame;
+  const double* pt_x;
+  const double* pt_y;
+  const double* pt_delta;
+  const double* pt_sw;
+  const int32_t* eik_w;
+  const int32_t* eik_h;
+  const double* eik_nx;
+  const double* eik_ny;
+  const double* eik_sw;
+  const int32_t* od_i;
+  const int32_t* od_j;
+  const double* od_dx;
+  const double* od_dy;
+  const double* od_dth;
+  const double* od_sw;
+  double huber_delta;
+  int smooth_gradient;
+  double smooth_gradient_step;
+};
+
+/// Per-iteration work buffers.
+struct DeviceWork {
+  double* map;
+  double* poses;
+  double* map_backup;
+  double* poses_backup;
+  double* r_pt;       // point residuals (scaled)
+  double* jw;         // 4 x num_points scaled node weights
+  double* jp;         // 3 x num_points scaled pose partials
+  uint32_t* cellkey;  // cell id or num_cells when the point has no Jacobian row
+  double* r_eik;
+  double* od_r;   // 3 per spec
+  double* od_ji;  // 9 per spec, row-major (residual component, pose component)
+  double* od_jj;
+  const uint32_t* sorted_key;
+  const int32_t* sorted_row;
+  int32_t* cell_begin;
+  int32_t* cell_end;
+  double* nn;  // 10 x num_cells
+  double* rn;  // 4 x num_cells
+  double* rhs;  // -J^T r in original column order
+  double* step;  // step in original column order
+  double* panel;
+  double* upd;
+  double* vec;  // forward-solve vectors, sum of front sizes
+  double* xnew;  // solution in elimination order
+  double* partials;
+  double* cost_history;
+  LmState* state;
+};
+
+/// Device copies of the symbolic analysis.
+struct DeviceAnalysis {
+  const int32_t* perm;
+  const int32_t* iperm;
+  const int32_t* sn_first;
+  const int32_t* sn_m;
+  const int32_t* sn_f;
+  const int64_t* sn_idx_ptr;
+  const int32_t* sn_idx;
+  const int64_t* panel_off;
+  const int64_t* upd_off;
+  const int32_t* sn_parent;
+  const int64_t* rel_off;
+  const int32_t* rel;
+  const int32_t* rel_split;
+  const int64_t* nn_off;
+  const double* nn_eik;
+  const int32_t* eik_at;
+  const int64_t* fp_ptr;
+  const int32_t* fp_node;
+  const int64_t* fp_off;
+  const int32_t* fp_stride;
+  int words_per_pose;
+  const uint32_t* fp_cellbits;
+  const int64_t* pr_ptr;
+  const int32_t* pr_row;
+  const int64_t* pd_off;
+  const int64_t* po_ptr;
+  const int32_t* po_spec;
+  int num_pairs;
+  const int32_t* op_pose;
+  const int64_t* op_ptr;
+  const int32_t* op_spec;
+  const int64_t* op_off;
+  const int64_t* children_ptr;  // children of each supernode, ascending
+  const int32_t* children;
+};
+
+constexpr int kPartialBlocks = 512;
+
+void LaunchFill(double* p, int64_t n, double value, hipStream_t s);
+void LaunchFillI32(int32_t* p, int64_t n, int32_t value, hipStream_t s);
+
+/// Point, eikonal and odometry rows. When `before`/`after` are given they are
+/// recorded around the point-row kernel alone.
+void LaunchAssemble(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
+                    bool check_footprint, hipStream_t s, hipEvent_t before = nullptr,
+                    hipEvent_t after = nullptr);
+void LaunchCellReduce(const DeviceProblem& P, const DeviceWork& W, hipStream_t s);
+void LaunchNormalSlots(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
+                       const LmOptions& o, int max_points_per_pose, hipStream_t s);
+
+void LaunchExtendAddPanel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
+                          int count, int max_cols, hipStream_t s);
+void LaunchExtendAddUpdate(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
+                           int count, int max_cols, hipStream_t s);
+void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
+                       int count, int max_n, int max_f, int max_m, hipStream_t s);
+void LaunchForwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
+                        int count, int max_f, hipStream_t s);
+void LaunchBackwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
+                         int count, int max_f, hipStream_t s);
+
+void LaunchAssembleEikonalJ(const DeviceProblem& P, const DeviceWork& W, double* j_eik,
+                            hipStream_t s);
+void LaunchCost(const DeviceProblem& P, const DeviceWork& W, const double* map,
+                const double* poses, hipStream_t s);
+/// Sums the kPartialBlocks cost partials into *out (fixed order).
+void LaunchSumPartials(const DeviceWork& W, double* out, hipStream_t s);
+void LaunchLmInit(const DeviceWork& W, double lambda_init, hipStream_t s);
+void LaunchStepAndControl(const DeviceProblem& P, const DeviceWork& W, const LmOptions& o,
+                          hipStream_t s);
+
+}  // namespace sdfslam

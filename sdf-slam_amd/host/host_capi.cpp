@@ -1,0 +1,380 @@
+// C API of the host library (include/sdfslam_host.h).
+#include <cstring>
+#include <exception>
+#include <fstream>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+
+#include "problem.hpp"
+#include "scene.hpp"
+#include "sdfslam_host.h"
+#include "solver.hpp"
+
+using namespace sdf_slam;
+
+struct sdfslam_host_problem {
+  std::unique_ptr<Problem> problem;
+  Workload workload;  // empty for problems built from arrays
+  bool has_workload{false};
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int Fail(int code, const std::string& message) {
+  g_error = message;
+  return code;
+}
+
+template <typename F>
+int Guard(F&& fn) {
+  try {
+    return fn();
+  } catch (const std::invalid_argument& e) {
+    return Fail(SDFSLAM_ERR_INVALID_ARGUMENT, e.what());
+  } catch (const std::logic_error& e) {
+    return Fail(SDFSLAM_ERR_INVALID_ARGUMENT, e.what());
+  } catch (const std::bad_alloc& e) {
+    return Fail(SDFSLAM_ERR_OOM, e.what());
+  } catch (const std::runtime_error& e) {
+    const std::string what = e.what();
+    if (what == "sparse Cholesky factorization failed") {
+      return Fail(SDFSLAM_ERR_NOT_SPD, what);
+    }
+    return Fail(SDFSLAM_ERR_INTERNAL, what);
+  } catch (const std::exception& e) {
+    return Fail(SDFSLAM_ERR_INTERNAL, e.what());
+  }
+}
+
+ProblemOptions FromC(const sdfslam_problem_options& o) {
+  ProblemOptions p;
+  p.weights.scan = o.weight_scan;
+  p.weights.hallucination = o.weight_hallucination;
+  p.weights.eikonal = o.weight_eikonal;
+  p.weights.odometry = o.weight_odometry;
+  p.weights.relation = o.weight_relation;
+  p.hallucination.points_per_scan_point = o.hallucination_points_per_scan_point;
+  p.hallucination.step_size = o.hallucination_step_size;
+  p.hallucination.both_directions = o.hallucination_both_directions != 0;
+  p.normals.method = static_cast<NormalMethod>(o.normals_method);
+  p.normals.k_neighbors = o.normals_k_neighbors;
+  p.normals.per_scan = o.normals_per_scan != 0;
+  p.normals.corner_threshold = o.normals_corner_threshold;
+  p.huber_delta = o.huber_delta;
+  p.smooth_gradient = o.smooth_gradient != 0;
+  p.smooth_gradient_step = o.smooth_gradient_step;
+  p.active_region = o.active_region != 0;
+  p.active_margin = o.active_margin;
+  return p;
+}
+
+std::string Trim(const std::string& s) {
+  const size_t a = s.find_first_not_of(" \t\r");
+  if (a == std::string::npos) {
+    return "";
+  }
+  const size_t b = s.find_last_not_of(" \t\r");
+  return s.substr(a, b - a + 1);
+}
+
+bool ParseBool(const std::string& v) {
+  if (v == "true" || v == "True" || v == "1") {
+    return true;
+  }
+  if (v == "false" || v == "False" || v == "0") {
+    return false;
+  }
+  throw std::runtime_error("bad boolean '" + v + "'");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sdfslam_host_last_error(void) { return g_error.c_str(); }
+
+void sdfslam_host_default_problem_options(sdfslam_problem_options* out) {
+  const ProblemOptions d;
+  out->weight_scan = d.weights.scan;
+  out->weight_hallucination = d.weights.hallucination;
+  out->weight_eikonal = d.weights.eikonal;
+  out->weight_odometry = d.weights.odometry;
+  out->weight_relation = d.weights.relation;
+  out->hallucination_points_per_scan_point = d.hallucination.points_per_scan_point;
+  out->hallucination_step_size = d.hallucination.step_size;
+  out->hallucination_both_directions = d.hallucination.both_directions ? 1 : 0;
+  out->normals_method = static_cast<int32_t>(d.normals.method);
+  out->normals_k_neighbors = d.normals.k_neighbors;
+  out->normals_per_scan = d.normals.per_scan ? 1 : 0;
+  out->normals_corner_threshold = d.normals.corner_threshold;
+  out->huber_delta = d.huber_delta;
+  out->smooth_gradient = d.smooth_gradient ? 1 : 0;
+  out->smooth_gradient_step = d.smooth_gradient_step;
+  out->active_region = d.active_region ? 1 : 0;
+  out->active_margin = d.active_margin;
+}
+
+void sdfslam_host_default_solver_options(sdfslam_solver_options* out) {
+  *out = ToC(SolverOptions{});
+}
+
+int sdfslam_host_problem_create(const sdfslam_grid* grid, const double* map_values,
+                                int32_t num_frames, const double* poses,
+                                const int64_t* scan_offsets, const double* scan_xy,
+                                const double* odometry, int32_t num_relations,
+                                const int32_t* rel_frames, const double* rel_meas,
+                                const double* rel_sqrt_weight,
+                                const sdfslam_problem_options* options,
+                                sdfslam_host_problem** out) {
+  return Guard([&]() -> int {
+    if (grid == nullptr || poses == nullptr || scan_offsets == nullptr || options == nullptr ||
+        out == nullptr || num_frames < 1) {
+      throw std::invalid_argument("null argument or no frames");
+    }
+    // GridMap takes min/max corners; dx = (max - min) / (n - 1).
+    GridMap map(grid->nx, grid->ny, {grid->min_x, grid->min_y},
+                {grid->min_x + grid->dx * (grid->nx - 1), grid->min_y + grid->dy * (grid->ny - 1)});
+    if (map_values != nullptr) {
+      std::copy(map_values, map_values + map.values().size(), map.values().begin());
+    }
+    std::vector<Pose2> pose_vec(static_cast<size_t>(num_frames));
+    std::vector<Scan> scans(static_cast<size_t>(num_frames));
+    for (int32_t f = 0; f < num_frames; ++f) {
+      pose_vec[static_cast<size_t>(f)] = {poses[3 * f], poses[3 * f + 1], poses[3 * f + 2]};
+      for (int64_t k = scan_offsets[f]; k < scan_offsets[f + 1]; ++k) {
+        scans[static_cast<size_t>(f)].points.push_back({scan_xy[2 * k], scan_xy[2 * k + 1]});
+      }
+    }
+    std::vector<Pose2> odo;
+    if (odometry != nullptr) {
+      for (int32_t f = 0; f + 1 < num_frames; ++f) {
+        odo.push_back({odometry[3 * f], odometry[3 * f + 1], odometry[3 * f + 2]});
+      }
+    }
+    std::vector<RelationMeasurement> rels;
+    for (int32_t r = 0; r < num_relations; ++r) {
+      RelationMeasurement m;
+      m.frame_i = rel_frames[2 * r];
+      m.frame_j = rel_frames[2 * r + 1];
+      m.measurement = {rel_meas[3 * r], rel_meas[3 * r + 1], rel_meas[3 * r + 2]};
+      m.sqrt_weight = rel_sqrt_weight != nullptr ? rel_sqrt_weight[r] : 1.0;
+      rels.push_back(m);
+    }
+    auto holder = std::make_unique<sdfslam_host_problem>();
+    holder->problem = std::make_unique<Problem>(std::move(map), std::move(pose_vec), scans, odo,
+                                                FromC(*options), rels);
+    *out = holder.release();
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_host_workload_create(const char* name, uint64_t seed, sdfslam_host_problem** out) {
+  return Guard([&]() -> int {
+    if (name == nullptr || out == nullptr) {
+      throw std::invalid_argument("null argument");
+    }
+    auto holder = std::make_unique<sdfslam_host_problem>();
+    holder->workload = MakeWorkload(name, seed);
+    holder->has_workload = true;
+    const Workload& wl = holder->workload;
+    holder->problem = std::make_unique<Problem>(wl.map, wl.initial, wl.scans, wl.odometry,
+                                                wl.problem, wl.relations);
+    *out = holder.release();
+    return SDFSLAM_OK;
+  });
+}
+
+void sdfslam_host_problem_destroy(sdfslam_host_problem* problem) { delete problem; }
+
+int sdfslam_host_problem_view(const sdfslam_host_problem* problem, sdfslam_problem_view* out) {
+  if (problem == nullptr || out == nullptr) {
+    return Fail(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  *out = problem->problem->View();
+  return SDFSLAM_OK;
+}
+
+int sdfslam_host_problem_get_state(const sdfslam_host_problem* problem, double* map_values,
+                                   double* poses) {
+  if (problem == nullptr) {
+    return Fail(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  std::vector<double> m;
+  std::vector<double> p;
+  problem->problem->GetState(m, p);
+  if (map_values != nullptr) {
+    std::copy(m.begin(), m.end(), map_values);
+  }
+  if (poses != nullptr) {
+    std::copy(p.begin(), p.end(), poses);
+  }
+  return SDFSLAM_OK;
+}
+
+int sdfslam_host_problem_set_state(sdfslam_host_problem* problem, const double* map_values,
+                                   const double* poses) {
+  if (problem == nullptr) {
+    return Fail(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  problem->problem->SetState(map_values, poses);
+  return SDFSLAM_OK;
+}
+
+int sdfslam_host_workload_info(const sdfslam_host_problem* problem,
+                               sdfslam_solver_options* solver, int32_t* incremental,
+                               int32_t* iterations_per_increment, double* ground_truth) {
+  if (problem == nullptr || !problem->has_workload) {
+    return Fail(SDFSLAM_ERR_INVALID_ARGUMENT, "not a synthetic workload");
+  }
+  const Workload& wl = problem->workload;
+  if (solver != nullptr) {
+    *solver = ToC(wl.solver);
+  }
+  if (incremental != nullptr) {
+    *incremental = wl.incremental ? 1 : 0;
+  }
+  if (iterations_per_increment != nullptr) {
+    *iterations_per_increment = wl.iterations_per_increment;
+  }
+  if (ground_truth != nullptr) {
+    for (size_t i = 0; i < wl.ground_truth.size(); ++i) {
+      ground_truth[3 * i] = wl.ground_truth[i].x;
+      ground_truth[3 * i + 1] = wl.ground_truth[i].y;
+      ground_truth[3 * i + 2] = wl.ground_truth[i].theta;
+    }
+  }
+  return SDFSLAM_OK;
+}
+
+int sdfslam_host_solve(sdfslam_host_problem* problem, const char* backend,
+                       const sdfslam_solver_options* options, sdfslam_solve_result* result) {
+  return Guard([&]() -> int {
+    if (problem == nullptr || options == nullptr || result == nullptr || backend == nullptr) {
+      throw std::invalid_argument("null argument");
+    }
+    SolverOptions o;
+    const std::string b = backend;
+    if (b == "hip") {
+      o.backend = SolverBackend::kHip;
+    } else if (b == "lm") {
+      o.backend = SolverBackend::kLm;
+    } else if (b == "ceres") {
+      o.backend = SolverBackend::kCeres;
+    } else {
+      throw std::runtime_error("unknown solver.backend '" + b + "'");
+    }
+    o.max_iterations = options->max_iterations;
+    o.step_tolerance = options->step_tolerance;
+    o.lambda_init = options->lambda_init;
+    o.lambda_factor = options->lambda_factor;
+    o.reject_worse_steps = options->reject_worse_steps != 0;
+    o.marquardt_scaling = options->marquardt_scaling != 0;
+    o.trust_region = options->trust_region != 0;
+    o.num_threads = options->num_threads;
+    o.linear_solver = static_cast<LinearSolver>(options->linear_solver);
+    const SolveResult r = Solve(*problem->problem, o);
+    result->converged = r.converged ? 1 : 0;
+    result->iterations = r.iterations;
+    result->initial_cost = r.initial_cost;
+    result->final_cost = r.final_cost;
+    result->duration_seconds = r.duration_seconds;
+    result->cost_history_length = static_cast<int32_t>(r.cost_history.size());
+    if (result->cost_history != nullptr) {
+      const size_t n = std::min(r.cost_history.size(),
+                                static_cast<size_t>(std::max(0, result->cost_history_capacity)));
+      std::copy(r.cost_history.begin(), r.cost_history.begin() + static_cast<std::ptrdiff_t>(n),
+                result->cost_history);
+    }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_host_parse_solver_yaml(const char* path, sdfslam_solver_options* out, char* backend,
+                                   int32_t backend_capacity) {
+  return Guard([&]() -> int {
+    if (path == nullptr || out == nullptr || backend == nullptr || backend_capacity < 8) {
+      throw std::invalid_argument("null argument or backend buffer below 8 bytes");
+    }
+    std::ifstream file(path);
+    if (!file.is_open()) {
+      throw std::runtime_error(std::string("cannot open config: ") + path);
+    }
+    SolverOptions o;
+    std::string backend_name = "lm";
+    // config.cpp:171-175 plus this backend's selection value.
+    const std::set<std::string> known = {"backend",          "max_iterations",     "step_tolerance",
+                                         "lambda_init",      "lambda_factor",      "reject_worse_steps",
+                                         "marquardt_scaling", "trust_region",      "num_threads",
+                                         "linear_solver"};
+    std::string line;
+    bool in_solver = false;
+    while (std::getline(file, line)) {
+      const size_t hash = line.find('#');
+      if (hash != std::string::npos) {
+        line = line.substr(0, hash);
+      }
+      if (Trim(line).empty()) {
+        continue;
+      }
+      const bool indented = line[0] == ' ' || line[0] == '\t';
+      const std::string body = Trim(line);
+      const size_t colon = body.find(':');
+      if (colon == std::string::npos) {
+        throw std::runtime_error("malformed config line: " + body);
+      }
+      const std::string key = Trim(body.substr(0, colon));
+      const std::string value = Trim(body.substr(colon + 1));
+      if (!indented) {
+        in_solver = key == "solver" && value.empty();
+        continue;
+      }
+      if (!in_solver) {
+        continue;
+      }
+      if (known.count(key) == 0) {
+        throw std::runtime_error("unknown config key 'solver." + key + "'");
+      }
+      if (key == "backend") {
+        if (value != "lm" && value != "ceres" && value != "hip") {
+          throw std::runtime_error("unknown solver.backend '" + value + "'");
+        }
+        backend_name = value;
+      } else if (key == "linear_solver") {
+        if (value == "eigen") {
+          o.linear_solver = LinearSolver::kEigen;
+        } else if (value == "cholmod") {
+          o.linear_solver = LinearSolver::kCholmod;
+        } else {
+          throw std::runtime_error("unknown solver.linear_solver '" + value + "'");
+        }
+      } else if (key == "max_iterations") {
+        o.max_iterations = std::stoi(value);
+      } else if (key == "step_tolerance") {
+        o.step_tolerance = std::stod(value);
+      } else if (key == "lambda_init") {
+        o.lambda_init = std::stod(value);
+      } else if (key == "lambda_factor") {
+        o.lambda_factor = std::stod(value);
+      } else if (key == "reject_worse_steps") {
+        o.reject_worse_steps = ParseBool(value);
+      } else if (key == "marquardt_scaling") {
+        o.marquardt_scaling = ParseBool(value);
+      } else if (key == "trust_region") {
+        o.trust_region = ParseBool(value);
+      } else if (key == "num_threads") {
+        o.num_threads = std::stoi(value);
+      }
+    }
+    *out = ToC(o);
+    std::strncpy(backend, backend_name.c_str(), static_cast<size_t>(backend_capacity) - 1);
+    backend[backend_capacity - 1] = '\0';
+    return SDFSLAM_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,439 @@
+"""Python bindings (ctypes) for the MI355X joint-LM backend and its host library.
+
+Mirrors the C ABIs in include/sdfslam_hip.h (libsdfslam_hip.so) and
+include/sdfslam_host.h (libsdfslam_host.so). The libraries are built in-tree
+into sdf-slam_amd/lib by `make -C sdf-slam_amd`; importing this module fails
+loudly when they are missing. There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+
+SDFSLAM_OK = 0
+SDFSLAM_ERR_INVALID_ARGUMENT = 1
+SDFSLAM_ERR_NOT_SPD = 2
+SDFSLAM_ERR_HIP = 3
+SDFSLAM_ERR_OOM = 4
+SDFSLAM_ERR_NO_DEVICE = 5
+SDFSLAM_ERR_INTERNAL = 6
+
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_f64p = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("min_x", C.c_double),
+                ("min_y", C.c_double), ("dx", C.c_double), ("dy", C.c_double)]
+
+
+class ProblemView(C.Structure):
+    _fields_ = [
+        ("grid", Grid), ("num_poses", C.c_int32), ("num_active_nodes", C.c_int32),
+        ("node_to_active", _i32p),
+        ("num_points", C.c_int64), ("point_frame", _i32p), ("point_x", _f64p),
+        ("point_y", _f64p), ("point_expected_sdf", _f64p), ("point_sqrt_weight", _f64p),
+        ("num_eikonal", C.c_int64), ("eik_w", _i32p), ("eik_h", _i32p), ("eik_nx", _f64p),
+        ("eik_ny", _f64p), ("eik_sqrt_weight", _f64p),
+        ("num_odom", C.c_int64), ("odom_i", _i32p), ("odom_j", _i32p), ("odom_dx", _f64p),
+        ("odom_dy", _f64p), ("odom_dtheta", _f64p), ("odom_sqrt_weight", _f64p),
+        ("huber_delta", C.c_double), ("smooth_gradient", C.c_int32),
+        ("smooth_gradient_step", C.c_double),
+    ]
+
+
+class SolverOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("step_tolerance", C.c_double),
+                ("lambda_init", C.c_double), ("lambda_factor", C.c_double),
+                ("reject_worse_steps", C.c_int32), ("marquardt_scaling", C.c_int32),
+                ("trust_region", C.c_int32), ("num_threads", C.c_int32),
+                ("linear_solver", C.c_int32)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32),
+                ("initial_cost", C.c_double), ("final_cost", C.c_double),
+                ("duration_seconds", C.c_double), ("cost_history", _f64p),
+                ("cost_history_capacity", C.c_int32), ("cost_history_length", C.c_int32)]
+
+
+class AnalysisInfo(C.Structure):
+    _fields_ = [("num_parameters", C.c_int64), ("num_residuals", C.c_int64),
+                ("pattern_nnz_lower", C.c_int64), ("num_supernodes", C.c_int64),
+                ("factor_nnz", C.c_int64), ("factor_flops", C.c_double),
+                ("update_doubles", C.c_int64), ("tree_levels", C.c_int32),
+                ("pattern_epochs", C.c_int32)]
+
+
+class PhaseTimes(C.Structure):
+    _fields_ = [("assembly_ms", C.c_double), ("normal_ms", C.c_double),
+                ("factor_ms", C.c_double), ("solve_ms", C.c_double),
+                ("update_ms", C.c_double), ("total_ms", C.c_double),
+                ("point_assembly_kernel_ms", C.c_double)]
+
+
+class ProblemOptions(C.Structure):
+    _fields_ = [
+        ("weight_scan", C.c_double), ("weight_hallucination", C.c_double),
+        ("weight_eikonal", C.c_double), ("weight_odometry", C.c_double),
+        ("weight_relation", C.c_double),
+        ("hallucination_points_per_scan_point", C.c_int32),
+        ("hallucination_step_size", C.c_double), ("hallucination_both_directions", C.c_int32),
+        ("normals_method", C.c_int32), ("normals_k_neighbors", C.c_int32),
+        ("normals_per_scan", C.c_int32), ("normals_corner_threshold", C.c_double),
+        ("huber_delta", C.c_double), ("smooth_gradient", C.c_int32),
+        ("smooth_gradient_step", C.c_double), ("active_region", C.c_int32),
+        ("active_margin", C.c_int32),
+    ]
+
+
+# Exported symbols of each library, as include/*.h declares them.
+HIP_SYMBOLS = [
+    "sdfslam_hip_status_string", "sdfslam_hip_last_error", "sdfslam_hip_solve",
+    "sdfslam_hip_context_create", "sdfslam_hip_context_destroy", "sdfslam_hip_set_state",
+    "sdfslam_hip_get_state", "sdfslam_hip_analysis", "sdfslam_hip_assemble",
+    "sdfslam_hip_normal_equations", "sdfslam_hip_linear_step", "sdfslam_hip_cost",
+    "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
+]
+HOST_SYMBOLS = [
+    "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
+    "sdfslam_host_last_error", "sdfslam_host_problem_create", "sdfslam_host_workload_create",
+    "sdfslam_host_problem_destroy", "sdfslam_host_problem_view", "sdfslam_host_problem_get_state",
+    "sdfslam_host_problem_set_state", "sdfslam_host_workload_info", "sdfslam_host_solve",
+    "sdfslam_host_parse_solver_yaml",
+]
+
+
+def _load(name):
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built; run `make -C sdf-slam_amd` (hipcc, gfx950)")
+    return C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+hip = _load("libsdfslam_hip.so")
+host = _load("libsdfslam_host.so")
+
+hip.sdfslam_hip_status_string.restype = C.c_char_p
+hip.sdfslam_hip_status_string.argtypes = [C.c_int]
+hip.sdfslam_hip_last_error.restype = C.c_char_p
+hip.sdfslam_hip_last_error.argtypes = []
+hip.sdfslam_hip_solve.argtypes = [C.POINTER(ProblemView), C.POINTER(SolverOptions), _f64p, _f64p,
+                                  C.POINTER(SolveResult)]
+hip.sdfslam_hip_context_create.argtypes = [C.POINTER(ProblemView), C.POINTER(SolverOptions),
+                                           _f64p, _f64p, C.POINTER(C.c_void_p)]
+hip.sdfslam_hip_context_destroy.argtypes = [C.c_void_p]
+hip.sdfslam_hip_set_state.argtypes = [C.c_void_p, _f64p, _f64p]
+hip.sdfslam_hip_get_state.argtypes = [C.c_void_p, _f64p, _f64p]
+hip.sdfslam_hip_analysis.argtypes = [C.c_void_p, C.POINTER(AnalysisInfo)]
+hip.sdfslam_hip_assemble.argtypes = [C.c_void_p, _f64p, _i32p, _f64p]
+hip.sdfslam_hip_normal_equations.argtypes = [C.c_void_p, C.c_double, _i32p, _i32p, _f64p, _u8p,
+                                             _f64p]
+hip.sdfslam_hip_linear_step.argtypes = [C.c_void_p, C.c_double, _f64p]
+hip.sdfslam_hip_cost.argtypes = [C.c_void_p, _f64p]
+hip.sdfslam_hip_context_solve.argtypes = [C.c_void_p, C.POINTER(SolverOptions),
+                                          C.POINTER(SolveResult)]
+hip.sdfslam_hip_context_iterate.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+hip.sdfslam_hip_phase_times_get.argtypes = [C.c_void_p, C.POINTER(PhaseTimes)]
+
+host.sdfslam_host_last_error.restype = C.c_char_p
+host.sdfslam_host_default_problem_options.argtypes = [C.POINTER(ProblemOptions)]
+host.sdfslam_host_default_solver_options.argtypes = [C.POINTER(SolverOptions)]
+host.sdfslam_host_problem_create.argtypes = [
+    C.POINTER(Grid), _f64p, C.c_int32, _f64p, _i64p, _f64p, _f64p, C.c_int32, _i32p, _f64p,
+    _f64p, C.POINTER(ProblemOptions), C.POINTER(C.c_void_p)]
+host.sdfslam_host_workload_create.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(C.c_void_p)]
+host.sdfslam_host_problem_destroy.argtypes = [C.c_void_p]
+host.sdfslam_host_problem_view.argtypes = [C.c_void_p, C.POINTER(ProblemView)]
+host.sdfslam_host_problem_get_state.argtypes = [C.c_void_p, _f64p, _f64p]
+host.sdfslam_host_problem_set_state.argtypes = [C.c_void_p, _f64p, _f64p]
+host.sdfslam_host_workload_info.argtypes = [C.c_void_p, C.POINTER(SolverOptions), _i32p, _i32p,
+                                            _f64p]
+host.sdfslam_host_solve.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(SolverOptions),
+                                    C.POINTER(SolveResult)]
+host.sdfslam_host_parse_solver_yaml.argtypes = [C.c_char_p, C.POINTER(SolverOptions), C.c_char_p,
+                                                C.c_int32]
+
+
+class SdfSlamError(RuntimeError):
+    """A non-zero status from either library; `.status` holds the code."""
+
+    def __init__(self, status, message):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+class FactorizationError(SdfSlamError):
+    """Zero pivot: the reference's "sparse Cholesky factorization failed"."""
+
+
+def _check_hip(status):
+    if status != SDFSLAM_OK:
+        msg = hip.sdfslam_hip_last_error().decode()
+        if status == SDFSLAM_ERR_NOT_SPD:
+            raise FactorizationError(status, msg or "sparse Cholesky factorization failed")
+        if status == SDFSLAM_ERR_INVALID_ARGUMENT:
+            raise ValueError(msg)
+        raise SdfSlamError(status, msg)
+
+
+def _check_host(status):
+    if status != SDFSLAM_OK:
+        msg = host.sdfslam_host_last_error().decode()
+        if status == SDFSLAM_ERR_NOT_SPD:
+            raise FactorizationError(status, msg)
+        if status == SDFSLAM_ERR_INVALID_ARGUMENT:
+            raise ValueError(msg)
+        raise SdfSlamError(status, msg)
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def default_solver_options(**kw):
+    o = SolverOptions()
+    host.sdfslam_host_default_solver_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def default_problem_options(**kw):
+    o = ProblemOptions()
+    host.sdfslam_host_default_problem_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def parse_solver_yaml(path):
+    o = SolverOptions()
+    buf = C.create_string_buffer(16)
+    _check_host(host.sdfslam_host_parse_solver_yaml(path.encode(), C.byref(o), buf, 16))
+    return o, buf.value.decode()
+
+
+class Problem:
+    """A Problem built by the host library (Problem::Problem, problem.cpp:11-115)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+        self.view = ProblemView()
+        _check_host(host.sdfslam_host_problem_view(self._h, C.byref(self.view)))
+
+    @classmethod
+    def workload(cls, name, seed=1):
+        h = C.c_void_p()
+        _check_host(host.sdfslam_host_workload_create(name.encode(), seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def build(cls, grid, poses, scans, odometry=None, relations=(), options=None, map_values=None):
+        """grid: (nx, ny, min_x, min_y, dx, dy); poses: (N, 3); scans: list of (k, 2) arrays;
+        relations: iterable of (i, j, (dx, dy, dtheta), sqrt_weight)."""
+        g = Grid(*grid)
+        poses = np.ascontiguousarray(poses, dtype=np.float64)
+        offsets = np.zeros(len(scans) + 1, dtype=np.int64)
+        offsets[1:] = np.cumsum([len(s) for s in scans])
+        xy = (np.ascontiguousarray(np.concatenate([np.asarray(s, float).reshape(-1, 2) for s in scans]))
+              if offsets[-1] > 0 else np.zeros((1, 2)))
+        odo = None if odometry is None else np.ascontiguousarray(odometry, dtype=np.float64)
+        rel = list(relations)
+        rf = np.array([[r[0], r[1]] for r in rel], dtype=np.int32).reshape(-1, 2)
+        rm = np.array([r[2] for r in rel], dtype=np.float64).reshape(-1, 3)
+        rw = np.array([r[3] for r in rel], dtype=np.float64)
+        opts = options if options is not None else default_problem_options()
+        mv = None if map_values is None else np.ascontiguousarray(map_values, dtype=np.float64)
+        h = C.c_void_p()
+        _check_host(host.sdfslam_host_problem_create(
+            C.byref(g), ptr(mv, _f64p), len(poses), ptr(poses, _f64p), ptr(offsets, _i64p),
+            ptr(xy, _f64p), ptr(odo, _f64p), len(rel), ptr(rf, _i32p), ptr(rm, _f64p),
+            ptr(rw, _f64p), C.byref(opts), C.byref(h)))
+        return cls(h.value)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            host.sdfslam_host_problem_destroy(h)
+            self._h = None
+
+    @property
+    def num_parameters(self):
+        return self.view.num_active_nodes + 3 * (self.view.num_poses - 1)
+
+    @property
+    def num_residuals(self):
+        v = self.view
+        return v.num_points + v.num_eikonal + 3 * v.num_odom
+
+    def state(self):
+        g = self.view.grid
+        m = np.zeros(g.nx * g.ny)
+        p = np.zeros(3 * self.view.num_poses)
+        _check_host(host.sdfslam_host_problem_get_state(self._h, ptr(m, _f64p), ptr(p, _f64p)))
+        return m, p
+
+    def set_state(self, map_values=None, poses=None):
+        mv = None if map_values is None else np.ascontiguousarray(map_values, dtype=np.float64)
+        pv = None if poses is None else np.ascontiguousarray(poses, dtype=np.float64)
+        _check_host(host.sdfslam_host_problem_set_state(self._h, ptr(mv, _f64p), ptr(pv, _f64p)))
+
+    def workload_info(self):
+        o = SolverOptions()
+        inc = C.c_int32()
+        ipi = C.c_int32()
+        gt = np.zeros(3 * self.view.num_poses)
+        _check_host(host.sdfslam_host_workload_info(self._h, C.byref(o), C.byref(inc),
+                                                    C.byref(ipi), ptr(gt, _f64p)))
+        return o, bool(inc.value), ipi.value, gt.reshape(-1, 3)
+
+    def solve(self, options, backend="hip"):
+        """Solve(problem, options) through the host library's dispatch."""
+        hist = np.zeros(max(options.max_iterations, 0) + 1)
+        r = SolveResult()
+        r.cost_history = ptr(hist, _f64p)
+        r.cost_history_capacity = len(hist)
+        _check_host(host.sdfslam_host_solve(self._h, backend.encode(), C.byref(options), C.byref(r)))
+        return _result_dict(r, hist)
+
+    def arrays(self):
+        """Numpy copies of the spec arrays behind the view."""
+        v = self.view
+        def a(p, n, dt):
+            return np.ctypeslib.as_array(p, shape=(n,)).astype(dt).copy() if n > 0 else np.zeros(0, dt)
+        g = v.grid
+        return dict(
+            node_to_active=a(v.node_to_active, g.nx * g.ny, np.int32),
+            point_frame=a(v.point_frame, v.num_points, np.int32),
+            point_x=a(v.point_x, v.num_points, np.float64),
+            point_y=a(v.point_y, v.num_points, np.float64),
+            point_expected_sdf=a(v.point_expected_sdf, v.num_points, np.float64),
+            point_sqrt_weight=a(v.point_sqrt_weight, v.num_points, np.float64),
+            eik_w=a(v.eik_w, v.num_eikonal, np.int32), eik_h=a(v.eik_h, v.num_eikonal, np.int32),
+            eik_nx=a(v.eik_nx, v.num_eikonal, np.float64), eik_ny=a(v.eik_ny, v.num_eikonal, np.float64),
+            eik_sqrt_weight=a(v.eik_sqrt_weight, v.num_eikonal, np.float64),
+            odom_i=a(v.odom_i, v.num_odom, np.int32), odom_j=a(v.odom_j, v.num_odom, np.int32),
+            odom_dx=a(v.odom_dx, v.num_odom, np.float64), odom_dy=a(v.odom_dy, v.num_odom, np.float64),
+            odom_dtheta=a(v.odom_dtheta, v.num_odom, np.float64),
+            odom_sqrt_weight=a(v.odom_sqrt_weight, v.num_odom, np.float64),
+        )
+
+
+def _result_dict(r, hist):
+    n = min(r.cost_history_length, len(hist))
+    return dict(converged=bool(r.converged), iterations=r.iterations, initial_cost=r.initial_cost,
+                final_cost=r.final_cost, duration_seconds=r.duration_seconds,
+                cost_history=hist[:n].copy())
+
+
+def solve(problem: Problem, options, map_values, poses):
+    """sdfslam_hip_solve on a view and explicit state; returns (result, map, poses)."""
+    m = np.ascontiguousarray(map_values, dtype=np.float64).copy()
+    p = np.ascontiguousarray(poses, dtype=np.float64).copy()
+    hist = np.zeros(max(options.max_iterations, 0) + 1)
+    r = SolveResult()
+    r.cost_history = ptr(hist, _f64p)
+    r.cost_history_capacity = len(hist)
+    _check_hip(hip.sdfslam_hip_solve(C.byref(problem.view), C.byref(options), ptr(m, _f64p),
+                                     ptr(p, _f64p), C.byref(r)))
+    return _result_dict(r, hist), m, p
+
+
+class Context:
+    """A device-resident context (sdfslam_hip_context_*)."""
+
+    def __init__(self, problem: Problem, options, map_values=None, poses=None):
+        self.problem = problem
+        m0, p0 = problem.state()
+        m = np.ascontiguousarray(m0 if map_values is None else map_values, dtype=np.float64)
+        p = np.ascontiguousarray(p0 if poses is None else poses, dtype=np.float64)
+        self.options = options
+        h = C.c_void_p()
+        _check_hip(hip.sdfslam_hip_context_create(C.byref(problem.view), C.byref(options),
+                                                  ptr(m, _f64p), ptr(p, _f64p), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            hip.sdfslam_hip_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def n(self):
+        return self.problem.num_parameters
+
+    def analysis(self):
+        info = AnalysisInfo()
+        _check_hip(hip.sdfslam_hip_analysis(self._h, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in AnalysisInfo._fields_}
+
+    def set_state(self, map_values=None, poses=None):
+        mv = None if map_values is None else np.ascontiguousarray(map_values, dtype=np.float64)
+        pv = None if poses is None else np.ascontiguousarray(poses, dtype=np.float64)
+        _check_hip(hip.sdfslam_hip_set_state(self._h, ptr(mv, _f64p), ptr(pv, _f64p)))
+
+    def state(self):
+        g = self.problem.view.grid
+        m = np.zeros(g.nx * g.ny)
+        p = np.zeros(3 * self.problem.view.num_poses)
+        _check_hip(hip.sdfslam_hip_get_state(self._h, ptr(m, _f64p), ptr(p, _f64p)))
+        return m, p
+
+    def assemble(self):
+        rows = self.problem.num_residuals
+        r = np.zeros(rows)
+        cols = np.zeros((rows, 7), dtype=np.int32)
+        vals = np.zeros((rows, 7))
+        _check_hip(hip.sdfslam_hip_assemble(self._h, ptr(r, _f64p), ptr(cols, _i32p), ptr(vals, _f64p)))
+        return r, cols, vals
+
+    def normal_equations(self, lam):
+        info = self.analysis()
+        nnz = info["pattern_nnz_lower"]
+        n = self.n
+        cp = np.zeros(n + 1, dtype=np.int32)
+        ri = np.zeros(nnz, dtype=np.int32)
+        va = np.zeros(nnz)
+        st = np.zeros(nnz, dtype=np.uint8)
+        rhs = np.zeros(n)
+        _check_hip(hip.sdfslam_hip_normal_equations(self._h, lam, ptr(cp, _i32p), ptr(ri, _i32p),
+                                                    ptr(va, _f64p), ptr(st, _u8p), ptr(rhs, _f64p)))
+        return cp, ri, va, st.astype(bool), rhs
+
+    def linear_step(self, lam):
+        step = np.zeros(self.n)
+        _check_hip(hip.sdfslam_hip_linear_step(self._h, lam, ptr(step, _f64p)))
+        return step
+
+    def cost(self):
+        c = C.c_double()
+        _check_hip(hip.sdfslam_hip_cost(self._h, C.byref(c)))
+        return c.value
+
+    def solve(self, options=None):
+        o = options if options is not None else self.options
+        hist = np.zeros(max(o.max_iterations, 0) + 1)
+        r = SolveResult()
+        r.cost_history = ptr(hist, _f64p)
+        r.cost_history_capacity = len(hist)
+        _check_hip(hip.sdfslam_hip_context_solve(self._h, C.byref(o), C.byref(r)))
+        return _result_dict(r, hist)
+
+    def iterate(self, k, phase_times=False):
+        _check_hip(hip.sdfslam_hip_context_iterate(self._h, k, 1 if phase_times else 0))
+
+    def phase_times(self):
+        t = PhaseTimes()
+        _check_hip(hip.sdfslam_hip_phase_times_get(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in PhaseTimes._fields_}

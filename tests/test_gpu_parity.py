@@ -1,0 +1,215 @@
+"""Device path vs the CPU oracle on identical problems and states.
+
+Tolerances (stated by the hot path's contract):
+  residuals / Jacobian values     1e-12 relative to the largest entry
+  J^T J entries                   1e-10 of sqrt(A_ii A_jj) (the entry's natural scale)
+  -J^T r                          1e-10 of sqrt(A_ii) * ||r||
+  LM step                         1e-10 relative (2-norm)
+  trajectories after a full solve 1e-6 m / 1e-7 rad, map 1e-6
+The J^T J structural pattern is compared exactly (integer mask).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+WORKLOADS = ["corner", "room_small", "office_small"]
+
+
+def _state_after(oracle, sdfslam, problem, iterations):
+    """A non-trivial state: the oracle's LM run for a few iterations."""
+    m, p = problem.state()
+    if iterations == 0:
+        return m, p
+    o, _, _, _ = problem.workload_info()
+    o.max_iterations = iterations
+    o.step_tolerance = 0.0
+    st, _, m2, p2 = oracle.solve(sdfslam, problem.view, o, m, p)
+    assert st == 0
+    return m2, p2
+
+
+def _rows_to_csr(cols, vals, n):
+    rows = np.repeat(np.arange(cols.shape[0]), 7).reshape(cols.shape)
+    mask = cols >= 0
+    return sp.csr_matrix((vals[mask], (rows[mask], cols[mask])), shape=(cols.shape[0], n))
+
+
+@pytest.fixture(scope="module")
+def problems(sdfslam):
+    return {name: sdfslam.Problem.workload(name, 3) for name in WORKLOADS}
+
+
+@pytest.mark.parametrize("name", WORKLOADS)
+@pytest.mark.parametrize("iters", [0, 2])
+def test_assembly_matches_oracle(sdfslam, oracle, problems, name, iters):
+    print(f"case assembly {name} iters={iters}", flush=True)
+    prob = problems[name]
+    m, p = _state_after(oracle, sdfslam, prob, iters)
+    ctx = sdfslam.Context(prob, sdfslam.default_solver_options(), m, p)
+    r_dev, cols, vals = ctx.assemble()
+    r_or, jac = oracle.assemble(prob.view, m, p)
+    j_or = jac.to_scipy().tocsr()
+    scale_r = max(1.0, np.abs(r_or).max())
+    assert np.abs(r_dev - r_or).max() <= 1e-12 * scale_r
+    # Same structural pattern (explicit zeros included), same values.
+    j_dev = _rows_to_csr(cols, vals, prob.num_parameters)
+    assert (cols >= 0).sum() == j_or.nnz
+    d = j_or.tocoo()
+    key_or = np.lexsort((d.col, d.row))
+    dd = j_dev.tocoo()
+    key_dev = np.lexsort((dd.col, dd.row))
+    assert np.array_equal(d.row[key_or], dd.row[key_dev])
+    assert np.array_equal(d.col[key_or], dd.col[key_dev])
+    scale_j = max(1.0, np.abs(d.data).max())
+    assert np.abs(d.data[key_or] - dd.data[key_dev]).max() <= 1e-12 * scale_j
+
+
+@pytest.mark.parametrize("name", WORKLOADS)
+@pytest.mark.parametrize("marquardt", [False, True])
+def test_normal_equations_match_oracle(sdfslam, oracle, problems, name, marquardt):
+    print(f"case normal {name} marquardt={marquardt}", flush=True)
+    prob = problems[name]
+    m, p = _state_after(oracle, sdfslam, prob, 2)
+    opts = sdfslam.default_solver_options(marquardt_scaling=int(marquardt))
+    ctx = sdfslam.Context(prob, opts, m, p)
+    lam = 0.7
+    cp, ri, va, structural, rhs = ctx.normal_equations(lam)
+    n = prob.num_parameters
+    r_or, jac = oracle.assemble(prob.view, m, p)
+    a_or = oracle.normal_matrix(jac, lam, marquardt).to_scipy()
+    rhs_or = oracle.neg_atb(jac, r_or, n)
+    lower_or = sp.tril(a_or).tocoo()
+    dev_cols = np.repeat(np.arange(n), np.diff(cp))
+    # Structural mask: the reference pattern (explicit zeros and the inserted
+    # diagonal included) is exactly the set of device slots flagged structural.
+    key_dev = np.sort(ri[structural].astype(np.int64) * n + dev_cols[structural])
+    key_or = np.sort(lower_or.row.astype(np.int64) * n + lower_or.col)
+    assert np.array_equal(key_dev, key_or)
+    # Slots outside the reference pattern hold exact zeros.
+    assert np.all(va[~structural] == 0.0)
+    # Values over the reference pattern.
+    order_dev = np.argsort(ri[structural].astype(np.int64) * n + dev_cols[structural])
+    order_or = np.argsort(lower_or.row.astype(np.int64) * n + lower_or.col)
+    v_dev = va[structural][order_dev]
+    v_or = lower_or.data[order_or]
+    rr = lower_or.row[order_or]
+    cc = lower_or.col[order_or]
+    diag = a_or.diagonal()
+    bound = 1e-10 * np.sqrt(diag[rr] * diag[cc])
+    assert np.all(np.abs(v_dev - v_or) <= bound)
+    rnorm = np.linalg.norm(r_or)
+    assert np.all(np.abs(rhs - rhs_or) <= 1e-10 * np.sqrt(diag) * rnorm)
+
+
+@pytest.mark.parametrize("name", WORKLOADS)
+def test_linear_step_matches_oracle(sdfslam, oracle, problems, name):
+    print(f"case step {name}", flush=True)
+    prob = problems[name]
+    m, p = _state_after(oracle, sdfslam, prob, 2)
+    ctx = sdfslam.Context(prob, sdfslam.default_solver_options(), m, p)
+    for lam in (1.0, 0.05):
+        x_dev = ctx.linear_step(lam)
+        st, x_or = oracle.linear_step(prob.view, m, p, lam)
+        assert st == 0
+        assert np.linalg.norm(x_dev - x_or) <= 1e-10 * np.linalg.norm(x_or)
+        # Bitwise reproducible run to run.
+        assert np.array_equal(ctx.linear_step(lam), x_dev)
+
+
+@pytest.mark.parametrize("name", WORKLOADS)
+def test_cost_matches_oracle(sdfslam, oracle, problems, name):
+    prob = problems[name]
+    m, p = _state_after(oracle, sdfslam, prob, 2)
+    ctx = sdfslam.Context(prob, sdfslam.default_solver_options(), m, p)
+    c_or = oracle.cost(prob.view, m, p)
+    assert abs(ctx.cost() - c_or) <= 1e-12 * c_or
+
+
+@pytest.mark.parametrize("name,iters", [("corner", 15), ("room_small", 10), ("office_small", 8)])
+def test_batch_trajectory_matches_oracle(sdfslam, oracle, problems, name, iters):
+    print(f"case trajectory {name}", flush=True)
+    prob = problems[name]
+    m0, p0 = prob.state()
+    o, _, _, _ = prob.workload_info()
+    o.max_iterations = iters
+    o.step_tolerance = 0.0
+    st, res_or, m_or, p_or = oracle.solve(sdfslam, prob.view, o, m0, p0)
+    assert st == 0
+    res, m_dev, p_dev = sdfslam.solve(prob, o, m0, p0)
+    assert res["iterations"] == res_or["iterations"] == iters
+    dp = (p_dev - p_or).reshape(-1, 3)
+    assert np.abs(dp[:, :2]).max() <= 1e-6
+    assert np.abs(dp[:, 2]).max() <= 1e-7
+    assert np.abs(m_dev - m_or).max() <= 1e-6
+    np.testing.assert_allclose(res["cost_history"], res_or["cost_history"], rtol=1e-9)
+    # Same result on a second run, bit for bit.
+    res2, m2, p2 = sdfslam.solve(prob, o, m0, p0)
+    assert np.array_equal(m2, m_dev) and np.array_equal(p2, p_dev)
+    assert np.array_equal(res2["cost_history"], res["cost_history"])
+
+
+@pytest.mark.parametrize("variant", ["reject", "trust", "marquardt", "factor"])
+def test_lm_control_variants_match_oracle(sdfslam, oracle, problems, variant):
+    print(f"case control {variant}", flush=True)
+    prob = problems["corner"]
+    m0, p0 = prob.state()
+    o = sdfslam.default_solver_options(max_iterations=12, step_tolerance=0.0)
+    if variant == "reject":
+        o.reject_worse_steps = 1
+        o.lambda_factor = 3.0
+        o.lambda_init = 1e-3
+    elif variant == "trust":
+        o.trust_region = 1
+        o.lambda_init = 1e-2
+    elif variant == "marquardt":
+        o.marquardt_scaling = 1
+        o.lambda_factor = 2.0
+    else:
+        o.lambda_factor = 0.5
+    st, res_or, m_or, p_or = oracle.solve(sdfslam, prob.view, o, m0, p0)
+    assert st == 0
+    res, m_dev, p_dev = sdfslam.solve(prob, o, m0, p0)
+    assert res["iterations"] == res_or["iterations"]
+    assert res["converged"] == res_or["converged"]
+    np.testing.assert_allclose(res["cost_history"], res_or["cost_history"], rtol=1e-8)
+    assert np.abs(p_dev - p_or).max() <= 1e-6
+    assert np.abs(m_dev - m_or).max() <= 1e-6
+
+
+def test_convergence_stop_matches_oracle(sdfslam, oracle, problems):
+    prob = problems["corner"]
+    m0, p0 = prob.state()
+    o = sdfslam.default_solver_options(max_iterations=60, step_tolerance=1e-3)
+    st, res_or, _, _ = oracle.solve(sdfslam, prob.view, o, m0, p0)
+    res, _, _ = sdfslam.solve(prob, o, m0, p0)
+    assert res_or["converged"] and res["converged"]
+    assert res["iterations"] == res_or["iterations"]
+    assert len(res["cost_history"]) == res["iterations"] + 1
+
+
+def test_zero_pivot_raises(sdfslam, problems):
+    # lambda = 0 leaves the far grid corner (no point or eikonal row) with a
+    # zero diagonal: the reference's "sparse Cholesky factorization failed".
+    prob = problems["corner"]
+    ctx = sdfslam.Context(prob, sdfslam.default_solver_options())
+    with pytest.raises(sdfslam.FactorizationError):
+        ctx.linear_step(0.0)
+    # The context stays usable afterwards.
+    assert np.isfinite(ctx.linear_step(1.0)).all()
+
+
+def test_pattern_epoch_follows_large_pose_motion(sdfslam, oracle, problems):
+    # Frame 1 starts 0.4 m / 0.15 rad off: its points cross several cells,
+    # which ends the first pattern epoch; the result still matches the oracle.
+    prob = problems["corner"]
+    m0, p0 = prob.state()
+    o = sdfslam.default_solver_options(max_iterations=20, step_tolerance=0.0)
+    ctx = sdfslam.Context(prob, o, m0, p0)
+    res = ctx.solve(o)
+    m_dev, p_dev = ctx.state()
+    st, res_or, m_or, p_or = oracle.solve(sdfslam, prob.view, o, m0, p0)
+    assert ctx.analysis()["pattern_epochs"] >= 1
+    assert np.abs(p_dev - p_or).max() <= 1e-6
+    np.testing.assert_allclose(res["cost_history"], res_or["cost_history"], rtol=1e-8)

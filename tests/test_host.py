@@ -1,0 +1,139 @@
+"""Host library and C-ABI boundary checks that need no GPU."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sdfslam_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,lib", [("sdfslam_hip.h", "libsdfslam_hip.so"),
+                                        ("sdfslam_host.h", "libsdfslam_host.so")])
+def test_library_exports_every_declared_symbol(sdfslam, header, lib):
+    path = os.path.join(ROOT, "sdf-slam_amd", "lib", lib)
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [f for f in declared_functions(header) if f not in exported]
+    assert not missing, missing
+    listed = sdfslam.HIP_SYMBOLS if lib.startswith("libsdfslam_hip") else sdfslam.HOST_SYMBOLS
+    assert sorted(listed) == declared_functions(header)
+
+
+def test_hallucination_order_matches_reference(sdfslam):
+    # tests/test_hallucination.cpp:8-30: +-j*step alternating, free side first.
+    opts = sdfslam.default_problem_options(hallucination_points_per_scan_point=6,
+                                           hallucination_step_size=0.1)
+    prob = sdfslam.Problem.build((10, 10, -5.0, -5.0, 1.0, 1.0), [[0.0, 0.0, 0.0]],
+                                 [np.array([[4.0, 0.0]])], options=opts)
+    a = prob.arrays()
+    assert len(a["point_x"]) == 7
+    assert a["point_x"][0] == 4.0 and a["point_expected_sdf"][0] == 0.0
+    np.testing.assert_allclose(a["point_x"][1:], [3.9, 4.1, 3.8, 4.2, 3.7, 4.3], atol=1e-12)
+    np.testing.assert_allclose(a["point_expected_sdf"][1:], [0.1, -0.1, 0.2, -0.2, 0.3, -0.3],
+                               atol=1e-12)
+    assert np.all(np.abs(a["point_y"]) < 1e-12)
+    one = sdfslam.default_problem_options(hallucination_points_per_scan_point=3,
+                                          hallucination_step_size=0.5,
+                                          hallucination_both_directions=0)
+    prob = sdfslam.Problem.build((10, 10, -5.0, -5.0, 1.0, 1.0), [[0.0, 0.0, 0.0]],
+                                 [np.array([[0.0, 2.0], [0.0, 0.0]])], options=one)
+    a = prob.arrays()
+    # Zero-length beam: no hallucinated samples (test_hallucination.cpp:49-54).
+    assert len(a["point_x"]) == 2 + 3
+    assert np.all(a["point_expected_sdf"][2:] > 0) and np.all(a["point_y"][2:] < 2.0)
+
+
+def test_problem_layout_matches_reference(sdfslam):
+    prob = sdfslam.Problem.workload("room", 1)
+    v = prob.view
+    a = prob.arrays()
+    # Rows: per frame, scan points then hallucinated points (problem.cpp:26-37).
+    assert np.all(np.diff(a["point_frame"]) >= 0)
+    assert v.num_points % 5 == 0
+    assert v.num_active_nodes == 100 * 100
+    assert prob.num_parameters == 10000 + 3 * 75
+    assert v.num_eikonal <= 99 * 99
+    # Eikonal rows h-major, then w (problem.cpp:79-94).
+    key = a["eik_h"].astype(np.int64) * 100 + a["eik_w"]
+    assert np.all(np.diff(key) > 0)
+    assert v.num_odom == 75
+    np.testing.assert_array_equal(a["odom_j"] - a["odom_i"], np.ones(75))
+
+
+def test_relations_bookkeeping(sdfslam):
+    # tests/test_e2e.cpp:118-135.
+    base = sdfslam.Problem.workload("corner", 1)
+    a = base.arrays()
+    m, p = base.state()
+    g = base.view.grid
+    scans = []
+    for f in range(2):
+        sel = (a["point_frame"] == f) & (a["point_expected_sdf"] == 0.0)
+        scans.append(np.stack([a["point_x"][sel], a["point_y"][sel]], 1))
+    meas = (a["odom_dx"][0], a["odom_dy"][0], a["odom_dtheta"][0])
+    grid = (g.nx, g.ny, g.min_x, g.min_y, g.dx, g.dy)
+    odo = np.array([meas])
+    prob = sdfslam.Problem.build(grid, p.reshape(-1, 3), scans, odometry=odo,
+                                 relations=[(0, 5, meas, 1.0)])
+    assert prob.view.num_odom == 1
+    prob = sdfslam.Problem.build(grid, p.reshape(-1, 3), scans, relations=[(0, 1, meas, 2.0)])
+    assert prob.view.num_odom == 1
+    assert prob.arrays()["odom_sqrt_weight"][0] == 2.0
+    with pytest.raises(ValueError):
+        sdfslam.Problem.build(grid, p.reshape(-1, 3), scans, relations=[(1, 1, meas, 1.0)])
+
+
+def test_workloads_are_deterministic(sdfslam):
+    a = sdfslam.Problem.workload("office_small", 4).arrays()
+    b = sdfslam.Problem.workload("office_small", 4).arrays()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    c = sdfslam.Problem.workload("office_small", 5).arrays()
+    assert not np.array_equal(a["point_x"], c["point_x"])
+
+
+def test_solver_yaml_selection(sdfslam, tmp_path):
+    cfg = tmp_path / "run.yaml"
+    cfg.write_text("dataset_dir: data/x\nmode: batch\nsolver:\n  backend: hip\n"
+                   "  linear_solver: cholmod\n  lambda_init: 5.0\n  lambda_factor: 0.9\n"
+                   "  step_tolerance: 0.001  # comment\n  max_iterations: 100\n"
+                   "  trust_region: true\n")
+    o, backend = sdfslam.parse_solver_yaml(str(cfg))
+    assert backend == "hip"
+    assert o.lambda_init == 5.0 and o.lambda_factor == 0.9 and o.step_tolerance == 0.001
+    assert o.max_iterations == 100 and o.trust_region == 1 and o.linear_solver == 1
+    bad = tmp_path / "bad.yaml"
+    bad.write_text("solver:\n  backend: hip\n  lamda_init: 5.0\n")
+    with pytest.raises(Exception, match="unknown config key 'solver.lamda_init'"):
+        sdfslam.parse_solver_yaml(str(bad))
+    worse = tmp_path / "worse.yaml"
+    worse.write_text("solver:\n  backend: cuda\n")
+    with pytest.raises(Exception, match="unknown solver.backend 'cuda'"):
+        sdfslam.parse_solver_yaml(str(worse))
+
+
+def test_cpu_backends_are_not_in_this_library(sdfslam):
+    prob = sdfslam.Problem.workload("corner", 1)
+    with pytest.raises(ValueError, match="only 'hip'"):
+        prob.solve(sdfslam.default_solver_options(), backend="lm")
+
+
+def test_default_options_match_reference(sdfslam):
+    o = sdfslam.default_solver_options()
+    # solver.hpp:29-55
+    assert (o.max_iterations, o.step_tolerance, o.lambda_init, o.lambda_factor) == (100, 1e-6, 1.0, 1.0)
+    assert (o.reject_worse_steps, o.marquardt_scaling, o.trust_region) == (0, 0, 0)
+    p = sdfslam.default_problem_options()
+    # problem.hpp:14-87, hallucination.hpp:18-24, normals.hpp:44-57
+    assert p.hallucination_points_per_scan_point == 6 and p.hallucination_step_size == 0.1
+    assert p.normals_method == 2 and p.normals_k_neighbors == 10 and p.smooth_gradient == 1
+    assert p.active_margin == 2 and p.huber_delta == 0.0

@@ -1,0 +1,326 @@
+"""Pins the CPU oracle (test infrastructure) before it is trusted as the checker.
+
+Each check is independent of the oracle's own formulas: the reference's
+known answers (tests/golden/reference_known_answers.json), finite
+differences, closed forms on fields whose derivatives are known exactly,
+and dense linear algebra (numpy/scipy) on the same matrices.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "reference_known_answers.json")))
+
+
+def grid(sdfslam, nx, ny, mn, mx):
+    return sdfslam.Grid(nx, ny, mn[0], mn[1], (mx[0] - mn[0]) / (nx - 1), (mx[1] - mn[1]) / (ny - 1))
+
+
+# ---------------------------------------------------------------- pose
+
+
+def test_pose_known_answers(oracle):
+    g = GOLD["pose_apply"]
+    p = g["pose"]
+    c, s = math.cos(p[2]), math.sin(p[2])
+    out = oracle.compose(p, [g["point"][0], g["point"][1], 0.0])
+    assert abs(out[0] - g["expected"][0]) < g["tol"] and abs(out[1] - g["expected"][1]) < g["tol"]
+    assert abs(c * g["point"][0] - s * g["point"][1] + p[0] - out[0]) < 1e-15
+    r = GOLD["pose_roundtrip"]
+    rel = oracle.relative(r["a"], r["b"])
+    back = oracle.compose(r["a"], rel)
+    assert np.allclose(back[:2], r["b"][:2], atol=r["tol"])
+    assert abs(oracle.wrap(back[2] - r["b"][2])) < r["tol"]
+    for a, expected, kind in GOLD["wrap_angle"]["cases"]:
+        w = oracle.wrap(a)
+        if kind == "abs":
+            assert abs(abs(w) - expected) < 1e-12
+        else:
+            assert abs(w - expected) < 1e-12
+
+
+# ---------------------------------------------------------------- grid
+
+
+def map_3x3(sdfslam):
+    g = GOLD["grid_map_3x3"]
+    gr = grid(sdfslam, g["nx"], g["ny"], g["min"], g["max"])
+    vals = np.array([w + 10.0 * h for h in range(3) for w in range(3)])
+    return gr, vals
+
+
+def test_grid_known_answers(sdfslam, oracle):
+    gr, vals = map_3x3(sdfslam)
+    g = GOLD["grid_map_3x3"]
+    assert abs(oracle.interpolate(gr, vals, *g["query"]) - g["expected"]) < g["tol"]
+    ok, w, h, a, b = oracle.locate(gr, *GOLD["grid_line_cell"]["query"])
+    assert ok and (w, h) == tuple(GOLD["grid_line_cell"]["cell"]) and abs(a) < 1e-12 and abs(b) < 1e-12
+    ok, w, h, a, b = oracle.locate(gr, *GOLD["grid_upper_boundary"]["query"])
+    assert ok and (w, h) == (1, 1) and abs(a - 1) < 1e-12 and abs(b - 1) < 1e-12
+    for q in GOLD["grid_outside"]["queries"]:
+        assert not oracle.locate(gr, *q)[0]
+
+
+def test_grid_gradient_matches_fd(sdfslam, oracle):
+    gr, vals = map_3x3(sdfslam)
+    q = GOLD["grid_gradient_fd"]["query"]
+    eps = GOLD["grid_gradient_fd"]["eps"]
+    ok, _, _, _, dp = oracle.eval_point(gr, vals, [0.0, 0.0, 0.0], q[0], q[1], 0.0)
+    assert ok
+    fx = (oracle.interpolate(gr, vals, q[0] + eps, q[1]) - oracle.interpolate(gr, vals, q[0] - eps, q[1])) / (2 * eps)
+    fy = (oracle.interpolate(gr, vals, q[0], q[1] + eps) - oracle.interpolate(gr, vals, q[0], q[1] - eps)) / (2 * eps)
+    assert abs(dp[0] - fx) < 1e-5 and abs(dp[1] - fy) < 1e-5
+
+
+# ---------------------------------------------------------------- residual Jacobians
+
+
+def random_map(sdfslam, rng):
+    j = GOLD["jacobian_fd"]["map"]
+    gr = grid(sdfslam, j["nx"], j["ny"], j["min"], j["max"])
+    return gr, rng.uniform(*j["value_range"], size=j["nx"] * j["ny"])
+
+
+@pytest.mark.parametrize("seed", [42, 7, 43])
+def test_point_jacobian_matches_fd(sdfslam, oracle, seed):
+    eps = GOLD["jacobian_fd"]["eps"]
+    tol = GOLD["jacobian_fd"]["tol"]
+    rng = np.random.default_rng(seed)
+    checked = 0
+    for _ in range(20):
+        gr, vals = random_map(sdfslam, rng)
+        pose = np.array([rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), 3 * rng.uniform(-0.5, 0.5)])
+        px, py = rng.uniform(-1, 1, size=2)
+        delta = rng.uniform(-1, 1)
+        ok, r, ids, dn, dp = oracle.eval_point(gr, vals, pose, px, py, delta)
+        if not ok:
+            continue
+        for k in range(4):
+            v = vals.copy()
+            v[ids[k]] += eps
+            rp = oracle.eval_point(gr, v, pose, px, py, delta)[1]
+            v[ids[k]] -= 2 * eps
+            rm = oracle.eval_point(gr, v, pose, px, py, delta)[1]
+            assert abs(dn[k] - (rp - rm) / (2 * eps)) < tol
+        for k in range(3):
+            pp = pose.copy()
+            pp[k] += eps
+            pm = pose.copy()
+            pm[k] -= eps
+            ep = oracle.eval_point(gr, vals, pp, px, py, delta)
+            em = oracle.eval_point(gr, vals, pm, px, py, delta)
+            if not (ep[0] and em[0]) or not np.array_equal(ep[2], ids) or not np.array_equal(em[2], ids):
+                continue
+            assert abs(dp[k] - (ep[1] - em[1]) / (2 * eps)) < tol
+            checked += 1
+    assert checked > 10
+
+
+def test_smooth_gradients_exact_on_affine_field(sdfslam, oracle):
+    # Both smooth-gradient modes return the exact gradient of an affine field,
+    # including one-sided node differences at the border and the clamped
+    # extrapolation of the meter stencil (residual_math.hpp:40-73).
+    gr = grid(sdfslam, 7, 6, [-3.0, -2.0], [3.0, 3.0])
+    a, b, c = 0.7, -1.3, 0.25
+    xs = gr.min_x + gr.dx * np.tile(np.arange(7), 6)
+    ys = gr.min_y + gr.dy * np.repeat(np.arange(6), 7)
+    vals = a * xs + b * ys + c
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        th = rng.uniform(-3, 3)
+        px, py = rng.uniform(-2.8, 2.8), rng.uniform(-1.8, 2.8)
+        pose = [0.0, 0.0, th]
+        cth, sth = math.cos(th), math.sin(th)
+        gxp, gyp = cth * px - sth * py, sth * px + cth * py
+        if not (-3 <= gxp <= 3 and -2 <= gyp <= 3):
+            continue
+        for smooth, step in ((1, 0.0), (1, 0.4)):
+            ok, r, ids, dn, dp = oracle.eval_point(gr, vals, pose, px, py, 0.0, smooth, step)
+            assert ok
+            assert abs(dp[0] - a) < 1e-12 and abs(dp[1] - b) < 1e-12
+            dxdt = -sth * px - cth * py
+            dydt = cth * px - sth * py
+            assert abs(dp[2] - (a * dxdt + b * dydt)) < 1e-11
+
+
+def test_node_smooth_gradient_exact_on_quadratic_interior(sdfslam, oracle):
+    # Central node differences are exact for x^2 + y^2 and bilinear blending
+    # reproduces the linear node gradients, so interior cells give (2x, 2y).
+    gr = grid(sdfslam, 9, 9, [0.0, 0.0], [8.0, 8.0])
+    xs = np.tile(np.arange(9.0), 9)
+    ys = np.repeat(np.arange(9.0), 9)
+    vals = xs ** 2 + ys ** 2
+    rng = np.random.default_rng(2)
+    for _ in range(40):
+        x, y = rng.uniform(1.0, 7.0, size=2)
+        ok, _, _, _, dp = oracle.eval_point(gr, vals, [0.0, 0.0, 0.0], x, y, 0.0, 1, 0.0)
+        assert ok
+        assert abs(dp[0] - 2 * x) < 1e-11 and abs(dp[1] - 2 * y) < 1e-11
+
+
+def test_eikonal_and_odometry_jacobians_match_fd(sdfslam, oracle):
+    eps = GOLD["jacobian_fd"]["eps"]
+    rng = np.random.default_rng(43)
+    gr, vals = random_map(sdfslam, rng)
+    w, h = GOLD["eikonal_fd"]["node"]
+    nx, ny = GOLD["eikonal_fd"]["normal"]
+    r, ids, d = oracle.eval_eikonal(gr, vals, w, h, nx, ny)
+    for k in range(3):
+        v = vals.copy()
+        v[ids[k]] += eps
+        rp = oracle.eval_eikonal(gr, v, w, h, nx, ny)[0]
+        v[ids[k]] -= 2 * eps
+        rm = oracle.eval_eikonal(gr, v, w, h, nx, ny)[0]
+        assert abs(d[k] - (rp - rm) / (2 * eps)) < 1e-5
+    o = GOLD["odometry_fd"]
+    pi, pj, m = np.array(o["pose_i"]), np.array(o["pose_j"]), np.array(o["meas"])
+    r, ji, jj = oracle.eval_odom(pi, pj, m)
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = eps
+        fdi = (oracle.eval_odom(pi + e, pj, m)[0] - oracle.eval_odom(pi - e, pj, m)[0]) / (2 * eps)
+        fdj = (oracle.eval_odom(pi, pj + e, m)[0] - oracle.eval_odom(pi, pj - e, m)[0]) / (2 * eps)
+        assert np.abs(ji[:, k] - fdi).max() < 1e-5
+        assert np.abs(jj[:, k] - fdj).max() < 1e-5
+
+
+# ---------------------------------------------------------------- assembly and normal equations
+
+
+def exact_gradient_problem(sdfslam):
+    """The reference's e2e corner setup with the exact bilinear gradient."""
+    base = sdfslam.Problem.workload("corner", 1)
+    a = base.arrays()
+    opts = sdfslam.default_problem_options(hallucination_points_per_scan_point=4,
+                                           hallucination_step_size=0.2, normals_k_neighbors=8,
+                                           normals_method=0, smooth_gradient=0)
+    m, p = base.state()
+    poses = p.reshape(-1, 3)
+    frames = a["point_frame"]
+    scans = []
+    for f in range(len(poses)):
+        sel = (frames == f) & (a["point_expected_sdf"] == 0.0)
+        scans.append(np.stack([a["point_x"][sel], a["point_y"][sel]], axis=1))
+    g = base.view.grid
+    return sdfslam.Problem.build((g.nx, g.ny, g.min_x, g.min_y, g.dx, g.dy), poses, scans,
+                                 odometry=np.array([[a["odom_dx"][0], a["odom_dy"][0], a["odom_dtheta"][0]]]),
+                                 options=opts)
+
+
+def test_assembled_jacobian_is_the_derivative_of_the_residuals(sdfslam, oracle):
+    prob = exact_gradient_problem(sdfslam)
+    m, p = prob.state()
+    rng = np.random.default_rng(9)
+    m = m + rng.normal(0, 0.3, size=m.shape)  # a map with gradients
+    r0, jac = oracle.assemble(prob.view, m, p)
+    J = jac.to_scipy()
+    n = prob.num_parameters
+    na = prob.view.num_active_nodes
+    for trial in range(3):
+        d = rng.normal(0, 1, size=n) * 1e-7
+        m1, p1 = m.copy(), p.copy()
+        m1[: len(m1)] += d[:na]  # all nodes are active in this problem
+        p1[3:] += d[na:]
+        r1, _ = oracle.assemble(prob.view, m1, p1)
+        pred = J @ d
+        # Rows whose point stays in its cell are differentiable.
+        ok = np.abs(r1 - r0 - pred) <= 1e-6 * np.abs(pred).max() + 1e-12
+        assert ok.mean() > 0.98
+
+
+def test_normal_matrix_and_rhs_match_dense_algebra(sdfslam, oracle):
+    prob = sdfslam.Problem.workload("room_small", 2)
+    m, p = prob.state()
+    rng = np.random.default_rng(1)
+    m = m + rng.normal(0, 0.2, size=m.shape)
+    r, jac = oracle.assemble(prob.view, m, p)
+    J = jac.to_scipy().toarray()
+    n = prob.num_parameters
+    A = oracle.normal_matrix(jac).to_scipy().toarray()
+    dense = J.T @ J
+    scale = np.sqrt(np.outer(np.diag(dense), np.diag(dense))) + 1e-300
+    assert np.all(np.abs(A - dense) <= 1e-13 * scale)
+    rhs = oracle.neg_atb(jac, r, n)
+    assert np.allclose(rhs, -J.T @ r, rtol=1e-12, atol=1e-12 * np.abs(J.T @ r).max())
+    lam = 0.3
+    Ad = oracle.normal_matrix(jac, lam, False).to_scipy().toarray()
+    assert np.allclose(np.diag(Ad), np.diag(dense) + lam, rtol=1e-14, atol=1e-14)
+    Am = oracle.normal_matrix(jac, lam, True).to_scipy().toarray()
+    dd = np.diag(dense)
+    assert np.allclose(np.diag(Am), dd + lam * np.maximum(dd, 1e-12), rtol=1e-14, atol=1e-14)
+
+
+def random_spd(n, width, seed):
+    # The reference's RandomSpd recipe (tests/test_parallel_ldlt.cpp:32-54)
+    # with numpy draws.
+    rng = np.random.default_rng(seed)
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        rows.append(i); cols.append(i); vals.append(30.0 + abs(rng.uniform(-1, 1)))
+        if i + 1 < n:
+            v = rng.uniform(-1, 1)
+            rows += [i, i + 1]; cols += [i + 1, i]; vals += [v, v]
+        if i + width < n:
+            v = rng.uniform(-1, 1)
+            rows += [i, i + width]; cols += [i + width, i]; vals += [v, v]
+    return sp.csc_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+@pytest.mark.parametrize("seed", GOLD["random_spd"]["seeds"])
+def test_ldlt_solves_random_spd(oracle, seed):
+    n = 3000 + 500 * seed
+    A = random_spd(n, GOLD["random_spd"]["width"], seed)
+    b = np.random.default_rng(seed + 100).uniform(-1, 1, size=n)
+    mat = oracle.from_csc(A)
+    for order in (None, np.arange(n)[::-1].astype(np.int32).copy()):
+        st, x = oracle.ldlt_solve(mat, b, order)
+        assert st == 0
+        x_ref = spla.spsolve(A, b)
+        assert np.linalg.norm(x - x_ref) <= 1e-12 * np.linalg.norm(x_ref)
+        assert np.linalg.norm(A @ x - b) <= 1e-13 * np.linalg.norm(b)
+
+
+def test_ldlt_reports_singular_matrix(oracle):
+    g = GOLD["singular_diag"]
+    d = np.full(g["n"], g["diag"])
+    d[g["zero_at"]] = 0.0
+    A = sp.diags(d).tocsc()
+    st, _ = oracle.ldlt_solve(oracle.from_csc(A), np.ones(g["n"]))
+    assert st == 2
+
+
+# ---------------------------------------------------------------- LM control end to end
+
+
+def _recovered(sdfslam, oracle, prob, options, thresholds):
+    m0, p0 = prob.state()
+    st, res, m, p = oracle.solve(sdfslam, prob.view, options, m0, p0)
+    assert st == 0
+    _, _, _, gt = prob.workload_info()
+    pose1 = p.reshape(-1, 3)[1]
+    init = p0.reshape(-1, 3)[1]
+    assert res["final_cost"] < thresholds["cost_fraction"] * res["initial_cost"]
+    err = np.hypot(*(pose1[:2] - gt[1, :2]))
+    assert err < thresholds["translation"] and err < np.hypot(*(init[:2] - gt[1, :2]))
+    assert abs(oracle.wrap(pose1[2] - gt[1, 2])) < thresholds["rotation"]
+    return res, m, p
+
+
+def test_oracle_lm_recovers_perturbed_pose(sdfslam, oracle):
+    prob = sdfslam.Problem.workload("corner", 1)
+    opts = sdfslam.default_solver_options(max_iterations=40)
+    _recovered(sdfslam, oracle, prob, opts, GOLD["e2e_thresholds"])
+
+
+def test_oracle_trust_region_is_monotone(sdfslam, oracle):
+    prob = sdfslam.Problem.workload("corner", 1)
+    opts = sdfslam.default_solver_options(max_iterations=40, trust_region=1)
+    res, _, _ = _recovered(sdfslam, oracle, prob, opts, GOLD["e2e_thresholds"])
+    h = res["cost_history"]
+    assert np.all(h[1:] <= h[:-1] + 1e-9)
