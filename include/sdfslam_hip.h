@@ -174,10 +174,14 @@ int sdfslam_hip_assemble(sdfslam_hip_context* ctx, double* residuals, int32_t* j
  * original column order as CSC over the fixed pattern: col_ptr (n+1), row_idx
  * and values (pattern_nnz_lower), structural (pattern_nnz_lower bytes: 1 where
  * the reference's J^T J has a structural entry, i.e. some row couples the two
- * columns, or on the diagonal), rhs = -J^T r (n). */
-int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int32_t* col_ptr,
-                                 int32_t* row_idx, double* values, uint8_t* structural,
-                                 double* rhs);
+ * columns, or on the diagonal), rhs = -J^T r (n). The pattern is first made
+ * valid for the current state (which may start a new pattern epoch); its
+ * size is stored in *nnz. When it exceeds `capacity`, nothing else is
+ * written and SDFSLAM_ERR_INVALID_ARGUMENT is returned: call again with
+ * buffers of *nnz entries. */
+int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int64_t capacity,
+                                 int64_t* nnz, int32_t* col_ptr, int32_t* row_idx, double* values,
+                                 uint8_t* structural, double* rhs);
 
 /* Solution of (J^T J + damping) step = -J^T r at the current state
  * (lm_solver.cpp:351-364). Returns SDFSLAM_ERR_NOT_SPD on a zero pivot. */

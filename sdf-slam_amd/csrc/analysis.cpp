@@ -97,7 +97,8 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   std::vector<int> node_stamp(static_cast<size_t>(num_nodes), -1);
   std::vector<std::vector<int32_t>> footprint(static_cast<size_t>(N));
   std::vector<int> fp_bbox(4 * static_cast<size_t>(N), -1);
-  for (int P = 1; P < N; ++P) {
+  std::vector<std::vector<int32_t>> dilated(static_cast<size_t>(N));
+  for (int P = 0; P < N; ++P) {
     const double c = std::cos(poses[3 * P + 2]);
     const double sn = std::sin(poses[3 * P + 2]);
     std::vector<int> cells;
@@ -138,6 +139,10 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
           }
           dil_stamp[static_cast<size_t>(d)] = P;
           bits[d >> 5] |= 1U << (d & 31);
+          dilated[static_cast<size_t>(P)].push_back(d);
+          if (P == 0) {
+            continue;  // frame 0 is fixed: no pose columns
+          }
           const int corner[4] = {h * a.nx + w, h * a.nx + w + 1, (h + 1) * a.nx + w,
                                  (h + 1) * a.nx + w + 1};
           for (const int g : corner) {
@@ -162,6 +167,22 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
       fp_bbox[4 * static_cast<size_t>(P) + 1] = bw1;
       fp_bbox[4 * static_cast<size_t>(P) + 2] = bh0;
       fp_bbox[4 * static_cast<size_t>(P) + 3] = bh1;
+    }
+  }
+  // Candidate frames per cell: every frame whose dilated cells contain it
+  // (ascending), for the per-cell gather of the point-row partial sums.
+  a.cp_ptr.assign(static_cast<size_t>(a.num_cells) + 1, 0);
+  for (int P = 0; P < N; ++P) {
+    for (const int d : dilated[static_cast<size_t>(P)]) ++a.cp_ptr[static_cast<size_t>(d) + 1];
+  }
+  for (int d = 0; d < a.num_cells; ++d) a.cp_ptr[static_cast<size_t>(d) + 1] += a.cp_ptr[static_cast<size_t>(d)];
+  a.cp_pose.resize(static_cast<size_t>(a.cp_ptr[static_cast<size_t>(a.num_cells)]));
+  {
+    std::vector<int64_t> fill(a.cp_ptr.begin(), a.cp_ptr.end() - 1);
+    for (int P = 0; P < N; ++P) {
+      for (const int d : dilated[static_cast<size_t>(P)]) {
+        a.cp_pose[static_cast<size_t>(fill[static_cast<size_t>(d)]++)] = P;
+      }
     }
   }
   // Inverse footprint: poses per node (ascending).

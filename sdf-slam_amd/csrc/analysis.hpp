@@ -28,7 +28,7 @@
 namespace sdfslam {
 
 struct AnalysisOptions {
-  int margin{1};       // footprint dilation in cells
+  int margin{3};       // footprint dilation in cells
   int leaf_nodes{64};  // dissection stops at regions of at most this many nodes
 };
 
@@ -71,6 +71,9 @@ struct Analysis {
   // Dilated cell set of each pose as a bitmap (num_poses x words_per_pose).
   int words_per_pose{0};
   std::vector<uint32_t> fp_cellbits;
+  // Frames whose dilated cells contain each cell (ascending).
+  std::vector<int64_t> cp_ptr;
+  std::vector<int32_t> cp_pose;
   // Point rows of each pose (ascending).
   std::vector<int64_t> pr_ptr;
   std::vector<int32_t> pr_row;

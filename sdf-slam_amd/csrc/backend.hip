@@ -4,8 +4,6 @@
 // only the small LM state between batches of iterations.
 #include <hip/hip_runtime.h>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -188,15 +186,14 @@ struct sdfslam_hip_context {
   DeviceProblem dp{};
   DeviceAnalysis da{};
   DeviceWork dw{};
-  int32_t* iota{nullptr};
-  uint32_t* sorted_key{nullptr};
-  int32_t* sorted_row{nullptr};
   double* j_eik{nullptr};
   double* scalar{nullptr};
-  void* sort_tmp{nullptr};
-  size_t sort_tmp_bytes{0};
-  unsigned sort_bits{1};
   int history_capacity{0};
+  // Captured iteration: valid for one buffer generation and one set of LM options.
+  hipGraphExec_t graph{nullptr};
+  int graph_generation{-1};
+  LmOptions graph_options{};
+  int generation{0};
   int32_t* level_sn{nullptr};
   std::vector<int64_t> level_off;
   int32_t* child_lists{nullptr};
@@ -208,6 +205,7 @@ struct sdfslam_hip_context {
   bool lm_ready{false};
 
   ~sdfslam_hip_context() {
+    if (graph != nullptr) (void)hipGraphExecDestroy(graph);
     for (hipEvent_t& e : events) {
       if (e != nullptr) (void)hipEventDestroy(e);
     }
@@ -287,15 +285,9 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   W.od_r = A.Alloc<double>(3 * static_cast<size_t>(v.num_odom));
   W.od_ji = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
   W.od_jj = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
-  c.sorted_key = A.Alloc<uint32_t>(np);
-  c.sorted_row = A.Alloc<int32_t>(np);
-  W.sorted_key = c.sorted_key;
-  W.sorted_row = c.sorted_row;
-  std::vector<int32_t> iota(np);
-  for (size_t i = 0; i < np; ++i) iota[i] = static_cast<int32_t>(i);
-  c.iota = A.Upload(iota);
-  W.cell_begin = A.Alloc<int32_t>(static_cast<size_t>(P.num_cells) + 1);
-  W.cell_end = A.Alloc<int32_t>(static_cast<size_t>(P.num_cells) + 1);
+  W.seg_cell = A.Alloc<int32_t>(np);
+  W.seg_part = A.Alloc<double>(14 * np);
+  W.seg_count = A.Alloc<int32_t>(static_cast<size_t>(v.num_poses));
   W.nn = A.Alloc<double>(10 * static_cast<size_t>(P.num_cells));
   W.rn = A.Alloc<double>(4 * static_cast<size_t>(P.num_cells));
   W.rhs = A.Alloc<double>(static_cast<size_t>(P.n));
@@ -308,15 +300,6 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   c.scalar = A.Alloc<double>(1);
   LmState zero{};
   HIPC(hipMemcpy(W.state, &zero, sizeof(zero), hipMemcpyHostToDevice));
-  // Radix sort of the point rows by cell.
-  unsigned bits = 1;
-  while ((1ULL << bits) <= static_cast<unsigned long long>(P.num_cells)) ++bits;
-  c.sort_bits = bits;
-  if (np > 0) {
-    HIPC(rocprim::radix_sort_pairs(nullptr, c.sort_tmp_bytes, W.cellkey, c.sorted_key, c.iota,
-                                   c.sorted_row, np, 0, bits, c.stream));
-    c.sort_tmp = A.Alloc<unsigned char>(c.sort_tmp_bytes);
-  }
 }
 
 void ReadState(sdfslam_hip_context& c, LmState& st) {
@@ -377,6 +360,8 @@ void BuildEpoch(sdfslam_hip_context& c) {
   D.fp_cellbits = E.Upload(an.fp_cellbits);
   D.pr_ptr = E.Upload(an.pr_ptr);
   D.pr_row = E.Upload(an.pr_row);
+  D.cp_ptr = E.Upload(an.cp_ptr);
+  D.cp_pose = E.Upload(an.cp_pose);
   D.pd_off = E.Upload(an.pd_off);
   D.po_ptr = E.Upload(an.po_ptr);
   D.po_spec = E.Upload(an.po_spec);
@@ -425,6 +410,7 @@ void BuildEpoch(sdfslam_hip_context& c) {
   c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
   c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
   c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
+  ++c.generation;
 }
 
 enum Phase { kAssembly = 0, kNormal = 1, kFactor = 2, kSolve = 3, kUpdate = 4 };
@@ -437,13 +423,7 @@ void EnqueueAssemblyAndNormal(sdfslam_hip_context& c, const LmOptions& o, bool t
   hipStream_t s = c.stream;
   LaunchAssemble(c.dp, c.da, c.dw, true, s, timing ? c.events[kPhases + 1] : nullptr,
                  timing ? c.events[kPhases + 2] : nullptr);
-  if (c.dp.num_points > 0) {
-    HIPC(rocprim::radix_sort_pairs(c.sort_tmp, c.sort_tmp_bytes, c.dw.cellkey, c.sorted_key, c.iota,
-                                   c.sorted_row, static_cast<size_t>(c.dp.num_points), 0,
-                                   c.sort_bits, s));
-  }
   Mark(c, timing, 1);
-  LaunchCellReduce(c.dp, c.dw, s);
   LaunchFill(c.dw.panel, c.an.panel_total, 0.0, s);
   LaunchNormalSlots(c.dp, c.da, c.dw, o, c.an.max_points_per_pose, s);
 }
@@ -487,6 +467,35 @@ void EnqueueIteration(sdfslam_hip_context& c, const LmOptions& o, bool timing) {
   Mark(c, timing, 5);
 }
 
+bool SameOptions(const LmOptions& a, const LmOptions& b) {
+  return a.max_iterations == b.max_iterations && a.history_capacity == b.history_capacity &&
+         a.step_tolerance == b.step_tolerance && a.lambda_factor == b.lambda_factor &&
+         a.reject_worse_steps == b.reject_worse_steps && a.marquardt_scaling == b.marquardt_scaling &&
+         a.trust_region == b.trust_region;
+}
+
+/// One LM iteration captured as a HIP graph (re-captured when buffers or
+/// options change), so a batch of iterations is a batch of graph launches.
+hipGraphExec_t IterationGraph(sdfslam_hip_context& c, const LmOptions& o) {
+  if (c.graph != nullptr && c.graph_generation == c.generation && SameOptions(c.graph_options, o)) {
+    return c.graph;
+  }
+  if (c.graph != nullptr) {
+    HIPC(hipGraphExecDestroy(c.graph));
+    c.graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIPC(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  EnqueueIteration(c, o, false);
+  HIPC(hipStreamEndCapture(c.stream, &g));
+  const hipError_t e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIPC(e);
+  c.graph_generation = c.generation;
+  c.graph_options = o;
+  return c.graph;
+}
+
 void AccumulateTimes(sdfslam_hip_context& c) {
   for (int k = 0; k < kPhases; ++k) {
     float ms = 0.0f;
@@ -524,21 +533,30 @@ LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations
   ReadState(c, st);
   const int target = st.iter + iterations;
   int last_overflow_iter = -1;
+  int same_iter_overflow = 0;
   while (true) {
     if (st.halt != 0 || st.iter >= target) break;
     const int chunk = timing ? (target - st.iter) : std::min(target - st.iter, 8);
-    for (int k = 0; k < chunk; ++k) {
-      EnqueueIteration(c, o, timing);
-      if (timing) {
+    if (timing) {
+      for (int k = 0; k < chunk; ++k) {
+        EnqueueIteration(c, o, true);
         HIPC(hipStreamSynchronize(c.stream));
         AccumulateTimes(c);
       }
+    } else {
+      hipGraphExec_t g = IterationGraph(c, o);
+      for (int k = 0; k < chunk; ++k) HIPC(hipGraphLaunch(g, c.stream));
     }
     ReadState(c, st);
     if (st.not_spd != 0) break;
     if (st.overflow != 0) {
-      if (st.iter == last_overflow_iter) {
+      if (st.iter == last_overflow_iter && same_iter_overflow++ > 0) {
         throw Failure(SDFSLAM_ERR_INTERNAL, "pattern re-analysis did not cover the point cells");
+      }
+      // Re-centre the footprints on the current state; widen the dilation
+      // only when the previous epoch lasted no more than two iterations.
+      if (last_overflow_iter >= 0 && st.iter - last_overflow_iter <= 2) {
+        c.aopt.margin = std::min(c.aopt.margin + 1, 8);
       }
       last_overflow_iter = st.iter;
       BuildEpoch(c);
@@ -610,6 +628,7 @@ int SolveContext(sdfslam_hip_context& c, const sdfslam_solver_options& options,
   if (options.max_iterations + 1 > c.history_capacity) {
     c.history_capacity = options.max_iterations + 1;
     c.dw.cost_history = c.base.Alloc<double>(static_cast<size_t>(c.history_capacity));
+    ++c.generation;
   }
   const LmOptions o = ToLm(options, c.history_capacity);
   InitLm(c, options.lambda_init);
@@ -811,11 +830,11 @@ int sdfslam_hip_assemble(sdfslam_hip_context* ctx, double* residuals, int32_t* j
   });
 }
 
-int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int32_t* col_ptr,
-                                 int32_t* row_idx, double* values, uint8_t* structural,
-                                 double* rhs) {
+int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int64_t capacity,
+                                 int64_t* nnz, int32_t* col_ptr, int32_t* row_idx, double* values,
+                                 uint8_t* structural, double* rhs) {
   return Guard([&]() -> int {
-    if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
+    if (ctx == nullptr || nnz == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
     sdfslam_hip_context& c = *ctx;
     const LmOptions o = ToLm(c.opts, c.history_capacity);
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -828,7 +847,15 @@ int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int32_
       BuildEpoch(c);
     }
     const Analysis& an = c.an;
+    *nnz = an.pattern_nnz_lower;
+    if (an.pattern_nnz_lower > capacity) {
+      g_last_error = "normal-equation buffers hold fewer entries than the pattern";
+      return SDFSLAM_ERR_INVALID_ARGUMENT;
+    }
     const std::vector<PatternEntry> pat = LowerPattern(an, c.hp->view);
+    if (static_cast<int64_t>(pat.size()) != an.pattern_nnz_lower) {
+      throw Failure(SDFSLAM_ERR_INTERNAL, "pattern size mismatch");
+    }
     std::vector<double> panel(static_cast<size_t>(an.panel_total));
     HIPC(hipMemcpy(panel.data(), c.dw.panel, panel.size() * sizeof(double), hipMemcpyDeviceToHost));
     if (rhs != nullptr) {

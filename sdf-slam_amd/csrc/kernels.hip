@@ -279,52 +279,33 @@ __global__ void KAssembleOdom(DeviceProblem P, DeviceWork W) {
 
 // ---------------------------------------------------------------- J^T J
 
-// Range of each cell in the cell-sorted point order.
-__global__ void KCellRanges(int64_t np, const uint32_t* key, int32_t* begin, int32_t* end,
-                            const LmState* st) {
-  if (Halted(st)) return;
-  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (t >= np) return;
-  const uint32_t k = key[t];
-  if (t == 0 || key[t - 1] != k) begin[k] = static_cast<int32_t>(t);
-  if (t == np - 1 || key[t + 1] != k) end[k] = static_cast<int32_t>(t + 1);
-}
-
 // Per cell: the 10 products of its node weights and the 4 weight x residual
-// sums over the cell's points (all frames), one wavefront per cell, lanes in
-// a fixed stride and a fixed butterfly.
-__global__ void KCellReduce(DeviceProblem P, DeviceWork W) {
+// sums over the cell's points of every frame. Each frame's pose kernel has
+// summed its own points per occupied cell; here one wavefront per cell walks
+// the cell's candidate frames in ascending order (lanes in a fixed stride,
+// one binary search each, then a fixed butterfly).
+__global__ void KCellGather(DeviceProblem P, DeviceAnalysis A, DeviceWork W) {
   if (Halted(W.state)) return;
   const int lane = threadIdx.x & 63;
   const int cell = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
   if (cell >= P.num_cells) return;
-  const int b = W.cell_begin[cell];
-  const int e = W.cell_end[cell];
   const int64_t np = P.num_points;
   double acc[14];
 #pragma unroll
   for (int k = 0; k < 14; ++k) acc[k] = 0.0;
-  for (int t = b + lane; t < e; t += 64) {
-    const int row = W.sorted_row[t];
-    const double w0 = W.jw[row];
-    const double w1 = W.jw[np + row];
-    const double w2 = W.jw[2 * np + row];
-    const double w3 = W.jw[3 * np + row];
-    const double r = W.r_pt[row];
-    acc[0] += w0 * w0;
-    acc[1] += w0 * w1;
-    acc[2] += w0 * w2;
-    acc[3] += w0 * w3;
-    acc[4] += w1 * w1;
-    acc[5] += w1 * w2;
-    acc[6] += w1 * w3;
-    acc[7] += w2 * w2;
-    acc[8] += w2 * w3;
-    acc[9] += w3 * w3;
-    acc[10] += w0 * r;
-    acc[11] += w1 * r;
-    acc[12] += w2 * r;
-    acc[13] += w3 * r;
+  for (int64_t q = A.cp_ptr[cell] + lane; q < A.cp_ptr[cell + 1]; q += 64) {
+    const int pose = A.cp_pose[q];
+    const int64_t rb = A.pr_ptr[pose];
+    int lo = 0;
+    int hi = W.seg_count[pose];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (W.seg_cell[rb + mid] < cell) lo = mid + 1; else hi = mid;
+    }
+    if (lo < W.seg_count[pose] && W.seg_cell[rb + lo] == cell) {
+#pragma unroll
+      for (int k = 0; k < 14; ++k) acc[k] += W.seg_part[k * np + rb + lo];
+    }
   }
 #pragma unroll
   for (int k = 0; k < 14; ++k) acc[k] = WaveSum(acc[k]);
@@ -422,7 +403,7 @@ __global__ void KNodeSlots(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int 
 __global__ void KPoseBlock(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int marquardt,
                            int sort_size) {
   if (Halted(W.state)) return;
-  const int pose = blockIdx.x + 1;
+  const int pose = blockIdx.x;
   const int tid = threadIdx.x;
   extern __shared__ unsigned char smem[];
   uint64_t* keys = reinterpret_cast<uint64_t*>(smem);
@@ -504,6 +485,40 @@ __global__ void KPoseBlock(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int 
   if (valid_end_local >= 0) ustart[num_u] = valid_end_local;
   if (num_u == 0 && tid == 0) ustart[0] = 0;
   __syncthreads();
+
+  // This frame's partial sums per occupied cell, in the cell-sorted order.
+  for (int u = tid; u < num_u; u += kThreads) {
+    double acc[14];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) acc[k] = 0.0;
+    for (int t = ustart[u]; t < ustart[u + 1]; ++t) {
+      const int row = rows[static_cast<uint32_t>(keys[t] & 0xffffffffULL)];
+      const double w0 = W.jw[row];
+      const double w1 = W.jw[np + row];
+      const double w2 = W.jw[2 * np + row];
+      const double w3 = W.jw[3 * np + row];
+      const double r = W.r_pt[row];
+      acc[0] += w0 * w0;
+      acc[1] += w0 * w1;
+      acc[2] += w0 * w2;
+      acc[3] += w0 * w3;
+      acc[4] += w1 * w1;
+      acc[5] += w1 * w2;
+      acc[6] += w1 * w3;
+      acc[7] += w2 * w2;
+      acc[8] += w2 * w3;
+      acc[9] += w3 * w3;
+      acc[10] += w0 * r;
+      acc[11] += w1 * r;
+      acc[12] += w2 * r;
+      acc[13] += w3 * r;
+    }
+    W.seg_cell[rb + u] = ucell[u];
+#pragma unroll
+    for (int k = 0; k < 14; ++k) W.seg_part[k * np + rb + u] = acc[k];
+  }
+  if (tid == 0) W.seg_count[pose] = num_u;
+  if (pose == 0) return;  // frame 0 is fixed: no pose rows
 
   // Footprint entries: sum over the (up to) four cells around the node.
   const int nx = P.grid.nx;
@@ -675,8 +690,42 @@ __global__ void KExtendAddUpdate(DeviceAnalysis A, DeviceWork W, const int32_t* 
   }
 }
 
-// LDL^T of the kb-th diagonal block of each front at the level (in LDS).
-__global__ void KDiagFactor(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn, int kb) {
+// LDL^T of a 32x32 block a[off..off+nb) held in LDS, by one wavefront:
+// lane i < nb owns row i in registers and pivots are broadcast by lane
+// shuffles. Leaves unit-lower L below the diagonal and D on it.
+__device__ bool WaveLdlt32(double (*a)[kNB + 1], int off, int nb) {
+  const int lane = threadIdx.x & 63;
+  double r[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) r[j] = (lane < nb && j <= lane) ? a[off + lane][off + j] : 0.0;
+  bool zero = false;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (k < nb) {
+      const double d = __shfl(r[k], k, 64);
+      zero = zero || d == 0.0;
+      const double lk = r[k] / d;
+#pragma unroll
+      for (int j = k + 1; j < 32; ++j) {
+        if (j < nb) {
+          const double ljk = __shfl(lk, j, 64);
+          if (lane >= j && lane < nb) r[j] = fma(-r[k], ljk, r[j]);
+        }
+      }
+      if (lane > k) r[k] = lk;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    if (lane < nb && j <= lane) a[off + lane][off + j] = r[j];
+  }
+  return zero;
+}
+
+// LDL^T of the kb-th (up to 32x32) diagonal block of each front at the level,
+// factored in registers by one wavefront.
+__global__ void __launch_bounds__(64) KDiagFactor(DeviceAnalysis A, DeviceWork W,
+                                                  const int32_t* level_sn, int kb) {
   if (Halted(W.state)) return;
   const int s = level_sn[blockIdx.x];
   const int ns = A.sn_first[s + 1] - A.sn_first[s];
@@ -693,20 +742,9 @@ __global__ void KDiagFactor(DeviceAnalysis A, DeviceWork W, const int32_t* level
     if (i >= j) a[i][j] = F[static_cast<int64_t>(c0 + j) * f + c0 + i];
   }
   __syncthreads();
-  bool zero_pivot = false;
-  for (int k = 0; k < nb; ++k) {
-    const double d = a[k][k];
-    if (d == 0.0) zero_pivot = true;
-    for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-      const int i = idx % nb;
-      const int j = idx / nb;
-      if (j > k && i >= j) a[i][j] = fma(-a[i][k], a[j][k] / d, a[i][j]);
-    }
-    __syncthreads();
-    for (int i = k + 1 + tid; i < nb; i += blockDim.x) a[i][k] = a[i][k] / d;
-    __syncthreads();
-  }
-  if (zero_pivot && tid == 0) {
+  const bool zero = WaveLdlt32(a, 0, nb);
+  __syncthreads();
+  if (tid == 0 && zero) {
     W.state->not_spd = 1;
     W.state->halt = 1;
   }
@@ -718,7 +756,8 @@ __global__ void KDiagFactor(DeviceAnalysis A, DeviceWork W, const int32_t* level
 }
 
 // Rows below the kb-th diagonal block: L = F L_kk^-T D_k^-1 (one row per thread).
-__global__ void KPanelTrsm(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn, int kb) {
+__global__ void __launch_bounds__(kThreads) KPanelTrsm(DeviceAnalysis A, DeviceWork W,
+                                                       const int32_t* level_sn, int kb) {
   if (Halted(W.state)) return;
   const int s = level_sn[blockIdx.y];
   const int ns = A.sn_first[s + 1] - A.sn_first[s];
@@ -877,18 +916,39 @@ __global__ void KForward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn
   for (int b0 = 0; b0 < n; b0 += 64) {
     const int b1 = min(b0 + 64, n);
     if (tid < 64) {
+      // Diagonal block: lane i holds v[i]; the pivots are broadcast in order.
       const int i = b0 + tid;
       double vi = i < b1 ? v[i] : 0.0;
-      for (int k = b0; k < b1; ++k) {
-        const double yk = __shfl(vi, k - b0, 64);
-        if (i > k && i < b1) vi = fma(-F[static_cast<int64_t>(k) * f + i], yk, vi);
+      for (int k0 = b0; k0 < b1; k0 += 16) {
+        double fv[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          fv[t] = (k0 + t < b1 && i < b1) ? F[static_cast<int64_t>(k0 + t) * f + i] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const int k = k0 + t;
+          if (k < b1) {
+            const double yk = __shfl(vi, k - b0, 64);
+            if (i > k && i < b1) vi = fma(-fv[t], yk, vi);
+          }
+        }
       }
       if (i < b1) v[i] = vi;
     }
     __syncthreads();
+    // Rows below the block; loads batched ahead of the ordered FMA chain.
     for (int i = b1 + tid; i < f; i += blockDim.x) {
       double acc = v[i];
-      for (int k = b0; k < b1; ++k) acc = fma(-F[static_cast<int64_t>(k) * f + i], v[k], acc);
+      int k = b0;
+      for (; k + 8 <= b1; k += 8) {
+        double fv[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) fv[t] = F[static_cast<int64_t>(k + t) * f + i];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc = fma(-fv[t], v[k + t], acc);
+      }
+      for (; k < b1; ++k) acc = fma(-F[static_cast<int64_t>(k) * f + i], v[k], acc);
       v[i] = acc;
     }
     __syncthreads();
@@ -917,8 +977,17 @@ __global__ void KBackward(DeviceAnalysis A, DeviceWork W, const int32_t* level_s
   for (int a = tid; a < m; a += blockDim.x) t[n + a] = W.xnew[idx[n + a]];
   __syncthreads();
   for (int j = wave; j < n; j += waves) {
+    const double* col = F + static_cast<int64_t>(j) * f + n;
     double acc = 0.0;
-    for (int a = lane; a < m; a += 64) acc = fma(F[static_cast<int64_t>(j) * f + n + a], t[n + a], acc);
+    int a = lane;
+    for (; a + 192 < m; a += 256) {
+      const double f0 = col[a], f1 = col[a + 64], f2 = col[a + 128], f3 = col[a + 192];
+      acc = fma(f0, t[n + a], acc);
+      acc = fma(f1, t[n + a + 64], acc);
+      acc = fma(f2, t[n + a + 128], acc);
+      acc = fma(f3, t[n + a + 192], acc);
+    }
+    for (; a < m; a += 64) acc = fma(col[a], t[n + a], acc);
     acc = WaveSum(acc);
     if (lane == 0) t[j] -= acc;
   }
@@ -926,18 +995,41 @@ __global__ void KBackward(DeviceAnalysis A, DeviceWork W, const int32_t* level_s
   for (int b1 = n; b1 > 0;) {
     const int b0 = max(0, b1 - 64);
     for (int j = b0 + wave; j < b1; j += waves) {
+      const double* col = F + static_cast<int64_t>(j) * f;
       double acc = 0.0;
-      for (int i = b1 + lane; i < n; i += 64) acc = fma(F[static_cast<int64_t>(j) * f + i], t[i], acc);
+      int i = b1 + lane;
+      for (; i + 192 < n; i += 256) {
+        const double f0 = col[i], f1 = col[i + 64], f2 = col[i + 128], f3 = col[i + 192];
+        acc = fma(f0, t[i], acc);
+        acc = fma(f1, t[i + 64], acc);
+        acc = fma(f2, t[i + 128], acc);
+        acc = fma(f3, t[i + 192], acc);
+      }
+      for (; i < n; i += 64) acc = fma(col[i], t[i], acc);
       acc = WaveSum(acc);
       if (lane == 0) t[j] -= acc;
     }
     __syncthreads();
     if (tid < 64) {
+      // Within the block: lane i holds t[b0 + i]; x_j is final when reached.
       const int i = b0 + lane;
       double tv = i < b1 ? t[i] : 0.0;
-      for (int j = b1 - 1; j >= b0; --j) {
-        const double xj = __shfl(tv, j - b0, 64);
-        if (i < j) tv = fma(-F[static_cast<int64_t>(i) * f + j], xj, tv);
+      const double* row = F + static_cast<int64_t>(i) * f;  // column i of the panel
+      for (int j0 = b1 - 1; j0 >= b0; j0 -= 16) {
+        double fv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = j0 - u;
+          fv[u] = (j >= b0 && i < j) ? row[j] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = j0 - u;
+          if (j >= b0) {
+            const double xj = __shfl(tv, j - b0, 64);
+            if (i < j) tv = fma(-fv[u], xj, tv);
+          }
+        }
       }
       if (i < b1) t[i] = tv;
     }
@@ -1208,28 +1300,17 @@ void LaunchAssembleEikonalJ(const DeviceProblem& P, const DeviceWork& W, double*
   }
 }
 
-void LaunchCellReduce(const DeviceProblem& P, const DeviceWork& W, hipStream_t s) {
-  LaunchFillI32(W.cell_begin, P.num_cells + 1, 0, s);
-  LaunchFillI32(W.cell_end, P.num_cells + 1, 0, s);
-  if (P.num_points > 0) {
-    KCellRanges<<<Blocks(P.num_points), kThreads, 0, s>>>(P.num_points, W.sorted_key, W.cell_begin,
-                                                          W.cell_end, W.state);
-  }
-  KCellReduce<<<Blocks(P.num_cells, kThreads / 64), kThreads, 0, s>>>(P, W);
-}
-
 void LaunchNormalSlots(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
                        const LmOptions& o, int max_points_per_pose, hipStream_t s) {
+  int sort_size = 1;
+  while (sort_size < max_points_per_pose) sort_size <<= 1;
+  const size_t lds = static_cast<size_t>(sort_size) * (8 + 4 + 4 + 4) + 4;
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KPoseBlock),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+  KPoseBlock<<<P.num_poses, kThreads, lds, s>>>(P, A, W, o.marquardt_scaling, sort_size);
+  KCellGather<<<Blocks(P.num_cells, kThreads / 64), kThreads, 0, s>>>(P, A, W);
   KNodeSlots<<<Blocks(static_cast<int64_t>(P.grid.nx) * P.grid.ny), kThreads, 0, s>>>(
       P, A, W, o.marquardt_scaling);
-  if (P.num_poses > 1) {
-    int sort_size = 1;
-    while (sort_size < max_points_per_pose) sort_size <<= 1;
-    const size_t lds = static_cast<size_t>(sort_size) * (8 + 4 + 4 + 4) + 4;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KPoseBlock),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-    KPoseBlock<<<P.num_poses - 1, kThreads, lds, s>>>(P, A, W, o.marquardt_scaling, sort_size);
-  }
   if (A.num_pairs > 0) {
     KOdomPairs<<<Blocks(A.num_pairs), kThreads, 0, s>>>(P, A, W);
   }
@@ -1251,7 +1332,7 @@ void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32
                        int count, int max_n, int max_f, int max_m, hipStream_t s) {
   const int nblocks = (max_n + kNB - 1) / kNB;
   for (int kb = 0; kb < nblocks; ++kb) {
-    KDiagFactor<<<count, kThreads, 0, s>>>(A, W, level_sn, kb);
+    KDiagFactor<<<count, 64, 0, s>>>(A, W, level_sn, kb);
     const int rows = max_f - kb * kNB;
     if (rows > 0) {
       KPanelTrsm<<<dim3((rows + kThreads - 1) / kThreads, count), kThreads, 0, s>>>(A, W, level_sn, kb);

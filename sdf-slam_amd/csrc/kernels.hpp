@@ -91,10 +91,9 @@ struct DeviceWork {
   double* od_r;   // 3 per spec
   double* od_ji;  // 9 per spec, row-major (residual component, pose component)
   double* od_jj;
-  const uint32_t* sorted_key;
-  const int32_t* sorted_row;
-  int32_t* cell_begin;
-  int32_t* cell_end;
+  int32_t* seg_cell;   // per frame, its occupied cells (ascending) at the frame's row offset
+  double* seg_part;    // 14 x num_points: per occupied cell, 10 weight products + 4 weight x residual
+  int32_t* seg_count;  // occupied cells per frame
   double* nn;  // 10 x num_cells
   double* rn;  // 4 x num_cells
   double* rhs;  // -J^T r in original column order
@@ -134,6 +133,8 @@ struct DeviceAnalysis {
   const uint32_t* fp_cellbits;
   const int64_t* pr_ptr;
   const int32_t* pr_row;
+  const int64_t* cp_ptr;  // candidate frames of each cell
+  const int32_t* cp_pose;
   const int64_t* pd_off;
   const int64_t* po_ptr;
   const int32_t* po_spec;
@@ -156,7 +157,6 @@ void LaunchFillI32(int32_t* p, int64_t n, int32_t value, hipStream_t s);
 void LaunchAssemble(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
                     bool check_footprint, hipStream_t s, hipEvent_t before = nullptr,
                     hipEvent_t after = nullptr);
-void LaunchCellReduce(const DeviceProblem& P, const DeviceWork& W, hipStream_t s);
 void LaunchNormalSlots(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
                        const LmOptions& o, int max_points_per_pose, hipStream_t s);
 

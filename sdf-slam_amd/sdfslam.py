@@ -134,8 +134,8 @@ hip.sdfslam_hip_set_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_get_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_analysis.argtypes = [C.c_void_p, C.POINTER(AnalysisInfo)]
 hip.sdfslam_hip_assemble.argtypes = [C.c_void_p, _f64p, _i32p, _f64p]
-hip.sdfslam_hip_normal_equations.argtypes = [C.c_void_p, C.c_double, _i32p, _i32p, _f64p, _u8p,
-                                             _f64p]
+hip.sdfslam_hip_normal_equations.argtypes = [C.c_void_p, C.c_double, C.c_int64, _i64p, _i32p,
+                                             _i32p, _f64p, _u8p, _f64p]
 hip.sdfslam_hip_linear_step.argtypes = [C.c_void_p, C.c_double, _f64p]
 hip.sdfslam_hip_cost.argtypes = [C.c_void_p, _f64p]
 hip.sdfslam_hip_context_solve.argtypes = [C.c_void_p, C.POINTER(SolverOptions),
@@ -399,17 +399,26 @@ class Context:
         return r, cols, vals
 
     def normal_equations(self, lam):
-        info = self.analysis()
-        nnz = info["pattern_nnz_lower"]
+        """Damped J^T J (lower, original column order, fixed pattern), the
+        structural mask of the reference pattern, and -J^T r."""
         n = self.n
-        cp = np.zeros(n + 1, dtype=np.int32)
-        ri = np.zeros(nnz, dtype=np.int32)
-        va = np.zeros(nnz)
-        st = np.zeros(nnz, dtype=np.uint8)
-        rhs = np.zeros(n)
-        _check_hip(hip.sdfslam_hip_normal_equations(self._h, lam, ptr(cp, _i32p), ptr(ri, _i32p),
-                                                    ptr(va, _f64p), ptr(st, _u8p), ptr(rhs, _f64p)))
-        return cp, ri, va, st.astype(bool), rhs
+        need = C.c_int64(0)
+        cap = self.analysis()["pattern_nnz_lower"]
+        while True:
+            cp = np.zeros(n + 1, dtype=np.int32)
+            ri = np.zeros(max(cap, 1), dtype=np.int32)
+            va = np.zeros(max(cap, 1))
+            st = np.zeros(max(cap, 1), dtype=np.uint8)
+            rhs = np.zeros(n)
+            status = hip.sdfslam_hip_normal_equations(
+                self._h, lam, cap, C.byref(need), ptr(cp, _i32p), ptr(ri, _i32p), ptr(va, _f64p),
+                ptr(st, _u8p), ptr(rhs, _f64p))
+            if status == SDFSLAM_ERR_INVALID_ARGUMENT and need.value > cap:
+                cap = need.value  # the pattern grew (a new epoch); retry with room for it
+                continue
+            _check_hip(status)
+            k = need.value
+            return cp, ri[:k], va[:k], st[:k].astype(bool), rhs
 
     def linear_step(self, lam):
         step = np.zeros(self.n)
