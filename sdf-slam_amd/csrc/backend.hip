@@ -409,6 +409,7 @@ void BuildEpoch(sdfslam_hip_context& c) {
   }
   c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
   c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
+  c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 64 * 64);
   c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
   ++c.generation;
 }

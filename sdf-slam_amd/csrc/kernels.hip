@@ -18,7 +18,7 @@ namespace sdfslam {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kNB = 32;     // block column width of the panel factorization
+constexpr int kNB = 64;     // block column width of the panel factorization
 constexpr int kTile = 64;   // GEMM tile edge
 constexpr int kTk = 16;     // GEMM k chunk
 
@@ -690,73 +690,70 @@ __global__ void KExtendAddUpdate(DeviceAnalysis A, DeviceWork W, const int32_t* 
   }
 }
 
-// LDL^T of a 32x32 block a[off..off+nb) held in LDS, by one wavefront:
-// lane i < nb owns row i in registers and pivots are broadcast by lane
-// shuffles. Leaves unit-lower L below the diagonal and D on it.
-__device__ bool WaveLdlt32(double (*a)[kNB + 1], int off, int nb) {
-  const int lane = threadIdx.x & 63;
-  double r[32];
+// One 64-column step of the blocked panel factorization of every front at
+// the level. Each workgroup factors the kb-th diagonal block in LDS (right
+// looking, every thread owning fixed lower-triangle pairs) and solves 128
+// rows below it, L = F L_kk^-T D_k^-1, two threads per row (columns 0..31,
+// then 32..63). Workgroup 0 stores the factored block in the front's staging
+// slot; the trailing-update kernel moves it into the panel, so no workgroup
+// of this launch reads a partly written block.
+// Columns 0..31 of a panel row below the diagonal block: y = x L11^-T,
+// L = y D^-1; y is kept in `ybuf` for the second half.
+__device__ __forceinline__ void SolveRowFirstHalf(double* F, int f, int c0, int nb, int r,
+                                               const double (*a)[kNB + 1], double* ybuf) {
+  // The per-column guards keep the compiler from hoisting every LDS load
+  // of the unrolled solve (which spills registers).
+  double y[32];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) r[j] = (lane < nb && j <= lane) ? a[off + lane][off + j] : 0.0;
-  bool zero = false;
+  for (int j = 0; j < 32; ++j) {
+    y[j] = 0.0;
+    if (j < nb) {
+      double x = F[static_cast<int64_t>(c0 + j) * f + r];
 #pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    if (k < nb) {
-      const double d = __shfl(r[k], k, 64);
-      zero = zero || d == 0.0;
-      const double lk = r[k] / d;
-#pragma unroll
-      for (int j = k + 1; j < 32; ++j) {
-        if (j < nb) {
-          const double ljk = __shfl(lk, j, 64);
-          if (lane >= j && lane < nb) r[j] = fma(-r[k], ljk, r[j]);
-        }
+      for (int t = 0; t < 32; ++t) {
+        if (t < j) x = fma(-y[t], a[j][t], x);
       }
-      if (lane > k) r[k] = lk;
+      y[j] = x;
     }
   }
 #pragma unroll
   for (int j = 0; j < 32; ++j) {
-    if (lane < nb && j <= lane) a[off + lane][off + j] = r[j];
-  }
-  return zero;
-}
-
-// LDL^T of the kb-th (up to 32x32) diagonal block of each front at the level,
-// factored in registers by one wavefront.
-__global__ void __launch_bounds__(64) KDiagFactor(DeviceAnalysis A, DeviceWork W,
-                                                  const int32_t* level_sn, int kb) {
-  if (Halted(W.state)) return;
-  const int s = level_sn[blockIdx.x];
-  const int ns = A.sn_first[s + 1] - A.sn_first[s];
-  const int c0 = kb * kNB;
-  if (c0 >= ns) return;
-  const int nb = min(kNB, ns - c0);
-  const int f = A.sn_f[s];
-  double* F = W.panel + A.panel_off[s];
-  __shared__ double a[kNB][kNB + 1];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-    const int i = idx % nb;
-    const int j = idx / nb;
-    if (i >= j) a[i][j] = F[static_cast<int64_t>(c0 + j) * f + c0 + i];
-  }
-  __syncthreads();
-  const bool zero = WaveLdlt32(a, 0, nb);
-  __syncthreads();
-  if (tid == 0 && zero) {
-    W.state->not_spd = 1;
-    W.state->halt = 1;
-  }
-  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-    const int i = idx % nb;
-    const int j = idx / nb;
-    if (i >= j) F[static_cast<int64_t>(c0 + j) * f + c0 + i] = a[i][j];
+    ybuf[j] = y[j];
+    if (j < nb) F[static_cast<int64_t>(c0 + j) * f + r] = y[j] / a[j][j];
   }
 }
 
-// Rows below the kb-th diagonal block: L = F L_kk^-T D_k^-1 (one row per thread).
-__global__ void __launch_bounds__(kThreads) KPanelTrsm(DeviceAnalysis A, DeviceWork W,
+// Columns 32..63: x = (F - y L21^T) L22^-T, L = x D^-1.
+__device__ __forceinline__ void SolveRowSecondHalf(double* F, int f, int c0, int nb, int r,
+                                                const double (*a)[kNB + 1], const double* ybuf) {
+  double x[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) x[j] = 32 + j < nb ? F[static_cast<int64_t>(c0 + 32 + j) * f + r] : 0.0;
+#pragma unroll 1
+  for (int t = 0; t < 32; ++t) {
+    const double yt = ybuf[t];
+#pragma unroll
+    for (int j = 0; j < 32; ++j) x[j] = fma(-yt, a[32 + j][t], x[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    if (32 + j < nb) {
+#pragma unroll
+      for (int t = 0; t < 32; ++t) {
+        if (t < j) x[j] = fma(-x[t], a[32 + j][32 + t], x[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    if (32 + j < nb) F[static_cast<int64_t>(c0 + 32 + j) * f + r] = x[j] / a[32 + j][32 + j];
+  }
+}
+
+constexpr int kStepRows = 128;
+constexpr int kPairSlots = (kNB * (kNB + 1) / 2 + kThreads - 1) / kThreads;
+
+__global__ void __launch_bounds__(kThreads) KPanelStep(DeviceAnalysis A, DeviceWork W,
                                                        const int32_t* level_sn, int kb) {
   if (Halted(W.state)) return;
   const int s = level_sn[blockIdx.y];
@@ -765,34 +762,68 @@ __global__ void __launch_bounds__(kThreads) KPanelTrsm(DeviceAnalysis A, DeviceW
   if (c0 >= ns) return;
   const int nb = min(kNB, ns - c0);
   const int f = A.sn_f[s];
-  const int r0 = c0 + nb + blockIdx.x * blockDim.x;
+  const int r0 = c0 + blockIdx.x * kStepRows;
   if (r0 >= f) return;
   double* F = W.panel + A.panel_off[s];
-  __shared__ double L[kNB][kNB + 1];
-  for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+  __shared__ double a[kNB][kNB + 1];
+  __shared__ double lcol[kNB];
+  __shared__ double ybuf[kStepRows][33];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
     const int i = idx % nb;
     const int j = idx / nb;
-    if (i >= j) L[i][j] = F[static_cast<int64_t>(c0 + j) * f + c0 + i];
+    if (i >= j) a[i][j] = F[static_cast<int64_t>(c0 + j) * f + c0 + i];
+  }
+  // Lower-triangle pairs p = tid + 256 q, p = i (i + 1) / 2 + j.
+  int pi[kPairSlots];
+  int pj[kPairSlots];
+#pragma unroll
+  for (int q = 0; q < kPairSlots; ++q) {
+    const int p = tid + kThreads * q;
+    int i = static_cast<int>((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+    while (i * (i + 1) / 2 > p) --i;
+    while ((i + 1) * (i + 2) / 2 <= p) ++i;
+    const int j = p - i * (i + 1) / 2;
+    const bool live = p < kNB * (kNB + 1) / 2 && i < nb;
+    pi[q] = live ? i : -1;
+    pj[q] = j;
   }
   __syncthreads();
-  const int r = r0 + threadIdx.x;
-  if (r >= f) return;
-  double y[kNB];
+  bool zero = false;
+  for (int k = 0; k < nb; ++k) {
+    const double d = a[k][k];
+    zero = zero || d == 0.0;
+    if (tid > k && tid < nb) {
+      const double l = a[tid][k] / d;
+      lcol[tid] = l;
+      a[tid][k] = l;
+    }
+    __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    if (j < nb) {
-      double x = F[static_cast<int64_t>(c0 + j) * f + r];
-#pragma unroll
-      for (int t = 0; t < kNB; ++t) {
-        if (t < j) x = fma(-y[t], L[j][t], x);
-      }
-      y[j] = x;
+    for (int q = 0; q < kPairSlots; ++q) {
+      if (pi[q] >= 0 && pj[q] > k) a[pi[q]][pj[q]] = fma(-lcol[pi[q]] * d, lcol[pj[q]], a[pi[q]][pj[q]]);
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0) {
+    if (tid == 0 && zero) {
+      W.state->not_spd = 1;
+      W.state->halt = 1;
+    }
+    double* stage = W.diag_stage + static_cast<int64_t>(s) * kNB * kNB;
+    for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+      const int i = idx % nb;
+      const int j = idx / nb;
+      if (i >= j) stage[j * kNB + i] = a[i][j];
     }
   }
-#pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    if (j < nb) F[static_cast<int64_t>(c0 + j) * f + r] = y[j] / L[j][j];
-  }
+  const int lr = tid & (kStepRows - 1);
+  const int half = tid / kStepRows;
+  const int r = r0 + lr;
+  const bool active = r >= c0 + nb && r < f;
+  if (half == 0 && active) SolveRowFirstHalf(F, f, c0, nb, r, a, ybuf[lr]);
+  __syncthreads();
+  if (half == 1 && active && nb > 32) SolveRowSecondHalf(F, f, c0, nb, r, a, ybuf[lr]);
 }
 
 // C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
@@ -807,7 +838,22 @@ __global__ void __launch_bounds__(kThreads) KGemmLower(DeviceAnalysis A, DeviceW
   const int s = level_sn[blockIdx.z];
   const int ns = A.sn_first[s + 1] - A.sn_first[s];
   const int f = A.sn_f[s];
-  const double* F = W.panel + A.panel_off[s];
+  double* Fw = W.panel + A.panel_off[s];
+  const double* F = Fw;
+  const double* stage = W.diag_stage + static_cast<int64_t>(s) * kNB * kNB;
+  if (mode == 0) {
+    const int c0 = kb * kNB;
+    if (c0 >= ns) return;
+    if (ti == 0 && tj == 0) {
+      // The step kernel's factored diagonal block goes into the panel here.
+      const int nb = min(kNB, ns - c0);
+      for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+        const int i = idx % nb;
+        const int j = idx / nb;
+        if (i >= j) Fw[static_cast<int64_t>(c0 + j) * f + c0 + i] = stage[j * kNB + i];
+      }
+    }
+  }
   int kbase, K, rbase, M, N;
   double* C;
   int64_t ldc;
@@ -854,7 +900,7 @@ __global__ void __launch_bounds__(kThreads) KGemmLower(DeviceAnalysis A, DeviceW
       const int kcol = kbase + k0 + kk;
       const bool kin = k0 + kk < K;
       const int64_t colbase = static_cast<int64_t>(kcol) * f;
-      const double dk = kin ? F[colbase + kcol] : 0.0;
+      const double dk = !kin ? 0.0 : (mode == 0 ? stage[(k0 + kk) * kNB + k0 + kk] : F[colbase + kcol]);
       As[kk][rr] = (kin && R0 + rr < M) ? F[colbase + rbase + R0 + rr] : 0.0;
       Bs[kk][rr] = (kin && C0 + rr < N) ? F[colbase + rbase + C0 + rr] * dk : 0.0;
     }
@@ -1332,18 +1378,13 @@ void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32
                        int count, int max_n, int max_f, int max_m, hipStream_t s) {
   const int nblocks = (max_n + kNB - 1) / kNB;
   for (int kb = 0; kb < nblocks; ++kb) {
-    KDiagFactor<<<count, 64, 0, s>>>(A, W, level_sn, kb);
     const int rows = max_f - kb * kNB;
-    if (rows > 0) {
-      KPanelTrsm<<<dim3((rows + kThreads - 1) / kThreads, count), kThreads, 0, s>>>(A, W, level_sn, kb);
-    }
+    KPanelStep<<<dim3((rows + kStepRows - 1) / kStepRows, count), kThreads, 0, s>>>(A, W, level_sn, kb);
     const int trail_rows = max_f - (kb + 1) * kNB;
     const int trail_cols = max_n - (kb + 1) * kNB;
-    if (trail_rows > 0 && trail_cols > 0) {
-      const int tr = (trail_rows + kTile - 1) / kTile;
-      const int tc = (trail_cols + kTile - 1) / kTile;
-      KGemmLower<<<dim3(tr, tc, count), kThreads, 0, s>>>(A, W, level_sn, kb, 0);
-    }
+    const int tr = trail_rows > 0 ? (trail_rows + kTile - 1) / kTile : 1;
+    const int tc = trail_cols > 0 ? (trail_cols + kTile - 1) / kTile : 1;
+    KGemmLower<<<dim3(tr, tc, count), kThreads, 0, s>>>(A, W, level_sn, kb, 0);
   }
   if (max_m > 0) {
     const int t = (max_m + kTile - 1) / kTile;
@@ -1353,12 +1394,15 @@ void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32
 
 void LaunchForwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
                         int count, int max_f, hipStream_t s) {
-  KForward<<<count, kThreads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn);
+  // Few large fronts: one 1024-thread workgroup each for more loads in flight.
+  const int threads = count <= 32 ? 1024 : kThreads;
+  KForward<<<count, threads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn);
 }
 
 void LaunchBackwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
                          int count, int max_f, hipStream_t s) {
-  KBackward<<<count, kThreads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn);
+  const int threads = count <= 32 ? 1024 : kThreads;
+  KBackward<<<count, threads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn);
 }
 
 void LaunchCost(const DeviceProblem& P, const DeviceWork& W, const double* map,
