@@ -84,11 +84,13 @@ def main():
     print(f"[bench] built {args.workload}: n={problem.num_parameters} rows={problem.num_residuals} "
           f"in {t_build:.2f} s", file=sys.stderr, flush=True)
     ctx.iterate(args.warmup)
-    epochs0 = ctx.analysis()["pattern_epochs"]
+    info0 = ctx.analysis()
     t0 = time.perf_counter()
     ctx.iterate(args.steps)
     elapsed = time.perf_counter() - t0
-    reanalyses = ctx.analysis()["pattern_epochs"] - epochs0
+    info1 = ctx.analysis()
+    reanalyses = info1["pattern_epochs"] - info0["pattern_epochs"]
+    reanalysis_s = info1["analysis_seconds"] - info0["analysis_seconds"]
     ms_per_step = 1e3 * elapsed / args.steps
     print(f"[bench] timed {args.steps} iterations: {ms_per_step:.3f} ms/iter", file=sys.stderr,
           flush=True)
@@ -121,6 +123,7 @@ def main():
             "supernodes": info["num_supernodes"],
             "pattern_epochs": info["pattern_epochs"],
             "reanalyses_in_timed_steps": reanalyses,
+            "reanalysis_ms_in_timed_steps": round(1e3 * reanalysis_s, 2),
             "phase_ms": {k: round(v, 4) for k, v in phases.items()},
         },
         "roofline": {

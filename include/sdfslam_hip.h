@@ -142,12 +142,21 @@ int sdfslam_hip_context_create(const sdfslam_problem_view* problem,
                                const sdfslam_solver_options* options, const double* map_values,
                                const double* poses, sdfslam_hip_context** out);
 int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx);
+/* Replaces the context's problem and state with a new one (the next
+ * increment of an incremental run: the problem rebuilt over more frames, as
+ * runner.cpp:249-260 rebuilds it before each Solve), keeping the context's
+ * stream and device memory. The next context_solve behaves as a Solve on a
+ * fresh context created with the new problem and the context's options. */
+int sdfslam_hip_context_update(sdfslam_hip_context* ctx, const sdfslam_problem_view* problem,
+                               const double* map_values, const double* poses);
 int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, const double* poses);
 int sdfslam_hip_get_state(sdfslam_hip_context* ctx, double* map_values, double* poses);
 
 /* Sizes of the analysis: num_parameters, num_residuals, nnz of the lower
  * triangle of the fixed J^T J pattern, supernodes, nnz of the supernodal factor
- * (incl. explicit zeros), and the factorization flop count sum_j (c_j^2 + 3 c_j). */
+ * (incl. explicit zeros), and the factorization flop count sum_j (c_j^2 + 3 c_j);
+ * pattern_epochs counts the analyses so far and analysis_seconds their total
+ * host time (analysis, uploads, allocation). */
 typedef struct sdfslam_hip_analysis_info {
   int64_t num_parameters;
   int64_t num_residuals;
@@ -158,6 +167,7 @@ typedef struct sdfslam_hip_analysis_info {
   int64_t update_doubles;
   int32_t tree_levels;
   int32_t pattern_epochs;
+  double analysis_seconds;
 } sdfslam_hip_analysis_info;
 int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* info);
 

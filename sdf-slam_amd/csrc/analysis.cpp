@@ -3,9 +3,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 namespace sdfslam {
 
@@ -22,6 +24,91 @@ struct NdNode {
 
 bool BoxContains(const NdNode& t, int bw0, int bw1, int bh0, int bh1) {
   return bw0 >= t.w0 && bw1 < t.w1 && bh0 >= t.h0 && bh1 < t.h1;
+}
+
+/// Host threads for the analysis (the machine's cores, at most 16).
+int AnalysisThreads() {
+  const unsigned hc = std::thread::hardware_concurrency();
+  return static_cast<int>(std::clamp(hc == 0 ? 1U : hc, 1U, 16U));
+}
+
+/// Splits [0, n) into `chunks` contiguous ranges; chunk k is
+/// [Bound(n, chunks, k), Bound(n, chunks, k + 1)).
+int64_t Bound(int64_t n, int chunks, int k) { return n * k / chunks; }
+
+/// Runs fn(k) for k in [0, chunks) on separate threads and rethrows the
+/// first exception.
+template <typename F>
+void RunChunks(int chunks, F&& fn) {
+  if (chunks <= 1) {
+    if (chunks == 1) fn(0);
+    return;
+  }
+  std::vector<std::thread> threads;
+  std::vector<std::exception_ptr> errors(static_cast<size_t>(chunks));
+  threads.reserve(static_cast<size_t>(chunks) - 1);
+  for (int k = 1; k < chunks; ++k) {
+    threads.emplace_back([&, k] {
+      try {
+        fn(k);
+      } catch (...) {
+        errors[static_cast<size_t>(k)] = std::current_exception();
+      }
+    });
+  }
+  try {
+    fn(0);
+  } catch (...) {
+    errors[0] = std::current_exception();
+  }
+  for (std::thread& t : threads) t.join();
+  for (const std::exception_ptr& e : errors) {
+    if (e) std::rethrow_exception(e);
+  }
+}
+
+/// Parallel loop over [0, n): fn(i) for every i, in contiguous chunks.
+template <typename F>
+void ParallelFor(int64_t n, int threads, int64_t grain, F&& fn) {
+  const int chunks = static_cast<int>(std::min<int64_t>(threads, std::max<int64_t>(n / grain, 1)));
+  RunChunks(chunks, [&](int k) {
+    for (int64_t i = Bound(n, chunks, k); i < Bound(n, chunks, k + 1); ++i) fn(i);
+  });
+}
+
+/// CSR inversion: for sources 0..num_src-1 (in chunks) with target lists
+/// lists[src], builds ptr (num_tgt + 1) and out such that each target's
+/// sources are ascending. Only sources in [src_begin, num_src) take part.
+void InvertLists(const std::vector<std::vector<int32_t>>& lists, int src_begin, int num_tgt,
+                 int threads, std::vector<int64_t>& ptr, std::vector<int32_t>& out) {
+  const int num_src = static_cast<int>(lists.size());
+  const int span = std::max(num_src - src_begin, 0);
+  const int chunks = std::max(1, std::min(threads, span / 16));
+  std::vector<int64_t> cnt(static_cast<size_t>(chunks) * static_cast<size_t>(num_tgt), 0);
+  RunChunks(chunks, [&](int k) {
+    int64_t* c = cnt.data() + static_cast<size_t>(k) * static_cast<size_t>(num_tgt);
+    for (int64_t s = src_begin + Bound(span, chunks, k); s < src_begin + Bound(span, chunks, k + 1); ++s) {
+      for (const int32_t t : lists[static_cast<size_t>(s)]) ++c[t];
+    }
+  });
+  ptr.assign(static_cast<size_t>(num_tgt) + 1, 0);
+  int64_t run = 0;
+  for (int t = 0; t < num_tgt; ++t) {
+    for (int k = 0; k < chunks; ++k) {
+      int64_t& c = cnt[static_cast<size_t>(k) * static_cast<size_t>(num_tgt) + static_cast<size_t>(t)];
+      const int64_t n = c;
+      c = run;  // becomes chunk k's write cursor for target t
+      run += n;
+    }
+    ptr[static_cast<size_t>(t) + 1] = run;
+  }
+  out.resize(static_cast<size_t>(run));
+  RunChunks(chunks, [&](int k) {
+    int64_t* c = cnt.data() + static_cast<size_t>(k) * static_cast<size_t>(num_tgt);
+    for (int64_t s = src_begin + Bound(span, chunks, k); s < src_begin + Bound(span, chunks, k + 1); ++s) {
+      for (const int32_t t : lists[static_cast<size_t>(s)]) out[static_cast<size_t>(c[t]++)] = static_cast<int32_t>(s);
+    }
+  });
 }
 
 }  // namespace
@@ -54,6 +141,14 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   const int num_nodes = a.nx * a.ny;
   const int N = a.num_poses;
   const int margin = std::max(1, opt.margin);
+  auto margin_of = [&](int P) {
+    return opt.pose_margin.empty() ? margin
+                                   : std::max(1, opt.pose_margin[static_cast<size_t>(P)]);
+  };
+  if (!opt.pose_margin.empty() && static_cast<int>(opt.pose_margin.size()) != N) {
+    throw std::invalid_argument("per-pose margins do not match the pose count");
+  }
+  const int threads = AnalysisThreads();
 
   std::vector<int32_t> active_to_node(static_cast<size_t>(a.num_active), -1);
   for (int g = 0; g < num_nodes; ++g) {
@@ -89,122 +184,151 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     }
   }
 
-  // Footprints: dilated cells of each pose's points at the current poses.
+  // Footprints: dilated cells of each pose's points at the current poses
+  // (pose P dilates by its own margin), one chunk of poses per thread.
   a.words_per_pose = (a.num_cells + 31) / 32;
   a.fp_cellbits.assign(static_cast<size_t>(N) * static_cast<size_t>(a.words_per_pose), 0U);
-  std::vector<int> cell_stamp(static_cast<size_t>(a.num_cells), -1);
-  std::vector<int> dil_stamp(static_cast<size_t>(a.num_cells), -1);
-  std::vector<int> node_stamp(static_cast<size_t>(num_nodes), -1);
   std::vector<std::vector<int32_t>> footprint(static_cast<size_t>(N));
   std::vector<int> fp_bbox(4 * static_cast<size_t>(N), -1);
   std::vector<std::vector<int32_t>> dilated(static_cast<size_t>(N));
-  for (int P = 0; P < N; ++P) {
-    const double c = std::cos(poses[3 * P + 2]);
-    const double sn = std::sin(poses[3 * P + 2]);
-    std::vector<int> cells;
-    for (int64_t k = a.pr_ptr[static_cast<size_t>(P)]; k < a.pr_ptr[static_cast<size_t>(P) + 1]; ++k) {
-      const int32_t s = a.pr_row[static_cast<size_t>(k)];
-      const double px = v.point_x[s];
-      const double py = v.point_y[s];
-      const double gx = c * px + (-sn) * py + poses[3 * P];
-      const double gy = sn * px + c * py + poses[3 * P + 1];
-      const double fx = (gx - v.grid.min_x) / v.grid.dx;
-      const double fy = (gy - v.grid.min_y) / v.grid.dy;
-      if (!(fx >= -1.0 && fy >= -1.0 && fx <= a.nx && fy <= a.ny)) {
-        continue;  // well outside the domain: no Jacobian row
-      }
-      const int w = std::clamp(static_cast<int>(std::floor(fx)), 0, ncx - 1);
-      const int h = std::clamp(static_cast<int>(std::floor(fy)), 0, ncy - 1);
-      const int cell = h * ncx + w;
-      if (cell_stamp[static_cast<size_t>(cell)] != P) {
-        cell_stamp[static_cast<size_t>(cell)] = P;
-        cells.push_back(cell);
-      }
-    }
-    uint32_t* bits = a.fp_cellbits.data() + static_cast<size_t>(P) * static_cast<size_t>(a.words_per_pose);
-    std::vector<int32_t>& fp = footprint[static_cast<size_t>(P)];
-    for (const int cell : cells) {
-      const int cw = cell % ncx;
-      const int ch = cell / ncx;
-      for (int dh = -margin; dh <= margin; ++dh) {
-        for (int dw = -margin; dw <= margin; ++dw) {
-          const int w = cw + dw;
-          const int h = ch + dh;
-          if (w < 0 || h < 0 || w >= ncx || h >= ncy) {
-            continue;
+  {
+    const int chunks = std::max(1, std::min(threads, N / 8));
+    RunChunks(chunks, [&](int k) {
+      std::vector<int> cell_stamp(static_cast<size_t>(a.num_cells), -1);
+      std::vector<int> cells;
+      std::vector<uint64_t> occ, row, dil;
+      for (int P = static_cast<int>(Bound(N, chunks, k)); P < Bound(N, chunks, k + 1); ++P) {
+        const int pm = margin_of(P);
+        const double c = std::cos(poses[3 * P + 2]);
+        const double sn = std::sin(poses[3 * P + 2]);
+        cells.clear();
+        for (int64_t q = a.pr_ptr[static_cast<size_t>(P)]; q < a.pr_ptr[static_cast<size_t>(P) + 1]; ++q) {
+          const int32_t s = a.pr_row[static_cast<size_t>(q)];
+          const double px = v.point_x[s];
+          const double py = v.point_y[s];
+          const double gx = c * px + (-sn) * py + poses[3 * P];
+          const double gy = sn * px + c * py + poses[3 * P + 1];
+          const double fx = (gx - v.grid.min_x) / v.grid.dx;
+          const double fy = (gy - v.grid.min_y) / v.grid.dy;
+          if (!(fx >= -1.0 && fy >= -1.0 && fx <= a.nx && fy <= a.ny)) {
+            continue;  // well outside the domain: no Jacobian row
           }
-          const int d = h * ncx + w;
-          if (dil_stamp[static_cast<size_t>(d)] == P) {
-            continue;
+          const int w = std::clamp(static_cast<int>(std::floor(fx)), 0, ncx - 1);
+          const int h = std::clamp(static_cast<int>(std::floor(fy)), 0, ncy - 1);
+          const int cell = h * ncx + w;
+          if (cell_stamp[static_cast<size_t>(cell)] != P) {
+            cell_stamp[static_cast<size_t>(cell)] = P;
+            cells.push_back(cell);
           }
-          dil_stamp[static_cast<size_t>(d)] = P;
-          bits[d >> 5] |= 1U << (d & 31);
-          dilated[static_cast<size_t>(P)].push_back(d);
-          if (P == 0) {
-            continue;  // frame 0 is fixed: no pose columns
+        }
+        uint32_t* bits = a.fp_cellbits.data() + static_cast<size_t>(P) * static_cast<size_t>(a.words_per_pose);
+        std::vector<int32_t>& fp = footprint[static_cast<size_t>(P)];
+        std::vector<int32_t>& dl = dilated[static_cast<size_t>(P)];
+        if (cells.empty()) continue;
+        // Occupied cells as row bitmaps over the dilated bounding box, then
+        // dilated by pm cells along w (word shifts) and along h (row ORs).
+        int cw0 = ncx, cw1 = -1, ch0 = ncy, ch1 = -1;
+        for (const int cell : cells) {
+          cw0 = std::min(cw0, cell % ncx);
+          cw1 = std::max(cw1, cell % ncx);
+          ch0 = std::min(ch0, cell / ncx);
+          ch1 = std::max(ch1, cell / ncx);
+        }
+        const int W0 = std::max(cw0 - pm, 0);
+        const int W1 = std::min(cw1 + pm, ncx - 1);
+        const int H0 = std::max(ch0 - pm, 0);
+        const int H1 = std::min(ch1 + pm, ncy - 1);
+        const int lw = W1 - W0 + 2;  // one spare column for the node corners
+        const int lh = H1 - H0 + 2;  // one spare row for the node corners
+        const int nw = (lw + 63) / 64;
+        occ.assign(static_cast<size_t>(lh) * nw, 0ULL);
+        row.assign(static_cast<size_t>(lh) * nw, 0ULL);
+        for (const int cell : cells) {
+          const int w = cell % ncx - W0;
+          const int h = cell / ncx - H0;
+          occ[static_cast<size_t>(h) * nw + (w >> 6)] |= 1ULL << (w & 63);
+        }
+        for (int h = 0; h < lh - 1; ++h) {
+          const uint64_t* x = occ.data() + static_cast<size_t>(h) * nw;
+          uint64_t* y = row.data() + static_cast<size_t>(h) * nw;
+          for (int i = 0; i < nw; ++i) y[i] = x[i];
+          for (int k = 1; k <= pm; ++k) {
+            for (int i = 0; i < nw; ++i) {
+              const uint64_t up = (x[i] << k) | (i > 0 ? x[i - 1] >> (64 - k) : 0ULL);
+              const uint64_t dn = (x[i] >> k) | (i + 1 < nw ? x[i + 1] << (64 - k) : 0ULL);
+              y[i] |= up | dn;
+            }
           }
-          const int corner[4] = {h * a.nx + w, h * a.nx + w + 1, (h + 1) * a.nx + w,
-                                 (h + 1) * a.nx + w + 1};
-          for (const int g : corner) {
-            if (v.node_to_active[g] >= 0 && node_stamp[static_cast<size_t>(g)] != P) {
-              node_stamp[static_cast<size_t>(g)] = P;
-              fp.push_back(g);
+          // Cell columns are local 0 .. lw - 2: clear what the shifts moved
+          // past the grid's last cell column (and the spare column).
+          for (int i = 0; i < nw; ++i) {
+            const int valid = lw - 1 - 64 * i;
+            if (valid < 64) y[i] &= valid <= 0 ? 0ULL : (1ULL << valid) - 1ULL;
+          }
+        }
+        dil.assign(static_cast<size_t>(lh) * nw, 0ULL);
+        for (int h = 0; h < lh - 1; ++h) {
+          uint64_t* y = dil.data() + static_cast<size_t>(h) * nw;
+          for (int dh = -pm; dh <= pm; ++dh) {
+            const int hh = h + dh;
+            if (hh < 0 || hh >= lh - 1) continue;
+            const uint64_t* x = row.data() + static_cast<size_t>(hh) * nw;
+            for (int i = 0; i < nw; ++i) y[i] |= x[i];
+          }
+          for (int i = 0; i < nw; ++i) {
+            uint64_t m = y[i];
+            while (m != 0ULL) {
+              const int b = __builtin_ctzll(m);
+              m &= m - 1;
+              const int d = (H0 + h) * ncx + W0 + 64 * i + b;
+              bits[d >> 5] |= 1U << (d & 31);
+              dl.push_back(d);
             }
           }
         }
+        if (P == 0) continue;  // frame 0 is fixed: no pose columns
+        // Nodes of the dilated cells: node (w, h) is a corner of cells
+        // (w-1..w, h-1..h); node rows are ascending, so fp comes out sorted.
+        int bw0 = a.nx, bw1 = -1, bh0 = a.ny, bh1 = -1;
+        for (int h = 0; h < lh; ++h) {
+          const uint64_t* y0 = dil.data() + static_cast<size_t>(h) * nw;
+          const uint64_t* y1 = h > 0 ? dil.data() + static_cast<size_t>(h - 1) * nw : nullptr;
+          for (int i = 0; i < nw; ++i) {
+            const uint64_t c0 = y0[i] | (y1 != nullptr ? y1[i] : 0ULL);
+            const uint64_t cm = i > 0 ? (y0[i - 1] | (y1 != nullptr ? y1[i - 1] : 0ULL)) : 0ULL;
+            uint64_t m = c0 | (c0 << 1) | (cm >> 63);
+            while (m != 0ULL) {
+              const int b = __builtin_ctzll(m);
+              m &= m - 1;
+              const int w = W0 + 64 * i + b;
+              const int hg = H0 + h;
+              if (w >= a.nx || hg >= a.ny) continue;
+              const int g = hg * a.nx + w;
+              if (v.node_to_active[g] < 0) continue;
+              fp.push_back(g);
+              bw0 = std::min(bw0, w);
+              bw1 = std::max(bw1, w);
+              bh0 = std::min(bh0, hg);
+              bh1 = std::max(bh1, hg);
+            }
+          }
+        }
+        if (!fp.empty()) {
+          fp_bbox[4 * static_cast<size_t>(P)] = bw0;
+          fp_bbox[4 * static_cast<size_t>(P) + 1] = bw1;
+          fp_bbox[4 * static_cast<size_t>(P) + 2] = bh0;
+          fp_bbox[4 * static_cast<size_t>(P) + 3] = bh1;
+        }
       }
-    }
-    std::sort(fp.begin(), fp.end());
-    if (!fp.empty()) {
-      int bw0 = a.nx, bw1 = -1, bh0 = a.ny, bh1 = -1;
-      for (const int g : fp) {
-        bw0 = std::min(bw0, g % a.nx);
-        bw1 = std::max(bw1, g % a.nx);
-        bh0 = std::min(bh0, g / a.nx);
-        bh1 = std::max(bh1, g / a.nx);
-      }
-      fp_bbox[4 * static_cast<size_t>(P)] = bw0;
-      fp_bbox[4 * static_cast<size_t>(P) + 1] = bw1;
-      fp_bbox[4 * static_cast<size_t>(P) + 2] = bh0;
-      fp_bbox[4 * static_cast<size_t>(P) + 3] = bh1;
-    }
+    });
   }
   // Candidate frames per cell: every frame whose dilated cells contain it
   // (ascending), for the per-cell gather of the point-row partial sums.
-  a.cp_ptr.assign(static_cast<size_t>(a.num_cells) + 1, 0);
-  for (int P = 0; P < N; ++P) {
-    for (const int d : dilated[static_cast<size_t>(P)]) ++a.cp_ptr[static_cast<size_t>(d) + 1];
-  }
-  for (int d = 0; d < a.num_cells; ++d) a.cp_ptr[static_cast<size_t>(d) + 1] += a.cp_ptr[static_cast<size_t>(d)];
-  a.cp_pose.resize(static_cast<size_t>(a.cp_ptr[static_cast<size_t>(a.num_cells)]));
-  {
-    std::vector<int64_t> fill(a.cp_ptr.begin(), a.cp_ptr.end() - 1);
-    for (int P = 0; P < N; ++P) {
-      for (const int d : dilated[static_cast<size_t>(P)]) {
-        a.cp_pose[static_cast<size_t>(fill[static_cast<size_t>(d)]++)] = P;
-      }
-    }
-  }
-  // Inverse footprint: poses per node (ascending).
-  std::vector<int64_t> np_ptr(static_cast<size_t>(num_nodes) + 1, 0);
-  for (int P = 1; P < N; ++P) {
-    for (const int g : footprint[static_cast<size_t>(P)]) {
-      ++np_ptr[static_cast<size_t>(g) + 1];
-    }
-  }
-  for (int g = 0; g < num_nodes; ++g) {
-    np_ptr[static_cast<size_t>(g) + 1] += np_ptr[static_cast<size_t>(g)];
-  }
-  std::vector<int32_t> np_pose(static_cast<size_t>(np_ptr[static_cast<size_t>(num_nodes)]));
-  {
-    std::vector<int64_t> fill(np_ptr.begin(), np_ptr.end() - 1);
-    for (int P = 1; P < N; ++P) {
-      for (const int g : footprint[static_cast<size_t>(P)]) {
-        np_pose[static_cast<size_t>(fill[static_cast<size_t>(g)]++)] = P;
-      }
-    }
-  }
-
+  InvertLists(dilated, 0, a.num_cells, threads, a.cp_ptr, a.cp_pose);
+  // Inverse footprint: poses >= 1 per node (ascending).
+  std::vector<int64_t> np_ptr;
+  std::vector<int32_t> np_pose;
+  InvertLists(footprint, 1, num_nodes, threads, np_ptr, np_pose);
   // Odometry/relation specs per pose and distinct pose pairs.
   a.po_ptr.assign(static_cast<size_t>(N) + 1, 0);
   for (int64_t o = 0; o < v.num_odom; ++o) {
@@ -243,7 +367,7 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   std::vector<NdNode> tree;
   std::function<int(int, int, int, int)> build = [&](int w0, int w1, int h0, int h1) -> int {
     const int idx = static_cast<int>(tree.size());
-    tree.push_back(NdNode{w0, w1, h0, h1});
+    tree.push_back(NdNode{w0, w1, h0, h1, -1, -1, -1, 0, {}});
     const int ww = w1 - w0;
     const int hh = h1 - h0;
     if (ww <= 0 || hh <= 0 || ww * hh <= opt.leaf_nodes || (ww < 3 && hh < 3)) {
@@ -338,7 +462,7 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   }
 
   // Neighbours of an original column in the A pattern (new indices).
-  auto neighbors = [&](int col, const std::function<void(int)>& fn) {
+  auto neighbors = [&](int col, auto&& fn) {
     if (col < a.num_active) {
       const int g = active_to_node[static_cast<size_t>(col)];
       const int w = g % a.nx;
@@ -393,7 +517,11 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
       }
     };
     for (int j = first; j <= last; ++j) {
-      neighbors(a.iperm[static_cast<size_t>(j)], add);
+      const int col = a.iperm[static_cast<size_t>(j)];
+      // The three columns of a pose share one supernode and one neighbour
+      // set beyond it: visit the first only.
+      if (col >= a.num_active && (col - a.num_active) % 3 != 0) continue;
+      neighbors(col, add);
     }
     for (const int c : a.children[static_cast<size_t>(s)]) {
       for (const int i : R[static_cast<size_t>(c)]) add(i);
@@ -446,9 +574,10 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     a.rel_off[static_cast<size_t>(s) + 1] = a.rel_off[static_cast<size_t>(s)] + a.sn_m[static_cast<size_t>(s)];
   }
   a.rel.resize(static_cast<size_t>(a.rel_off[static_cast<size_t>(a.num_sn)]));
-  for (int c = 0; c < a.num_sn; ++c) {
+  ParallelFor(a.num_sn, threads, 32, [&](int64_t ci) {
+    const int c = static_cast<int>(ci);
     const int p = a.sn_parent[static_cast<size_t>(c)];
-    if (p < 0) continue;
+    if (p < 0) return;
     const int32_t* pb = a.sn_idx.data() + a.sn_idx_ptr[static_cast<size_t>(p)];
     const int32_t* pe = a.sn_idx.data() + a.sn_idx_ptr[static_cast<size_t>(p) + 1];
     const int pn = a.sn_first[static_cast<size_t>(p) + 1] - a.sn_first[static_cast<size_t>(p)];
@@ -464,15 +593,16 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
       if (pos < pn) ++split;
     }
     a.rel_split[static_cast<size_t>(c)] = split;
-  }
+  });
   // Node-node slots and constant eikonal contributions.
   a.nn_off.assign(5 * static_cast<size_t>(num_nodes), -1);
   a.nn_eik.assign(5 * static_cast<size_t>(num_nodes), 0.0);
   const int dws[5] = {0, 1, -1, 0, 1};
   const int dhs[5] = {0, 0, 1, 1, 1};
-  for (int g = 0; g < num_nodes; ++g) {
+  ParallelFor(num_nodes, threads, 1024, [&](int64_t gi) {
+    const int g = static_cast<int>(gi);
     const int cu = v.node_to_active[g];
-    if (cu < 0) continue;
+    if (cu < 0) return;
     const int w = g % a.nx;
     const int h = g / a.nx;
     for (int k = 0; k < 5; ++k) {
@@ -487,7 +617,7 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
       if (off < 0) throw std::logic_error("node-node slot missing from the factor");
       a.nn_off[5 * static_cast<size_t>(g) + k] = off;
     }
-  }
+  });
   a.eik_at.assign(static_cast<size_t>(num_nodes), -1);
   for (int64_t e = 0; e < v.num_eikonal; ++e) {
     const int w = v.eik_w[e];
@@ -527,31 +657,70 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   a.fp_node.resize(static_cast<size_t>(a.fp_ptr[static_cast<size_t>(N)]));
   a.fp_off.resize(a.fp_node.size());
   a.fp_stride.resize(a.fp_node.size());
-  for (int P = 1; P < N; ++P) {
-    const int p0 = a.perm[static_cast<size_t>(a.num_active + 3 * (P - 1))];
-    int64_t q = a.fp_ptr[static_cast<size_t>(P)];
-    for (const int g : footprint[static_cast<size_t>(P)]) {
-      const int pu = a.perm[static_cast<size_t>(v.node_to_active[g])];
-      int64_t off;
-      int stride;
-      if (pu < p0) {
-        off = PanelOffset(a, p0, pu);
-        stride = 1;
-        if (PanelOffset(a, p0 + 2, pu) != off + 2) throw std::logic_error("pose rows not adjacent");
-      } else {
-        off = PanelOffset(a, pu, p0);
-        const int s = a.snode_of[static_cast<size_t>(p0)];
-        stride = a.sn_f[static_cast<size_t>(s)];
-        if (PanelOffset(a, pu, p0 + 2) != off + 2 * static_cast<int64_t>(stride)) {
-          throw std::logic_error("pose columns not adjacent");
+  {
+    // Offsets without a search per entry: the position of the pose's rows
+    // inside a node supernode's front is looked up once per (pose,
+    // supernode), and the pose supernode's front positions sit in a dense
+    // per-thread map, rebuilt when the supernode changes.
+    const int chunks = std::max(1, std::min(threads, (N - 1) / 8));
+    RunChunks(chunks, [&](int k) {
+      std::vector<int32_t> pos_in(static_cast<size_t>(a.n), -1);
+      int cur_sp = -1;
+      constexpr int kMemo = 256;
+      int memo_s[kMemo];
+      int32_t memo_pos[kMemo];
+      for (int P = 1 + static_cast<int>(Bound(N - 1, chunks, k)); P < 1 + Bound(N - 1, chunks, k + 1); ++P) {
+        const int p0 = a.perm[static_cast<size_t>(a.num_active + 3 * (P - 1))];
+        const int sp = a.snode_of[static_cast<size_t>(p0)];
+        const int32_t* sp_idx = a.sn_idx.data() + a.sn_idx_ptr[static_cast<size_t>(sp)];
+        const int sp_f = a.sn_f[static_cast<size_t>(sp)];
+        if (sp != cur_sp) {
+          if (cur_sp >= 0) {
+            const int32_t* old = a.sn_idx.data() + a.sn_idx_ptr[static_cast<size_t>(cur_sp)];
+            for (int r = 0; r < a.sn_f[static_cast<size_t>(cur_sp)]; ++r) pos_in[static_cast<size_t>(old[r])] = -1;
+          }
+          for (int r = 0; r < sp_f; ++r) pos_in[static_cast<size_t>(sp_idx[r])] = r;
+          cur_sp = sp;
+        }
+        std::fill(memo_s, memo_s + kMemo, -1);
+        const int64_t p_col = a.panel_off[static_cast<size_t>(sp)] +
+                              static_cast<int64_t>(p0 - a.sn_first[static_cast<size_t>(sp)]) * sp_f;
+        int64_t q = a.fp_ptr[static_cast<size_t>(P)];
+        for (const int g : footprint[static_cast<size_t>(P)]) {
+          const int pu = a.perm[static_cast<size_t>(v.node_to_active[g])];
+          int64_t off;
+          int stride;
+          if (pu < p0) {
+            // Rows p0..p0+2 of column pu, in the node's supernode.
+            const int su = a.snode_of[static_cast<size_t>(pu)];
+            const int slot = su & (kMemo - 1);
+            const int32_t* su_idx = a.sn_idx.data() + a.sn_idx_ptr[static_cast<size_t>(su)];
+            const int su_f = a.sn_f[static_cast<size_t>(su)];
+            if (memo_s[slot] != su) {
+              const int32_t* it = std::lower_bound(su_idx, su_idx + su_f, p0);
+              memo_s[slot] = su;
+              memo_pos[slot] = (it == su_idx + su_f || *it != p0) ? -1 : static_cast<int32_t>(it - su_idx);
+            }
+            const int32_t r = memo_pos[slot];
+            if (r < 0) throw std::logic_error("footprint slot missing from the factor");
+            if (r + 2 >= su_f || su_idx[r + 2] != p0 + 2) throw std::logic_error("pose rows not adjacent");
+            off = a.panel_off[static_cast<size_t>(su)] +
+                  static_cast<int64_t>(pu - a.sn_first[static_cast<size_t>(su)]) * su_f + r;
+            stride = 1;
+          } else {
+            // Row pu of columns p0..p0+2, in the pose's supernode.
+            const int32_t r = pos_in[static_cast<size_t>(pu)];
+            if (r < 0) throw std::logic_error("footprint slot missing from the factor");
+            off = p_col + r;
+            stride = sp_f;
+          }
+          a.fp_node[static_cast<size_t>(q)] = g;
+          a.fp_off[static_cast<size_t>(q)] = off;
+          a.fp_stride[static_cast<size_t>(q)] = stride;
+          ++q;
         }
       }
-      if (off < 0) throw std::logic_error("footprint slot missing from the factor");
-      a.fp_node[static_cast<size_t>(q)] = g;
-      a.fp_off[static_cast<size_t>(q)] = off;
-      a.fp_stride[static_cast<size_t>(q)] = stride;
-      ++q;
-    }
+    });
   }
   // Pose diagonal blocks.
   a.pd_off.assign(6 * static_cast<size_t>(N), -1);

@@ -77,6 +77,103 @@ class DeviceArena {
   size_t bytes_{0};
 };
 
+/// Device memory of one layout (the problem, or one pattern epoch). One
+/// allocation holds every array and is kept while later layouts fit (it
+/// grows with 25 % slack);
+/// host arrays are staged in pinned memory and uploaded with one copy per
+/// contiguous run. The layout is laid out twice by the same code: once to
+/// measure (Upload/Alloc return null), once to place.
+class EpochPool {
+ public:
+  EpochPool() = default;
+  EpochPool(const EpochPool&) = delete;
+  EpochPool& operator=(const EpochPool&) = delete;
+  ~EpochPool() { Release(); }
+
+  void BeginMeasure() {
+    measuring_ = true;
+    off_ = 0;
+    upload_end_ = 0;
+  }
+  /// Ends measuring and makes room for what was measured.
+  void BeginPlace() {
+    const size_t need = off_;
+    const size_t staged = upload_end_;
+    if (need > cap_) {
+      if (dev_ != nullptr) HIPC(hipFree(dev_));
+      dev_ = nullptr;
+      cap_ = 0;
+      const size_t want = need + need / 4;
+      HIPC(hipMalloc(&dev_, want));
+      cap_ = want;
+    }
+    if (staged > host_cap_) {
+      if (host_ != nullptr) HIPC(hipHostFree(host_));
+      host_ = nullptr;
+      host_cap_ = 0;
+      const size_t want = staged + staged / 4;
+      HIPC(hipHostMalloc(&host_, want, hipHostMallocDefault));
+      host_cap_ = want;
+    }
+    measuring_ = false;
+    off_ = 0;
+    runs_.clear();
+  }
+  /// Copies the staged arrays to the device and waits for the copies.
+  void Finish(hipStream_t s) {
+    for (const auto& r : runs_) {
+      HIPC(hipMemcpyAsync(static_cast<char*>(dev_) + r.first, static_cast<char*>(host_) + r.first,
+                          r.second - r.first, hipMemcpyHostToDevice, s));
+    }
+    HIPC(hipStreamSynchronize(s));
+  }
+  template <typename T>
+  T* Upload(const T* data, size_t n) {
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    const size_t at = Take(bytes);
+    if (measuring_) {
+      upload_end_ = std::max(upload_end_, at + bytes);
+      return nullptr;
+    }
+    if (n > 0) std::memcpy(static_cast<char*>(host_) + at, data, n * sizeof(T));
+    if (!runs_.empty() && runs_.back().second == at) {
+      runs_.back().second = at + bytes;
+    } else {
+      runs_.push_back({at, at + bytes});
+    }
+    return reinterpret_cast<T*>(static_cast<char*>(dev_) + at);
+  }
+  template <typename T>
+  T* Upload(const std::vector<T>& v) {
+    return Upload(v.data(), v.size());
+  }
+  template <typename T>
+  T* Alloc(size_t n) {
+    const size_t at = Take(std::max<size_t>(n, 1) * sizeof(T));
+    return measuring_ ? nullptr : reinterpret_cast<T*>(static_cast<char*>(dev_) + at);
+  }
+  size_t bytes() const { return cap_; }
+  void Release() {
+    if (dev_ != nullptr) (void)hipFree(dev_);
+    if (host_ != nullptr) (void)hipHostFree(host_);
+    dev_ = host_ = nullptr;
+    cap_ = host_cap_ = 0;
+  }
+
+ private:
+  size_t Take(size_t bytes) {
+    const size_t at = off_;
+    off_ += (bytes + 255) & ~static_cast<size_t>(255);
+    return at;
+  }
+  bool measuring_{true};
+  size_t off_{0}, upload_end_{0};
+  void* dev_{nullptr};
+  void* host_{nullptr};
+  size_t cap_{0}, host_cap_{0};
+  std::vector<std::pair<size_t, size_t>> runs_;
+};
+
 template <typename T>
 std::vector<T> CopyArray(const T* p, int64_t n) {
   if (n <= 0) return {};
@@ -181,8 +278,11 @@ struct sdfslam_hip_context {
   AnalysisOptions aopt;
   Analysis an;
   int pattern_epochs{0};
-  DeviceArena base;
-  DeviceArena epoch;
+  bool short_epoch{false};  // the ending epoch lasted at most two iterations
+  double analysis_seconds{0.0};  // host time spent building pattern epochs
+  EpochPool base;   // problem specs, state and per-row work arrays
+  EpochPool epoch;  // symbolic analysis and factor storage
+  DeviceArena hist;  // cost history
   DeviceProblem dp{};
   DeviceAnalysis da{};
   DeviceWork dw{};
@@ -209,8 +309,9 @@ struct sdfslam_hip_context {
     for (hipEvent_t& e : events) {
       if (e != nullptr) (void)hipEventDestroy(e);
     }
-    epoch.Clear();
-    base.Clear();
+    epoch.Release();
+    base.Release();
+    hist.Clear();
     if (stream != nullptr) (void)hipStreamDestroy(stream);
   }
 };
@@ -245,61 +346,74 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   for (size_t g = 0; g < hp.node_to_active.size(); ++g) {
     if (hp.node_to_active[g] >= 0) active_to_node[static_cast<size_t>(hp.node_to_active[g])] = static_cast<int32_t>(g);
   }
-  DeviceArena& A = c.base;
-  P.node_to_active = A.Upload(hp.node_to_active);
-  P.active_to_node = A.Upload(active_to_node);
-  P.pt_frame = A.Upload(hp.pt_frame);
-  P.pt_x = A.Upload(hp.pt_x);
-  P.pt_y = A.Upload(hp.pt_y);
-  P.pt_delta = A.Upload(hp.pt_delta);
-  P.pt_sw = A.Upload(hp.pt_sw);
-  P.eik_w = A.Upload(hp.eik_w);
-  P.eik_h = A.Upload(hp.eik_h);
-  P.eik_nx = A.Upload(hp.eik_nx);
-  P.eik_ny = A.Upload(hp.eik_ny);
-  P.eik_sw = A.Upload(hp.eik_sw);
-  P.od_i = A.Upload(hp.od_i);
-  P.od_j = A.Upload(hp.od_j);
-  P.od_dx = A.Upload(hp.od_dx);
-  P.od_dy = A.Upload(hp.od_dy);
-  P.od_dth = A.Upload(hp.od_dth);
-  P.od_sw = A.Upload(hp.od_sw);
   P.huber_delta = v.huber_delta;
   P.smooth_gradient = v.smooth_gradient;
   P.smooth_gradient_step = v.smooth_gradient_step;
-
+  EpochPool& A = c.base;
   DeviceWork& W = c.dw;
   const size_t nmap = static_cast<size_t>(v.grid.nx) * v.grid.ny;
   const size_t npose = 3 * static_cast<size_t>(v.num_poses);
-  W.map = A.Upload(map_values, nmap);
-  W.poses = A.Upload(poses, npose);
-  W.map_backup = A.Alloc<double>(nmap);
-  W.poses_backup = A.Alloc<double>(npose);
   const auto np = static_cast<size_t>(v.num_points);
-  W.r_pt = A.Alloc<double>(np);
-  W.jw = A.Alloc<double>(4 * np);
-  W.jp = A.Alloc<double>(3 * np);
-  W.cellkey = A.Alloc<uint32_t>(np);
-  W.r_eik = A.Alloc<double>(static_cast<size_t>(v.num_eikonal));
-  c.j_eik = A.Alloc<double>(3 * static_cast<size_t>(v.num_eikonal));
-  W.od_r = A.Alloc<double>(3 * static_cast<size_t>(v.num_odom));
-  W.od_ji = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
-  W.od_jj = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
-  W.seg_cell = A.Alloc<int32_t>(np);
-  W.seg_part = A.Alloc<double>(14 * np);
-  W.seg_count = A.Alloc<int32_t>(static_cast<size_t>(v.num_poses));
-  W.nn = A.Alloc<double>(10 * static_cast<size_t>(P.num_cells));
-  W.rn = A.Alloc<double>(4 * static_cast<size_t>(P.num_cells));
-  W.rhs = A.Alloc<double>(static_cast<size_t>(P.n));
-  W.step = A.Alloc<double>(static_cast<size_t>(P.n));
-  W.xnew = A.Alloc<double>(static_cast<size_t>(P.n));
-  W.partials = A.Alloc<double>(3 * static_cast<size_t>(kPartialBlocks));
-  c.history_capacity = std::max(c.opts.max_iterations, 0) + 1;
-  W.cost_history = A.Alloc<double>(static_cast<size_t>(c.history_capacity));
-  W.state = A.Alloc<LmState>(1);
-  c.scalar = A.Alloc<double>(1);
+  auto layout = [&]() {
+    P.node_to_active = A.Upload(hp.node_to_active);
+    P.active_to_node = A.Upload(active_to_node);
+    P.pt_frame = A.Upload(hp.pt_frame);
+    P.pt_x = A.Upload(hp.pt_x);
+    P.pt_y = A.Upload(hp.pt_y);
+    P.pt_delta = A.Upload(hp.pt_delta);
+    P.pt_sw = A.Upload(hp.pt_sw);
+    P.eik_w = A.Upload(hp.eik_w);
+    P.eik_h = A.Upload(hp.eik_h);
+    P.eik_nx = A.Upload(hp.eik_nx);
+    P.eik_ny = A.Upload(hp.eik_ny);
+    P.eik_sw = A.Upload(hp.eik_sw);
+    P.od_i = A.Upload(hp.od_i);
+    P.od_j = A.Upload(hp.od_j);
+    P.od_dx = A.Upload(hp.od_dx);
+    P.od_dy = A.Upload(hp.od_dy);
+    P.od_dth = A.Upload(hp.od_dth);
+    P.od_sw = A.Upload(hp.od_sw);
+    W.map = A.Upload(map_values, nmap);
+    W.poses = A.Upload(poses, npose);
+    W.map_backup = A.Alloc<double>(nmap);
+    W.poses_backup = A.Alloc<double>(npose);
+    W.r_pt = A.Alloc<double>(np);
+    W.jw = A.Alloc<double>(4 * np);
+    W.jp = A.Alloc<double>(3 * np);
+    W.cellkey = A.Alloc<uint32_t>(np);
+    W.r_eik = A.Alloc<double>(static_cast<size_t>(v.num_eikonal));
+    c.j_eik = A.Alloc<double>(3 * static_cast<size_t>(v.num_eikonal));
+    W.od_r = A.Alloc<double>(3 * static_cast<size_t>(v.num_odom));
+    W.od_ji = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
+    W.od_jj = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
+    W.seg_cell = A.Alloc<int32_t>(np);
+    W.seg_part = A.Alloc<double>(14 * np);
+    W.seg_count = A.Alloc<int32_t>(static_cast<size_t>(v.num_poses));
+    W.nn = A.Alloc<double>(10 * static_cast<size_t>(P.num_cells));
+    W.rn = A.Alloc<double>(4 * static_cast<size_t>(P.num_cells));
+    W.rhs = A.Alloc<double>(static_cast<size_t>(P.n));
+    W.step = A.Alloc<double>(static_cast<size_t>(P.n));
+    W.xnew = A.Alloc<double>(static_cast<size_t>(P.n));
+    W.partials = A.Alloc<double>(3 * static_cast<size_t>(kPartialBlocks));
+    W.state = A.Alloc<LmState>(1);
+    W.pose_overflow = A.Alloc<int32_t>(static_cast<size_t>(v.num_poses));
+    c.scalar = A.Alloc<double>(1);
+  };
+  A.BeginMeasure();
+  layout();
+  A.BeginPlace();
+  layout();
+  A.Finish(c.stream);
+  HIPC(hipMemset(W.pose_overflow, 0, sizeof(int32_t) * static_cast<size_t>(v.num_poses)));
   LmState zero{};
   HIPC(hipMemcpy(W.state, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  const int want = std::max(c.opts.max_iterations, 0) + 1;
+  if (W.cost_history == nullptr || want > c.history_capacity) {
+    c.hist.Clear();
+    c.history_capacity = std::max(want, c.history_capacity);
+    W.cost_history = c.hist.Alloc<double>(static_cast<size_t>(c.history_capacity));
+  }
+  ++c.generation;
 }
 
 void ReadState(sdfslam_hip_context& c, LmState& st) {
@@ -321,63 +435,54 @@ void DownloadState(sdfslam_hip_context& c, std::vector<double>& map, std::vector
   HIPC(hipStreamSynchronize(c.stream));
 }
 
+/// Widens the footprint dilation of every pose whose points left their
+/// footprint in the ending epoch: by one cell, by two when that epoch lasted
+/// at most two iterations, up to kMaxMargin. Poses that stay put keep the
+/// initial margin, so the pattern grows only where the state moves.
+constexpr int kMaxMargin = 8;
+
+void GrowMargins(sdfslam_hip_context& c) {
+  const int N = c.hp->view.num_poses;
+  if (static_cast<int>(c.aopt.pose_margin.size()) != N) {
+    c.aopt.pose_margin.assign(static_cast<size_t>(N), c.aopt.margin);
+  }
+  std::vector<int32_t> flags(static_cast<size_t>(N));
+  HIPC(hipMemcpy(flags.data(), c.dw.pose_overflow, sizeof(int32_t) * flags.size(), hipMemcpyDeviceToHost));
+  const int step = c.short_epoch ? 2 : 1;
+  for (int P = 0; P < N; ++P) {
+    if (flags[static_cast<size_t>(P)] != 0) {
+      int32_t& m = c.aopt.pose_margin[static_cast<size_t>(P)];
+      m = std::min(m + step, kMaxMargin);
+    }
+  }
+  HIPC(hipMemset(c.dw.pose_overflow, 0, sizeof(int32_t) * flags.size()));
+}
+
 /// Symbolic analysis at the current device state and the device buffers
 /// that depend on it (a new pattern epoch).
 void BuildEpoch(sdfslam_hip_context& c) {
   std::vector<double> map;
   std::vector<double> poses;
+  const auto t0 = std::chrono::steady_clock::now();
   DownloadState(c, map, poses);
-  c.epoch.Clear();
+  GrowMargins(c);
   c.an = Analyze(c.hp->view, map.data(), poses.data(), c.aopt);
   ++c.pattern_epochs;
   const Analysis& an = c.an;
   if (an.max_points_per_pose > 4096) {
     throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "more than 4096 point rows in one frame");
   }
-  DeviceArena& E = c.epoch;
-  DeviceAnalysis& D = c.da;
-  D.perm = E.Upload(an.perm);
-  D.iperm = E.Upload(an.iperm);
-  D.sn_first = E.Upload(an.sn_first);
-  D.sn_m = E.Upload(an.sn_m);
-  D.sn_f = E.Upload(an.sn_f);
-  D.sn_idx_ptr = E.Upload(an.sn_idx_ptr);
-  D.sn_idx = E.Upload(an.sn_idx);
-  D.panel_off = E.Upload(an.panel_off);
-  D.upd_off = E.Upload(an.upd_off);
-  D.sn_parent = E.Upload(an.sn_parent);
-  D.rel_off = E.Upload(an.rel_off);
-  D.rel = E.Upload(an.rel);
-  D.rel_split = E.Upload(an.rel_split);
-  D.nn_off = E.Upload(an.nn_off);
-  D.nn_eik = E.Upload(an.nn_eik);
-  D.eik_at = E.Upload(an.eik_at);
-  D.fp_ptr = E.Upload(an.fp_ptr);
-  D.fp_node = E.Upload(an.fp_node);
-  D.fp_off = E.Upload(an.fp_off);
-  D.fp_stride = E.Upload(an.fp_stride);
-  D.words_per_pose = an.words_per_pose;
-  D.fp_cellbits = E.Upload(an.fp_cellbits);
-  D.pr_ptr = E.Upload(an.pr_ptr);
-  D.pr_row = E.Upload(an.pr_row);
-  D.cp_ptr = E.Upload(an.cp_ptr);
-  D.cp_pose = E.Upload(an.cp_pose);
-  D.pd_off = E.Upload(an.pd_off);
-  D.po_ptr = E.Upload(an.po_ptr);
-  D.po_spec = E.Upload(an.po_spec);
-  D.num_pairs = static_cast<int>(an.op_pose.size() / 2);
-  D.op_pose = E.Upload(an.op_pose);
-  D.op_ptr = E.Upload(an.op_ptr);
-  D.op_spec = E.Upload(an.op_spec);
-  D.op_off = E.Upload(an.op_off);
+  c.max_f = 0;
+  for (const int f : an.sn_f) c.max_f = std::max(c.max_f, f);
+  if (static_cast<size_t>(c.max_f) * sizeof(double) > 150 * 1024) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "front larger than the solve kernels' LDS budget");
+  }
   std::vector<int64_t> cptr(static_cast<size_t>(an.num_sn) + 1, 0);
   std::vector<int32_t> clist;
   for (int s = 0; s < an.num_sn; ++s) {
     for (const int ch : an.children[static_cast<size_t>(s)]) clist.push_back(ch);
     cptr[static_cast<size_t>(s) + 1] = static_cast<int64_t>(clist.size());
   }
-  D.children_ptr = E.Upload(cptr);
-  D.children = E.Upload(clist);
   // Level lists and, per level and child rank, the children to extend-add.
   std::vector<int32_t> lvl;
   c.level_off.assign(static_cast<size_t>(an.num_levels) + 1, 0);
@@ -400,18 +505,59 @@ void BuildEpoch(sdfslam_hip_context& c) {
       c.level_children[static_cast<size_t>(L)].push_back(cl);
     }
   }
-  c.level_sn = E.Upload(lvl);
-  c.child_lists = E.Upload(ranks);
-  c.max_f = 0;
-  for (const int f : an.sn_f) c.max_f = std::max(c.max_f, f);
-  if (static_cast<size_t>(c.max_f) * sizeof(double) > 150 * 1024) {
-    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "front larger than the solve kernels' LDS budget");
-  }
-  c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
-  c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
-  c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 64 * 64);
-  c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
+  EpochPool& E = c.epoch;
+  DeviceAnalysis& D = c.da;
+  auto layout = [&]() {
+    D.perm = E.Upload(an.perm);
+    D.iperm = E.Upload(an.iperm);
+    D.sn_first = E.Upload(an.sn_first);
+    D.sn_m = E.Upload(an.sn_m);
+    D.sn_f = E.Upload(an.sn_f);
+    D.sn_idx_ptr = E.Upload(an.sn_idx_ptr);
+    D.sn_idx = E.Upload(an.sn_idx);
+    D.panel_off = E.Upload(an.panel_off);
+    D.upd_off = E.Upload(an.upd_off);
+    D.sn_parent = E.Upload(an.sn_parent);
+    D.rel_off = E.Upload(an.rel_off);
+    D.rel = E.Upload(an.rel);
+    D.rel_split = E.Upload(an.rel_split);
+    D.nn_off = E.Upload(an.nn_off);
+    D.nn_eik = E.Upload(an.nn_eik);
+    D.eik_at = E.Upload(an.eik_at);
+    D.fp_ptr = E.Upload(an.fp_ptr);
+    D.fp_node = E.Upload(an.fp_node);
+    D.fp_off = E.Upload(an.fp_off);
+    D.fp_stride = E.Upload(an.fp_stride);
+    D.words_per_pose = an.words_per_pose;
+    D.fp_cellbits = E.Upload(an.fp_cellbits);
+    D.pr_ptr = E.Upload(an.pr_ptr);
+    D.pr_row = E.Upload(an.pr_row);
+    D.cp_ptr = E.Upload(an.cp_ptr);
+    D.cp_pose = E.Upload(an.cp_pose);
+    D.pd_off = E.Upload(an.pd_off);
+    D.po_ptr = E.Upload(an.po_ptr);
+    D.po_spec = E.Upload(an.po_spec);
+    D.num_pairs = static_cast<int>(an.op_pose.size() / 2);
+    D.op_pose = E.Upload(an.op_pose);
+    D.op_ptr = E.Upload(an.op_ptr);
+    D.op_spec = E.Upload(an.op_spec);
+    D.op_off = E.Upload(an.op_off);
+    D.children_ptr = E.Upload(cptr);
+    D.children = E.Upload(clist);
+    c.level_sn = E.Upload(lvl);
+    c.child_lists = E.Upload(ranks);
+    c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
+    c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
+    c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 64 * 64);
+    c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
+  };
+  E.BeginMeasure();
+  layout();
+  E.BeginPlace();
+  layout();
+  E.Finish(c.stream);
   ++c.generation;
+  c.analysis_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 enum Phase { kAssembly = 0, kNormal = 1, kFactor = 2, kSolve = 3, kUpdate = 4 };
@@ -475,12 +621,18 @@ bool SameOptions(const LmOptions& a, const LmOptions& b) {
          a.trust_region == b.trust_region;
 }
 
+/// Runs shorter than this many iterations launch kernels directly unless a
+/// captured graph for the current buffers already exists.
+constexpr int kGraphMinIterations = 6;
+
+bool GraphValid(const sdfslam_hip_context& c, const LmOptions& o) {
+  return c.graph != nullptr && c.graph_generation == c.generation && SameOptions(c.graph_options, o);
+}
+
 /// One LM iteration captured as a HIP graph (re-captured when buffers or
 /// options change), so a batch of iterations is a batch of graph launches.
 hipGraphExec_t IterationGraph(sdfslam_hip_context& c, const LmOptions& o) {
-  if (c.graph != nullptr && c.graph_generation == c.generation && SameOptions(c.graph_options, o)) {
-    return c.graph;
-  }
+  if (GraphValid(c, o)) return c.graph;
   if (c.graph != nullptr) {
     HIPC(hipGraphExecDestroy(c.graph));
     c.graph = nullptr;
@@ -544,6 +696,9 @@ LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations
         HIPC(hipStreamSynchronize(c.stream));
         AccumulateTimes(c);
       }
+    } else if (target - st.iter < kGraphMinIterations && !GraphValid(c, o)) {
+      // Too few iterations left to repay a capture: launch directly.
+      for (int k = 0; k < chunk; ++k) EnqueueIteration(c, o, false);
     } else {
       hipGraphExec_t g = IterationGraph(c, o);
       for (int k = 0; k < chunk; ++k) HIPC(hipGraphLaunch(g, c.stream));
@@ -554,13 +709,12 @@ LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations
       if (st.iter == last_overflow_iter && same_iter_overflow++ > 0) {
         throw Failure(SDFSLAM_ERR_INTERNAL, "pattern re-analysis did not cover the point cells");
       }
-      // Re-centre the footprints on the current state; widen the dilation
-      // only when the previous epoch lasted no more than two iterations.
-      if (last_overflow_iter >= 0 && st.iter - last_overflow_iter <= 2) {
-        c.aopt.margin = std::min(c.aopt.margin + 1, 8);
-      }
+      // Re-centre the footprints on the current state and widen those of
+      // the poses that left theirs.
+      c.short_epoch = last_overflow_iter >= 0 && st.iter - last_overflow_iter <= 2;
       last_overflow_iter = st.iter;
       BuildEpoch(c);
+      c.short_epoch = false;
       st.overflow = 0;
       st.halt = 0;
       WriteState(c, st);
@@ -610,6 +764,25 @@ void CreateContext(const sdfslam_problem_view* problem, const sdfslam_solver_opt
   *out = c.release();
 }
 
+/// Replaces the context's problem and state, keeping its stream and device
+/// memory (pools grow only when the new problem needs more).
+void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
+                   const double* map_values, const double* poses) {
+  if (problem == nullptr || map_values == nullptr || poses == nullptr) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+  }
+  HIPC(hipSetDevice(c.device));
+  auto hp = std::make_unique<HostProblem>(*problem);
+  HIPC(hipStreamSynchronize(c.stream));
+  c.hp = std::move(hp);
+  c.aopt.pose_margin.clear();
+  c.aopt.margin = AnalysisOptions{}.margin;
+  AllocateBase(c, map_values, poses);
+  BuildEpoch(c);
+  InitLm(c, c.opts.lambda_init);
+  HIPC(hipStreamSynchronize(c.stream));
+}
+
 void FillResult(sdfslam_hip_context& c, const LmState& st, sdfslam_solve_result* result) {
   result->converged = st.converged;
   result->iterations = st.iter;
@@ -628,7 +801,8 @@ int SolveContext(sdfslam_hip_context& c, const sdfslam_solver_options& options,
                  sdfslam_solve_result* result) {
   if (options.max_iterations + 1 > c.history_capacity) {
     c.history_capacity = options.max_iterations + 1;
-    c.dw.cost_history = c.base.Alloc<double>(static_cast<size_t>(c.history_capacity));
+    c.hist.Clear();
+    c.dw.cost_history = c.hist.Alloc<double>(static_cast<size_t>(c.history_capacity));
     ++c.generation;
   }
   const LmOptions o = ToLm(options, c.history_capacity);
@@ -666,6 +840,15 @@ int sdfslam_hip_context_create(const sdfslam_problem_view* problem,
                                const double* poses, sdfslam_hip_context** out) {
   return Guard([&]() -> int {
     CreateContext(problem, options, map_values, poses, out);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_context_update(sdfslam_hip_context* ctx, const sdfslam_problem_view* problem,
+                               const double* map_values, const double* poses) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    UpdateContext(*ctx, problem, map_values, poses);
     return SDFSLAM_OK;
   });
 }
@@ -714,6 +897,7 @@ int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* in
     info->update_doubles = a.upd_total;
     info->tree_levels = a.num_levels;
     info->pattern_epochs = ctx->pattern_epochs;
+    info->analysis_seconds = ctx->analysis_seconds;
     return SDFSLAM_OK;
   });
 }

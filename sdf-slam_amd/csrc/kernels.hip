@@ -204,6 +204,7 @@ __global__ void KAssemblePoints(DeviceProblem P, DeviceAnalysis A, DeviceWork W,
   if (check != 0 && valid && f >= 1) {
     const uint32_t word = A.fp_cellbits[static_cast<int64_t>(f) * A.words_per_pose + (key >> 5)];
     if (((word >> (key & 31U)) & 1U) == 0U) {
+      W.pose_overflow[f] = 1;
       W.state->overflow = 1;
       W.state->halt = 1;
     }

@@ -105,6 +105,7 @@ struct DeviceWork {
   double* xnew;  // solution in elimination order
   double* partials;
   double* cost_history;
+  int32_t* pose_overflow;  // per frame: 1 when one of its points left the footprint
   LmState* state;
 };
 

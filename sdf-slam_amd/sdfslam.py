@@ -69,7 +69,7 @@ class AnalysisInfo(C.Structure):
                 ("pattern_nnz_lower", C.c_int64), ("num_supernodes", C.c_int64),
                 ("factor_nnz", C.c_int64), ("factor_flops", C.c_double),
                 ("update_doubles", C.c_int64), ("tree_levels", C.c_int32),
-                ("pattern_epochs", C.c_int32)]
+                ("pattern_epochs", C.c_int32), ("analysis_seconds", C.c_double)]
 
 
 class PhaseTimes(C.Structure):
