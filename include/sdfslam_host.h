@@ -8,6 +8,7 @@
  *   sdfslam_host_problem_create  <- Problem::Problem, src/problem/problem.cpp:11-115
  *   sdfslam_host_solve           <- Solve, src/solvers/solver.hpp:67 (+ kHip)
  *   sdfslam_host_parse_solver_yaml <- LoadConfig's solver block, src/pipeline/config.cpp:170-190
+ *   sdfslam_host_run_incremental <- Run's incremental branch, src/pipeline/runner.cpp:226-285
  */
 #ifndef SDFSLAM_HOST_H_
 #define SDFSLAM_HOST_H_
@@ -90,6 +91,39 @@ int sdfslam_host_workload_info(const sdfslam_host_problem* problem,
  * status code; the exception message is in sdfslam_host_last_error(). */
 int sdfslam_host_solve(sdfslam_host_problem* problem, const char* backend,
                        const sdfslam_solver_options* options, sdfslam_solve_result* result);
+
+/* One increment of an incremental run. */
+typedef struct sdfslam_increment_record {
+  int32_t frames;       /* frames in the increment's problem */
+  int32_t iterations;
+  double initial_cost;
+  double final_cost;
+  double build_seconds; /* Problem construction (hallucination, normals, specs) */
+  double solve_seconds; /* the increment's Solve, transfers included */
+} sdfslam_increment_record;
+
+/* Solve function of one increment, with sdfslam_hip_solve's contract. */
+typedef int (*sdfslam_solve_fn)(void* user, const sdfslam_problem_view* problem,
+                                const sdfslam_solver_options* options, double* map_values,
+                                double* poses, sdfslam_solve_result* result);
+
+/* Incremental run of a synthetic workload (runner.cpp:226-285): frames are
+ * appended `increment_size` at a time by composing odometry onto the last
+ * optimized pose, the Problem is rebuilt over all frames so far and refined
+ * with `options` (NULL: the workload's solver options with max_iterations =
+ * its iterations per increment). Runs over the first max_frames frames (all
+ * when <= 0). solve_fn NULL solves on the device through one reused context;
+ * otherwise solve_fn(user, ...) solves each increment. total: accumulated as
+ * runner.cpp:262-270 (cost_history is caller storage, may be NULL). records:
+ * up to records_capacity increments, *num_records = increments run.
+ * poses_out (3 * frames) and map_out (nx * ny) receive the final estimate
+ * (either may be NULL). */
+int sdfslam_host_run_incremental(const char* workload, uint64_t seed,
+                                 const sdfslam_solver_options* options, int32_t increment_size,
+                                 int32_t max_frames, sdfslam_solve_fn solve_fn, void* user,
+                                 sdfslam_solve_result* total, sdfslam_increment_record* records,
+                                 int32_t records_capacity, int32_t* num_records, double* poses_out,
+                                 double* map_out);
 
 /* Parses the `solver:` block of a run-config YAML file with the reference's
  * strictness (unknown keys are errors, config.cpp:13-26, 170-190). backend

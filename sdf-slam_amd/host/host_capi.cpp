@@ -9,6 +9,7 @@
 #include <string>
 
 #include "problem.hpp"
+#include "runner.hpp"
 #include "scene.hpp"
 #include "sdfslam_host.h"
 #include "solver.hpp"
@@ -48,6 +49,35 @@ int Guard(F&& fn) {
     return Fail(SDFSLAM_ERR_INTERNAL, what);
   } catch (const std::exception& e) {
     return Fail(SDFSLAM_ERR_INTERNAL, e.what());
+  }
+}
+
+SolverOptions FromC(const sdfslam_solver_options& c) {
+  SolverOptions o;
+  o.max_iterations = c.max_iterations;
+  o.step_tolerance = c.step_tolerance;
+  o.lambda_init = c.lambda_init;
+  o.lambda_factor = c.lambda_factor;
+  o.reject_worse_steps = c.reject_worse_steps != 0;
+  o.marquardt_scaling = c.marquardt_scaling != 0;
+  o.trust_region = c.trust_region != 0;
+  o.num_threads = c.num_threads;
+  o.linear_solver = static_cast<LinearSolver>(c.linear_solver);
+  return o;
+}
+
+void ToC(const SolveResult& r, sdfslam_solve_result* result) {
+  result->converged = r.converged ? 1 : 0;
+  result->iterations = r.iterations;
+  result->initial_cost = r.initial_cost;
+  result->final_cost = r.final_cost;
+  result->duration_seconds = r.duration_seconds;
+  result->cost_history_length = static_cast<int32_t>(r.cost_history.size());
+  if (result->cost_history != nullptr) {
+    const size_t n = std::min(r.cost_history.size(),
+                              static_cast<size_t>(std::max(0, result->cost_history_capacity)));
+    std::copy(r.cost_history.begin(), r.cost_history.begin() + static_cast<std::ptrdiff_t>(n),
+              result->cost_history);
   }
 }
 
@@ -257,7 +287,7 @@ int sdfslam_host_solve(sdfslam_host_problem* problem, const char* backend,
     if (problem == nullptr || options == nullptr || result == nullptr || backend == nullptr) {
       throw std::invalid_argument("null argument");
     }
-    SolverOptions o;
+    SolverOptions o = FromC(*options);
     const std::string b = backend;
     if (b == "hip") {
       o.backend = SolverBackend::kHip;
@@ -268,27 +298,50 @@ int sdfslam_host_solve(sdfslam_host_problem* problem, const char* backend,
     } else {
       throw std::runtime_error("unknown solver.backend '" + b + "'");
     }
-    o.max_iterations = options->max_iterations;
-    o.step_tolerance = options->step_tolerance;
-    o.lambda_init = options->lambda_init;
-    o.lambda_factor = options->lambda_factor;
-    o.reject_worse_steps = options->reject_worse_steps != 0;
-    o.marquardt_scaling = options->marquardt_scaling != 0;
-    o.trust_region = options->trust_region != 0;
-    o.num_threads = options->num_threads;
-    o.linear_solver = static_cast<LinearSolver>(options->linear_solver);
     const SolveResult r = Solve(*problem->problem, o);
-    result->converged = r.converged ? 1 : 0;
-    result->iterations = r.iterations;
-    result->initial_cost = r.initial_cost;
-    result->final_cost = r.final_cost;
-    result->duration_seconds = r.duration_seconds;
-    result->cost_history_length = static_cast<int32_t>(r.cost_history.size());
-    if (result->cost_history != nullptr) {
-      const size_t n = std::min(r.cost_history.size(),
-                                static_cast<size_t>(std::max(0, result->cost_history_capacity)));
-      std::copy(r.cost_history.begin(), r.cost_history.begin() + static_cast<std::ptrdiff_t>(n),
-                result->cost_history);
+    ToC(r, result);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_host_run_incremental(const char* workload, uint64_t seed,
+                                 const sdfslam_solver_options* options, int32_t increment_size,
+                                 int32_t max_frames, sdfslam_solve_fn solve_fn, void* user,
+                                 sdfslam_solve_result* total, sdfslam_increment_record* records,
+                                 int32_t records_capacity, int32_t* num_records, double* poses_out,
+                                 double* map_out) {
+  return Guard([&]() -> int {
+    if (workload == nullptr || total == nullptr) throw std::invalid_argument("null argument");
+    const Workload wl = MakeWorkload(workload, seed);
+    SolverOptions o = wl.solver;
+    if (options != nullptr) {
+      o = FromC(*options);
+    } else {
+      o.max_iterations = wl.iterations_per_increment;
+    }
+    o.backend = SolverBackend::kHip;
+    IncrementSolveFn fn;
+    if (solve_fn != nullptr) {
+      fn = [solve_fn, user](const sdfslam_problem_view* v, const sdfslam_solver_options* so, double* m,
+                            double* p, sdfslam_solve_result* r) { return solve_fn(user, v, so, m, p, r); };
+    }
+    const IncrementalRun run = RunIncremental(wl, o, increment_size, max_frames, fn);
+    ToC(run.total, total);
+    const auto count = static_cast<int32_t>(run.increments.size());
+    if (num_records != nullptr) *num_records = count;
+    for (int32_t k = 0; records != nullptr && k < std::min(count, records_capacity); ++k) {
+      const IncrementRecord& r = run.increments[static_cast<size_t>(k)];
+      records[k] = {r.frames, r.iterations, r.initial_cost, r.final_cost, r.build_seconds, r.solve_seconds};
+    }
+    if (poses_out != nullptr) {
+      for (size_t i = 0; i < run.estimated.size(); ++i) {
+        poses_out[3 * i] = run.estimated[i].x;
+        poses_out[3 * i + 1] = run.estimated[i].y;
+        poses_out[3 * i + 2] = run.estimated[i].theta;
+      }
+    }
+    if (map_out != nullptr) {
+      std::copy(run.map.values().begin(), run.map.values().end(), map_out);
     }
     return SDFSLAM_OK;
   });

@@ -95,19 +95,32 @@ class ProblemOptions(C.Structure):
 
 
 # Exported symbols of each library, as include/*.h declares them.
+class IncrementRecord(C.Structure):
+    _fields_ = [("frames", C.c_int32), ("iterations", C.c_int32), ("initial_cost", C.c_double),
+                ("final_cost", C.c_double), ("build_seconds", C.c_double),
+                ("solve_seconds", C.c_double)]
+
+
+# Solve function of one increment (sdfslam_solve_fn): user, view, options,
+# map values, poses, result -> status.
+SOLVE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(ProblemView), C.POINTER(SolverOptions),
+                       C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(SolveResult))
+
+
 HIP_SYMBOLS = [
     "sdfslam_hip_status_string", "sdfslam_hip_last_error", "sdfslam_hip_solve",
     "sdfslam_hip_context_create", "sdfslam_hip_context_destroy", "sdfslam_hip_set_state",
     "sdfslam_hip_get_state", "sdfslam_hip_analysis", "sdfslam_hip_assemble",
     "sdfslam_hip_normal_equations", "sdfslam_hip_linear_step", "sdfslam_hip_cost",
     "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
+    "sdfslam_hip_context_update",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
     "sdfslam_host_last_error", "sdfslam_host_problem_create", "sdfslam_host_workload_create",
     "sdfslam_host_problem_destroy", "sdfslam_host_problem_view", "sdfslam_host_problem_get_state",
     "sdfslam_host_problem_set_state", "sdfslam_host_workload_info", "sdfslam_host_solve",
-    "sdfslam_host_parse_solver_yaml",
+    "sdfslam_host_parse_solver_yaml", "sdfslam_host_run_incremental",
 ]
 
 
@@ -142,6 +155,7 @@ hip.sdfslam_hip_context_solve.argtypes = [C.c_void_p, C.POINTER(SolverOptions),
                                           C.POINTER(SolveResult)]
 hip.sdfslam_hip_context_iterate.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
 hip.sdfslam_hip_phase_times_get.argtypes = [C.c_void_p, C.POINTER(PhaseTimes)]
+hip.sdfslam_hip_context_update.argtypes = [C.c_void_p, C.POINTER(ProblemView), _f64p, _f64p]
 
 host.sdfslam_host_last_error.restype = C.c_char_p
 host.sdfslam_host_default_problem_options.argtypes = [C.POINTER(ProblemOptions)]
@@ -158,6 +172,9 @@ host.sdfslam_host_workload_info.argtypes = [C.c_void_p, C.POINTER(SolverOptions)
                                             _f64p]
 host.sdfslam_host_solve.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(SolverOptions),
                                     C.POINTER(SolveResult)]
+host.sdfslam_host_run_incremental.argtypes = [
+    C.c_char_p, C.c_uint64, C.POINTER(SolverOptions), C.c_int32, C.c_int32, SOLVE_FN, C.c_void_p,
+    C.POINTER(SolveResult), C.POINTER(IncrementRecord), C.c_int32, _i32p, _f64p, _f64p]
 host.sdfslam_host_parse_solver_yaml.argtypes = [C.c_char_p, C.POINTER(SolverOptions), C.c_char_p,
                                                 C.c_int32]
 
@@ -439,6 +456,15 @@ class Context:
         _check_hip(hip.sdfslam_hip_context_solve(self._h, C.byref(o), C.byref(r)))
         return _result_dict(r, hist)
 
+    def update(self, problem, map_values=None, poses=None):
+        """Replaces the context's problem (sdfslam_hip_context_update)."""
+        m0, p0 = problem.state()
+        m = np.ascontiguousarray(m0 if map_values is None else map_values, dtype=np.float64)
+        p = np.ascontiguousarray(p0 if poses is None else poses, dtype=np.float64)
+        _check_hip(hip.sdfslam_hip_context_update(self._h, C.byref(problem.view), ptr(m, _f64p),
+                                                  ptr(p, _f64p)))
+        self.problem = problem
+
     def iterate(self, k, phase_times=False):
         _check_hip(hip.sdfslam_hip_context_iterate(self._h, k, 1 if phase_times else 0))
 
@@ -446,3 +472,34 @@ class Context:
         t = PhaseTimes()
         _check_hip(hip.sdfslam_hip_phase_times_get(self._h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in PhaseTimes._fields_}
+
+
+def run_incremental(name, seed, options=None, increment_size=1, max_frames=0, solve_fn=None):
+    """Incremental run of a synthetic workload (sdfslam_host_run_incremental).
+
+    solve_fn: None solves on the device; otherwise a Python callable
+    (view_ptr, options_ptr, map_ptr, poses_ptr, result_ptr) -> status used
+    for every increment. Returns dict(total, records, poses, map).
+    """
+    full = Problem.workload(name, seed)
+    frames = full.view.num_poses if max_frames <= 0 else min(max_frames, full.view.num_poses)
+    g = full.view.grid
+    poses = np.zeros(3 * frames)
+    map_values = np.zeros(g.nx * g.ny)
+    recs = (IncrementRecord * frames)()
+    count = C.c_int32()
+    hist = np.zeros(frames * ((options.max_iterations if options is not None else 64) + 1) + 1)
+    total = SolveResult()
+    total.cost_history = ptr(hist, _f64p)
+    total.cost_history_capacity = len(hist)
+    cb = SOLVE_FN(0)
+    if solve_fn is not None:
+        cb = SOLVE_FN(lambda user, v, o, m, p, r: solve_fn(v, o, m, p, r))
+    _check_host(host.sdfslam_host_run_incremental(
+        name.encode(), seed, C.byref(options) if options is not None else None, increment_size,
+        frames, cb, None, C.byref(total), recs, frames, C.byref(count), ptr(poses, _f64p),
+        ptr(map_values, _f64p)))
+    records = [{k: getattr(recs[i], k) for k, _ in IncrementRecord._fields_}
+               for i in range(count.value)]
+    return dict(total=_result_dict(total, hist), records=records, poses=poses.reshape(-1, 3),
+                map=map_values)

@@ -137,3 +137,40 @@ def test_default_options_match_reference(sdfslam):
     assert p.hallucination_points_per_scan_point == 6 and p.hallucination_step_size == 0.1
     assert p.normals_method == 2 and p.normals_k_neighbors == 10 and p.smooth_gradient == 1
     assert p.active_margin == 2 and p.huber_delta == 0.0
+
+
+def test_incremental_driver_composes_odometry(sdfslam, oracle):
+    # runner.cpp:240-260 with a solve that changes nothing: each increment
+    # appends one frame composed from the last estimate, so consecutive
+    # estimates reproduce the odometry measurements of the rebuilt problem,
+    # and the problems grow by one frame per increment.
+    import ctypes as C
+    seen = []
+
+    def noop(view, opts, m, p, r):
+        v = view.contents
+        seen.append((v.num_poses, v.num_odom))
+        res = r.contents
+        res.converged, res.iterations = 0, 0
+        res.initial_cost = res.final_cost = 1.0
+        res.cost_history_length = 1
+        if res.cost_history_capacity > 0:
+            res.cost_history[0] = 1.0
+        if v.num_poses == 9:
+            seen.append(np.array([[v.odom_dx[k], v.odom_dy[k], v.odom_dtheta[k]]
+                                  for k in range(v.num_odom)]))
+        return 0
+
+    out = sdfslam.run_incremental("office_small", 1, max_frames=9, solve_fn=noop)
+    assert [s for s in seen if isinstance(s, tuple)] == [(f, f - 1) for f in range(2, 10)]
+    odo = seen[-1]
+    est = out["poses"]
+    rel = np.zeros(3)
+    for i in range(1, 9):
+        oracle.lib.oracle_relative(est[i - 1].ctypes.data_as(C.POINTER(C.c_double)),
+                                   est[i].ctypes.data_as(C.POINTER(C.c_double)),
+                                   rel.ctypes.data_as(C.POINTER(C.c_double)))
+        assert np.allclose(rel, odo[i - 1], atol=1e-12)
+    assert len(out["records"]) == 8 and out["total"]["iterations"] == 0
+    # totals follow runner.cpp:262-270: first initial cost, concatenated histories
+    assert out["total"]["initial_cost"] == 1.0 and len(out["total"]["cost_history"]) == 8
