@@ -120,6 +120,10 @@ typedef struct sdfslam_solve_result {
   int32_t cost_history_length;
 } sdfslam_solve_result;
 
+/* Free and total memory of the current HIP device (hipMemGetInfo), for
+ * admission decisions of callers that keep many contexts. */
+int sdfslam_hip_device_memory(int64_t* free_bytes, int64_t* total_bytes);
+
 /* Human-readable text for a status code (static storage). */
 const char* sdfslam_hip_status_string(int status);
 
@@ -168,6 +172,12 @@ typedef struct sdfslam_hip_analysis_info {
   int32_t tree_levels;
   int32_t pattern_epochs;
   double analysis_seconds;
+  int64_t device_bytes; /* device memory the context holds */
+  /* Launches of the 64-column panel-step kernel per factorization and
+   * their algorithmic FP64 flops (diagonal-block LDL^T 2/3 nb^3 plus
+   * nb^2 per TRSM row, summed over fronts and steps). */
+  int32_t panel_step_launches;
+  double panel_step_flops;
 } sdfslam_hip_analysis_info;
 int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* info);
 
@@ -217,7 +227,9 @@ int sdfslam_hip_context_iterate(sdfslam_hip_context* ctx, int32_t iterations,
 /* Average device time in milliseconds per iteration of each phase over the
  * last context_iterate with collect_phase_times (HIP events on the context's
  * stream): assembly, normal equations, factorization, triangular solves,
- * step+cost+control, their total, and the point-row assembly kernel alone. */
+ * step+cost+control, their total, the point-row assembly kernel alone, and
+ * the average panel-step launch (diagonal block + TRSM kernels of one
+ * 64-column step, bracketed by their own events). */
 typedef struct sdfslam_hip_phase_times {
   double assembly_ms;
   double normal_ms;
@@ -226,6 +238,7 @@ typedef struct sdfslam_hip_phase_times {
   double update_ms;
   double total_ms;
   double point_assembly_kernel_ms;
+  double panel_step_kernel_ms; /* average duration of one panel-step launch */
 } sdfslam_hip_phase_times;
 int sdfslam_hip_phase_times_get(sdfslam_hip_context* ctx, sdfslam_hip_phase_times* out);
 

@@ -125,6 +125,94 @@ int sdfslam_host_run_incremental(const char* workload, uint64_t seed,
                                  int32_t records_capacity, int32_t* num_records, double* poses_out,
                                  double* map_out);
 
+/* ---- Datasets: the frames of a synthetic workload -------------------------
+ * A dataset holds a workload's scans (sensor frame), its odometry
+ * (RelativeOdometry of the dataset poses, runner.cpp:52-59), its dataset
+ * poses, map grid and options, without building a Problem. */
+typedef struct sdfslam_host_dataset sdfslam_host_dataset;
+int sdfslam_host_dataset_create(const char* workload, uint64_t seed, sdfslam_host_dataset** out);
+void sdfslam_host_dataset_destroy(sdfslam_host_dataset* dataset);
+/* Frame count, grid, the workload's problem options and its per-increment
+ * solver options (max_iterations = iterations per increment). */
+int sdfslam_host_dataset_info(const sdfslam_host_dataset* dataset, int32_t* num_frames,
+                              sdfslam_grid* grid, sdfslam_problem_options* problem,
+                              sdfslam_solver_options* step);
+/* Frame k: its scan (2 * *num_points doubles into scan_xy, at most
+ * capacity points), the dataset pose (3) and the odometry from frame k-1
+ * to k (3; zeros for frame 0). Any output may be NULL. */
+int sdfslam_host_dataset_frame(const sdfslam_host_dataset* dataset, int32_t frame, double* scan_xy,
+                               int64_t capacity, int64_t* num_points, double* pose,
+                               double* odometry);
+
+/* ---- Served form: many incremental sessions on one GPU --------------------
+ * No counterpart in the reference (one session per process, runner.cpp:
+ * 226-285). Each session's increments run in submission order on the
+ * session's own device context; increments of different sessions run
+ * concurrently on `workers` host threads. Sessions are admitted while the
+ * device memory of all sessions stays within device_budget_bytes (0: 80 %
+ * of the device). */
+typedef struct sdfslam_server sdfslam_server;
+int sdfslam_server_create(int32_t workers, int64_t device_budget_bytes, sdfslam_server** out);
+/* Waits for the increments in flight, then frees every session. */
+void sdfslam_server_destroy(sdfslam_server* server);
+/* Opens a session at its first frame. map_values: nx*ny initial values
+ * (NULL = 0). Refused admission returns SDFSLAM_ERR_OOM. */
+int sdfslam_server_open_session(sdfslam_server* server, const sdfslam_grid* grid,
+                                const double* map_values, const sdfslam_problem_options* problem,
+                                const sdfslam_solver_options* step, const double* first_pose,
+                                const double* scan_xy, int64_t num_points, int32_t* session);
+/* Queues the session's next frame: its scan and the odometry from the
+ * previous frame (3). *ticket numbers the session's increments 1, 2, ... */
+int sdfslam_server_submit(sdfslam_server* server, int32_t session, const double* scan_xy,
+                          int64_t num_points, const double* odometry, int64_t* ticket);
+/* Blocks until increment `ticket` of the session is done; latency_seconds
+ * (may be NULL) receives completion minus submission time. */
+int sdfslam_server_wait(sdfslam_server* server, int32_t session, int64_t ticket,
+                        sdfslam_increment_record* record, double* latency_seconds);
+/* The session's estimate after its finished increments: *frames poses
+ * (3 each, at most capacity frames) and nx*ny map values (may be NULL). */
+int sdfslam_server_state(sdfslam_server* server, int32_t session, int32_t capacity,
+                         int32_t* frames, double* poses, double* map_values);
+
+/* Served benchmark: `sessions` sessions ("served<i>" workloads, seed seed0)
+ * with Poisson frame arrivals of rate_hz each for duration_s seconds. */
+typedef struct sdfslam_served_options {
+  int32_t sessions;
+  double rate_hz;
+  double duration_s;
+  int32_t workers;
+  uint64_t seed0;
+} sdfslam_served_options;
+typedef struct sdfslam_served_result {
+  int64_t increments;
+  int64_t lm_iterations;
+  double wall_seconds;
+  double increments_per_second;
+  double latency_p50_ms;
+  double latency_p99_ms;
+  double latency_mean_ms;
+  double build_mean_ms;
+  double solve_mean_ms;
+  int64_t device_bytes;
+} sdfslam_served_result;
+int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam_served_result* out);
+
+/* Batched benchmark: `instances` uneven batch problems ("batch<i>", seed
+ * seed0), `iterations` LM iterations each, `workers` at a time. */
+typedef struct sdfslam_batch_options {
+  int32_t instances;
+  int32_t iterations;
+  int32_t workers;
+  uint64_t seed0;
+} sdfslam_batch_options;
+typedef struct sdfslam_batch_result {
+  int64_t lm_iterations;
+  int64_t unknowns;
+  double wall_seconds;
+  double lm_iterations_per_second;
+} sdfslam_batch_result;
+int sdfslam_host_batch_benchmark(const sdfslam_batch_options* options, sdfslam_batch_result* out);
+
 /* Parses the `solver:` block of a run-config YAML file with the reference's
  * strictness (unknown keys are errors, config.cpp:13-26, 170-190). backend
  * receives "lm", "ceres" or "hip" (capacity >= 8). */

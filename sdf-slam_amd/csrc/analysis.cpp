@@ -2,6 +2,7 @@
 #include "analysis.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <exception>
 #include <functional>
@@ -26,10 +27,23 @@ bool BoxContains(const NdNode& t, int bw0, int bw1, int bh0, int bh1) {
   return bw0 >= t.w0 && bw1 < t.w1 && bh0 >= t.h0 && bh1 < t.h1;
 }
 
-/// Host threads for the analysis (the machine's cores, at most 16).
+/// Analyses running at once in this process (contexts analysed from
+/// several host threads share the cores).
+std::atomic<int> g_active_analyses{0};
+
+struct ActiveAnalysis {
+  ActiveAnalysis() { ++g_active_analyses; }
+  ~ActiveAnalysis() { --g_active_analyses; }
+  ActiveAnalysis(const ActiveAnalysis&) = delete;
+  ActiveAnalysis& operator=(const ActiveAnalysis&) = delete;
+};
+
+/// Host threads for one analysis: the machine's cores (at most 16) divided
+/// among the analyses running at once.
 int AnalysisThreads() {
   const unsigned hc = std::thread::hardware_concurrency();
-  return static_cast<int>(std::clamp(hc == 0 ? 1U : hc, 1U, 16U));
+  const int cores = static_cast<int>(std::clamp(hc == 0 ? 1U : hc, 1U, 16U));
+  return std::max(1, cores / std::max(1, g_active_analyses.load()));
 }
 
 /// Splits [0, n) into `chunks` contiguous ranges; chunk k is
@@ -148,6 +162,7 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   if (!opt.pose_margin.empty() && static_cast<int>(opt.pose_margin.size()) != N) {
     throw std::invalid_argument("per-pose margins do not match the pose count");
   }
+  const ActiveAnalysis active;
   const int threads = AnalysisThreads();
 
   std::vector<int32_t> active_to_node(static_cast<size_t>(a.num_active), -1);
@@ -770,6 +785,18 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     a.level_max_m[L] = std::max(a.level_max_m[L], a.sn_m[static_cast<size_t>(s)]);
     a.level_max_children[L] =
         std::max(a.level_max_children[L], static_cast<int32_t>(a.children[static_cast<size_t>(s)].size()));
+  }
+  for (int L = 0; L < a.num_levels; ++L) {
+    a.panel_step_launches += (a.level_max_nb[static_cast<size_t>(L)] + 63) / 64;
+  }
+  for (int s = 0; s < a.num_sn; ++s) {
+    const int ns = a.sn_first[static_cast<size_t>(s) + 1] - a.sn_first[static_cast<size_t>(s)];
+    const int f = a.sn_f[static_cast<size_t>(s)];
+    for (int c0 = 0; c0 < ns; c0 += 64) {
+      const double nb = std::min(64, ns - c0);
+      const double rows = f - c0 - nb;
+      a.panel_step_flops += 2.0 / 3.0 * nb * nb * nb + rows * nb * nb;
+    }
   }
   // Pattern size (lower triangle): node-node slots, footprint triples, pose
   // diagonal blocks, odometry pair blocks.

@@ -53,18 +53,6 @@ class DeviceArena {
     bytes_ += std::max<size_t>(n, 1) * sizeof(T);
     return static_cast<T*>(p);
   }
-  template <typename T>
-  T* Upload(const std::vector<T>& v) {
-    T* p = Alloc<T>(v.size());
-    if (!v.empty()) HIPC(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return p;
-  }
-  template <typename T>
-  T* Upload(const T* data, size_t n) {
-    T* p = Alloc<T>(n);
-    if (n > 0) HIPC(hipMemcpy(p, data, n * sizeof(T), hipMemcpyHostToDevice));
-    return p;
-  }
   void Clear() {
     for (void* p : ptrs_) (void)hipFree(p);
     ptrs_.clear();
@@ -301,6 +289,10 @@ struct sdfslam_hip_context {
   int max_f{0};
   hipEvent_t events[kPhases + 3]{};  // phase marks, then the point-row kernel pair
   double phase_ms[kPhases + 1]{};
+  std::vector<hipEvent_t> step_events;  // pairs around panel steps (timed runs)
+  int step_events_used{0};
+  double panel_step_ms{0.0};
+  int64_t panel_steps_timed{0};
   int timed_iterations{0};
   bool lm_ready{false};
 
@@ -309,6 +301,7 @@ struct sdfslam_hip_context {
     for (hipEvent_t& e : events) {
       if (e != nullptr) (void)hipEventDestroy(e);
     }
+    for (hipEvent_t e : step_events) (void)hipEventDestroy(e);
     epoch.Release();
     base.Release();
     hist.Clear();
@@ -397,6 +390,7 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
     W.partials = A.Alloc<double>(3 * static_cast<size_t>(kPartialBlocks));
     W.state = A.Alloc<LmState>(1);
     W.pose_overflow = A.Alloc<int32_t>(static_cast<size_t>(v.num_poses));
+    W.solve_tmp = A.Alloc<double>(static_cast<size_t>(P.n));
     c.scalar = A.Alloc<double>(1);
   };
   A.BeginMeasure();
@@ -404,9 +398,10 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   A.BeginPlace();
   layout();
   A.Finish(c.stream);
-  HIPC(hipMemset(W.pose_overflow, 0, sizeof(int32_t) * static_cast<size_t>(v.num_poses)));
-  LmState zero{};
-  HIPC(hipMemcpy(W.state, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  HIPC(hipMemsetAsync(W.pose_overflow, 0, sizeof(int32_t) * static_cast<size_t>(v.num_poses), c.stream));
+  const LmState zero{};
+  HIPC(hipMemcpyAsync(W.state, &zero, sizeof(zero), hipMemcpyHostToDevice, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
   const int want = std::max(c.opts.max_iterations, 0) + 1;
   if (W.cost_history == nullptr || want > c.history_capacity) {
     c.hist.Clear();
@@ -447,7 +442,9 @@ void GrowMargins(sdfslam_hip_context& c) {
     c.aopt.pose_margin.assign(static_cast<size_t>(N), c.aopt.margin);
   }
   std::vector<int32_t> flags(static_cast<size_t>(N));
-  HIPC(hipMemcpy(flags.data(), c.dw.pose_overflow, sizeof(int32_t) * flags.size(), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpyAsync(flags.data(), c.dw.pose_overflow, sizeof(int32_t) * flags.size(),
+                      hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
   const int step = c.short_epoch ? 2 : 1;
   for (int P = 0; P < N; ++P) {
     if (flags[static_cast<size_t>(P)] != 0) {
@@ -455,7 +452,7 @@ void GrowMargins(sdfslam_hip_context& c) {
       m = std::min(m + step, kMaxMargin);
     }
   }
-  HIPC(hipMemset(c.dw.pose_overflow, 0, sizeof(int32_t) * flags.size()));
+  HIPC(hipMemsetAsync(c.dw.pose_overflow, 0, sizeof(int32_t) * flags.size(), c.stream));
 }
 
 /// Symbolic analysis at the current device state and the device buffers
@@ -578,6 +575,16 @@ void EnqueueAssemblyAndNormal(sdfslam_hip_context& c, const LmOptions& o, bool t
 void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
   hipStream_t s = c.stream;
   const Analysis& an = c.an;
+  StepEvents step_ev{nullptr, 0, 0};
+  if (timing) {
+    const size_t want = 2 * static_cast<size_t>(std::min(an.panel_step_launches, 1024));
+    while (c.step_events.size() < want) {
+      hipEvent_t e = nullptr;
+      HIPC(hipEventCreate(&e));
+      c.step_events.push_back(e);
+    }
+    step_ev = {c.step_events.data(), static_cast<int>(c.step_events.size() / 2), 0};
+  }
   for (int L = 0; L < an.num_levels; ++L) {
     const auto& cls = c.level_children[static_cast<size_t>(L)];
     for (const ChildList& cl : cls) {
@@ -586,21 +593,25 @@ void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
     const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
     const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
     LaunchFactorLevel(c.da, c.dw, lvl, count, an.level_max_nb[static_cast<size_t>(L)],
-                      an.level_max_f[static_cast<size_t>(L)], an.level_max_m[static_cast<size_t>(L)], s);
+                      an.level_max_f[static_cast<size_t>(L)], an.level_max_m[static_cast<size_t>(L)], s,
+                      timing ? &step_ev : nullptr);
     for (const ChildList& cl : cls) {
       LaunchExtendAddUpdate(c.da, c.dw, c.child_lists + cl.offset, cl.count, cl.max_m, s);
     }
   }
+  c.step_events_used = step_ev.used;
   Mark(c, timing, 3);
   for (int L = 0; L < an.num_levels; ++L) {
     const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
     const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
-    LaunchForwardLevel(c.da, c.dw, lvl, count, an.level_max_f[static_cast<size_t>(L)], s);
+    LaunchForwardLevel(c.da, c.dw, lvl, count, an.level_max_m[static_cast<size_t>(L)],
+                       an.level_max_f[static_cast<size_t>(L)], s);
   }
   for (int L = an.num_levels - 1; L >= 0; --L) {
     const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
     const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
-    LaunchBackwardLevel(c.da, c.dw, lvl, count, an.level_max_f[static_cast<size_t>(L)], s);
+    LaunchBackwardLevel(c.da, c.dw, lvl, count, an.level_max_nb[static_cast<size_t>(L)],
+                        an.level_max_m[static_cast<size_t>(L)], an.level_max_f[static_cast<size_t>(L)], s);
   }
 }
 
@@ -622,8 +633,9 @@ bool SameOptions(const LmOptions& a, const LmOptions& b) {
 }
 
 /// Runs shorter than this many iterations launch kernels directly unless a
-/// captured graph for the current buffers already exists.
-constexpr int kGraphMinIterations = 6;
+/// captured graph for the current buffers already exists (a capture costs
+/// about one direct enqueue of the iteration plus its instantiation).
+constexpr int kGraphMinIterations = 2;
 
 bool GraphValid(const sdfslam_hip_context& c, const LmOptions& o) {
   return c.graph != nullptr && c.graph_generation == c.generation && SameOptions(c.graph_options, o);
@@ -638,7 +650,9 @@ hipGraphExec_t IterationGraph(sdfslam_hip_context& c, const LmOptions& o) {
     c.graph = nullptr;
   }
   hipGraph_t g = nullptr;
-  HIPC(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  // Relaxed: other host threads driving their own contexts may call HIP
+  // (allocations, copies) while this stream is being captured.
+  HIPC(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
   EnqueueIteration(c, o, false);
   HIPC(hipStreamEndCapture(c.stream, &g));
   const hipError_t e = hipGraphInstantiate(&c.graph, g, nullptr, nullptr, 0);
@@ -650,6 +664,14 @@ hipGraphExec_t IterationGraph(sdfslam_hip_context& c, const LmOptions& o) {
 }
 
 void AccumulateTimes(sdfslam_hip_context& c) {
+  for (int k = 0; k < c.step_events_used; ++k) {
+    float ms = 0.0f;
+    HIPC(hipEventElapsedTime(&ms, c.step_events[2 * static_cast<size_t>(k)],
+                             c.step_events[2 * static_cast<size_t>(k) + 1]));
+    c.panel_step_ms += ms;
+    ++c.panel_steps_timed;
+  }
+  c.step_events_used = 0;
   for (int k = 0; k < kPhases; ++k) {
     float ms = 0.0f;
     HIPC(hipEventElapsedTime(&ms, c.events[k], c.events[k + 1]));
@@ -792,8 +814,9 @@ void FillResult(sdfslam_hip_context& c, const LmState& st, sdfslam_solve_result*
   result->cost_history_length = len;
   if (result->cost_history != nullptr && result->cost_history_capacity > 0) {
     const int n = std::min({len, result->cost_history_capacity, c.history_capacity});
-    HIPC(hipMemcpy(result->cost_history, c.dw.cost_history, static_cast<size_t>(n) * sizeof(double),
-                   hipMemcpyDeviceToHost));
+    HIPC(hipMemcpyAsync(result->cost_history, c.dw.cost_history, static_cast<size_t>(n) * sizeof(double),
+                        hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
   }
 }
 
@@ -844,6 +867,24 @@ int sdfslam_hip_context_create(const sdfslam_problem_view* problem,
   });
 }
 
+int sdfslam_hip_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+  return Guard([&]() -> int {
+    if (free_bytes == nullptr || total_bytes == nullptr) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+      throw Failure(SDFSLAM_ERR_NO_DEVICE, "no HIP device");
+    }
+    size_t f = 0;
+    size_t t = 0;
+    HIPC(hipMemGetInfo(&f, &t));
+    *free_bytes = static_cast<int64_t>(f);
+    *total_bytes = static_cast<int64_t>(t);
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_hip_context_update(sdfslam_hip_context* ctx, const sdfslam_problem_view* problem,
                                const double* map_values, const double* poses) {
   return Guard([&]() -> int {
@@ -863,10 +904,12 @@ int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, co
     if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
     const sdfslam_problem_view& v = ctx->hp->view;
     if (map_values != nullptr) {
-      HIPC(hipMemcpy(ctx->dw.map, map_values, sizeof(double) * v.grid.nx * v.grid.ny, hipMemcpyHostToDevice));
+      HIPC(hipMemcpyAsync(ctx->dw.map, map_values, sizeof(double) * v.grid.nx * v.grid.ny,
+                          hipMemcpyHostToDevice, ctx->stream));
     }
     if (poses != nullptr) {
-      HIPC(hipMemcpy(ctx->dw.poses, poses, sizeof(double) * 3 * v.num_poses, hipMemcpyHostToDevice));
+      HIPC(hipMemcpyAsync(ctx->dw.poses, poses, sizeof(double) * 3 * v.num_poses, hipMemcpyHostToDevice,
+                          ctx->stream));
     }
     BuildEpoch(*ctx);
     return SDFSLAM_OK;
@@ -898,6 +941,9 @@ int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* in
     info->tree_levels = a.num_levels;
     info->pattern_epochs = ctx->pattern_epochs;
     info->analysis_seconds = ctx->analysis_seconds;
+    info->device_bytes = static_cast<int64_t>(ctx->base.bytes() + ctx->epoch.bytes() + ctx->hist.bytes());
+    info->panel_step_launches = a.panel_step_launches;
+    info->panel_step_flops = a.panel_step_flops;
     return SDFSLAM_OK;
   });
 }
@@ -1042,9 +1088,12 @@ int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int64_
       throw Failure(SDFSLAM_ERR_INTERNAL, "pattern size mismatch");
     }
     std::vector<double> panel(static_cast<size_t>(an.panel_total));
-    HIPC(hipMemcpy(panel.data(), c.dw.panel, panel.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpyAsync(panel.data(), c.dw.panel, panel.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
     if (rhs != nullptr) {
-      HIPC(hipMemcpy(rhs, c.dw.rhs, sizeof(double) * an.n, hipMemcpyDeviceToHost));
+      HIPC(hipMemcpyAsync(rhs, c.dw.rhs, sizeof(double) * an.n, hipMemcpyDeviceToHost, c.stream));
+      HIPC(hipStreamSynchronize(c.stream));
     }
     if (col_ptr != nullptr) {
       std::fill(col_ptr, col_ptr + an.n + 1, 0);
@@ -1103,7 +1152,8 @@ int sdfslam_hip_linear_step(sdfslam_hip_context* ctx, double lambda, double* ste
       g_last_error = "sparse Cholesky factorization failed";
       return SDFSLAM_ERR_NOT_SPD;
     }
-    HIPC(hipMemcpy(step, c.dw.step, sizeof(double) * c.an.n, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpyAsync(step, c.dw.step, sizeof(double) * c.an.n, hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
     return SDFSLAM_OK;
   });
 }
@@ -1144,6 +1194,8 @@ int sdfslam_hip_context_iterate(sdfslam_hip_context* ctx, int32_t iterations,
     o.max_iterations = INT_MAX;
     for (double& p : c.phase_ms) p = 0.0;
     c.timed_iterations = 0;
+    c.panel_step_ms = 0.0;
+    c.panel_steps_timed = 0;
     LmState st{};
     ReadState(c, st);
     if (st.halt != 0) {
@@ -1170,6 +1222,8 @@ int sdfslam_hip_phase_times_get(sdfslam_hip_context* ctx, sdfslam_hip_phase_time
   out->total_ms = (ctx->phase_ms[0] + ctx->phase_ms[1] + ctx->phase_ms[2] + ctx->phase_ms[3] +
                    ctx->phase_ms[4]) / n;
   out->point_assembly_kernel_ms = ctx->phase_ms[kPhases] / n;
+  out->panel_step_kernel_ms =
+      ctx->panel_steps_timed > 0 ? ctx->panel_step_ms / static_cast<double>(ctx->panel_steps_timed) : 0.0;
   return SDFSLAM_OK;
 }
 

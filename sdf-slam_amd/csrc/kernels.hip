@@ -698,64 +698,150 @@ __global__ void KExtendAddUpdate(DeviceAnalysis A, DeviceWork W, const int32_t* 
 // then 32..63). Workgroup 0 stores the factored block in the front's staging
 // slot; the trailing-update kernel moves it into the panel, so no workgroup
 // of this launch reads a partly written block.
+// Zero, computed from `dep` as far as the compiler can tell: an LDS address
+// offset by it cannot be loaded before `dep` is final, which keeps a fully
+// unrolled solve from hoisting hundreds of loads ahead of their use.
+__device__ __forceinline__ int ZeroAfter(double dep) {
+  int z;
+  __asm__ volatile("v_mov_b32 %0, 0" : "=v"(z) : "v"(dep));
+  return z;
+}
+
 // Columns 0..31 of a panel row below the diagonal block: y = x L11^-T,
-// L = y D^-1; y is kept in `ybuf` for the second half.
+// L = y D^-1; y is kept in `ybuf` for the second half. Right-looking: once
+// y_t is final it is applied to every later column, so each column sees the
+// same fma sequence (t = 0, 1, ...) as a left-looking solve while the
+// updates of different columns are independent. lt[t][j] = L(j, t).
 __device__ __forceinline__ void SolveRowFirstHalf(double* F, int f, int c0, int nb, int r,
-                                               const double (*a)[kNB + 1], double* ybuf) {
-  // The per-column guards keep the compiler from hoisting every LDS load
-  // of the unrolled solve (which spills registers).
-  double y[32];
+                                               const double (*lt)[kNB + 2], double* ybuf) {
+  double x[32];
 #pragma unroll
   for (int j = 0; j < 32; ++j) {
-    y[j] = 0.0;
-    if (j < nb) {
-      double x = F[static_cast<int64_t>(c0 + j) * f + r];
+    const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + r];
+    x[j] = j < nb ? v : 0.0;
+  }
 #pragma unroll
-      for (int t = 0; t < 32; ++t) {
-        if (t < j) x = fma(-y[t], a[j][t], x);
-      }
-      y[j] = x;
-    }
+  for (int t = 0; t < 32; ++t) {
+    const double* col = &lt[t][0] + ZeroAfter(x[t]);
+#pragma unroll
+    for (int j = t + 1; j < 32; ++j) x[j] = fma(-x[t], col[j], x[j]);
   }
 #pragma unroll
   for (int j = 0; j < 32; ++j) {
-    ybuf[j] = y[j];
-    if (j < nb) F[static_cast<int64_t>(c0 + j) * f + r] = y[j] / a[j][j];
+    ybuf[j] = x[j];
+    if (j < nb) F[static_cast<int64_t>(c0 + j) * f + r] = x[j] / lt[j][j];
   }
 }
 
-// Columns 32..63: x = (F - y L21^T) L22^-T, L = x D^-1.
+// Columns 32..63: x = (F - y L21^T) L22^-T, L = x D^-1 (right-looking as
+// above).
 __device__ __forceinline__ void SolveRowSecondHalf(double* F, int f, int c0, int nb, int r,
-                                                const double (*a)[kNB + 1], const double* ybuf) {
+                                                const double (*lt)[kNB + 2], const double* ybuf) {
   double x[32];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) x[j] = 32 + j < nb ? F[static_cast<int64_t>(c0 + 32 + j) * f + r] : 0.0;
+  for (int j = 0; j < 32; ++j) {
+    const double v = F[static_cast<int64_t>(c0 + min(32 + j, nb - 1)) * f + r];
+    x[j] = 32 + j < nb ? v : 0.0;
+  }
 #pragma unroll 1
   for (int t = 0; t < 32; ++t) {
     const double yt = ybuf[t];
+    const double* col = &lt[t][32];
 #pragma unroll
-    for (int j = 0; j < 32; ++j) x[j] = fma(-yt, a[32 + j][t], x[j]);
+    for (int j = 0; j < 32; ++j) x[j] = fma(-yt, col[j], x[j]);
+  }
+#pragma unroll
+  for (int t = 0; t < 32; ++t) {
+    const double* col = &lt[32 + t][32] + ZeroAfter(x[t]);
+#pragma unroll
+    for (int j = t + 1; j < 32; ++j) x[j] = fma(-x[t], col[j], x[j]);
   }
 #pragma unroll
   for (int j = 0; j < 32; ++j) {
-    if (32 + j < nb) {
-#pragma unroll
-      for (int t = 0; t < 32; ++t) {
-        if (t < j) x[j] = fma(-x[t], a[32 + j][32 + t], x[j]);
-      }
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    if (32 + j < nb) F[static_cast<int64_t>(c0 + 32 + j) * f + r] = x[j] / a[32 + j][32 + j];
+    if (32 + j < nb) F[static_cast<int64_t>(c0 + 32 + j) * f + r] = x[j] / lt[32 + j][32 + j];
   }
 }
 
 constexpr int kStepRows = 128;
-constexpr int kPairSlots = (kNB * (kNB + 1) / 2 + kThreads - 1) / kThreads;
 
+// Reads lane `lane`'s value of v (a wave-uniform lane index).
+__device__ __forceinline__ double ReadLane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+// Right-looking LDL^T of a 64x64 block held by one wavefront, row `lane` in
+// `row` (lower triangle and diagonal meaningful; the upper part is scratch;
+// rows and columns from nb on are an identity pad).
+// Step k reads the pivot and the multipliers of column k from their lanes
+// with v_readlane, so the 64 dependent steps need neither LDS nor barriers.
+// On return row[j < lane] = L(lane, j) and row[lane] = D(lane). Returns true
+// when a pivot is exactly zero.
+__device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int nb) {
+  bool zero = false;
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) {
+    // The identity pad needs no steps; a guard (not a break) keeps the loop
+    // unrolled and `row` in registers.
+    if (k < nb) {
+      const double d = ReadLane(row[k], k);
+      zero = zero || d == 0.0;
+      const double l = row[k] / d;
+      if (lane > k) row[k] = l;
+      const double t = -l * d;
+#pragma unroll
+      for (int j = k + 1; j < kNB; ++j) row[j] = fma(t, ReadLane(l, j), row[j]);
+    }
+  }
+  return zero;
+}
+
+// The kb-th diagonal block of every front at the level, one wavefront per
+// front: factored in registers and stored in the front's staging slot
+// (with the zero-pivot flag), for KPanelStep's staged mode.
+__global__ void __launch_bounds__(kNB) KDiagBlock(DeviceAnalysis A, DeviceWork W,
+                                                 const int32_t* level_sn, int kb) {
+  if (Halted(W.state)) return;
+  const int s = level_sn[blockIdx.x];
+  const int ns = A.sn_first[s + 1] - A.sn_first[s];
+  const int c0 = kb * kNB;
+  if (c0 >= ns) return;
+  const int nb = min(kNB, ns - c0);
+  const int f = A.sn_f[s];
+  const double* F = W.panel + A.panel_off[s];
+  const int tid = threadIdx.x;
+  double row[kNB];
+  const int ti = min(tid, nb - 1);
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + c0 + ti];
+    row[j] = (tid < nb && j < nb && j <= tid) ? v : (j == tid ? 1.0 : 0.0);
+  }
+  const bool zero = FactorDiagWave(row, tid, nb);
+  double* stage = W.diag_stage + static_cast<int64_t>(s) * kNB * kNB;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (j <= tid && tid < nb) stage[j * kNB + tid] = row[j];
+  }
+  if (tid == 0 && zero) {
+    W.state->not_spd = 1;
+    W.state->halt = 1;
+  }
+}
+
+// One 64-column step of the blocked panel factorization of every front at
+// the level: the factored diagonal block L_kk D_k (computed here by wave 0
+// of every workgroup, or, with `staged`, read from the staging slot that
+// KDiagBlock filled) and the TRSM of 128 panel rows below it per workgroup,
+// L = F L_kk^-T D_k^-1, two threads per row (columns 0..31, then 32..63).
+// In fused mode workgroup 0 stores the factored block in the staging slot;
+// the trailing-update kernel moves it into the panel, so no workgroup of
+// this launch reads a partly written block.
 __global__ void __launch_bounds__(kThreads) KPanelStep(DeviceAnalysis A, DeviceWork W,
-                                                       const int32_t* level_sn, int kb) {
+                                                       const int32_t* level_sn, int kb, int staged) {
   if (Halted(W.state)) return;
   const int s = level_sn[blockIdx.y];
   const int ns = A.sn_first[s + 1] - A.sn_first[s];
@@ -766,47 +852,40 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(DeviceAnalysis A, DeviceW
   const int r0 = c0 + blockIdx.x * kStepRows;
   if (r0 >= f) return;
   double* F = W.panel + A.panel_off[s];
-  __shared__ double a[kNB][kNB + 1];
-  __shared__ double lcol[kNB];
+  __shared__ double lt[kNB][kNB + 2];  // lt[j][i] = L(i, j), lt[i][i] = D(i)
   __shared__ double ybuf[kStepRows][33];
+  __shared__ int zero_pivot;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-    const int i = idx % nb;
-    const int j = idx / nb;
-    if (i >= j) a[i][j] = F[static_cast<int64_t>(c0 + j) * f + c0 + i];
-  }
-  // Lower-triangle pairs p = tid + 256 q, p = i (i + 1) / 2 + j.
-  int pi[kPairSlots];
-  int pj[kPairSlots];
-#pragma unroll
-  for (int q = 0; q < kPairSlots; ++q) {
-    const int p = tid + kThreads * q;
-    int i = static_cast<int>((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
-    while (i * (i + 1) / 2 > p) --i;
-    while ((i + 1) * (i + 2) / 2 <= p) ++i;
-    const int j = p - i * (i + 1) / 2;
-    const bool live = p < kNB * (kNB + 1) / 2 && i < nb;
-    pi[q] = live ? i : -1;
-    pj[q] = j;
-  }
-  __syncthreads();
-  bool zero = false;
-  for (int k = 0; k < nb; ++k) {
-    const double d = a[k][k];
-    zero = zero || d == 0.0;
-    if (tid > k && tid < nb) {
-      const double l = a[tid][k] / d;
-      lcol[tid] = l;
-      a[tid][k] = l;
+  if (staged != 0) {
+    const double* stage = W.diag_stage + static_cast<int64_t>(s) * kNB * kNB;
+    for (int idx = tid; idx < kNB * kNB; idx += blockDim.x) {
+      const int i = idx % kNB;
+      const int j = idx / kNB;
+      if (i >= j) lt[j][i] = (i < nb && j < nb) ? stage[idx] : (i == j ? 1.0 : 0.0);
     }
     __syncthreads();
+  } else if (tid < kNB) {
+    // Wave 0: row tid of the diagonal block; rows and columns beyond nb
+    // are an identity pad.
+    // Every load is issued (clamped into the block) and the pad selected
+    // afterwards, so the 64 loads are in flight together.
+    double row[kNB];
+    const int ti = min(tid, nb - 1);
 #pragma unroll
-    for (int q = 0; q < kPairSlots; ++q) {
-      if (pi[q] >= 0 && pj[q] > k) a[pi[q]][pj[q]] = fma(-lcol[pi[q]] * d, lcol[pj[q]], a[pi[q]][pj[q]]);
+    for (int j = 0; j < kNB; ++j) {
+      const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + c0 + ti];
+      row[j] = (tid < nb && j < nb && j <= tid) ? v : (j == tid ? 1.0 : 0.0);
     }
-    __syncthreads();
+    const bool zero = FactorDiagWave(row, tid, nb);
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      if (j <= tid) lt[j][tid] = row[j];
+    }
+    if (tid == 0) zero_pivot = zero ? 1 : 0;
   }
-  if (blockIdx.x == 0) {
+  if (staged == 0) __syncthreads();
+  if (staged == 0 && blockIdx.x == 0) {
+    const bool zero = zero_pivot != 0;
     if (tid == 0 && zero) {
       W.state->not_spd = 1;
       W.state->halt = 1;
@@ -815,16 +894,16 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(DeviceAnalysis A, DeviceW
     for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
       const int i = idx % nb;
       const int j = idx / nb;
-      if (i >= j) stage[j * kNB + i] = a[i][j];
+      if (i >= j) stage[j * kNB + i] = lt[j][i];
     }
   }
   const int lr = tid & (kStepRows - 1);
   const int half = tid / kStepRows;
   const int r = r0 + lr;
   const bool active = r >= c0 + nb && r < f;
-  if (half == 0 && active) SolveRowFirstHalf(F, f, c0, nb, r, a, ybuf[lr]);
+  if (half == 0 && active) SolveRowFirstHalf(F, f, c0, nb, r, lt, ybuf[lr]);
   __syncthreads();
-  if (half == 1 && active && nb > 32) SolveRowSecondHalf(F, f, c0, nb, r, a, ybuf[lr]);
+  if (half == 1 && active && nb > 32) SolveRowSecondHalf(F, f, c0, nb, r, lt, ybuf[lr]);
 }
 
 // C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
@@ -938,7 +1017,9 @@ __global__ void __launch_bounds__(kThreads) KGemmLower(DeviceAnalysis A, DeviceW
 
 // Forward: gather the children's update vectors, then L y = b over the front
 // (unit lower), leaving y in the own rows and the update for the parent in R.
-__global__ void KForward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn) {
+// With `tri` the rows below the front's own columns (i >= n) are left as
+// assembled and KForwardRest applies L21 to them across many workgroups.
+__global__ void KForward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn, int tri) {
   if (Halted(W.state)) return;
   const int s = level_sn[blockIdx.x];
   const int first = A.sn_first[s];
@@ -985,7 +1066,8 @@ __global__ void KForward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn
     }
     __syncthreads();
     // Rows below the block; loads batched ahead of the ordered FMA chain.
-    for (int i = b1 + tid; i < f; i += blockDim.x) {
+    const int rows_end = tri != 0 ? n : f;
+    for (int i = b1 + tid; i < rows_end; i += blockDim.x) {
       double acc = v[i];
       int k = b0;
       for (; k + 8 <= b1; k += 8) {
@@ -1004,8 +1086,63 @@ __global__ void KForward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn
   for (int i = tid; i < f; i += blockDim.x) out[i] = v[i];
 }
 
-// Backward: x1 = L11^-T (D^-1 y1 - L21^T x2), x2 from the ancestors.
-__global__ void KBackward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn) {
+// v2 -= L21 v1 for the fronts KForward solved with `tri`: one thread per
+// row, the same ordered fma chain over k = 0 .. n-1 as the fused kernel.
+__global__ void __launch_bounds__(64) KForwardRest(DeviceAnalysis A, DeviceWork W,
+                                                   const int32_t* level_sn) {
+  if (Halted(W.state)) return;
+  const int s = level_sn[blockIdx.y];
+  const int n = A.sn_first[s + 1] - A.sn_first[s];
+  const int f = A.sn_f[s];
+  const int i = n + blockIdx.x * 64 + threadIdx.x;
+  if (i >= f) return;
+  const double* F = W.panel + A.panel_off[s];
+  double* v = W.vec + A.sn_idx_ptr[s];
+  double acc = v[i];
+  int k = 0;
+  for (; k + 8 <= n; k += 8) {
+    double fv[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) fv[t] = F[static_cast<int64_t>(k + t) * f + i];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc = fma(-fv[t], v[k + t], acc);
+  }
+  for (; k < n; ++k) acc = fma(-F[static_cast<int64_t>(k) * f + i], v[k], acc);
+  v[i] = acc;
+}
+
+// L21^T x2 per column of the fronts at the level, one wavefront per column
+// (the same lane-strided sum and wave reduction as the fused backward
+// kernel), into tmp at the column's elimination index.
+__global__ void KBackwardRest(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn) {
+  if (Halted(W.state)) return;
+  const int s = level_sn[blockIdx.y];
+  const int first = A.sn_first[s];
+  const int n = A.sn_first[s + 1] - first;
+  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (j >= n) return;
+  const int m = A.sn_m[s];
+  const int f = n + m;
+  const int lane = threadIdx.x & 63;
+  const int32_t* idx = A.sn_idx + A.sn_idx_ptr[s];
+  const double* col = W.panel + A.panel_off[s] + static_cast<int64_t>(j) * f + n;
+  double acc = 0.0;
+  int a = lane;
+  for (; a + 192 < m; a += 256) {
+    const double f0 = col[a], f1 = col[a + 64], f2 = col[a + 128], f3 = col[a + 192];
+    acc = fma(f0, W.xnew[idx[n + a]], acc);
+    acc = fma(f1, W.xnew[idx[n + a + 64]], acc);
+    acc = fma(f2, W.xnew[idx[n + a + 128]], acc);
+    acc = fma(f3, W.xnew[idx[n + a + 192]], acc);
+  }
+  for (; a < m; a += 64) acc = fma(col[a], W.xnew[idx[n + a]], acc);
+  acc = WaveSum(acc);
+  if (lane == 0) W.solve_tmp[first + j] = acc;
+}
+
+// Backward: x1 = L11^-T (D^-1 y1 - L21^T x2), x2 from the ancestors. With
+// `tri`, L21^T x2 comes from KBackwardRest.
+__global__ void KBackward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn, int tri) {
   if (Halted(W.state)) return;
   const int s = level_sn[blockIdx.x];
   const int first = A.sn_first[s];
@@ -1020,10 +1157,13 @@ __global__ void KBackward(DeviceAnalysis A, DeviceWork W, const int32_t* level_s
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int waves = blockDim.x >> 6;
-  for (int j = tid; j < n; j += blockDim.x) t[j] = y[j] / F[static_cast<int64_t>(j) * f + j];
+  for (int j = tid; j < n; j += blockDim.x) {
+    const double yd = y[j] / F[static_cast<int64_t>(j) * f + j];
+    t[j] = tri != 0 ? yd - W.solve_tmp[first + j] : yd;
+  }
   for (int a = tid; a < m; a += blockDim.x) t[n + a] = W.xnew[idx[n + a]];
   __syncthreads();
-  for (int j = wave; j < n; j += waves) {
+  for (int j = wave; tri == 0 && j < n; j += waves) {
     const double* col = F + static_cast<int64_t>(j) * f + n;
     double acc = 0.0;
     int a = lane;
@@ -1105,8 +1245,9 @@ __device__ double BlockSum(double v, double* red) {
 }
 
 // Problem::Cost terms (problem.cpp:132-156) into per-block partial sums.
-__global__ void KCost(DeviceProblem P, const double* map, const double* poses, double* partials,
-                      const LmState* st, int ignore_halt) {
+__global__ void __launch_bounds__(kThreads) KCost(DeviceProblem P, const double* map,
+                                                  const double* poses, double* partials,
+                                                  const LmState* st, int ignore_halt) {
   if (ignore_halt == 0 && Halted(st)) return;
   __shared__ double red[kThreads];
   const GridParams g = P.grid;
@@ -1375,12 +1516,28 @@ void LaunchExtendAddUpdate(const DeviceAnalysis& A, const DeviceWork& W, const i
   KExtendAddUpdate<<<dim3(max_cols, count), kThreads, 0, s>>>(A, W, children);
 }
 
+// Above this many TRSM workgroups a level factors its diagonal blocks in a
+// launch of their own instead of once per workgroup.
+constexpr int kStagedMinBlocks = 512;
+
 void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                       int count, int max_n, int max_f, int max_m, hipStream_t s) {
+                       int count, int max_n, int max_f, int max_m, hipStream_t s,
+                       StepEvents* step_events) {
   const int nblocks = (max_n + kNB - 1) / kNB;
   for (int kb = 0; kb < nblocks; ++kb) {
     const int rows = max_f - kb * kNB;
-    KPanelStep<<<dim3((rows + kStepRows - 1) / kStepRows, count), kThreads, 0, s>>>(A, W, level_sn, kb);
+    const int row_blocks = (rows + kStepRows - 1) / kStepRows;
+    // Few fronts: each workgroup factors the diagonal block itself (one
+    // launch on the critical path). Many fronts: factor each block once.
+    const int staged = row_blocks * count > kStagedMinBlocks ? 1 : 0;
+    const bool timed = step_events != nullptr && step_events->used < step_events->capacity;
+    if (timed) (void)hipEventRecord(step_events->events[2 * step_events->used], s);
+    if (staged != 0) KDiagBlock<<<count, kNB, 0, s>>>(A, W, level_sn, kb);
+    KPanelStep<<<dim3(row_blocks, count), kThreads, 0, s>>>(A, W, level_sn, kb, staged);
+    if (timed) {
+      (void)hipEventRecord(step_events->events[2 * step_events->used + 1], s);
+      ++step_events->used;
+    }
     const int trail_rows = max_f - (kb + 1) * kNB;
     const int trail_cols = max_n - (kb + 1) * kNB;
     const int tr = trail_rows > 0 ? (trail_rows + kTile - 1) / kTile : 1;
@@ -1393,17 +1550,25 @@ void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32
   }
 }
 
+// Levels of at most this many fronts move the L21 products of the solves
+// into launches of their own that spread them over many workgroups.
+constexpr int kSplitSolveFronts = 16;
+
 void LaunchForwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                        int count, int max_f, hipStream_t s) {
+                        int count, int max_m, int max_f, hipStream_t s) {
   // Few large fronts: one 1024-thread workgroup each for more loads in flight.
   const int threads = count <= 32 ? 1024 : kThreads;
-  KForward<<<count, threads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn);
+  const int tri = count <= kSplitSolveFronts && max_m > 0 ? 1 : 0;
+  KForward<<<count, threads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn, tri);
+  if (tri != 0) KForwardRest<<<dim3((max_m + 63) / 64, count), 64, 0, s>>>(A, W, level_sn);
 }
 
 void LaunchBackwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                         int count, int max_f, hipStream_t s) {
+                         int count, int max_n, int max_m, int max_f, hipStream_t s) {
   const int threads = count <= 32 ? 1024 : kThreads;
-  KBackward<<<count, threads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn);
+  const int tri = count <= kSplitSolveFronts && max_m > 0 ? 1 : 0;
+  if (tri != 0) KBackwardRest<<<dim3((max_n + 3) / 4, count), kThreads, 0, s>>>(A, W, level_sn);
+  KBackward<<<count, threads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn, tri);
 }
 
 void LaunchCost(const DeviceProblem& P, const DeviceWork& W, const double* map,

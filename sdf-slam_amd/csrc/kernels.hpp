@@ -106,6 +106,7 @@ struct DeviceWork {
   double* partials;
   double* cost_history;
   int32_t* pose_overflow;  // per frame: 1 when one of its points left the footprint
+  double* solve_tmp;       // L21^T x2 per column (elimination order), backward solve
   LmState* state;
 };
 
@@ -166,12 +167,20 @@ void LaunchExtendAddPanel(const DeviceAnalysis& A, const DeviceWork& W, const in
                           int count, int max_cols, hipStream_t s);
 void LaunchExtendAddUpdate(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
                            int count, int max_cols, hipStream_t s);
+/// Optional timing of the panel steps: pairs (events[2k], events[2k+1])
+/// bracket the diagonal-block and TRSM kernels of one 64-column step.
+struct StepEvents {
+  hipEvent_t* events;
+  int capacity;  // pairs
+  int used;
+};
 void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                       int count, int max_n, int max_f, int max_m, hipStream_t s);
+                       int count, int max_n, int max_f, int max_m, hipStream_t s,
+                       StepEvents* step_events = nullptr);
 void LaunchForwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                        int count, int max_f, hipStream_t s);
+                        int count, int max_m, int max_f, hipStream_t s);
 void LaunchBackwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                         int count, int max_f, hipStream_t s);
+                         int count, int max_n, int max_m, int max_f, hipStream_t s);
 
 void LaunchAssembleEikonalJ(const DeviceProblem& P, const DeviceWork& W, double* j_eik,
                             hipStream_t s);

@@ -11,6 +11,7 @@
 #include "problem.hpp"
 #include "runner.hpp"
 #include "scene.hpp"
+#include "server.hpp"
 #include "sdfslam_host.h"
 #include "solver.hpp"
 
@@ -22,9 +23,25 @@ struct sdfslam_host_problem {
   bool has_workload{false};
 };
 
+struct sdfslam_host_dataset {
+  Workload workload;
+};
+
+struct sdfslam_server {
+  std::unique_ptr<Server> server;
+  std::vector<GridMap> maps;  // grid of each session, for state queries
+};
+
 namespace {
 
 thread_local std::string g_error;
+
+std::vector<Vec2> ScanFrom(const double* xy, int64_t n) {
+  if (n < 0 || (n > 0 && xy == nullptr)) throw std::invalid_argument("bad scan arrays");
+  std::vector<Vec2> pts(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) pts[static_cast<size_t>(i)] = {xy[2 * i], xy[2 * i + 1]};
+  return pts;
+}
 
 int Fail(int code, const std::string& message) {
   g_error = message;
@@ -81,6 +98,26 @@ void ToC(const SolveResult& r, sdfslam_solve_result* result) {
   }
 }
 
+void ToCProblem(const ProblemOptions& d, sdfslam_problem_options* out) {
+  out->weight_scan = d.weights.scan;
+  out->weight_hallucination = d.weights.hallucination;
+  out->weight_eikonal = d.weights.eikonal;
+  out->weight_odometry = d.weights.odometry;
+  out->weight_relation = d.weights.relation;
+  out->hallucination_points_per_scan_point = d.hallucination.points_per_scan_point;
+  out->hallucination_step_size = d.hallucination.step_size;
+  out->hallucination_both_directions = d.hallucination.both_directions ? 1 : 0;
+  out->normals_method = static_cast<int32_t>(d.normals.method);
+  out->normals_k_neighbors = d.normals.k_neighbors;
+  out->normals_per_scan = d.normals.per_scan ? 1 : 0;
+  out->normals_corner_threshold = d.normals.corner_threshold;
+  out->huber_delta = d.huber_delta;
+  out->smooth_gradient = d.smooth_gradient ? 1 : 0;
+  out->smooth_gradient_step = d.smooth_gradient_step;
+  out->active_region = d.active_region ? 1 : 0;
+  out->active_margin = d.active_margin;
+}
+
 ProblemOptions FromC(const sdfslam_problem_options& o) {
   ProblemOptions p;
   p.weights.scan = o.weight_scan;
@@ -129,24 +166,7 @@ extern "C" {
 const char* sdfslam_host_last_error(void) { return g_error.c_str(); }
 
 void sdfslam_host_default_problem_options(sdfslam_problem_options* out) {
-  const ProblemOptions d;
-  out->weight_scan = d.weights.scan;
-  out->weight_hallucination = d.weights.hallucination;
-  out->weight_eikonal = d.weights.eikonal;
-  out->weight_odometry = d.weights.odometry;
-  out->weight_relation = d.weights.relation;
-  out->hallucination_points_per_scan_point = d.hallucination.points_per_scan_point;
-  out->hallucination_step_size = d.hallucination.step_size;
-  out->hallucination_both_directions = d.hallucination.both_directions ? 1 : 0;
-  out->normals_method = static_cast<int32_t>(d.normals.method);
-  out->normals_k_neighbors = d.normals.k_neighbors;
-  out->normals_per_scan = d.normals.per_scan ? 1 : 0;
-  out->normals_corner_threshold = d.normals.corner_threshold;
-  out->huber_delta = d.huber_delta;
-  out->smooth_gradient = d.smooth_gradient ? 1 : 0;
-  out->smooth_gradient_step = d.smooth_gradient_step;
-  out->active_region = d.active_region ? 1 : 0;
-  out->active_margin = d.active_margin;
+  ToCProblem(ProblemOptions{}, out);
 }
 
 void sdfslam_host_default_solver_options(sdfslam_solver_options* out) {
@@ -426,6 +446,196 @@ int sdfslam_host_parse_solver_yaml(const char* path, sdfslam_solver_options* out
     *out = ToC(o);
     std::strncpy(backend, backend_name.c_str(), static_cast<size_t>(backend_capacity) - 1);
     backend[backend_capacity - 1] = '\0';
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_host_dataset_create(const char* workload, uint64_t seed, sdfslam_host_dataset** out) {
+  return Guard([&]() -> int {
+    if (workload == nullptr || out == nullptr) throw std::invalid_argument("null argument");
+    auto d = std::make_unique<sdfslam_host_dataset>();
+    d->workload = MakeWorkload(workload, seed);
+    *out = d.release();
+    return SDFSLAM_OK;
+  });
+}
+
+void sdfslam_host_dataset_destroy(sdfslam_host_dataset* dataset) { delete dataset; }
+
+int sdfslam_host_dataset_info(const sdfslam_host_dataset* dataset, int32_t* num_frames,
+                              sdfslam_grid* grid, sdfslam_problem_options* problem,
+                              sdfslam_solver_options* step) {
+  return Guard([&]() -> int {
+    if (dataset == nullptr) throw std::invalid_argument("null argument");
+    const Workload& wl = dataset->workload;
+    if (num_frames != nullptr) *num_frames = static_cast<int32_t>(wl.scans.size());
+    if (grid != nullptr) {
+      *grid = {wl.map.nx(), wl.map.ny(), wl.map.min_corner().x, wl.map.min_corner().y, wl.map.dx(),
+               wl.map.dy()};
+    }
+    if (problem != nullptr) ToCProblem(wl.problem, problem);
+    if (step != nullptr) {
+      SolverOptions o = wl.solver;
+      o.max_iterations = wl.iterations_per_increment;
+      *step = ToC(o);
+    }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_host_dataset_frame(const sdfslam_host_dataset* dataset, int32_t frame, double* scan_xy,
+                               int64_t capacity, int64_t* num_points, double* pose,
+                               double* odometry) {
+  return Guard([&]() -> int {
+    if (dataset == nullptr) throw std::invalid_argument("null argument");
+    const Workload& wl = dataset->workload;
+    if (frame < 0 || frame >= static_cast<int32_t>(wl.scans.size())) {
+      throw std::invalid_argument("frame out of range");
+    }
+    const auto& pts = wl.scans[static_cast<size_t>(frame)].points;
+    if (num_points != nullptr) *num_points = static_cast<int64_t>(pts.size());
+    if (scan_xy != nullptr) {
+      const size_t n = std::min(pts.size(), static_cast<size_t>(std::max<int64_t>(capacity, 0)));
+      for (size_t i = 0; i < n; ++i) {
+        scan_xy[2 * i] = pts[i].x;
+        scan_xy[2 * i + 1] = pts[i].y;
+      }
+    }
+    if (pose != nullptr) {
+      const Pose2& p = wl.dataset_poses[static_cast<size_t>(frame)];
+      pose[0] = p.x;
+      pose[1] = p.y;
+      pose[2] = p.theta;
+    }
+    if (odometry != nullptr) {
+      const Pose2 o = frame == 0 ? Pose2{0.0, 0.0, 0.0} : wl.odometry[static_cast<size_t>(frame) - 1];
+      odometry[0] = o.x;
+      odometry[1] = o.y;
+      odometry[2] = o.theta;
+    }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_server_create(int32_t workers, int64_t device_budget_bytes, sdfslam_server** out) {
+  return Guard([&]() -> int {
+    if (out == nullptr) throw std::invalid_argument("null argument");
+    ServerOptions o;
+    o.workers = workers;
+    o.device_budget_bytes = device_budget_bytes;
+    auto s = std::make_unique<sdfslam_server>();
+    s->server = std::make_unique<Server>(o);
+    *out = s.release();
+    return SDFSLAM_OK;
+  });
+}
+
+void sdfslam_server_destroy(sdfslam_server* server) { delete server; }
+
+int sdfslam_server_open_session(sdfslam_server* server, const sdfslam_grid* grid,
+                                const double* map_values, const sdfslam_problem_options* problem,
+                                const sdfslam_solver_options* step, const double* first_pose,
+                                const double* scan_xy, int64_t num_points, int32_t* session) {
+  return Guard([&]() -> int {
+    if (server == nullptr || grid == nullptr || problem == nullptr || step == nullptr ||
+        first_pose == nullptr || session == nullptr) {
+      throw std::invalid_argument("null argument");
+    }
+    if (grid->nx < 2 || grid->ny < 2) throw std::invalid_argument("grid needs nx, ny >= 2");
+    GridMap map(grid->nx, grid->ny, {grid->min_x, grid->min_y},
+                {grid->min_x + grid->dx * (grid->nx - 1), grid->min_y + grid->dy * (grid->ny - 1)}, 0.0);
+    if (map_values != nullptr) {
+      std::copy(map_values, map_values + map.num_nodes(), map.values().begin());
+    }
+    Scan first;
+    first.points = ScanFrom(scan_xy, num_points);
+    try {
+      *session = server->server->OpenSession(map, FromC(*problem), FromC(*step),
+                                             {first_pose[0], first_pose[1], first_pose[2]}, first);
+    } catch (const std::runtime_error& e) {
+      if (std::string(e.what()).rfind("admission refused", 0) == 0) {
+        return Fail(SDFSLAM_ERR_OOM, e.what());
+      }
+      throw;
+    }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_server_submit(sdfslam_server* server, int32_t session, const double* scan_xy,
+                          int64_t num_points, const double* odometry, int64_t* ticket) {
+  return Guard([&]() -> int {
+    if (server == nullptr || odometry == nullptr || ticket == nullptr) {
+      throw std::invalid_argument("null argument");
+    }
+    Scan scan;
+    scan.points = ScanFrom(scan_xy, num_points);
+    *ticket = server->server->Submit(session, scan, {odometry[0], odometry[1], odometry[2]},
+                                     std::chrono::steady_clock::now());
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_server_wait(sdfslam_server* server, int32_t session, int64_t ticket,
+                        sdfslam_increment_record* record, double* latency_seconds) {
+  return Guard([&]() -> int {
+    if (server == nullptr) throw std::invalid_argument("null argument");
+    const IncrementRecord r = server->server->Wait(session, ticket);
+    if (record != nullptr) {
+      *record = {r.frames, r.iterations, r.initial_cost, r.final_cost, r.build_seconds, r.solve_seconds};
+    }
+    if (latency_seconds != nullptr) *latency_seconds = server->server->Latency(session, ticket);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_server_state(sdfslam_server* server, int32_t session, int32_t capacity,
+                         int32_t* frames, double* poses, double* map_values) {
+  return Guard([&]() -> int {
+    if (server == nullptr) throw std::invalid_argument("null argument");
+    std::vector<Pose2> est;
+    GridMap map;
+    server->server->State(session, est, map);
+    if (frames != nullptr) *frames = static_cast<int32_t>(est.size());
+    if (poses != nullptr) {
+      const size_t n = std::min(est.size(), static_cast<size_t>(std::max(capacity, 0)));
+      for (size_t i = 0; i < n; ++i) {
+        poses[3 * i] = est[i].x;
+        poses[3 * i + 1] = est[i].y;
+        poses[3 * i + 2] = est[i].theta;
+      }
+    }
+    if (map_values != nullptr) std::copy(map.values().begin(), map.values().end(), map_values);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam_served_result* out) {
+  return Guard([&]() -> int {
+    if (options == nullptr || out == nullptr) throw std::invalid_argument("null argument");
+    ServedBenchmark b;
+    b.sessions = options->sessions;
+    b.rate_hz = options->rate_hz;
+    b.duration_s = options->duration_s;
+    b.workers = options->workers;
+    b.seed0 = options->seed0;
+    const ServedResult r = RunServedBenchmark(b);
+    *out = {r.increments, r.lm_iterations, r.wall_seconds, r.increments_per_second, r.latency_p50_ms,
+            r.latency_p99_ms, r.latency_mean_ms, r.build_mean_ms, r.solve_mean_ms, r.device_bytes};
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_host_batch_benchmark(const sdfslam_batch_options* options, sdfslam_batch_result* out) {
+  return Guard([&]() -> int {
+    if (options == nullptr || out == nullptr) throw std::invalid_argument("null argument");
+    BatchBenchmark b;
+    b.instances = options->instances;
+    b.iterations = options->iterations;
+    b.workers = options->workers;
+    b.seed0 = options->seed0;
+    const BatchResult r = RunBatchBenchmark(b);
+    *out = {r.lm_iterations, r.unknowns, r.wall_seconds, r.lm_iterations_per_second};
     return SDFSLAM_OK;
   });
 }

@@ -29,6 +29,13 @@ HipSession::~HipSession() {
   if (ctx_ != nullptr) (void)sdfslam_hip_context_destroy(ctx_);
 }
 
+int64_t HipSession::DeviceBytes() const {
+  if (ctx_ == nullptr) return 0;
+  sdfslam_hip_analysis_info info{};
+  if (sdfslam_hip_analysis(ctx_, &info) != SDFSLAM_OK) return 0;
+  return info.device_bytes;
+}
+
 SolveResult HipSession::Solve(Problem& problem, const SolverOptions& options) {
   const auto t0 = std::chrono::steady_clock::now();
   const sdfslam_problem_view view = problem.View();

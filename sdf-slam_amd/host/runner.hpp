@@ -30,6 +30,8 @@ class HipSession {
   /// SolveHip); the first call creates the context, later calls replace its
   /// problem and keep its memory.
   SolveResult Solve(Problem& problem, const SolverOptions& options);
+  /// Device memory the context holds (0 before the first Solve).
+  int64_t DeviceBytes() const;
 
  private:
   sdfslam_hip_context* ctx_{nullptr};

@@ -488,6 +488,49 @@ Workload MakeWorkload(const std::string& name, uint64_t seed) {
     }
     return wl;
   }
+  if (name.rfind("batch", 0) == 0 || name.rfind("served", 0) == 0) {
+    // Uneven instances of the batched and served forms: the seed picks the
+    // grid (48..160 nodes a side, cells 0.1..0.36 m) for "batch", or
+    // (100..150 nodes, 20..50 m floorplans) for "served", the pose count,
+    // and 180 or 360 beams; the suffix number offsets the seed.
+    const bool batch = name[0] == 'b';
+    const std::string digits = name.substr(batch ? 5 : 6);
+    const uint64_t inst = digits.empty() ? 0 : std::stoull(digits);
+    Rng pick(seed * 0x9E3779B97F4A7C15ULL + inst);
+    OfficeParams prm;
+    if (batch) {
+      prm.grid = 48 + static_cast<int>(pick.Uniform() * 113.0);
+      const double cell = pick.Uniform(0.1, 0.36);
+      prm.side = cell * (prm.grid - 1) / 1.074;
+      prm.num_poses = 20 + static_cast<int>(pick.Uniform() * 281.0);
+      prm.beams = pick.Uniform() < 0.5 ? 180 : 360;
+    } else {
+      prm.grid = 100 + static_cast<int>(pick.Uniform() * 51.0);
+      prm.side = pick.Uniform(20.0, 50.0);
+      prm.num_poses = 400;
+      prm.beams = 180;
+    }
+    prm.blocks = std::max(1, static_cast<int>(std::lround(prm.side / 16.0)));
+    prm.max_range = std::min(30.0, prm.side);
+    Workload wl = MakeOffice(name, seed + inst, prm);
+    Rng warm((seed + inst) ^ 0x5EEDF00DULL);
+    if (batch) {
+      wl.problem.hallucination = {4, 0.05, true};
+      wl.problem.weights.eikonal = 0.01;
+      wl.solver.step_tolerance = 0.0;
+      wl.solver.max_iterations = 30;
+      WarmStart(wl, 0.05, 0.01, warm);
+    } else {
+      wl.problem.hallucination = {6, 0.1, true};
+      wl.problem.weights.eikonal = 0.018;
+      wl.solver.step_tolerance = 1e-3;
+      wl.solver.max_iterations = 5;
+      wl.incremental = true;
+      wl.iterations_per_increment = 5;
+      wl.initial = wl.dataset_poses;
+    }
+    return wl;
+  }
   if (name == "corner") {
     // tests/test_e2e.cpp:15-61.
     Workload wl;

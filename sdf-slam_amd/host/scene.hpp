@@ -67,6 +67,11 @@ struct Workload {
 ///   "room_small", "office_small"  reduced versions for quick parity checks
 ///   "corner"      two corner scans with a perturbed second frame
 ///                 (the reference's e2e setup, tests/test_e2e.cpp:15-61)
+///   "batch<k>"    uneven batch instance k: 48..160 nodes a side at
+///                 0.1..0.36 m, 20..300 poses, 180 or 360 beams, warm start,
+///                 30 iterations
+///   "served<k>"   incremental session k: 20..50 m office, 100..150 nodes a
+///                 side, 400 frames, 5 iterations per increment
 Workload MakeWorkload(const std::string& name, uint64_t seed);
 
 /// Ray-casts one beam against the walls; returns the hit distance or a

@@ -1,0 +1,137 @@
+// Served and batched forms of the device backend.
+//
+// The reference solves one Problem per call (Solve, src/solvers/solver.hpp:67)
+// and runs one incremental session per process (src/pipeline/runner.cpp:
+// 226-285). The server keeps many incremental sessions on one GPU: each
+// session owns a device context (its own stream, memory reused from one
+// increment to the next), and a pool of host workers runs the increments of
+// different sessions concurrently (continuous batching across sessions),
+// while each session's increments run one at a time in submission order, so
+// a session's results do not depend on what else is being served.
+//
+// Admission: a session is opened only while the device memory of all open
+// sessions plus an estimate for the new one stays within the budget.
+#pragma once
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "runner.hpp"
+
+namespace sdf_slam {
+
+struct ServerOptions {
+  int workers{8};                  // host threads running increments
+  int64_t device_budget_bytes{0};  // 0: 80 % of the device's memory
+};
+
+class Server {
+ public:
+  explicit Server(const ServerOptions& options);
+  ~Server();
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+
+  /// Opens a session at its first frame (pose and scan, no solve yet).
+  /// Throws std::runtime_error("admission refused: ...") when the device
+  /// memory budget would be exceeded.
+  int OpenSession(GridMap map, const ProblemOptions& problem, const SolverOptions& step,
+                  const Pose2& first_pose, const Scan& first_scan);
+  /// Queues the next frame of a session (scan and odometry relative to the
+  /// previous frame); `arrival` is the steady-clock time latency counts from.
+  /// Returns the increment's ticket (1, 2, ... per session).
+  int64_t Submit(int session, const Scan& scan, const Pose2& odometry,
+                 std::chrono::steady_clock::time_point arrival);
+  /// Blocks until the increment with `ticket` of `session` is done and
+  /// returns its record; rethrows the increment's failure.
+  IncrementRecord Wait(int session, int64_t ticket);
+  /// Latency of a finished increment (completion minus arrival), seconds.
+  double Latency(int session, int64_t ticket);
+  /// Current estimate of a session (after its last finished increment).
+  void State(int session, std::vector<Pose2>& poses, GridMap& map);
+  /// Device memory held by all sessions.
+  int64_t DeviceBytes();
+
+ private:
+  struct Pending {
+    Scan scan;
+    Pose2 odometry;
+    std::chrono::steady_clock::time_point arrival;
+  };
+  struct Session {
+    GridMap map;
+    std::vector<Pose2> estimated;
+    std::vector<Scan> scans;
+    std::vector<Pose2> odometry;
+    ProblemOptions problem;
+    SolverOptions step;
+    HipSession hip;
+    std::deque<Pending> pending;
+    bool busy{false};
+    int64_t submitted{0};
+    std::vector<IncrementRecord> records;
+    std::vector<double> latency;
+    std::vector<std::string> errors;
+    int64_t device_bytes{0};
+  };
+
+  void WorkerLoop();
+  void RunIncrement(Session& s, Pending p, IncrementRecord& rec);
+
+  ServerOptions options_;
+  std::mutex mu_;
+  std::condition_variable work_cv_;
+  std::condition_variable done_cv_;
+  std::vector<std::unique_ptr<Session>> sessions_;
+  std::vector<std::thread> workers_;
+  bool stop_{false};
+};
+
+/// Served benchmark: `sessions` incremental sessions (synthetic floorplans,
+/// seeds seed0 + i), each receiving frames as a Poisson process of
+/// `rate_hz` for `duration_s` seconds of wall time, served by `workers`.
+struct ServedBenchmark {
+  int sessions{32};
+  double rate_hz{10.0};
+  double duration_s{10.0};
+  int workers{8};
+  uint64_t seed0{2000};
+};
+struct ServedResult {
+  int64_t increments{0};
+  int64_t lm_iterations{0};
+  double wall_seconds{0.0};  // first arrival to last completion
+  double increments_per_second{0.0};
+  double latency_p50_ms{0.0};
+  double latency_p99_ms{0.0};
+  double latency_mean_ms{0.0};
+  double build_mean_ms{0.0};
+  double solve_mean_ms{0.0};
+  int64_t device_bytes{0};
+};
+ServedResult RunServedBenchmark(const ServedBenchmark& options);
+
+/// Batched benchmark: `instances` independent batch problems (seeds
+/// seed0 + i, uneven sizes), each solved for `iterations` LM iterations,
+/// `workers` at a time. Host problem construction is done before the clock
+/// starts; the clock covers device setup and the iterations.
+struct BatchBenchmark {
+  int instances{64};
+  int iterations{30};
+  int workers{8};
+  uint64_t seed0{1000};
+};
+struct BatchResult {
+  int64_t lm_iterations{0};
+  int64_t unknowns{0};
+  double wall_seconds{0.0};
+  double lm_iterations_per_second{0.0};
+};
+BatchResult RunBatchBenchmark(const BatchBenchmark& options);
+
+}  // namespace sdf_slam

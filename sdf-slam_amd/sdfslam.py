@@ -69,14 +69,16 @@ class AnalysisInfo(C.Structure):
                 ("pattern_nnz_lower", C.c_int64), ("num_supernodes", C.c_int64),
                 ("factor_nnz", C.c_int64), ("factor_flops", C.c_double),
                 ("update_doubles", C.c_int64), ("tree_levels", C.c_int32),
-                ("pattern_epochs", C.c_int32), ("analysis_seconds", C.c_double)]
+                ("pattern_epochs", C.c_int32), ("analysis_seconds", C.c_double),
+                ("device_bytes", C.c_int64), ("panel_step_launches", C.c_int32),
+                ("panel_step_flops", C.c_double)]
 
 
 class PhaseTimes(C.Structure):
     _fields_ = [("assembly_ms", C.c_double), ("normal_ms", C.c_double),
                 ("factor_ms", C.c_double), ("solve_ms", C.c_double),
                 ("update_ms", C.c_double), ("total_ms", C.c_double),
-                ("point_assembly_kernel_ms", C.c_double)]
+                ("point_assembly_kernel_ms", C.c_double), ("panel_step_kernel_ms", C.c_double)]
 
 
 class ProblemOptions(C.Structure):
@@ -101,6 +103,29 @@ class IncrementRecord(C.Structure):
                 ("solve_seconds", C.c_double)]
 
 
+class ServedOptions(C.Structure):
+    _fields_ = [("sessions", C.c_int32), ("rate_hz", C.c_double), ("duration_s", C.c_double),
+                ("workers", C.c_int32), ("seed0", C.c_uint64)]
+
+
+class ServedResult(C.Structure):
+    _fields_ = [("increments", C.c_int64), ("lm_iterations", C.c_int64),
+                ("wall_seconds", C.c_double), ("increments_per_second", C.c_double),
+                ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
+                ("latency_mean_ms", C.c_double), ("build_mean_ms", C.c_double),
+                ("solve_mean_ms", C.c_double), ("device_bytes", C.c_int64)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("instances", C.c_int32), ("iterations", C.c_int32), ("workers", C.c_int32),
+                ("seed0", C.c_uint64)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("lm_iterations", C.c_int64), ("unknowns", C.c_int64),
+                ("wall_seconds", C.c_double), ("lm_iterations_per_second", C.c_double)]
+
+
 # Solve function of one increment (sdfslam_solve_fn): user, view, options,
 # map values, poses, result -> status.
 SOLVE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(ProblemView), C.POINTER(SolverOptions),
@@ -113,7 +138,7 @@ HIP_SYMBOLS = [
     "sdfslam_hip_get_state", "sdfslam_hip_analysis", "sdfslam_hip_assemble",
     "sdfslam_hip_normal_equations", "sdfslam_hip_linear_step", "sdfslam_hip_cost",
     "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
-    "sdfslam_hip_context_update",
+    "sdfslam_hip_context_update", "sdfslam_hip_device_memory",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -121,6 +146,10 @@ HOST_SYMBOLS = [
     "sdfslam_host_problem_destroy", "sdfslam_host_problem_view", "sdfslam_host_problem_get_state",
     "sdfslam_host_problem_set_state", "sdfslam_host_workload_info", "sdfslam_host_solve",
     "sdfslam_host_parse_solver_yaml", "sdfslam_host_run_incremental",
+    "sdfslam_host_dataset_create", "sdfslam_host_dataset_destroy", "sdfslam_host_dataset_info",
+    "sdfslam_host_dataset_frame", "sdfslam_server_create", "sdfslam_server_destroy",
+    "sdfslam_server_open_session", "sdfslam_server_submit", "sdfslam_server_wait",
+    "sdfslam_server_state", "sdfslam_host_served_benchmark", "sdfslam_host_batch_benchmark",
 ]
 
 
@@ -175,6 +204,25 @@ host.sdfslam_host_solve.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(SolverOpti
 host.sdfslam_host_run_incremental.argtypes = [
     C.c_char_p, C.c_uint64, C.POINTER(SolverOptions), C.c_int32, C.c_int32, SOLVE_FN, C.c_void_p,
     C.POINTER(SolveResult), C.POINTER(IncrementRecord), C.c_int32, _i32p, _f64p, _f64p]
+host.sdfslam_host_dataset_create.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(C.c_void_p)]
+host.sdfslam_host_dataset_destroy.argtypes = [C.c_void_p]
+host.sdfslam_host_dataset_destroy.restype = None
+host.sdfslam_host_dataset_info.argtypes = [C.c_void_p, _i32p, C.POINTER(Grid),
+                                           C.POINTER(ProblemOptions), C.POINTER(SolverOptions)]
+host.sdfslam_host_dataset_frame.argtypes = [C.c_void_p, C.c_int32, _f64p, C.c_int64, _i64p,
+                                            _f64p, _f64p]
+host.sdfslam_server_create.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]
+host.sdfslam_server_destroy.argtypes = [C.c_void_p]
+host.sdfslam_server_destroy.restype = None
+host.sdfslam_server_open_session.argtypes = [C.c_void_p, C.POINTER(Grid), _f64p,
+                                             C.POINTER(ProblemOptions), C.POINTER(SolverOptions),
+                                             _f64p, _f64p, C.c_int64, _i32p]
+host.sdfslam_server_submit.argtypes = [C.c_void_p, C.c_int32, _f64p, C.c_int64, _f64p, _i64p]
+host.sdfslam_server_wait.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.POINTER(IncrementRecord),
+                                     _f64p]
+host.sdfslam_server_state.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, _f64p, _f64p]
+host.sdfslam_host_served_benchmark.argtypes = [C.POINTER(ServedOptions), C.POINTER(ServedResult)]
+host.sdfslam_host_batch_benchmark.argtypes = [C.POINTER(BatchOptions), C.POINTER(BatchResult)]
 host.sdfslam_host_parse_solver_yaml.argtypes = [C.c_char_p, C.POINTER(SolverOptions), C.c_char_p,
                                                 C.c_int32]
 
@@ -503,3 +551,105 @@ def run_incremental(name, seed, options=None, increment_size=1, max_frames=0, so
                for i in range(count.value)]
     return dict(total=_result_dict(total, hist), records=records, poses=poses.reshape(-1, 3),
                 map=map_values)
+
+
+class Dataset:
+    """Frames of a synthetic workload (sdfslam_host_dataset_*)."""
+
+    def __init__(self, name, seed):
+        h = C.c_void_p()
+        _check_host(host.sdfslam_host_dataset_create(name.encode(), seed, C.byref(h)))
+        self._h = h
+        n = C.c_int32()
+        self.grid = Grid()
+        self.problem_options = ProblemOptions()
+        self.step_options = SolverOptions()
+        _check_host(host.sdfslam_host_dataset_info(self._h, C.byref(n), C.byref(self.grid),
+                                                   C.byref(self.problem_options),
+                                                   C.byref(self.step_options)))
+        self.num_frames = n.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            host.sdfslam_host_dataset_destroy(self._h)
+            self._h = None
+
+    def frame(self, k):
+        """(scan points (n, 2), dataset pose (3), odometry from frame k-1 (3))."""
+        n = C.c_int64()
+        _check_host(host.sdfslam_host_dataset_frame(self._h, k, None, 0, C.byref(n), None, None))
+        xy = np.zeros(2 * n.value)
+        pose = np.zeros(3)
+        odo = np.zeros(3)
+        _check_host(host.sdfslam_host_dataset_frame(self._h, k, ptr(xy, _f64p), n.value, C.byref(n),
+                                                     ptr(pose, _f64p), ptr(odo, _f64p)))
+        return xy.reshape(-1, 2), pose, odo
+
+
+class Server:
+    """Served form: many incremental sessions on one GPU (sdfslam_server_*)."""
+
+    def __init__(self, workers=8, device_budget_bytes=0):
+        h = C.c_void_p()
+        _check_host(host.sdfslam_server_create(workers, device_budget_bytes, C.byref(h)))
+        self._h = h
+        self._grids = {}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            host.sdfslam_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def open_session(self, grid, problem_options, step_options, first_pose, scan_xy, map_values=None):
+        sid = C.c_int32()
+        pose = np.ascontiguousarray(first_pose, dtype=np.float64)
+        xy = np.ascontiguousarray(scan_xy, dtype=np.float64).reshape(-1)
+        mv = None if map_values is None else np.ascontiguousarray(map_values, dtype=np.float64)
+        _check_host(host.sdfslam_server_open_session(
+            self._h, C.byref(grid), ptr(mv, _f64p), C.byref(problem_options), C.byref(step_options),
+            ptr(pose, _f64p), ptr(xy, _f64p), len(xy) // 2, C.byref(sid)))
+        self._grids[sid.value] = (grid.nx, grid.ny)
+        return sid.value
+
+    def submit(self, session, scan_xy, odometry):
+        xy = np.ascontiguousarray(scan_xy, dtype=np.float64).reshape(-1)
+        odo = np.ascontiguousarray(odometry, dtype=np.float64)
+        ticket = C.c_int64()
+        _check_host(host.sdfslam_server_submit(self._h, session, ptr(xy, _f64p), len(xy) // 2,
+                                               ptr(odo, _f64p), C.byref(ticket)))
+        return ticket.value
+
+    def wait(self, session, ticket):
+        rec = IncrementRecord()
+        lat = C.c_double()
+        _check_host(host.sdfslam_server_wait(self._h, session, ticket, C.byref(rec), C.byref(lat)))
+        out = {k: getattr(rec, k) for k, _ in IncrementRecord._fields_}
+        out["latency_seconds"] = lat.value
+        return out
+
+    def state(self, session):
+        frames = C.c_int32()
+        _check_host(host.sdfslam_server_state(self._h, session, 0, C.byref(frames), None, None))
+        nx, ny = self._grids[session]
+        poses = np.zeros(3 * frames.value)
+        m = np.zeros(nx * ny)
+        _check_host(host.sdfslam_server_state(self._h, session, frames.value, C.byref(frames),
+                                              ptr(poses, _f64p), ptr(m, _f64p)))
+        return poses.reshape(-1, 3), m
+
+
+def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=8, seed0=2000):
+    o = ServedOptions(sessions, rate_hz, duration_s, workers, seed0)
+    r = ServedResult()
+    _check_host(host.sdfslam_host_served_benchmark(C.byref(o), C.byref(r)))
+    return {k: getattr(r, k) for k, _ in ServedResult._fields_}
+
+
+def batch_benchmark(instances=64, iterations=30, workers=8, seed0=1000):
+    o = BatchOptions(instances, iterations, workers, seed0)
+    r = BatchResult()
+    _check_host(host.sdfslam_host_batch_benchmark(C.byref(o), C.byref(r)))
+    return {k: getattr(r, k) for k, _ in BatchResult._fields_}

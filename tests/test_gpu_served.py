@@ -1,0 +1,50 @@
+"""Served form (many incremental sessions on one GPU) and the batched form:
+a session gives the same results alone and in a mix, the same as the
+incremental runner, and the benchmarks run end to end."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _open(sdfslam, server, name, frames):
+    d = sdfslam.Dataset(name, 2000)
+    xy, pose, _ = d.frame(0)
+    sid = server.open_session(d.grid, d.problem_options, d.step_options, pose, xy)
+    return d, sid, [d.frame(k) for k in range(1, frames)]
+
+
+def test_session_alone_matches_mix_and_runner(sdfslam):
+    print("case served alone vs mix", flush=True)
+    frames = 6
+    alone = sdfslam.Server(workers=1)
+    d0, s0, f0 = _open(sdfslam, alone, "served0", frames)
+    for xy, _, odo in f0:
+        alone.wait(s0, alone.submit(s0, xy, odo))
+    p_alone, m_alone = alone.state(s0)
+    alone.close()
+
+    mix = sdfslam.Server(workers=4)
+    opened = [_open(sdfslam, mix, name, frames) for name in ("served0", "served1", "served2")]
+    tickets = []
+    for k in range(frames - 1):
+        for _, sid, fr in opened:
+            xy, _, odo = fr[k]
+            tickets.append((sid, mix.submit(sid, xy, odo)))
+    recs = [mix.wait(sid, t) for sid, t in tickets]
+    assert all(r["frames"] >= 2 and r["latency_seconds"] > 0 for r in recs)
+    p_mix, m_mix = mix.state(opened[0][1])
+    mix.close()
+    assert np.array_equal(p_alone, p_mix) and np.array_equal(m_alone, m_mix)
+
+    run = sdfslam.run_incremental("served0", 2000, max_frames=frames)
+    assert np.array_equal(run["poses"], p_alone) and np.array_equal(run["map"], m_alone)
+
+
+def test_benchmarks_run(sdfslam):
+    print("case served/batch benchmarks", flush=True)
+    b = sdfslam.batch_benchmark(instances=3, iterations=3, workers=2)
+    assert b["lm_iterations"] == 9 and b["lm_iterations_per_second"] > 0
+    r = sdfslam.served_benchmark(sessions=2, rate_hz=20.0, duration_s=0.4, workers=2)
+    assert r["increments"] > 0 and r["latency_p50_ms"] <= r["latency_p99_ms"]
+    assert r["lm_iterations"] > 0 and r["device_bytes"] > 0

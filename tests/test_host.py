@@ -174,3 +174,32 @@ def test_incremental_driver_composes_odometry(sdfslam, oracle):
     assert len(out["records"]) == 8 and out["total"]["iterations"] == 0
     # totals follow runner.cpp:262-270: first initial cost, concatenated histories
     assert out["total"]["initial_cost"] == 1.0 and len(out["total"]["cost_history"]) == 8
+
+
+def test_server_refuses_sessions_beyond_the_memory_budget(sdfslam):
+    # Admission is decided on the host before any device work.
+    server = sdfslam.Server(workers=1, device_budget_bytes=1)
+    d = sdfslam.Dataset("office_small", 1)
+    xy, pose, _ = d.frame(0)
+    with pytest.raises(sdfslam.SdfSlamError, match="admission refused"):
+        server.open_session(d.grid, d.problem_options, d.step_options, pose, xy)
+    server.close()
+
+
+def test_dataset_frames_follow_the_odometry(sdfslam, oracle):
+    # Frame k's odometry is RelativeOdometry of the dataset poses
+    # (runner.cpp:52-59): composing it onto pose k-1 gives pose k.
+    import ctypes as C
+    d = sdfslam.Dataset("served3", 2000)
+    assert d.num_frames == 400 and d.step_options.max_iterations == 5
+    _, p0, o0 = d.frame(0)
+    assert np.array_equal(o0, np.zeros(3))
+    prev = p0
+    for k in range(1, 20):
+        xy, pk, ok = d.frame(k)
+        assert xy.shape[1] == 2 and len(xy) > 0
+        out = np.zeros(3)
+        f64 = C.POINTER(C.c_double)
+        oracle.lib.oracle_compose(prev.ctypes.data_as(f64), ok.ctypes.data_as(f64), out.ctypes.data_as(f64))
+        assert np.allclose(out, pk, atol=1e-9)
+        prev = pk
