@@ -909,7 +909,7 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(DeviceAnalysis A, DeviceW
 // C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
 // mode 0: trailing panel update after block column kb (K = nb);
 // mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated.
-__global__ void __launch_bounds__(kThreads) KGemmLower(DeviceAnalysis A, DeviceWork W,
+__global__ void __launch_bounds__(kThreads, 4) KGemmLower(DeviceAnalysis A, DeviceWork W,
                                                        const int32_t* level_sn, int kb, int mode) {
   if (Halted(W.state)) return;
   const int ti = blockIdx.x;
@@ -961,9 +961,12 @@ __global__ void __launch_bounds__(kThreads) KGemmLower(DeviceAnalysis A, DeviceW
   const int R0 = ti * kTile;
   const int C0 = tj * kTile;
   if (R0 >= M || C0 >= N) return;
-  __shared__ double As[kTk][kTile + 1];
-  __shared__ double Bs[kTk][kTile + 1];
+  __shared__ double As[2][kTk][kTile];
+  __shared__ double Bs[2][kTk][kTile];
   const int tid = threadIdx.x;
+  // Output mapping: lane tx owns rows tx + 16 i, ty owns columns ty + 16 j,
+  // so a wavefront's C accesses are 16-row runs of 4 columns (coalesced in
+  // the column-major panel / update matrix).
   const int tx = tid & 15;
   const int ty = tid >> 4;
   double acc[4][4];
@@ -971,7 +974,10 @@ __global__ void __launch_bounds__(kThreads) KGemmLower(DeviceAnalysis A, DeviceW
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-  for (int k0 = 0; k0 < K; k0 += kTk) {
+  // Staging of one K chunk: each thread loads 4 A and 4 B entries (rows
+  // rr = idx & 63 of k row kk = idx >> 6); B is scaled by D_k.
+  double ra[4], rb[4];
+  auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int idx = tid + kThreads * e;
@@ -979,34 +985,54 @@ __global__ void __launch_bounds__(kThreads) KGemmLower(DeviceAnalysis A, DeviceW
       const int kk = idx >> 6;
       const int kcol = kbase + k0 + kk;
       const bool kin = k0 + kk < K;
-      const int64_t colbase = static_cast<int64_t>(kcol) * f;
-      const double dk = !kin ? 0.0 : (mode == 0 ? stage[(k0 + kk) * kNB + k0 + kk] : F[colbase + kcol]);
-      As[kk][rr] = (kin && R0 + rr < M) ? F[colbase + rbase + R0 + rr] : 0.0;
-      Bs[kk][rr] = (kin && C0 + rr < N) ? F[colbase + rbase + C0 + rr] * dk : 0.0;
+      const int64_t colbase = static_cast<int64_t>(min(kcol, kbase + K - 1)) * f;
+      const double dk = !kin ? 0.0
+                             : (mode == 0 ? stage[(k0 + kk) * kNB + k0 + kk] : F[colbase + kcol]);
+      ra[e] = (kin && R0 + rr < M) ? F[colbase + rbase + R0 + rr] : 0.0;
+      rb[e] = (kin && C0 + rr < N) ? F[colbase + rbase + C0 + rr] * dk : 0.0;
     }
-    __syncthreads();
+  };
+  auto store_chunk = [&](int buf) {
 #pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + kThreads * e;
+      As[buf][idx >> 6][idx & 63] = ra[e];
+      Bs[buf][idx >> 6][idx & 63] = rb[e];
+    }
+  };
+  load_chunk(0);
+  store_chunk(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = 0; k0 < K; k0 += kTk) {
+    const bool more = k0 + kTk < K;
+    if (more) load_chunk(k0 + kTk);  // in flight while this chunk is used
+#pragma unroll 2
     for (int kk = 0; kk < kTk; ++kk) {
       double av[4], bv[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+      for (int i = 0; i < 4; ++i) av[i] = As[buf][kk][tx + 16 * i];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[buf][kk][ty + 16 * j];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
     }
-    __syncthreads();
+    if (more) {
+      store_chunk(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = R0 + ty + 16 * i;
-    if (r >= M) continue;
+  for (int j = 0; j < 4; ++j) {
+    const int c = C0 + ty + 16 * j;
+    if (c >= N) continue;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = C0 + tx + 16 * j;
-      if (c >= N || c > r) continue;
+    for (int i = 0; i < 4; ++i) {
+      const int r = R0 + tx + 16 * i;
+      if (r >= M || c > r) continue;
       double* dst = C + static_cast<int64_t>(c) * ldc + r;
       *dst = mode == 0 ? *dst - acc[i][j] : -acc[i][j];
     }
