@@ -87,11 +87,14 @@ def incremental_leg(S, frames):
 
 def served_leg(S, duration, plain_duration, workers):
     """32 incremental sessions, Poisson frame arrivals at 10 Hz each; the
-    served form (`workers` concurrent sessions on one GPU) against the plain
-    one-session-at-a-time device path and one session alone."""
+    served form (continuous batching: every waiting session's increment in
+    one batched solve, `workers` host threads, two batches in flight) against
+    the plain path (one increment at a time) and one session alone."""
     full = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=duration, workers=workers)
-    plain = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=plain_duration, workers=1)
-    alone = S.served_benchmark(sessions=1, rate_hz=10.0, duration_s=duration, workers=1)
+    plain = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=plain_duration, workers=1,
+                               max_batch=1, pipelines=1)
+    alone = S.served_benchmark(sessions=1, rate_hz=10.0, duration_s=duration, workers=1,
+                               max_batch=1, pipelines=1)
     r = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
     return {"sessions": 32, "rate_hz_per_session": 10.0, "duration_s": duration, "workers": workers,
             "increments_per_s": full["increments_per_second"],
@@ -101,12 +104,12 @@ def served_leg(S, duration, plain_duration, workers):
             "served": r(full), "plain_one_at_a_time": r(plain), "single_session": r(alone)}
 
 
-def batched_leg(S, workers):
+def batched_leg(S):
     """64 uneven batch instances, 30 LM iterations each: aggregate LM
-    iterations/s with `workers` concurrent instances and one at a time."""
-    full = S.batch_benchmark(instances=64, iterations=30, workers=workers)
-    serial = S.batch_benchmark(instances=64, iterations=30, workers=1)
-    return {"instances": 64, "iterations": 30, "workers": workers,
+    iterations/s with all 64 in one batched solve and one after another."""
+    full = S.batch_benchmark(instances=64, iterations=30, max_batch=64)
+    serial = S.batch_benchmark(instances=64, iterations=30, max_batch=1)
+    return {"instances": 64, "iterations": 30, "max_batch": 64,
             "lm_iters_per_s": full["lm_iterations_per_second"],
             "serial_lm_iters_per_s": serial["lm_iterations_per_second"],
             "unknowns_total": full["unknowns"]}
@@ -225,7 +228,7 @@ def main():
         print("[bench] served leg", file=sys.stderr, flush=True)
         line["served"] = served_leg(S, args.served_duration, args.plain_duration, args.workers)
         print("[bench] batched leg", file=sys.stderr, flush=True)
-        line["batched"] = batched_leg(S, args.workers)
+        line["batched"] = batched_leg(S)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(problem, args.cpu_iterations)
     print(json.dumps(line), flush=True)

@@ -150,9 +150,18 @@ int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx);
  * increment of an incremental run: the problem rebuilt over more frames, as
  * runner.cpp:249-260 rebuilds it before each Solve), keeping the context's
  * stream and device memory. The next context_solve behaves as a Solve on a
- * fresh context created with the new problem and the context's options. */
+ * fresh context created with the new problem and the context's options, to
+ * rounding: when the new problem extends the old one (same grid, more
+ * poses), the earlier poses keep their footprint margins and the appended
+ * ones start wider, so the J^T J pattern superset (and with it the order of
+ * the factorization's sums) can differ from a fresh context's. */
 int sdfslam_hip_context_update(sdfslam_hip_context* ctx, const sdfslam_problem_view* problem,
                                const double* map_values, const double* poses);
+/* Footprint margin (cells) of every pose in the current pattern epoch:
+ * min(capacity, num_poses) entries into margins (may be NULL), the pose
+ * count into *num_poses (may be NULL). */
+int sdfslam_hip_pose_margins(sdfslam_hip_context* ctx, int32_t* margins, int32_t capacity,
+                             int32_t* num_poses);
 int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, const double* poses);
 int sdfslam_hip_get_state(sdfslam_hip_context* ctx, double* map_values, double* poses);
 
@@ -215,6 +224,29 @@ int sdfslam_hip_cost(sdfslam_hip_context* ctx, double* cost);
  * lm_solver.cpp:201-434); the state stays on the device. */
 int sdfslam_hip_context_solve(sdfslam_hip_context* ctx, const sdfslam_solver_options* options,
                               sdfslam_solve_result* result);
+
+/* Runs the LM loop (as sdfslam_hip_context_solve) on `count` contexts at
+ * once: every stage of an iteration is one launch over all of them, so many
+ * small problems share each launch (the batched and served forms). Each
+ * context keeps its own LM state, pattern epochs and result, identical to
+ * solving it alone. statuses[i] (may be NULL) receives SDFSLAM_OK or
+ * SDFSLAM_ERR_NOT_SPD per context; results (may be NULL) one result each.
+ * The contexts must be distinct, on one device, and idle. info (may be NULL)
+ * receives where the call spent its time. No counterpart in the reference
+ * (one Solve per call, solver.hpp:67). */
+typedef struct {
+  int32_t graph_captures;   /* captured iterations (one per change of the batch's buffers) */
+  int32_t graph_launches;   /* launches of the captured iteration */
+  int32_t epochs_rebuilt;   /* pattern re-analyses of any context during the run */
+  double setup_seconds;     /* LM initialisation of every context */
+  double capture_seconds;   /* slot table, launch plans, capture and instantiation */
+  double analysis_seconds;  /* re-analyses (host) with their uploads */
+  double run_seconds;       /* the whole call */
+} sdfslam_hip_batch_info;
+
+int sdfslam_hip_batch_solve(sdfslam_hip_context* const* contexts, int32_t count,
+                            const sdfslam_solver_options* options, sdfslam_solve_result* results,
+                            int32_t* statuses, sdfslam_hip_batch_info* info);
 
 /* Runs exactly `iterations` LM iterations from the current device state,
  * carrying the LM state (lambda, previous cost) in the context and ignoring

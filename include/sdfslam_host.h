@@ -147,12 +147,20 @@ int sdfslam_host_dataset_frame(const sdfslam_host_dataset* dataset, int32_t fram
 /* ---- Served form: many incremental sessions on one GPU --------------------
  * No counterpart in the reference (one session per process, runner.cpp:
  * 226-285). Each session's increments run in submission order on the
- * session's own device context; increments of different sessions run
- * concurrently on `workers` host threads. Sessions are admitted while the
+ * session's own device context. Continuous batching: a scheduler takes every
+ * idle session with a waiting frame (up to max_batch), prepares them on
+ * `workers` host threads and solves them in one sdfslam_hip_batch_solve;
+ * `pipelines` batches are in flight at once. Sessions are admitted while the
  * device memory of all sessions stays within device_budget_bytes (0: 80 %
  * of the device). */
 typedef struct sdfslam_server sdfslam_server;
-int sdfslam_server_create(int32_t workers, int64_t device_budget_bytes, sdfslam_server** out);
+typedef struct sdfslam_server_options {
+  int32_t workers;
+  int32_t max_batch;
+  int32_t pipelines;
+  int64_t device_budget_bytes;
+} sdfslam_server_options;
+int sdfslam_server_create(const sdfslam_server_options* options, sdfslam_server** out);
 /* Waits for the increments in flight, then frees every session. */
 void sdfslam_server_destroy(sdfslam_server* server);
 /* Opens a session at its first frame. map_values: nx*ny initial values
@@ -181,6 +189,8 @@ typedef struct sdfslam_served_options {
   double rate_hz;
   double duration_s;
   int32_t workers;
+  int32_t max_batch;
+  int32_t pipelines;
   uint64_t seed0;
 } sdfslam_served_options;
 typedef struct sdfslam_served_result {
@@ -194,15 +204,25 @@ typedef struct sdfslam_served_result {
   double build_mean_ms;
   double solve_mean_ms;
   int64_t device_bytes;
+  /* Where the server spent its time, per batched solve (means). */
+  int64_t batches;
+  double mean_batch;         /* increments per batched solve */
+  double prepare_batch_ms;   /* host problems + context loads (parallel) */
+  double solve_batch_ms;     /* the batched solve call */
+  double collect_batch_ms;   /* states back into the sessions (parallel) */
+  double capture_batch_ms;   /* inside the solve: slot table, plans, graph capture */
+  double analysis_batch_ms;  /* inside the solve: pattern re-analyses */
+  int64_t epochs_rebuilt;    /* re-analyses inside batched solves */
 } sdfslam_served_result;
 int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam_served_result* out);
 
 /* Batched benchmark: `instances` uneven batch problems ("batch<i>", seed
- * seed0), `iterations` LM iterations each, `workers` at a time. */
+ * seed0), `iterations` LM iterations each, max_batch per batched solve
+ * (1: one after another). */
 typedef struct sdfslam_batch_options {
   int32_t instances;
   int32_t iterations;
-  int32_t workers;
+  int32_t max_batch;
   uint64_t seed0;
 } sdfslam_batch_options;
 typedef struct sdfslam_batch_result {

@@ -248,15 +248,70 @@ struct HostProblem {
   int64_t rows() const { return view.num_points + view.num_eikonal + 3 * view.num_odom; }
 };
 
-struct ChildList {
-  int64_t offset;
-  int count;
-  int max_m;
-};
-
 constexpr int kPhases = 5;
 
 }  // namespace
+
+using SteadyTime = std::chrono::steady_clock::time_point;
+static SteadyTime Now() { return std::chrono::steady_clock::now(); }
+static double SecondsSince(SteadyTime t0) { return std::chrono::duration<double>(Now() - t0).count(); }
+
+/// Buffers of the batches a context leads (grown, never shrunk): the slot
+/// table and launch plans with their pinned staging, and the gathered LM
+/// states.
+struct BatchSpace {
+  int capacity{0};
+  Slot* table{nullptr};
+  Slot* slots_host{nullptr};
+  LmState* states_dev{nullptr};
+  LmState* states_host{nullptr};
+  size_t plan_capacity{0};
+  ListRef* plan{nullptr};
+  ListRef* plan_host{nullptr};
+
+  void Reserve(int k) {
+    if (k <= capacity) return;
+    Release(false);
+    const int want = std::max(k, 2 * capacity);
+    HIPC(hipMalloc(&table, sizeof(Slot) * static_cast<size_t>(want)));
+    HIPC(hipHostMalloc(&slots_host, sizeof(Slot) * static_cast<size_t>(want), hipHostMallocDefault));
+    HIPC(hipMalloc(&states_dev, sizeof(LmState) * static_cast<size_t>(want)));
+    HIPC(hipHostMalloc(&states_host, sizeof(LmState) * static_cast<size_t>(want), hipHostMallocDefault));
+    capacity = want;
+  }
+  void ReservePlans(size_t entries) {
+    if (entries <= plan_capacity) return;
+    if (plan != nullptr) HIPC(hipFree(plan));
+    if (plan_host != nullptr) HIPC(hipHostFree(plan_host));
+    plan = nullptr;
+    plan_host = nullptr;
+    plan_capacity = 0;
+    const size_t want = std::max(entries, 2 * plan_capacity) + 16;
+    HIPC(hipMalloc(&plan, sizeof(ListRef) * want));
+    HIPC(hipHostMalloc(&plan_host, sizeof(ListRef) * want, hipHostMallocDefault));
+    plan_capacity = want;
+  }
+  /// Frees the slot and state buffers (and the plans when `all`); errors
+  /// are ignored so that this can run from a destructor.
+  void Release(bool all) {
+    if (table != nullptr) (void)hipFree(table);
+    if (slots_host != nullptr) (void)hipHostFree(slots_host);
+    if (states_dev != nullptr) (void)hipFree(states_dev);
+    if (states_host != nullptr) (void)hipHostFree(states_host);
+    table = nullptr;
+    slots_host = nullptr;
+    states_dev = nullptr;
+    states_host = nullptr;
+    capacity = 0;
+    if (all) {
+      if (plan != nullptr) (void)hipFree(plan);
+      if (plan_host != nullptr) (void)hipHostFree(plan_host);
+      plan = nullptr;
+      plan_host = nullptr;
+      plan_capacity = 0;
+    }
+  }
+};
 
 struct sdfslam_hip_context {
   int device{0};
@@ -282,10 +337,16 @@ struct sdfslam_hip_context {
   int graph_generation{-1};
   LmOptions graph_options{};
   int generation{0};
-  int32_t* level_sn{nullptr};
-  std::vector<int64_t> level_off;
-  int32_t* child_lists{nullptr};
-  std::vector<std::vector<ChildList>> level_children;
+  Slot slot{};            // this context's problem as a launch slot
+  Slot* slot_dev{nullptr};  // device copy (a table of one)
+  LaunchShape shape;      // grid sizes of this context alone
+  // Level schedule: fronts per level and extend-add lists per (level, child
+  // rank), on the device with their host offsets, for launch plans.
+  const int32_t* level_sn_d{nullptr};
+  const int32_t* child_lists_d{nullptr};
+  std::vector<int64_t> level_off_h, rank_first_h, rank_off_h;
+  ListRef* plan_dev{nullptr};  // this context's level and rank plans
+  size_t plan_capacity{0};
   int max_f{0};
   hipEvent_t events[kPhases + 3]{};  // phase marks, then the point-row kernel pair
   double phase_ms[kPhases + 1]{};
@@ -295,9 +356,13 @@ struct sdfslam_hip_context {
   int64_t panel_steps_timed{0};
   int timed_iterations{0};
   bool lm_ready{false};
+  BatchSpace batch;  // buffers of the batches this context leads
 
   ~sdfslam_hip_context() {
+    batch.Release(true);
     if (graph != nullptr) (void)hipGraphExecDestroy(graph);
+    if (slot_dev != nullptr) (void)hipFree(slot_dev);
+    if (plan_dev != nullptr) (void)hipFree(plan_dev);
     for (hipEvent_t& e : events) {
       if (e != nullptr) (void)hipEventDestroy(e);
     }
@@ -408,6 +473,17 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
     c.history_capacity = std::max(want, c.history_capacity);
     W.cost_history = c.hist.Alloc<double>(static_cast<size_t>(c.history_capacity));
   }
+  LaunchShape& sh = c.shape;
+  sh.slots = 1;
+  sh.points = P.num_points;
+  sh.eikonal = P.num_eikonal;
+  sh.odom = P.num_odom;
+  sh.cells = P.num_cells;
+  sh.nodes = static_cast<int64_t>(nmap);
+  sh.n = P.n;
+  sh.state = static_cast<int64_t>(nmap + npose);
+  sh.poses = P.num_poses;
+  sh.step_items = P.num_active + P.num_poses;
   ++c.generation;
 }
 
@@ -430,11 +506,82 @@ void DownloadState(sdfslam_hip_context& c, std::vector<double>& map, std::vector
   HIPC(hipStreamSynchronize(c.stream));
 }
 
+/// Copies the context's current buffers into its launch slot and the slot
+/// to the device (after the problem or the pattern epoch changed).
+void SyncSlot(sdfslam_hip_context& c) {
+  c.slot.P = c.dp;
+  c.slot.A = c.da;
+  c.slot.W = c.dw;
+  c.slot.panel_total = c.an.panel_total;
+  c.slot.nmap = c.hp->view.grid.nx * c.hp->view.grid.ny;
+  if (c.slot_dev == nullptr) HIPC(hipMalloc(&c.slot_dev, sizeof(Slot)));
+  HIPC(hipMemcpyAsync(c.slot_dev, &c.slot, sizeof(Slot), hipMemcpyHostToDevice, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+}
+
+/// Launch plans of a launch over `ctxs` (slot b = ctxs[b]) for the merged
+/// shape `sh`: per level and slot the level's fronts, per (level, child rank)
+/// and slot the extend-add list; sets each level's rank_base.
+void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape& sh,
+                std::vector<ListRef>& level_plan, std::vector<ListRef>& rank_plan) {
+  const size_t k = ctxs.size();
+  const size_t levels = sh.levels.size();
+  level_plan.assign(levels * k, ListRef{nullptr, 0});
+  int ranks = 0;
+  for (LaunchShape::Level& lv : sh.levels) {
+    lv.rank_base = ranks;
+    ranks += static_cast<int>(lv.rank_count.size());
+  }
+  rank_plan.assign(static_cast<size_t>(ranks) * k, ListRef{nullptr, 0});
+  for (size_t b = 0; b < k; ++b) {
+    const sdfslam_hip_context& c = *ctxs[b];
+    const size_t own_levels = c.level_off_h.empty() ? 0 : c.level_off_h.size() - 1;
+    for (size_t L = 0; L < std::min(levels, own_levels); ++L) {
+      const int64_t l0 = c.level_off_h[L];
+      level_plan[L * k + b] = {c.level_sn_d + l0, static_cast<int32_t>(c.level_off_h[L + 1] - l0)};
+      const int64_t e0 = c.rank_first_h[L];
+      const int64_t e1 = c.rank_first_h[L + 1];
+      for (int64_t e = e0; e < e1; ++e) {
+        const size_t r = static_cast<size_t>(e - e0);
+        const size_t slot_index = (static_cast<size_t>(sh.levels[L].rank_base) + r) * k + b;
+        rank_plan[slot_index] = {c.child_lists_d + c.rank_off_h[static_cast<size_t>(e)],
+                                 static_cast<int32_t>(c.rank_off_h[static_cast<size_t>(e) + 1] -
+                                                      c.rank_off_h[static_cast<size_t>(e)])};
+      }
+    }
+  }
+}
+
+/// The context's own launch plans, uploaded next to its slot.
+void UploadPlans(sdfslam_hip_context& c) {
+  std::vector<ListRef> level_plan;
+  std::vector<ListRef> rank_plan;
+  BuildPlans({&c}, c.shape, level_plan, rank_plan);
+  const size_t need = level_plan.size() + rank_plan.size() + 1;
+  if (need > c.plan_capacity) {
+    if (c.plan_dev != nullptr) HIPC(hipFree(c.plan_dev));
+    c.plan_dev = nullptr;
+    c.plan_capacity = 0;
+    HIPC(hipMalloc(&c.plan_dev, sizeof(ListRef) * (need + need / 4)));
+    c.plan_capacity = need + need / 4;
+  }
+  std::vector<ListRef> all(level_plan);
+  all.insert(all.end(), rank_plan.begin(), rank_plan.end());
+  HIPC(hipMemcpyAsync(c.plan_dev, all.data(), sizeof(ListRef) * all.size(), hipMemcpyHostToDevice, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  c.shape.level_plan = c.plan_dev;
+  c.shape.rank_plan = c.plan_dev + level_plan.size();
+  c.shape.level_plan_h = level_plan;
+  c.shape.rank_plan_h = rank_plan;
+}
+
 /// Widens the footprint dilation of every pose whose points left their
 /// footprint in the ending epoch: by one cell, by two when that epoch lasted
 /// at most two iterations, up to kMaxMargin. Poses that stay put keep the
 /// initial margin, so the pattern grows only where the state moves.
 constexpr int kMaxMargin = 8;
+// Margin of the poses an incremental update appends.
+constexpr int kFreshPoseMargin = 5;
 
 void GrowMargins(sdfslam_hip_context& c) {
   const int N = c.hp->view.num_poses;
@@ -480,27 +627,40 @@ void BuildEpoch(sdfslam_hip_context& c) {
     for (const int ch : an.children[static_cast<size_t>(s)]) clist.push_back(ch);
     cptr[static_cast<size_t>(s) + 1] = static_cast<int64_t>(clist.size());
   }
-  // Level lists and, per level and child rank, the children to extend-add.
+  // Level lists and, per level and child rank, the children to extend-add,
+  // with the launch shape of this context.
   std::vector<int32_t> lvl;
-  c.level_off.assign(static_cast<size_t>(an.num_levels) + 1, 0);
+  std::vector<int64_t> level_off(static_cast<size_t>(an.num_levels) + 1, 0);
   std::vector<int32_t> ranks;
-  c.level_children.assign(static_cast<size_t>(an.num_levels), {});
+  std::vector<int64_t> rank_first(static_cast<size_t>(an.num_levels) + 1, 0);
+  std::vector<int64_t> rank_off{0};
+  LaunchShape& sh = c.shape;
+  sh.levels.assign(static_cast<size_t>(an.num_levels), {});
   for (int L = 0; L < an.num_levels; ++L) {
+    LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
     for (const int s : an.level_sn[static_cast<size_t>(L)]) lvl.push_back(s);
-    c.level_off[static_cast<size_t>(L) + 1] = static_cast<int64_t>(lvl.size());
+    level_off[static_cast<size_t>(L) + 1] = static_cast<int64_t>(lvl.size());
+    lv.count = static_cast<int>(an.level_sn[static_cast<size_t>(L)].size());
+    lv.max_nb = an.level_max_nb[static_cast<size_t>(L)];
+    lv.max_f = an.level_max_f[static_cast<size_t>(L)];
+    lv.max_m = an.level_max_m[static_cast<size_t>(L)];
     for (int r = 0; r < an.level_max_children[static_cast<size_t>(L)]; ++r) {
-      ChildList cl{static_cast<int64_t>(ranks.size()), 0, 0};
+      int count = 0;
+      int max_m = 0;
       for (const int s : an.level_sn[static_cast<size_t>(L)]) {
         const auto& ch = an.children[static_cast<size_t>(s)];
         if (static_cast<int>(ch.size()) > r) {
           const int child = ch[static_cast<size_t>(r)];
           ranks.push_back(child);
-          ++cl.count;
-          cl.max_m = std::max(cl.max_m, an.sn_m[static_cast<size_t>(child)]);
+          ++count;
+          max_m = std::max(max_m, an.sn_m[static_cast<size_t>(child)]);
         }
       }
-      c.level_children[static_cast<size_t>(L)].push_back(cl);
+      rank_off.push_back(static_cast<int64_t>(ranks.size()));
+      lv.rank_count.push_back(count);
+      lv.rank_max_m.push_back(max_m);
     }
+    rank_first[static_cast<size_t>(L) + 1] = static_cast<int64_t>(rank_off.size()) - 1;
   }
   EpochPool& E = c.epoch;
   DeviceAnalysis& D = c.da;
@@ -541,8 +701,8 @@ void BuildEpoch(sdfslam_hip_context& c) {
     D.op_off = E.Upload(an.op_off);
     D.children_ptr = E.Upload(cptr);
     D.children = E.Upload(clist);
-    c.level_sn = E.Upload(lvl);
-    c.child_lists = E.Upload(ranks);
+    c.level_sn_d = E.Upload(lvl);
+    c.child_lists_d = E.Upload(ranks);
     c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
     c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
     c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 64 * 64);
@@ -553,6 +713,14 @@ void BuildEpoch(sdfslam_hip_context& c) {
   E.BeginPlace();
   layout();
   E.Finish(c.stream);
+  sh.max_points_per_pose = std::max(1, an.max_points_per_pose);
+  sh.panel_total = an.panel_total;
+  sh.pairs = static_cast<int64_t>(an.op_pose.size() / 2);
+  c.level_off_h = std::move(level_off);
+  c.rank_first_h = std::move(rank_first);
+  c.rank_off_h = std::move(rank_off);
+  SyncSlot(c);
+  UploadPlans(c);
   ++c.generation;
   c.analysis_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -565,11 +733,10 @@ void Mark(sdfslam_hip_context& c, bool timing, int k) {
 
 void EnqueueAssemblyAndNormal(sdfslam_hip_context& c, const LmOptions& o, bool timing) {
   hipStream_t s = c.stream;
-  LaunchAssemble(c.dp, c.da, c.dw, true, s, timing ? c.events[kPhases + 1] : nullptr,
+  LaunchAssemble(SlotArgs{c.slot_dev, &c.slot}, c.shape, true, s, timing ? c.events[kPhases + 1] : nullptr,
                  timing ? c.events[kPhases + 2] : nullptr);
   Mark(c, timing, 1);
-  LaunchFill(c.dw.panel, c.an.panel_total, 0.0, s);
-  LaunchNormalSlots(c.dp, c.da, c.dw, o, c.an.max_points_per_pose, s);
+  LaunchNormalSlots(SlotArgs{c.slot_dev, &c.slot}, c.shape, o, s);
 }
 
 void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
@@ -585,34 +752,10 @@ void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
     }
     step_ev = {c.step_events.data(), static_cast<int>(c.step_events.size() / 2), 0};
   }
-  for (int L = 0; L < an.num_levels; ++L) {
-    const auto& cls = c.level_children[static_cast<size_t>(L)];
-    for (const ChildList& cl : cls) {
-      LaunchExtendAddPanel(c.da, c.dw, c.child_lists + cl.offset, cl.count, cl.max_m, s);
-    }
-    const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
-    const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
-    LaunchFactorLevel(c.da, c.dw, lvl, count, an.level_max_nb[static_cast<size_t>(L)],
-                      an.level_max_f[static_cast<size_t>(L)], an.level_max_m[static_cast<size_t>(L)], s,
-                      timing ? &step_ev : nullptr);
-    for (const ChildList& cl : cls) {
-      LaunchExtendAddUpdate(c.da, c.dw, c.child_lists + cl.offset, cl.count, cl.max_m, s);
-    }
-  }
+  LaunchFactor(SlotArgs{c.slot_dev, &c.slot}, c.shape, s, timing ? &step_ev : nullptr);
   c.step_events_used = step_ev.used;
   Mark(c, timing, 3);
-  for (int L = 0; L < an.num_levels; ++L) {
-    const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
-    const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
-    LaunchForwardLevel(c.da, c.dw, lvl, count, an.level_max_m[static_cast<size_t>(L)],
-                       an.level_max_f[static_cast<size_t>(L)], s);
-  }
-  for (int L = an.num_levels - 1; L >= 0; --L) {
-    const int32_t* lvl = c.level_sn + c.level_off[static_cast<size_t>(L)];
-    const int count = static_cast<int>(c.level_off[static_cast<size_t>(L) + 1] - c.level_off[static_cast<size_t>(L)]);
-    LaunchBackwardLevel(c.da, c.dw, lvl, count, an.level_max_nb[static_cast<size_t>(L)],
-                        an.level_max_m[static_cast<size_t>(L)], an.level_max_f[static_cast<size_t>(L)], s);
-  }
+  LaunchSolve(SlotArgs{c.slot_dev, &c.slot}, c.shape, s);
 }
 
 void EnqueueIteration(sdfslam_hip_context& c, const LmOptions& o, bool timing) {
@@ -621,7 +764,7 @@ void EnqueueIteration(sdfslam_hip_context& c, const LmOptions& o, bool timing) {
   Mark(c, timing, 2);
   EnqueueFactorAndSolve(c, timing);
   Mark(c, timing, 4);
-  LaunchStepAndControl(c.dp, c.dw, o, c.stream);
+  LaunchStepAndControl(SlotArgs{c.slot_dev, &c.slot}, c.shape, o, c.stream);
   Mark(c, timing, 5);
 }
 
@@ -684,8 +827,8 @@ void AccumulateTimes(sdfslam_hip_context& c) {
 }
 
 void InitLm(sdfslam_hip_context& c, double lambda_init) {
-  LaunchCost(c.dp, c.dw, c.dw.map, c.dw.poses, c.stream);
-  LaunchLmInit(c.dw, lambda_init, c.stream);
+  LaunchCost(SlotArgs{c.slot_dev, &c.slot}, c.shape, c.stream);
+  LaunchLmInit(SlotArgs{c.slot_dev, &c.slot}, c.shape, lambda_init, c.stream);
   c.lm_ready = true;
 }
 
@@ -745,6 +888,163 @@ LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations
   return st;
 }
 
+/// LM loops of several contexts with every stage of an iteration launched
+/// once over all of them (their slots in one table, one captured graph on
+/// the lead context's stream, the lead being the first context). Each
+/// context keeps its own LM state, pattern epochs and results: the same as
+/// solving it alone. The table, plans and state buffers live in the lead
+/// context and are reused by its later batches.
+class BatchRun {
+ public:
+  BatchRun(sdfslam_hip_context* const* ctxs, int count)
+      : ctxs_(ctxs, ctxs + count), space_(ctxs[0]->batch) {}
+  ~BatchRun() {
+    if (graph_ != nullptr) (void)hipGraphExecDestroy(graph_);
+  }
+  BatchRun(const BatchRun&) = delete;
+  BatchRun& operator=(const BatchRun&) = delete;
+
+  const sdfslam_hip_batch_info& info() const { return info_; }
+
+  /// Runs every context's LM loop to its end; status per context.
+  void Run(const sdfslam_solver_options& options, std::vector<int>& status,
+           std::vector<LmState>& states) {
+    const int k = static_cast<int>(ctxs_.size());
+    stream_ = ctxs_[0]->stream;
+    const auto t_setup = Now();
+    int capacity = INT_MAX;
+    for (sdfslam_hip_context* c : ctxs_) {
+      HIPC(hipStreamSynchronize(c->stream));
+      if (options.max_iterations + 1 > c->history_capacity) {
+        c->history_capacity = options.max_iterations + 1;
+        c->hist.Clear();
+        c->dw.cost_history = c->hist.Alloc<double>(static_cast<size_t>(c->history_capacity));
+        SyncSlot(*c);
+        ++c->generation;
+      }
+      capacity = std::min(capacity, c->history_capacity);
+      c->lm_ready = true;
+    }
+    options_ = ToLm(options, capacity);
+    space_.Reserve(k);
+    info_.setup_seconds += SecondsSince(t_setup);
+    status.assign(static_cast<size_t>(k), SDFSLAM_OK);
+    states.assign(static_cast<size_t>(k), LmState{});
+    std::vector<int> last_overflow(static_cast<size_t>(k), -1);
+    std::vector<int> repeats(static_cast<size_t>(k), 0);
+    Rebuild();
+    // LM initialisation of every slot: two launches over the table.
+    const SlotArgs all{space_.table, nullptr};
+    LaunchCost(all, shape_, stream_);
+    LaunchLmInit(all, shape_, options.lambda_init, stream_);
+    ReadAll(states);
+    while (true) {
+      int remaining = 0;
+      for (const LmState& st : states) {
+        if (st.halt == 0) remaining = std::max(remaining, options.max_iterations - st.iter);
+      }
+      if (remaining <= 0) break;
+      const int chunk = std::min(remaining, 8);
+      for (int i = 0; i < chunk; ++i) HIPC(hipGraphLaunch(graph_, stream_));
+      info_.graph_launches += chunk;
+      ReadAll(states);
+      bool changed = false;
+      for (int b = 0; b < k; ++b) {
+        LmState& st = states[static_cast<size_t>(b)];
+        sdfslam_hip_context& c = *ctxs_[static_cast<size_t>(b)];
+        if (st.not_spd != 0) {
+          status[static_cast<size_t>(b)] = SDFSLAM_ERR_NOT_SPD;
+          continue;
+        }
+        if (st.overflow == 0) continue;
+        if (st.iter == last_overflow[static_cast<size_t>(b)] && repeats[static_cast<size_t>(b)]++ > 0) {
+          throw Failure(SDFSLAM_ERR_INTERNAL, "pattern re-analysis did not cover the point cells");
+        }
+        c.short_epoch = last_overflow[static_cast<size_t>(b)] >= 0 &&
+                        st.iter - last_overflow[static_cast<size_t>(b)] <= 2;
+        last_overflow[static_cast<size_t>(b)] = st.iter;
+        const auto t_epoch = Now();
+        BuildEpoch(c);
+        info_.analysis_seconds += SecondsSince(t_epoch);
+        ++info_.epochs_rebuilt;
+        c.short_epoch = false;
+        st.overflow = 0;
+        st.halt = 0;
+        WriteState(c, st);
+        changed = true;
+      }
+      if (changed) Rebuild();
+    }
+  }
+
+ private:
+  /// Slot table, merged shape and the captured iteration for the current
+  /// buffers of every context.
+  void Rebuild() {
+    const auto t_capture = Now();
+    const int k = static_cast<int>(ctxs_.size());
+    shape_ = LaunchShape{};
+    for (int b = 0; b < k; ++b) {
+      space_.slots_host[b] = ctxs_[static_cast<size_t>(b)]->slot;
+      shape_.Merge(ctxs_[static_cast<size_t>(b)]->shape);
+    }
+    std::vector<const sdfslam_hip_context*> cs(ctxs_.begin(), ctxs_.end());
+    std::vector<ListRef> level_plan;
+    std::vector<ListRef> rank_plan;
+    BuildPlans(cs, shape_, level_plan, rank_plan);
+    space_.ReservePlans(level_plan.size() + rank_plan.size());
+    std::copy(level_plan.begin(), level_plan.end(), space_.plan_host);
+    std::copy(rank_plan.begin(), rank_plan.end(), space_.plan_host + level_plan.size());
+    // Pinned staging: the two uploads are true async copies; the stream is
+    // synchronised before the staging is touched again.
+    HIPC(hipMemcpyAsync(space_.table, space_.slots_host, sizeof(Slot) * static_cast<size_t>(k),
+                        hipMemcpyHostToDevice, stream_));
+    HIPC(hipMemcpyAsync(space_.plan, space_.plan_host, sizeof(ListRef) * (level_plan.size() + rank_plan.size()),
+                        hipMemcpyHostToDevice, stream_));
+    shape_.level_plan = space_.plan;
+    shape_.rank_plan = space_.plan + level_plan.size();
+    shape_.level_plan_h = level_plan;
+    shape_.rank_plan_h = rank_plan;
+    HIPC(hipStreamSynchronize(stream_));
+    if (graph_ != nullptr) {
+      HIPC(hipGraphExecDestroy(graph_));
+      graph_ = nullptr;
+    }
+    const LmOptions o = options_;
+    hipGraph_t g = nullptr;
+    const SlotArgs args{space_.table, k == 1 ? &ctxs_[0]->slot : nullptr};
+    HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+    LaunchAssemble(args, shape_, true, stream_);
+    LaunchNormalSlots(args, shape_, o, stream_);
+    LaunchFactor(args, shape_, stream_);
+    LaunchSolve(args, shape_, stream_);
+    LaunchStepAndControl(args, shape_, o, stream_);
+    HIPC(hipStreamEndCapture(stream_, &g));
+    const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPC(e);
+    ++info_.graph_captures;
+    info_.capture_seconds += SecondsSince(t_capture);
+  }
+
+  /// Every slot's LM state with one gather launch and one copy.
+  void ReadAll(std::vector<LmState>& states) {
+    LaunchGatherStates(SlotArgs{space_.table, nullptr}, shape_, space_.states_dev, stream_);
+    HIPC(hipMemcpyAsync(space_.states_host, space_.states_dev, sizeof(LmState) * states.size(),
+                        hipMemcpyDeviceToHost, stream_));
+    HIPC(hipStreamSynchronize(stream_));
+    std::copy(space_.states_host, space_.states_host + states.size(), states.begin());
+  }
+
+  std::vector<sdfslam_hip_context*> ctxs_;
+  BatchSpace& space_;
+  LmOptions options_{};
+  hipStream_t stream_{nullptr};
+  LaunchShape shape_;
+  hipGraphExec_t graph_{nullptr};
+  sdfslam_hip_batch_info info_{};
+};
+
 int Guard(const std::function<int()>& fn) {
   try {
     return fn();
@@ -796,9 +1096,23 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
   HIPC(hipSetDevice(c.device));
   auto hp = std::make_unique<HostProblem>(*problem);
   HIPC(hipStreamSynchronize(c.stream));
+  // An appended frame (same grid, the poses extended): the earlier poses
+  // keep the margins they grew to, and the new ones start wider, since a new
+  // frame's pose moves most in its first increments.
+  const sdfslam_grid& og = c.hp->view.grid;
+  const sdfslam_grid& ng = problem->grid;
+  const int old_poses = c.hp->view.num_poses;
+  const bool extends = og.nx == ng.nx && og.ny == ng.ny && og.min_x == ng.min_x && og.min_y == ng.min_y &&
+                       og.dx == ng.dx && og.dy == ng.dy && problem->num_poses >= old_poses;
+  std::vector<int32_t> margins;
+  if (extends) {
+    margins = c.aopt.pose_margin;
+    if (static_cast<int>(margins.size()) != old_poses) margins.assign(static_cast<size_t>(old_poses), c.aopt.margin);
+    margins.resize(static_cast<size_t>(problem->num_poses), kFreshPoseMargin);
+  }
   c.hp = std::move(hp);
-  c.aopt.pose_margin.clear();
   c.aopt.margin = AnalysisOptions{}.margin;
+  c.aopt.pose_margin = std::move(margins);
   AllocateBase(c, map_values, poses);
   BuildEpoch(c);
   InitLm(c, c.opts.lambda_init);
@@ -826,6 +1140,7 @@ int SolveContext(sdfslam_hip_context& c, const sdfslam_solver_options& options,
     c.history_capacity = options.max_iterations + 1;
     c.hist.Clear();
     c.dw.cost_history = c.hist.Alloc<double>(static_cast<size_t>(c.history_capacity));
+    SyncSlot(c);
     ++c.generation;
   }
   const LmOptions o = ToLm(options, c.history_capacity);
@@ -894,6 +1209,22 @@ int sdfslam_hip_context_update(sdfslam_hip_context* ctx, const sdfslam_problem_v
   });
 }
 
+int sdfslam_hip_pose_margins(sdfslam_hip_context* ctx, int32_t* margins, int32_t capacity,
+                             int32_t* num_poses) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || capacity < 0) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "bad argument");
+    const int n = ctx->hp->view.num_poses;
+    if (num_poses != nullptr) *num_poses = n;
+    if (margins != nullptr) {
+      const std::vector<int32_t>& pm = ctx->aopt.pose_margin;
+      for (int P = 0; P < std::min(n, capacity); ++P) {
+        margins[P] = static_cast<int>(pm.size()) == n ? pm[static_cast<size_t>(P)] : ctx->aopt.margin;
+      }
+    }
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx) {
   delete ctx;
   return SDFSLAM_OK;
@@ -956,8 +1287,8 @@ int sdfslam_hip_assemble(sdfslam_hip_context* ctx, double* residuals, int32_t* j
     }
     sdfslam_hip_context& c = *ctx;
     PrepareStage(c, nullptr);
-    LaunchAssemble(c.dp, c.da, c.dw, false, c.stream);
-    LaunchAssembleEikonalJ(c.dp, c.dw, c.j_eik, c.stream);
+    LaunchAssemble(SlotArgs{c.slot_dev, &c.slot}, c.shape, false, c.stream);
+    LaunchAssembleEikonalJ(SlotArgs{c.slot_dev, &c.slot}, c.shape, c.j_eik, c.stream);
     const sdfslam_problem_view& v = c.hp->view;
     const auto np = static_cast<size_t>(v.num_points);
     const auto ne = static_cast<size_t>(v.num_eikonal);
@@ -1162,8 +1493,8 @@ int sdfslam_hip_cost(sdfslam_hip_context* ctx, double* cost) {
   return Guard([&]() -> int {
     if (ctx == nullptr || cost == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
     sdfslam_hip_context& c = *ctx;
-    LaunchCost(c.dp, c.dw, c.dw.map, c.dw.poses, c.stream);
-    LaunchSumPartials(c.dw, c.scalar, c.stream);
+    LaunchCost(SlotArgs{c.slot_dev, &c.slot}, c.shape, c.stream);
+    LaunchSumPartials(SlotArgs{c.slot_dev, &c.slot}, c.scalar, c.stream);
     HIPC(hipMemcpyAsync(cost, c.scalar, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     HIPC(hipStreamSynchronize(c.stream));
     return SDFSLAM_OK;
@@ -1182,6 +1513,44 @@ int sdfslam_hip_context_solve(sdfslam_hip_context* ctx, const sdfslam_solver_opt
           std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return st;
+  });
+}
+
+int sdfslam_hip_batch_solve(sdfslam_hip_context* const* contexts, int32_t count,
+                            const sdfslam_solver_options* options, sdfslam_solve_result* results,
+                            int32_t* statuses, sdfslam_hip_batch_info* info) {
+  return Guard([&]() -> int {
+    if (contexts == nullptr || options == nullptr || count < 1) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "batch needs contexts and options");
+    }
+    for (int b = 0; b < count; ++b) {
+      if (contexts[b] == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context in batch");
+      if (contexts[b]->device != contexts[0]->device) {
+        throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "batch contexts on different devices");
+      }
+      for (int q = 0; q < b; ++q) {
+        if (contexts[q] == contexts[b]) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "context twice in batch");
+      }
+    }
+    HIPC(hipSetDevice(contexts[0]->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    BatchRun run(contexts, count);
+    std::vector<int> status;
+    std::vector<LmState> states;
+    run.Run(*options, status, states);
+    const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (info != nullptr) {
+      *info = run.info();
+      info->run_seconds = seconds;
+    }
+    for (int b = 0; b < count; ++b) {
+      if (statuses != nullptr) statuses[b] = status[static_cast<size_t>(b)];
+      if (results != nullptr) {
+        FillResult(*contexts[b], states[static_cast<size_t>(b)], &results[b]);
+        results[b].duration_seconds = seconds;
+      }
+    }
+    return SDFSLAM_OK;
   });
 }
 

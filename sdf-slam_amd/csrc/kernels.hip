@@ -8,6 +8,7 @@
 // LM control of lm_solver.cpp:366-425 runs in a one-block kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -34,6 +35,41 @@ __device__ __forceinline__ double WaveSum(double v) {
   return v;
 }
 
+// Where a kernel finds its slot: one slot passed by value as a kernel
+// argument (a single problem: no memory indirection before the kernel's
+// first loads), or a table of slots in device memory (a batch).
+struct OneSlot {
+  Slot s;
+  __device__ __forceinline__ const Slot& get(int) const { return s; }
+};
+struct SlotTable {
+  const Slot* __restrict__ t;
+  __device__ __forceinline__ const Slot& get(int i) const { return t[i]; }
+};
+// Likewise for the list a level kernel works on (see ListRef).
+struct OneList {
+  ListRef r;
+  __device__ __forceinline__ ListRef get(int) const { return r; }
+};
+struct ListTable {
+  const ListRef* __restrict__ t;
+  __device__ __forceinline__ ListRef get(int i) const { return t[i]; }
+};
+
+// The slot of this workgroup: its problem, analysis and work buffers.
+#define BIND_SLOT(index)                             \
+  const Slot& S = slots.get(index);                  \
+  [[maybe_unused]] const DeviceProblem& P = S.P;     \
+  [[maybe_unused]] const DeviceAnalysis& A = S.A;    \
+  [[maybe_unused]] const DeviceWork& W = S.W
+
+// Item `i` of a launch plan's list for `slot`, or -1 beyond its length.
+template <class Lists>
+__device__ __forceinline__ int PlanItem(const Lists& plan, int slot, int i) {
+  const ListRef lr = plan.get(slot);
+  return i < lr.count ? lr.items[i] : -1;
+}
+
 unsigned Blocks(int64_t n, int threads = kThreads, int64_t cap = 1 << 20) {
   int64_t b = (n + threads - 1) / threads;
   if (b < 1) b = 1;
@@ -43,17 +79,15 @@ unsigned Blocks(int64_t n, int threads = kThreads, int64_t cap = 1 << 20) {
 
 // ---------------------------------------------------------------- utilities
 
-__global__ void KFill(double* p, int64_t n, double value) {
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+// Clears the slot's factor panels (the normal-equation kernels then write
+// only the pattern's slots).
+template <class Slots>
+__global__ void KZeroPanels(Slots slots) {
+  BIND_SLOT(blockIdx.y);
+  if (Halted(W.state)) return;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < S.panel_total;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    p[i] = value;
-  }
-}
-
-__global__ void KFillI32(int32_t* p, int64_t n, int32_t value) {
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
-       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    p[i] = value;
+    W.panel[i] = 0.0;
   }
 }
 
@@ -100,7 +134,9 @@ __device__ double InterpolateClamped(const GridParams& g, const double* map, dou
 
 // Point residual and Jacobian (residual_math.hpp:75-106) with the IRLS
 // scaling of AssemblePointRows (lm_solver.cpp:30-61). One thread per row.
-__global__ void KAssemblePoints(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int check) {
+template <class Slots>
+__global__ void KAssemblePoints(Slots slots, int check) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int64_t s = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (s >= P.num_points) return;
@@ -213,7 +249,9 @@ __global__ void KAssemblePoints(DeviceProblem P, DeviceAnalysis A, DeviceWork W,
 
 // EvalEikonalResidual (residual_math.hpp:158-171), AssembleEikonalRows
 // (lm_solver.cpp:67-82): residual and the three scaled Jacobian entries.
-__global__ void KAssembleEikonal(DeviceProblem P, DeviceWork W, double* j_eik) {
+template <class Slots>
+__global__ void KAssembleEikonal(Slots slots, double* j_eik) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (e >= P.num_eikonal) return;
@@ -238,7 +276,9 @@ __global__ void KAssembleEikonal(DeviceProblem P, DeviceWork W, double* j_eik) {
 
 // EvalOdomResidual (residual_math.hpp:182-205), scaled as in the odometry
 // tail of SolveLm (lm_solver.cpp:275-296).
-__global__ void KAssembleOdom(DeviceProblem P, DeviceWork W) {
+template <class Slots>
+__global__ void KAssembleOdom(Slots slots) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int64_t o = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (o >= P.num_odom) return;
@@ -285,7 +325,9 @@ __global__ void KAssembleOdom(DeviceProblem P, DeviceWork W) {
 // summed its own points per occupied cell; here one wavefront per cell walks
 // the cell's candidate frames in ascending order (lanes in a fixed stride,
 // one binary search each, then a fixed butterfly).
-__global__ void KCellGather(DeviceProblem P, DeviceAnalysis A, DeviceWork W) {
+template <class Slots>
+__global__ void KCellGather(Slots slots) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int lane = threadIdx.x & 63;
   const int cell = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
@@ -325,7 +367,9 @@ __device__ __forceinline__ double Damp(double diag, double lambda, int marquardt
 
 // Node-node slots (9-point stencil, lower triangle owned by the node with the
 // smaller grid id) and the node entries of -J^T r. One thread per grid node.
-__global__ void KNodeSlots(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int marquardt) {
+template <class Slots>
+__global__ void KNodeSlots(Slots slots, int marquardt) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = P.grid.nx;
@@ -401,10 +445,12 @@ __global__ void KNodeSlots(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int 
 // and its -J^T r entries. One workgroup per pose; the pose's point rows are
 // sorted by cell in LDS (bitonic) so every entry sums its points in a fixed
 // order.
-__global__ void KPoseBlock(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int marquardt,
-                           int sort_size) {
+template <class Slots>
+__global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int pose = blockIdx.x;
+  if (pose >= P.num_poses) return;
   const int tid = threadIdx.x;
   extern __shared__ unsigned char smem[];
   uint64_t* keys = reinterpret_cast<uint64_t*>(smem);
@@ -629,7 +675,9 @@ __global__ void KPoseBlock(DeviceProblem P, DeviceAnalysis A, DeviceWork W, int 
 }
 
 // Off-diagonal pose-pose blocks from odometry/relation rows (spec order).
-__global__ void KOdomPairs(DeviceProblem P, DeviceAnalysis A, DeviceWork W) {
+template <class Slots>
+__global__ void KOdomPairs(Slots slots) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= A.num_pairs) return;
@@ -656,9 +704,12 @@ __global__ void KOdomPairs(DeviceProblem P, DeviceAnalysis A, DeviceWork W) {
 
 // Extend-add of a child's update matrix: columns [0, split) land in the
 // parent's panel, columns [split, m_c) in the parent's update matrix.
-__global__ void KExtendAddPanel(DeviceAnalysis A, DeviceWork W, const int32_t* children) {
+template <class Slots, class Lists>
+__global__ void KExtendAddPanel(Slots slots, Lists plan) {
+  BIND_SLOT(blockIdx.z);
   if (Halted(W.state)) return;
-  const int c = children[blockIdx.y];
+  const int c = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (c < 0) return;
   const int b = blockIdx.x;
   if (b >= A.rel_split[c]) return;
   const int p = A.sn_parent[c];
@@ -673,9 +724,12 @@ __global__ void KExtendAddPanel(DeviceAnalysis A, DeviceWork W, const int32_t* c
   }
 }
 
-__global__ void KExtendAddUpdate(DeviceAnalysis A, DeviceWork W, const int32_t* children) {
+template <class Slots, class Lists>
+__global__ void KExtendAddUpdate(Slots slots, Lists plan) {
+  BIND_SLOT(blockIdx.z);
   if (Halted(W.state)) return;
-  const int c = children[blockIdx.y];
+  const int c = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (c < 0) return;
   const int mc = A.sn_m[c];
   const int b = A.rel_split[c] + blockIdx.x;
   if (b >= mc) return;
@@ -802,10 +856,13 @@ __device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int
 // The kb-th diagonal block of every front at the level, one wavefront per
 // front: factored in registers and stored in the front's staging slot
 // (with the zero-pivot flag), for KPanelStep's staged mode.
-__global__ void __launch_bounds__(kNB) KDiagBlock(DeviceAnalysis A, DeviceWork W,
-                                                 const int32_t* level_sn, int kb) {
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
+                                                 Lists plan, int kb) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
-  const int s = level_sn[blockIdx.x];
+  const int s = PlanItem(plan, blockIdx.y, blockIdx.x);
+  if (s < 0) return;
   const int ns = A.sn_first[s + 1] - A.sn_first[s];
   const int c0 = kb * kNB;
   if (c0 >= ns) return;
@@ -840,10 +897,14 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(DeviceAnalysis A, DeviceWork W
 // In fused mode workgroup 0 stores the factored block in the staging slot;
 // the trailing-update kernel moves it into the panel, so no workgroup of
 // this launch reads a partly written block.
-__global__ void __launch_bounds__(kThreads) KPanelStep(DeviceAnalysis A, DeviceWork W,
-                                                       const int32_t* level_sn, int kb, int staged) {
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots,
+                                                       Lists plan, int kb,
+                                                       int staged) {
+  BIND_SLOT(blockIdx.z);
   if (Halted(W.state)) return;
-  const int s = level_sn[blockIdx.y];
+  const int s = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (s < 0) return;
   const int ns = A.sn_first[s + 1] - A.sn_first[s];
   const int c0 = kb * kNB;
   if (c0 >= ns) return;
@@ -909,13 +970,18 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(DeviceAnalysis A, DeviceW
 // C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
 // mode 0: trailing panel update after block column kb (K = nb);
 // mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated.
-__global__ void __launch_bounds__(kThreads, 4) KGemmLower(DeviceAnalysis A, DeviceWork W,
-                                                       const int32_t* level_sn, int kb, int mode) {
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots,
+                                                          Lists plan, int kb,
+                                                          int mode, int fronts) {
+  const int slot = static_cast<int>(blockIdx.z) / fronts;
+  BIND_SLOT(slot);
   if (Halted(W.state)) return;
   const int ti = blockIdx.x;
   const int tj = blockIdx.y;
   if (tj > ti) return;
-  const int s = level_sn[blockIdx.z];
+  const int s = PlanItem(plan, slot, static_cast<int>(blockIdx.z) % fronts);
+  if (s < 0) return;
   const int ns = A.sn_first[s + 1] - A.sn_first[s];
   const int f = A.sn_f[s];
   double* Fw = W.panel + A.panel_off[s];
@@ -1045,9 +1111,12 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(DeviceAnalysis A, Devi
 // (unit lower), leaving y in the own rows and the update for the parent in R.
 // With `tri` the rows below the front's own columns (i >= n) are left as
 // assembled and KForwardRest applies L21 to them across many workgroups.
-__global__ void KForward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn, int tri) {
+template <class Slots, class Lists>
+__global__ void KForward(Slots slots, Lists plan, int tri) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
-  const int s = level_sn[blockIdx.x];
+  const int s = PlanItem(plan, blockIdx.y, blockIdx.x);
+  if (s < 0) return;
   const int first = A.sn_first[s];
   const int n = A.sn_first[s + 1] - first;
   const int f = A.sn_f[s];
@@ -1114,10 +1183,13 @@ __global__ void KForward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn
 
 // v2 -= L21 v1 for the fronts KForward solved with `tri`: one thread per
 // row, the same ordered fma chain over k = 0 .. n-1 as the fused kernel.
-__global__ void __launch_bounds__(64) KForwardRest(DeviceAnalysis A, DeviceWork W,
-                                                   const int32_t* level_sn) {
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(64) KForwardRest(Slots slots,
+                                                   Lists plan) {
+  BIND_SLOT(blockIdx.z);
   if (Halted(W.state)) return;
-  const int s = level_sn[blockIdx.y];
+  const int s = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (s < 0) return;
   const int n = A.sn_first[s + 1] - A.sn_first[s];
   const int f = A.sn_f[s];
   const int i = n + blockIdx.x * 64 + threadIdx.x;
@@ -1140,9 +1212,12 @@ __global__ void __launch_bounds__(64) KForwardRest(DeviceAnalysis A, DeviceWork 
 // L21^T x2 per column of the fronts at the level, one wavefront per column
 // (the same lane-strided sum and wave reduction as the fused backward
 // kernel), into tmp at the column's elimination index.
-__global__ void KBackwardRest(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn) {
+template <class Slots, class Lists>
+__global__ void KBackwardRest(Slots slots, Lists plan) {
+  BIND_SLOT(blockIdx.z);
   if (Halted(W.state)) return;
-  const int s = level_sn[blockIdx.y];
+  const int s = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (s < 0) return;
   const int first = A.sn_first[s];
   const int n = A.sn_first[s + 1] - first;
   const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1168,9 +1243,12 @@ __global__ void KBackwardRest(DeviceAnalysis A, DeviceWork W, const int32_t* lev
 
 // Backward: x1 = L11^-T (D^-1 y1 - L21^T x2), x2 from the ancestors. With
 // `tri`, L21^T x2 comes from KBackwardRest.
-__global__ void KBackward(DeviceAnalysis A, DeviceWork W, const int32_t* level_sn, int tri) {
+template <class Slots, class Lists>
+__global__ void KBackward(Slots slots, Lists plan, int tri) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
-  const int s = level_sn[blockIdx.x];
+  const int s = PlanItem(plan, blockIdx.y, blockIdx.x);
+  if (s < 0) return;
   const int first = A.sn_first[s];
   const int n = A.sn_first[s + 1] - first;
   const int m = A.sn_m[s];
@@ -1271,10 +1349,13 @@ __device__ double BlockSum(double v, double* red) {
 }
 
 // Problem::Cost terms (problem.cpp:132-156) into per-block partial sums.
-__global__ void __launch_bounds__(kThreads) KCost(DeviceProblem P, const double* map,
-                                                  const double* poses, double* partials,
-                                                  const LmState* st, int ignore_halt) {
-  if (ignore_halt == 0 && Halted(st)) return;
+template <class Slots>
+__global__ void __launch_bounds__(kThreads) KCost(Slots slots, int ignore_halt) {
+  BIND_SLOT(blockIdx.y);
+  const double* map = W.map;
+  const double* poses = W.poses;
+  double* partials = W.partials;
+  if (ignore_halt == 0 && Halted(W.state)) return;
   __shared__ double red[kThreads];
   const GridParams g = P.grid;
   const int64_t total = P.num_points + P.num_eikonal + P.num_odom;
@@ -1338,7 +1419,9 @@ __global__ void __launch_bounds__(kThreads) KCost(DeviceProblem P, const double*
   if (threadIdx.x == 0) partials[blockIdx.x] = total_sum;
 }
 
-__global__ void KLmInit(DeviceWork W, double lambda_init) {
+template <class Slots>
+__global__ void KLmInit(Slots slots, double lambda_init) {
+  BIND_SLOT(blockIdx.y);
   __shared__ double red[kThreads];
   double acc = 0.0;
   for (int i = threadIdx.x; i < kPartialBlocks; i += blockDim.x) acc += W.partials[i];
@@ -1363,8 +1446,14 @@ __global__ void KLmInit(DeviceWork W, double lambda_init) {
 }
 
 // Step norm and the predicted reduction step . (lambda step + rhs).
-__global__ void KStepReduce(int n, const double* step, const double* rhs, double* partials,
-                            const LmState* st) {
+template <class Slots>
+__global__ void KStepReduce(Slots slots) {
+  BIND_SLOT(blockIdx.y);
+  const int n = P.n;
+  const double* step = W.step;
+  const double* rhs = W.rhs;
+  double* partials = W.partials;
+  const LmState* st = W.state;
   if (Halted(st)) return;
   __shared__ double red[kThreads];
   const double lambda = st->lambda;
@@ -1383,7 +1472,11 @@ __global__ void KStepReduce(int n, const double* step, const double* rhs, double
   }
 }
 
-__global__ void KBackup(int nmap, int npose, DeviceWork W) {
+template <class Slots>
+__global__ void KBackup(Slots slots) {
+  BIND_SLOT(blockIdx.y);
+  const int nmap = S.nmap;
+  const int npose = 3 * P.num_poses;
   if (Halted(W.state)) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nmap + npose; i += gridDim.x * blockDim.x) {
     if (i < nmap) W.map_backup[i] = W.map[i];
@@ -1392,7 +1485,9 @@ __global__ void KBackup(int nmap, int npose, DeviceWork W) {
 }
 
 // Problem::ApplyStep (problem.cpp:117-130)
-__global__ void KApplyStep(DeviceProblem P, DeviceWork W) {
+template <class Slots>
+__global__ void KApplyStep(Slots slots) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < P.num_active) {
@@ -1407,7 +1502,9 @@ __global__ void KApplyStep(DeviceProblem P, DeviceWork W) {
 }
 
 // Convergence test and damping update of SolveLm (lm_solver.cpp:376-425).
-__global__ void KControl(DeviceWork W, LmOptions o) {
+template <class Slots>
+__global__ void KControl(Slots slots, LmOptions o) {
+  BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   __shared__ double red[kThreads];
   double c = 0.0, a = 0.0, b = 0.0;
@@ -1464,7 +1561,18 @@ __global__ void KControl(DeviceWork W, LmOptions o) {
   if (iter >= o.max_iterations) st->halt = 1;
 }
 
-__global__ void KSumPartials(const double* partials, double* out) {
+/// Copies the LM state of every slot into out[slot] (one thread per slot).
+template <class Slots>
+__global__ void KGatherStates(Slots slots, int count, LmState* out) {
+  const int b = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b >= count) return;
+  out[b] = *slots.get(b).W.state;
+}
+
+template <class Slots>
+__global__ void KSumPartials(Slots slots, double* out) {
+  BIND_SLOT(0);
+  const double* partials = W.partials;
   __shared__ double red[kThreads];
   double acc = 0.0;
   for (int i = threadIdx.x; i < kPartialBlocks; i += blockDim.x) acc += partials[i];
@@ -1472,7 +1580,11 @@ __global__ void KSumPartials(const double* partials, double* out) {
   if (threadIdx.x == 0) *out = total;
 }
 
-__global__ void KRevert(int nmap, int npose, DeviceWork W) {
+template <class Slots>
+__global__ void KRevert(Slots slots) {
+  BIND_SLOT(blockIdx.y);
+  const int nmap = S.nmap;
+  const int npose = 3 * P.num_poses;
   if (W.state->revert == 0) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nmap + npose; i += gridDim.x * blockDim.x) {
     if (i < nmap) W.map[i] = W.map_backup[i];
@@ -1482,97 +1594,164 @@ __global__ void KRevert(int nmap, int npose, DeviceWork W) {
 
 }  // namespace
 
-void LaunchFill(double* p, int64_t n, double value, hipStream_t s) {
-  if (n <= 0) return;
-  KFill<<<Blocks(n, kThreads, 4096), kThreads, 0, s>>>(p, n, value);
+void LaunchShape::Merge(const LaunchShape& o) {
+  slots += o.slots;
+  points = std::max(points, o.points);
+  eikonal = std::max(eikonal, o.eikonal);
+  odom = std::max(odom, o.odom);
+  cells = std::max(cells, o.cells);
+  nodes = std::max(nodes, o.nodes);
+  n = std::max(n, o.n);
+  pairs = std::max(pairs, o.pairs);
+  state = std::max(state, o.state);
+  poses = std::max(poses, o.poses);
+  step_items = std::max(step_items, o.step_items);
+  max_points_per_pose = std::max(max_points_per_pose, o.max_points_per_pose);
+  panel_total = std::max(panel_total, o.panel_total);
+  if (levels.size() < o.levels.size()) levels.resize(o.levels.size());
+  for (size_t L = 0; L < o.levels.size(); ++L) {
+    Level& a = levels[L];
+    const Level& b = o.levels[L];
+    a.count = std::max(a.count, b.count);
+    a.max_nb = std::max(a.max_nb, b.max_nb);
+    a.max_f = std::max(a.max_f, b.max_f);
+    a.max_m = std::max(a.max_m, b.max_m);
+    if (a.rank_count.size() < b.rank_count.size()) {
+      a.rank_count.resize(b.rank_count.size(), 0);
+      a.rank_max_m.resize(b.rank_max_m.size(), 0);
+    }
+    for (size_t r = 0; r < b.rank_count.size(); ++r) {
+      a.rank_count[r] = std::max(a.rank_count[r], b.rank_count[r]);
+      a.rank_max_m[r] = std::max(a.rank_max_m[r], b.rank_max_m[r]);
+    }
+  }
 }
 
-void LaunchFillI32(int32_t* p, int64_t n, int32_t value, hipStream_t s) {
-  if (n <= 0) return;
-  KFillI32<<<Blocks(n, kThreads, 4096), kThreads, 0, s>>>(p, n, value);
+namespace {
+
+template <bool kOne>
+auto SlotsOf(const SlotArgs& a) {
+  if constexpr (kOne) {
+    return OneSlot{*a.one};
+  } else {
+    return SlotTable{a.table};
+  }
 }
 
-void LaunchAssemble(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
-                    bool check_footprint, hipStream_t s, hipEvent_t before, hipEvent_t after) {
+template <bool kOne>
+auto LevelLists(const LaunchShape& sh, int L) {
+  if constexpr (kOne) {
+    return OneList{sh.level_plan_h[static_cast<size_t>(L)]};
+  } else {
+    return ListTable{sh.level_plan + static_cast<size_t>(L) * static_cast<size_t>(sh.slots)};
+  }
+}
+
+template <bool kOne>
+auto RankLists(const LaunchShape& sh, int L, int r) {
+  const size_t e = static_cast<size_t>(sh.levels[static_cast<size_t>(L)].rank_base + r);
+  if constexpr (kOne) {
+    return OneList{sh.rank_plan_h[e]};
+  } else {
+    return ListTable{sh.rank_plan + e * static_cast<size_t>(sh.slots)};
+  }
+}
+
+bool OneOf(const SlotArgs& a, const LaunchShape& sh) { return a.one != nullptr && sh.slots == 1; }
+
+
+template <bool kOne>
+void AssembleImpl(const SlotArgs& a, const LaunchShape& sh, bool check_footprint, hipStream_t s,
+                  hipEvent_t before, hipEvent_t after) {
+  const auto slots = SlotsOf<kOne>(a);
+  const unsigned k = static_cast<unsigned>(sh.slots);
   if (before != nullptr) (void)hipEventRecord(before, s);
-  if (P.num_points > 0) {
-    KAssemblePoints<<<Blocks(P.num_points), kThreads, 0, s>>>(P, A, W, check_footprint ? 1 : 0);
+  if (sh.points > 0) {
+    KAssemblePoints<<<dim3(Blocks(sh.points), k), kThreads, 0, s>>>(slots, check_footprint ? 1 : 0);
   }
   if (after != nullptr) (void)hipEventRecord(after, s);
-  if (P.num_eikonal > 0) {
-    KAssembleEikonal<<<Blocks(P.num_eikonal), kThreads, 0, s>>>(P, W, nullptr);
+  if (sh.eikonal > 0) {
+    KAssembleEikonal<<<dim3(Blocks(sh.eikonal), k), kThreads, 0, s>>>(slots, nullptr);
   }
-  if (P.num_odom > 0) {
-    KAssembleOdom<<<Blocks(P.num_odom), kThreads, 0, s>>>(P, W);
-  }
-}
-
-void LaunchAssembleEikonalJ(const DeviceProblem& P, const DeviceWork& W, double* j_eik,
-                            hipStream_t s) {
-  if (P.num_eikonal > 0) {
-    KAssembleEikonal<<<Blocks(P.num_eikonal), kThreads, 0, s>>>(P, W, j_eik);
+  if (sh.odom > 0) {
+    KAssembleOdom<<<dim3(Blocks(sh.odom), k), kThreads, 0, s>>>(slots);
   }
 }
 
-void LaunchNormalSlots(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
-                       const LmOptions& o, int max_points_per_pose, hipStream_t s) {
+template <bool kOne>
+void NormalSlotsImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& o, hipStream_t s) {
+  const auto slots = SlotsOf<kOne>(a);
+  using Slots = decltype(slots);
+  const unsigned k = static_cast<unsigned>(sh.slots);
+  if (sh.panel_total > 0) {
+    KZeroPanels<<<dim3(Blocks(sh.panel_total, kThreads, 4096), k), kThreads, 0, s>>>(slots);
+  }
   int sort_size = 1;
-  while (sort_size < max_points_per_pose) sort_size <<= 1;
+  while (sort_size < sh.max_points_per_pose) sort_size <<= 1;
   const size_t lds = static_cast<size_t>(sort_size) * (8 + 4 + 4 + 4) + 4;
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KPoseBlock),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KPoseBlock<Slots>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-  KPoseBlock<<<P.num_poses, kThreads, lds, s>>>(P, A, W, o.marquardt_scaling, sort_size);
-  KCellGather<<<Blocks(P.num_cells, kThreads / 64), kThreads, 0, s>>>(P, A, W);
-  KNodeSlots<<<Blocks(static_cast<int64_t>(P.grid.nx) * P.grid.ny), kThreads, 0, s>>>(
-      P, A, W, o.marquardt_scaling);
-  if (A.num_pairs > 0) {
-    KOdomPairs<<<Blocks(A.num_pairs), kThreads, 0, s>>>(P, A, W);
+  KPoseBlock<<<dim3(sh.poses, k), kThreads, lds, s>>>(slots, o.marquardt_scaling, sort_size);
+  KCellGather<<<dim3(Blocks(sh.cells, kThreads / 64), k), kThreads, 0, s>>>(slots);
+  KNodeSlots<<<dim3(Blocks(sh.nodes), k), kThreads, 0, s>>>(slots, o.marquardt_scaling);
+  if (sh.pairs > 0) {
+    KOdomPairs<<<dim3(Blocks(sh.pairs), k), kThreads, 0, s>>>(slots);
   }
-}
-
-void LaunchExtendAddPanel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
-                          int count, int max_cols, hipStream_t s) {
-  if (count <= 0 || max_cols <= 0) return;
-  KExtendAddPanel<<<dim3(max_cols, count), kThreads, 0, s>>>(A, W, children);
-}
-
-void LaunchExtendAddUpdate(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
-                           int count, int max_cols, hipStream_t s) {
-  if (count <= 0 || max_cols <= 0) return;
-  KExtendAddUpdate<<<dim3(max_cols, count), kThreads, 0, s>>>(A, W, children);
 }
 
 // Above this many TRSM workgroups a level factors its diagonal blocks in a
 // launch of their own instead of once per workgroup.
 constexpr int kStagedMinBlocks = 512;
 
-void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                       int count, int max_n, int max_f, int max_m, hipStream_t s,
-                       StepEvents* step_events) {
-  const int nblocks = (max_n + kNB - 1) / kNB;
-  for (int kb = 0; kb < nblocks; ++kb) {
-    const int rows = max_f - kb * kNB;
-    const int row_blocks = (rows + kStepRows - 1) / kStepRows;
-    // Few fronts: each workgroup factors the diagonal block itself (one
-    // launch on the critical path). Many fronts: factor each block once.
-    const int staged = row_blocks * count > kStagedMinBlocks ? 1 : 0;
-    const bool timed = step_events != nullptr && step_events->used < step_events->capacity;
-    if (timed) (void)hipEventRecord(step_events->events[2 * step_events->used], s);
-    if (staged != 0) KDiagBlock<<<count, kNB, 0, s>>>(A, W, level_sn, kb);
-    KPanelStep<<<dim3(row_blocks, count), kThreads, 0, s>>>(A, W, level_sn, kb, staged);
-    if (timed) {
-      (void)hipEventRecord(step_events->events[2 * step_events->used + 1], s);
-      ++step_events->used;
+template <bool kOne>
+void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEvents* step_events) {
+  const auto slots = SlotsOf<kOne>(a);
+  const unsigned k = static_cast<unsigned>(sh.slots);
+  for (int L = 0; L < static_cast<int>(sh.levels.size()); ++L) {
+    const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
+    const int ranks = static_cast<int>(lv.rank_count.size());
+    for (int r = 0; r < ranks; ++r) {
+      if (lv.rank_count[static_cast<size_t>(r)] > 0 && lv.rank_max_m[static_cast<size_t>(r)] > 0) {
+        KExtendAddPanel<<<dim3(lv.rank_max_m[static_cast<size_t>(r)], lv.rank_count[static_cast<size_t>(r)], k),
+                          kThreads, 0, s>>>(slots, RankLists<kOne>(sh, L, r));
+      }
     }
-    const int trail_rows = max_f - (kb + 1) * kNB;
-    const int trail_cols = max_n - (kb + 1) * kNB;
-    const int tr = trail_rows > 0 ? (trail_rows + kTile - 1) / kTile : 1;
-    const int tc = trail_cols > 0 ? (trail_cols + kTile - 1) / kTile : 1;
-    KGemmLower<<<dim3(tr, tc, count), kThreads, 0, s>>>(A, W, level_sn, kb, 0);
-  }
-  if (max_m > 0) {
-    const int t = (max_m + kTile - 1) / kTile;
-    KGemmLower<<<dim3(t, t, count), kThreads, 0, s>>>(A, W, level_sn, 0, 1);
+    const int count = lv.count;
+    if (count > 0) {
+      const auto plan = LevelLists<kOne>(sh, L);
+      const int nblocks = (lv.max_nb + kNB - 1) / kNB;
+      for (int kb = 0; kb < nblocks; ++kb) {
+        const int rows = lv.max_f - kb * kNB;
+        const int row_blocks = (rows + kStepRows - 1) / kStepRows;
+        // Few fronts: each workgroup factors the diagonal block itself (one
+        // launch on the critical path). Many fronts: factor each block once.
+        const int staged = static_cast<int64_t>(row_blocks) * count * sh.slots > kStagedMinBlocks ? 1 : 0;
+        const bool timed = step_events != nullptr && step_events->used < step_events->capacity;
+        if (timed) (void)hipEventRecord(step_events->events[2 * step_events->used], s);
+        if (staged != 0) KDiagBlock<<<dim3(count, k), kNB, 0, s>>>(slots, plan, kb);
+        KPanelStep<<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb, staged);
+        if (timed) {
+          (void)hipEventRecord(step_events->events[2 * step_events->used + 1], s);
+          ++step_events->used;
+        }
+        const int trail_rows = lv.max_f - (kb + 1) * kNB;
+        const int trail_cols = lv.max_nb - (kb + 1) * kNB;
+        const int tr = trail_rows > 0 ? (trail_rows + kTile - 1) / kTile : 1;
+        const int tc = trail_cols > 0 ? (trail_cols + kTile - 1) / kTile : 1;
+        KGemmLower<<<dim3(tr, tc, static_cast<unsigned>(count) * k), kThreads, 0, s>>>(slots, plan, kb, 0,
+                                                                                     count);
+      }
+      if (lv.max_m > 0) {
+        const int t = (lv.max_m + kTile - 1) / kTile;
+        KGemmLower<<<dim3(t, t, static_cast<unsigned>(count) * k), kThreads, 0, s>>>(slots, plan, 0, 1, count);
+      }
+    }
+    for (int r = 0; r < ranks; ++r) {
+      if (lv.rank_count[static_cast<size_t>(r)] > 0 && lv.rank_max_m[static_cast<size_t>(r)] > 0) {
+        KExtendAddUpdate<<<dim3(lv.rank_max_m[static_cast<size_t>(r)], lv.rank_count[static_cast<size_t>(r)], k),
+                           kThreads, 0, s>>>(slots, RankLists<kOne>(sh, L, r));
+      }
+    }
   }
 }
 
@@ -1580,49 +1759,114 @@ void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32
 // into launches of their own that spread them over many workgroups.
 constexpr int kSplitSolveFronts = 16;
 
-void LaunchForwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                        int count, int max_m, int max_f, hipStream_t s) {
-  // Few large fronts: one 1024-thread workgroup each for more loads in flight.
-  const int threads = count <= 32 ? 1024 : kThreads;
-  const int tri = count <= kSplitSolveFronts && max_m > 0 ? 1 : 0;
-  KForward<<<count, threads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn, tri);
-  if (tri != 0) KForwardRest<<<dim3((max_m + 63) / 64, count), 64, 0, s>>>(A, W, level_sn);
-}
-
-void LaunchBackwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                         int count, int max_n, int max_m, int max_f, hipStream_t s) {
-  const int threads = count <= 32 ? 1024 : kThreads;
-  const int tri = count <= kSplitSolveFronts && max_m > 0 ? 1 : 0;
-  if (tri != 0) KBackwardRest<<<dim3((max_n + 3) / 4, count), kThreads, 0, s>>>(A, W, level_sn);
-  KBackward<<<count, threads, static_cast<size_t>(max_f) * sizeof(double), s>>>(A, W, level_sn, tri);
-}
-
-void LaunchCost(const DeviceProblem& P, const DeviceWork& W, const double* map,
-                const double* poses, hipStream_t s) {
-  KCost<<<kPartialBlocks, kThreads, 0, s>>>(P, map, poses, W.partials, W.state, 1);
-}
-
-void LaunchSumPartials(const DeviceWork& W, double* out, hipStream_t s) {
-  KSumPartials<<<1, kThreads, 0, s>>>(W.partials, out);
-}
-
-void LaunchLmInit(const DeviceWork& W, double lambda_init, hipStream_t s) {
-  KLmInit<<<1, kThreads, 0, s>>>(W, lambda_init);
-}
-
-void LaunchStepAndControl(const DeviceProblem& P, const DeviceWork& W, const LmOptions& o,
-                          hipStream_t s) {
-  const int nmap = P.grid.nx * P.grid.ny;
-  const int npose = 3 * P.num_poses;
-  KStepReduce<<<kPartialBlocks, kThreads, 0, s>>>(P.n, W.step, W.rhs, W.partials, W.state);
-  if (o.reject_worse_steps != 0 || o.trust_region != 0) {
-    KBackup<<<Blocks(nmap + npose, kThreads, 1024), kThreads, 0, s>>>(nmap, npose, W);
+template <bool kOne>
+void SolveImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s) {
+  const auto slots = SlotsOf<kOne>(a);
+  const unsigned k = static_cast<unsigned>(sh.slots);
+  const int levels = static_cast<int>(sh.levels.size());
+  for (int L = 0; L < levels; ++L) {
+    const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
+    if (lv.count <= 0) continue;
+    // Few large fronts: one 1024-thread workgroup each for more loads in flight.
+    const int threads = lv.count <= 32 ? 1024 : kThreads;
+    const int tri = lv.count <= kSplitSolveFronts && lv.max_m > 0 ? 1 : 0;
+    const auto plan = LevelLists<kOne>(sh, L);
+    KForward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan,
+                                                                                               tri);
+    if (tri != 0) KForwardRest<<<dim3((lv.max_m + 63) / 64, lv.count, k), 64, 0, s>>>(slots, plan);
   }
-  KApplyStep<<<Blocks(P.num_active + P.num_poses), kThreads, 0, s>>>(P, W);
-  KCost<<<kPartialBlocks, kThreads, 0, s>>>(P, W.map, W.poses, W.partials, W.state, 0);
-  KControl<<<1, kThreads, 0, s>>>(W, o);
-  if (o.reject_worse_steps != 0 || o.trust_region != 0) {
-    KRevert<<<Blocks(nmap + npose, kThreads, 1024), kThreads, 0, s>>>(nmap, npose, W);
+  for (int L = levels - 1; L >= 0; --L) {
+    const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
+    if (lv.count <= 0) continue;
+    const int threads = lv.count <= 32 ? 1024 : kThreads;
+    const int tri = lv.count <= kSplitSolveFronts && lv.max_m > 0 ? 1 : 0;
+    const auto plan = LevelLists<kOne>(sh, L);
+    if (tri != 0) KBackwardRest<<<dim3((lv.max_nb + 3) / 4, lv.count, k), kThreads, 0, s>>>(slots, plan);
+    KBackward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan,
+                                                                                                tri);
+  }
+}
+
+template <bool kOne>
+void StepAndControlImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& o, hipStream_t s) {
+  const auto slots = SlotsOf<kOne>(a);
+  const unsigned k = static_cast<unsigned>(sh.slots);
+  KStepReduce<<<dim3(kPartialBlocks, k), kThreads, 0, s>>>(slots);
+  const bool revertible = o.reject_worse_steps != 0 || o.trust_region != 0;
+  if (revertible) {
+    KBackup<<<dim3(Blocks(sh.state, kThreads, 1024), k), kThreads, 0, s>>>(slots);
+  }
+  KApplyStep<<<dim3(Blocks(sh.step_items), k), kThreads, 0, s>>>(slots);
+  KCost<<<dim3(kPartialBlocks, k), kThreads, 0, s>>>(slots, 0);
+  KControl<<<dim3(1, k), kThreads, 0, s>>>(slots, o);
+  if (revertible) {
+    KRevert<<<dim3(Blocks(sh.state, kThreads, 1024), k), kThreads, 0, s>>>(slots);
+  }
+}
+
+}  // namespace
+
+void LaunchAssemble(const SlotArgs& slots, const LaunchShape& sh, bool check_footprint, hipStream_t s,
+                    hipEvent_t before, hipEvent_t after) {
+  if (OneOf(slots, sh)) {
+    AssembleImpl<true>(slots, sh, check_footprint, s, before, after);
+  } else {
+    AssembleImpl<false>(slots, sh, check_footprint, s, before, after);
+  }
+}
+
+void LaunchAssembleEikonalJ(const SlotArgs& slots, const LaunchShape& sh, double* j_eik, hipStream_t s) {
+  if (sh.eikonal > 0) {
+    KAssembleEikonal<<<dim3(Blocks(sh.eikonal), 1), kThreads, 0, s>>>(SlotTable{slots.table}, j_eik);
+  }
+}
+
+void LaunchNormalSlots(const SlotArgs& slots, const LaunchShape& sh, const LmOptions& o, hipStream_t s) {
+  if (OneOf(slots, sh)) {
+    NormalSlotsImpl<true>(slots, sh, o, s);
+  } else {
+    NormalSlotsImpl<false>(slots, sh, o, s);
+  }
+}
+
+void LaunchFactor(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s, StepEvents* step_events) {
+  if (OneOf(slots, sh)) {
+    FactorImpl<true>(slots, sh, s, step_events);
+  } else {
+    FactorImpl<false>(slots, sh, s, step_events);
+  }
+}
+
+void LaunchSolve(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s) {
+  if (OneOf(slots, sh)) {
+    SolveImpl<true>(slots, sh, s);
+  } else {
+    SolveImpl<false>(slots, sh, s);
+  }
+}
+
+void LaunchCost(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s) {
+  KCost<<<dim3(kPartialBlocks, static_cast<unsigned>(sh.slots)), kThreads, 0, s>>>(SlotTable{slots.table}, 1);
+}
+
+void LaunchSumPartials(const SlotArgs& slots, double* out, hipStream_t s) {
+  KSumPartials<<<1, kThreads, 0, s>>>(SlotTable{slots.table}, out);
+}
+
+void LaunchLmInit(const SlotArgs& slots, const LaunchShape& sh, double lambda_init, hipStream_t s) {
+  KLmInit<<<dim3(1, static_cast<unsigned>(sh.slots)), kThreads, 0, s>>>(SlotTable{slots.table}, lambda_init);
+}
+
+void LaunchGatherStates(const SlotArgs& slots, const LaunchShape& sh, LmState* out, hipStream_t s) {
+  const int k = sh.slots;
+  KGatherStates<<<(k + 63) / 64, 64, 0, s>>>(SlotTable{slots.table}, k, out);
+}
+
+void LaunchStepAndControl(const SlotArgs& slots, const LaunchShape& sh, const LmOptions& o, hipStream_t s) {
+  if (OneOf(slots, sh)) {
+    StepAndControlImpl<true>(slots, sh, o, s);
+  } else {
+    StepAndControlImpl<false>(slots, sh, o, s);
   }
 }
 

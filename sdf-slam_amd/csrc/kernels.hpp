@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace sdfslam {
 
@@ -152,21 +153,69 @@ struct DeviceAnalysis {
 
 constexpr int kPartialBlocks = 512;
 
-void LaunchFill(double* p, int64_t n, double value, hipStream_t s);
-void LaunchFillI32(int32_t* p, int64_t n, int32_t value, hipStream_t s);
+/// One problem of a launch. Every kernel takes a table of slots in device
+/// memory and works on the slot named by its last grid dimension, so one
+/// launch serves many independent problems (the batched and served forms);
+/// a single problem is a table of one slot. A slot's results do not depend
+/// on the other slots of the launch or on the launch's grid sizes.
+struct Slot {
+  DeviceProblem P;
+  DeviceAnalysis A;
+  DeviceWork W;
+  int64_t panel_total;
+  int nmap;  // nx * ny
+};
+
+/// The supernodes a level kernel works on in one slot: the fronts of a level,
+/// or one child rank's extend-add list (device pointer and length).
+struct ListRef {
+  const int32_t* items;
+  int32_t count;
+};
+
+/// Grid sizes of a launch over a slot table: the largest sizes over its
+/// slots, level by level for the factorization and the solves.
+struct LaunchShape {
+  int slots{0};
+  int64_t points{0}, eikonal{0}, odom{0}, cells{0}, nodes{0}, n{0}, pairs{0};
+  int64_t state{0};  // nx * ny + 3 * num_poses
+  int poses{0};
+  int64_t step_items{0};  // num_active + num_poses
+  int max_points_per_pose{1};
+  int64_t panel_total{0};
+  struct Level {
+    int count{0}, max_nb{0}, max_f{0}, max_m{0};
+    std::vector<int> rank_count;  // largest child-list length per child rank
+    std::vector<int> rank_max_m;  // largest child update size per child rank
+    int rank_base{0};             // index of the level's first rank in rank_plan
+  };
+  std::vector<Level> levels;
+  // Launch plans in device memory: level_plan[L * slots + b] = the fronts of
+  // level L in slot b; rank_plan[(rank_base + r) * slots + b] = the
+  // children of rank r at that level (count 0 where a slot has none).
+  const ListRef* level_plan{nullptr};
+  const ListRef* rank_plan{nullptr};
+  // Host copies of the plans, for launches over a single slot.
+  std::vector<ListRef> level_plan_h, rank_plan_h;
+  /// Grows this shape to cover `o` as well.
+  void Merge(const LaunchShape& o);
+};
+
+/// The slots of a launch: the device table of sh.slots slots and, when the
+/// launch has one slot, its host copy (the kernels then receive the slot and
+/// their lists by value, with no memory indirection before they start).
+struct SlotArgs {
+  const Slot* table;
+  const Slot* one;
+};
 
 /// Point, eikonal and odometry rows. When `before`/`after` are given they are
 /// recorded around the point-row kernel alone.
-void LaunchAssemble(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
-                    bool check_footprint, hipStream_t s, hipEvent_t before = nullptr,
-                    hipEvent_t after = nullptr);
-void LaunchNormalSlots(const DeviceProblem& P, const DeviceAnalysis& A, const DeviceWork& W,
-                       const LmOptions& o, int max_points_per_pose, hipStream_t s);
+void LaunchAssemble(const SlotArgs& slots, const LaunchShape& sh, bool check_footprint, hipStream_t s,
+                    hipEvent_t before = nullptr, hipEvent_t after = nullptr);
+/// Zeroes the factor panels and forms the damped J^T J and -J^T r in them.
+void LaunchNormalSlots(const SlotArgs& slots, const LaunchShape& sh, const LmOptions& o, hipStream_t s);
 
-void LaunchExtendAddPanel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
-                          int count, int max_cols, hipStream_t s);
-void LaunchExtendAddUpdate(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* children,
-                           int count, int max_cols, hipStream_t s);
 /// Optional timing of the panel steps: pairs (events[2k], events[2k+1])
 /// bracket the diagonal-block and TRSM kernels of one 64-column step.
 struct StepEvents {
@@ -174,22 +223,24 @@ struct StepEvents {
   int capacity;  // pairs
   int used;
 };
-void LaunchFactorLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                       int count, int max_n, int max_f, int max_m, hipStream_t s,
-                       StepEvents* step_events = nullptr);
-void LaunchForwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                        int count, int max_m, int max_f, hipStream_t s);
-void LaunchBackwardLevel(const DeviceAnalysis& A, const DeviceWork& W, const int32_t* level_sn,
-                         int count, int max_n, int max_m, int max_f, hipStream_t s);
+/// Multifrontal LDL^T, level by level (extend-add, panel steps, trailing
+/// updates, update matrices).
+void LaunchFactor(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s,
+                  StepEvents* step_events = nullptr);
+/// Forward and backward solves, level by level; the step in original order.
+void LaunchSolve(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s);
 
-void LaunchAssembleEikonalJ(const DeviceProblem& P, const DeviceWork& W, double* j_eik,
-                            hipStream_t s);
-void LaunchCost(const DeviceProblem& P, const DeviceWork& W, const double* map,
-                const double* poses, hipStream_t s);
-/// Sums the kPartialBlocks cost partials into *out (fixed order).
-void LaunchSumPartials(const DeviceWork& W, double* out, hipStream_t s);
-void LaunchLmInit(const DeviceWork& W, double lambda_init, hipStream_t s);
-void LaunchStepAndControl(const DeviceProblem& P, const DeviceWork& W, const LmOptions& o,
+/// Scaled eikonal Jacobian entries (3 per row) into j_eik (one slot).
+void LaunchAssembleEikonalJ(const SlotArgs& slots, const LaunchShape& sh, double* j_eik, hipStream_t s);
+/// Cost partials of the current state (ignores the halt flag).
+void LaunchCost(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s);
+/// Sums the kPartialBlocks cost partials of slot 0 into *out (fixed order).
+void LaunchSumPartials(const SlotArgs& slots, double* out, hipStream_t s);
+/// LM state from the cost partials (one slot per workgroup).
+void LaunchLmInit(const SlotArgs& slots, const LaunchShape& sh, double lambda_init, hipStream_t s);
+/// LM state of every slot into out[0 .. sh.slots).
+void LaunchGatherStates(const SlotArgs& slots, const LaunchShape& sh, LmState* out, hipStream_t s);
+void LaunchStepAndControl(const SlotArgs& slots, const LaunchShape& sh, const LmOptions& o,
                           hipStream_t s);
 
 }  // namespace sdfslam

@@ -517,12 +517,14 @@ int sdfslam_host_dataset_frame(const sdfslam_host_dataset* dataset, int32_t fram
   });
 }
 
-int sdfslam_server_create(int32_t workers, int64_t device_budget_bytes, sdfslam_server** out) {
+int sdfslam_server_create(const sdfslam_server_options* options, sdfslam_server** out) {
   return Guard([&]() -> int {
-    if (out == nullptr) throw std::invalid_argument("null argument");
+    if (options == nullptr || out == nullptr) throw std::invalid_argument("null argument");
     ServerOptions o;
-    o.workers = workers;
-    o.device_budget_bytes = device_budget_bytes;
+    o.workers = options->workers;
+    o.max_batch = options->max_batch;
+    o.pipelines = options->pipelines;
+    o.device_budget_bytes = options->device_budget_bytes;
     auto s = std::make_unique<sdfslam_server>();
     s->server = std::make_unique<Server>(o);
     *out = s.release();
@@ -618,10 +620,15 @@ int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam
     b.rate_hz = options->rate_hz;
     b.duration_s = options->duration_s;
     b.workers = options->workers;
+    b.max_batch = options->max_batch;
+    b.pipelines = options->pipelines;
     b.seed0 = options->seed0;
     const ServedResult r = RunServedBenchmark(b);
-    *out = {r.increments, r.lm_iterations, r.wall_seconds, r.increments_per_second, r.latency_p50_ms,
-            r.latency_p99_ms, r.latency_mean_ms, r.build_mean_ms, r.solve_mean_ms, r.device_bytes};
+    *out = {r.increments,       r.lm_iterations,    r.wall_seconds,     r.increments_per_second,
+            r.latency_p50_ms,   r.latency_p99_ms,   r.latency_mean_ms,  r.build_mean_ms,
+            r.solve_mean_ms,    r.device_bytes,     r.batches,          r.mean_batch,
+            r.prepare_batch_ms, r.solve_batch_ms,   r.collect_batch_ms, r.capture_batch_ms,
+            r.analysis_batch_ms, r.epochs_rebuilt};
     return SDFSLAM_OK;
   });
 }
@@ -632,7 +639,7 @@ int sdfslam_host_batch_benchmark(const sdfslam_batch_options* options, sdfslam_b
     BatchBenchmark b;
     b.instances = options->instances;
     b.iterations = options->iterations;
-    b.workers = options->workers;
+    b.max_batch = options->max_batch;
     b.seed0 = options->seed0;
     const BatchResult r = RunBatchBenchmark(b);
     *out = {r.lm_iterations, r.unknowns, r.wall_seconds, r.lm_iterations_per_second};

@@ -36,6 +36,41 @@ int64_t HipSession::DeviceBytes() const {
   return info.device_bytes;
 }
 
+sdfslam_hip_context* HipSession::Load(const Problem& problem, const SolverOptions& options) {
+  const sdfslam_problem_view view = problem.View();
+  const sdfslam_solver_options copts = ToC(options);
+  std::vector<double> map_values;
+  std::vector<double> poses;
+  problem.GetState(map_values, poses);
+  if (ctx_ == nullptr) {
+    Rethrow(sdfslam_hip_context_create(&view, &copts, map_values.data(), poses.data(), &ctx_));
+  } else {
+    Rethrow(sdfslam_hip_context_update(ctx_, &view, map_values.data(), poses.data()));
+  }
+  return ctx_;
+}
+
+SolveResult HipSession::Collect(Problem& problem, const sdfslam_solve_result& r, int status,
+                                const std::vector<double>& history) {
+  Rethrow(status);
+  std::vector<double> map_values;
+  std::vector<double> poses;
+  problem.GetState(map_values, poses);
+  Rethrow(sdfslam_hip_get_state(ctx_, map_values.data(), poses.data()));
+  problem.SetState(map_values.data(), poses.data());
+  SolveResult result;
+  result.converged = r.converged != 0;
+  result.iterations = r.iterations;
+  result.initial_cost = r.initial_cost;
+  result.final_cost = r.final_cost;
+  result.duration_seconds = r.duration_seconds;
+  result.cost_history.assign(history.begin(),
+                             history.begin() + std::min<std::ptrdiff_t>(
+                                                   static_cast<std::ptrdiff_t>(history.size()),
+                                                   std::min(r.cost_history_length, r.cost_history_capacity)));
+  return result;
+}
+
 SolveResult HipSession::Solve(Problem& problem, const SolverOptions& options) {
   const auto t0 = std::chrono::steady_clock::now();
   const sdfslam_problem_view view = problem.View();

@@ -33,6 +33,15 @@ class HipSession {
   /// Device memory the context holds (0 before the first Solve).
   int64_t DeviceBytes() const;
 
+  /// The two halves of Solve around a solve of the context, for callers that
+  /// solve many sessions' contexts in one batch (sdfslam_hip_batch_solve):
+  /// Load puts the problem and its state into the context (created on first
+  /// use); Collect reads the solved state back into the problem and returns
+  /// the result, rethrowing `status` as Solve would.
+  sdfslam_hip_context* Load(const Problem& problem, const SolverOptions& options);
+  SolveResult Collect(Problem& problem, const sdfslam_solve_result& result, int status,
+                      const std::vector<double>& history);
+
  private:
   sdfslam_hip_context* ctx_{nullptr};
 };

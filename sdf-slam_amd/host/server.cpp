@@ -47,7 +47,10 @@ Server::Server(const ServerOptions& options) : options_(options) {
     }
     options_.device_budget_bytes = static_cast<int64_t>(0.8 * static_cast<double>(total_b));
   }
-  for (int w = 0; w < options_.workers; ++w) workers_.emplace_back([this] { WorkerLoop(); });
+  if (options_.max_batch < 1 || options_.pipelines < 1) {
+    throw std::invalid_argument("server needs max_batch >= 1 and pipelines >= 1");
+  }
+  for (int w = 0; w < options_.pipelines; ++w) workers_.emplace_back([this] { SchedulerLoop(); });
 }
 
 Server::~Server() {
@@ -128,64 +131,173 @@ int64_t Server::DeviceBytes() {
   return used;
 }
 
-void Server::RunIncrement(Session& s, Pending p, IncrementRecord& rec) {
-  // runner.cpp:240-260 for one appended frame.
-  s.estimated.push_back(s.estimated.back().Compose(p.odometry));
-  s.scans.push_back(std::move(p.scan));
-  s.odometry.push_back(p.odometry);
-  rec.frames = static_cast<int>(s.scans.size());
-  auto t0 = Clock::now();
-  Problem problem(std::move(s.map), s.estimated, s.scans, s.odometry, s.problem);
-  rec.build_seconds = Seconds(t0, Clock::now());
-  t0 = Clock::now();
-  const SolveResult step = s.hip.Solve(problem, s.step);
-  rec.solve_seconds = Seconds(t0, Clock::now());
-  s.estimated = problem.poses();
-  s.map = problem.map();
-  rec.iterations = step.iterations;
-  rec.initial_cost = step.initial_cost;
-  rec.final_cost = step.final_cost;
+namespace {
+
+/// Runs fn(i) for i in [0, n) on up to `threads` host threads.
+template <typename F>
+void ForEachParallel(int n, int threads, F&& fn) {
+  const int t = std::max(1, std::min(threads, n));
+  if (t <= 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> pool;
+  pool.reserve(static_cast<size_t>(t) - 1);
+  auto body = [&] {
+    for (int i = next++; i < n; i = next++) fn(i);
+  };
+  for (int w = 1; w < t; ++w) pool.emplace_back(body);
+  body();
+  for (std::thread& th : pool) th.join();
 }
 
-void Server::WorkerLoop() {
+}  // namespace
+
+void Server::RunBatch(const std::vector<Session*>& batch, std::vector<Pending>& frames,
+                      std::vector<IncrementRecord>& recs, std::vector<std::string>& errors) {
+  const int k = static_cast<int>(batch.size());
+  std::vector<std::unique_ptr<Problem>> problems(static_cast<size_t>(k));
+  std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(k), nullptr);
+  const auto t_prepare = Clock::now();
+  // Prepare: runner.cpp:240-260 for one appended frame, then load the
+  // session's context.
+  ForEachParallel(k, options_.workers, [&](int i) {
+    Session& s = *batch[static_cast<size_t>(i)];
+    Pending& p = frames[static_cast<size_t>(i)];
+    IncrementRecord& rec = recs[static_cast<size_t>(i)];
+    try {
+      s.estimated.push_back(s.estimated.back().Compose(p.odometry));
+      s.scans.push_back(std::move(p.scan));
+      s.odometry.push_back(p.odometry);
+      rec.frames = static_cast<int>(s.scans.size());
+      const auto t0 = Clock::now();
+      problems[static_cast<size_t>(i)] =
+          std::make_unique<Problem>(s.map, s.estimated, s.scans, s.odometry, s.problem);
+      rec.build_seconds = Seconds(t0, Clock::now());
+      ctxs[static_cast<size_t>(i)] = s.hip.Load(*problems[static_cast<size_t>(i)], s.step);
+    } catch (const std::exception& e) {
+      errors[static_cast<size_t>(i)] = e.what();
+    }
+  });
+  std::vector<int> live;
+  for (int i = 0; i < k; ++i) {
+    if (errors[static_cast<size_t>(i)].empty()) live.push_back(i);
+  }
+  const double prepare_s = Seconds(t_prepare, Clock::now());
+  if (live.empty()) return;
+  // One batched solve (all sessions of a server share the step options).
+  const SolverOptions& step = batch[static_cast<size_t>(live[0])]->step;
+  const sdfslam_solver_options copts = ToC(step);
+  const auto nl = static_cast<int>(live.size());
+  std::vector<sdfslam_hip_context*> solve_ctx(static_cast<size_t>(nl));
+  std::vector<sdfslam_solve_result> results(static_cast<size_t>(nl));
+  std::vector<std::vector<double>> hist(static_cast<size_t>(nl),
+                                        std::vector<double>(static_cast<size_t>(std::max(0, step.max_iterations)) + 1));
+  std::vector<int32_t> status(static_cast<size_t>(nl), SDFSLAM_OK);
+  for (int j = 0; j < nl; ++j) {
+    solve_ctx[static_cast<size_t>(j)] = ctxs[static_cast<size_t>(live[static_cast<size_t>(j)])];
+    results[static_cast<size_t>(j)].cost_history = hist[static_cast<size_t>(j)].data();
+    results[static_cast<size_t>(j)].cost_history_capacity = static_cast<int32_t>(hist[static_cast<size_t>(j)].size());
+  }
+  const auto t0 = Clock::now();
+  sdfslam_hip_batch_info info{};
+  const int st = sdfslam_hip_batch_solve(solve_ctx.data(), nl, &copts, results.data(), status.data(), &info);
+  const double solve_s = Seconds(t0, Clock::now());
+  if (st != SDFSLAM_OK) {
+    const std::string msg = std::string(sdfslam_hip_status_string(st)) + ": " + sdfslam_hip_last_error();
+    for (const int i : live) errors[static_cast<size_t>(i)] = msg;
+    return;
+  }
+  // Collect the states back into the sessions.
+  const auto t_collect = Clock::now();
+  ForEachParallel(nl, options_.workers, [&](int j) {
+    const int i = live[static_cast<size_t>(j)];
+    Session& s = *batch[static_cast<size_t>(i)];
+    IncrementRecord& rec = recs[static_cast<size_t>(i)];
+    try {
+      Problem& problem = *problems[static_cast<size_t>(i)];
+      const SolveResult r = s.hip.Collect(problem, results[static_cast<size_t>(j)],
+                                          status[static_cast<size_t>(j)], hist[static_cast<size_t>(j)]);
+      s.estimated = problem.poses();
+      s.map = problem.map();
+      rec.iterations = r.iterations;
+      rec.initial_cost = r.initial_cost;
+      rec.final_cost = r.final_cost;
+      rec.solve_seconds = solve_s;
+    } catch (const std::exception& e) {
+      errors[static_cast<size_t>(i)] = e.what();
+    }
+  });
+  const double collect_s = Seconds(t_collect, Clock::now());
+  std::lock_guard<std::mutex> lock(stats_mu_);
+  stats_.batches += 1;
+  stats_.increments += nl;
+  stats_.graph_captures += info.graph_captures;
+  stats_.epochs_rebuilt += info.epochs_rebuilt;
+  stats_.prepare_seconds += prepare_s;
+  stats_.solve_seconds += solve_s;
+  stats_.collect_seconds += collect_s;
+  stats_.capture_seconds += info.capture_seconds;
+  stats_.analysis_seconds += info.analysis_seconds;
+}
+
+ServerStats Server::Stats() {
+  std::lock_guard<std::mutex> lock(stats_mu_);
+  return stats_;
+}
+
+void Server::SchedulerLoop() {
   std::unique_lock<std::mutex> lock(mu_);
   while (true) {
-    // The idle session whose oldest pending frame arrived first.
-    Session* next = nullptr;
+    std::vector<Session*> batch;
     work_cv_.wait(lock, [&] {
       if (stop_) return true;
-      next = nullptr;
       for (const auto& s : sessions_) {
-        if (s->busy || s->pending.empty()) continue;
-        if (next == nullptr || s->pending.front().arrival < next->pending.front().arrival) next = s.get();
+        if (!s->busy && !s->pending.empty()) return true;
       }
-      return next != nullptr;
+      return false;
     });
     if (stop_) return;
-    Session& s = *next;
-    s.busy = true;
-    Pending p = std::move(s.pending.front());
-    s.pending.pop_front();
-    const Clock::time_point arrival = p.arrival;
-    lock.unlock();
-    IncrementRecord rec;
-    std::string error;
-    try {
-      RunIncrement(s, std::move(p), rec);
-    } catch (const std::exception& e) {
-      error = e.what();
+    // Every idle session with a waiting frame, oldest frame first.
+    for (const auto& s : sessions_) {
+      if (!s->busy && !s->pending.empty()) batch.push_back(s.get());
     }
+    std::sort(batch.begin(), batch.end(), [](const Session* a, const Session* b) {
+      return a->pending.front().arrival < b->pending.front().arrival;
+    });
+    // With several pipelines, split the waiting sessions between them so
+    // that one batch is prepared on the host while another runs on the GPU.
+    const int share = (static_cast<int>(batch.size()) + options_.pipelines - 1) / options_.pipelines;
+    const int take = std::max(1, std::min(options_.max_batch, share));
+    if (static_cast<int>(batch.size()) > take) batch.resize(static_cast<size_t>(take));
+    std::vector<Pending> frames;
+    std::vector<Clock::time_point> arrivals;
+    for (Session* s : batch) {
+      s->busy = true;
+      arrivals.push_back(s->pending.front().arrival);
+      frames.push_back(std::move(s->pending.front()));
+      s->pending.pop_front();
+    }
+    lock.unlock();
+    std::vector<IncrementRecord> recs(batch.size());
+    std::vector<std::string> errors(batch.size());
+    RunBatch(batch, frames, recs, errors);
     const auto done = Clock::now();
-    const int64_t bytes = s.hip.DeviceBytes();
+    std::vector<int64_t> bytes(batch.size());
+    for (size_t i = 0; i < batch.size(); ++i) bytes[i] = batch[i]->hip.DeviceBytes();
     lock.lock();
-    s.records.push_back(rec);
-    s.latency.push_back(Seconds(arrival, done));
-    s.errors.push_back(error);
-    if (bytes > 0) s.device_bytes = bytes;
-    s.busy = false;
+    for (size_t i = 0; i < batch.size(); ++i) {
+      Session& s = *batch[i];
+      s.records.push_back(recs[i]);
+      s.latency.push_back(Seconds(arrivals[i], done));
+      s.errors.push_back(errors[i]);
+      if (bytes[i] > 0) s.device_bytes = bytes[i];
+      s.busy = false;
+    }
     done_cv_.notify_all();
-    // Another frame of this session may be waiting for it to become idle.
-    work_cv_.notify_one();
+    // Sessions of this batch may have more frames waiting.
+    work_cv_.notify_all();
   }
 }
 
@@ -228,6 +340,8 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
             [](const Arrival& a, const Arrival& b) { return a.t != b.t ? a.t < b.t : a.session < b.session; });
   ServerOptions so;
   so.workers = o.workers;
+  so.max_batch = o.max_batch;
+  so.pipelines = o.pipelines;
   Server server(so);
   for (int i = 0; i < o.sessions; ++i) {
     const Workload& wl = wls[static_cast<size_t>(i)];
@@ -269,11 +383,23 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   r.build_mean_ms = r.increments > 0 ? 1e3 * build / static_cast<double>(r.increments) : 0.0;
   r.solve_mean_ms = r.increments > 0 ? 1e3 * solve / static_cast<double>(r.increments) : 0.0;
   r.device_bytes = server.DeviceBytes();
+  const ServerStats st = server.Stats();
+  const double nb = static_cast<double>(std::max<int64_t>(st.batches, 1));
+  r.batches = st.batches;
+  r.mean_batch = static_cast<double>(st.increments) / nb;
+  r.prepare_batch_ms = 1e3 * st.prepare_seconds / nb;
+  r.solve_batch_ms = 1e3 * st.solve_seconds / nb;
+  r.collect_batch_ms = 1e3 * st.collect_seconds / nb;
+  r.capture_batch_ms = 1e3 * st.capture_seconds / nb;
+  r.analysis_batch_ms = 1e3 * st.analysis_seconds / nb;
+  r.epochs_rebuilt = st.epochs_rebuilt;
   return r;
 }
 
 BatchResult RunBatchBenchmark(const BatchBenchmark& o) {
-  if (o.instances < 1 || o.workers < 1) throw std::invalid_argument("batch benchmark needs instances and workers");
+  if (o.instances < 1 || o.max_batch < 1) {
+    throw std::invalid_argument("batch benchmark needs instances and max_batch >= 1");
+  }
   std::vector<std::unique_ptr<Problem>> problems(static_cast<size_t>(o.instances));
   {
     std::vector<std::thread> gen;
@@ -296,31 +422,33 @@ BatchResult RunBatchBenchmark(const BatchBenchmark& o) {
   opts.step_tolerance = 0.0;
   opts.lambda_init = 1.0;
   opts.lambda_factor = 1.0;
+  const sdfslam_solver_options copts = ToC(opts);
   BatchResult r;
   for (const auto& p : problems) r.unknowns += p->num_parameters();
-  std::atomic<int> next{0};
-  std::atomic<int64_t> iters{0};
-  std::vector<std::string> errors(static_cast<size_t>(o.instances));
+  int64_t iters = 0;
   const auto start = Clock::now();
-  std::vector<std::thread> pool;
-  for (int w = 0; w < std::min(o.workers, o.instances); ++w) {
-    pool.emplace_back([&] {
-      HipSession session;
-      for (int i = next++; i < o.instances; i = next++) {
-        try {
-          const SolveResult res = session.Solve(*problems[static_cast<size_t>(i)], opts);
-          iters += res.iterations;
-        } catch (const std::exception& e) {
-          errors[static_cast<size_t>(i)] = e.what();
-        }
-      }
-    });
+  std::vector<std::unique_ptr<HipSession>> sessions(static_cast<size_t>(o.instances));
+  for (int first = 0; first < o.instances; first += o.max_batch) {
+    const int last = std::min(o.instances, first + o.max_batch);
+    const int nb = last - first;
+    std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(nb));
+    for (int i = first; i < last; ++i) {
+      sessions[static_cast<size_t>(i)] = std::make_unique<HipSession>();
+      ctxs[static_cast<size_t>(i - first)] = sessions[static_cast<size_t>(i)]->Load(*problems[static_cast<size_t>(i)], opts);
+    }
+    std::vector<sdfslam_solve_result> res(static_cast<size_t>(nb));
+    std::vector<int32_t> status(static_cast<size_t>(nb), SDFSLAM_OK);
+    const int st = sdfslam_hip_batch_solve(ctxs.data(), nb, &copts, res.data(), status.data(), nullptr);
+    if (st != SDFSLAM_OK) {
+      throw std::runtime_error(std::string("batch solve failed: ") + sdfslam_hip_last_error());
+    }
+    for (int j = 0; j < nb; ++j) {
+      if (status[static_cast<size_t>(j)] != SDFSLAM_OK) throw std::runtime_error("batch instance failed");
+      iters += res[static_cast<size_t>(j)].iterations;
+    }
+    for (int i = first; i < last; ++i) sessions[static_cast<size_t>(i)].reset();
   }
-  for (std::thread& t : pool) t.join();
   r.wall_seconds = Seconds(start, Clock::now());
-  for (const std::string& e : errors) {
-    if (!e.empty()) throw std::runtime_error("batch instance failed: " + e);
-  }
   r.lm_iterations = iters;
   r.lm_iterations_per_second = static_cast<double>(r.lm_iterations) / std::max(r.wall_seconds, 1e-9);
   return r;

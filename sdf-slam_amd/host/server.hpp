@@ -2,12 +2,17 @@
 //
 // The reference solves one Problem per call (Solve, src/solvers/solver.hpp:67)
 // and runs one incremental session per process (src/pipeline/runner.cpp:
-// 226-285). The server keeps many incremental sessions on one GPU: each
-// session owns a device context (its own stream, memory reused from one
-// increment to the next), and a pool of host workers runs the increments of
-// different sessions concurrently (continuous batching across sessions),
-// while each session's increments run one at a time in submission order, so
-// a session's results do not depend on what else is being served.
+// 226-285). The server keeps many incremental sessions on one GPU, each with
+// its own device context (memory reused from one increment to the next).
+// Continuous batching: a scheduler takes every idle session with a waiting
+// frame (oldest first, up to max_batch), rebuilds their problems and loads
+// their contexts on host worker threads, solves them together with
+// sdfslam_hip_batch_solve (every stage of an LM iteration is one launch over
+// all of them), and reads the states back. Frames arriving meanwhile join
+// the next batch; with two pipelines one batch is prepared while the other
+// runs on the GPU. Each session's increments run one at a time in
+// submission order, and a session's results do not depend on what else is
+// being served.
 //
 // Admission: a session is opened only while the device memory of all open
 // sessions plus an estimate for the new one stays within the budget.
@@ -25,8 +30,23 @@
 
 namespace sdf_slam {
 
+/// Totals over the batches a server ran.
+struct ServerStats {
+  int64_t batches{0};
+  int64_t increments{0};
+  int64_t graph_captures{0};
+  int64_t epochs_rebuilt{0};
+  double prepare_seconds{0.0};
+  double solve_seconds{0.0};
+  double collect_seconds{0.0};
+  double capture_seconds{0.0};
+  double analysis_seconds{0.0};
+};
+
 struct ServerOptions {
-  int workers{8};                  // host threads running increments
+  int workers{8};                  // host threads preparing and collecting a batch
+  int max_batch{64};               // sessions per batched solve (1: one at a time)
+  int pipelines{2};                // batches in flight (scheduler threads)
   int64_t device_budget_bytes{0};  // 0: 80 % of the device's memory
 };
 
@@ -56,6 +76,8 @@ class Server {
   void State(int session, std::vector<Pose2>& poses, GridMap& map);
   /// Device memory held by all sessions.
   int64_t DeviceBytes();
+  /// Totals over the batches run so far.
+  ServerStats Stats();
 
  private:
   struct Pending {
@@ -80,8 +102,11 @@ class Server {
     int64_t device_bytes{0};
   };
 
-  void WorkerLoop();
-  void RunIncrement(Session& s, Pending p, IncrementRecord& rec);
+  void SchedulerLoop();
+  /// Prepares, solves (one batched solve) and collects one increment of each
+  /// session; records and errors per session.
+  void RunBatch(const std::vector<Session*>& batch, std::vector<Pending>& frames,
+                std::vector<IncrementRecord>& recs, std::vector<std::string>& errors);
 
   ServerOptions options_;
   std::mutex mu_;
@@ -90,6 +115,8 @@ class Server {
   std::vector<std::unique_ptr<Session>> sessions_;
   std::vector<std::thread> workers_;
   bool stop_{false};
+  std::mutex stats_mu_;
+  ServerStats stats_;
 };
 
 /// Served benchmark: `sessions` incremental sessions (synthetic floorplans,
@@ -100,6 +127,8 @@ struct ServedBenchmark {
   double rate_hz{10.0};
   double duration_s{10.0};
   int workers{8};
+  int max_batch{64};
+  int pipelines{2};
   uint64_t seed0{2000};
 };
 struct ServedResult {
@@ -113,17 +142,26 @@ struct ServedResult {
   double build_mean_ms{0.0};
   double solve_mean_ms{0.0};
   int64_t device_bytes{0};
+  int64_t batches{0};
+  double mean_batch{0.0};
+  double prepare_batch_ms{0.0};
+  double solve_batch_ms{0.0};
+  double collect_batch_ms{0.0};
+  double capture_batch_ms{0.0};
+  double analysis_batch_ms{0.0};
+  int64_t epochs_rebuilt{0};
 };
 ServedResult RunServedBenchmark(const ServedBenchmark& options);
 
 /// Batched benchmark: `instances` independent batch problems (seeds
 /// seed0 + i, uneven sizes), each solved for `iterations` LM iterations,
-/// `workers` at a time. Host problem construction is done before the clock
-/// starts; the clock covers device setup and the iterations.
+/// `max_batch` per batched solve (1: one after another). Host problem
+/// construction is done before the clock starts; the clock covers device
+/// setup (one context per instance) and the iterations.
 struct BatchBenchmark {
   int instances{64};
   int iterations{30};
-  int workers{8};
+  int max_batch{64};
   uint64_t seed0{1000};
 };
 struct BatchResult {

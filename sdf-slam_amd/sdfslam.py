@@ -103,9 +103,15 @@ class IncrementRecord(C.Structure):
                 ("solve_seconds", C.c_double)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("max_batch", C.c_int32), ("pipelines", C.c_int32),
+                ("device_budget_bytes", C.c_int64)]
+
+
 class ServedOptions(C.Structure):
     _fields_ = [("sessions", C.c_int32), ("rate_hz", C.c_double), ("duration_s", C.c_double),
-                ("workers", C.c_int32), ("seed0", C.c_uint64)]
+                ("workers", C.c_int32), ("max_batch", C.c_int32), ("pipelines", C.c_int32),
+                ("seed0", C.c_uint64)]
 
 
 class ServedResult(C.Structure):
@@ -113,11 +119,22 @@ class ServedResult(C.Structure):
                 ("wall_seconds", C.c_double), ("increments_per_second", C.c_double),
                 ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double),
                 ("latency_mean_ms", C.c_double), ("build_mean_ms", C.c_double),
-                ("solve_mean_ms", C.c_double), ("device_bytes", C.c_int64)]
+                ("solve_mean_ms", C.c_double), ("device_bytes", C.c_int64),
+                ("batches", C.c_int64), ("mean_batch", C.c_double),
+                ("prepare_batch_ms", C.c_double), ("solve_batch_ms", C.c_double),
+                ("collect_batch_ms", C.c_double), ("capture_batch_ms", C.c_double),
+                ("analysis_batch_ms", C.c_double), ("epochs_rebuilt", C.c_int64)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("graph_captures", C.c_int32), ("graph_launches", C.c_int32),
+                ("epochs_rebuilt", C.c_int32), ("setup_seconds", C.c_double),
+                ("capture_seconds", C.c_double), ("analysis_seconds", C.c_double),
+                ("run_seconds", C.c_double)]
 
 
 class BatchOptions(C.Structure):
-    _fields_ = [("instances", C.c_int32), ("iterations", C.c_int32), ("workers", C.c_int32),
+    _fields_ = [("instances", C.c_int32), ("iterations", C.c_int32), ("max_batch", C.c_int32),
                 ("seed0", C.c_uint64)]
 
 
@@ -138,7 +155,8 @@ HIP_SYMBOLS = [
     "sdfslam_hip_get_state", "sdfslam_hip_analysis", "sdfslam_hip_assemble",
     "sdfslam_hip_normal_equations", "sdfslam_hip_linear_step", "sdfslam_hip_cost",
     "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
-    "sdfslam_hip_context_update", "sdfslam_hip_device_memory",
+    "sdfslam_hip_context_update", "sdfslam_hip_device_memory", "sdfslam_hip_batch_solve",
+    "sdfslam_hip_pose_margins",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -185,6 +203,9 @@ hip.sdfslam_hip_context_solve.argtypes = [C.c_void_p, C.POINTER(SolverOptions),
 hip.sdfslam_hip_context_iterate.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
 hip.sdfslam_hip_phase_times_get.argtypes = [C.c_void_p, C.POINTER(PhaseTimes)]
 hip.sdfslam_hip_context_update.argtypes = [C.c_void_p, C.POINTER(ProblemView), _f64p, _f64p]
+hip.sdfslam_hip_pose_margins.argtypes = [C.c_void_p, _i32p, C.c_int32, _i32p]
+hip.sdfslam_hip_batch_solve.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(SolverOptions),
+                                        C.POINTER(SolveResult), _i32p, C.POINTER(BatchInfo)]
 
 host.sdfslam_host_last_error.restype = C.c_char_p
 host.sdfslam_host_default_problem_options.argtypes = [C.POINTER(ProblemOptions)]
@@ -211,7 +232,7 @@ host.sdfslam_host_dataset_info.argtypes = [C.c_void_p, _i32p, C.POINTER(Grid),
                                            C.POINTER(ProblemOptions), C.POINTER(SolverOptions)]
 host.sdfslam_host_dataset_frame.argtypes = [C.c_void_p, C.c_int32, _f64p, C.c_int64, _i64p,
                                             _f64p, _f64p]
-host.sdfslam_server_create.argtypes = [C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]
+host.sdfslam_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]
 host.sdfslam_server_destroy.argtypes = [C.c_void_p]
 host.sdfslam_server_destroy.restype = None
 host.sdfslam_server_open_session.argtypes = [C.c_void_p, C.POINTER(Grid), _f64p,
@@ -504,6 +525,14 @@ class Context:
         _check_hip(hip.sdfslam_hip_context_solve(self._h, C.byref(o), C.byref(r)))
         return _result_dict(r, hist)
 
+    def pose_margins(self):
+        """Footprint margin (cells) of every pose in the current pattern epoch."""
+        n = np.zeros(1, dtype=np.int32)
+        _check_hip(hip.sdfslam_hip_pose_margins(self._h, None, 0, ptr(n, _i32p)))
+        out = np.zeros(int(n[0]), dtype=np.int32)
+        _check_hip(hip.sdfslam_hip_pose_margins(self._h, ptr(out, _i32p), len(out), ptr(n, _i32p)))
+        return out
+
     def update(self, problem, map_values=None, poses=None):
         """Replaces the context's problem (sdfslam_hip_context_update)."""
         m0, p0 = problem.state()
@@ -589,9 +618,10 @@ class Dataset:
 class Server:
     """Served form: many incremental sessions on one GPU (sdfslam_server_*)."""
 
-    def __init__(self, workers=8, device_budget_bytes=0):
+    def __init__(self, workers=8, device_budget_bytes=0, max_batch=64, pipelines=2):
         h = C.c_void_p()
-        _check_host(host.sdfslam_server_create(workers, device_budget_bytes, C.byref(h)))
+        o = ServerOptions(workers, max_batch, pipelines, device_budget_bytes)
+        _check_host(host.sdfslam_server_create(C.byref(o), C.byref(h)))
         self._h = h
         self._grids = {}
 
@@ -641,15 +671,35 @@ class Server:
         return poses.reshape(-1, 3), m
 
 
-def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=8, seed0=2000):
-    o = ServedOptions(sessions, rate_hz, duration_s, workers, seed0)
+def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=8, max_batch=64,
+                     pipelines=2, seed0=2000):
+    o = ServedOptions(sessions, rate_hz, duration_s, workers, max_batch, pipelines, seed0)
     r = ServedResult()
     _check_host(host.sdfslam_host_served_benchmark(C.byref(o), C.byref(r)))
     return {k: getattr(r, k) for k, _ in ServedResult._fields_}
 
 
-def batch_benchmark(instances=64, iterations=30, workers=8, seed0=1000):
-    o = BatchOptions(instances, iterations, workers, seed0)
+def batch_benchmark(instances=64, iterations=30, max_batch=64, seed0=1000):
+    o = BatchOptions(instances, iterations, max_batch, seed0)
     r = BatchResult()
     _check_host(host.sdfslam_host_batch_benchmark(C.byref(o), C.byref(r)))
     return {k: getattr(r, k) for k, _ in BatchResult._fields_}
+
+
+def batch_solve(contexts, options, info=None):
+    """sdfslam_hip_batch_solve over Context objects; returns (results,
+    statuses). When `info` is a dict it receives the call's timing
+    breakdown (BatchInfo fields)."""
+    k = len(contexts)
+    handles = (C.c_void_p * k)(*[c._h.value for c in contexts])
+    hists = [np.zeros(max(options.max_iterations, 0) + 1) for _ in range(k)]
+    res = (SolveResult * k)()
+    for i in range(k):
+        res[i].cost_history = ptr(hists[i], _f64p)
+        res[i].cost_history_capacity = len(hists[i])
+    st = np.zeros(k, dtype=np.int32)
+    bi = BatchInfo()
+    _check_hip(hip.sdfslam_hip_batch_solve(handles, k, C.byref(options), res, ptr(st, _i32p), C.byref(bi)))
+    if info is not None:
+        info.update({f: getattr(bi, f) for f, _ in BatchInfo._fields_})
+    return [_result_dict(res[i], hists[i]) for i in range(k)], st

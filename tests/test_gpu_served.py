@@ -17,7 +17,7 @@ def _open(sdfslam, server, name, frames):
 def test_session_alone_matches_mix_and_runner(sdfslam):
     print("case served alone vs mix", flush=True)
     frames = 6
-    alone = sdfslam.Server(workers=1)
+    alone = sdfslam.Server(workers=1, max_batch=1, pipelines=1)
     d0, s0, f0 = _open(sdfslam, alone, "served0", frames)
     for xy, _, odo in f0:
         alone.wait(s0, alone.submit(s0, xy, odo))
@@ -43,8 +43,46 @@ def test_session_alone_matches_mix_and_runner(sdfslam):
 
 def test_benchmarks_run(sdfslam):
     print("case served/batch benchmarks", flush=True)
-    b = sdfslam.batch_benchmark(instances=3, iterations=3, workers=2)
+    b = sdfslam.batch_benchmark(instances=3, iterations=3, max_batch=2)
     assert b["lm_iterations"] == 9 and b["lm_iterations_per_second"] > 0
     r = sdfslam.served_benchmark(sessions=2, rate_hz=20.0, duration_s=0.4, workers=2)
     assert r["increments"] > 0 and r["latency_p50_ms"] <= r["latency_p99_ms"]
     assert r["lm_iterations"] > 0 and r["device_bytes"] > 0
+
+
+@pytest.mark.parametrize("tol", [0.0, 1e-3])
+def test_batch_solve_matches_each_alone(sdfslam, tol):
+    # One launch per stage over uneven problems gives each problem exactly
+    # the result it gets alone (also when some stop early on the step
+    # tolerance while the others go on).
+    print(f"case batch solve tol={tol}", flush=True)
+    names = [("office_small", 1), ("corner", 1), ("room_small", 1), ("office_small", 2), ("batch3", 1000)]
+    o = sdfslam.default_solver_options(max_iterations=8, step_tolerance=tol)
+    alone = []
+    alone_ctx = []
+    for name, seed in names:
+        ctx = sdfslam.Context(sdfslam.Problem.workload(name, seed), o)
+        res = ctx.solve(o)
+        alone.append((res, ctx.state()))
+        alone_ctx.append(ctx)
+    ctxs = [sdfslam.Context(sdfslam.Problem.workload(name, seed), o) for name, seed in names]
+    info = {}
+    results, status = sdfslam.batch_solve(ctxs, o, info)
+    assert np.all(status == 0)
+    assert info["graph_captures"] >= 1 and info["graph_launches"] >= 1
+    assert info["run_seconds"] >= info["capture_seconds"] > 0.0
+    for (res, (m, p)), r, ctx in zip(alone, results, ctxs):
+        bm, bp = ctx.state()
+        assert r["iterations"] == res["iterations"]
+        assert np.array_equal(r["cost_history"], res["cost_history"])
+        assert np.array_equal(bm, m) and np.array_equal(bp, p)
+    # A second, smaller batch led by the same context (its batch buffers are
+    # reused) continues from the current states exactly as solving alone.
+    again = [(c.solve(o), c.state()) for c in alone_ctx[:3]]
+    results, status = sdfslam.batch_solve(ctxs[:3], o)
+    assert np.all(status == 0)
+    for (res, (m, p)), r, ctx in zip(again, results, ctxs[:3]):
+        bm, bp = ctx.state()
+        assert r["iterations"] == res["iterations"]
+        assert np.array_equal(r["cost_history"], res["cost_history"])
+        assert np.array_equal(bm, m) and np.array_equal(bp, p)
