@@ -10,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -67,7 +68,8 @@ class DeviceArena {
 
 /// Device memory of one layout (the problem, or one pattern epoch). One
 /// allocation holds every array and is kept while later layouts fit (it
-/// grows with 25 % slack);
+/// grows with 50 % slack; an outgrown allocation is kept until the pool is
+/// released, since freeing device memory waits for the whole device);
 /// host arrays are staged in pinned memory and uploaded with one copy per
 /// contiguous run. The layout is laid out twice by the same code: once to
 /// measure (Upload/Alloc return null), once to place.
@@ -88,18 +90,19 @@ class EpochPool {
     const size_t need = off_;
     const size_t staged = upload_end_;
     if (need > cap_) {
-      if (dev_ != nullptr) HIPC(hipFree(dev_));
+      if (dev_ != nullptr) retired_dev_.push_back(dev_);
+      retired_bytes_ += cap_;
       dev_ = nullptr;
       cap_ = 0;
-      const size_t want = need + need / 4;
+      const size_t want = need + need / 2;
       HIPC(hipMalloc(&dev_, want));
       cap_ = want;
     }
     if (staged > host_cap_) {
-      if (host_ != nullptr) HIPC(hipHostFree(host_));
+      if (host_ != nullptr) retired_host_.push_back(host_);
       host_ = nullptr;
       host_cap_ = 0;
-      const size_t want = staged + staged / 4;
+      const size_t want = staged + staged / 2;
       HIPC(hipHostMalloc(&host_, want, hipHostMallocDefault));
       host_cap_ = want;
     }
@@ -107,14 +110,16 @@ class EpochPool {
     off_ = 0;
     runs_.clear();
   }
-  /// Copies the staged arrays to the device and waits for the copies.
-  void Finish(hipStream_t s) {
+  /// Enqueues the copies of the staged arrays to the device on `s`. Every
+  /// call of the library synchronizes its stream before it returns, so the
+  /// copies are done before the staging memory is written again.
+  void Copy(hipStream_t s) {
     for (const auto& r : runs_) {
       HIPC(hipMemcpyAsync(static_cast<char*>(dev_) + r.first, static_cast<char*>(host_) + r.first,
                           r.second - r.first, hipMemcpyHostToDevice, s));
     }
-    HIPC(hipStreamSynchronize(s));
   }
+  bool measuring() const { return measuring_; }
   template <typename T>
   T* Upload(const T* data, size_t n) {
     const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
@@ -140,12 +145,17 @@ class EpochPool {
     const size_t at = Take(std::max<size_t>(n, 1) * sizeof(T));
     return measuring_ ? nullptr : reinterpret_cast<T*>(static_cast<char*>(dev_) + at);
   }
-  size_t bytes() const { return cap_; }
+  /// Device memory held, outgrown allocations included.
+  size_t bytes() const { return cap_ + retired_bytes_; }
   void Release() {
     if (dev_ != nullptr) (void)hipFree(dev_);
     if (host_ != nullptr) (void)hipHostFree(host_);
+    for (void* p : retired_dev_) (void)hipFree(p);
+    for (void* p : retired_host_) (void)hipHostFree(p);
+    retired_dev_.clear();
+    retired_host_.clear();
     dev_ = host_ = nullptr;
-    cap_ = host_cap_ = 0;
+    cap_ = host_cap_ = retired_bytes_ = 0;
   }
 
  private:
@@ -158,7 +168,8 @@ class EpochPool {
   size_t off_{0}, upload_end_{0};
   void* dev_{nullptr};
   void* host_{nullptr};
-  size_t cap_{0}, host_cap_{0};
+  size_t cap_{0}, host_cap_{0}, retired_bytes_{0};
+  std::vector<void*> retired_dev_, retired_host_;
   std::vector<std::pair<size_t, size_t>> runs_;
 };
 
@@ -252,6 +263,66 @@ constexpr int kPhases = 5;
 
 }  // namespace
 
+/// Every call runs on a stream of the calling thread, so calls from
+/// different threads never queue behind each other's work; a context has no
+/// work in flight between calls, so any thread may drive it next. A thread
+/// borrows one stream per device from a process-wide pool on first use and
+/// returns it when it ends: streams are never destroyed while the process
+/// runs (destroying one can disturb a graph capture on another thread), and
+/// short-lived threads reuse the streams of earlier ones.
+class StreamPool {
+ public:
+  static StreamPool& Get() {
+    static StreamPool* pool = new StreamPool();  // lives until the process ends
+    return *pool;
+  }
+  hipStream_t Borrow(int device) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      for (size_t i = 0; i < free_.size(); ++i) {
+        if (free_[i].first == device) {
+          const hipStream_t s = free_[i].second;
+          free_.erase(free_.begin() + static_cast<std::ptrdiff_t>(i));
+          return s;
+        }
+      }
+    }
+    hipStream_t stream = nullptr;
+    HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    return stream;
+  }
+  void Return(int device, hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(mu_);
+    free_.emplace_back(device, stream);
+  }
+
+ private:
+  std::mutex mu_;
+  std::vector<std::pair<int, hipStream_t>> free_;
+};
+
+class ThreadStreams {
+ public:
+  ThreadStreams() = default;
+  ThreadStreams(const ThreadStreams&) = delete;
+  ThreadStreams& operator=(const ThreadStreams&) = delete;
+  ~ThreadStreams() {
+    for (const auto& [device, stream] : streams_) StreamPool::Get().Return(device, stream);
+  }
+  hipStream_t Get(int device) {
+    for (const auto& [d, stream] : streams_) {
+      if (d == device) return stream;
+    }
+    const hipStream_t stream = StreamPool::Get().Borrow(device);
+    streams_.emplace_back(device, stream);
+    return stream;
+  }
+
+ private:
+  std::vector<std::pair<int, hipStream_t>> streams_;
+};
+thread_local ThreadStreams t_streams;
+
 using SteadyTime = std::chrono::steady_clock::time_point;
 static SteadyTime Now() { return std::chrono::steady_clock::now(); }
 static double SecondsSince(SteadyTime t0) { return std::chrono::duration<double>(Now() - t0).count(); }
@@ -271,7 +342,16 @@ struct BatchSpace {
 
   void Reserve(int k) {
     if (k <= capacity) return;
-    Release(false);
+    // Outgrown buffers are kept until the context ends: freeing device
+    // memory waits for the whole device.
+    retired_dev.push_back(table);
+    retired_dev.push_back(states_dev);
+    retired_host.push_back(slots_host);
+    retired_host.push_back(states_host);
+    table = nullptr;
+    states_dev = nullptr;
+    slots_host = nullptr;
+    states_host = nullptr;
     const int want = std::max(k, 2 * capacity);
     HIPC(hipMalloc(&table, sizeof(Slot) * static_cast<size_t>(want)));
     HIPC(hipHostMalloc(&slots_host, sizeof(Slot) * static_cast<size_t>(want), hipHostMallocDefault));
@@ -281,41 +361,52 @@ struct BatchSpace {
   }
   void ReservePlans(size_t entries) {
     if (entries <= plan_capacity) return;
-    if (plan != nullptr) HIPC(hipFree(plan));
-    if (plan_host != nullptr) HIPC(hipHostFree(plan_host));
+    retired_dev.push_back(plan);
+    retired_host.push_back(plan_host);
     plan = nullptr;
     plan_host = nullptr;
-    plan_capacity = 0;
     const size_t want = std::max(entries, 2 * plan_capacity) + 16;
     HIPC(hipMalloc(&plan, sizeof(ListRef) * want));
     HIPC(hipHostMalloc(&plan_host, sizeof(ListRef) * want, hipHostMallocDefault));
     plan_capacity = want;
   }
-  /// Frees the slot and state buffers (and the plans when `all`); errors
-  /// are ignored so that this can run from a destructor.
-  void Release(bool all) {
-    if (table != nullptr) (void)hipFree(table);
-    if (slots_host != nullptr) (void)hipHostFree(slots_host);
-    if (states_dev != nullptr) (void)hipFree(states_dev);
-    if (states_host != nullptr) (void)hipHostFree(states_host);
+  std::vector<void*> retired_dev, retired_host;
+
+  /// Frees everything; errors are ignored so that this can run from a
+  /// destructor.
+  void Release() {
+    retired_dev.push_back(table);
+    retired_dev.push_back(states_dev);
+    retired_dev.push_back(plan);
+    retired_host.push_back(slots_host);
+    retired_host.push_back(states_host);
+    retired_host.push_back(plan_host);
+    for (void* p : retired_dev) {
+      if (p != nullptr) (void)hipFree(p);
+    }
+    for (void* p : retired_host) {
+      if (p != nullptr) (void)hipHostFree(p);
+    }
+    retired_dev.clear();
+    retired_host.clear();
     table = nullptr;
     slots_host = nullptr;
     states_dev = nullptr;
     states_host = nullptr;
+    plan = nullptr;
+    plan_host = nullptr;
     capacity = 0;
-    if (all) {
-      if (plan != nullptr) (void)hipFree(plan);
-      if (plan_host != nullptr) (void)hipHostFree(plan_host);
-      plan = nullptr;
-      plan_host = nullptr;
-      plan_capacity = 0;
-    }
+    plan_capacity = 0;
   }
 };
 
 struct sdfslam_hip_context {
   int device{0};
-  hipStream_t stream{nullptr};
+  hipStream_t stream{nullptr};  // the calling thread's stream, set at every call
+  // Device work of a context update, deferred to the context's next call
+  // (which runs it first, on its own stream): an update does host work only.
+  std::vector<std::function<void(hipStream_t)>> pending;
+  bool deferring{false};
   std::unique_ptr<HostProblem> hp;
   sdfslam_solver_options opts{};
   AnalysisOptions aopt;
@@ -338,15 +429,14 @@ struct sdfslam_hip_context {
   LmOptions graph_options{};
   int generation{0};
   Slot slot{};            // this context's problem as a launch slot
-  Slot* slot_dev{nullptr};  // device copy (a table of one)
+  Slot* slot_dev{nullptr};  // device copy (a table of one, in the epoch pool)
   LaunchShape shape;      // grid sizes of this context alone
   // Level schedule: fronts per level and extend-add lists per (level, child
   // rank), on the device with their host offsets, for launch plans.
   const int32_t* level_sn_d{nullptr};
   const int32_t* child_lists_d{nullptr};
   std::vector<int64_t> level_off_h, rank_first_h, rank_off_h;
-  ListRef* plan_dev{nullptr};  // this context's level and rank plans
-  size_t plan_capacity{0};
+
   int max_f{0};
   hipEvent_t events[kPhases + 3]{};  // phase marks, then the point-row kernel pair
   double phase_ms[kPhases + 1]{};
@@ -359,10 +449,9 @@ struct sdfslam_hip_context {
   BatchSpace batch;  // buffers of the batches this context leads
 
   ~sdfslam_hip_context() {
-    batch.Release(true);
+    batch.Release();
     if (graph != nullptr) (void)hipGraphExecDestroy(graph);
-    if (slot_dev != nullptr) (void)hipFree(slot_dev);
-    if (plan_dev != nullptr) (void)hipFree(plan_dev);
+
     for (hipEvent_t& e : events) {
       if (e != nullptr) (void)hipEventDestroy(e);
     }
@@ -370,7 +459,6 @@ struct sdfslam_hip_context {
     epoch.Release();
     base.Release();
     hist.Clear();
-    if (stream != nullptr) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -386,6 +474,26 @@ LmOptions ToLm(const sdfslam_solver_options& o, int capacity) {
   l.marquardt_scaling = o.marquardt_scaling;
   l.trust_region = o.trust_region;
   return l;
+}
+
+/// Runs `work` on the context's stream now, or, while an update defers its
+/// device work, at the start of the context's next call.
+void Enqueue(sdfslam_hip_context& c, std::function<void(hipStream_t)> work) {
+  if (c.deferring) {
+    c.pending.push_back(std::move(work));
+  } else {
+    work(c.stream);
+  }
+}
+
+/// Makes the calling thread's stream the context's stream and runs the
+/// context's deferred work on it.
+void Bind(sdfslam_hip_context& c) {
+  HIPC(hipSetDevice(c.device));
+  c.stream = t_streams.Get(c.device);
+  std::vector<std::function<void(hipStream_t)>> work;
+  work.swap(c.pending);
+  for (auto& w : work) w(c.stream);
 }
 
 void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double* poses) {
@@ -462,14 +570,16 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   layout();
   A.BeginPlace();
   layout();
-  A.Finish(c.stream);
-  HIPC(hipMemsetAsync(W.pose_overflow, 0, sizeof(int32_t) * static_cast<size_t>(v.num_poses), c.stream));
-  const LmState zero{};
-  HIPC(hipMemcpyAsync(W.state, &zero, sizeof(zero), hipMemcpyHostToDevice, c.stream));
-  HIPC(hipStreamSynchronize(c.stream));
+  int32_t* overflow = W.pose_overflow;
+  LmState* state = W.state;
+  const size_t poses_n = static_cast<size_t>(v.num_poses);
+  Enqueue(c, [&A, overflow, state, poses_n](hipStream_t s) {
+    A.Copy(s);
+    HIPC(hipMemsetAsync(overflow, 0, sizeof(int32_t) * poses_n, s));
+    HIPC(hipMemsetAsync(state, 0, sizeof(LmState), s));  // a zero LmState
+  });
   const int want = std::max(c.opts.max_iterations, 0) + 1;
   if (W.cost_history == nullptr || want > c.history_capacity) {
-    c.hist.Clear();
     c.history_capacity = std::max(want, c.history_capacity);
     W.cost_history = c.hist.Alloc<double>(static_cast<size_t>(c.history_capacity));
   }
@@ -506,15 +616,19 @@ void DownloadState(sdfslam_hip_context& c, std::vector<double>& map, std::vector
   HIPC(hipStreamSynchronize(c.stream));
 }
 
-/// Copies the context's current buffers into its launch slot and the slot
-/// to the device (after the problem or the pattern epoch changed).
-void SyncSlot(sdfslam_hip_context& c) {
+/// The context's current buffers as its launch slot.
+void FillSlot(sdfslam_hip_context& c) {
   c.slot.P = c.dp;
   c.slot.A = c.da;
   c.slot.W = c.dw;
   c.slot.panel_total = c.an.panel_total;
   c.slot.nmap = c.hp->view.grid.nx * c.hp->view.grid.ny;
-  if (c.slot_dev == nullptr) HIPC(hipMalloc(&c.slot_dev, sizeof(Slot)));
+}
+
+/// Refreshes the slot and its device copy (which lives in the epoch pool)
+/// after a work buffer moved outside a pattern epoch change.
+void SyncSlot(sdfslam_hip_context& c) {
+  FillSlot(c);
   HIPC(hipMemcpyAsync(c.slot_dev, &c.slot, sizeof(Slot), hipMemcpyHostToDevice, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
 }
@@ -552,29 +666,6 @@ void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape
   }
 }
 
-/// The context's own launch plans, uploaded next to its slot.
-void UploadPlans(sdfslam_hip_context& c) {
-  std::vector<ListRef> level_plan;
-  std::vector<ListRef> rank_plan;
-  BuildPlans({&c}, c.shape, level_plan, rank_plan);
-  const size_t need = level_plan.size() + rank_plan.size() + 1;
-  if (need > c.plan_capacity) {
-    if (c.plan_dev != nullptr) HIPC(hipFree(c.plan_dev));
-    c.plan_dev = nullptr;
-    c.plan_capacity = 0;
-    HIPC(hipMalloc(&c.plan_dev, sizeof(ListRef) * (need + need / 4)));
-    c.plan_capacity = need + need / 4;
-  }
-  std::vector<ListRef> all(level_plan);
-  all.insert(all.end(), rank_plan.begin(), rank_plan.end());
-  HIPC(hipMemcpyAsync(c.plan_dev, all.data(), sizeof(ListRef) * all.size(), hipMemcpyHostToDevice, c.stream));
-  HIPC(hipStreamSynchronize(c.stream));
-  c.shape.level_plan = c.plan_dev;
-  c.shape.rank_plan = c.plan_dev + level_plan.size();
-  c.shape.level_plan_h = level_plan;
-  c.shape.rank_plan_h = rank_plan;
-}
-
 /// Widens the footprint dilation of every pose whose points left their
 /// footprint in the ending epoch: by one cell, by two when that epoch lasted
 /// at most two iterations, up to kMaxMargin. Poses that stay put keep the
@@ -602,15 +693,25 @@ void GrowMargins(sdfslam_hip_context& c) {
   HIPC(hipMemsetAsync(c.dw.pose_overflow, 0, sizeof(int32_t) * flags.size(), c.stream));
 }
 
-/// Symbolic analysis at the current device state and the device buffers
-/// that depend on it (a new pattern epoch).
-void BuildEpoch(sdfslam_hip_context& c) {
+/// Symbolic analysis and the device buffers that depend on it (a new
+/// pattern epoch): at the given host state of a new problem (its poses keep
+/// the margins they have), or at the current device state (a point left its
+/// footprint: the flagged poses' margins grow).
+void BuildEpoch(sdfslam_hip_context& c, const double* host_map = nullptr, const double* host_poses = nullptr) {
   std::vector<double> map;
   std::vector<double> poses;
   const auto t0 = std::chrono::steady_clock::now();
-  DownloadState(c, map, poses);
-  GrowMargins(c);
-  c.an = Analyze(c.hp->view, map.data(), poses.data(), c.aopt);
+  if (host_map != nullptr && host_poses != nullptr) {
+    const int N = c.hp->view.num_poses;
+    if (static_cast<int>(c.aopt.pose_margin.size()) != N) {
+      c.aopt.pose_margin.assign(static_cast<size_t>(N), c.aopt.margin);
+    }
+    c.an = Analyze(c.hp->view, host_map, host_poses, c.aopt);
+  } else {
+    DownloadState(c, map, poses);
+    GrowMargins(c);
+    c.an = Analyze(c.hp->view, map.data(), poses.data(), c.aopt);
+  }
   ++c.pattern_epochs;
   const Analysis& an = c.an;
   if (an.max_points_per_pose > 4096) {
@@ -662,6 +763,16 @@ void BuildEpoch(sdfslam_hip_context& c) {
     }
     rank_first[static_cast<size_t>(L) + 1] = static_cast<int64_t>(rank_off.size()) - 1;
   }
+  sh.max_points_per_pose = std::max(1, an.max_points_per_pose);
+  sh.panel_total = an.panel_total;
+  sh.pairs = static_cast<int64_t>(an.op_pose.size() / 2);
+  c.level_off_h = std::move(level_off);
+  c.rank_first_h = std::move(rank_first);
+  c.rank_off_h = std::move(rank_off);
+  size_t plan_entries = sh.levels.size();
+  for (const LaunchShape::Level& lv : sh.levels) plan_entries += lv.rank_count.size();
+  std::vector<ListRef> level_plan;
+  std::vector<ListRef> rank_plan;
   EpochPool& E = c.epoch;
   DeviceAnalysis& D = c.da;
   auto layout = [&]() {
@@ -707,20 +818,29 @@ void BuildEpoch(sdfslam_hip_context& c) {
     c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
     c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 64 * 64);
     c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
+    // The launch plans and the slot come last: they hold addresses placed
+    // above (known in the placing pass).
+    if (E.measuring()) {
+      E.Upload(static_cast<const ListRef*>(nullptr), plan_entries);
+      E.Upload(static_cast<const Slot*>(nullptr), 1);
+    } else {
+      BuildPlans({&c}, c.shape, level_plan, rank_plan);
+      std::vector<ListRef> all(level_plan);
+      all.insert(all.end(), rank_plan.begin(), rank_plan.end());
+      const ListRef* plans = E.Upload(all);
+      c.shape.level_plan = plans;
+      c.shape.rank_plan = plans + level_plan.size();
+      FillSlot(c);
+      c.slot_dev = E.Upload(&c.slot, 1);
+    }
   };
   E.BeginMeasure();
   layout();
   E.BeginPlace();
   layout();
-  E.Finish(c.stream);
-  sh.max_points_per_pose = std::max(1, an.max_points_per_pose);
-  sh.panel_total = an.panel_total;
-  sh.pairs = static_cast<int64_t>(an.op_pose.size() / 2);
-  c.level_off_h = std::move(level_off);
-  c.rank_first_h = std::move(rank_first);
-  c.rank_off_h = std::move(rank_off);
-  SyncSlot(c);
-  UploadPlans(c);
+  c.shape.level_plan_h = level_plan;
+  c.shape.rank_plan_h = rank_plan;
+  Enqueue(c, [&E](hipStream_t s) { E.Copy(s); });
   ++c.generation;
   c.analysis_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -917,7 +1037,6 @@ class BatchRun {
       HIPC(hipStreamSynchronize(c->stream));
       if (options.max_iterations + 1 > c->history_capacity) {
         c->history_capacity = options.max_iterations + 1;
-        c->hist.Clear();
         c->dw.cost_history = c->hist.Alloc<double>(static_cast<size_t>(c->history_capacity));
         SyncSlot(*c);
         ++c->generation;
@@ -1077,17 +1196,19 @@ void CreateContext(const sdfslam_problem_view* problem, const sdfslam_solver_opt
   c->hp = std::make_unique<HostProblem>(*problem);
   c->opts = *options;
   HIPC(hipGetDevice(&c->device));
-  HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  Bind(*c);
   for (hipEvent_t& e : c->events) HIPC(hipEventCreate(&e));
   AllocateBase(*c, map_values, poses);
-  BuildEpoch(*c);
+  BuildEpoch(*c, map_values, poses);
   InitLm(*c, options->lambda_init);
   HIPC(hipStreamSynchronize(c->stream));
   *out = c.release();
 }
 
-/// Replaces the context's problem and state, keeping its stream and device
-/// memory (pools grow only when the new problem needs more).
+/// Replaces the context's problem and state, keeping its device memory
+/// (pools grow only when the new problem needs more). Host work only: the
+/// uploads and the LM initialisation run at the start of the context's next
+/// call, on that call's stream.
 void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
                    const double* map_values, const double* poses) {
   if (problem == nullptr || map_values == nullptr || poses == nullptr) {
@@ -1095,7 +1216,6 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
   }
   HIPC(hipSetDevice(c.device));
   auto hp = std::make_unique<HostProblem>(*problem);
-  HIPC(hipStreamSynchronize(c.stream));
   // An appended frame (same grid, the poses extended): the earlier poses
   // keep the margins they grew to, and the new ones start wider, since a new
   // frame's pose moves most in its first increments.
@@ -1113,10 +1233,19 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
   c.hp = std::move(hp);
   c.aopt.margin = AnalysisOptions{}.margin;
   c.aopt.pose_margin = std::move(margins);
+  // Work an earlier update deferred is superseded (the new layout uploads
+  // everything again).
+  c.pending.clear();
+  struct Deferring {
+    sdfslam_hip_context& c;
+    explicit Deferring(sdfslam_hip_context& ctx) : c(ctx) { c.deferring = true; }
+    ~Deferring() { c.deferring = false; }
+  } deferring(c);
   AllocateBase(c, map_values, poses);
-  BuildEpoch(c);
-  InitLm(c, c.opts.lambda_init);
-  HIPC(hipStreamSynchronize(c.stream));
+  BuildEpoch(c, map_values, poses);
+  const double lambda = c.opts.lambda_init;
+  Enqueue(c, [&c, lambda](hipStream_t) { InitLm(c, lambda); });
+  c.lm_ready = true;
 }
 
 void FillResult(sdfslam_hip_context& c, const LmState& st, sdfslam_solve_result* result) {
@@ -1138,7 +1267,6 @@ int SolveContext(sdfslam_hip_context& c, const sdfslam_solver_options& options,
                  sdfslam_solve_result* result) {
   if (options.max_iterations + 1 > c.history_capacity) {
     c.history_capacity = options.max_iterations + 1;
-    c.hist.Clear();
     c.dw.cost_history = c.hist.Alloc<double>(static_cast<size_t>(c.history_capacity));
     SyncSlot(c);
     ++c.generation;
@@ -1226,6 +1354,7 @@ int sdfslam_hip_pose_margins(sdfslam_hip_context* ctx, int32_t* margins, int32_t
 }
 
 int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx) {
+  if (ctx != nullptr) (void)hipSetDevice(ctx->device);
   delete ctx;
   return SDFSLAM_OK;
 }
@@ -1233,6 +1362,7 @@ int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx) {
 int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, const double* poses) {
   return Guard([&]() -> int {
     if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
+    Bind(*ctx);
     const sdfslam_problem_view& v = ctx->hp->view;
     if (map_values != nullptr) {
       HIPC(hipMemcpyAsync(ctx->dw.map, map_values, sizeof(double) * v.grid.nx * v.grid.ny,
@@ -1243,6 +1373,7 @@ int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, co
                           ctx->stream));
     }
     BuildEpoch(*ctx);
+    HIPC(hipStreamSynchronize(ctx->stream));
     return SDFSLAM_OK;
   });
 }
@@ -1250,6 +1381,7 @@ int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, co
 int sdfslam_hip_get_state(sdfslam_hip_context* ctx, double* map_values, double* poses) {
   return Guard([&]() -> int {
     if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
+    Bind(*ctx);
     std::vector<double> m, p;
     DownloadState(*ctx, m, p);
     if (map_values != nullptr) std::copy(m.begin(), m.end(), map_values);
@@ -1286,6 +1418,7 @@ int sdfslam_hip_assemble(sdfslam_hip_context* ctx, double* residuals, int32_t* j
       throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
     }
     sdfslam_hip_context& c = *ctx;
+    Bind(c);
     PrepareStage(c, nullptr);
     LaunchAssemble(SlotArgs{c.slot_dev, &c.slot}, c.shape, false, c.stream);
     LaunchAssembleEikonalJ(SlotArgs{c.slot_dev, &c.slot}, c.shape, c.j_eik, c.stream);
@@ -1398,6 +1531,7 @@ int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int64_
   return Guard([&]() -> int {
     if (ctx == nullptr || nnz == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
     sdfslam_hip_context& c = *ctx;
+    Bind(c);
     const LmOptions o = ToLm(c.opts, c.history_capacity);
     for (int attempt = 0; attempt < 2; ++attempt) {
       PrepareStage(c, &lambda);
@@ -1468,6 +1602,7 @@ int sdfslam_hip_linear_step(sdfslam_hip_context* ctx, double lambda, double* ste
   return Guard([&]() -> int {
     if (ctx == nullptr || step == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
     sdfslam_hip_context& c = *ctx;
+    Bind(c);
     const LmOptions o = ToLm(c.opts, c.history_capacity);
     LmState st{};
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -1493,6 +1628,7 @@ int sdfslam_hip_cost(sdfslam_hip_context* ctx, double* cost) {
   return Guard([&]() -> int {
     if (ctx == nullptr || cost == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
     sdfslam_hip_context& c = *ctx;
+    Bind(c);
     LaunchCost(SlotArgs{c.slot_dev, &c.slot}, c.shape, c.stream);
     LaunchSumPartials(SlotArgs{c.slot_dev, &c.slot}, c.scalar, c.stream);
     HIPC(hipMemcpyAsync(cost, c.scalar, sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -1506,6 +1642,7 @@ int sdfslam_hip_context_solve(sdfslam_hip_context* ctx, const sdfslam_solver_opt
   return Guard([&]() -> int {
     if (ctx == nullptr || options == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
     const auto t0 = std::chrono::steady_clock::now();
+    Bind(*ctx);
     ctx->opts = *options;
     const int st = SolveContext(*ctx, *options, result);
     if (result != nullptr) {
@@ -1532,8 +1669,8 @@ int sdfslam_hip_batch_solve(sdfslam_hip_context* const* contexts, int32_t count,
         if (contexts[q] == contexts[b]) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "context twice in batch");
       }
     }
-    HIPC(hipSetDevice(contexts[0]->device));
     const auto t0 = std::chrono::steady_clock::now();
+    for (int b = 0; b < count; ++b) Bind(*contexts[b]);
     BatchRun run(contexts, count);
     std::vector<int> status;
     std::vector<LmState> states;
@@ -1559,6 +1696,7 @@ int sdfslam_hip_context_iterate(sdfslam_hip_context* ctx, int32_t iterations,
   return Guard([&]() -> int {
     if (ctx == nullptr || iterations < 0) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "bad argument");
     sdfslam_hip_context& c = *ctx;
+    Bind(c);
     LmOptions o = ToLm(c.opts, c.history_capacity);
     o.max_iterations = INT_MAX;
     for (double& p : c.phase_ms) p = 0.0;

@@ -209,9 +209,10 @@ void Server::RunBatch(const std::vector<Session*>& batch, std::vector<Pending>& 
     for (const int i : live) errors[static_cast<size_t>(i)] = msg;
     return;
   }
-  // Collect the states back into the sessions.
+  // Collect the states back into the sessions, on this thread (its GPU
+  // stream has just run the batch; other threads' streams may be busy).
   const auto t_collect = Clock::now();
-  ForEachParallel(nl, options_.workers, [&](int j) {
+  for (int j = 0; j < nl; ++j) {
     const int i = live[static_cast<size_t>(j)];
     Session& s = *batch[static_cast<size_t>(i)];
     IncrementRecord& rec = recs[static_cast<size_t>(i)];
@@ -228,7 +229,7 @@ void Server::RunBatch(const std::vector<Session*>& batch, std::vector<Pending>& 
     } catch (const std::exception& e) {
       errors[static_cast<size_t>(i)] = e.what();
     }
-  });
+  }
   const double collect_s = Seconds(t_collect, Clock::now());
   std::lock_guard<std::mutex> lock(stats_mu_);
   stats_.batches += 1;
