@@ -153,6 +153,7 @@ def main():
     elapsed = time.perf_counter() - t0
     info1 = ctx.analysis()
     reanalyses = info1["pattern_epochs"] - info0["pattern_epochs"]
+    speculative = info1["speculative_epochs"] - info0["speculative_epochs"]
     reanalysis_s = info1["analysis_seconds"] - info0["analysis_seconds"]
     ms_per_step = 1e3 * elapsed / args.steps
     print(f"[bench] timed {args.steps} iterations: {ms_per_step:.3f} ms/iter", file=sys.stderr,
@@ -191,6 +192,7 @@ def main():
             "supernodes": info["num_supernodes"],
             "pattern_epochs": info["pattern_epochs"],
             "reanalyses_in_timed_steps": reanalyses,
+            "speculative_reanalyses_in_timed_steps": speculative,
             "reanalysis_ms_in_timed_steps": round(1e3 * reanalysis_s, 2),
             "phase_ms": {k: round(v, 4) for k, v in phases.items()},
         },

@@ -168,8 +168,10 @@ int sdfslam_hip_get_state(sdfslam_hip_context* ctx, double* map_values, double* 
 /* Sizes of the analysis: num_parameters, num_residuals, nnz of the lower
  * triangle of the fixed J^T J pattern, supernodes, nnz of the supernodal factor
  * (incl. explicit zeros), and the factorization flop count sum_j (c_j^2 + 3 c_j);
- * pattern_epochs counts the analyses so far and analysis_seconds their total
- * host time (analysis, uploads, allocation). */
+ * pattern_epochs counts the pattern epochs so far (speculative_epochs of them
+ * analysed on a host thread while the device iterated) and analysis_seconds
+ * the host time they took on the calling thread (analysis, uploads,
+ * allocation). */
 typedef struct sdfslam_hip_analysis_info {
   int64_t num_parameters;
   int64_t num_residuals;
@@ -187,6 +189,7 @@ typedef struct sdfslam_hip_analysis_info {
    * nb^2 per TRSM row, summed over fronts and steps). */
   int32_t panel_step_launches;
   double panel_step_flops;
+  int32_t speculative_epochs;
 } sdfslam_hip_analysis_info;
 int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* info);
 

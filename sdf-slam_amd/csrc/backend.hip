@@ -6,9 +6,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <climits>
 #include <cstring>
 #include <functional>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -447,8 +449,21 @@ struct sdfslam_hip_context {
   int timed_iterations{0};
   bool lm_ready{false};
   BatchSpace batch;  // buffers of the batches this context leads
+  // Speculative pattern epochs: the poses the current epoch's footprints
+  // were computed at, how far each pose's points reach from it (metres),
+  // and a re-analysis running on a host thread.
+  std::vector<double> anchor_poses;
+  std::vector<double> pose_reach;
+  struct Speculation {
+    std::future<Analysis> job;
+    std::vector<double> map, poses;
+    AnalysisOptions aopt;
+  };
+  std::unique_ptr<Speculation> spec;
+  int speculative_epochs{0};
 
   ~sdfslam_hip_context() {
+    spec.reset();  // waits for a running re-analysis (it reads the problem)
     batch.Release();
     if (graph != nullptr) (void)hipGraphExecDestroy(graph);
 
@@ -515,6 +530,11 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   P.huber_delta = v.huber_delta;
   P.smooth_gradient = v.smooth_gradient;
   P.smooth_gradient_step = v.smooth_gradient_step;
+  c.pose_reach.assign(static_cast<size_t>(v.num_poses), 0.0);
+  for (size_t r = 0; r < hp.pt_frame.size(); ++r) {
+    double& reach = c.pose_reach[static_cast<size_t>(hp.pt_frame[r])];
+    reach = std::max(reach, std::hypot(hp.pt_x[r], hp.pt_y[r]));
+  }
   EpochPool& A = c.base;
   DeviceWork& W = c.dw;
   const size_t nmap = static_cast<size_t>(v.grid.nx) * v.grid.ny;
@@ -666,28 +686,44 @@ void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape
   }
 }
 
-/// Widens the footprint dilation of every pose whose points left their
-/// footprint in the ending epoch: by one cell, by two when that epoch lasted
-/// at most two iterations, up to kMaxMargin. Poses that stay put keep the
+/// Widens the footprint dilation of the poses whose points left their
+/// footprint in the ending epoch: by one cell (two when that epoch lasted at
+/// most two iterations), and at least to twice the pose's motion bound over
+/// the epoch (|dt| + reach * |dtheta|, in cells), since a pose that moved
+/// fast tends to keep moving; poses that came within a cell of leaving
+/// theirs grow by one. Up to kMaxMargin. Poses that stay put keep the
 /// initial margin, so the pattern grows only where the state moves.
 constexpr int kMaxMargin = 8;
 // Margin of the poses an incremental update appends.
 constexpr int kFreshPoseMargin = 5;
 
-void GrowMargins(sdfslam_hip_context& c) {
-  const int N = c.hp->view.num_poses;
+void GrowMargins(sdfslam_hip_context& c, const std::vector<double>& poses) {
+  const sdfslam_problem_view& v = c.hp->view;
+  const int N = v.num_poses;
   if (static_cast<int>(c.aopt.pose_margin.size()) != N) {
     c.aopt.pose_margin.assign(static_cast<size_t>(N), c.aopt.margin);
   }
+  const bool motion = static_cast<int>(c.anchor_poses.size()) == 3 * N &&
+                      static_cast<int>(c.pose_reach.size()) == N && static_cast<int>(poses.size()) == 3 * N;
+  const double h = std::min(v.grid.dx, v.grid.dy);
   std::vector<int32_t> flags(static_cast<size_t>(N));
   HIPC(hipMemcpyAsync(flags.data(), c.dw.pose_overflow, sizeof(int32_t) * flags.size(),
                       hipMemcpyDeviceToHost, c.stream));
   HIPC(hipStreamSynchronize(c.stream));
   const int step = c.short_epoch ? 2 : 1;
   for (int P = 0; P < N; ++P) {
+    int32_t& m = c.aopt.pose_margin[static_cast<size_t>(P)];
+    double cells = 0.0;  // motion bound over the epoch, in cells
+    if (motion) {
+      const size_t i = 3 * static_cast<size_t>(P);
+      const double dth = std::remainder(poses[i + 2] - c.anchor_poses[i + 2], 2.0 * M_PI);
+      cells = (std::hypot(poses[i] - c.anchor_poses[i], poses[i + 1] - c.anchor_poses[i + 1]) +
+               c.pose_reach[static_cast<size_t>(P)] * std::fabs(dth)) / h;
+    }
     if (flags[static_cast<size_t>(P)] != 0) {
-      int32_t& m = c.aopt.pose_margin[static_cast<size_t>(P)];
-      m = std::min(m + step, kMaxMargin);
+      m = std::min(std::max(m + step, static_cast<int>(std::ceil(2.0 * cells))), kMaxMargin);
+    } else if (cells > m - 1) {
+      m = std::min(m + 1, kMaxMargin);
     }
   }
   HIPC(hipMemsetAsync(c.dw.pose_overflow, 0, sizeof(int32_t) * flags.size(), c.stream));
@@ -697,6 +733,9 @@ void GrowMargins(sdfslam_hip_context& c) {
 /// pattern epoch): at the given host state of a new problem (its poses keep
 /// the margins they have), or at the current device state (a point left its
 /// footprint: the flagged poses' margins grow).
+void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anchor_poses,
+                  SteadyTime t0);
+
 void BuildEpoch(sdfslam_hip_context& c, const double* host_map = nullptr, const double* host_poses = nullptr) {
   std::vector<double> map;
   std::vector<double> poses;
@@ -706,12 +745,21 @@ void BuildEpoch(sdfslam_hip_context& c, const double* host_map = nullptr, const 
     if (static_cast<int>(c.aopt.pose_margin.size()) != N) {
       c.aopt.pose_margin.assign(static_cast<size_t>(N), c.aopt.margin);
     }
-    c.an = Analyze(c.hp->view, host_map, host_poses, c.aopt);
+    InstallEpoch(c, Analyze(c.hp->view, host_map, host_poses, c.aopt), host_poses, t0);
   } else {
     DownloadState(c, map, poses);
-    GrowMargins(c);
-    c.an = Analyze(c.hp->view, map.data(), poses.data(), c.aopt);
+    GrowMargins(c, poses);
+    InstallEpoch(c, Analyze(c.hp->view, map.data(), poses.data(), c.aopt), poses.data(), t0);
   }
+}
+
+/// Makes `analysis` (of the current problem, with the context's margins,
+/// at a state whose poses were `anchor_poses`) the context's pattern epoch:
+/// device copies, launch shape and plans, slot.
+void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anchor_poses,
+                  SteadyTime t0) {
+  c.an = std::move(analysis);
+  c.anchor_poses.assign(anchor_poses, anchor_poses + 3 * static_cast<size_t>(c.hp->view.num_poses));
   ++c.pattern_epochs;
   const Analysis& an = c.an;
   if (an.max_points_per_pose > 4096) {
@@ -964,8 +1012,74 @@ void PrepareStage(sdfslam_hip_context& c, const double* lambda) {
   WriteState(c, st);
 }
 
+/// Waits for a running speculative re-analysis and discards it.
+void DropSpeculation(sdfslam_hip_context& c) {
+  if (c.spec == nullptr) return;
+  c.spec->job.wait();
+  c.spec.reset();
+}
+
+/// Between chunks of iterations (the device idle): installs the
+/// speculative re-analysis started at the previous chunk's end, or starts
+/// one when a pose has moved so far from where its footprint was computed
+/// that its points may soon leave it.
+/// A point moves by at most |dt| + reach * |dtheta| when its pose moves by
+/// (dt, dtheta), and its cell by at most that over the cell size: while
+/// this stays below the pose's margin (in cells) no point can leave the
+/// footprint. At one cell short of the margin the pose's margin grows by
+/// one and the analysis of the current state runs on a host thread while
+/// the device runs the next chunk in the current epoch; the new epoch takes
+/// over at that chunk's end (waiting for the analysis if need be, so that
+/// where epochs change depends on the device state alone and runs repeat
+/// bit for bit). A point that leaves a footprint still ends the epoch at
+/// once.
+void Speculate(sdfslam_hip_context& c) {
+  if (c.spec != nullptr) {
+    const auto t0 = Now();
+    std::unique_ptr<sdfslam_hip_context::Speculation> sp = std::move(c.spec);
+    c.aopt = sp->aopt;
+    InstallEpoch(c, sp->job.get(), sp->poses.data(), t0);
+    ++c.speculative_epochs;
+    return;
+  }
+  const sdfslam_problem_view& v = c.hp->view;
+  const int N = v.num_poses;
+  if (static_cast<int>(c.anchor_poses.size()) != 3 * N || static_cast<int>(c.pose_reach.size()) != N) return;
+  std::vector<double> poses(3 * static_cast<size_t>(N));
+  HIPC(hipMemcpyAsync(poses.data(), c.dw.poses, poses.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  const double h = std::min(v.grid.dx, v.grid.dy);
+  AnalysisOptions aopt = c.aopt;
+  if (static_cast<int>(aopt.pose_margin.size()) != N) aopt.pose_margin.assign(static_cast<size_t>(N), aopt.margin);
+  bool grow = false;
+  for (int P = 0; P < N; ++P) {
+    const size_t i = 3 * static_cast<size_t>(P);
+    const double dth = std::remainder(poses[i + 2] - c.anchor_poses[i + 2], 2.0 * M_PI);
+    const double d = std::hypot(poses[i] - c.anchor_poses[i], poses[i + 1] - c.anchor_poses[i + 1]) +
+                     c.pose_reach[static_cast<size_t>(P)] * std::fabs(dth);
+    int32_t& m = aopt.pose_margin[static_cast<size_t>(P)];
+    if (d > (m - 1) * h && m < kMaxMargin) {
+      ++m;
+      grow = true;
+    }
+  }
+  if (!grow) return;
+  auto sp = std::make_unique<sdfslam_hip_context::Speculation>();
+  sp->map.resize(static_cast<size_t>(v.grid.nx) * v.grid.ny);
+  HIPC(hipMemcpyAsync(sp->map.data(), c.dw.map, sp->map.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPC(hipStreamSynchronize(c.stream));
+  sp->poses = std::move(poses);
+  sp->aopt = std::move(aopt);
+  const HostProblem* hp = c.hp.get();
+  sdfslam_hip_context::Speculation* raw = sp.get();
+  sp->job = std::async(std::launch::async,
+                       [hp, raw]() { return Analyze(hp->view, raw->map.data(), raw->poses.data(), raw->aopt); });
+  c.spec = std::move(sp);
+}
+
 /// Runs `iterations` LM iterations, re-analysing when a point leaves its
-/// pose's footprint. Returns the final LM state.
+/// pose's footprint (and speculatively before that, see Speculate).
+/// Returns the final LM state.
 LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations, bool timing) {
   LmState st{};
   ReadState(c, st);
@@ -996,6 +1110,7 @@ LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations
       }
       // Re-centre the footprints on the current state and widen those of
       // the poses that left theirs.
+      DropSpeculation(c);
       c.short_epoch = last_overflow_iter >= 0 && st.iter - last_overflow_iter <= 2;
       last_overflow_iter = st.iter;
       BuildEpoch(c);
@@ -1003,8 +1118,12 @@ LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations
       st.overflow = 0;
       st.halt = 0;
       WriteState(c, st);
+    } else if (!timing && st.halt == 0 && st.iter < target) {
+      Speculate(c);
     }
   }
+  // A re-analysis started for iterations this call does not run is dropped.
+  DropSpeculation(c);
   return st;
 }
 
@@ -1216,6 +1335,7 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
   }
   HIPC(hipSetDevice(c.device));
   auto hp = std::make_unique<HostProblem>(*problem);
+  DropSpeculation(c);
   // An appended frame (same grid, the poses extended): the earlier poses
   // keep the margins they grew to, and the new ones start wider, since a new
   // frame's pose moves most in its first increments.
@@ -1363,6 +1483,7 @@ int sdfslam_hip_set_state(sdfslam_hip_context* ctx, const double* map_values, co
   return Guard([&]() -> int {
     if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
     Bind(*ctx);
+    DropSpeculation(*ctx);
     const sdfslam_problem_view& v = ctx->hp->view;
     if (map_values != nullptr) {
       HIPC(hipMemcpyAsync(ctx->dw.map, map_values, sizeof(double) * v.grid.nx * v.grid.ny,
@@ -1407,6 +1528,7 @@ int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* in
     info->device_bytes = static_cast<int64_t>(ctx->base.bytes() + ctx->epoch.bytes() + ctx->hist.bytes());
     info->panel_step_launches = a.panel_step_launches;
     info->panel_step_flops = a.panel_step_flops;
+    info->speculative_epochs = ctx->speculative_epochs;
     return SDFSLAM_OK;
   });
 }
@@ -1670,7 +1792,10 @@ int sdfslam_hip_batch_solve(sdfslam_hip_context* const* contexts, int32_t count,
       }
     }
     const auto t0 = std::chrono::steady_clock::now();
-    for (int b = 0; b < count; ++b) Bind(*contexts[b]);
+    for (int b = 0; b < count; ++b) {
+      Bind(*contexts[b]);
+      DropSpeculation(*contexts[b]);
+    }
     BatchRun run(contexts, count);
     std::vector<int> status;
     std::vector<LmState> states;

@@ -71,7 +71,7 @@ class AnalysisInfo(C.Structure):
                 ("update_doubles", C.c_int64), ("tree_levels", C.c_int32),
                 ("pattern_epochs", C.c_int32), ("analysis_seconds", C.c_double),
                 ("device_bytes", C.c_int64), ("panel_step_launches", C.c_int32),
-                ("panel_step_flops", C.c_double)]
+                ("panel_step_flops", C.c_double), ("speculative_epochs", C.c_int32)]
 
 
 class PhaseTimes(C.Structure):
