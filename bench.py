@@ -87,9 +87,9 @@ def incremental_leg(S, frames):
 
 def served_leg(S, duration, plain_duration, workers):
     """32 incremental sessions, Poisson frame arrivals at 10 Hz each; the
-    served form (continuous batching: every waiting session's increment in
-    one batched solve, `workers` host threads, two batches in flight) against
-    the plain path (one increment at a time) and one session alone."""
+    served form (continuous batching: waiting sessions' increments in batched
+    solves, `workers` host threads, four batches in flight) against the plain
+    path (one increment at a time) and one session alone."""
     full = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=duration, workers=workers)
     plain = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=plain_duration, workers=1,
                                max_batch=1, pipelines=1)
@@ -104,14 +104,16 @@ def served_leg(S, duration, plain_duration, workers):
             "served": r(full), "plain_one_at_a_time": r(plain), "single_session": r(alone)}
 
 
-def batched_leg(S):
+def batched_leg(S, max_batch=8, workers=8):
     """64 uneven batch instances, 30 LM iterations each: aggregate LM
-    iterations/s with all 64 in one batched solve and one after another."""
-    full = S.batch_benchmark(instances=64, iterations=30, max_batch=64)
-    serial = S.batch_benchmark(instances=64, iterations=30, max_batch=1)
-    return {"instances": 64, "iterations": 30, "max_batch": 64,
+    iterations/s with batched solves of up to `max_batch` similar-sized
+    instances on `workers` concurrent streams, against one solve at a time."""
+    full = S.batch_benchmark(instances=64, iterations=30, max_batch=max_batch, workers=workers)
+    serial = S.batch_benchmark(instances=64, iterations=30, max_batch=1, workers=1)
+    return {"instances": 64, "iterations": 30, "max_batch": max_batch, "workers": workers,
             "lm_iters_per_s": full["lm_iterations_per_second"],
             "serial_lm_iters_per_s": serial["lm_iterations_per_second"],
+            "speedup_vs_serial": full["lm_iterations_per_second"] / serial["lm_iterations_per_second"],
             "unknowns_total": full["unknowns"]}
 
 
@@ -126,7 +128,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--served-duration", type=float, default=5.0)
     ap.add_argument("--plain-duration", type=float, default=2.0)
-    ap.add_argument("--workers", type=int, default=8)
+    ap.add_argument("--workers", type=int, default=12)
     ap.add_argument("--incremental-frames", type=int, default=150)
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the incremental, served and batched legs")

@@ -185,8 +185,10 @@ typedef struct sdfslam_hip_analysis_info {
   double analysis_seconds;
   int64_t device_bytes; /* device memory the context holds */
   /* Launches of the 64-column panel-step kernel per factorization and
-   * their algorithmic FP64 flops (diagonal-block LDL^T 2/3 nb^3 plus
-   * nb^2 per TRSM row, summed over fronts and steps). */
+   * their algorithmic FP64 flops (nb^2 per TRSM row, plus 2/3 nb^3 for the
+   * diagonal-block LDL^T of a front's first step -- later steps' blocks are
+   * factored by the previous step's trailing update -- summed over fronts
+   * and steps). */
   int32_t panel_step_launches;
   double panel_step_flops;
   int32_t speculative_epochs;

@@ -217,12 +217,14 @@ typedef struct sdfslam_served_result {
 int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam_served_result* out);
 
 /* Batched benchmark: `instances` uneven batch problems ("batch<i>", seed
- * seed0), `iterations` LM iterations each, max_batch per batched solve
- * (1: one after another). */
+ * seed0), `iterations` LM iterations each. The instances, largest first, are
+ * cut into batched solves of at most max_batch (1: one at a time), which
+ * `workers` host threads run concurrently (each on its own stream). */
 typedef struct sdfslam_batch_options {
   int32_t instances;
   int32_t iterations;
   int32_t max_batch;
+  int32_t workers;
   uint64_t seed0;
 } sdfslam_batch_options;
 typedef struct sdfslam_batch_result {

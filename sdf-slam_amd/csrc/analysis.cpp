@@ -795,7 +795,9 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     for (int c0 = 0; c0 < ns; c0 += 64) {
       const double nb = std::min(64, ns - c0);
       const double rows = f - c0 - nb;
-      a.panel_step_flops += 2.0 / 3.0 * nb * nb * nb + rows * nb * nb;
+      // The first step factors its diagonal block; later steps find theirs
+      // factored by the previous step's trailing update.
+      a.panel_step_flops += (c0 == 0 ? 2.0 / 3.0 * nb * nb * nb : 0.0) + rows * nb * nb;
     }
   }
   // Pattern size (lower triangle): node-node slots, footprint triples, pose

@@ -102,8 +102,10 @@ struct Analysis {
   int64_t factor_nnz{0};
   double factor_flops{0.0};
   // Panel steps per factorization (launches of the 64-column step) and
-  // their algorithmic FP64 flops: per front and step, the diagonal block's
-  // LDL^T (2/3 nb^3) and the TRSM of the rows below it (nb^2 per row).
+  // their algorithmic FP64 flops: per front and step, the TRSM of the rows
+  // below the diagonal block (nb^2 per row), and for the first step the
+  // block's LDL^T (2/3 nb^3; later blocks are factored by the previous
+  // step's trailing update).
   int32_t panel_step_launches{0};
   double panel_step_flops{0.0};
 };

@@ -864,7 +864,7 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     c.child_lists_d = E.Upload(ranks);
     c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
     c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
-    c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 64 * 64);
+    c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 2 * 64 * 64);
     c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
     // The launch plans and the slot come last: they hold addresses placed
     // above (known in the placing pass).

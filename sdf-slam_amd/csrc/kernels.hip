@@ -27,6 +27,8 @@ __device__ __forceinline__ bool Halted(const LmState* st) { return st->halt != 0
 
 __device__ __forceinline__ double WrapAngle(double a) { return atan2(sin(a), cos(a)); }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ double WaveSum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -818,6 +820,13 @@ __device__ __forceinline__ void SolveRowSecondHalf(double* F, int f, int c0, int
 
 constexpr int kStepRows = 128;
 
+// Staging slot of supernode s's factored diagonal block of step kb: two
+// per supernode, alternating with the step, so the block of step kb + 1
+// can be written while step kb's is still being read.
+__device__ __forceinline__ double* StageSlot(const DeviceWork& W, int s, int kb) {
+  return W.diag_stage + (2 * static_cast<int64_t>(s) + (kb & 1)) * kNB * kNB;
+}
+
 // Reads lane `lane`'s value of v (a wave-uniform lane index).
 __device__ __forceinline__ double ReadLane(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -829,12 +838,14 @@ __device__ __forceinline__ double ReadLane(double v, int lane) {
 
 // Right-looking LDL^T of a 64x64 block held by one wavefront, row `lane` in
 // `row` (lower triangle and diagonal meaningful; the upper part is scratch;
-// rows and columns from nb on are an identity pad).
-// Step k reads the pivot and the multipliers of column k from their lanes
-// with v_readlane, so the 64 dependent steps need neither LDS nor barriers.
+// rows and columns from nb on are an identity pad). Step k takes the pivot
+// from its lane with v_readlane and broadcasts the multipliers of column k
+// through the LDS vector `col` (kNB doubles): every lane writes its
+// multiplier, then reads all of them (the same address in every lane). LDS
+// operations of one wavefront complete in order, so no barrier is needed.
 // On return row[j < lane] = L(lane, j) and row[lane] = D(lane). Returns true
 // when a pivot is exactly zero.
-__device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int nb) {
+__device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int nb, double* col) {
   bool zero = false;
 #pragma unroll
   for (int k = 0; k < kNB; ++k) {
@@ -845,9 +856,12 @@ __device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int
       zero = zero || d == 0.0;
       const double l = row[k] / d;
       if (lane > k) row[k] = l;
+      col[lane] = lane > k ? l : 0.0;
+      __builtin_amdgcn_wave_barrier();
       const double t = -l * d;
 #pragma unroll
-      for (int j = k + 1; j < kNB; ++j) row[j] = fma(t, ReadLane(l, j), row[j]);
+      for (int j = k + 1; j < kNB; ++j) row[j] = fma(t, col[j], row[j]);
+      __builtin_amdgcn_wave_barrier();
     }
   }
   return zero;
@@ -870,6 +884,7 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
   const int f = A.sn_f[s];
   const double* F = W.panel + A.panel_off[s];
   const int tid = threadIdx.x;
+  __shared__ double colbuf[kNB];
   double row[kNB];
   const int ti = min(tid, nb - 1);
 #pragma unroll
@@ -877,8 +892,8 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
     const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + c0 + ti];
     row[j] = (tid < nb && j < nb && j <= tid) ? v : (j == tid ? 1.0 : 0.0);
   }
-  const bool zero = FactorDiagWave(row, tid, nb);
-  double* stage = W.diag_stage + static_cast<int64_t>(s) * kNB * kNB;
+  const bool zero = FactorDiagWave(row, tid, nb, colbuf);
+  double* stage = StageSlot(W, s, kb);
 #pragma unroll
   for (int j = 0; j < kNB; ++j) {
     if (j <= tid && tid < nb) stage[j * kNB + tid] = row[j];
@@ -918,7 +933,7 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots,
   __shared__ int zero_pivot;
   const int tid = threadIdx.x;
   if (staged != 0) {
-    const double* stage = W.diag_stage + static_cast<int64_t>(s) * kNB * kNB;
+    const double* stage = StageSlot(W, s, kb);
     for (int idx = tid; idx < kNB * kNB; idx += blockDim.x) {
       const int i = idx % kNB;
       const int j = idx / kNB;
@@ -937,7 +952,8 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots,
       const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + c0 + ti];
       row[j] = (tid < nb && j < nb && j <= tid) ? v : (j == tid ? 1.0 : 0.0);
     }
-    const bool zero = FactorDiagWave(row, tid, nb);
+    // ybuf (not yet in use) holds the broadcast column.
+    const bool zero = FactorDiagWave(row, tid, nb, &ybuf[0][0]);
 #pragma unroll
     for (int j = 0; j < kNB; ++j) {
       if (j <= tid) lt[j][tid] = row[j];
@@ -951,7 +967,7 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots,
       W.state->not_spd = 1;
       W.state->halt = 1;
     }
-    double* stage = W.diag_stage + static_cast<int64_t>(s) * kNB * kNB;
+    double* stage = StageSlot(W, s, kb);
     for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
       const int i = idx % nb;
       const int j = idx / nb;
@@ -970,10 +986,14 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots,
 // C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
 // mode 0: trailing panel update after block column kb (K = nb);
 // mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated.
-template <class Slots, class Lists>
-__global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots,
-                                                          Lists plan, int kb,
-                                                          int mode, int fronts) {
+// With kLookAhead (mode 0), the workgroup of tile (0, 0) -- the next
+// block column's diagonal block, final once its update is stored -- then
+// factors that block and stages it for step kb + 1, whose panel step then
+// only solves rows: the factorization leaves the step's critical path
+// (the other tiles' updates run beside it).
+template <bool kLookAhead, class Slots, class Lists>
+__device__ __forceinline__ void GemmLowerBody(const Slots& slots, const Lists& plan, int kb, int mode,
+                                              int fronts) {
   const int slot = static_cast<int>(blockIdx.z) / fronts;
   BIND_SLOT(slot);
   if (Halted(W.state)) return;
@@ -986,7 +1006,7 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots,
   const int f = A.sn_f[s];
   double* Fw = W.panel + A.panel_off[s];
   const double* F = Fw;
-  const double* stage = W.diag_stage + static_cast<int64_t>(s) * kNB * kNB;
+  const double* stage = StageSlot(W, s, kb);
   if (mode == 0) {
     const int c0 = kb * kNB;
     if (c0 >= ns) return;
@@ -1027,19 +1047,26 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots,
   const int R0 = ti * kTile;
   const int C0 = tj * kTile;
   if (R0 >= M || C0 >= N) return;
-  __shared__ double As[2][kTk][kTile];
-  __shared__ double Bs[2][kTk][kTile];
+  // Rows of the staged chunks are padded by two doubles, so the two
+  // k rows an MFMA operand read touches fall in different LDS banks.
+  __shared__ double As[2][kTk][kTile + 2];
+  __shared__ double Bs[2][kTk][kTile + 2];
   const int tid = threadIdx.x;
-  // Output mapping: lane tx owns rows tx + 16 i, ty owns columns ty + 16 j,
-  // so a wavefront's C accesses are 16-row runs of 4 columns (coalesced in
-  // the column-major panel / update matrix).
-  const int tx = tid & 15;
-  const int ty = tid >> 4;
-  double acc[4][4];
+  // FP64 MFMA (16x16x4): each wavefront owns a 32x32 quadrant of the tile,
+  // four 16x16 blocks. The MFMA's A operand is the B chunk (its 16 rows are
+  // tile columns) and its B operand the A chunk (its 16 columns are tile
+  // rows), so that lane l's result e is tile row (l & 15), column
+  // 4 e + (l >> 4) of its block: a wavefront's C accesses are 16-row runs of
+  // 4 columns (coalesced in the column-major panel / update matrix).
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wr = 32 * (wave & 1);
+  const int wc = 32 * (wave >> 1);
+  v4d acc[2][2];  // [column block][row block]
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int rb = 0; rb < 2; ++rb) acc[cb][rb] = v4d{0.0, 0.0, 0.0, 0.0};
   // Staging of one K chunk: each thread loads 4 A and 4 B entries (rows
   // rr = idx & 63 of k row kk = idx >> 6); B is scaled by D_k.
   double ra[4], rb[4];
@@ -1073,17 +1100,20 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots,
   for (int k0 = 0; k0 < K; k0 += kTk) {
     const bool more = k0 + kTk < K;
     if (more) load_chunk(k0 + kTk);  // in flight while this chunk is used
-#pragma unroll 2
-    for (int kk = 0; kk < kTk; ++kk) {
-      double av[4], bv[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[buf][kk][tx + 16 * i];
+    for (int kk = 0; kk < kTk; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      double av[2], bv[2];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[buf][kk][ty + 16 * j];
+      for (int rb = 0; rb < 2; ++rb) av[rb] = As[buf][kr][wr + 16 * rb + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int cb = 0; cb < 2; ++cb) bv[cb] = Bs[buf][kr][wc + 16 * cb + (lane & 15)];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          acc[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[cb], av[rb], acc[cb][rb], 0, 0, 0);
+        }
     }
     if (more) {
       store_chunk(buf ^ 1);
@@ -1092,17 +1122,60 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots,
     }
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int c = C0 + ty + 16 * j;
-    if (c >= N) continue;
+  for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = R0 + tx + 16 * i;
-      if (r >= M || c > r) continue;
-      double* dst = C + static_cast<int64_t>(c) * ldc + r;
-      *dst = mode == 0 ? *dst - acc[i][j] : -acc[i][j];
+    for (int e = 0; e < 4; ++e) {
+      const int c = C0 + wc + 16 * cb + 4 * e + (lane >> 4);
+      if (c >= N) continue;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int r = R0 + wr + 16 * rb + (lane & 15);
+        if (r >= M || c > r) continue;
+        double* dst = C + static_cast<int64_t>(c) * ldc + r;
+        *dst = mode == 0 ? *dst - acc[cb][rb][e] : -acc[cb][rb][e];
+      }
     }
   }
+  if constexpr (kLookAhead) {
+    const int c1 = (kb + 1) * kNB;
+    if (mode != 0 || ti != 0 || tj != 0 || c1 >= ns) return;
+    // This tile is rows and columns c1 .. c1 + 63 of the panel: the next
+    // diagonal block, updated for the last time above.
+    __syncthreads();  // its stores are visible to the workgroup; As is free
+    if (tid >= kNB) return;
+    const int nb1 = min(kNB, ns - c1);
+    double row[kNB];
+    const int tr = min(tid, nb1 - 1);
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      const double v = F[static_cast<int64_t>(c1 + min(j, nb1 - 1)) * f + c1 + tr];
+      row[j] = (tid < nb1 && j < nb1 && j <= tid) ? v : (j == tid ? 1.0 : 0.0);
+    }
+    const bool zero = FactorDiagWave(row, tid, nb1, &As[0][0][0]);
+    double* next = StageSlot(W, s, kb + 1);
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      if (j <= tid && tid < nb1) next[j * kNB + tid] = row[j];
+    }
+    if (tid == 0 && zero) {
+      W.state->not_spd = 1;
+      W.state->halt = 1;
+    }
+  }
+}
+
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots, Lists plan, int kb, int mode,
+                                                          int fronts) {
+  GemmLowerBody<false>(slots, plan, kb, mode, fronts);
+}
+
+// Trailing update with look-ahead (the diagonal factorization needs a row of
+// 64 doubles in registers: fewer workgroups per CU).
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads, 2) KGemmLowerAhead(Slots slots, Lists plan, int kb,
+                                                               int fronts) {
+  GemmLowerBody<true>(slots, plan, kb, 0, fronts);
 }
 
 // ---------------------------------------------------------------- solves
@@ -1723,12 +1796,15 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
       for (int kb = 0; kb < nblocks; ++kb) {
         const int rows = lv.max_f - kb * kNB;
         const int row_blocks = (rows + kStepRows - 1) / kStepRows;
-        // Few fronts: each workgroup factors the diagonal block itself (one
-        // launch on the critical path). Many fronts: factor each block once.
-        const int staged = static_cast<int64_t>(row_blocks) * count * sh.slots > kStagedMinBlocks ? 1 : 0;
+        // Step 0: with few fronts each workgroup factors the diagonal block
+        // itself (one launch on the critical path), with many the block is
+        // factored once per front first. Later steps find their block staged
+        // by the previous step's trailing update (look-ahead).
+        const int staged =
+            kb > 0 || static_cast<int64_t>(row_blocks) * count * sh.slots > kStagedMinBlocks ? 1 : 0;
         const bool timed = step_events != nullptr && step_events->used < step_events->capacity;
         if (timed) (void)hipEventRecord(step_events->events[2 * step_events->used], s);
-        if (staged != 0) KDiagBlock<<<dim3(count, k), kNB, 0, s>>>(slots, plan, kb);
+        if (kb == 0 && staged != 0) KDiagBlock<<<dim3(count, k), kNB, 0, s>>>(slots, plan, kb);
         KPanelStep<<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb, staged);
         if (timed) {
           (void)hipEventRecord(step_events->events[2 * step_events->used + 1], s);
@@ -1738,8 +1814,12 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
         const int trail_cols = lv.max_nb - (kb + 1) * kNB;
         const int tr = trail_rows > 0 ? (trail_rows + kTile - 1) / kTile : 1;
         const int tc = trail_cols > 0 ? (trail_cols + kTile - 1) / kTile : 1;
-        KGemmLower<<<dim3(tr, tc, static_cast<unsigned>(count) * k), kThreads, 0, s>>>(slots, plan, kb, 0,
-                                                                                     count);
+        const dim3 grid(tr, tc, static_cast<unsigned>(count) * k);
+        if (kb + 1 < nblocks) {
+          KGemmLowerAhead<<<grid, kThreads, 0, s>>>(slots, plan, kb, count);
+        } else {
+          KGemmLower<<<grid, kThreads, 0, s>>>(slots, plan, kb, 0, count);
+        }
       }
       if (lv.max_m > 0) {
         const int t = (lv.max_m + kTile - 1) / kTile;

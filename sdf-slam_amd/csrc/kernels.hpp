@@ -100,7 +100,7 @@ struct DeviceWork {
   double* rhs;  // -J^T r in original column order
   double* step;  // step in original column order
   double* panel;
-  double* diag_stage;  // factored 64x64 diagonal block per supernode, in flight
+  double* diag_stage;  // factored 64x64 diagonal blocks in flight, two per supernode
   double* upd;
   double* vec;  // forward-solve vectors, sum of front sizes
   double* xnew;  // solution in elimination order

@@ -640,6 +640,7 @@ int sdfslam_host_batch_benchmark(const sdfslam_batch_options* options, sdfslam_b
     b.instances = options->instances;
     b.iterations = options->iterations;
     b.max_batch = options->max_batch;
+    b.workers = options->workers;
     b.seed0 = options->seed0;
     const BatchResult r = RunBatchBenchmark(b);
     *out = {r.lm_iterations, r.unknowns, r.wall_seconds, r.lm_iterations_per_second};

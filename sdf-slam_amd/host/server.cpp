@@ -426,30 +426,56 @@ BatchResult RunBatchBenchmark(const BatchBenchmark& o) {
   const sdfslam_solver_options copts = ToC(opts);
   BatchResult r;
   for (const auto& p : problems) r.unknowns += p->num_parameters();
-  int64_t iters = 0;
-  const auto start = Clock::now();
+  // Largest first, so each batch holds instances of similar size.
+  std::vector<int> order(static_cast<size_t>(o.instances));
+  for (int i = 0; i < o.instances; ++i) order[static_cast<size_t>(i)] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return problems[static_cast<size_t>(a)]->num_parameters() > problems[static_cast<size_t>(b)]->num_parameters();
+  });
+  const int batches = (o.instances + o.max_batch - 1) / o.max_batch;
+  std::atomic<int> next_batch{0};
+  std::atomic<int64_t> iters{0};
+  std::mutex err_mu;
+  std::string error;
+  // Contexts live until the clock stops (freeing device memory waits for the
+  // whole device).
   std::vector<std::unique_ptr<HipSession>> sessions(static_cast<size_t>(o.instances));
-  for (int first = 0; first < o.instances; first += o.max_batch) {
-    const int last = std::min(o.instances, first + o.max_batch);
-    const int nb = last - first;
-    std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(nb));
-    for (int i = first; i < last; ++i) {
-      sessions[static_cast<size_t>(i)] = std::make_unique<HipSession>();
-      ctxs[static_cast<size_t>(i - first)] = sessions[static_cast<size_t>(i)]->Load(*problems[static_cast<size_t>(i)], opts);
+  const auto start = Clock::now();
+  auto body = [&] {
+    for (int b = next_batch++; b < batches; b = next_batch++) {
+      try {
+        const int first = b * o.max_batch;
+        const int last = std::min(o.instances, first + o.max_batch);
+        const int nb = last - first;
+        std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(nb));
+        for (int q = first; q < last; ++q) {
+          const int i = order[static_cast<size_t>(q)];
+          sessions[static_cast<size_t>(i)] = std::make_unique<HipSession>();
+          ctxs[static_cast<size_t>(q - first)] =
+              sessions[static_cast<size_t>(i)]->Load(*problems[static_cast<size_t>(i)], opts);
+        }
+        std::vector<sdfslam_solve_result> res(static_cast<size_t>(nb));
+        std::vector<int32_t> status(static_cast<size_t>(nb), SDFSLAM_OK);
+        const int st = sdfslam_hip_batch_solve(ctxs.data(), nb, &copts, res.data(), status.data(), nullptr);
+        if (st != SDFSLAM_OK) {
+          throw std::runtime_error(std::string("batch solve failed: ") + sdfslam_hip_last_error());
+        }
+        for (int j = 0; j < nb; ++j) {
+          if (status[static_cast<size_t>(j)] != SDFSLAM_OK) throw std::runtime_error("batch instance failed");
+          iters += res[static_cast<size_t>(j)].iterations;
+        }
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lock(err_mu);
+        if (error.empty()) error = e.what();
+      }
     }
-    std::vector<sdfslam_solve_result> res(static_cast<size_t>(nb));
-    std::vector<int32_t> status(static_cast<size_t>(nb), SDFSLAM_OK);
-    const int st = sdfslam_hip_batch_solve(ctxs.data(), nb, &copts, res.data(), status.data(), nullptr);
-    if (st != SDFSLAM_OK) {
-      throw std::runtime_error(std::string("batch solve failed: ") + sdfslam_hip_last_error());
-    }
-    for (int j = 0; j < nb; ++j) {
-      if (status[static_cast<size_t>(j)] != SDFSLAM_OK) throw std::runtime_error("batch instance failed");
-      iters += res[static_cast<size_t>(j)].iterations;
-    }
-    for (int i = first; i < last; ++i) sessions[static_cast<size_t>(i)].reset();
-  }
+  };
+  std::vector<std::thread> pool;
+  for (int w = 1; w < std::max(1, o.workers); ++w) pool.emplace_back(body);
+  body();
+  for (std::thread& t : pool) t.join();
   r.wall_seconds = Seconds(start, Clock::now());
+  if (!error.empty()) throw std::runtime_error(error);
   r.lm_iterations = iters;
   r.lm_iterations_per_second = static_cast<double>(r.lm_iterations) / std::max(r.wall_seconds, 1e-9);
   return r;

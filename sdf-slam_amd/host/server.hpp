@@ -9,8 +9,9 @@
 // their contexts on host worker threads, solves them together with
 // sdfslam_hip_batch_solve (every stage of an LM iteration is one launch over
 // all of them), and reads the states back. Frames arriving meanwhile join
-// the next batch; with two pipelines one batch is prepared while the other
-// runs on the GPU. Each session's increments run one at a time in
+// the next batch; with several pipelines (scheduler threads) one batch is
+// prepared on the host while others run on the GPU, each pipeline on its own
+// stream. Each session's increments run one at a time in
 // submission order, and a session's results do not depend on what else is
 // being served.
 //
@@ -44,9 +45,9 @@ struct ServerStats {
 };
 
 struct ServerOptions {
-  int workers{8};                  // host threads preparing and collecting a batch
+  int workers{12};                 // host threads preparing a batch
   int max_batch{64};               // sessions per batched solve (1: one at a time)
-  int pipelines{2};                // batches in flight (scheduler threads)
+  int pipelines{4};                // batches in flight (scheduler threads)
   int64_t device_budget_bytes{0};  // 0: 80 % of the device's memory
 };
 
@@ -126,9 +127,9 @@ struct ServedBenchmark {
   int sessions{32};
   double rate_hz{10.0};
   double duration_s{10.0};
-  int workers{8};
+  int workers{12};
   int max_batch{64};
-  int pipelines{2};
+  int pipelines{4};
   uint64_t seed0{2000};
 };
 struct ServedResult {
@@ -154,14 +155,17 @@ struct ServedResult {
 ServedResult RunServedBenchmark(const ServedBenchmark& options);
 
 /// Batched benchmark: `instances` independent batch problems (seeds
-/// seed0 + i, uneven sizes), each solved for `iterations` LM iterations,
-/// `max_batch` per batched solve (1: one after another). Host problem
-/// construction is done before the clock starts; the clock covers device
-/// setup (one context per instance) and the iterations.
+/// seed0 + i, uneven sizes), each solved for `iterations` LM iterations.
+/// The instances, largest first, are cut into batched solves of at most
+/// `max_batch` (similar sizes share a launch, so few workgroups idle), and
+/// `workers` host threads run the batches concurrently, each on its own
+/// stream. Host problem construction is done before the clock starts; the
+/// clock covers device setup (one context per instance) and the iterations.
 struct BatchBenchmark {
   int instances{64};
   int iterations{30};
-  int max_batch{64};
+  int max_batch{8};
+  int workers{8};
   uint64_t seed0{1000};
 };
 struct BatchResult {

@@ -135,7 +135,7 @@ class BatchInfo(C.Structure):
 
 class BatchOptions(C.Structure):
     _fields_ = [("instances", C.c_int32), ("iterations", C.c_int32), ("max_batch", C.c_int32),
-                ("seed0", C.c_uint64)]
+                ("workers", C.c_int32), ("seed0", C.c_uint64)]
 
 
 class BatchResult(C.Structure):
@@ -618,7 +618,7 @@ class Dataset:
 class Server:
     """Served form: many incremental sessions on one GPU (sdfslam_server_*)."""
 
-    def __init__(self, workers=8, device_budget_bytes=0, max_batch=64, pipelines=2):
+    def __init__(self, workers=12, device_budget_bytes=0, max_batch=64, pipelines=4):
         h = C.c_void_p()
         o = ServerOptions(workers, max_batch, pipelines, device_budget_bytes)
         _check_host(host.sdfslam_server_create(C.byref(o), C.byref(h)))
@@ -671,16 +671,16 @@ class Server:
         return poses.reshape(-1, 3), m
 
 
-def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=8, max_batch=64,
-                     pipelines=2, seed0=2000):
+def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=12, max_batch=64,
+                     pipelines=4, seed0=2000):
     o = ServedOptions(sessions, rate_hz, duration_s, workers, max_batch, pipelines, seed0)
     r = ServedResult()
     _check_host(host.sdfslam_host_served_benchmark(C.byref(o), C.byref(r)))
     return {k: getattr(r, k) for k, _ in ServedResult._fields_}
 
 
-def batch_benchmark(instances=64, iterations=30, max_batch=64, seed0=1000):
-    o = BatchOptions(instances, iterations, max_batch, seed0)
+def batch_benchmark(instances=64, iterations=30, max_batch=8, workers=8, seed0=1000):
+    o = BatchOptions(instances, iterations, max_batch, workers, seed0)
     r = BatchResult()
     _check_host(host.sdfslam_host_batch_benchmark(C.byref(o), C.byref(r)))
     return {k: getattr(r, k) for k, _ in BatchResult._fields_}
