@@ -253,6 +253,23 @@ int sdfslam_hip_batch_solve(sdfslam_hip_context* const* contexts, int32_t count,
                             const sdfslam_solver_options* options, sdfslam_solve_result* results,
                             int32_t* statuses, sdfslam_hip_batch_info* info);
 
+/* Node normals of a problem build on the device, for incremental and served
+ * runs whose problem is rebuilt every increment (problem.cpp:72-94 via
+ * normals.cpp:128-220, the all-scans neighbourhood; the per-scan variant
+ * stays on the host): the kNN PCA normal of every scan point (points_xy and
+ * their sensor origins origins_xy are x, y pairs in world coordinates, k =
+ * k_neighbors, at most 32), then per grid node the normal of the nearest
+ * scan point and its eikonal scale for `method`. Exact neighbour sets with
+ * ties broken by point index and the same arithmetic as the host builder,
+ * so the outputs equal AssignNodeNormals(ComputeScanNormals(...)) bit for
+ * bit. node_normal_xy receives 2 * nx * ny values, node_scale nx * ny. */
+#define SDFSLAM_NORMALS_PCA 0
+#define SDFSLAM_NORMALS_HYBRID 1
+#define SDFSLAM_NORMALS_WEIGHTED 2
+int sdfslam_hip_node_normals(const double* points_xy, const double* origins_xy, int64_t num_points,
+                             int32_t k_neighbors, const sdfslam_grid* grid, int32_t method,
+                             double corner_threshold, double* node_normal_xy, double* node_scale);
+
 /* Runs exactly `iterations` LM iterations from the current device state,
  * carrying the LM state (lambda, previous cost) in the context and ignoring
  * max_iterations; used to time iterations. With collect_phase_times != 0

@@ -66,6 +66,14 @@ int sdfslam_host_problem_create(const sdfslam_grid* grid, const double* map_valu
                                 const sdfslam_problem_options* options,
                                 sdfslam_host_problem** out);
 
+/* Node normals on the host (ComputeScanNormals' all-scans neighbourhood and
+ * AssignNodeNormals, normals.cpp:128-220), with the arguments and outputs
+ * of sdfslam_hip_node_normals: the reference values the device ones are
+ * checked against. */
+int sdfslam_host_node_normals(const double* points_xy, const double* origins_xy, int64_t num_points,
+                              int32_t k_neighbors, const sdfslam_grid* grid, int32_t method,
+                              double corner_threshold, double* node_normal_xy, double* node_scale);
+
 /* Builds the Problem of a seeded synthetic workload at its initial state:
  * "room", "room_small", "office150", "office200", "office_small", "corner". */
 int sdfslam_host_workload_create(const char* name, uint64_t seed, sdfslam_host_problem** out);

@@ -19,6 +19,7 @@
 
 #include "analysis.hpp"
 #include "kernels.hpp"
+#include "normals.hpp"
 #include "sdfslam_hip.h"
 
 using namespace sdfslam;
@@ -1812,6 +1813,24 @@ int sdfslam_hip_batch_solve(sdfslam_hip_context* const* contexts, int32_t count,
         results[b].duration_seconds = seconds;
       }
     }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_node_normals(const double* points_xy, const double* origins_xy, int64_t num_points,
+                             int32_t k_neighbors, const sdfslam_grid* grid, int32_t method,
+                             double corner_threshold, double* node_normal_xy, double* node_scale) {
+  return Guard([&]() -> int {
+    if (grid == nullptr || node_normal_xy == nullptr || node_scale == nullptr || num_points < 0 ||
+        (num_points > 0 && (points_xy == nullptr || origins_xy == nullptr))) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    if (grid->nx < 2 || grid->ny < 2 || method < SDFSLAM_NORMALS_PCA || method > SDFSLAM_NORMALS_WEIGHTED) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "bad grid or normal method");
+    }
+    const int st = NodeNormalsOnDevice(points_xy, origins_xy, num_points, k_neighbors, *grid, method,
+                                       corner_threshold, node_normal_xy, node_scale);
+    if (st != SDFSLAM_OK) throw Failure(st, NormalsLastError());
     return SDFSLAM_OK;
   });
 }

@@ -1,0 +1,432 @@
+// Scan normals and node normals on the device: the per-increment normal
+// estimation of the problem build, restated from normals.cpp:70-103 (kNN
+// PCA normal of a scan point, oriented toward its sensor origin),
+// normals.cpp:128-160 (one neighbourhood structure over all scans) and
+// normals.cpp:162-220 (each grid node takes the normal of its nearest scan
+// point; pca / hybrid / weighted).
+//
+// Neighbour search: a uniform grid over the points (counting sort by cell
+// with integer atomics; the order inside a cell does not matter) and, per
+// query, rings of cells around the query's cell until no unvisited point can
+// enter the k best. The k best are kept sorted by (squared distance, point
+// index), the order an exact search with ties broken by index returns, and
+// the PCA sums run in that order; with contraction off (-ffp-contract=off)
+// every value equals the host builder's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "normals.hpp"
+
+namespace sdfslam {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxK = 32;
+
+struct PointGridDev {
+  double x0, y0, cell;
+  int gx, gy;
+};
+
+__device__ __forceinline__ int CellCoord(double v, double v0, double cell, int n) {
+  const double f = floor((v - v0) / cell);
+  return f < 0.0 ? 0 : (f >= n ? n - 1 : static_cast<int>(f));
+}
+
+__global__ void KCellCount(const double* __restrict__ px, const double* __restrict__ py, int64_t n,
+                           PointGridDev g, int* __restrict__ cell_of, int* __restrict__ count) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const int c = CellCoord(py[i], g.y0, g.cell, g.gy) * g.gx + CellCoord(px[i], g.x0, g.cell, g.gx);
+  cell_of[i] = c;
+  atomicAdd(&count[c], 1);
+}
+
+// Exclusive scan of count[0 .. cells) into start[0 .. cells] (one
+// workgroup of kScanThreads; serial over chunks).
+constexpr int kScanThreads = 1024;
+__global__ void __launch_bounds__(kScanThreads) KCellScan(const int* __restrict__ count, int cells,
+                                                          int* __restrict__ start) {
+  __shared__ int part[kScanThreads];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < cells; base += kScanThreads) {
+    const int i = base + static_cast<int>(threadIdx.x);
+    const int v = i < cells ? count[i] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+      const int t = threadIdx.x >= static_cast<unsigned>(off) ? part[threadIdx.x - off] : 0;
+      __syncthreads();
+      part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < cells) start[i] = carry + part[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == kScanThreads - 1) carry += part[kScanThreads - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) start[cells] = carry;
+}
+
+__global__ void KCellScatter(const int* __restrict__ cell_of, int64_t n, const int* __restrict__ start,
+                             int* __restrict__ fill, int* __restrict__ sorted) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_of[i];
+  sorted[start[c] + atomicAdd(&fill[c], 1)] = static_cast<int>(i);
+}
+
+// Inserts point i at squared distance dd into the k best, kept sorted by
+// (distance, index) (the host builder's rule).
+__device__ __forceinline__ void Insert(int i, double dd, int k, int& found, int* idx, double* d2) {
+  if (found == k) {
+    if (dd > d2[k - 1] || (dd == d2[k - 1] && i > idx[k - 1])) return;
+    --found;
+  }
+  int pos = found;
+  while (pos > 0 && (d2[pos - 1] > dd || (d2[pos - 1] == dd && idx[pos - 1] > i))) {
+    d2[pos] = d2[pos - 1];
+    idx[pos] = idx[pos - 1];
+    --pos;
+  }
+  d2[pos] = dd;
+  idx[pos] = i;
+  ++found;
+}
+
+// The k nearest points to (qx, qy), sorted by (distance, index); returns
+// min(k, n). Ring r holds the cells at Chebyshev distance r from the
+// query's cell (clamped into the grid); a point in ring r + 1 or beyond is
+// farther than r cells from the query, and farther than r - 1 cells even if
+// rounding put it in a neighbouring cell, so after ring r the search stops
+// once the k-th distance is below (r - 1) cells.
+__device__ int Knn(double qx, double qy, int k, const double* __restrict__ px, const double* __restrict__ py,
+                   const PointGridDev& g, const int* __restrict__ start, const int* __restrict__ sorted,
+                   int* idx, double* d2) {
+  const int cx = CellCoord(qx, g.x0, g.cell, g.gx);
+  const int cy = CellCoord(qy, g.y0, g.cell, g.gy);
+  const int rmax = max(max(cx, g.gx - 1 - cx), max(cy, g.gy - 1 - cy));
+  int found = 0;
+  for (int r = 0; r <= rmax; ++r) {
+    for (int yy = max(cy - r, 0); yy <= min(cy + r, g.gy - 1); ++yy) {
+      const bool edge = yy == cy - r || yy == cy + r;
+      const int step = edge || r == 0 ? 1 : 2 * r;
+      for (int xx = cx - r; xx <= cx + r; xx += step) {
+        if (xx < 0 || xx >= g.gx) continue;
+        const int c = yy * g.gx + xx;
+        for (int t = start[c]; t < start[c + 1]; ++t) {
+          const int i = sorted[t];
+          const double dx = px[i] - qx;
+          const double dy = py[i] - qy;
+          Insert(i, dx * dx + dy * dy, k, found, idx, d2);
+        }
+      }
+    }
+    if (found == k && r >= 2) {
+      const double lim = static_cast<double>(r - 1) * g.cell;
+      if (d2[k - 1] < lim * lim) break;
+    }
+  }
+  return found;
+}
+
+// kNN PCA normal of every point (PcaNormal, normals.cpp:70-103): the
+// eigenvector of the smaller covariance eigenvalue (closed form), oriented
+// toward the point's origin, and the eigenvalue ratio.
+__global__ void KPointNormals(const double* __restrict__ px, const double* __restrict__ py,
+                              const double* __restrict__ ox, const double* __restrict__ oy, int64_t n, int k,
+                              PointGridDev g, const int* __restrict__ start, const int* __restrict__ sorted,
+                              double* __restrict__ nrm_x, double* __restrict__ nrm_y, double* __restrict__ corner) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  int idx[kMaxK];
+  double d2[kMaxK];
+  const double qx = px[i];
+  const double qy = py[i];
+  const int found = Knn(qx, qy, k, px, py, g, start, sorted, idx, d2);
+  double mx = 0.0;
+  double my = 0.0;
+  for (int j = 0; j < found; ++j) {
+    mx = mx + px[idx[j]];
+    my = my + py[idx[j]];
+  }
+  mx = mx / static_cast<double>(found);
+  my = my / static_cast<double>(found);
+  double a = 0.0;
+  double b = 0.0;
+  double c = 0.0;
+  for (int j = 0; j < found; ++j) {
+    const double dx = px[idx[j]] - mx;
+    const double dy = py[idx[j]] - my;
+    a += dx * dx;
+    b += dx * dy;
+    c += dy * dy;
+  }
+  const double half_tr = 0.5 * (a + c);
+  const double half_diff = 0.5 * (a - c);
+  const double rad = sqrt(half_diff * half_diff + b * b);
+  const double l_min = half_tr - rad;
+  const double l_max = half_tr + rad;
+  const double v1x = b;
+  const double v1y = l_min - a;
+  const double v2x = l_min - c;
+  const double v2y = b;
+  const double n1 = sqrt(v1x * v1x + v1y * v1y);
+  const double n2 = sqrt(v2x * v2x + v2y * v2y);
+  double nx;
+  double ny;
+  if (n1 < 1e-300 && n2 < 1e-300) {
+    nx = a <= c ? 1.0 : 0.0;
+    ny = a <= c ? 0.0 : 1.0;
+  } else if (n1 >= n2) {
+    nx = v1x / n1;
+    ny = v1y / n1;
+  } else {
+    nx = v2x / n2;
+    ny = v2y / n2;
+  }
+  if (nx * (ox[i] - qx) + ny * (oy[i] - qy) < 0.0) {
+    nx = -nx;
+    ny = -ny;
+  }
+  nrm_x[i] = nx;
+  nrm_y[i] = ny;
+  corner[i] = fmax(l_min, 0.0) / fmax(l_max, 1e-12);
+}
+
+// Normal and eikonal scale of every grid node from its nearest scan point
+// (AssignNodeNormals, normals.cpp:162-220).
+__global__ void KNodeNormals(int nx, int ny, double min_x, double min_y, double dx, double dy,
+                             const double* __restrict__ px, const double* __restrict__ py,
+                             const double* __restrict__ pnx, const double* __restrict__ pny,
+                             const double* __restrict__ pcorner, PointGridDev g, const int* __restrict__ start,
+                             const int* __restrict__ sorted, int method, double corner_threshold,
+                             double* __restrict__ out_nx, double* __restrict__ out_ny,
+                             double* __restrict__ out_scale) {
+  const int64_t node = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (node >= static_cast<int64_t>(nx) * ny) return;
+  const int w = static_cast<int>(node % nx);
+  const int h = static_cast<int>(node / nx);
+  const double qx = min_x + w * dx;
+  const double qy = min_y + h * dy;
+  int nearest = 0;
+  double dist = 0.0;
+  Knn(qx, qy, 1, px, py, g, start, sorted, &nearest, &dist);
+  const double n_x = pnx[nearest];
+  const double n_y = pny[nearest];
+  const double corner = fmin(fmax(pcorner[nearest] / corner_threshold, 0.0), 1.0);
+  double rx = n_x;
+  double ry = n_y;
+  double scale = 1.0;
+  if (method == SDFSLAM_NORMALS_HYBRID) {
+    const double tx = px[nearest] - qx;
+    const double ty = py[nearest] - qy;
+    const double tn = sqrt(tx * tx + ty * ty);
+    if (!(tn < 1e-9)) {
+      double cx = tx / tn;
+      double cy = ty / tn;
+      if (cx * n_x + cy * n_y < 0.0) {
+        cx = -cx;
+        cy = -cy;
+      }
+      const double bx = n_x * (1.0 - corner) + cx * corner;
+      const double by = n_y * (1.0 - corner) + cy * corner;
+      const double bn = sqrt(bx * bx + by * by);
+      if (bn > 1e-9) {
+        rx = bx / bn;
+        ry = by / bn;
+      }
+    }
+  } else if (method == SDFSLAM_NORMALS_WEIGHTED) {
+    scale = 1.0 - corner;
+  }
+  out_nx[node] = rx;
+  out_ny[node] = ry;
+  out_scale[node] = scale;
+}
+
+#define HIPN(x)                                        \
+  do {                                                 \
+    const hipError_t e_ = (x);                         \
+    if (e_ != hipSuccess) return Fail(e_, #x);         \
+  } while (0)
+
+thread_local std::string t_error;
+
+int Fail(hipError_t e, const char* what) {
+  t_error = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? SDFSLAM_ERR_OOM : SDFSLAM_ERR_HIP;
+}
+
+/// Device buffers of one call, grown as needed and never freed while the
+/// process runs (freeing device memory waits for the whole device);
+/// borrowed from a process-wide pool so that concurrent calls do not share.
+struct Workspace {
+  size_t bytes{0};
+  char* dev{nullptr};
+  hipStream_t stream{nullptr};
+};
+
+class WorkspacePool {
+ public:
+  static WorkspacePool& Get() {
+    static WorkspacePool* pool = new WorkspacePool();  // lives until the process ends
+    return *pool;
+  }
+  Workspace* Borrow() {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (!free_.empty()) {
+      Workspace* w = free_.back();
+      free_.pop_back();
+      return w;
+    }
+    return new Workspace();
+  }
+  void Return(Workspace* w) {
+    std::lock_guard<std::mutex> lock(mu_);
+    free_.push_back(w);
+  }
+
+ private:
+  std::mutex mu_;
+  std::vector<Workspace*> free_;
+};
+
+struct Lease {
+  Workspace* w;
+  Lease() : w(WorkspacePool::Get().Borrow()) {}
+  ~Lease() { WorkspacePool::Get().Return(w); }
+};
+
+}  // namespace
+
+const char* NormalsLastError() { return t_error.c_str(); }
+
+int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64_t num_points, int32_t k,
+                        const sdfslam_grid& grid, int32_t method, double corner_threshold,
+                        double* node_normal_xy, double* node_scale) {
+  const int64_t nodes = static_cast<int64_t>(grid.nx) * grid.ny;
+  if (num_points <= 0) {
+    // No scan points: every node keeps the default (zero normal, scale 1).
+    for (int64_t v = 0; v < nodes; ++v) {
+      node_normal_xy[2 * v] = 0.0;
+      node_normal_xy[2 * v + 1] = 0.0;
+      node_scale[v] = 1.0;
+    }
+    return SDFSLAM_OK;
+  }
+  const int64_t n = num_points;
+  if (k > kMaxK) {
+    t_error = "k_neighbors above " + std::to_string(kMaxK);
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  if (n > (int64_t{1} << 30) || static_cast<int64_t>(grid.nx) * grid.ny > (int64_t{1} << 30)) {
+    t_error = "too many points or nodes";
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  const int kk = static_cast<int>(std::min<int64_t>(std::max(2, k), n));
+  // Grid: about two points per cell over the points' bounding box.
+  double x0 = points_xy[0];
+  double y0 = points_xy[1];
+  double x1 = x0;
+  double y1 = y0;
+  for (int64_t i = 1; i < n; ++i) {
+    x0 = std::min(x0, points_xy[2 * i]);
+    x1 = std::max(x1, points_xy[2 * i]);
+    y0 = std::min(y0, points_xy[2 * i + 1]);
+    y1 = std::max(y1, points_xy[2 * i + 1]);
+  }
+  const double extent = std::max({x1 - x0, y1 - y0, 1e-6});
+  double cell = std::sqrt((x1 - x0 + 1e-6) * (y1 - y0 + 1e-6) * 2.0 / static_cast<double>(n));
+  cell = std::max(cell, extent / 2048.0);
+  PointGridDev g{x0, y0, cell, static_cast<int>((x1 - x0) / cell) + 1, static_cast<int>((y1 - y0) / cell) + 1};
+  const int64_t cells = static_cast<int64_t>(g.gx) * g.gy;
+  // Layout of the workspace (256-byte aligned parts).
+  auto align = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+  const size_t nd = align(static_cast<size_t>(n) * sizeof(double));
+  const size_t ni = align(static_cast<size_t>(n) * sizeof(int));
+  const size_t ci = align(static_cast<size_t>(cells + 1) * sizeof(int));
+  const size_t vd = align(static_cast<size_t>(nodes) * sizeof(double));
+  const size_t need = 7 * nd + 2 * ni + 3 * ci + 3 * vd;
+  Lease lease;
+  Workspace& ws = *lease.w;
+  if (ws.stream == nullptr) HIPN(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
+  if (need > ws.bytes) {
+    // The outgrown buffer is kept (never freed while the process runs).
+    ws.dev = nullptr;
+    ws.bytes = 0;
+    const size_t want = need + need / 2;
+    HIPN(hipMalloc(&ws.dev, want));
+    ws.bytes = want;
+  }
+  char* p = ws.dev;
+  auto take = [&p](size_t b) {
+    char* q = p;
+    p += b;
+    return q;
+  };
+  double* px = reinterpret_cast<double*>(take(nd));
+  double* py = reinterpret_cast<double*>(take(nd));
+  double* ox = reinterpret_cast<double*>(take(nd));
+  double* oy = reinterpret_cast<double*>(take(nd));
+  double* pnx = reinterpret_cast<double*>(take(nd));
+  double* pny = reinterpret_cast<double*>(take(nd));
+  double* pcorner = reinterpret_cast<double*>(take(nd));
+  int* cell_of = reinterpret_cast<int*>(take(ni));
+  int* sorted = reinterpret_cast<int*>(take(ni));
+  int* count = reinterpret_cast<int*>(take(ci));
+  int* start = reinterpret_cast<int*>(take(ci));
+  int* fill = reinterpret_cast<int*>(take(ci));
+  double* vnx = reinterpret_cast<double*>(take(vd));
+  double* vny = reinterpret_cast<double*>(take(vd));
+  double* vscale = reinterpret_cast<double*>(take(vd));
+  // Split the interleaved host arrays.
+  std::vector<double> hx(static_cast<size_t>(n)), hy(static_cast<size_t>(n)), hox(static_cast<size_t>(n)),
+      hoy(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) {
+    hx[static_cast<size_t>(i)] = points_xy[2 * i];
+    hy[static_cast<size_t>(i)] = points_xy[2 * i + 1];
+    hox[static_cast<size_t>(i)] = origins_xy[2 * i];
+    hoy[static_cast<size_t>(i)] = origins_xy[2 * i + 1];
+  }
+  hipStream_t s = ws.stream;
+  const size_t nb = static_cast<size_t>(n) * sizeof(double);
+  HIPN(hipMemcpyAsync(px, hx.data(), nb, hipMemcpyHostToDevice, s));
+  HIPN(hipMemcpyAsync(py, hy.data(), nb, hipMemcpyHostToDevice, s));
+  HIPN(hipMemcpyAsync(ox, hox.data(), nb, hipMemcpyHostToDevice, s));
+  HIPN(hipMemcpyAsync(oy, hoy.data(), nb, hipMemcpyHostToDevice, s));
+  HIPN(hipMemsetAsync(count, 0, static_cast<size_t>(cells + 1) * sizeof(int), s));
+  HIPN(hipMemsetAsync(fill, 0, static_cast<size_t>(cells + 1) * sizeof(int), s));
+  const unsigned pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
+  KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, g, cell_of, count);
+  KCellScan<<<1, kScanThreads, 0, s>>>(count, static_cast<int>(cells), start);
+  KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
+  KPointNormals<<<pb, kThreads, 0, s>>>(px, py, ox, oy, n, kk, g, start, sorted, pnx, pny, pcorner);
+  const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
+  KNodeNormals<<<vb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, pnx,
+                                       pny, pcorner, g, start, sorted, method, corner_threshold, vnx, vny,
+                                       vscale);
+  HIPN(hipGetLastError());
+  std::vector<double> rx(static_cast<size_t>(nodes)), ry(static_cast<size_t>(nodes));
+  const size_t vb_bytes = static_cast<size_t>(nodes) * sizeof(double);
+  HIPN(hipMemcpyAsync(rx.data(), vnx, vb_bytes, hipMemcpyDeviceToHost, s));
+  HIPN(hipMemcpyAsync(ry.data(), vny, vb_bytes, hipMemcpyDeviceToHost, s));
+  HIPN(hipMemcpyAsync(node_scale, vscale, vb_bytes, hipMemcpyDeviceToHost, s));
+  HIPN(hipStreamSynchronize(s));
+  for (int64_t v = 0; v < nodes; ++v) {
+    node_normal_xy[2 * v] = rx[static_cast<size_t>(v)];
+    node_normal_xy[2 * v + 1] = ry[static_cast<size_t>(v)];
+  }
+  return SDFSLAM_OK;
+}
+
+}  // namespace sdfslam

@@ -1,0 +1,20 @@
+// Device normal estimation of the problem build (see normals.hip).
+#pragma once
+
+#include <cstdint>
+
+#include "sdfslam_hip.h"
+
+namespace sdfslam {
+
+/// Node normals and eikonal scales (nx * ny each, node-major, the normal as
+/// x, y pairs) from scan points in world coordinates and their sensor
+/// origins (x, y pairs): kNN PCA normal per point over all points, then the
+/// nearest point's normal per node, with the host builder's values bit for
+/// bit. Returns an SDFSLAM_* status; NormalsLastError() names a failure.
+int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64_t num_points, int32_t k,
+                        const sdfslam_grid& grid, int32_t method, double corner_threshold,
+                        double* node_normal_xy, double* node_scale);
+const char* NormalsLastError();
+
+}  // namespace sdfslam

@@ -83,6 +83,16 @@ class GridMap {
     }
   }
 
+  /// A grid given by its spacing (as a sdfslam_grid carries it) rather than
+  /// its corners, so node positions are min + w * dx with this exact dx.
+  static GridMap FromSpacing(int nx, int ny, const Vec2& min_corner, double dx, double dy) {
+    GridMap g(nx, ny, min_corner, Vec2{min_corner.x + 1.0, min_corner.y + 1.0});
+    g.dx_ = dx;
+    g.dy_ = dy;
+    if (!(dx > 0.0) || !(dy > 0.0)) throw std::invalid_argument("grid needs dx, dy > 0");
+    return g;
+  }
+
   [[nodiscard]] int nx() const { return nx_; }
   [[nodiscard]] int ny() const { return ny_; }
   [[nodiscard]] double dx() const { return dx_; }

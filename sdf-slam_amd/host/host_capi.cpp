@@ -241,6 +241,41 @@ int sdfslam_host_workload_create(const char* name, uint64_t seed, sdfslam_host_p
 
 void sdfslam_host_problem_destroy(sdfslam_host_problem* problem) { delete problem; }
 
+int sdfslam_host_node_normals(const double* points_xy, const double* origins_xy, int64_t num_points,
+                              int32_t k_neighbors, const sdfslam_grid* grid, int32_t method,
+                              double corner_threshold, double* node_normal_xy, double* node_scale) {
+  return Guard([&]() -> int {
+    if (grid == nullptr || node_normal_xy == nullptr || node_scale == nullptr || num_points < 0 ||
+        (num_points > 0 && (points_xy == nullptr || origins_xy == nullptr))) {
+      throw std::invalid_argument("null argument");
+    }
+    if (method < SDFSLAM_NORMALS_PCA || method > SDFSLAM_NORMALS_WEIGHTED) {
+      throw std::invalid_argument("bad normal method");
+    }
+    std::vector<Vec2> points(static_cast<size_t>(num_points));
+    std::vector<Vec2> origins(static_cast<size_t>(num_points));
+    for (int64_t i = 0; i < num_points; ++i) {
+      points[static_cast<size_t>(i)] = Vec2{points_xy[2 * i], points_xy[2 * i + 1]};
+      origins[static_cast<size_t>(i)] = Vec2{origins_xy[2 * i], origins_xy[2 * i + 1]};
+    }
+    const GridMap map = GridMap::FromSpacing(grid->nx, grid->ny, Vec2{grid->min_x, grid->min_y}, grid->dx, grid->dy);
+    NormalOptions options;
+    options.method = method == SDFSLAM_NORMALS_PCA      ? NormalMethod::kPca
+                     : method == SDFSLAM_NORMALS_HYBRID ? NormalMethod::kHybrid
+                                                        : NormalMethod::kWeighted;
+    options.k_neighbors = k_neighbors;
+    options.corner_threshold = corner_threshold;
+    const std::vector<NodeNormal> nn =
+        AssignNodeNormals(map, ScanNormalsOf(std::move(points), origins, k_neighbors), options);
+    for (size_t v = 0; v < nn.size(); ++v) {
+      node_normal_xy[2 * v] = nn[v].normal.x;
+      node_normal_xy[2 * v + 1] = nn[v].normal.y;
+      node_scale[v] = nn[v].eikonal_scale;
+    }
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_host_problem_view(const sdfslam_host_problem* problem, sdfslam_problem_view* out) {
   if (problem == nullptr || out == nullptr) {
     return Fail(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");

@@ -117,19 +117,32 @@ ScanNormals ComputeScanNormals(const std::vector<Scan>& scans, const std::vector
     return result;
   }
   // normals.cpp:128-160: one neighbourhood structure over all scans.
+  std::vector<Vec2> points;
   std::vector<Vec2> origins;
   for (size_t i = 0; i < scans.size(); ++i) {
     for (const Vec2& p : scans[i].points) {
-      result.points.push_back(poses[i].Apply(p));
+      points.push_back(poses[i].Apply(p));
       origins.push_back(poses[i].Translation());
     }
   }
+  return ScanNormalsOf(std::move(points), origins, k_neighbors);
+}
+
+ScanNormals ScanNormalsOf(std::vector<Vec2> points, const std::vector<Vec2>& origins, int k_neighbors) {
+  if (points.size() != origins.size()) {
+    throw std::invalid_argument("point count != origin count");
+  }
+  const int k = std::max(2, k_neighbors);
+  ScanNormals result;
+  result.points = std::move(points);
   const size_t n = result.points.size();
   result.normals.resize(n);
   result.cornerness.resize(n);
   if (n == 0) {
     return result;
   }
+  std::vector<int> idx(static_cast<size_t>(k));
+  std::vector<double> d2(static_cast<size_t>(k));
   const PointGrid grid(result.points);
   const int kk = static_cast<int>(std::min<size_t>(static_cast<size_t>(k), n));
   for (size_t i = 0; i < n; ++i) {
@@ -191,7 +204,7 @@ std::vector<NodeNormal> AssignNodeNormals(const GridMap& map, const ScanNormals&
 
 Problem::Problem(GridMap map, std::vector<Pose2> poses, const std::vector<Scan>& scans,
                  const std::vector<Pose2>& odometry, const ProblemOptions& options,
-                 const std::vector<RelationMeasurement>& relations)
+                 const std::vector<RelationMeasurement>& relations, const NodeNormalsFn& node_normals_fn)
     : map_(std::move(map)),
       poses_(std::move(poses)),
       huber_delta_(options.huber_delta),
@@ -245,9 +258,23 @@ Problem::Problem(GridMap map, std::vector<Pose2> poses, const std::vector<Scan>&
     }
   }
   // Eikonal rows (problem.cpp:72-94).
-  const ScanNormals scan_normals =
-      ComputeScanNormals(scans, poses_, options.normals.k_neighbors, options.normals.per_scan);
-  const std::vector<NodeNormal> node_normals = AssignNodeNormals(map_, scan_normals, options.normals);
+  std::vector<NodeNormal> node_normals;
+  if (node_normals_fn && !options.normals.per_scan) {
+    // The all-scans neighbourhood's inputs (ComputeScanNormals' order).
+    std::vector<Vec2> points;
+    std::vector<Vec2> origins;
+    for (size_t i = 0; i < scans.size(); ++i) {
+      for (const Vec2& p : scans[i].points) {
+        points.push_back(poses_[i].Apply(p));
+        origins.push_back(poses_[i].Translation());
+      }
+    }
+    node_normals = node_normals_fn(points, origins, map_, options.normals);
+  } else {
+    const ScanNormals scan_normals =
+        ComputeScanNormals(scans, poses_, options.normals.k_neighbors, options.normals.per_scan);
+    node_normals = AssignNodeNormals(map_, scan_normals, options.normals);
+  }
   const double sqrt_we = std::sqrt(options.weights.eikonal);
   for (int h = 0; h + 1 < map_.ny(); ++h) {
     for (int w = 0; w + 1 < map_.nx(); ++w) {

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 #include "geometry.hpp"
@@ -109,9 +110,21 @@ std::vector<HallucinatedPoint> GenerateHallucinatedPoints(const Scan& scan,
 ScanNormals ComputeScanNormals(const std::vector<Scan>& scans, const std::vector<Pose2>& poses,
                                int k_neighbors, bool per_scan = false);
 
+/// The all-scans branch of ComputeScanNormals on points already in world
+/// coordinates, each with its sensor origin.
+ScanNormals ScanNormalsOf(std::vector<Vec2> points, const std::vector<Vec2>& origins, int k_neighbors);
+
 /// Nearest-scan-point normal per grid node (normals.cpp:162-220).
 std::vector<NodeNormal> AssignNodeNormals(const GridMap& map, const ScanNormals& scan_normals,
                                           const NormalOptions& options);
+
+/// Node normals computed elsewhere (the device backend): from every scan
+/// point in world coordinates and its sensor origin (frame by frame), the
+/// values AssignNodeNormals(ComputeScanNormals(...)) gives with the
+/// all-scans neighbourhood.
+using NodeNormalsFn = std::function<std::vector<NodeNormal>(
+    const std::vector<Vec2>& points, const std::vector<Vec2>& origins, const GridMap& map,
+    const NormalOptions& options)>;
 
 class Problem {
  public:
@@ -119,9 +132,12 @@ class Problem {
   /// (scan points, then hallucinated points), the active node set, eikonal
   /// rows h-major, consecutive odometry, then relations with frame_j below
   /// the frame count.
+  /// With `node_normals` (and the all-scans neighbourhood) the node normals
+  /// come from that function instead of the host computation.
   Problem(GridMap map, std::vector<Pose2> poses, const std::vector<Scan>& scans,
           const std::vector<Pose2>& odometry, const ProblemOptions& options,
-          const std::vector<RelationMeasurement>& relations = {});
+          const std::vector<RelationMeasurement>& relations = {},
+          const NodeNormalsFn& node_normals = nullptr);
 
   [[nodiscard]] const GridMap& map() const { return map_; }
   GridMap& map() { return map_; }

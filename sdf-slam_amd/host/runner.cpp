@@ -71,6 +71,34 @@ SolveResult HipSession::Collect(Problem& problem, const sdfslam_solve_result& r,
   return result;
 }
 
+std::vector<NodeNormal> DeviceNodeNormals(const std::vector<Vec2>& points, const std::vector<Vec2>& origins,
+                                          const GridMap& map, const NormalOptions& options) {
+  const size_t n = points.size();
+  std::vector<double> xy(2 * n);
+  std::vector<double> oxy(2 * n);
+  for (size_t i = 0; i < n; ++i) {
+    xy[2 * i] = points[i].x;
+    xy[2 * i + 1] = points[i].y;
+    oxy[2 * i] = origins[i].x;
+    oxy[2 * i + 1] = origins[i].y;
+  }
+  const sdfslam_grid g{map.nx(), map.ny(), map.min_corner().x, map.min_corner().y, map.dx(), map.dy()};
+  const auto nodes = static_cast<size_t>(map.num_nodes());
+  std::vector<double> nxy(2 * nodes);
+  std::vector<double> scale(nodes);
+  const int method = options.method == NormalMethod::kPca      ? SDFSLAM_NORMALS_PCA
+                     : options.method == NormalMethod::kHybrid ? SDFSLAM_NORMALS_HYBRID
+                                                               : SDFSLAM_NORMALS_WEIGHTED;
+  Rethrow(sdfslam_hip_node_normals(xy.data(), oxy.data(), static_cast<int64_t>(n), options.k_neighbors, &g,
+                                   method, options.corner_threshold, nxy.data(), scale.data()));
+  std::vector<NodeNormal> out(nodes);
+  for (size_t v = 0; v < nodes; ++v) {
+    out[v].normal = Vec2{nxy[2 * v], nxy[2 * v + 1]};
+    out[v].eikonal_scale = scale[v];
+  }
+  return out;
+}
+
 SolveResult HipSession::Solve(Problem& problem, const SolverOptions& options) {
   const auto t0 = std::chrono::steady_clock::now();
   const sdfslam_problem_view view = problem.View();
@@ -130,8 +158,9 @@ IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, i
                                          wl.scans.begin() + static_cast<std::ptrdiff_t>(next_frame));
     const std::vector<Pose2> odom_so_far(wl.odometry.begin(),
                                          wl.odometry.begin() + static_cast<std::ptrdiff_t>(next_frame - 1));
+    // The device runs build their node normals on the device too.
     Problem problem(std::move(run.map), run.estimated, scans_so_far, odom_so_far, wl.problem,
-                    wl.relations);
+                    wl.relations, solve_fn ? NodeNormalsFn() : NodeNormalsFn(DeviceNodeNormals));
     rec.build_seconds = Seconds(t0);
     t0 = std::chrono::steady_clock::now();
     SolveResult step;

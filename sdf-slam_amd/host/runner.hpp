@@ -72,6 +72,11 @@ using IncrementSolveFn = std::function<int(const sdfslam_problem_view*, const sd
 /// when <= 0), `increment_size` frames per increment, each refined with
 /// `solver` (whose max_iterations is the iterations per increment). With a
 /// null `solve_fn` the device backend solves through one HipSession.
+/// Node normals of a problem build on the device (sdfslam_hip_node_normals):
+/// a NodeNormalsFn for Problem's constructor.
+std::vector<NodeNormal> DeviceNodeNormals(const std::vector<Vec2>& points, const std::vector<Vec2>& origins,
+                                          const GridMap& map, const NormalOptions& options);
+
 IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, int increment_size,
                               int max_frames, const IncrementSolveFn& solve_fn = nullptr);
 

@@ -161,7 +161,9 @@ void Server::RunBatch(const std::vector<Session*>& batch, std::vector<Pending>& 
   std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(k), nullptr);
   const auto t_prepare = Clock::now();
   // Prepare: runner.cpp:240-260 for one appended frame, then load the
-  // session's context.
+  // session's context. The builds of a batch's sessions run in parallel on
+  // host cores; their node normals stay on the host, where they do not
+  // queue behind the batched solves of the other pipelines on the GPU.
   ForEachParallel(k, options_.workers, [&](int i) {
     Session& s = *batch[static_cast<size_t>(i)];
     Pending& p = frames[static_cast<size_t>(i)];

@@ -156,7 +156,7 @@ HIP_SYMBOLS = [
     "sdfslam_hip_normal_equations", "sdfslam_hip_linear_step", "sdfslam_hip_cost",
     "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
     "sdfslam_hip_context_update", "sdfslam_hip_device_memory", "sdfslam_hip_batch_solve",
-    "sdfslam_hip_pose_margins",
+    "sdfslam_hip_pose_margins", "sdfslam_hip_node_normals",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -168,6 +168,7 @@ HOST_SYMBOLS = [
     "sdfslam_host_dataset_frame", "sdfslam_server_create", "sdfslam_server_destroy",
     "sdfslam_server_open_session", "sdfslam_server_submit", "sdfslam_server_wait",
     "sdfslam_server_state", "sdfslam_host_served_benchmark", "sdfslam_host_batch_benchmark",
+    "sdfslam_host_node_normals",
 ]
 
 
@@ -204,6 +205,8 @@ hip.sdfslam_hip_context_iterate.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
 hip.sdfslam_hip_phase_times_get.argtypes = [C.c_void_p, C.POINTER(PhaseTimes)]
 hip.sdfslam_hip_context_update.argtypes = [C.c_void_p, C.POINTER(ProblemView), _f64p, _f64p]
 hip.sdfslam_hip_pose_margins.argtypes = [C.c_void_p, _i32p, C.c_int32, _i32p]
+hip.sdfslam_hip_node_normals.argtypes = [_f64p, _f64p, C.c_int64, C.c_int32, C.POINTER(Grid), C.c_int32,
+                                         C.c_double, _f64p, _f64p]
 hip.sdfslam_hip_batch_solve.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(SolverOptions),
                                         C.POINTER(SolveResult), _i32p, C.POINTER(BatchInfo)]
 
@@ -244,6 +247,8 @@ host.sdfslam_server_wait.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.POINTER
 host.sdfslam_server_state.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, _f64p, _f64p]
 host.sdfslam_host_served_benchmark.argtypes = [C.POINTER(ServedOptions), C.POINTER(ServedResult)]
 host.sdfslam_host_batch_benchmark.argtypes = [C.POINTER(BatchOptions), C.POINTER(BatchResult)]
+host.sdfslam_host_node_normals.argtypes = [_f64p, _f64p, C.c_int64, C.c_int32, C.POINTER(Grid), C.c_int32,
+                                           C.c_double, _f64p, _f64p]
 host.sdfslam_host_parse_solver_yaml.argtypes = [C.c_char_p, C.POINTER(SolverOptions), C.c_char_p,
                                                 C.c_int32]
 
@@ -677,6 +682,27 @@ def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=12, max
     r = ServedResult()
     _check_host(host.sdfslam_host_served_benchmark(C.byref(o), C.byref(r)))
     return {k: getattr(r, k) for k, _ in ServedResult._fields_}
+
+
+NORMAL_METHODS = {"pca": 0, "hybrid": 1, "weighted": 2}
+
+
+def node_normals(points_xy, origins_xy, k_neighbors, grid, method="pca", corner_threshold=0.2,
+                 device=True):
+    """Node normals and eikonal scales of a problem build from scan points and
+    their sensor origins in world coordinates ((n, 2) arrays): on the device
+    (sdfslam_hip_node_normals) or on the host (sdfslam_host_node_normals).
+    grid: a Grid. Returns (normals (nx*ny, 2), scales (nx*ny,))."""
+    p = np.ascontiguousarray(points_xy, dtype=np.float64).reshape(-1, 2)
+    o = np.ascontiguousarray(origins_xy, dtype=np.float64).reshape(-1, 2)
+    nodes = grid.nx * grid.ny
+    nrm = np.zeros((nodes, 2))
+    scale = np.zeros(nodes)
+    fn = hip.sdfslam_hip_node_normals if device else host.sdfslam_host_node_normals
+    st = fn(ptr(p, _f64p), ptr(o, _f64p), len(p), k_neighbors, C.byref(grid), NORMAL_METHODS[method],
+            corner_threshold, ptr(nrm, _f64p), ptr(scale, _f64p))
+    (_check_hip if device else _check_host)(st)
+    return nrm, scale
 
 
 def batch_benchmark(instances=64, iterations=30, max_batch=8, workers=8, seed0=1000):

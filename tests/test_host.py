@@ -203,3 +203,31 @@ def test_dataset_frames_follow_the_odometry(sdfslam, oracle):
         oracle.lib.oracle_compose(prev.ctypes.data_as(f64), ok.ctypes.data_as(f64), out.ctypes.data_as(f64))
         assert np.allclose(out, pk, atol=1e-9)
         prev = pk
+
+
+def test_host_node_normals_closed_forms(sdfslam):
+    # Points on the line y = 0 seen from above: every PCA normal is (0, 1)
+    # and a straight wall has no cornerness, so the weighted scale is 1.
+    xs = np.arange(0.0, 10.0, 0.1)
+    pts = np.stack([xs, np.zeros_like(xs)], axis=1)
+    org = np.stack([xs, np.full_like(xs, 5.0)], axis=1)
+    grid = sdfslam.Grid(11, 6, 0.0, -2.5, 1.0, 1.0)
+    for method in ("pca", "weighted"):
+        nrm, scale = sdfslam.node_normals(pts, org, 7, grid, method, device=False)
+        np.testing.assert_allclose(nrm, np.tile([0.0, 1.0], (66, 1)), atol=1e-12)
+        np.testing.assert_allclose(scale, 1.0, atol=1e-9)
+    # Seen from below the same wall flips every normal.
+    nrm, _ = sdfslam.node_normals(pts, org * [1.0, -1.0], 7, grid, "pca", device=False)
+    np.testing.assert_allclose(nrm, np.tile([0.0, -1.0], (66, 1)), atol=1e-12)
+    # A corner (two perpendicular walls): points near it have cornerness, so
+    # the weighted scale drops below 1 at the nodes nearest to it and the
+    # hybrid normal leans toward the node-to-surface direction there.
+    a = np.arange(0.0, 3.0, 0.1)
+    corner = np.concatenate([np.stack([a, np.zeros_like(a)], 1), np.stack([np.zeros_like(a[1:]), a[1:]], 1)])
+    org = np.tile([1.5, 1.5], (len(corner), 1))
+    g2 = sdfslam.Grid(7, 7, -1.5, -1.5, 1.0, 1.0)
+    _, sw = sdfslam.node_normals(corner, org, 7, g2, "weighted", device=False)
+    assert sw.min() < 0.9 and sw.max() == 1.0
+    # No scan points: default node normals (zero vector, scale 1).
+    nrm, scale = sdfslam.node_normals(np.zeros((0, 2)), np.zeros((0, 2)), 7, g2, "pca", device=False)
+    assert not nrm.any() and np.all(scale == 1.0)
