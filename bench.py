@@ -46,6 +46,36 @@ def point_assembly_bytes(problem, info):
     return per_row * v.num_points + once
 
 
+def calibration_copy(mib):
+    """One device-to-device copy of `mib` MiB: under a counter run, its
+    FETCH_SIZE / WRITE_SIZE readings give the scale from counter units to
+    bytes (it reads and writes exactly that many bytes)."""
+    import torch
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/office200_hbm_traffic.txt: lines `kernel fetch_bytes
+    write_bytes`), or None when it has no line for the kernel."""
+    path = os.path.join(ROOT, "profiles", "office200_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if line.startswith("#"):
+            continue
+        if len(parts) == 3 and parts[0] == kernel:
+            return float(parts[1]) + float(parts[2])
+    return None
+
+
 def cpu_baseline(problem, iterations):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import sdfslam as S
@@ -132,6 +162,9 @@ def main():
     ap.add_argument("--incremental-frames", type=int, default=150)
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the incremental, served and batched legs")
+    ap.add_argument("--calibration-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (the reference "
+                         "kernel for scaling HBM counter readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the hot path is measured "
@@ -139,6 +172,8 @@ def main():
         sys.exit(2)
 
     import sdfslam as S
+    if args.calibration_copy_mb > 0:
+        calibration_copy(args.calibration_copy_mb)
     t_build = time.perf_counter()
     problem = S.Problem.workload(args.workload, args.seed)
     opts, _, _, _ = problem.workload_info()
@@ -205,7 +240,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": step_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic("KPanelStep"),
             "flops_per_launch": step_flops,
             "launches_per_iteration": info["panel_step_launches"],
             "avg_launch_ms": step_ms,
@@ -219,7 +254,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic("KAssemblePoints"),
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": kernel_ms,
         },
