@@ -695,8 +695,11 @@ void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape
 /// theirs grow by one. Up to kMaxMargin. Poses that stay put keep the
 /// initial margin, so the pattern grows only where the state moves.
 constexpr int kMaxMargin = 8;
-// Margin of the poses an incremental update appends.
+// Margin of the poses an incremental update appends; while a run has few
+// frames (its map still barely constrained, its poses moving most, its
+// pattern small) they start at the largest margin.
 constexpr int kFreshPoseMargin = 5;
+constexpr int kEarlyFrames = 10;
 
 void GrowMargins(sdfslam_hip_context& c, const std::vector<double>& poses) {
   const sdfslam_problem_view& v = c.hp->view;
@@ -1349,7 +1352,8 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
   if (extends) {
     margins = c.aopt.pose_margin;
     if (static_cast<int>(margins.size()) != old_poses) margins.assign(static_cast<size_t>(old_poses), c.aopt.margin);
-    margins.resize(static_cast<size_t>(problem->num_poses), kFreshPoseMargin);
+    margins.resize(static_cast<size_t>(problem->num_poses),
+                   problem->num_poses <= kEarlyFrames ? kMaxMargin : kFreshPoseMargin);
   }
   c.hp = std::move(hp);
   c.aopt.margin = AnalysisOptions{}.margin;
