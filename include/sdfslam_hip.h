@@ -192,6 +192,11 @@ typedef struct sdfslam_hip_analysis_info {
   int32_t panel_step_launches;
   double panel_step_flops;
   int32_t speculative_epochs;
+  /* Launches of the trailing-update / update-matrix kernels (FP64 MFMA)
+   * per factorization and their algorithmic flops (2 K per lower-triangle
+   * output entry). */
+  int32_t gemm_launches;
+  double gemm_flops;
 } sdfslam_hip_analysis_info;
 int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* info);
 
@@ -293,6 +298,7 @@ typedef struct sdfslam_hip_phase_times {
   double total_ms;
   double point_assembly_kernel_ms;
   double panel_step_kernel_ms; /* average duration of one panel-step launch */
+  double gemm_kernel_ms;       /* average duration of one trailing-update launch */
 } sdfslam_hip_phase_times;
 int sdfslam_hip_phase_times_get(sdfslam_hip_context* ctx, sdfslam_hip_phase_times* out);
 

@@ -788,6 +788,10 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   }
   for (int L = 0; L < a.num_levels; ++L) {
     a.panel_step_launches += (a.level_max_nb[static_cast<size_t>(L)] + 63) / 64;
+    if (!a.level_sn[static_cast<size_t>(L)].empty()) {
+      a.gemm_launches += (a.level_max_nb[static_cast<size_t>(L)] + 63) / 64 +
+                         (a.level_max_m[static_cast<size_t>(L)] > 0 ? 1 : 0);
+    }
   }
   for (int s = 0; s < a.num_sn; ++s) {
     const int ns = a.sn_first[static_cast<size_t>(s) + 1] - a.sn_first[static_cast<size_t>(s)];
@@ -798,7 +802,13 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
       // The first step factors its diagonal block; later steps find theirs
       // factored by the previous step's trailing update.
       a.panel_step_flops += (c0 == 0 ? 2.0 / 3.0 * nb * nb * nb : 0.0) + rows * nb * nb;
+      // Trailing update: the lower part of the rows and columns beyond the block.
+      const double big_n = std::max(0.0, ns - c0 - nb);
+      const double big_m = f - c0 - nb;
+      a.gemm_flops += 2.0 * nb * (big_n * (big_n + 1.0) / 2.0 + (big_m - big_n) * big_n);
     }
+    const double m = a.sn_m[static_cast<size_t>(s)];
+    a.gemm_flops += 2.0 * ns * m * (m + 1.0) / 2.0;
   }
   // Pattern size (lower triangle): node-node slots, footprint triples, pose
   // diagonal blocks, odometry pair blocks.

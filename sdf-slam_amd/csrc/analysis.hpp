@@ -108,6 +108,11 @@ struct Analysis {
   // step's trailing update).
   int32_t panel_step_launches{0};
   double panel_step_flops{0.0};
+  // Trailing-update and update-matrix launches per factorization (one per
+  // step and level, one per level with update matrices) and their flops:
+  // 2 K per lower-triangle output entry.
+  int32_t gemm_launches{0};
+  double gemm_flops{0.0};
 };
 
 /// Builds the analysis for the problem at the given host state.

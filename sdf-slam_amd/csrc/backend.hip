@@ -447,6 +447,10 @@ struct sdfslam_hip_context {
   int step_events_used{0};
   double panel_step_ms{0.0};
   int64_t panel_steps_timed{0};
+  std::vector<hipEvent_t> gemm_events;  // pairs around trailing updates (timed runs)
+  int gemm_events_used{0};
+  double gemm_ms{0.0};
+  int64_t gemms_timed{0};
   int timed_iterations{0};
   bool lm_ready{false};
   BatchSpace batch;  // buffers of the batches this context leads
@@ -472,6 +476,7 @@ struct sdfslam_hip_context {
       if (e != nullptr) (void)hipEventDestroy(e);
     }
     for (hipEvent_t e : step_events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : gemm_events) (void)hipEventDestroy(e);
     epoch.Release();
     base.Release();
     hist.Clear();
@@ -914,7 +919,7 @@ void EnqueueAssemblyAndNormal(sdfslam_hip_context& c, const LmOptions& o, bool t
 void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
   hipStream_t s = c.stream;
   const Analysis& an = c.an;
-  StepEvents step_ev{nullptr, 0, 0};
+  StepEvents step_ev{nullptr, 0, 0, nullptr, 0, 0};
   if (timing) {
     const size_t want = 2 * static_cast<size_t>(std::min(an.panel_step_launches, 1024));
     while (c.step_events.size() < want) {
@@ -922,10 +927,18 @@ void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
       HIPC(hipEventCreate(&e));
       c.step_events.push_back(e);
     }
-    step_ev = {c.step_events.data(), static_cast<int>(c.step_events.size() / 2), 0};
+    const size_t want_gemm = 2 * static_cast<size_t>(std::min(an.gemm_launches, 1024));
+    while (c.gemm_events.size() < want_gemm) {
+      hipEvent_t e = nullptr;
+      HIPC(hipEventCreate(&e));
+      c.gemm_events.push_back(e);
+    }
+    step_ev = {c.step_events.data(), static_cast<int>(c.step_events.size() / 2), 0,
+               c.gemm_events.data(), static_cast<int>(c.gemm_events.size() / 2), 0};
   }
   LaunchFactor(SlotArgs{c.slot_dev, &c.slot}, c.shape, s, timing ? &step_ev : nullptr);
   c.step_events_used = step_ev.used;
+  c.gemm_events_used = step_ev.gemm_used;
   Mark(c, timing, 3);
   LaunchSolve(SlotArgs{c.slot_dev, &c.slot}, c.shape, s);
 }
@@ -987,6 +1000,14 @@ void AccumulateTimes(sdfslam_hip_context& c) {
     ++c.panel_steps_timed;
   }
   c.step_events_used = 0;
+  for (int k = 0; k < c.gemm_events_used; ++k) {
+    float ms = 0.0f;
+    HIPC(hipEventElapsedTime(&ms, c.gemm_events[2 * static_cast<size_t>(k)],
+                             c.gemm_events[2 * static_cast<size_t>(k) + 1]));
+    c.gemm_ms += ms;
+    ++c.gemms_timed;
+  }
+  c.gemm_events_used = 0;
   for (int k = 0; k < kPhases; ++k) {
     float ms = 0.0f;
     HIPC(hipEventElapsedTime(&ms, c.events[k], c.events[k + 1]));
@@ -1534,6 +1555,8 @@ int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* in
     info->panel_step_launches = a.panel_step_launches;
     info->panel_step_flops = a.panel_step_flops;
     info->speculative_epochs = ctx->speculative_epochs;
+    info->gemm_launches = a.gemm_launches;
+    info->gemm_flops = a.gemm_flops;
     return SDFSLAM_OK;
   });
 }
@@ -1851,6 +1874,8 @@ int sdfslam_hip_context_iterate(sdfslam_hip_context* ctx, int32_t iterations,
     c.timed_iterations = 0;
     c.panel_step_ms = 0.0;
     c.panel_steps_timed = 0;
+    c.gemm_ms = 0.0;
+    c.gemms_timed = 0;
     LmState st{};
     ReadState(c, st);
     if (st.halt != 0) {
@@ -1879,6 +1904,7 @@ int sdfslam_hip_phase_times_get(sdfslam_hip_context* ctx, sdfslam_hip_phase_time
   out->point_assembly_kernel_ms = ctx->phase_ms[kPhases] / n;
   out->panel_step_kernel_ms =
       ctx->panel_steps_timed > 0 ? ctx->panel_step_ms / static_cast<double>(ctx->panel_steps_timed) : 0.0;
+  out->gemm_kernel_ms = ctx->gemms_timed > 0 ? ctx->gemm_ms / static_cast<double>(ctx->gemms_timed) : 0.0;
   return SDFSLAM_OK;
 }
 

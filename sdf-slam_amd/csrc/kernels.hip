@@ -1815,15 +1815,21 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
         const int tr = trail_rows > 0 ? (trail_rows + kTile - 1) / kTile : 1;
         const int tc = trail_cols > 0 ? (trail_cols + kTile - 1) / kTile : 1;
         const dim3 grid(tr, tc, static_cast<unsigned>(count) * k);
+        const bool gt = step_events != nullptr && step_events->gemm_used < step_events->gemm_capacity;
+        if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used], s);
         if (kb + 1 < nblocks) {
           KGemmLowerAhead<<<grid, kThreads, 0, s>>>(slots, plan, kb, count);
         } else {
           KGemmLower<<<grid, kThreads, 0, s>>>(slots, plan, kb, 0, count);
         }
+        if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used++ + 1], s);
       }
       if (lv.max_m > 0) {
         const int t = (lv.max_m + kTile - 1) / kTile;
+        const bool gt = step_events != nullptr && step_events->gemm_used < step_events->gemm_capacity;
+        if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used], s);
         KGemmLower<<<dim3(t, t, static_cast<unsigned>(count) * k), kThreads, 0, s>>>(slots, plan, 0, 1, count);
+        if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used++ + 1], s);
       }
     }
     for (int r = 0; r < ranks; ++r) {

@@ -217,11 +217,15 @@ void LaunchAssemble(const SlotArgs& slots, const LaunchShape& sh, bool check_foo
 void LaunchNormalSlots(const SlotArgs& slots, const LaunchShape& sh, const LmOptions& o, hipStream_t s);
 
 /// Optional timing of the panel steps: pairs (events[2k], events[2k+1])
-/// bracket the diagonal-block and TRSM kernels of one 64-column step.
+/// bracket the diagonal-block and TRSM kernels of one 64-column step; pairs
+/// of gemm_events bracket one trailing-update or update-matrix launch.
 struct StepEvents {
   hipEvent_t* events;
   int capacity;  // pairs
   int used;
+  hipEvent_t* gemm_events;
+  int gemm_capacity;  // pairs
+  int gemm_used;
 };
 /// Multifrontal LDL^T, level by level (extend-add, panel steps, trailing
 /// updates, update matrices).

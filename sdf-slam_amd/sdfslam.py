@@ -71,14 +71,16 @@ class AnalysisInfo(C.Structure):
                 ("update_doubles", C.c_int64), ("tree_levels", C.c_int32),
                 ("pattern_epochs", C.c_int32), ("analysis_seconds", C.c_double),
                 ("device_bytes", C.c_int64), ("panel_step_launches", C.c_int32),
-                ("panel_step_flops", C.c_double), ("speculative_epochs", C.c_int32)]
+                ("panel_step_flops", C.c_double), ("speculative_epochs", C.c_int32),
+                ("gemm_launches", C.c_int32), ("gemm_flops", C.c_double)]
 
 
 class PhaseTimes(C.Structure):
     _fields_ = [("assembly_ms", C.c_double), ("normal_ms", C.c_double),
                 ("factor_ms", C.c_double), ("solve_ms", C.c_double),
                 ("update_ms", C.c_double), ("total_ms", C.c_double),
-                ("point_assembly_kernel_ms", C.c_double), ("panel_step_kernel_ms", C.c_double)]
+                ("point_assembly_kernel_ms", C.c_double), ("panel_step_kernel_ms", C.c_double),
+                ("gemm_kernel_ms", C.c_double)]
 
 
 class ProblemOptions(C.Structure):
