@@ -805,18 +805,26 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     for (int r = 0; r < an.level_max_children[static_cast<size_t>(L)]; ++r) {
       int count = 0;
       int max_m = 0;
+      int max_split = 0;
+      int max_rest = 0;
       for (const int s : an.level_sn[static_cast<size_t>(L)]) {
         const auto& ch = an.children[static_cast<size_t>(s)];
         if (static_cast<int>(ch.size()) > r) {
           const int child = ch[static_cast<size_t>(r)];
           ranks.push_back(child);
           ++count;
-          max_m = std::max(max_m, an.sn_m[static_cast<size_t>(child)]);
+          const int mc = an.sn_m[static_cast<size_t>(child)];
+          const int split = an.rel_split[static_cast<size_t>(child)];
+          max_m = std::max(max_m, mc);
+          max_split = std::max(max_split, split);
+          max_rest = std::max(max_rest, mc - split);
         }
       }
       rank_off.push_back(static_cast<int64_t>(ranks.size()));
       lv.rank_count.push_back(count);
       lv.rank_max_m.push_back(max_m);
+      lv.rank_max_split.push_back(max_split);
+      lv.rank_max_rest.push_back(max_rest);
     }
     rank_first[static_cast<size_t>(L) + 1] = static_cast<int64_t>(rank_off.size()) - 1;
   }

@@ -19,6 +19,7 @@ namespace sdfslam {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;  // wavefronts per workgroup
 constexpr int kNB = 64;     // block column width of the panel factorization
 constexpr int kTile = 64;   // GEMM tile edge
 constexpr int kTk = 16;     // GEMM k chunk
@@ -705,14 +706,15 @@ __global__ void KOdomPairs(Slots slots) {
 // ---------------------------------------------------------------- factorization
 
 // Extend-add of a child's update matrix: columns [0, split) land in the
-// parent's panel, columns [split, m_c) in the parent's update matrix.
+// parent's panel, columns [split, m_c) in the parent's update matrix. One
+// wavefront per column, kWaves columns per workgroup.
 template <class Slots, class Lists>
 __global__ void KExtendAddPanel(Slots slots, Lists plan) {
   BIND_SLOT(blockIdx.z);
   if (Halted(W.state)) return;
   const int c = PlanItem(plan, blockIdx.z, blockIdx.y);
   if (c < 0) return;
-  const int b = blockIdx.x;
+  const int b = static_cast<int>(blockIdx.x) * kWaves + static_cast<int>(threadIdx.x >> 6);
   if (b >= A.rel_split[c]) return;
   const int p = A.sn_parent[c];
   const int mc = A.sn_m[c];
@@ -721,7 +723,7 @@ __global__ void KExtendAddPanel(Slots slots, Lists plan) {
   const int fp = A.sn_f[p];
   double* fpanel = W.panel + A.panel_off[p];
   const int64_t col = static_cast<int64_t>(rel[b]) * fp;
-  for (int a = b + threadIdx.x; a < mc; a += blockDim.x) {
+  for (int a = b + static_cast<int>(threadIdx.x & 63); a < mc; a += 64) {
     fpanel[col + rel[a]] += uc[static_cast<int64_t>(b) * mc + a];
   }
 }
@@ -733,7 +735,7 @@ __global__ void KExtendAddUpdate(Slots slots, Lists plan) {
   const int c = PlanItem(plan, blockIdx.z, blockIdx.y);
   if (c < 0) return;
   const int mc = A.sn_m[c];
-  const int b = A.rel_split[c] + blockIdx.x;
+  const int b = A.rel_split[c] + static_cast<int>(blockIdx.x) * kWaves + static_cast<int>(threadIdx.x >> 6);
   if (b >= mc) return;
   const int p = A.sn_parent[c];
   const int np = A.sn_first[p + 1] - A.sn_first[p];
@@ -742,7 +744,7 @@ __global__ void KExtendAddUpdate(Slots slots, Lists plan) {
   const int32_t* rel = A.rel + A.rel_off[c];
   double* up = W.upd + A.upd_off[p];
   const int64_t col = static_cast<int64_t>(rel[b] - np) * mp;
-  for (int a = b + threadIdx.x; a < mc; a += blockDim.x) {
+  for (int a = b + static_cast<int>(threadIdx.x & 63); a < mc; a += 64) {
     up[col + (rel[a] - np)] += uc[static_cast<int64_t>(b) * mc + a];
   }
 }
@@ -1692,10 +1694,14 @@ void LaunchShape::Merge(const LaunchShape& o) {
     if (a.rank_count.size() < b.rank_count.size()) {
       a.rank_count.resize(b.rank_count.size(), 0);
       a.rank_max_m.resize(b.rank_max_m.size(), 0);
+      a.rank_max_split.resize(b.rank_max_split.size(), 0);
+      a.rank_max_rest.resize(b.rank_max_rest.size(), 0);
     }
     for (size_t r = 0; r < b.rank_count.size(); ++r) {
       a.rank_count[r] = std::max(a.rank_count[r], b.rank_count[r]);
       a.rank_max_m[r] = std::max(a.rank_max_m[r], b.rank_max_m[r]);
+      a.rank_max_split[r] = std::max(a.rank_max_split[r], b.rank_max_split[r]);
+      a.rank_max_rest[r] = std::max(a.rank_max_rest[r], b.rank_max_rest[r]);
     }
   }
 }
@@ -1784,9 +1790,10 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
     const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
     const int ranks = static_cast<int>(lv.rank_count.size());
     for (int r = 0; r < ranks; ++r) {
-      if (lv.rank_count[static_cast<size_t>(r)] > 0 && lv.rank_max_m[static_cast<size_t>(r)] > 0) {
-        KExtendAddPanel<<<dim3(lv.rank_max_m[static_cast<size_t>(r)], lv.rank_count[static_cast<size_t>(r)], k),
-                          kThreads, 0, s>>>(slots, RankLists<kOne>(sh, L, r));
+      const int cols = lv.rank_max_split[static_cast<size_t>(r)];
+      if (lv.rank_count[static_cast<size_t>(r)] > 0 && cols > 0) {
+        KExtendAddPanel<<<dim3((cols + kWaves - 1) / kWaves, lv.rank_count[static_cast<size_t>(r)], k), kThreads, 0,
+                          s>>>(slots, RankLists<kOne>(sh, L, r));
       }
     }
     const int count = lv.count;
@@ -1833,9 +1840,10 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
       }
     }
     for (int r = 0; r < ranks; ++r) {
-      if (lv.rank_count[static_cast<size_t>(r)] > 0 && lv.rank_max_m[static_cast<size_t>(r)] > 0) {
-        KExtendAddUpdate<<<dim3(lv.rank_max_m[static_cast<size_t>(r)], lv.rank_count[static_cast<size_t>(r)], k),
-                           kThreads, 0, s>>>(slots, RankLists<kOne>(sh, L, r));
+      const int cols = lv.rank_max_rest[static_cast<size_t>(r)];
+      if (lv.rank_count[static_cast<size_t>(r)] > 0 && cols > 0) {
+        KExtendAddUpdate<<<dim3((cols + kWaves - 1) / kWaves, lv.rank_count[static_cast<size_t>(r)], k), kThreads,
+                           0, s>>>(slots, RankLists<kOne>(sh, L, r));
       }
     }
   }

@@ -187,6 +187,10 @@ struct LaunchShape {
     int count{0}, max_nb{0}, max_f{0}, max_m{0};
     std::vector<int> rank_count;  // largest child-list length per child rank
     std::vector<int> rank_max_m;  // largest child update size per child rank
+    // Largest number of a rank's child columns landing in the parent's own
+    // columns (the panel part of the extend-add) and beyond them (its update
+    // matrix part).
+    std::vector<int> rank_max_split, rank_max_rest;
     int rank_base{0};             // index of the level's first rank in rank_plan
   };
   std::vector<Level> levels;
