@@ -438,7 +438,8 @@ struct sdfslam_hip_context {
   // rank), on the device with their host offsets, for launch plans.
   const int32_t* level_sn_d{nullptr};
   const int32_t* child_lists_d{nullptr};
-  std::vector<int64_t> level_off_h, rank_first_h, rank_off_h;
+  const int32_t* tiles_d{nullptr};  // update-matrix tiles, level by level (pairs)
+  std::vector<int64_t> level_off_h, rank_first_h, rank_off_h, tile_off_h;
 
   int max_f{0};
   hipEvent_t events[kPhases + 3]{};  // phase marks, then the point-row kernel pair
@@ -660,13 +661,18 @@ void SyncSlot(sdfslam_hip_context& c) {
 }
 
 /// Launch plans of a launch over `ctxs` (slot b = ctxs[b]) for the merged
-/// shape `sh`: per level and slot the level's fronts, per (level, child rank)
-/// and slot the extend-add list; sets each level's rank_base.
-void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape& sh,
-                std::vector<ListRef>& level_plan, std::vector<ListRef>& rank_plan) {
+/// shape `sh`: per level and slot the level's fronts and its update-matrix
+/// tiles, per (level, child rank) and slot the extend-add list; sets each
+/// level's rank_base. `all` receives the level, rank and tile plans in that
+/// order (the layout of their device copy, see SetPlans).
+void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape& sh, std::vector<ListRef>& all) {
   const size_t k = ctxs.size();
   const size_t levels = sh.levels.size();
+  std::vector<ListRef>& level_plan = sh.level_plan_h;
+  std::vector<ListRef>& rank_plan = sh.rank_plan_h;
+  std::vector<ListRef>& tile_plan = sh.tile_plan_h;
   level_plan.assign(levels * k, ListRef{nullptr, 0});
+  tile_plan.assign(levels * k, ListRef{nullptr, 0});
   int ranks = 0;
   for (LaunchShape::Level& lv : sh.levels) {
     lv.rank_base = ranks;
@@ -679,6 +685,8 @@ void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape
     for (size_t L = 0; L < std::min(levels, own_levels); ++L) {
       const int64_t l0 = c.level_off_h[L];
       level_plan[L * k + b] = {c.level_sn_d + l0, static_cast<int32_t>(c.level_off_h[L + 1] - l0)};
+      const int64_t t0 = c.tile_off_h[L];
+      tile_plan[L * k + b] = {c.tiles_d + 2 * t0, static_cast<int32_t>(c.tile_off_h[L + 1] - t0)};
       const int64_t e0 = c.rank_first_h[L];
       const int64_t e1 = c.rank_first_h[L + 1];
       for (int64_t e = e0; e < e1; ++e) {
@@ -690,6 +698,16 @@ void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape
       }
     }
   }
+  all = level_plan;
+  all.insert(all.end(), rank_plan.begin(), rank_plan.end());
+  all.insert(all.end(), tile_plan.begin(), tile_plan.end());
+}
+
+/// Points the shape's device plans into `dev`, a copy of BuildPlans' `all`.
+void SetPlans(LaunchShape& sh, const ListRef* dev) {
+  sh.level_plan = dev;
+  sh.rank_plan = dev + sh.level_plan_h.size();
+  sh.tile_plan = sh.rank_plan + sh.rank_plan_h.size();
 }
 
 /// Widens the footprint dilation of the poses whose points left their
@@ -792,6 +810,8 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
   std::vector<int32_t> ranks;
   std::vector<int64_t> rank_first(static_cast<size_t>(an.num_levels) + 1, 0);
   std::vector<int64_t> rank_off{0};
+  std::vector<int32_t> tiles;
+  std::vector<int64_t> tile_off(static_cast<size_t>(an.num_levels) + 1, 0);
   LaunchShape& sh = c.shape;
   sh.levels.assign(static_cast<size_t>(an.num_levels), {});
   for (int L = 0; L < an.num_levels; ++L) {
@@ -802,6 +822,17 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     lv.max_nb = an.level_max_nb[static_cast<size_t>(L)];
     lv.max_f = an.level_max_f[static_cast<size_t>(L)];
     lv.max_m = an.level_max_m[static_cast<size_t>(L)];
+    for (const int s : an.level_sn[static_cast<size_t>(L)]) {
+      const int t = (an.sn_m[static_cast<size_t>(s)] + 63) / 64;
+      for (int ti = 0; ti < t; ++ti) {
+        for (int tj = 0; tj <= ti; ++tj) {
+          tiles.push_back(s);
+          tiles.push_back((ti << 16) | tj);
+        }
+      }
+    }
+    tile_off[static_cast<size_t>(L) + 1] = static_cast<int64_t>(tiles.size() / 2);
+    lv.tiles = static_cast<int>(tile_off[static_cast<size_t>(L) + 1] - tile_off[static_cast<size_t>(L)]);
     for (int r = 0; r < an.level_max_children[static_cast<size_t>(L)]; ++r) {
       int count = 0;
       int max_m = 0;
@@ -834,10 +865,10 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
   c.level_off_h = std::move(level_off);
   c.rank_first_h = std::move(rank_first);
   c.rank_off_h = std::move(rank_off);
-  size_t plan_entries = sh.levels.size();
+  c.tile_off_h = std::move(tile_off);
+  size_t plan_entries = 2 * sh.levels.size();
   for (const LaunchShape::Level& lv : sh.levels) plan_entries += lv.rank_count.size();
-  std::vector<ListRef> level_plan;
-  std::vector<ListRef> rank_plan;
+  std::vector<ListRef> plans;
   EpochPool& E = c.epoch;
   DeviceAnalysis& D = c.da;
   auto layout = [&]() {
@@ -879,6 +910,7 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     D.children = E.Upload(clist);
     c.level_sn_d = E.Upload(lvl);
     c.child_lists_d = E.Upload(ranks);
+    c.tiles_d = E.Upload(tiles);
     c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
     c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
     c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 2 * 64 * 64);
@@ -889,12 +921,8 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
       E.Upload(static_cast<const ListRef*>(nullptr), plan_entries);
       E.Upload(static_cast<const Slot*>(nullptr), 1);
     } else {
-      BuildPlans({&c}, c.shape, level_plan, rank_plan);
-      std::vector<ListRef> all(level_plan);
-      all.insert(all.end(), rank_plan.begin(), rank_plan.end());
-      const ListRef* plans = E.Upload(all);
-      c.shape.level_plan = plans;
-      c.shape.rank_plan = plans + level_plan.size();
+      BuildPlans({&c}, c.shape, plans);
+      SetPlans(c.shape, E.Upload(plans));
       FillSlot(c);
       c.slot_dev = E.Upload(&c.slot, 1);
     }
@@ -903,8 +931,6 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
   layout();
   E.BeginPlace();
   layout();
-  c.shape.level_plan_h = level_plan;
-  c.shape.rank_plan_h = rank_plan;
   Enqueue(c, [&E](hipStream_t s) { E.Copy(s); });
   ++c.generation;
   c.analysis_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1260,22 +1286,17 @@ class BatchRun {
       shape_.Merge(ctxs_[static_cast<size_t>(b)]->shape);
     }
     std::vector<const sdfslam_hip_context*> cs(ctxs_.begin(), ctxs_.end());
-    std::vector<ListRef> level_plan;
-    std::vector<ListRef> rank_plan;
-    BuildPlans(cs, shape_, level_plan, rank_plan);
-    space_.ReservePlans(level_plan.size() + rank_plan.size());
-    std::copy(level_plan.begin(), level_plan.end(), space_.plan_host);
-    std::copy(rank_plan.begin(), rank_plan.end(), space_.plan_host + level_plan.size());
+    std::vector<ListRef> plans;
+    BuildPlans(cs, shape_, plans);
+    space_.ReservePlans(plans.size());
+    std::copy(plans.begin(), plans.end(), space_.plan_host);
     // Pinned staging: the two uploads are true async copies; the stream is
     // synchronised before the staging is touched again.
     HIPC(hipMemcpyAsync(space_.table, space_.slots_host, sizeof(Slot) * static_cast<size_t>(k),
                         hipMemcpyHostToDevice, stream_));
-    HIPC(hipMemcpyAsync(space_.plan, space_.plan_host, sizeof(ListRef) * (level_plan.size() + rank_plan.size()),
+    HIPC(hipMemcpyAsync(space_.plan, space_.plan_host, sizeof(ListRef) * plans.size(),
                         hipMemcpyHostToDevice, stream_));
-    shape_.level_plan = space_.plan;
-    shape_.rank_plan = space_.plan + level_plan.size();
-    shape_.level_plan_h = level_plan;
-    shape_.rank_plan_h = rank_plan;
+    SetPlans(shape_, space_.plan);
     HIPC(hipStreamSynchronize(stream_));
     if (graph_ != nullptr) {
       HIPC(hipGraphExecDestroy(graph_));

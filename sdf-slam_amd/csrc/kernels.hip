@@ -749,84 +749,25 @@ __global__ void KExtendAddUpdate(Slots slots, Lists plan) {
   }
 }
 
-// One 64-column step of the blocked panel factorization of every front at
-// the level. Each workgroup factors the kb-th diagonal block in LDS (right
-// looking, every thread owning fixed lower-triangle pairs) and solves 128
-// rows below it, L = F L_kk^-T D_k^-1, two threads per row (columns 0..31,
-// then 32..63). Workgroup 0 stores the factored block in the front's staging
-// slot; the trailing-update kernel moves it into the panel, so no workgroup
-// of this launch reads a partly written block.
-// Zero, computed from `dep` as far as the compiler can tell: an LDS address
-// offset by it cannot be loaded before `dep` is final, which keeps a fully
-// unrolled solve from hoisting hundreds of loads ahead of their use.
-__device__ __forceinline__ int ZeroAfter(double dep) {
-  int z;
-  __asm__ volatile("v_mov_b32 %0, 0" : "=v"(z) : "v"(dep));
-  return z;
-}
-
-// Columns 0..31 of a panel row below the diagonal block: y = x L11^-T,
-// L = y D^-1; y is kept in `ybuf` for the second half. Right-looking: once
-// y_t is final it is applied to every later column, so each column sees the
-// same fma sequence (t = 0, 1, ...) as a left-looking solve while the
-// updates of different columns are independent. lt[t][j] = L(j, t).
-__device__ __forceinline__ void SolveRowFirstHalf(double* F, int f, int c0, int nb, int r,
-                                               const double (*lt)[kNB + 2], double* ybuf) {
-  double x[32];
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + r];
-    x[j] = j < nb ? v : 0.0;
-  }
-#pragma unroll
-  for (int t = 0; t < 32; ++t) {
-    const double* col = &lt[t][0] + ZeroAfter(x[t]);
-#pragma unroll
-    for (int j = t + 1; j < 32; ++j) x[j] = fma(-x[t], col[j], x[j]);
-  }
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    ybuf[j] = x[j];
-    if (j < nb) F[static_cast<int64_t>(c0 + j) * f + r] = x[j] / lt[j][j];
-  }
-}
-
-// Columns 32..63: x = (F - y L21^T) L22^-T, L = x D^-1 (right-looking as
-// above).
-__device__ __forceinline__ void SolveRowSecondHalf(double* F, int f, int c0, int nb, int r,
-                                                const double (*lt)[kNB + 2], const double* ybuf) {
-  double x[32];
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    const double v = F[static_cast<int64_t>(c0 + min(32 + j, nb - 1)) * f + r];
-    x[j] = 32 + j < nb ? v : 0.0;
-  }
-#pragma unroll 1
-  for (int t = 0; t < 32; ++t) {
-    const double yt = ybuf[t];
-    const double* col = &lt[t][32];
-#pragma unroll
-    for (int j = 0; j < 32; ++j) x[j] = fma(-yt, col[j], x[j]);
-  }
-#pragma unroll
-  for (int t = 0; t < 32; ++t) {
-    const double* col = &lt[32 + t][32] + ZeroAfter(x[t]);
-#pragma unroll
-    for (int j = t + 1; j < 32; ++j) x[j] = fma(-x[t], col[j], x[j]);
-  }
-#pragma unroll
-  for (int j = 0; j < 32; ++j) {
-    if (32 + j < nb) F[static_cast<int64_t>(c0 + 32 + j) * f + r] = x[j] / lt[32 + j][32 + j];
-  }
-}
-
-constexpr int kStepRows = 128;
+// Row stride (doubles) of the panel step's LDS tiles: two doubles of
+// padding put the four k rows an MFMA operand read touches in different
+// banks.
+constexpr int kLdT = kNB + 2;
 
 // Staging slot of supernode s's factored diagonal block of step kb: two
 // per supernode, alternating with the step, so the block of step kb + 1
 // can be written while step kb's is still being read.
 __device__ __forceinline__ double* StageSlot(const DeviceWork& W, int s, int kb) {
   return W.diag_stage + (2 * static_cast<int64_t>(s) + (kb & 1)) * kNB * kNB;
+}
+
+// The value itself, opaque to the optimizer. A load whose result feeds a
+// select on a wave-uniform condition is otherwise moved under a branch on
+// that condition, where its data is awaited before the next load issues;
+// loads feeding Opaque are issued where they stand.
+__device__ __forceinline__ double Opaque(double v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
 }
 
 // Reads lane `lane`'s value of v (a wave-uniform lane index).
@@ -869,9 +810,146 @@ __device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int
   return zero;
 }
 
+// M = L^-1 of the unit lower triangle of a factored diagonal block, one
+// wavefront, in LDS: `lt` holds L(j, t) at lt[t * kLd + j] (j > t) and
+// receives M(r, c) at lt[r * kLd + c] (r > c; M's diagonal is 1), the other
+// triangle of the same tile. Blocked in 16 x 16 blocks: the four diagonal
+// blocks are inverted by forward substitution, lane 16 p + c forming column
+// c of block p; then, by distance d = p - q from the diagonal, each block
+// M_pq = -M_pp (sum_{k = q}^{p - 1} L_pk M_kq) is two chains of FP64 MFMAs
+// (16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and
+// holds D[4 e + l / 16][l % 16] in element e, so the accumulator of the sum
+// is already the B operand of the product with M_pp).
+template <int kLd>
+__device__ __forceinline__ void InvertUnitLowerWave(double* lt, int lane) {
+  {
+    const int base = 16 * (lane >> 4);
+    const int c = lane & 15;
+    double y[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) y[j] = j == c ? 1.0 : 0.0;
+#pragma unroll
+    for (int t = 0; t < 15; ++t) {
+#pragma unroll
+      for (int j = t + 1; j < 16; ++j) y[j] = fma(-lt[(base + t) * kLd + base + j], y[t], y[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (j > c) lt[(base + j) * kLd + base + c] = y[j];
+    }
+  }
+  const int lr = lane & 15;  // A row / B column within a block
+  const int lk = lane >> 4;  // k within a 4-wide chunk
+  // M(r, c) of a block on or below the diagonal (unit diagonal, zero above);
+  // the load is unconditional (every index is inside the tile).
+  auto m_at = [&](int r, int c) {
+    const double v = lt[r * kLd + c];
+    return r > c ? v : (r == c ? 1.0 : 0.0);
+  };
+  // Blocks at distance d are independent: their operands are loaded first,
+  // then their MFMA chains interleave.
+#pragma unroll
+  for (int d = 1; d < 4; ++d) {
+    double a[3][12], b[3][12], mp[3][4];
+#pragma unroll
+    for (int i = 0; i < 4 - d; ++i) {
+      const int p = d + i;
+      const int q = i;
+#pragma unroll
+      for (int k = q; k < p; ++k) {
+#pragma unroll
+        for (int kc = 0; kc < 16; kc += 4) {
+          const int x = 4 * (k - q) + kc / 4;
+          a[i][x] = lt[(16 * k + kc + lk) * kLd + 16 * p + lr];  // L_pk
+          b[i][x] = m_at(16 * k + kc + lk, 16 * q + lr);       // M_kq
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) mp[i][e] = m_at(16 * p + lr, 16 * p + 4 * e + lk);  // M_pp
+    }
+    v4d sum[3], prod[3];
+#pragma unroll
+    for (int i = 0; i < 4 - d; ++i) {
+      sum[i] = v4d{0.0, 0.0, 0.0, 0.0};
+      prod[i] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll
+    for (int x = 0; x < 4 * d; ++x) {
+#pragma unroll
+      for (int i = 0; i < 4 - d; ++i) sum[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][x], b[i][x], sum[i], 0, 0, 0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+#pragma unroll
+      for (int i = 0; i < 4 - d; ++i) {
+        prod[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(mp[i][e], sum[i][e], prod[i], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4 - d; ++i) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) lt[(16 * (d + i) + 4 * e + lk) * kLd + 16 * i + lr] = -prod[i][e];
+    }
+  }
+}
+
+// Factors the diagonal block held in `row` (one wavefront, as
+// FactorDiagWave) and forms its TRSM operator W = L^-T D^-1 in the LDS tile
+// `lt` (kNB rows of kLd doubles): afterwards lt[j * kLd + j] = 1 / D(j) and
+// lt[j * kLd + i] = M(j, i) for i < j, M = L^-1, so W(i, j) = M(j, i) / D(j)
+// for i < j and W(j, j) = 1 / D(j). With `stage`, stores the staging slot:
+// the lower triangle holds L with D on the diagonal, the strict upper
+// triangle W (the readers form W's diagonal); the padding beyond nb is
+// identity in both. Returns true when a pivot is exactly zero.
+template <int kLd>
+__device__ __forceinline__ bool FactorInvertWave(double (&row)[kNB], int lane, int nb, double* lt,
+                                                 double* stage) {
+  const bool zero = FactorDiagWave(row, lane, nb, lt);
+  // LDS operations of one wavefront complete in order: the factorization's
+  // last broadcast reads are done before these writes land.
+#pragma unroll
+  for (int t = 0; t < kNB; ++t) {
+    if (t < lane) lt[t * kLd + lane] = row[t];
+  }
+  double d = 1.0;  // D(lane), selected with constant indices (row stays in registers)
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) d = j == lane ? row[j] : d;
+  lt[lane * kLd + lane] = 1.0 / d;
+  if (stage != nullptr) {
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      if (j <= lane) stage[j * kNB + lane] = row[j];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  InvertUnitLowerWave<kLd>(lt, lane);
+  __builtin_amdgcn_wave_barrier();
+  if (stage != nullptr) {
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      if (j > lane) stage[j * kNB + lane] = lt[j * kLd + lane] * lt[j * kLd + j];
+    }
+  }
+  return zero;
+}
+
+// Row `lane` of the kb-th diagonal block of front s (lower triangle and
+// diagonal; rows and columns beyond nb are an identity pad). Every load is
+// issued (clamped into the block) and the pad selected afterwards, so the 64
+// loads are in flight together.
+__device__ __forceinline__ void LoadDiagRow(const double* F, int f, int c0, int nb, int lane,
+                                            double (&row)[kNB]) {
+  const int ti = min(lane, nb - 1);
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + c0 + ti];
+    row[j] = (lane < nb && j < nb && j <= lane) ? v : (j == lane ? 1.0 : 0.0);
+  }
+}
+
 // The kb-th diagonal block of every front at the level, one wavefront per
-// front: factored in registers and stored in the front's staging slot
-// (with the zero-pivot flag), for KPanelStep's staged mode.
+// front: factored and inverted in registers and stored in the front's
+// staging slot (with the zero-pivot flag), for KPanelStep's staged mode.
 template <class Slots, class Lists>
 __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
                                                  Lists plan, int kb) {
@@ -886,20 +964,10 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
   const int f = A.sn_f[s];
   const double* F = W.panel + A.panel_off[s];
   const int tid = threadIdx.x;
-  __shared__ double colbuf[kNB];
+  __shared__ double lt[kNB * (kNB + 1)];
   double row[kNB];
-  const int ti = min(tid, nb - 1);
-#pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + c0 + ti];
-    row[j] = (tid < nb && j < nb && j <= tid) ? v : (j == tid ? 1.0 : 0.0);
-  }
-  const bool zero = FactorDiagWave(row, tid, nb, colbuf);
-  double* stage = StageSlot(W, s, kb);
-#pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    if (j <= tid && tid < nb) stage[j * kNB + tid] = row[j];
-  }
+  LoadDiagRow(F, f, c0, nb, tid, row);
+  const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb, lt, StageSlot(W, s, kb));
   if (tid == 0 && zero) {
     W.state->not_spd = 1;
     W.state->halt = 1;
@@ -907,17 +975,15 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
 }
 
 // One 64-column step of the blocked panel factorization of every front at
-// the level: the factored diagonal block L_kk D_k (computed here by wave 0
-// of every workgroup, or, with `staged`, read from the staging slot that
-// KDiagBlock filled) and the TRSM of 128 panel rows below it per workgroup,
-// L = F L_kk^-T D_k^-1, two threads per row (columns 0..31, then 32..63).
-// In fused mode workgroup 0 stores the factored block in the staging slot;
-// the trailing-update kernel moves it into the panel, so no workgroup of
-// this launch reads a partly written block.
-template <class Slots, class Lists>
-__global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots,
-                                                       Lists plan, int kb,
-                                                       int staged) {
+// the level: the panel rows below the diagonal block, 64 per workgroup,
+// become L21 = A21 W with W = L_kk^-T D_k^-1, a 64 x 64 x 64 FP64 MFMA
+// product. W is read from the staging slot (KDiagBlock or the previous
+// step's look-ahead filled it) or, with kFused, formed by wave 0 of
+// every workgroup, which factors and inverts the block itself; workgroup 0
+// then stores the staging slot. The trailing-update kernel moves the factored block into the
+// panel, so no workgroup of this launch reads a partly written block.
+template <bool kFused, class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, int kb) {
   BIND_SLOT(blockIdx.z);
   if (Halted(W.state)) return;
   const int s = PlanItem(plan, blockIdx.z, blockIdx.y);
@@ -927,62 +993,125 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots,
   if (c0 >= ns) return;
   const int nb = min(kNB, ns - c0);
   const int f = A.sn_f[s];
-  const int r0 = c0 + blockIdx.x * kStepRows;
-  if (r0 >= f) return;
+  const int r0 = c0 + nb + static_cast<int>(blockIdx.x) * kTile;  // first row of this workgroup
+  constexpr bool fused = kFused;
+  const bool owner = fused && blockIdx.x == 0;  // stores the staging slot
+  const bool rows = r0 < f;
+  if (!rows && !owner) return;
   double* F = W.panel + A.panel_off[s];
-  __shared__ double lt[kNB][kNB + 2];  // lt[j][i] = L(i, j), lt[i][i] = D(i)
-  __shared__ double ybuf[kStepRows][33];
+  __shared__ double wt[kNB][kLdT];  // wt[k][j] = W(k, j)
+  __shared__ double at[kNB][kLdT];  // at[k][r] = F(r0 + r, c0 + k)
   __shared__ int zero_pivot;
   const int tid = threadIdx.x;
-  if (staged != 0) {
+  // A21: rows r0 .. r0 + 63 of the block column, zero beyond the front and
+  // beyond nb (W's padding is identity). Loaded at clamped addresses into
+  // registers first, so the loads are in flight together (with the staged
+  // W's), then selected into the LDS.
+  constexpr int kPer = kNB * kTile / kThreads;
+  double av[kPer];
+  auto load_rows = [&]() {
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int idx = tid + kThreads * e;
+      const int r = idx % kTile;
+      const int k = idx / kTile;
+      av[e] = F[static_cast<int64_t>(c0 + min(k, nb - 1)) * f + min(r0 + r, f - 1)];
+    }
+  };
+  auto store_rows = [&]() {
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int idx = tid + kThreads * e;
+      const int r = idx % kTile;
+      const int k = idx / kTile;
+      const double x = Opaque(av[e]);
+      at[k][r] = (k < nb && r0 + r < f) ? x : 0.0;
+    }
+  };
+  if constexpr (!fused) {
+    load_rows();  // every workgroup left here has rows
+    // W from the staging slot: this thread's entries are (i, j) = (tid % 64,
+    // tid / 64 + 4 e); at most one of them is diagonal, so one division.
     const double* stage = StageSlot(W, s, kb);
-    for (int idx = tid; idx < kNB * kNB; idx += blockDim.x) {
-      const int i = idx % kNB;
-      const int j = idx / kNB;
-      if (i >= j) lt[j][i] = (i < nb && j < nb) ? stage[idx] : (i == j ? 1.0 : 0.0);
-    }
-    __syncthreads();
-  } else if (tid < kNB) {
-    // Wave 0: row tid of the diagonal block; rows and columns beyond nb
-    // are an identity pad.
-    // Every load is issued (clamped into the block) and the pad selected
-    // afterwards, so the 64 loads are in flight together.
-    double row[kNB];
-    const int ti = min(tid, nb - 1);
+    double v[kPer];
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-      const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + c0 + ti];
-      row[j] = (tid < nb && j < nb && j <= tid) ? v : (j == tid ? 1.0 : 0.0);
-    }
-    // ybuf (not yet in use) holds the broadcast column.
-    const bool zero = FactorDiagWave(row, tid, nb, &ybuf[0][0]);
+    for (int e = 0; e < kPer; ++e) v[e] = stage[tid + kThreads * e];
+    store_rows();
+    const int i = tid % kNB;
+    double dv = 1.0;
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-      if (j <= tid) lt[j][tid] = row[j];
+    for (int e = 0; e < kPer; ++e) dv = tid / kNB + 4 * e == i ? v[e] : dv;
+    const double rd = 1.0 / dv;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+      const int j = tid / kNB + 4 * e;
+      wt[i][j] = i < j ? v[e] : (i == j ? rd : 0.0);
     }
-    if (tid == 0) zero_pivot = zero ? 1 : 0;
-  }
-  if (staged == 0) __syncthreads();
-  if (staged == 0 && blockIdx.x == 0) {
-    const bool zero = zero_pivot != 0;
-    if (tid == 0 && zero) {
-      W.state->not_spd = 1;
-      W.state->halt = 1;
+  } else {
+    if (tid < kNB) {
+      // at is the factorization's scratch until the rows are loaded.
+      double row[kNB];
+      LoadDiagRow(F, f, c0, nb, tid, row);
+      const bool zero = FactorInvertWave<kLdT>(row, tid, nb, &at[0][0], owner ? StageSlot(W, s, kb) : nullptr);
+#pragma unroll
+      for (int j = 0; j < kNB; ++j) {
+        const double m = at[j][tid];  // M(j, tid) above the diagonal, 1 / D(j) on it
+        wt[tid][j] = j > tid ? m * at[j][j] : (j == tid ? m : 0.0);
+      }
+      if (tid == 0) zero_pivot = zero ? 1 : 0;
     }
-    double* stage = StageSlot(W, s, kb);
-    for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-      const int i = idx % nb;
-      const int j = idx / nb;
-      if (i >= j) stage[j * kNB + i] = lt[j][i];
+    __syncthreads();  // at is free again
+    if (rows) {
+      load_rows();
+      store_rows();
     }
   }
-  const int lr = tid & (kStepRows - 1);
-  const int half = tid / kStepRows;
-  const int r = r0 + lr;
-  const bool active = r >= c0 + nb && r < f;
-  if (half == 0 && active) SolveRowFirstHalf(F, f, c0, nb, r, lt, ybuf[lr]);
   __syncthreads();
-  if (half == 1 && active && nb > 32) SolveRowSecondHalf(F, f, c0, nb, r, lt, ybuf[lr]);
+  if (owner && tid == 0 && zero_pivot != 0) {
+    W.state->not_spd = 1;
+    W.state->halt = 1;
+  }
+  if (!rows) return;
+  // Each wavefront owns a 32 x 32 quadrant: four 16 x 16 FP64 MFMA blocks
+  // (operand layout as in the trailing update below: lane l's result e is
+  // row (l & 15), column 4 e + (l >> 4) of its block).
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wr = 32 * (wave & 1);
+  const int wc = 32 * (wave >> 1);
+  v4d acc[2][2];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) acc[cb][rb] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kNB; k += 4) {
+    const int kr = k + (lane >> 4);
+    double av[2], bv[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) av[rb] = at[kr][wr + 16 * rb + (lane & 15)];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) bv[cb] = wt[kr][wc + 16 * cb + (lane & 15)];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        acc[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[cb], av[rb], acc[cb][rb], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = wc + 16 * cb + 4 * e + (lane >> 4);
+      if (c >= nb) continue;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int r = r0 + wr + 16 * rb + (lane & 15);
+        if (r < f) F[static_cast<int64_t>(c0 + c) * f + r] = acc[cb][rb][e];
+      }
+    }
+  }
 }
 
 // C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
@@ -990,26 +1119,21 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots,
 // mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated.
 // With kLookAhead (mode 0), the workgroup of tile (0, 0) -- the next
 // block column's diagonal block, final once its update is stored -- then
-// factors that block and stages it for step kb + 1, whose panel step then
-// only solves rows: the factorization leaves the step's critical path
-// (the other tiles' updates run beside it).
-template <bool kLookAhead, class Slots, class Lists>
-__device__ __forceinline__ void GemmLowerBody(const Slots& slots, const Lists& plan, int kb, int mode,
-                                              int fronts) {
-  const int slot = static_cast<int>(blockIdx.z) / fronts;
+// factors and inverts that block and stages it for step kb + 1, whose panel
+// step then only multiplies rows by W (the other tiles' updates run beside
+// the factorization).
+// Tile (ti, tj) (tj <= ti) of front s in slot `slot`.
+template <bool kLookAhead, int kMode, class Slots>
+__device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int s, int ti, int tj, int kb) {
+  constexpr int mode = kMode;
   BIND_SLOT(slot);
   if (Halted(W.state)) return;
-  const int ti = blockIdx.x;
-  const int tj = blockIdx.y;
-  if (tj > ti) return;
-  const int s = PlanItem(plan, slot, static_cast<int>(blockIdx.z) % fronts);
-  if (s < 0) return;
   const int ns = A.sn_first[s + 1] - A.sn_first[s];
   const int f = A.sn_f[s];
   double* Fw = W.panel + A.panel_off[s];
   const double* F = Fw;
   const double* stage = StageSlot(W, s, kb);
-  if (mode == 0) {
+  if constexpr (mode == 0) {
     const int c0 = kb * kNB;
     if (c0 >= ns) return;
     if (ti == 0 && tj == 0) {
@@ -1025,7 +1149,7 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, const Lists& p
   int kbase, K, rbase, M, N;
   double* C;
   int64_t ldc;
-  if (mode == 0) {
+  if constexpr (mode == 0) {
     const int c0 = kb * kNB;
     if (c0 >= ns) return;
     const int nb = min(kNB, ns - c0);
@@ -1051,8 +1175,11 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, const Lists& p
   if (R0 >= M || C0 >= N) return;
   // Rows of the staged chunks are padded by two doubles, so the two
   // k rows an MFMA operand read touches fall in different LDS banks.
-  __shared__ double As[2][kTk][kTile + 2];
-  __shared__ double Bs[2][kTk][kTile + 2];
+  // One LDS array: the look-ahead factorization reuses it whole.
+  __shared__ double lds[2 * 2 * kTk * (kTile + 2)];
+  static_assert(2 * 2 * kTk * (kTile + 2) >= kNB * (kNB + 1), "look-ahead scratch");
+  auto As = reinterpret_cast<double (*)[kTk][kTile + 2]>(lds);
+  auto Bs = reinterpret_cast<double (*)[kTk][kTile + 2]>(lds + 2 * kTk * (kTile + 2));
   const int tid = threadIdx.x;
   // FP64 MFMA (16x16x4): each wavefront owns a 32x32 quadrant of the tile,
   // four 16x16 blocks. The MFMA's A operand is the B chunk (its 16 rows are
@@ -1070,7 +1197,10 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, const Lists& p
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb) acc[cb][rb] = v4d{0.0, 0.0, 0.0, 0.0};
   // Staging of one K chunk: each thread loads 4 A and 4 B entries (rows
-  // rr = idx & 63 of k row kk = idx >> 6); B is scaled by D_k.
+  // rr = idx & 63 of k row kk = idx >> 6); B is scaled by D_k. Every load
+  // is issued at an address clamped into the operands and the padding
+  // selected afterwards: a conditional load would become a branch that
+  // waits for its data before the next load is issued.
   double ra[4], rb[4];
   auto load_chunk = [&](int k0) {
 #pragma unroll
@@ -1078,13 +1208,15 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, const Lists& p
       const int idx = tid + kThreads * e;
       const int rr = idx & 63;
       const int kk = idx >> 6;
-      const int kcol = kbase + k0 + kk;
       const bool kin = k0 + kk < K;
-      const int64_t colbase = static_cast<int64_t>(min(kcol, kbase + K - 1)) * f;
-      const double dk = !kin ? 0.0
-                             : (mode == 0 ? stage[(k0 + kk) * kNB + k0 + kk] : F[colbase + kcol]);
-      ra[e] = (kin && R0 + rr < M) ? F[colbase + rbase + R0 + rr] : 0.0;
-      rb[e] = (kin && C0 + rr < N) ? F[colbase + rbase + C0 + rr] * dk : 0.0;
+      const int kq = min(k0 + kk, K - 1);
+      const int64_t colbase = static_cast<int64_t>(kbase + kq) * f;
+      const double* dk_at = mode == 0 ? stage + kq * kNB + kq : F + colbase + kbase + kq;
+      const double dk = *dk_at;
+      const double a = F[colbase + rbase + min(R0 + rr, M - 1)];
+      const double b = F[colbase + rbase + min(C0 + rr, N - 1)];
+      ra[e] = (kin && R0 + rr < M) ? a : 0.0;
+      rb[e] = (kin && C0 + rr < N) ? b * dk : 0.0;
     }
   };
   auto store_chunk = [&](int buf) {
@@ -1123,42 +1255,57 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, const Lists& p
       buf ^= 1;
     }
   }
+  // With look-ahead, tile (0, 0) is rows and columns c1 .. c1 + 63 of the
+  // panel: the next diagonal block, final once updated here. Its values
+  // also go to the LDS (column-major, kNB + 1 apart), once every wavefront is
+  // done with the staged chunks, for the factorization below.
+  const int c1 = (kb + 1) * kNB;
+  const bool ahead = kLookAhead && mode == 0 && ti == 0 && tj == 0 && c1 < ns;
+  if (ahead) __syncthreads();
+  // Mode 0 accumulates into C: its 16 entries are loaded together first
+  // (clamped addresses, as the chunks above), then the lower ones stored.
+  double prev[2][4][2];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = min(C0 + wc + 16 * cb + 4 * e + (lane >> 4), N - 1);
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int r = min(R0 + wr + 16 * rb + (lane & 15), M - 1);
+        prev[cb][e][rb] = mode == 0 ? C[static_cast<int64_t>(c) * ldc + r] : 0.0;
+      }
+    }
+  }
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int c = C0 + wc + 16 * cb + 4 * e + (lane >> 4);
-      if (c >= N) continue;
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb) {
         const int r = R0 + wr + 16 * rb + (lane & 15);
-        if (r >= M || c > r) continue;
-        double* dst = C + static_cast<int64_t>(c) * ldc + r;
-        *dst = mode == 0 ? *dst - acc[cb][rb][e] : -acc[cb][rb][e];
+        if (c < N && r < M && c <= r) {
+          const double v = prev[cb][e][rb] - acc[cb][rb][e];
+          C[static_cast<int64_t>(c) * ldc + r] = v;
+          if (ahead) lds[c * (kNB + 1) + r] = v;
+        }
       }
     }
   }
   if constexpr (kLookAhead) {
-    const int c1 = (kb + 1) * kNB;
-    if (mode != 0 || ti != 0 || tj != 0 || c1 >= ns) return;
-    // This tile is rows and columns c1 .. c1 + 63 of the panel: the next
-    // diagonal block, updated for the last time above.
-    __syncthreads();  // its stores are visible to the workgroup; As is free
+    if (!ahead) return;
+    __syncthreads();
     if (tid >= kNB) return;
+    // Row tid of the block (lower triangle; identity pad beyond nb1). Every
+    // entry read was written above: nb1 <= N <= M.
     const int nb1 = min(kNB, ns - c1);
     double row[kNB];
-    const int tr = min(tid, nb1 - 1);
 #pragma unroll
     for (int j = 0; j < kNB; ++j) {
-      const double v = F[static_cast<int64_t>(c1 + min(j, nb1 - 1)) * f + c1 + tr];
-      row[j] = (tid < nb1 && j < nb1 && j <= tid) ? v : (j == tid ? 1.0 : 0.0);
+      row[j] = (tid < nb1 && j < nb1 && j <= tid) ? lds[j * (kNB + 1) + tid] : (j == tid ? 1.0 : 0.0);
     }
-    const bool zero = FactorDiagWave(row, tid, nb1, &As[0][0][0]);
-    double* next = StageSlot(W, s, kb + 1);
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-      if (j <= tid && tid < nb1) next[j * kNB + tid] = row[j];
-    }
+    const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb1, lds, StageSlot(W, s, kb + 1));
     if (tid == 0 && zero) {
       W.state->not_spd = 1;
       W.state->halt = 1;
@@ -1166,10 +1313,14 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, const Lists& p
   }
 }
 
+// Grid (tile rows, tile columns, fronts x slots) over the level's fronts.
 template <class Slots, class Lists>
-__global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots, Lists plan, int kb, int mode,
-                                                          int fronts) {
-  GemmLowerBody<false>(slots, plan, kb, mode, fronts);
+__global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots, Lists plan, int kb, int fronts) {
+  if (blockIdx.y > blockIdx.x) return;
+  const int slot = static_cast<int>(blockIdx.z) / fronts;
+  const int s = PlanItem(plan, slot, static_cast<int>(blockIdx.z) % fronts);
+  if (s < 0) return;
+  GemmLowerBody<false, 0>(slots, slot, s, blockIdx.x, blockIdx.y, kb);
 }
 
 // Trailing update with look-ahead (the diagonal factorization needs a row of
@@ -1177,7 +1328,25 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots, Lists pla
 template <class Slots, class Lists>
 __global__ void __launch_bounds__(kThreads, 2) KGemmLowerAhead(Slots slots, Lists plan, int kb,
                                                                int fronts) {
-  GemmLowerBody<true>(slots, plan, kb, 0, fronts);
+  if (blockIdx.y > blockIdx.x) return;
+  const int slot = static_cast<int>(blockIdx.z) / fronts;
+  const int s = PlanItem(plan, slot, static_cast<int>(blockIdx.z) % fronts);
+  if (s < 0) return;
+  GemmLowerBody<true, 0>(slots, slot, s, blockIdx.x, blockIdx.y, kb);
+}
+
+// Update matrices of the level's fronts over a flat list of their lower
+// tiles (grid: tiles, slots): item t of a slot's list is the pair (front,
+// ti << 16 | tj), so every workgroup has a tile to compute.
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads, 4) KGemmUpdate(Slots slots, Lists tiles) {
+  const int slot = blockIdx.y;
+  const ListRef lr = tiles.get(slot);
+  const int t = blockIdx.x;
+  if (t >= lr.count) return;
+  const int s = lr.items[2 * t];
+  const int packed = lr.items[2 * t + 1];
+  GemmLowerBody<false, 1>(slots, slot, s, packed >> 16, packed & 0xffff, 0);
 }
 
 // ---------------------------------------------------------------- solves
@@ -1691,6 +1860,7 @@ void LaunchShape::Merge(const LaunchShape& o) {
     a.max_nb = std::max(a.max_nb, b.max_nb);
     a.max_f = std::max(a.max_f, b.max_f);
     a.max_m = std::max(a.max_m, b.max_m);
+    a.tiles = std::max(a.tiles, b.tiles);
     if (a.rank_count.size() < b.rank_count.size()) {
       a.rank_count.resize(b.rank_count.size(), 0);
       a.rank_max_m.resize(b.rank_max_m.size(), 0);
@@ -1723,6 +1893,15 @@ auto LevelLists(const LaunchShape& sh, int L) {
     return OneList{sh.level_plan_h[static_cast<size_t>(L)]};
   } else {
     return ListTable{sh.level_plan + static_cast<size_t>(L) * static_cast<size_t>(sh.slots)};
+  }
+}
+
+template <bool kOne>
+auto TileLists(const LaunchShape& sh, int L) {
+  if constexpr (kOne) {
+    return OneList{sh.tile_plan_h[static_cast<size_t>(L)]};
+  } else {
+    return ListTable{sh.tile_plan + static_cast<size_t>(L) * static_cast<size_t>(sh.slots)};
   }
 }
 
@@ -1780,7 +1959,7 @@ void NormalSlotsImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& 
 
 // Above this many TRSM workgroups a level factors its diagonal blocks in a
 // launch of their own instead of once per workgroup.
-constexpr int kStagedMinBlocks = 512;
+constexpr int kStagedMinBlocks = 1024;
 
 template <bool kOne>
 void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEvents* step_events) {
@@ -1801,8 +1980,10 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
       const auto plan = LevelLists<kOne>(sh, L);
       const int nblocks = (lv.max_nb + kNB - 1) / kNB;
       for (int kb = 0; kb < nblocks; ++kb) {
-        const int rows = lv.max_f - kb * kNB;
-        const int row_blocks = (rows + kStepRows - 1) / kStepRows;
+        // Rows below the block: at most f - c0 - 1 (a front's block has at
+        // least one column); one 64-row workgroup each, at least one.
+        const int rows = lv.max_f - kb * kNB - 1;
+        const int row_blocks = rows > 0 ? (rows + kTile - 1) / kTile : 1;
         // Step 0: with few fronts each workgroup factors the diagonal block
         // itself (one launch on the critical path), with many the block is
         // factored once per front first. Later steps find their block staged
@@ -1812,14 +1993,20 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
         const bool timed = step_events != nullptr && step_events->used < step_events->capacity;
         if (timed) (void)hipEventRecord(step_events->events[2 * step_events->used], s);
         if (kb == 0 && staged != 0) KDiagBlock<<<dim3(count, k), kNB, 0, s>>>(slots, plan, kb);
-        KPanelStep<<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb, staged);
+        if (staged != 0) {
+          KPanelStep<false><<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb);
+        } else {
+          KPanelStep<true><<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb);
+        }
         if (timed) {
           (void)hipEventRecord(step_events->events[2 * step_events->used + 1], s);
           ++step_events->used;
         }
         const int trail_rows = lv.max_f - (kb + 1) * kNB;
         const int trail_cols = lv.max_nb - (kb + 1) * kNB;
-        const int tr = trail_rows > 0 ? (trail_rows + kTile - 1) / kTile : 1;
+        // With no columns beyond the block in any front only tile (0, 0)
+        // has work: moving the factored block into the panel.
+        const int tr = trail_rows > 0 && trail_cols > 0 ? (trail_rows + kTile - 1) / kTile : 1;
         const int tc = trail_cols > 0 ? (trail_cols + kTile - 1) / kTile : 1;
         const dim3 grid(tr, tc, static_cast<unsigned>(count) * k);
         const bool gt = step_events != nullptr && step_events->gemm_used < step_events->gemm_capacity;
@@ -1827,15 +2014,14 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
         if (kb + 1 < nblocks) {
           KGemmLowerAhead<<<grid, kThreads, 0, s>>>(slots, plan, kb, count);
         } else {
-          KGemmLower<<<grid, kThreads, 0, s>>>(slots, plan, kb, 0, count);
+          KGemmLower<<<grid, kThreads, 0, s>>>(slots, plan, kb, count);
         }
         if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used++ + 1], s);
       }
-      if (lv.max_m > 0) {
-        const int t = (lv.max_m + kTile - 1) / kTile;
+      if (lv.tiles > 0) {
         const bool gt = step_events != nullptr && step_events->gemm_used < step_events->gemm_capacity;
         if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used], s);
-        KGemmLower<<<dim3(t, t, static_cast<unsigned>(count) * k), kThreads, 0, s>>>(slots, plan, 0, 1, count);
+        KGemmUpdate<<<dim3(lv.tiles, k), kThreads, 0, s>>>(slots, TileLists<kOne>(sh, L));
         if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used++ + 1], s);
       }
     }

@@ -185,6 +185,7 @@ struct LaunchShape {
   int64_t panel_total{0};
   struct Level {
     int count{0}, max_nb{0}, max_f{0}, max_m{0};
+    int tiles{0};  // 64 x 64 lower tiles of the fronts' update matrices
     std::vector<int> rank_count;  // largest child-list length per child rank
     std::vector<int> rank_max_m;  // largest child update size per child rank
     // Largest number of a rank's child columns landing in the parent's own
@@ -196,11 +197,14 @@ struct LaunchShape {
   std::vector<Level> levels;
   // Launch plans in device memory: level_plan[L * slots + b] = the fronts of
   // level L in slot b; rank_plan[(rank_base + r) * slots + b] = the
-  // children of rank r at that level (count 0 where a slot has none).
+  // children of rank r at that level (count 0 where a slot has none);
+  // tile_plan[L * slots + b] = the update-matrix tiles of level L in slot b
+  // (count pairs (front, ti << 16 | tj) at items).
   const ListRef* level_plan{nullptr};
   const ListRef* rank_plan{nullptr};
+  const ListRef* tile_plan{nullptr};
   // Host copies of the plans, for launches over a single slot.
-  std::vector<ListRef> level_plan_h, rank_plan_h;
+  std::vector<ListRef> level_plan_h, rank_plan_h, tile_plan_h;
   /// Grows this shape to cover `o` as well.
   void Merge(const LaunchShape& o);
 };
