@@ -779,33 +779,43 @@ __device__ __forceinline__ double ReadLane(double v, int lane) {
                               static_cast<long long>(static_cast<unsigned>(lo)));
 }
 
+// 1 / d: hardware reciprocal estimate refined by two Newton steps (each
+// doubles the correct bits; the result is within an ulp of the quotient).
+__device__ __forceinline__ double Recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return r;
+}
+
 // Right-looking LDL^T of a 64x64 block held by one wavefront, row `lane` in
 // `row` (lower triangle and diagonal meaningful; the upper part is scratch;
-// rows and columns from nb on are an identity pad). Step k takes the pivot
-// from its lane with v_readlane and broadcasts the multipliers of column k
-// through the LDS vector `col` (kNB doubles): every lane writes its
-// multiplier, then reads all of them (the same address in every lane). LDS
-// operations of one wavefront complete in order, so no barrier is needed.
-// On return row[j < lane] = L(lane, j) and row[lane] = D(lane). Returns true
+// rows and columns from nb on are an identity pad, whose steps change
+// nothing in the lower triangle, so every step runs and the 64 steps form
+// one straight-line block the compiler can interleave). Step k: every lane
+// broadcasts its unscaled c_ik = row[k] through the LDS vector `col` (kNB
+// doubles) while the pivot d = c_kk comes from lane k with v_readlane, so
+// the broadcast overlaps the reciprocal; then row[j] -= (c_ik / d) c_jk for
+// j > k. Every lane stores its multiplier in row[k] (on and above the
+// diagonal that is scratch: a per-step lane test would hold 64 masks live)
+// and the pivots go to col[kNB + k]. LDS operations of one wavefront
+// complete in order, so no barrier is needed. `col` holds 2 kNB doubles. On
+// return row[j < lane] = L(lane, j) and col[kNB + j] = D(j). Returns true
 // when a pivot is exactly zero.
 __device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int nb, double* col) {
+  (void)nb;
   bool zero = false;
 #pragma unroll
   for (int k = 0; k < kNB; ++k) {
-    // The identity pad needs no steps; a guard (not a break) keeps the loop
-    // unrolled and `row` in registers.
-    if (k < nb) {
-      const double d = ReadLane(row[k], k);
-      zero = zero || d == 0.0;
-      const double l = row[k] / d;
-      if (lane > k) row[k] = l;
-      col[lane] = lane > k ? l : 0.0;
-      __builtin_amdgcn_wave_barrier();
-      const double t = -l * d;
+    const double c = row[k];
+    col[lane] = c;
+    const double d = ReadLane(c, k);
+    zero = zero || d == 0.0;
+    col[kNB + k] = d;
+    const double l = c * Recip(d);
+    row[k] = l;
 #pragma unroll
-      for (int j = k + 1; j < kNB; ++j) row[j] = fma(t, col[j], row[j]);
-      __builtin_amdgcn_wave_barrier();
-    }
+    for (int j = k + 1; j < kNB; ++j) row[j] = fma(-l, col[j], row[j]);
   }
   return zero;
 }
@@ -905,29 +915,21 @@ template <int kLd>
 __device__ __forceinline__ bool FactorInvertWave(double (&row)[kNB], int lane, int nb, double* lt,
                                                  double* stage) {
   const bool zero = FactorDiagWave(row, lane, nb, lt);
-  // LDS operations of one wavefront complete in order: the factorization's
-  // last broadcast reads are done before these writes land.
+  // LDS operations of one wavefront complete in order: D is read and the
+  // factorization's broadcasts are done before these writes land. Entries
+  // on and above the diagonal receive scratch here and 1 / D and M below.
+  const double dl = lt[kNB + lane];
 #pragma unroll
-  for (int t = 0; t < kNB; ++t) {
-    if (t < lane) lt[t * kLd + lane] = row[t];
-  }
-  double d = 1.0;  // D(lane), selected with constant indices (row stays in registers)
-#pragma unroll
-  for (int j = 0; j < kNB; ++j) d = j == lane ? row[j] : d;
-  lt[lane * kLd + lane] = 1.0 / d;
-  if (stage != nullptr) {
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-      if (j <= lane) stage[j * kNB + lane] = row[j];
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
+  for (int t = 0; t < kNB; ++t) lt[t * kLd + lane] = row[t];
+  lt[lane * kLd + lane] = 1.0 / dl;
   InvertUnitLowerWave<kLd>(lt, lane);
-  __builtin_amdgcn_wave_barrier();
   if (stage != nullptr) {
-#pragma unroll
+    // L below the diagonal and M (scaled to W) above it share addresses.
+#pragma unroll 4
     for (int j = 0; j < kNB; ++j) {
-      if (j > lane) stage[j * kNB + lane] = lt[j * kLd + lane] * lt[j * kLd + j];
+      const double v = lt[j * kLd + lane];
+      const double rdj = lt[j * kLd + j];
+      stage[j * kNB + lane] = lane > j ? v : (lane < j ? v * rdj : dl);
     }
   }
   return zero;
@@ -1053,7 +1055,7 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
       double row[kNB];
       LoadDiagRow(F, f, c0, nb, tid, row);
       const bool zero = FactorInvertWave<kLdT>(row, tid, nb, &at[0][0], owner ? StageSlot(W, s, kb) : nullptr);
-#pragma unroll
+#pragma unroll 4
       for (int j = 0; j < kNB; ++j) {
         const double m = at[j][tid];  // M(j, tid) above the diagonal, 1 / D(j) on it
         wt[tid][j] = j > tid ? m * at[j][j] : (j == tid ? m : 0.0);
@@ -1323,10 +1325,11 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots, Lists pla
   GemmLowerBody<false, 0>(slots, slot, s, blockIdx.x, blockIdx.y, kb);
 }
 
-// Trailing update with look-ahead (the diagonal factorization needs a row of
-// 64 doubles in registers: fewer workgroups per CU).
+// Trailing update with look-ahead (the diagonal factorization keeps a row of
+// 64 doubles and a straight-line schedule in registers: one workgroup per
+// CU).
 template <class Slots, class Lists>
-__global__ void __launch_bounds__(kThreads, 2) KGemmLowerAhead(Slots slots, Lists plan, int kb,
+__global__ void __launch_bounds__(kThreads, 1) KGemmLowerAhead(Slots slots, Lists plan, int kb,
                                                                int fronts) {
   if (blockIdx.y > blockIdx.x) return;
   const int slot = static_cast<int>(blockIdx.z) / fronts;
