@@ -799,9 +799,13 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
   }
   std::vector<int64_t> cptr(static_cast<size_t>(an.num_sn) + 1, 0);
   std::vector<int32_t> clist;
+  // Diagonal-block inverses: 64 x 64 per block column of every supernode.
+  std::vector<int64_t> inv_off(static_cast<size_t>(an.num_sn) + 1, 0);
   for (int s = 0; s < an.num_sn; ++s) {
     for (const int ch : an.children[static_cast<size_t>(s)]) clist.push_back(ch);
     cptr[static_cast<size_t>(s) + 1] = static_cast<int64_t>(clist.size());
+    const int ns = an.sn_first[static_cast<size_t>(s) + 1] - an.sn_first[static_cast<size_t>(s)];
+    inv_off[static_cast<size_t>(s) + 1] = inv_off[static_cast<size_t>(s)] + static_cast<int64_t>((ns + 63) / 64) * 64 * 64;
   }
   // Level lists and, per level and child rank, the children to extend-add,
   // with the launch shape of this context.
@@ -908,12 +912,14 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     D.op_off = E.Upload(an.op_off);
     D.children_ptr = E.Upload(cptr);
     D.children = E.Upload(clist);
+    D.inv_off = E.Upload(inv_off);
     c.level_sn_d = E.Upload(lvl);
     c.child_lists_d = E.Upload(ranks);
     c.tiles_d = E.Upload(tiles);
     c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
     c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
     c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 2 * 64 * 64);
+    c.dw.diag_inv = E.Alloc<double>(static_cast<size_t>(inv_off.back()));
     c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
     // The launch plans and the slot come last: they hold addresses placed
     // above (known in the placing pass).

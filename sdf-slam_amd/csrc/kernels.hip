@@ -770,6 +770,12 @@ __device__ __forceinline__ double Opaque(double v) {
   return v;
 }
 
+// L_kk^-1 of supernode s's kb-th diagonal block (column-major 64 x 64,
+// identity beyond the block's nb columns), kept for the solves.
+__device__ __forceinline__ double* DiagInv(const DeviceAnalysis& A, const DeviceWork& W, int s, int kb) {
+  return W.diag_inv + A.inv_off[s] + static_cast<int64_t>(kb) * kNB * kNB;
+}
+
 // Reads lane `lane`'s value of v (a wave-uniform lane index).
 __device__ __forceinline__ double ReadLane(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -910,10 +916,11 @@ __device__ __forceinline__ void InvertUnitLowerWave(double* lt, int lane) {
 // for i < j and W(j, j) = 1 / D(j). With `stage`, stores the staging slot:
 // the lower triangle holds L with D on the diagonal, the strict upper
 // triangle W (the readers form W's diagonal); the padding beyond nb is
-// identity in both. Returns true when a pivot is exactly zero.
+// identity in both. With `inv`, stores M there as well (column-major 64 x
+// 64). Returns true when a pivot is exactly zero.
 template <int kLd>
 __device__ __forceinline__ bool FactorInvertWave(double (&row)[kNB], int lane, int nb, double* lt,
-                                                 double* stage) {
+                                                 double* stage, double* inv) {
   const bool zero = FactorDiagWave(row, lane, nb, lt);
   // LDS operations of one wavefront complete in order: D is read and the
   // factorization's broadcasts are done before these writes land. Entries
@@ -930,6 +937,14 @@ __device__ __forceinline__ bool FactorInvertWave(double (&row)[kNB], int lane, i
       const double v = lt[j * kLd + lane];
       const double rdj = lt[j * kLd + j];
       stage[j * kNB + lane] = lane > j ? v : (lane < j ? v * rdj : dl);
+    }
+  }
+  if (inv != nullptr) {
+    // M itself, column-major with its unit diagonal and zeros above.
+#pragma unroll 4
+    for (int j = 0; j < kNB; ++j) {
+      const double m = lt[lane * kLd + j];
+      inv[j * kNB + lane] = lane > j ? m : (lane == j ? 1.0 : 0.0);
     }
   }
   return zero;
@@ -969,7 +984,7 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
   __shared__ double lt[kNB * (kNB + 1)];
   double row[kNB];
   LoadDiagRow(F, f, c0, nb, tid, row);
-  const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb, lt, StageSlot(W, s, kb));
+  const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb, lt, StageSlot(W, s, kb), DiagInv(A, W, s, kb));
   if (tid == 0 && zero) {
     W.state->not_spd = 1;
     W.state->halt = 1;
@@ -1054,7 +1069,8 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
       // at is the factorization's scratch until the rows are loaded.
       double row[kNB];
       LoadDiagRow(F, f, c0, nb, tid, row);
-      const bool zero = FactorInvertWave<kLdT>(row, tid, nb, &at[0][0], owner ? StageSlot(W, s, kb) : nullptr);
+      const bool zero = FactorInvertWave<kLdT>(row, tid, nb, &at[0][0], owner ? StageSlot(W, s, kb) : nullptr,
+                                               owner ? DiagInv(A, W, s, kb) : nullptr);
 #pragma unroll 4
       for (int j = 0; j < kNB; ++j) {
         const double m = at[j][tid];  // M(j, tid) above the diagonal, 1 / D(j) on it
@@ -1307,7 +1323,7 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
     for (int j = 0; j < kNB; ++j) {
       row[j] = (tid < nb1 && j < nb1 && j <= tid) ? lds[j * (kNB + 1) + tid] : (j == tid ? 1.0 : 0.0);
     }
-    const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb1, lds, StageSlot(W, s, kb + 1));
+    const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb1, lds, StageSlot(W, s, kb + 1), DiagInv(A, W, s, kb + 1));
     if (tid == 0 && zero) {
       W.state->not_spd = 1;
       W.state->halt = 1;
@@ -1386,25 +1402,15 @@ __global__ void KForward(Slots slots, Lists plan, int tri) {
   for (int b0 = 0; b0 < n; b0 += 64) {
     const int b1 = min(b0 + 64, n);
     if (tid < 64) {
-      // Diagonal block: lane i holds v[i]; the pivots are broadcast in order.
-      const int i = b0 + tid;
-      double vi = i < b1 ? v[i] : 0.0;
-      for (int k0 = b0; k0 < b1; k0 += 16) {
-        double fv[16];
+      // Diagonal block: y = M v with M = L_kk^-1 (identity padded, so the
+      // entries v[j] past the block meet zeros); lane tid forms row tid,
+      // four partial sums. Every lane reads v before any writes it.
+      const double* Mb = DiagInv(A, W, s, b0 / kNB);
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          fv[t] = (k0 + t < b1 && i < b1) ? F[static_cast<int64_t>(k0 + t) * f + i] : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          const int k = k0 + t;
-          if (k < b1) {
-            const double yk = __shfl(vi, k - b0, 64);
-            if (i > k && i < b1) vi = fma(-fv[t], yk, vi);
-          }
-        }
-      }
-      if (i < b1) v[i] = vi;
+      for (int j = 0; j < kNB; ++j) acc[j & 3] = fma(Mb[j * kNB + tid], v[min(b0 + j, f - 1)], acc[j & 3]);
+      const double y = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+      if (b0 + tid < b1) v[b0 + tid] = y;
     }
     __syncthreads();
     // Rows below the block; loads batched ahead of the ordered FMA chain.
@@ -1530,8 +1536,10 @@ __global__ void KBackward(Slots slots, Lists plan, int tri) {
     if (lane == 0) t[j] -= acc;
   }
   __syncthreads();
-  for (int b1 = n; b1 > 0;) {
-    const int b0 = max(0, b1 - 64);
+  for (int kb = (n + kNB - 1) / kNB - 1; kb >= 0; --kb) {
+    // The factorization's block columns, last first.
+    const int b0 = kb * kNB;
+    const int b1 = min(n, b0 + kNB);
     for (int j = b0 + wave; j < b1; j += waves) {
       const double* col = F + static_cast<int64_t>(j) * f;
       double acc = 0.0;
@@ -1549,30 +1557,22 @@ __global__ void KBackward(Slots slots, Lists plan, int tri) {
     }
     __syncthreads();
     if (tid < 64) {
-      // Within the block: lane i holds t[b0 + i]; x_j is final when reached.
-      const int i = b0 + lane;
-      double tv = i < b1 ? t[i] : 0.0;
-      const double* row = F + static_cast<int64_t>(i) * f;  // column i of the panel
-      for (int j0 = b1 - 1; j0 >= b0; j0 -= 16) {
-        double fv[16];
+      // Within the block: x = M^T t with M = L_kk^-1 (identity padded, so
+      // the entries t[i] past the block meet zeros); lane tid forms x_tid
+      // from column tid of M, two doubles a load, four partial sums. Every
+      // lane reads t before any writes it.
+      const double2* Mc = reinterpret_cast<const double2*>(DiagInv(A, W, s, kb) + tid * kNB);
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int j = j0 - u;
-          fv[u] = (j >= b0 && i < j) ? row[j] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int j = j0 - u;
-          if (j >= b0) {
-            const double xj = __shfl(tv, j - b0, 64);
-            if (i < j) tv = fma(-fv[u], xj, tv);
-          }
-        }
+      for (int i = 0; i < kNB / 2; ++i) {
+        const double2 m = Mc[i];
+        acc[(2 * i) & 3] = fma(m.x, t[min(b0 + 2 * i, f - 1)], acc[(2 * i) & 3]);
+        acc[(2 * i + 1) & 3] = fma(m.y, t[min(b0 + 2 * i + 1, f - 1)], acc[(2 * i + 1) & 3]);
       }
-      if (i < b1) t[i] = tv;
+      const double x = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+      if (b0 + tid < b1) t[b0 + tid] = x;
     }
     __syncthreads();
-    b1 = b0;
   }
   for (int j = tid; j < n; j += blockDim.x) {
     W.xnew[first + j] = t[j];

@@ -101,6 +101,7 @@ struct DeviceWork {
   double* step;  // step in original column order
   double* panel;
   double* diag_stage;  // factored 64x64 diagonal blocks in flight, two per supernode
+  double* diag_inv;    // L_kk^-1 of every 64x64 diagonal block (identity padded), for the solves
   double* upd;
   double* vec;  // forward-solve vectors, sum of front sizes
   double* xnew;  // solution in elimination order
@@ -149,6 +150,7 @@ struct DeviceAnalysis {
   const int64_t* op_off;
   const int64_t* children_ptr;  // children of each supernode, ascending
   const int32_t* children;
+  const int64_t* inv_off;  // per supernode: its block columns' entries in diag_inv
 };
 
 constexpr int kPartialBlocks = 512;
