@@ -910,44 +910,47 @@ __device__ __forceinline__ void InvertUnitLowerWave(double* lt, int lane) {
 }
 
 // Factors the diagonal block held in `row` (one wavefront, as
-// FactorDiagWave) and forms its TRSM operator W = L^-T D^-1 in the LDS tile
-// `lt` (kNB rows of kLd doubles): afterwards lt[j * kLd + j] = 1 / D(j) and
-// lt[j * kLd + i] = M(j, i) for i < j, M = L^-1, so W(i, j) = M(j, i) / D(j)
-// for i < j and W(j, j) = 1 / D(j). With `stage`, stores the staging slot:
-// the lower triangle holds L with D on the diagonal, the strict upper
-// triangle W (the readers form W's diagonal); the padding beyond nb is
-// identity in both. With `inv`, stores M there as well (column-major 64 x
-// 64). Returns true when a pivot is exactly zero.
+// FactorDiagWave) and inverts its unit lower triangle in the LDS tile `lt`
+// (kNB rows of kLd doubles): afterwards lt[j * kLd + j] = 1 / D(j),
+// lt[j * kLd + i] = L(i, j) for i > j and lt[i * kLd + j] = M(i, j) for
+// i > j, M = L^-1 (so W = L^-T D^-1 has W(i, j) = M(j, i) / D(j) above the
+// diagonal and 1 / D(j) on it); dvec[j] = D(j). Returns true when a pivot is
+// exactly zero.
 template <int kLd>
 __device__ __forceinline__ bool FactorInvertWave(double (&row)[kNB], int lane, int nb, double* lt,
-                                                 double* stage, double* inv) {
+                                                 double* dvec) {
   const bool zero = FactorDiagWave(row, lane, nb, lt);
   // LDS operations of one wavefront complete in order: D is read and the
   // factorization's broadcasts are done before these writes land. Entries
   // on and above the diagonal receive scratch here and 1 / D and M below.
   const double dl = lt[kNB + lane];
+  dvec[lane] = dl;
 #pragma unroll
   for (int t = 0; t < kNB; ++t) lt[t * kLd + lane] = row[t];
   lt[lane * kLd + lane] = 1.0 / dl;
   InvertUnitLowerWave<kLd>(lt, lane);
-  if (stage != nullptr) {
-    // L below the diagonal and M (scaled to W) above it share addresses.
-#pragma unroll 4
-    for (int j = 0; j < kNB; ++j) {
-      const double v = lt[j * kLd + lane];
-      const double rdj = lt[j * kLd + j];
-      stage[j * kNB + lane] = lane > j ? v : (lane < j ? v * rdj : dl);
-    }
-  }
-  if (inv != nullptr) {
-    // M itself, column-major with its unit diagonal and zeros above.
-#pragma unroll 4
-    for (int j = 0; j < kNB; ++j) {
-      const double m = lt[lane * kLd + j];
-      inv[j * kNB + lane] = lane > j ? m : (lane == j ? 1.0 : 0.0);
-    }
-  }
   return zero;
+}
+
+// Stores a block left by FactorInvertWave, kN threads cooperating (thread
+// tid takes entries tid, tid + kN, ... in column-major order): the staging
+// slot (L below the diagonal, D on it, W above; identity padding beyond nb
+// in both) and, with `inv`, M itself (unit diagonal, zeros above).
+template <int kLd, int kN>
+__device__ __forceinline__ void StoreFactored(const double* lt, const double* dvec, double* stage, double* inv,
+                                              int tid) {
+#pragma unroll 4
+  for (int e = 0; e < kNB * kNB / kN; ++e) {
+    const int idx = tid + kN * e;
+    const int i = idx % kNB;
+    const int j = idx / kNB;
+    const double v = lt[j * kLd + i];
+    stage[idx] = i > j ? v : (i < j ? v * lt[j * kLd + j] : dvec[j]);
+    if (inv != nullptr) {
+      const double m = lt[i * kLd + j];
+      inv[idx] = i > j ? m : (i == j ? 1.0 : 0.0);
+    }
+  }
 }
 
 // Row `lane` of the kb-th diagonal block of front s (lower triangle and
@@ -981,10 +984,11 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
   const int f = A.sn_f[s];
   const double* F = W.panel + A.panel_off[s];
   const int tid = threadIdx.x;
-  __shared__ double lt[kNB * (kNB + 1)];
+  __shared__ double lt[kNB * (kNB + 1) + kNB];  // the tile, then D
   double row[kNB];
   LoadDiagRow(F, f, c0, nb, tid, row);
-  const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb, lt, StageSlot(W, s, kb), DiagInv(A, W, s, kb));
+  const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb, lt, lt + kNB * (kNB + 1));
+  StoreFactored<kNB + 1, kNB>(lt, lt + kNB * (kNB + 1), StageSlot(W, s, kb), DiagInv(A, W, s, kb), tid);
   if (tid == 0 && zero) {
     W.state->not_spd = 1;
     W.state->halt = 1;
@@ -1018,6 +1022,7 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
   double* F = W.panel + A.panel_off[s];
   __shared__ double wt[kNB][kLdT];  // wt[k][j] = W(k, j)
   __shared__ double at[kNB][kLdT];  // at[k][r] = F(r0 + r, c0 + k)
+  __shared__ double dsh[kNB];  // D, with kFused
   __shared__ int zero_pivot;
   const int tid = threadIdx.x;
   // A21: rows r0 .. r0 + 63 of the block column, zero beyond the front and
@@ -1069,15 +1074,20 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
       // at is the factorization's scratch until the rows are loaded.
       double row[kNB];
       LoadDiagRow(F, f, c0, nb, tid, row);
-      const bool zero = FactorInvertWave<kLdT>(row, tid, nb, &at[0][0], owner ? StageSlot(W, s, kb) : nullptr,
-                                               owner ? DiagInv(A, W, s, kb) : nullptr);
-#pragma unroll 4
-      for (int j = 0; j < kNB; ++j) {
-        const double m = at[j][tid];  // M(j, tid) above the diagonal, 1 / D(j) on it
-        wt[tid][j] = j > tid ? m * at[j][j] : (j == tid ? m : 0.0);
-      }
+      const bool zero = FactorInvertWave<kLdT>(row, tid, nb, &at[0][0], dsh);
       if (tid == 0) zero_pivot = zero ? 1 : 0;
     }
+    __syncthreads();
+    // W from the tile: M(j, i) above the diagonal, 1 / D(j) on it.
+#pragma unroll 4
+    for (int e = 0; e < kNB * kNB / kThreads; ++e) {
+      const int idx = tid + kThreads * e;
+      const int i = idx % kNB;
+      const int j = idx / kNB;
+      const double m = at[j][i];
+      wt[i][j] = i < j ? m * at[j][j] : (i == j ? m : 0.0);
+    }
+    if (owner) StoreFactored<kLdT, kThreads>(&at[0][0], dsh, StageSlot(W, s, kb), DiagInv(A, W, s, kb), tid);
     __syncthreads();  // at is free again
     if (rows) {
       load_rows();
@@ -1195,7 +1205,7 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
   // k rows an MFMA operand read touches fall in different LDS banks.
   // One LDS array: the look-ahead factorization reuses it whole.
   __shared__ double lds[2 * 2 * kTk * (kTile + 2)];
-  static_assert(2 * 2 * kTk * (kTile + 2) >= kNB * (kNB + 1), "look-ahead scratch");
+  static_assert(2 * 2 * kTk * (kTile + 2) >= kNB * (kNB + 2), "look-ahead scratch");
   auto As = reinterpret_cast<double (*)[kTk][kTile + 2]>(lds);
   auto Bs = reinterpret_cast<double (*)[kTk][kTile + 2]>(lds + 2 * kTk * (kTile + 2));
   const int tid = threadIdx.x;
@@ -1314,20 +1324,26 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
   if constexpr (kLookAhead) {
     if (!ahead) return;
     __syncthreads();
-    if (tid >= kNB) return;
-    // Row tid of the block (lower triangle; identity pad beyond nb1). Every
-    // entry read was written above: nb1 <= N <= M.
-    const int nb1 = min(kNB, ns - c1);
-    double row[kNB];
+    // Wave 0 factors and inverts the block (the tile, then D, in the LDS);
+    // then the whole workgroup stores it.
+    double* dvec = lds + kNB * (kNB + 1);
+    if (tid < kNB) {
+      // Row tid of the block (lower triangle; identity pad beyond nb1). Every
+      // entry read was written above: nb1 <= N <= M.
+      const int nb1 = min(kNB, ns - c1);
+      double row[kNB];
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-      row[j] = (tid < nb1 && j < nb1 && j <= tid) ? lds[j * (kNB + 1) + tid] : (j == tid ? 1.0 : 0.0);
+      for (int j = 0; j < kNB; ++j) {
+        row[j] = (tid < nb1 && j < nb1 && j <= tid) ? lds[j * (kNB + 1) + tid] : (j == tid ? 1.0 : 0.0);
+      }
+      const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb1, lds, dvec);
+      if (tid == 0 && zero) {
+        W.state->not_spd = 1;
+        W.state->halt = 1;
+      }
     }
-    const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb1, lds, StageSlot(W, s, kb + 1), DiagInv(A, W, s, kb + 1));
-    if (tid == 0 && zero) {
-      W.state->not_spd = 1;
-      W.state->halt = 1;
-    }
+    __syncthreads();
+    StoreFactored<kNB + 1, kThreads>(lds, dvec, StageSlot(W, s, kb + 1), DiagInv(A, W, s, kb + 1), tid);
   }
 }
 
