@@ -1978,7 +1978,7 @@ void NormalSlotsImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& 
 
 // Above this many TRSM workgroups a level factors its diagonal blocks in a
 // launch of their own instead of once per workgroup.
-constexpr int kStagedMinBlocks = 1024;
+constexpr int kStagedMinBlocks = 256;
 
 template <bool kOne>
 void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEvents* step_events) {
