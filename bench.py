@@ -204,11 +204,13 @@ def main():
     kernel_ms = phases["point_assembly_kernel_ms"]
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
     # Dominant kernel family: the trailing updates and update matrices of
-    # the supernodal LDL^T (KGemmLower / KGemmLowerAhead, FP64 MFMA).
+    # the supernodal LDL^T (KGemmLower / KGemmLowerAhead / KGemmUpdate,
+    # FP64 MFMA).
     gemm_flops = info["gemm_flops"] / max(info["gemm_launches"], 1)
     gemm_ms = phases["gemm_kernel_ms"]
     gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
-    # The 64-column panel step (TRSM below the diagonal block), FP64.
+    # The 64-column panel step (the rows below the diagonal block times its
+    # TRSM operator W = L_kk^-T D^-1, FP64 MFMA).
     step_flops = info["panel_step_flops"] / max(info["panel_step_launches"], 1)
     step_ms = phases["panel_step_kernel_ms"]
     step_tflops = step_flops / (step_ms * 1e-3) / 1e12 if step_ms > 0 else 0.0
@@ -240,14 +242,15 @@ def main():
             "phase_ms": {k: round(v, 4) for k, v in phases.items()},
         },
         "roofline": {
-            "kernel": "KGemmLower + KGemmLowerAhead: trailing updates and update matrices of the "
-                      "supernodal LDL^T on FP64 MFMA",
+            "kernel": "KGemmLower + KGemmLowerAhead + KGemmUpdate: trailing updates (the look-ahead "
+                      "workgroup then factors and inverts the next diagonal block) and update matrices "
+                      "(flat lists of 64x64 tiles) of the supernodal LDL^T on FP64 MFMA",
             "bound": "mfma",
             "achieved": gemm_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": gemm_tflops / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic("KGemmLower"),
+            "traffic": measured_traffic("KGemm"),
             "flops_per_launch": gemm_flops,
             "launches_per_iteration": info["gemm_launches"],
             "avg_launch_ms": gemm_ms,
@@ -255,8 +258,9 @@ def main():
                     "the top fronts' steps form a dependent chain",
         },
         "roofline_panel_step": {
-            "kernel": "KPanelStep (+KDiagBlock): 64-column panel step of the supernodal LDL^T",
-            "bound": "fp64",
+            "kernel": "KPanelStep (+KDiagBlock): 64-column panel step of the supernodal LDL^T, "
+                      "L21 = A21 W as a 64x64x64 FP64 MFMA product per 64 rows",
+            "bound": "mfma",
             "achieved": step_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -265,8 +269,9 @@ def main():
             "flops_per_launch": step_flops,
             "launches_per_iteration": info["panel_step_launches"],
             "avg_launch_ms": step_ms,
-            "note": "latency-bound: each row of the TRSM below the diagonal block is a 64-deep "
-                    "dependent chain, over at most a few thousand rows",
+            "note": "latency-bound: at most a few thousand rows per step; flops are the TRSM's "
+                    "(rows x nb^2, plus the diagonal block's 2/3 nb^3 on the first step), "
+                    "not the product's",
         },
         "roofline_assembly": {
             "kernel": "KAssemblePoints (point-row residual + Jacobian assembly)",

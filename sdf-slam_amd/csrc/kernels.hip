@@ -464,6 +464,11 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   __shared__ double red[9][kThreads];
   const int64_t rb = A.pr_ptr[pose];
   const int m = static_cast<int>(A.pr_ptr[pose + 1] - rb);
+  // This frame's sort size: its row count rounded up to a power of two
+  // (the LDS is sized for the largest frame of the launch).
+  int ss = 1;
+  while (ss < m) ss <<= 1;
+  sort_size = min(sort_size, ss);
   const int64_t np = P.num_points;
   const uint32_t invalid = static_cast<uint32_t>(P.num_cells);
   for (int i = tid; i < sort_size; i += kThreads) {
@@ -705,9 +710,35 @@ __global__ void KOdomPairs(Slots slots) {
 
 // ---------------------------------------------------------------- factorization
 
+// dst[col + rel[a] - off] += src[a] for a = b + lane, b + lane + 64, ... < mc
+// (column b of a child's update matrix, one wavefront). In groups of four
+// per lane: the group's list and source loads are issued together, then its
+// destination loads (at indices clamped into the lists), then the sums are
+// stored for a < mc; a per-element loop would wait on two loads per element.
+__device__ __forceinline__ void ScatterAddColumn(double* dst, int64_t col, int off, const int32_t* rel,
+                                                 const double* src, int b, int lane, int mc) {
+  for (int g = b; g < mc; g += 4 * 64) {
+    int r[4];
+    double u[4], d[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int a = min(g + lane + 64 * q, mc - 1);
+      r[q] = rel[a];
+      u[q] = src[a];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) d[q] = dst[col + r[q] - off];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (g + lane + 64 * q < mc) dst[col + r[q] - off] = d[q] + u[q];
+    }
+  }
+}
+
 // Extend-add of a child's update matrix: columns [0, split) land in the
 // parent's panel, columns [split, m_c) in the parent's update matrix. One
-// wavefront per column, kWaves columns per workgroup.
+// wavefront per column, kWaves columns per workgroup; within one launch no
+// two children share a parent.
 template <class Slots, class Lists>
 __global__ void KExtendAddPanel(Slots slots, Lists plan) {
   BIND_SLOT(blockIdx.z);
@@ -723,9 +754,7 @@ __global__ void KExtendAddPanel(Slots slots, Lists plan) {
   const int fp = A.sn_f[p];
   double* fpanel = W.panel + A.panel_off[p];
   const int64_t col = static_cast<int64_t>(rel[b]) * fp;
-  for (int a = b + static_cast<int>(threadIdx.x & 63); a < mc; a += 64) {
-    fpanel[col + rel[a]] += uc[static_cast<int64_t>(b) * mc + a];
-  }
+  ScatterAddColumn(fpanel, col, 0, rel, uc + static_cast<int64_t>(b) * mc, b, static_cast<int>(threadIdx.x & 63), mc);
 }
 
 template <class Slots, class Lists>
@@ -744,9 +773,7 @@ __global__ void KExtendAddUpdate(Slots slots, Lists plan) {
   const int32_t* rel = A.rel + A.rel_off[c];
   double* up = W.upd + A.upd_off[p];
   const int64_t col = static_cast<int64_t>(rel[b] - np) * mp;
-  for (int a = b + static_cast<int>(threadIdx.x & 63); a < mc; a += 64) {
-    up[col + (rel[a] - np)] += uc[static_cast<int64_t>(b) * mc + a];
-  }
+  ScatterAddColumn(up, col, np, rel, uc + static_cast<int64_t>(b) * mc, b, static_cast<int>(threadIdx.x & 63), mc);
 }
 
 // Row stride (doubles) of the panel step's LDS tiles: two doubles of
