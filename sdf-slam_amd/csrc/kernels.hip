@@ -821,25 +821,14 @@ __device__ __forceinline__ double Recip(double d) {
   return r;
 }
 
-// Right-looking LDL^T of a 64x64 block held by one wavefront, row `lane` in
-// `row` (lower triangle and diagonal meaningful; the upper part is scratch;
-// rows and columns from nb on are an identity pad, whose steps change
-// nothing in the lower triangle, so every step runs and the 64 steps form
-// one straight-line block the compiler can interleave). Step k: every lane
-// broadcasts its unscaled c_ik = row[k] through the LDS vector `col` (kNB
-// doubles) while the pivot d = c_kk comes from lane k with v_readlane, so
-// the broadcast overlaps the reciprocal; then row[j] -= (c_ik / d) c_jk for
-// j > k. Every lane stores its multiplier in row[k] (on and above the
-// diagonal that is scratch: a per-step lane test would hold 64 masks live)
-// and the pivots go to col[kNB + k]. LDS operations of one wavefront
-// complete in order, so no barrier is needed. `col` holds 2 kNB doubles. On
-// return row[j < lane] = L(lane, j) and col[kNB + j] = D(j). Returns true
-// when a pivot is exactly zero.
-__device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int nb, double* col) {
-  (void)nb;
-  bool zero = false;
+// Steps K0 .. K0 + 15 of FactorDiagWave (compile-time indices keep `row` in
+// registers).
+template <int K0>
+__device__ __forceinline__ void FactorSteps(double (&row)[kNB], int lane, double* col, bool& zero) {
 #pragma unroll
-  for (int k = 0; k < kNB; ++k) {
+  for (int kk = 0; kk < 16; ++kk) {
+    constexpr int kBase = K0;
+    const int k = kBase + kk;
     const double c = row[k];
     col[lane] = c;
     const double d = ReadLane(c, k);
@@ -850,6 +839,30 @@ __device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int
 #pragma unroll
     for (int j = k + 1; j < kNB; ++j) row[j] = fma(-l, col[j], row[j]);
   }
+}
+
+// Right-looking LDL^T of a 64x64 block held by one wavefront, row `lane` in
+// `row` (lower triangle and diagonal meaningful; the upper part is scratch;
+// rows and columns from nb on are an identity pad, whose steps would change
+// nothing in the lower triangle: they are skipped in groups of 16, and
+// within a group the steps form one straight-line block the compiler can
+// interleave). Step k: every lane broadcasts its unscaled c_ik = row[k]
+// through the LDS vector `col` (kNB doubles) while the pivot d = c_kk comes
+// from lane k with v_readlane, so the broadcast overlaps the reciprocal;
+// then row[j] -= (c_ik / d) c_jk for j > k. Every lane stores its
+// multiplier in row[k] (on and above the diagonal that is scratch: a
+// per-step lane test would hold 64 masks live) and the pivots go to
+// col[kNB + k] (1 for the padding). LDS operations of one wavefront complete
+// in order, so no barrier is needed. `col` holds 2 kNB doubles. On return
+// row[j < lane] = L(lane, j) and col[kNB + j] = D(j). Returns true when a
+// pivot is exactly zero.
+__device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int nb, double* col) {
+  bool zero = false;
+  col[kNB + lane] = 1.0;
+  FactorSteps<0>(row, lane, col, zero);
+  if (nb > 16) FactorSteps<16>(row, lane, col, zero);
+  if (nb > 32) FactorSteps<32>(row, lane, col, zero);
+  if (nb > 48) FactorSteps<48>(row, lane, col, zero);
   return zero;
 }
 
@@ -1413,6 +1426,33 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmUpdate(Slots slots, Lists ti
 
 // ---------------------------------------------------------------- solves
 
+// acc - sum_k F(k, i) v[k] over one block column k in [k0, k1): column k
+// goes to partial sum (k - k0) % 4, in increasing k, whatever the batch of
+// loads issued together (kBatch, a multiple of 4), so the forward solve's
+// fused and split forms produce the same values.
+template <int kBatch>
+__device__ __forceinline__ double RowUpdate(const double* F, int f, int i, const double* v, int k0, int k1,
+                                            double acc0) {
+  double acc[4] = {acc0, 0.0, 0.0, 0.0};
+  int k = k0;
+  for (; k + kBatch <= k1; k += kBatch) {
+    double fv[kBatch];
+#pragma unroll
+    for (int t = 0; t < kBatch; ++t) fv[t] = F[static_cast<int64_t>(k + t) * f + i];
+#pragma unroll
+    for (int t = 0; t < kBatch; ++t) acc[t & 3] = fma(-fv[t], v[k + t], acc[t & 3]);
+  }
+  for (; k + 4 <= k1; k += 4) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = fma(-F[static_cast<int64_t>(k + t) * f + i], v[k + t], acc[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    if (k + t < k1) acc[t] = fma(-F[static_cast<int64_t>(k + t) * f + i], v[k + t], acc[t]);
+  }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
 // Forward: gather the children's update vectors, then L y = b over the front
 // (unit lower), leaving y in the own rows and the update for the parent in R.
 // With `tri` the rows below the front's own columns (i >= n) are left as
@@ -1450,27 +1490,20 @@ __global__ void KForward(Slots slots, Lists plan, int tri) {
       // four partial sums. Every lane reads v before any writes it.
       const double* Mb = DiagInv(A, W, s, b0 / kNB);
       double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+      for (int h = 0; h < kNB; h += 32) {  // 32 loads in flight (the register budget of 1024 threads)
 #pragma unroll
-      for (int j = 0; j < kNB; ++j) acc[j & 3] = fma(Mb[j * kNB + tid], v[min(b0 + j, f - 1)], acc[j & 3]);
+        for (int t = 0; t < 32; ++t) {
+          acc[t & 3] = fma(Mb[(h + t) * kNB + tid], v[min(b0 + h + t, f - 1)], acc[t & 3]);
+        }
+      }
       const double y = (acc[0] + acc[1]) + (acc[2] + acc[3]);
       if (b0 + tid < b1) v[b0 + tid] = y;
     }
     __syncthreads();
-    // Rows below the block; loads batched ahead of the ordered FMA chain.
+    // Rows below the block.
     const int rows_end = tri != 0 ? n : f;
-    for (int i = b1 + tid; i < rows_end; i += blockDim.x) {
-      double acc = v[i];
-      int k = b0;
-      for (; k + 8 <= b1; k += 8) {
-        double fv[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) fv[t] = F[static_cast<int64_t>(k + t) * f + i];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) acc = fma(-fv[t], v[k + t], acc);
-      }
-      for (; k < b1; ++k) acc = fma(-F[static_cast<int64_t>(k) * f + i], v[k], acc);
-      v[i] = acc;
-    }
+    for (int i = b1 + tid; i < rows_end; i += blockDim.x) v[i] = RowUpdate<8>(F, f, i, v, b0, b1, v[i]);
     __syncthreads();
   }
   double* out = W.vec + A.sn_idx_ptr[s];
@@ -1478,7 +1511,7 @@ __global__ void KForward(Slots slots, Lists plan, int tri) {
 }
 
 // v2 -= L21 v1 for the fronts KForward solved with `tri`: one thread per
-// row, the same ordered fma chain over k = 0 .. n-1 as the fused kernel.
+// row, block column by block column as the fused kernel (RowUpdate).
 template <class Slots, class Lists>
 __global__ void __launch_bounds__(64) KForwardRest(Slots slots,
                                                    Lists plan) {
@@ -1493,15 +1526,7 @@ __global__ void __launch_bounds__(64) KForwardRest(Slots slots,
   const double* F = W.panel + A.panel_off[s];
   double* v = W.vec + A.sn_idx_ptr[s];
   double acc = v[i];
-  int k = 0;
-  for (; k + 8 <= n; k += 8) {
-    double fv[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) fv[t] = F[static_cast<int64_t>(k + t) * f + i];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) acc = fma(-fv[t], v[k + t], acc);
-  }
-  for (; k < n; ++k) acc = fma(-F[static_cast<int64_t>(k) * f + i], v[k], acc);
+  for (int k0 = 0; k0 < n; k0 += kNB) acc = RowUpdate<32>(F, f, i, v, k0, min(n, k0 + kNB), acc);
   v[i] = acc;
 }
 
