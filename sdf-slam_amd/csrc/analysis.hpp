@@ -28,7 +28,7 @@
 namespace sdfslam {
 
 struct AnalysisOptions {
-  int margin{4};       // footprint dilation in cells
+  int margin{6};       // footprint dilation in cells
   int leaf_nodes{64};  // dissection stops at regions of at most this many nodes
   // Per-pose dilation (one entry per pose); empty = `margin` for every pose.
   std::vector<int32_t> pose_margin;

@@ -227,7 +227,7 @@ void LaunchAssemble(const SlotArgs& slots, const LaunchShape& sh, bool check_foo
 void LaunchNormalSlots(const SlotArgs& slots, const LaunchShape& sh, const LmOptions& o, hipStream_t s);
 
 /// Optional timing of the panel steps: pairs (events[2k], events[2k+1])
-/// bracket the diagonal-block and TRSM kernels of one 64-column step; pairs
+/// bracket the diagonal-block and panel-step kernels of one 64-column step; pairs
 /// of gemm_events bracket one trailing-update or update-matrix launch.
 struct StepEvents {
   hipEvent_t* events;
