@@ -430,10 +430,14 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   a.iperm.assign(static_cast<size_t>(a.n), -1);
   a.sn_first.push_back(0);
   int next = 0;
-  std::function<void(int)> emit = [&](int t) {
+  // Height of each supernode's tree node above the leaves (the symbolic
+  // factorization below runs one height at a time).
+  std::vector<int> sn_height;
+  std::function<int(int)> emit = [&](int t) -> int {
     const NdNode& node = tree[static_cast<size_t>(t)];
-    if (node.left >= 0) emit(node.left);
-    if (node.right >= 0) emit(node.right);
+    int height = 0;
+    if (node.left >= 0) height = std::max(height, emit(node.left) + 1);
+    if (node.right >= 0) height = std::max(height, emit(node.right) + 1);
     const int start = next;
     auto take_node = [&](int w, int h) {
       const int col = v.node_to_active[h * a.nx + w];
@@ -462,7 +466,9 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     }
     if (next > start) {
       a.sn_first.push_back(next);
+      sn_height.push_back(height);
     }
+    return height;
   };
   emit(0);
   if (next != a.n) {
@@ -520,40 +526,57 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   a.sn_f.assign(static_cast<size_t>(a.num_sn), 0);
   a.sn_idx_ptr.assign(static_cast<size_t>(a.num_sn) + 1, 0);
   std::vector<std::vector<int32_t>> R(static_cast<size_t>(a.num_sn));
-  std::vector<int> mark(static_cast<size_t>(a.n), -1);
-  for (int s = 0; s < a.num_sn; ++s) {
-    const int first = a.sn_first[static_cast<size_t>(s)];
-    const int last = a.sn_first[static_cast<size_t>(s) + 1] - 1;
-    std::vector<int32_t>& rs = R[static_cast<size_t>(s)];
-    auto add = [&](int i) {
-      if (i > last && mark[static_cast<size_t>(i)] != s) {
-        mark[static_cast<size_t>(i)] = s;
-        rs.push_back(i);
+  // A supernode's parent holds the first row of its front beyond it: a
+  // column of a tree ancestor, so every child lies lower in the tree and the
+  // supernodes of one height are independent. They run in parallel (a mark
+  // array per thread); the children lists are appended after each height
+  // and sorted at the end, the order the serial loop gives.
+  int max_height = 0;
+  for (const int h : sn_height) max_height = std::max(max_height, h);
+  std::vector<std::vector<int>> by_height(static_cast<size_t>(max_height) + 1);
+  for (int s = 0; s < a.num_sn; ++s) by_height[static_cast<size_t>(sn_height[static_cast<size_t>(s)])].push_back(s);
+  std::vector<std::vector<int>> marks(static_cast<size_t>(threads));
+  for (const std::vector<int>& list : by_height) {
+    const int count = static_cast<int>(list.size());
+    const int chunks = std::max(1, std::min(threads, count / 4));
+    RunChunks(chunks, [&](int k) {
+      std::vector<int>& mark = marks[static_cast<size_t>(k)];
+      if (mark.empty()) mark.assign(static_cast<size_t>(a.n), -1);
+      for (int64_t idx = Bound(count, chunks, k); idx < Bound(count, chunks, k + 1); ++idx) {
+        const int s = list[static_cast<size_t>(idx)];
+        const int first = a.sn_first[static_cast<size_t>(s)];
+        const int last = a.sn_first[static_cast<size_t>(s) + 1] - 1;
+        std::vector<int32_t>& rs = R[static_cast<size_t>(s)];
+        auto add = [&](int i) {
+          if (i > last && mark[static_cast<size_t>(i)] != s) {
+            mark[static_cast<size_t>(i)] = s;
+            rs.push_back(i);
+          }
+        };
+        for (int j = first; j <= last; ++j) {
+          const int col = a.iperm[static_cast<size_t>(j)];
+          // The three columns of a pose share one supernode and one
+          // neighbour set beyond it: visit the first only.
+          if (col >= a.num_active && (col - a.num_active) % 3 != 0) continue;
+          neighbors(col, add);
+        }
+        int lvl = 0;
+        for (const int c : a.children[static_cast<size_t>(s)]) {
+          for (const int i : R[static_cast<size_t>(c)]) add(i);
+          lvl = std::max(lvl, a.sn_level[static_cast<size_t>(c)] + 1);
+        }
+        std::sort(rs.begin(), rs.end());
+        if (!rs.empty()) a.sn_parent[static_cast<size_t>(s)] = a.snode_of[static_cast<size_t>(rs.front())];
+        a.sn_level[static_cast<size_t>(s)] = lvl;
       }
-    };
-    for (int j = first; j <= last; ++j) {
-      const int col = a.iperm[static_cast<size_t>(j)];
-      // The three columns of a pose share one supernode and one neighbour
-      // set beyond it: visit the first only.
-      if (col >= a.num_active && (col - a.num_active) % 3 != 0) continue;
-      neighbors(col, add);
+    });
+    for (const int s : list) {
+      const int p = a.sn_parent[static_cast<size_t>(s)];
+      if (p >= 0) a.children[static_cast<size_t>(p)].push_back(s);
+      a.num_levels = std::max(a.num_levels, a.sn_level[static_cast<size_t>(s)] + 1);
     }
-    for (const int c : a.children[static_cast<size_t>(s)]) {
-      for (const int i : R[static_cast<size_t>(c)]) add(i);
-    }
-    std::sort(rs.begin(), rs.end());
-    if (!rs.empty()) {
-      const int p = a.snode_of[static_cast<size_t>(rs.front())];
-      a.sn_parent[static_cast<size_t>(s)] = p;
-      a.children[static_cast<size_t>(p)].push_back(s);
-    }
-    int lvl = 0;
-    for (const int c : a.children[static_cast<size_t>(s)]) {
-      lvl = std::max(lvl, a.sn_level[static_cast<size_t>(c)] + 1);
-    }
-    a.sn_level[static_cast<size_t>(s)] = lvl;
-    a.num_levels = std::max(a.num_levels, lvl + 1);
   }
+  for (std::vector<int32_t>& ch : a.children) std::sort(ch.begin(), ch.end());
   a.panel_off.assign(static_cast<size_t>(a.num_sn), 0);
   a.upd_off.assign(static_cast<size_t>(a.num_sn), 0);
   for (int s = 0; s < a.num_sn; ++s) {
