@@ -581,6 +581,7 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
     W.od_jj = A.Alloc<double>(9 * static_cast<size_t>(v.num_odom));
     W.seg_cell = A.Alloc<int32_t>(np);
     W.seg_part = A.Alloc<double>(14 * np);
+    W.seg_jp = A.Alloc<double>(12 * np);
     W.seg_count = A.Alloc<int32_t>(static_cast<size_t>(v.num_poses));
     W.nn = A.Alloc<double>(10 * static_cast<size_t>(P.num_cells));
     W.rn = A.Alloc<double>(4 * static_cast<size_t>(P.num_cells));

@@ -541,11 +541,19 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   if (num_u == 0 && tid == 0) ustart[0] = 0;
   __syncthreads();
 
-  // This frame's partial sums per occupied cell, in the cell-sorted order.
+  // This frame's partial sums per occupied cell, in the cell-sorted order:
+  // the 14 node sums, and the 12 pose-partial x weight sums the footprint
+  // entries below add up over the node's cells.
+  const double* jp0 = W.jp;
+  const double* jp1 = W.jp + np;
+  const double* jp2 = W.jp + 2 * np;
   for (int u = tid; u < num_u; u += kThreads) {
     double acc[14];
+    double accp[12];
 #pragma unroll
     for (int k = 0; k < 14; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) accp[k] = 0.0;
     for (int t = ustart[u]; t < ustart[u + 1]; ++t) {
       const int row = rows[static_cast<uint32_t>(keys[t] & 0xffffffffULL)];
       const double w0 = W.jw[row];
@@ -553,6 +561,16 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
       const double w2 = W.jw[2 * np + row];
       const double w3 = W.jw[3 * np + row];
       const double r = W.r_pt[row];
+      const double p0 = jp0[row];
+      const double p1 = jp1[row];
+      const double p2 = jp2[row];
+      const double wc[4] = {w0, w1, w2, w3};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        accp[3 * c] += p0 * wc[c];
+        accp[3 * c + 1] += p1 * wc[c];
+        accp[3 * c + 2] += p2 * wc[c];
+      }
       acc[0] += w0 * w0;
       acc[1] += w0 * w1;
       acc[2] += w0 * w2;
@@ -571,17 +589,19 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
     W.seg_cell[rb + u] = ucell[u];
 #pragma unroll
     for (int k = 0; k < 14; ++k) W.seg_part[k * np + rb + u] = acc[k];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) W.seg_jp[k * np + rb + u] = accp[k];
   }
   if (tid == 0) W.seg_count[pose] = num_u;
   if (pose == 0) return;  // frame 0 is fixed: no pose rows
 
-  // Footprint entries: sum over the (up to) four cells around the node.
+  // Footprint entries: sum over the (up to) four cells around the node of
+  // the cell's pose-partial x corner-weight sums (this workgroup wrote them
+  // above; the barrier makes them visible).
+  __syncthreads();
   const int nx = P.grid.nx;
   const int ncx = nx - 1;
   const int ncy = P.grid.ny - 1;
-  const double* jp0 = W.jp;
-  const double* jp1 = W.jp + np;
-  const double* jp2 = W.jp + 2 * np;
   for (int64_t q = A.fp_ptr[pose] + tid; q < A.fp_ptr[pose + 1]; q += kThreads) {
     const int g = A.fp_node[q];
     const int w = g % nx;
@@ -602,14 +622,10 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
         if (ucell[mid] < cid) lo2 = mid + 1; else hi2 = mid;
       }
       if (lo2 >= num_u || ucell[lo2] != cid) continue;
-      const double* jw = W.jw + static_cast<int64_t>(corner[k]) * np;
-      for (int t = ustart[lo2]; t < ustart[lo2 + 1]; ++t) {
-        const int row = rows[static_cast<uint32_t>(keys[t] & 0xffffffffULL)];
-        const double wv = jw[row];
-        v0 += jp0[row] * wv;
-        v1 += jp1[row] * wv;
-        v2 += jp2[row] * wv;
-      }
+      const double* sp = W.seg_jp + static_cast<int64_t>(3 * corner[k]) * np + rb + lo2;
+      v0 += sp[0];
+      v1 += sp[np];
+      v2 += sp[2 * np];
     }
     const int64_t o = A.fp_off[q];
     const int64_t st = A.fp_stride[q];

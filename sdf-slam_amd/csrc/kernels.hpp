@@ -94,6 +94,7 @@ struct DeviceWork {
   double* od_jj;
   int32_t* seg_cell;   // per frame, its occupied cells (ascending) at the frame's row offset
   double* seg_part;    // 14 x num_points: per occupied cell, 10 weight products + 4 weight x residual
+  double* seg_jp;      // 12 x num_points: per occupied cell, the 3 pose partials x each of the 4 weights
   int32_t* seg_count;  // occupied cells per frame
   double* nn;  // 10 x num_cells
   double* rn;  // 4 x num_cells
