@@ -430,14 +430,10 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   a.iperm.assign(static_cast<size_t>(a.n), -1);
   a.sn_first.push_back(0);
   int next = 0;
-  // Height of each supernode's tree node above the leaves (the symbolic
-  // factorization below runs one height at a time).
-  std::vector<int> sn_height;
-  std::function<int(int)> emit = [&](int t) -> int {
+  std::function<void(int)> emit = [&](int t) {
     const NdNode& node = tree[static_cast<size_t>(t)];
-    int height = 0;
-    if (node.left >= 0) height = std::max(height, emit(node.left) + 1);
-    if (node.right >= 0) height = std::max(height, emit(node.right) + 1);
+    if (node.left >= 0) emit(node.left);
+    if (node.right >= 0) emit(node.right);
     const int start = next;
     auto take_node = [&](int w, int h) {
       const int col = v.node_to_active[h * a.nx + w];
@@ -466,9 +462,7 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     }
     if (next > start) {
       a.sn_first.push_back(next);
-      sn_height.push_back(height);
     }
-    return height;
   };
   emit(0);
   if (next != a.n) {
@@ -526,16 +520,85 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   a.sn_f.assign(static_cast<size_t>(a.num_sn), 0);
   a.sn_idx_ptr.assign(static_cast<size_t>(a.num_sn) + 1, 0);
   std::vector<std::vector<int32_t>> R(static_cast<size_t>(a.num_sn));
-  // A supernode's parent holds the first row of its front beyond it: a
-  // column of a tree ancestor, so every child lies lower in the tree and the
-  // supernodes of one height are independent. They run in parallel (a mark
-  // array per thread); the children lists are appended after each height
-  // and sorted at the end, the order the serial loop gives.
+  // A supernode's parent holds the first row of its front beyond it. The
+  // block elimination tree that this defines is the elimination tree of the
+  // quotient graph (one vertex per supernode, an edge where A has an entry
+  // between two supernodes), so it is known before the fronts are: Liu's
+  // algorithm with path compression over the quotient graph's lower
+  // adjacency. Odometry and relation partners of poses placed in disjoint
+  // dissection subtrees make this tree differ from the dissection tree, so
+  // the heights come from it. Supernodes of one height are independent and
+  // run in parallel (a mark array per thread); the children lists are
+  // appended after each height and sorted at the end, the order the serial
+  // loop gives.
+  std::vector<std::vector<int>> marks(static_cast<size_t>(threads));
+  std::vector<std::vector<int32_t>> q_upper(static_cast<size_t>(a.num_sn));
+  {
+    const int chunks = std::max(1, std::min(threads, a.num_sn / 16));
+    RunChunks(chunks, [&](int k) {
+      std::vector<int>& mark = marks[static_cast<size_t>(k)];
+      if (mark.empty()) mark.assign(static_cast<size_t>(a.n), -1);
+      for (int64_t s = Bound(a.num_sn, chunks, k); s < Bound(a.num_sn, chunks, k + 1); ++s) {
+        const int first = a.sn_first[static_cast<size_t>(s)];
+        const int last = a.sn_first[static_cast<size_t>(s) + 1] - 1;
+        std::vector<int32_t>& up = q_upper[static_cast<size_t>(s)];
+        auto add = [&](int i) {
+          if (i <= last) return;
+          const int t = a.snode_of[static_cast<size_t>(i)];
+          if (mark[static_cast<size_t>(t)] != static_cast<int>(s)) {
+            mark[static_cast<size_t>(t)] = static_cast<int>(s);
+            up.push_back(t);
+          }
+        };
+        for (int j = first; j <= last; ++j) {
+          const int col = a.iperm[static_cast<size_t>(j)];
+          if (col >= a.num_active && (col - a.num_active) % 3 != 0) continue;
+          neighbors(col, add);
+        }
+      }
+    });
+    for (std::vector<int>& mark : marks) std::fill(mark.begin(), mark.end(), -1);
+  }
+  std::vector<int> q_parent(static_cast<size_t>(a.num_sn), -1);
+  std::vector<int> sn_height(static_cast<size_t>(a.num_sn), 0);
+  {
+    std::vector<int64_t> lo_ptr(static_cast<size_t>(a.num_sn) + 1, 0);
+    for (int s = 0; s < a.num_sn; ++s) {
+      for (const int t : q_upper[static_cast<size_t>(s)]) ++lo_ptr[static_cast<size_t>(t) + 1];
+    }
+    for (int s = 0; s < a.num_sn; ++s) lo_ptr[static_cast<size_t>(s) + 1] += lo_ptr[static_cast<size_t>(s)];
+    std::vector<int32_t> lo(static_cast<size_t>(lo_ptr[static_cast<size_t>(a.num_sn)]));
+    std::vector<int64_t> fill(lo_ptr.begin(), lo_ptr.end() - 1);
+    for (int s = 0; s < a.num_sn; ++s) {
+      for (const int t : q_upper[static_cast<size_t>(s)]) lo[static_cast<size_t>(fill[static_cast<size_t>(t)]++)] = s;
+    }
+    std::vector<int> ancestor(static_cast<size_t>(a.num_sn), -1);
+    for (int s = 0; s < a.num_sn; ++s) {
+      for (int64_t e = lo_ptr[static_cast<size_t>(s)]; e < lo_ptr[static_cast<size_t>(s) + 1]; ++e) {
+        int r = lo[static_cast<size_t>(e)];
+        while (ancestor[static_cast<size_t>(r)] != -1 && ancestor[static_cast<size_t>(r)] != s) {
+          const int up = ancestor[static_cast<size_t>(r)];
+          ancestor[static_cast<size_t>(r)] = s;
+          r = up;
+        }
+        if (ancestor[static_cast<size_t>(r)] == -1) {
+          ancestor[static_cast<size_t>(r)] = s;
+          q_parent[static_cast<size_t>(r)] = s;
+        }
+      }
+    }
+    for (int s = 0; s < a.num_sn; ++s) {
+      const int p = q_parent[static_cast<size_t>(s)];
+      if (p >= 0) {
+        sn_height[static_cast<size_t>(p)] =
+            std::max(sn_height[static_cast<size_t>(p)], sn_height[static_cast<size_t>(s)] + 1);
+      }
+    }
+  }
   int max_height = 0;
   for (const int h : sn_height) max_height = std::max(max_height, h);
   std::vector<std::vector<int>> by_height(static_cast<size_t>(max_height) + 1);
   for (int s = 0; s < a.num_sn; ++s) by_height[static_cast<size_t>(sn_height[static_cast<size_t>(s)])].push_back(s);
-  std::vector<std::vector<int>> marks(static_cast<size_t>(threads));
   for (const std::vector<int>& list : by_height) {
     const int count = static_cast<int>(list.size());
     const int chunks = std::max(1, std::min(threads, count / 4));
@@ -572,6 +635,9 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     });
     for (const int s : list) {
       const int p = a.sn_parent[static_cast<size_t>(s)];
+      if (p != q_parent[static_cast<size_t>(s)]) {
+        throw std::logic_error("supernode parent differs from the quotient-graph elimination tree");
+      }
       if (p >= 0) a.children[static_cast<size_t>(p)].push_back(s);
       a.num_levels = std::max(a.num_levels, a.sn_level[static_cast<size_t>(s)] + 1);
     }
