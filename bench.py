@@ -124,9 +124,9 @@ def served_leg(S, duration, plain_duration, workers):
     path (one increment at a time) and one session alone."""
     full = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=duration, workers=workers)
     plain = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=plain_duration, workers=1,
-                               max_batch=1, pipelines=1)
+                               max_batch=1, pipelines=1, max_in_flight=1)
     alone = S.served_benchmark(sessions=1, rate_hz=10.0, duration_s=duration, workers=1,
-                               max_batch=1, pipelines=1)
+                               max_batch=1, pipelines=1, max_in_flight=1)
     r = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
     return {"sessions": 32, "rate_hz_per_session": 10.0, "duration_s": duration, "workers": workers,
             "increments_per_s": full["increments_per_second"],

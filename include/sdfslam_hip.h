@@ -275,6 +275,19 @@ int sdfslam_hip_node_normals(const double* points_xy, const double* origins_xy, 
                              int32_t k_neighbors, const sdfslam_grid* grid, int32_t method,
                              double corner_threshold, double* node_normal_xy, double* node_scale);
 
+/* Map seeding from scans, replacing InitializeFromScans
+ * (src/core/map_init.hpp:27-28, map_init.cpp:31-69; called by
+ * runner.cpp:219-221 over all scans of a batch run and runner.cpp:231-233
+ * over the first scan of an incremental run). points_xy are the scan points
+ * in world coordinates and origins_xy their sensor positions (x, y pairs,
+ * num_points of each). Every grid node receives the distance to its nearest
+ * point, sqrt(dx*dx + dy*dy); with signed_distance != 0 it is negated when
+ * the node is farther from that point's origin than the point is. Ties in
+ * distance go to the lower point index. num_points == 0 leaves map_values
+ * (nx * ny, node-major) unchanged. */
+int sdfslam_hip_init_from_scans(const double* points_xy, const double* origins_xy, int64_t num_points,
+                                const sdfslam_grid* grid, int32_t signed_distance, double* map_values);
+
 /* Runs exactly `iterations` LM iterations from the current device state,
  * carrying the LM state (lambda, previous cost) in the context and ignoring
  * max_iterations; used to time iterations. With collect_phase_times != 0

@@ -9,6 +9,8 @@
  *   sdfslam_host_solve           <- Solve, src/solvers/solver.hpp:67 (+ kHip)
  *   sdfslam_host_parse_solver_yaml <- LoadConfig's solver block, src/pipeline/config.cpp:170-190
  *   sdfslam_host_run_incremental <- Run's incremental branch, src/pipeline/runner.cpp:226-285
+ *   sdfslam_host_run             <- Run, src/pipeline/runner.cpp:196-303 (+ output writers 73-113, 172-192)
+ *   sdfslam_host_init_from_scans <- InitializeFromScans, src/core/map_init.cpp:31-69
  */
 #ifndef SDFSLAM_HOST_H_
 #define SDFSLAM_HOST_H_
@@ -133,6 +135,40 @@ int sdfslam_host_run_incremental(const char* workload, uint64_t seed,
                                  int32_t records_capacity, int32_t* num_records, double* poses_out,
                                  double* map_out);
 
+/* A whole run of a synthetic workload, replacing Run
+ * (src/pipeline/runner.cpp:196-303, declared in runner.hpp): batch (one
+ * solve from the workload's initial poses) or incremental (as
+ * sdfslam_host_run_incremental), with map seeding from the scans when
+ * init_from_scans != 0 (all scans at the initial poses for a batch run, the
+ * first scan for an incremental one; init_signed as map.init_signed) and,
+ * when output_dir is not NULL, the reference's output files there:
+ * poses_odometry.csv, poses_estimated.csv, map.txt, scan_points_global.csv,
+ * poses_ground_truth.csv, metrics.json, and snapshots/map_NNNNNN.txt +
+ * poses_NNNNNN.csv every snapshot_every increments and after the last.
+ * options NULL: the workload's solver options (max_iterations per increment
+ * for incremental runs). solve_fn as in sdfslam_host_run_incremental. */
+typedef struct sdfslam_run_options {
+  int32_t incremental;     /* 0: batch, 1: incremental */
+  int32_t increment_size;  /* frames per increment (incremental) */
+  int32_t max_frames;      /* incremental: frames used, all when <= 0 */
+  int32_t snapshot_every;  /* incremental: increments between snapshots, 0: none */
+  int32_t init_from_scans; /* map.init_from_scans */
+  int32_t init_signed;     /* map.init_signed */
+} sdfslam_run_options;
+int sdfslam_host_run(const char* workload, uint64_t seed, const sdfslam_run_options* run,
+                     const sdfslam_solver_options* options, sdfslam_solve_fn solve_fn, void* user,
+                     const char* output_dir, sdfslam_solve_result* total);
+
+/* Map seeding from scans in the sensor frame (InitializeFromScans,
+ * src/core/map_init.hpp:27-28): scan i holds the points
+ * scan_xy[2 * scan_ptr[i] .. 2 * scan_ptr[i + 1]) and is placed by pose i
+ * (x, y, theta); scans beyond num_poses are ignored. The points are placed on
+ * the host as the problem builder places them and the distances are taken on
+ * the device (sdfslam_hip_init_from_scans). map_values (nx * ny) in/out. */
+int sdfslam_host_init_from_scans(const sdfslam_grid* grid, double* map_values, const double* scan_xy,
+                                 const int64_t* scan_ptr, int32_t num_scans, const double* poses,
+                                 int32_t num_poses, int32_t signed_distance);
+
 /* ---- Datasets: the frames of a synthetic workload -------------------------
  * A dataset holds a workload's scans (sensor frame), its odometry
  * (RelativeOdometry of the dataset poses, runner.cpp:52-59), its dataset
@@ -167,6 +203,8 @@ typedef struct sdfslam_server_options {
   int32_t max_batch;
   int32_t pipelines;
   int64_t device_budget_bytes;
+  int32_t max_in_flight;  /* increments prepared or solving at once; 0: no limit,
+                             1: one increment at a time (the plain path) */
 } sdfslam_server_options;
 int sdfslam_server_create(const sdfslam_server_options* options, sdfslam_server** out);
 /* Waits for the increments in flight, then frees every session. */
@@ -200,6 +238,7 @@ typedef struct sdfslam_served_options {
   int32_t max_batch;
   int32_t pipelines;
   uint64_t seed0;
+  int32_t max_in_flight;  /* as in sdfslam_server_options */
 } sdfslam_served_options;
 typedef struct sdfslam_served_result {
   int64_t increments;
@@ -221,6 +260,11 @@ typedef struct sdfslam_served_result {
   double capture_batch_ms;   /* inside the solve: slot table, plans, graph capture */
   double analysis_batch_ms;  /* inside the solve: pattern re-analyses */
   int64_t epochs_rebuilt;    /* re-analyses inside batched solves */
+  /* Latency over every increment but each session's first (which creates
+   * the session's device context). */
+  double latency_p50_after_first_ms;
+  double latency_p99_after_first_ms;
+  double first_increment_mean_ms;
 } sdfslam_served_result;
 int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam_served_result* out);
 

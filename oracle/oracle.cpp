@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <functional>
 #include <vector>
@@ -745,6 +746,57 @@ struct oracle_matrix {
 extern "C" {
 
 double oracle_wrap_angle(double angle) { return Wrap(angle); }
+
+void oracle_init_from_scans(const sdfslam_grid* grid, double* map_values, const double* scan_xy,
+                            const int64_t* scan_ptr, int32_t num_scans, const double* poses,
+                            int32_t num_poses, int signed_distance) {
+  // map_init.cpp:34-43: the scans placed by their poses, each point with its
+  // pose's translation as origin (Pose2::Apply, pose.hpp:27-30).
+  std::vector<double> px, py, ox, oy;
+  for (int32_t i = 0; i < num_scans && i < num_poses; ++i) {
+    const double* q = poses + 3 * i;
+    const double c = std::cos(q[2]);
+    const double s = std::sin(q[2]);
+    for (int64_t k = scan_ptr[i]; k < scan_ptr[i + 1]; ++k) {
+      const double x = scan_xy[2 * k];
+      const double y = scan_xy[2 * k + 1];
+      px.push_back(c * x + (-s) * y + q[0]);
+      py.push_back(s * x + c * y + q[1]);
+      ox.push_back(q[0]);
+      oy.push_back(q[1]);
+    }
+  }
+  if (px.empty()) return;  // map_init.cpp:44-46
+  // map_init.cpp:51-68, with an exhaustive search in place of the k-d tree.
+  for (int h = 0; h < grid->ny; ++h) {
+    for (int w = 0; w < grid->nx; ++w) {
+      const double qx = grid->min_x + w * grid->dx;
+      const double qy = grid->min_y + h * grid->dy;
+      size_t index = 0;
+      double squared = std::numeric_limits<double>::max();
+      for (size_t i = 0; i < px.size(); ++i) {
+        const double d0 = qx - px[i];
+        const double d1 = qy - py[i];
+        const double d = d0 * d0 + d1 * d1;
+        if (d < squared) {
+          squared = d;
+          index = i;
+        }
+      }
+      double sign = 1.0;
+      if (signed_distance != 0) {
+        const double ax = qx - ox[index];
+        const double ay = qy - oy[index];
+        const double bx = px[index] - ox[index];
+        const double by = py[index] - oy[index];
+        const double to_node = std::sqrt(ax * ax + ay * ay);
+        const double to_surface = std::sqrt(bx * bx + by * by);
+        sign = to_node <= to_surface ? 1.0 : -1.0;
+      }
+      map_values[static_cast<size_t>(h) * grid->nx + w] = sign * std::sqrt(squared);
+    }
+  }
+}
 
 void oracle_compose(const double a[3], const double rel[3], double out[3]) {
   const double c = std::cos(a[2]);

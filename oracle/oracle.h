@@ -21,6 +21,8 @@
  *   oracle_ldlt_solve                   dense solves (numpy), residual norms, singular -> error
  *   oracle_solve (LM control)           e2e thresholds (test_e2e.cpp:63-88) and the
  *                                       trust-region monotonicity (test_e2e.cpp:168-184)
+ *   oracle_init_from_scans              tests/test_map_init.cpp known answers, plus a
+ *                                       vectorised numpy nearest-point search
  */
 #ifndef SDFSLAM_ORACLE_H_
 #define SDFSLAM_ORACLE_H_
@@ -105,6 +107,16 @@ int oracle_linear_step(const sdfslam_problem_view* problem, const double* map_va
 /* SolveLm (lm_solver.cpp:201-434): map_values/poses updated in place. */
 int oracle_solve(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
                  double* map_values, double* poses, sdfslam_solve_result* result);
+
+/* InitializeFromScans (map_init.cpp:31-69): scan i (points
+ * scan_xy[2*scan_ptr[i] .. 2*scan_ptr[i+1]), sensor frame) placed by pose i;
+ * every node gets the distance to the nearest point by exhaustive search
+ * (ties to the lower point index), negated when signed and the node is
+ * farther from that point's sensor origin than the point. No points: the
+ * map is left unchanged. */
+void oracle_init_from_scans(const sdfslam_grid* grid, double* map_values, const double* scan_xy,
+                            const int64_t* scan_ptr, int32_t num_scans, const double* poses,
+                            int32_t num_poses, int signed_distance);
 
 /* Per-phase wall time (seconds) of the last oracle_solve on this thread:
  * assembly, normal equations, factorization+solve, step+cost. */

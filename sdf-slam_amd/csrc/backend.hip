@@ -1880,6 +1880,20 @@ int sdfslam_hip_batch_solve(sdfslam_hip_context* const* contexts, int32_t count,
   });
 }
 
+int sdfslam_hip_init_from_scans(const double* points_xy, const double* origins_xy, int64_t num_points,
+                                const sdfslam_grid* grid, int32_t signed_distance, double* map_values) {
+  return Guard([&]() -> int {
+    if (grid == nullptr || map_values == nullptr || num_points < 0 ||
+        (num_points > 0 && (points_xy == nullptr || origins_xy == nullptr))) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    if (grid->nx < 2 || grid->ny < 2) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "bad grid");
+    const int st = InitFromScansOnDevice(points_xy, origins_xy, num_points, *grid, signed_distance, map_values);
+    if (st != SDFSLAM_OK) throw Failure(st, NormalsLastError());
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_hip_node_normals(const double* points_xy, const double* origins_xy, int64_t num_points,
                              int32_t k_neighbors, const sdfslam_grid* grid, int32_t method,
                              double corner_threshold, double* node_normal_xy, double* node_scale) {

@@ -254,6 +254,56 @@ __global__ void KNodeNormals(int nx, int ny, double min_x, double min_y, double 
   out_scale[node] = scale;
 }
 
+// Map seeding from scans (InitializeFromScans, map_init.cpp:31-69): every
+// node takes the distance to its nearest scan point, negated behind the
+// surface when signed (the nearest point's sensor origin decides: the node
+// is in front when it is no farther from that origin than the point).
+__global__ void KInitFromScans(int nx, int ny, double min_x, double min_y, double dx, double dy,
+                               const double* __restrict__ px, const double* __restrict__ py,
+                               const double* __restrict__ ox, const double* __restrict__ oy, PointGridDev g,
+                               const int* __restrict__ start, const int* __restrict__ sorted, int signed_distance,
+                               double* __restrict__ out) {
+  const int64_t node = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (node >= static_cast<int64_t>(nx) * ny) return;
+  const int w = static_cast<int>(node % nx);
+  const int h = static_cast<int>(node / nx);
+  const double qx = min_x + w * dx;
+  const double qy = min_y + h * dy;
+  int nearest = 0;
+  double squared = 0.0;
+  Knn(qx, qy, 1, px, py, g, start, sorted, &nearest, &squared);
+  double sign = 1.0;
+  if (signed_distance != 0) {
+    const double ax = qx - ox[nearest];
+    const double ay = qy - oy[nearest];
+    const double bx = px[nearest] - ox[nearest];
+    const double by = py[nearest] - oy[nearest];
+    const double to_node = sqrt(ax * ax + ay * ay);
+    const double to_surface = sqrt(bx * bx + by * by);
+    sign = to_node <= to_surface ? 1.0 : -1.0;
+  }
+  out[node] = sign * sqrt(squared);
+}
+
+/// Uniform grid over the points: about two points per cell over their
+/// bounding box (at most 2048 cells a side).
+PointGridDev MakePointGrid(const double* points_xy, int64_t n) {
+  double x0 = points_xy[0];
+  double y0 = points_xy[1];
+  double x1 = x0;
+  double y1 = y0;
+  for (int64_t i = 1; i < n; ++i) {
+    x0 = std::min(x0, points_xy[2 * i]);
+    x1 = std::max(x1, points_xy[2 * i]);
+    y0 = std::min(y0, points_xy[2 * i + 1]);
+    y1 = std::max(y1, points_xy[2 * i + 1]);
+  }
+  const double extent = std::max({x1 - x0, y1 - y0, 1e-6});
+  double cell = std::sqrt((x1 - x0 + 1e-6) * (y1 - y0 + 1e-6) * 2.0 / static_cast<double>(n));
+  cell = std::max(cell, extent / 2048.0);
+  return PointGridDev{x0, y0, cell, static_cast<int>((x1 - x0) / cell) + 1, static_cast<int>((y1 - y0) / cell) + 1};
+}
+
 #define HIPN(x)                                        \
   do {                                                 \
     const hipError_t e_ = (x);                         \
@@ -334,21 +384,7 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
     return SDFSLAM_ERR_INVALID_ARGUMENT;
   }
   const int kk = static_cast<int>(std::min<int64_t>(std::max(2, k), n));
-  // Grid: about two points per cell over the points' bounding box.
-  double x0 = points_xy[0];
-  double y0 = points_xy[1];
-  double x1 = x0;
-  double y1 = y0;
-  for (int64_t i = 1; i < n; ++i) {
-    x0 = std::min(x0, points_xy[2 * i]);
-    x1 = std::max(x1, points_xy[2 * i]);
-    y0 = std::min(y0, points_xy[2 * i + 1]);
-    y1 = std::max(y1, points_xy[2 * i + 1]);
-  }
-  const double extent = std::max({x1 - x0, y1 - y0, 1e-6});
-  double cell = std::sqrt((x1 - x0 + 1e-6) * (y1 - y0 + 1e-6) * 2.0 / static_cast<double>(n));
-  cell = std::max(cell, extent / 2048.0);
-  PointGridDev g{x0, y0, cell, static_cast<int>((x1 - x0) / cell) + 1, static_cast<int>((y1 - y0) / cell) + 1};
+  const PointGridDev g = MakePointGrid(points_xy, n);
   const int64_t cells = static_cast<int64_t>(g.gx) * g.gy;
   // Layout of the workspace (256-byte aligned parts).
   auto align = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
@@ -426,6 +462,78 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
     node_normal_xy[2 * v] = rx[static_cast<size_t>(v)];
     node_normal_xy[2 * v + 1] = ry[static_cast<size_t>(v)];
   }
+  return SDFSLAM_OK;
+}
+
+int InitFromScansOnDevice(const double* points_xy, const double* origins_xy, int64_t num_points,
+                          const sdfslam_grid& grid, int32_t signed_distance, double* map_values) {
+  if (num_points <= 0) return SDFSLAM_OK;  // an empty scan list leaves the map unchanged
+  const int64_t n = num_points;
+  const int64_t nodes = static_cast<int64_t>(grid.nx) * grid.ny;
+  if (n > (int64_t{1} << 30) || nodes > (int64_t{1} << 30)) {
+    t_error = "too many points or nodes";
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  const PointGridDev g = MakePointGrid(points_xy, n);
+  const int64_t cells = static_cast<int64_t>(g.gx) * g.gy;
+  auto align = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+  const size_t nd = align(static_cast<size_t>(n) * sizeof(double));
+  const size_t ni = align(static_cast<size_t>(n) * sizeof(int));
+  const size_t ci = align(static_cast<size_t>(cells + 1) * sizeof(int));
+  const size_t vd = align(static_cast<size_t>(nodes) * sizeof(double));
+  const size_t need = 4 * nd + 2 * ni + 3 * ci + vd;
+  Lease lease;
+  Workspace& ws = *lease.w;
+  if (ws.stream == nullptr) HIPN(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
+  if (need > ws.bytes) {
+    ws.dev = nullptr;
+    ws.bytes = 0;
+    const size_t want = need + need / 2;
+    HIPN(hipMalloc(&ws.dev, want));
+    ws.bytes = want;
+  }
+  char* p = ws.dev;
+  auto take = [&p](size_t b) {
+    char* q = p;
+    p += b;
+    return q;
+  };
+  double* px = reinterpret_cast<double*>(take(nd));
+  double* py = reinterpret_cast<double*>(take(nd));
+  double* ox = reinterpret_cast<double*>(take(nd));
+  double* oy = reinterpret_cast<double*>(take(nd));
+  int* cell_of = reinterpret_cast<int*>(take(ni));
+  int* sorted = reinterpret_cast<int*>(take(ni));
+  int* count = reinterpret_cast<int*>(take(ci));
+  int* start = reinterpret_cast<int*>(take(ci));
+  int* fill = reinterpret_cast<int*>(take(ci));
+  double* out = reinterpret_cast<double*>(take(vd));
+  std::vector<double> hx(static_cast<size_t>(n)), hy(static_cast<size_t>(n)), hox(static_cast<size_t>(n)),
+      hoy(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) {
+    hx[static_cast<size_t>(i)] = points_xy[2 * i];
+    hy[static_cast<size_t>(i)] = points_xy[2 * i + 1];
+    hox[static_cast<size_t>(i)] = origins_xy[2 * i];
+    hoy[static_cast<size_t>(i)] = origins_xy[2 * i + 1];
+  }
+  hipStream_t s = ws.stream;
+  const size_t nb = static_cast<size_t>(n) * sizeof(double);
+  HIPN(hipMemcpyAsync(px, hx.data(), nb, hipMemcpyHostToDevice, s));
+  HIPN(hipMemcpyAsync(py, hy.data(), nb, hipMemcpyHostToDevice, s));
+  HIPN(hipMemcpyAsync(ox, hox.data(), nb, hipMemcpyHostToDevice, s));
+  HIPN(hipMemcpyAsync(oy, hoy.data(), nb, hipMemcpyHostToDevice, s));
+  HIPN(hipMemsetAsync(count, 0, static_cast<size_t>(cells + 1) * sizeof(int), s));
+  HIPN(hipMemsetAsync(fill, 0, static_cast<size_t>(cells + 1) * sizeof(int), s));
+  const unsigned pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
+  KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, g, cell_of, count);
+  KCellScan<<<1, kScanThreads, 0, s>>>(count, static_cast<int>(cells), start);
+  KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
+  const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
+  KInitFromScans<<<vb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, ox,
+                                         oy, g, start, sorted, signed_distance, out);
+  HIPN(hipGetLastError());
+  HIPN(hipMemcpyAsync(map_values, out, static_cast<size_t>(nodes) * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPN(hipStreamSynchronize(s));
   return SDFSLAM_OK;
 }
 

@@ -15,6 +15,13 @@ namespace sdfslam {
 int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64_t num_points, int32_t k,
                         const sdfslam_grid& grid, int32_t method, double corner_threshold,
                         double* node_normal_xy, double* node_scale);
+/// Map values (nx * ny, node-major) seeded from scan points in world
+/// coordinates and their sensor origins (x, y pairs): the distance to the
+/// nearest point (ties to the lower index), negated behind the surface when
+/// signed_distance != 0 (InitializeFromScans, map_init.cpp:31-69). With no
+/// points the values are left unchanged.
+int InitFromScansOnDevice(const double* points_xy, const double* origins_xy, int64_t num_points,
+                          const sdfslam_grid& grid, int32_t signed_distance, double* map_values);
 const char* NormalsLastError();
 
 }  // namespace sdfslam

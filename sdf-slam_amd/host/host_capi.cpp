@@ -276,6 +276,64 @@ int sdfslam_host_node_normals(const double* points_xy, const double* origins_xy,
   });
 }
 
+int sdfslam_host_init_from_scans(const sdfslam_grid* grid, double* map_values, const double* scan_xy,
+                                 const int64_t* scan_ptr, int32_t num_scans, const double* poses,
+                                 int32_t num_poses, int32_t signed_distance) {
+  return Guard([&]() -> int {
+    if (grid == nullptr || map_values == nullptr || num_scans < 0 || num_poses < 0 ||
+        (num_scans > 0 && (scan_ptr == nullptr || scan_xy == nullptr)) || (num_poses > 0 && poses == nullptr)) {
+      throw std::invalid_argument("null argument");
+    }
+    GridMap map = GridMap::FromSpacing(grid->nx, grid->ny, Vec2{grid->min_x, grid->min_y}, grid->dx, grid->dy);
+    const size_t nodes = static_cast<size_t>(map.num_nodes());
+    std::copy(map_values, map_values + nodes, map.values().begin());
+    std::vector<Scan> scans(static_cast<size_t>(num_scans));
+    for (int32_t i = 0; i < num_scans; ++i) {
+      if (scan_ptr[i + 1] < scan_ptr[i]) throw std::invalid_argument("scan_ptr is not ascending");
+      for (int64_t k = scan_ptr[i]; k < scan_ptr[i + 1]; ++k) {
+        scans[static_cast<size_t>(i)].points.push_back(Vec2{scan_xy[2 * k], scan_xy[2 * k + 1]});
+      }
+    }
+    std::vector<Pose2> ps(static_cast<size_t>(num_poses));
+    for (int32_t i = 0; i < num_poses; ++i) ps[static_cast<size_t>(i)] = {poses[3 * i], poses[3 * i + 1], poses[3 * i + 2]};
+    InitializeFromScans(map, scans, ps, signed_distance != 0);
+    std::copy(map.values().begin(), map.values().end(), map_values);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_host_run(const char* workload, uint64_t seed, const sdfslam_run_options* run,
+                     const sdfslam_solver_options* options, sdfslam_solve_fn solve_fn, void* user,
+                     const char* output_dir, sdfslam_solve_result* total) {
+  return Guard([&]() -> int {
+    if (workload == nullptr || run == nullptr || total == nullptr) throw std::invalid_argument("null argument");
+    const Workload wl = MakeWorkload(workload, seed);
+    SolverOptions o = wl.solver;
+    if (options != nullptr) {
+      o = FromC(*options);
+    } else if (run->incremental != 0) {
+      o.max_iterations = wl.iterations_per_increment;
+    }
+    o.backend = SolverBackend::kHip;
+    IncrementSolveFn fn;
+    if (solve_fn != nullptr) {
+      fn = [solve_fn, user](const sdfslam_problem_view* v, const sdfslam_solver_options* so, double* m,
+                            double* p, sdfslam_solve_result* r) { return solve_fn(user, v, so, m, p, r); };
+    }
+    RunOptions ro;
+    ro.incremental = run->incremental != 0;
+    ro.increment_size = run->increment_size;
+    ro.max_frames = run->max_frames;
+    ro.snapshot_every = run->snapshot_every;
+    ro.init_from_scans = run->init_from_scans != 0;
+    ro.init_signed = run->init_signed != 0;
+    if (output_dir != nullptr) ro.output_dir = output_dir;
+    const IncrementalRun r = RunWorkload(wl, o, ro, fn);
+    ToC(r.total, total);
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_host_problem_view(const sdfslam_host_problem* problem, sdfslam_problem_view* out) {
   if (problem == nullptr || out == nullptr) {
     return Fail(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
@@ -560,6 +618,7 @@ int sdfslam_server_create(const sdfslam_server_options* options, sdfslam_server*
     o.max_batch = options->max_batch;
     o.pipelines = options->pipelines;
     o.device_budget_bytes = options->device_budget_bytes;
+    o.max_in_flight = options->max_in_flight;
     auto s = std::make_unique<sdfslam_server>();
     s->server = std::make_unique<Server>(o);
     *out = s.release();
@@ -658,12 +717,14 @@ int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam
     b.max_batch = options->max_batch;
     b.pipelines = options->pipelines;
     b.seed0 = options->seed0;
+    b.max_in_flight = options->max_in_flight;
     const ServedResult r = RunServedBenchmark(b);
     *out = {r.increments,       r.lm_iterations,    r.wall_seconds,     r.increments_per_second,
             r.latency_p50_ms,   r.latency_p99_ms,   r.latency_mean_ms,  r.build_mean_ms,
             r.solve_mean_ms,    r.device_bytes,     r.batches,          r.mean_batch,
             r.prepare_batch_ms, r.solve_batch_ms,   r.collect_batch_ms, r.capture_batch_ms,
-            r.analysis_batch_ms, r.epochs_rebuilt};
+            r.analysis_batch_ms, r.epochs_rebuilt, r.latency_p50_after_first_ms,
+            r.latency_p99_after_first_ms, r.first_increment_mean_ms};
     return SDFSLAM_OK;
   });
 }

@@ -3,6 +3,13 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 
@@ -129,8 +136,55 @@ SolveResult HipSession::Solve(Problem& problem, const SolverOptions& options) {
   return result;
 }
 
+void InitializeFromScans(GridMap& map, const std::vector<Scan>& scans, const std::vector<Pose2>& poses,
+                         bool signed_distance) {
+  std::vector<double> xy;
+  std::vector<double> oxy;
+  for (size_t i = 0; i < scans.size() && i < poses.size(); ++i) {
+    const Vec2 origin = poses[i].Translation();
+    for (const Vec2& p : scans[i].points) {
+      const Vec2 g = poses[i].Apply(p);
+      xy.push_back(g.x);
+      xy.push_back(g.y);
+      oxy.push_back(origin.x);
+      oxy.push_back(origin.y);
+    }
+  }
+  if (xy.empty()) return;
+  const sdfslam_grid g{map.nx(), map.ny(), map.min_corner().x, map.min_corner().y, map.dx(), map.dy()};
+  Rethrow(sdfslam_hip_init_from_scans(xy.data(), oxy.data(), static_cast<int64_t>(xy.size() / 2), &g,
+                                      signed_distance ? 1 : 0, map.values().data()));
+}
+
+namespace {
+
+/// One solve through a function with the signature of sdfslam_hip_solve.
+SolveResult SolveWith(const IncrementSolveFn& solve_fn, Problem& problem, const SolverOptions& solver) {
+  SolveResult step;
+  const sdfslam_problem_view view = problem.View();
+  const sdfslam_solver_options copts = ToC(solver);
+  std::vector<double> map_values;
+  std::vector<double> poses;
+  problem.GetState(map_values, poses);
+  step.cost_history.assign(static_cast<size_t>(std::max(0, solver.max_iterations)) + 1, 0.0);
+  sdfslam_solve_result cres{};
+  cres.cost_history = step.cost_history.data();
+  cres.cost_history_capacity = static_cast<int32_t>(step.cost_history.size());
+  Rethrow(solve_fn(&view, &copts, map_values.data(), poses.data(), &cres));
+  problem.SetState(map_values.data(), poses.data());
+  step.converged = cres.converged != 0;
+  step.iterations = cres.iterations;
+  step.initial_cost = cres.initial_cost;
+  step.final_cost = cres.final_cost;
+  step.duration_seconds = cres.duration_seconds;
+  step.cost_history.resize(static_cast<size_t>(std::min(cres.cost_history_length, cres.cost_history_capacity)));
+  return step;
+}
+
+}  // namespace
+
 IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, int increment_size,
-                              int max_frames, const IncrementSolveFn& solve_fn) {
+                              int max_frames, const IncrementSolveFn& solve_fn, const IncrementHooks& hooks) {
   const size_t total_frames =
       max_frames > 0 ? std::min(wl.scans.size(), static_cast<size_t>(max_frames)) : wl.scans.size();
   if (total_frames < 2) {
@@ -144,7 +198,12 @@ IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, i
   run.map = wl.map;
   run.estimated = {wl.dataset_poses[0]};
   run.total.initial_cost = -1.0;
+  if (hooks.init_from_scans) {
+    // runner.cpp:231-233: the first scan at the first dataset pose.
+    InitializeFromScans(run.map, {wl.scans[0]}, {wl.dataset_poses[0]}, hooks.init_signed);
+  }
   size_t next_frame = 1;
+  size_t increment = 0;
   while (next_frame < total_frames) {
     const size_t last = std::min(total_frames, next_frame + static_cast<size_t>(increment_size));
     for (size_t i = next_frame; i < last; ++i) {
@@ -163,28 +222,7 @@ IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, i
                     wl.relations, solve_fn ? NodeNormalsFn() : NodeNormalsFn(DeviceNodeNormals));
     rec.build_seconds = Seconds(t0);
     t0 = std::chrono::steady_clock::now();
-    SolveResult step;
-    if (solve_fn) {
-      const sdfslam_problem_view view = problem.View();
-      const sdfslam_solver_options copts = ToC(solver);
-      std::vector<double> map_values;
-      std::vector<double> poses;
-      problem.GetState(map_values, poses);
-      step.cost_history.assign(static_cast<size_t>(std::max(0, solver.max_iterations)) + 1, 0.0);
-      sdfslam_solve_result cres{};
-      cres.cost_history = step.cost_history.data();
-      cres.cost_history_capacity = static_cast<int32_t>(step.cost_history.size());
-      Rethrow(solve_fn(&view, &copts, map_values.data(), poses.data(), &cres));
-      problem.SetState(map_values.data(), poses.data());
-      step.converged = cres.converged != 0;
-      step.iterations = cres.iterations;
-      step.initial_cost = cres.initial_cost;
-      step.final_cost = cres.final_cost;
-      step.cost_history.resize(static_cast<size_t>(
-          std::min(cres.cost_history_length, cres.cost_history_capacity)));
-    } else {
-      step = session.Solve(problem, solver);
-    }
+    SolveResult step = solve_fn ? SolveWith(solve_fn, problem, solver) : session.Solve(problem, solver);
     rec.solve_seconds = Seconds(t0);
     step.duration_seconds = rec.solve_seconds;
     run.estimated = problem.poses();
@@ -203,6 +241,169 @@ IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, i
     run.total.converged = step.converged;
     run.total.cost_history.insert(run.total.cost_history.end(), step.cost_history.begin(),
                                   step.cost_history.end());
+    ++increment;
+    if (hooks.after_increment) hooks.after_increment(next_frame, increment, run.estimated, run.map);
+  }
+  return run;
+}
+
+TrajectoryMetrics Evaluate(const std::vector<Pose2>& estimated, const std::vector<Pose2>& ground_truth) {
+  if (estimated.size() != ground_truth.size() || estimated.size() < 2) {
+    throw std::runtime_error("pose count mismatch between estimate and ground truth");
+  }
+  TrajectoryMetrics m;
+  for (size_t i = 0; i < estimated.size(); ++i) {
+    m.mean_translation_error += (estimated[i].Translation() - ground_truth[i].Translation()).norm();
+    m.mean_rotation_error += std::abs(WrapAngle(estimated[i].theta - ground_truth[i].theta));
+  }
+  m.mean_translation_error /= static_cast<double>(estimated.size());
+  m.mean_rotation_error /= static_cast<double>(estimated.size());
+  const size_t pairs = estimated.size() - 1;
+  for (size_t i = 0; i < pairs; ++i) {
+    const Pose2 rel_est = estimated[i].RelativeTo(estimated[i + 1]);
+    const Pose2 rel_gt = ground_truth[i].RelativeTo(ground_truth[i + 1]);
+    m.mean_relative_translation_error += (rel_est.Translation() - rel_gt.Translation()).norm();
+    m.mean_relative_rotation_error += std::abs(WrapAngle(rel_est.theta - rel_gt.theta));
+  }
+  m.mean_relative_translation_error /= static_cast<double>(pairs);
+  m.mean_relative_rotation_error /= static_cast<double>(pairs);
+  return m;
+}
+
+namespace {
+
+std::ofstream OpenOutput(const std::string& path) {
+  std::ofstream file(path);
+  if (!file.is_open()) throw std::runtime_error("cannot write " + path);
+  return file;
+}
+
+}  // namespace
+
+void WritePoses(const std::string& path, const std::vector<Pose2>& poses) {
+  std::ofstream file = OpenOutput(path);
+  file << std::setprecision(17) << "x,y,theta\n";
+  for (const Pose2& q : poses) file << q.x << "," << q.y << "," << q.theta << "\n";
+}
+
+void WriteMap(const std::string& path, const GridMap& map) {
+  std::ofstream file = OpenOutput(path);
+  file << std::setprecision(17) << map.nx() << " " << map.ny() << " " << map.min_corner().x << " "
+       << map.min_corner().y << " " << map.dx() << " " << map.dy() << "\n";
+  for (int h = 0; h < map.ny(); ++h) {
+    for (int w = 0; w < map.nx(); ++w) {
+      file << map.values()[static_cast<size_t>(map.NodeId(w, h))] << (w + 1 < map.nx() ? " " : "\n");
+    }
+  }
+}
+
+void WriteGlobalScanPoints(const std::string& path, const std::vector<Scan>& scans,
+                           const std::vector<Pose2>& poses) {
+  std::ofstream file = OpenOutput(path);
+  file << std::setprecision(9) << "x,y\n";
+  for (size_t i = 0; i < scans.size() && i < poses.size(); ++i) {
+    for (const Vec2& p : scans[i].points) {
+      const Vec2 g = poses[i].Apply(p);
+      file << g.x << "," << g.y << "\n";
+    }
+  }
+}
+
+void WriteMetricsJson(const std::string& path, const SolveResult& result, const TrajectoryMetrics* metrics) {
+  std::ofstream file = OpenOutput(path);
+  file << std::setprecision(17) << "{\n";
+  file << "  \"converged\": " << (result.converged ? "true" : "false") << ",\n";
+  file << "  \"iterations\": " << result.iterations << ",\n";
+  file << "  \"initial_cost\": " << result.initial_cost << ",\n";
+  file << "  \"final_cost\": " << result.final_cost << ",\n";
+  file << "  \"duration_seconds\": " << result.duration_seconds;
+  if (metrics != nullptr) {
+    file << ",\n";
+    file << "  \"mean_translation_error\": " << metrics->mean_translation_error << ",\n";
+    file << "  \"mean_rotation_error\": " << metrics->mean_rotation_error << ",\n";
+    file << "  \"mean_relative_translation_error\": " << metrics->mean_relative_translation_error << ",\n";
+    file << "  \"mean_relative_rotation_error\": " << metrics->mean_relative_rotation_error << "\n";
+  } else {
+    file << "\n";
+  }
+  file << "}\n";
+}
+
+IncrementalRun RunWorkload(const Workload& wl, const SolverOptions& solver, const RunOptions& options,
+                           const IncrementSolveFn& solve_fn) {
+  if (wl.scans.size() < 2) throw std::runtime_error("dataset needs at least two scans");
+  namespace fs = std::filesystem;
+  const bool write = !options.output_dir.empty();
+  IncrementalRun run;
+  size_t frames = wl.scans.size();
+  if (!options.incremental) {
+    // runner.cpp:209-225.
+    std::vector<Pose2> initial = wl.initial.empty() ? wl.dataset_poses : wl.initial;
+    GridMap map = wl.map;
+    if (options.init_from_scans) InitializeFromScans(map, wl.scans, initial, options.init_signed);
+    Problem problem(std::move(map), initial, wl.scans, wl.odometry, wl.problem, wl.relations);
+    SolverOptions o = solver;
+    o.backend = SolverBackend::kHip;
+    run.total = solve_fn ? SolveWith(solve_fn, problem, o) : Solve(problem, o);
+    run.estimated = problem.poses();
+    run.map = problem.map();
+  } else {
+    // runner.cpp:226-285.
+    IncrementHooks hooks;
+    hooks.init_from_scans = options.init_from_scans;
+    hooks.init_signed = options.init_signed;
+    const fs::path snapshot_dir = fs::path(options.output_dir) / "snapshots";
+    if (write && options.snapshot_every > 0) {
+      fs::create_directories(snapshot_dir);
+      const size_t last_frame =
+          options.max_frames > 0 ? std::min(frames, static_cast<size_t>(options.max_frames)) : frames;
+      hooks.after_increment = [&, last_frame](size_t next_frame, size_t increment, const std::vector<Pose2>& est,
+                                              const GridMap& map) {
+        if (increment % static_cast<size_t>(options.snapshot_every) == 0 || next_frame == last_frame) {
+          // File names carry the number of frames in the snapshot.
+          char tag[32];
+          std::snprintf(tag, sizeof(tag), "%06zu", next_frame);
+          WriteMap((snapshot_dir / ("map_" + std::string(tag) + ".txt")).string(), map);
+          WritePoses((snapshot_dir / ("poses_" + std::string(tag) + ".csv")).string(), est);
+        }
+      };
+    }
+    run = RunIncremental(wl, solver, options.increment_size, options.max_frames, solve_fn, hooks);
+    frames = run.estimated.size();
+  }
+  if (!write) return run;
+  const std::vector<Scan> scans(wl.scans.begin(), wl.scans.begin() + static_cast<std::ptrdiff_t>(frames));
+  const std::vector<Pose2> dataset(wl.dataset_poses.begin(),
+                                   wl.dataset_poses.begin() + static_cast<std::ptrdiff_t>(frames));
+  const fs::path out(options.output_dir);
+  fs::create_directories(out);
+  WritePoses((out / "poses_odometry.csv").string(), dataset);
+  WritePoses((out / "poses_estimated.csv").string(), run.estimated);
+  WriteMap((out / "map.txt").string(), run.map);
+  WriteGlobalScanPoints((out / "scan_points_global.csv").string(), scans, run.estimated);
+  // runner.cpp:114-135: scan points outside the map domain are unanchored.
+  size_t outside = 0;
+  size_t total = 0;
+  GridMap::CellRef cell;
+  for (size_t i = 0; i < scans.size(); ++i) {
+    for (const Vec2& p : scans[i].points) {
+      if (!run.map.Locate(run.estimated[i].Apply(p), cell)) ++outside;
+      ++total;
+    }
+  }
+  if (outside > 0) {
+    std::cerr << "WARNING: " << outside << " of " << total << " scan points ("
+              << 100.0 * static_cast<double>(outside) / static_cast<double>(total)
+              << "%) fall outside the map domain and are unanchored (SDF 0, no gradient).\n";
+  }
+  if (!wl.ground_truth.empty()) {
+    const std::vector<Pose2> gt(wl.ground_truth.begin(),
+                                wl.ground_truth.begin() + static_cast<std::ptrdiff_t>(frames));
+    WritePoses((out / "poses_ground_truth.csv").string(), gt);
+    const TrajectoryMetrics metrics = Evaluate(run.estimated, gt);
+    WriteMetricsJson((out / "metrics.json").string(), run.total, &metrics);
+  } else {
+    WriteMetricsJson((out / "metrics.json").string(), run.total, nullptr);
   }
   return run;
 }

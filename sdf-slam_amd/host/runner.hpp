@@ -11,6 +11,7 @@
 #pragma once
 
 #include <functional>
+#include <string>
 #include <vector>
 
 #include "problem.hpp"
@@ -77,7 +78,67 @@ using IncrementSolveFn = std::function<int(const sdfslam_problem_view*, const sd
 std::vector<NodeNormal> DeviceNodeNormals(const std::vector<Vec2>& points, const std::vector<Vec2>& origins,
                                           const GridMap& map, const NormalOptions& options);
 
+/// Seeds every map node with the distance to the nearest scan point on the
+/// device (InitializeFromScans, src/core/map_init.cpp:31-69): points are the
+/// scans placed by their poses, each with its pose's translation as origin;
+/// with `signed_distance` a node farther from the nearest point's origin
+/// than the point is gets the negative distance. No scans: map unchanged.
+void InitializeFromScans(GridMap& map, const std::vector<Scan>& scans, const std::vector<Pose2>& poses,
+                         bool signed_distance);
+
+/// Optional parts of an incremental run (runner.cpp:229-233, 271-284): map
+/// seeding from the first scan before the first increment, and a callback
+/// after each increment with the frames so far, the estimate and the map.
+struct IncrementHooks {
+  bool init_from_scans{false};
+  bool init_signed{true};
+  std::function<void(size_t frames, size_t increment, const std::vector<Pose2>& estimated, const GridMap& map)>
+      after_increment;
+};
+
 IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, int increment_size,
-                              int max_frames, const IncrementSolveFn& solve_fn = nullptr);
+                              int max_frames, const IncrementSolveFn& solve_fn = nullptr,
+                              const IncrementHooks& hooks = IncrementHooks());
+
+/// Trajectory errors against ground truth (runner.cpp:145-170): mean
+/// translation and absolute wrapped rotation errors, and the same for the
+/// relative motions of consecutive frames.
+struct TrajectoryMetrics {
+  double mean_translation_error{0.0};
+  double mean_rotation_error{0.0};
+  double mean_relative_translation_error{0.0};
+  double mean_relative_rotation_error{0.0};
+};
+TrajectoryMetrics Evaluate(const std::vector<Pose2>& estimated, const std::vector<Pose2>& ground_truth);
+
+/// Output files of a run, in the reference's formats (runner.cpp:73-113,
+/// 172-192): poses as "x,y,theta" CSV and the map as a header line
+/// "nx ny min_x min_y dx dy" plus one line of values per grid row, both at
+/// 17 significant digits; global scan points as "x,y" CSV at 9 digits;
+/// metrics.json with the solve totals and, given ground truth, the metrics.
+void WritePoses(const std::string& path, const std::vector<Pose2>& poses);
+void WriteMap(const std::string& path, const GridMap& map);
+void WriteGlobalScanPoints(const std::string& path, const std::vector<Scan>& scans,
+                           const std::vector<Pose2>& poses);
+void WriteMetricsJson(const std::string& path, const SolveResult& result, const TrajectoryMetrics* metrics);
+
+/// A whole run over a workload (Run, runner.cpp:196-303): batch (one solve
+/// from the workload's initial poses) or incremental (`increment_size`
+/// frames per increment, the solver's max_iterations per increment), map
+/// seeding from the scans when asked, snapshots every `snapshot_every`
+/// increments (and after the last), and the output files in `output_dir`
+/// (none when empty): poses_odometry.csv, poses_estimated.csv, map.txt,
+/// scan_points_global.csv, poses_ground_truth.csv and metrics.json.
+struct RunOptions {
+  bool incremental{false};
+  int increment_size{1};
+  int max_frames{0};  // incremental runs: frames used (all when <= 0)
+  int snapshot_every{0};
+  bool init_from_scans{false};
+  bool init_signed{true};
+  std::string output_dir;
+};
+IncrementalRun RunWorkload(const Workload& wl, const SolverOptions& solver, const RunOptions& options,
+                           const IncrementSolveFn& solve_fn = nullptr);
 
 }  // namespace sdf_slam

@@ -47,10 +47,11 @@ Server::Server(const ServerOptions& options) : options_(options) {
     }
     options_.device_budget_bytes = static_cast<int64_t>(0.8 * static_cast<double>(total_b));
   }
-  if (options_.max_batch < 1 || options_.pipelines < 1) {
-    throw std::invalid_argument("server needs max_batch >= 1 and pipelines >= 1");
+  if (options_.max_batch < 1 || options_.pipelines < 1 || options_.max_in_flight < 0) {
+    throw std::invalid_argument("server needs max_batch >= 1, pipelines >= 1 and max_in_flight >= 0");
   }
-  for (int w = 0; w < options_.pipelines; ++w) workers_.emplace_back([this] { SchedulerLoop(); });
+  for (int w = 0; w < options_.workers; ++w) preparers_.emplace_back([this] { PrepareLoop(); });
+  for (int w = 0; w < options_.pipelines; ++w) solvers_.emplace_back([this] { SolveLoop(); });
 }
 
 Server::~Server() {
@@ -59,7 +60,14 @@ Server::~Server() {
     stop_ = true;
   }
   work_cv_.notify_all();
-  for (std::thread& t : workers_) t.join();
+  for (std::thread& t : preparers_) t.join();
+  // Solvers drain what the preparers queued before they stopped.
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    stop_solvers_ = true;
+  }
+  solve_cv_.notify_all();
+  for (std::thread& t : solvers_) t.join();
 }
 
 int Server::OpenSession(GridMap map, const ProblemOptions& problem, const SolverOptions& step,
@@ -131,105 +139,64 @@ int64_t Server::DeviceBytes() {
   return used;
 }
 
-namespace {
-
-/// Runs fn(i) for i in [0, n) on up to `threads` host threads.
-template <typename F>
-void ForEachParallel(int n, int threads, F&& fn) {
-  const int t = std::max(1, std::min(threads, n));
-  if (t <= 1) {
-    for (int i = 0; i < n; ++i) fn(i);
-    return;
+void Server::Prepare(Prepared& item, Pending& frame) {
+  Session& s = *item.session;
+  IncrementRecord& rec = item.record;
+  try {
+    s.estimated.push_back(s.estimated.back().Compose(frame.odometry));
+    s.scans.push_back(std::move(frame.scan));
+    s.odometry.push_back(frame.odometry);
+    rec.frames = static_cast<int>(s.scans.size());
+    const auto t0 = Clock::now();
+    item.problem = std::make_unique<Problem>(s.map, s.estimated, s.scans, s.odometry, s.problem);
+    rec.build_seconds = Seconds(t0, Clock::now());
+    item.ctx = s.hip.Load(*item.problem, s.step);
+  } catch (const std::exception& e) {
+    item.error = e.what();
   }
-  std::atomic<int> next{0};
-  std::vector<std::thread> pool;
-  pool.reserve(static_cast<size_t>(t) - 1);
-  auto body = [&] {
-    for (int i = next++; i < n; i = next++) fn(i);
-  };
-  for (int w = 1; w < t; ++w) pool.emplace_back(body);
-  body();
-  for (std::thread& th : pool) th.join();
 }
 
-}  // namespace
-
-void Server::RunBatch(const std::vector<Session*>& batch, std::vector<Pending>& frames,
-                      std::vector<IncrementRecord>& recs, std::vector<std::string>& errors) {
-  const int k = static_cast<int>(batch.size());
-  std::vector<std::unique_ptr<Problem>> problems(static_cast<size_t>(k));
-  std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(k), nullptr);
-  const auto t_prepare = Clock::now();
-  // Prepare: runner.cpp:240-260 for one appended frame, then load the
-  // session's context. The builds of a batch's sessions run in parallel on
-  // host cores; their node normals stay on the host, where they do not
-  // queue behind the batched solves of the other pipelines on the GPU.
-  ForEachParallel(k, options_.workers, [&](int i) {
-    Session& s = *batch[static_cast<size_t>(i)];
-    Pending& p = frames[static_cast<size_t>(i)];
-    IncrementRecord& rec = recs[static_cast<size_t>(i)];
-    try {
-      s.estimated.push_back(s.estimated.back().Compose(p.odometry));
-      s.scans.push_back(std::move(p.scan));
-      s.odometry.push_back(p.odometry);
-      rec.frames = static_cast<int>(s.scans.size());
-      const auto t0 = Clock::now();
-      problems[static_cast<size_t>(i)] =
-          std::make_unique<Problem>(s.map, s.estimated, s.scans, s.odometry, s.problem);
-      rec.build_seconds = Seconds(t0, Clock::now());
-      ctxs[static_cast<size_t>(i)] = s.hip.Load(*problems[static_cast<size_t>(i)], s.step);
-    } catch (const std::exception& e) {
-      errors[static_cast<size_t>(i)] = e.what();
-    }
-  });
-  std::vector<int> live;
-  for (int i = 0; i < k; ++i) {
-    if (errors[static_cast<size_t>(i)].empty()) live.push_back(i);
-  }
-  const double prepare_s = Seconds(t_prepare, Clock::now());
-  if (live.empty()) return;
-  // One batched solve (all sessions of a server share the step options).
-  const SolverOptions& step = batch[static_cast<size_t>(live[0])]->step;
+void Server::SolveBatch(std::vector<std::unique_ptr<Prepared>>& batch) {
+  const auto nl = static_cast<int>(batch.size());
+  // All sessions of a server share the step options.
+  const SolverOptions& step = batch[0]->session->step;
   const sdfslam_solver_options copts = ToC(step);
-  const auto nl = static_cast<int>(live.size());
-  std::vector<sdfslam_hip_context*> solve_ctx(static_cast<size_t>(nl));
+  std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(nl));
   std::vector<sdfslam_solve_result> results(static_cast<size_t>(nl));
   std::vector<std::vector<double>> hist(static_cast<size_t>(nl),
                                         std::vector<double>(static_cast<size_t>(std::max(0, step.max_iterations)) + 1));
   std::vector<int32_t> status(static_cast<size_t>(nl), SDFSLAM_OK);
   for (int j = 0; j < nl; ++j) {
-    solve_ctx[static_cast<size_t>(j)] = ctxs[static_cast<size_t>(live[static_cast<size_t>(j)])];
+    ctxs[static_cast<size_t>(j)] = batch[static_cast<size_t>(j)]->ctx;
     results[static_cast<size_t>(j)].cost_history = hist[static_cast<size_t>(j)].data();
     results[static_cast<size_t>(j)].cost_history_capacity = static_cast<int32_t>(hist[static_cast<size_t>(j)].size());
   }
   const auto t0 = Clock::now();
   sdfslam_hip_batch_info info{};
-  const int st = sdfslam_hip_batch_solve(solve_ctx.data(), nl, &copts, results.data(), status.data(), &info);
+  const int st = sdfslam_hip_batch_solve(ctxs.data(), nl, &copts, results.data(), status.data(), &info);
   const double solve_s = Seconds(t0, Clock::now());
   if (st != SDFSLAM_OK) {
     const std::string msg = std::string(sdfslam_hip_status_string(st)) + ": " + sdfslam_hip_last_error();
-    for (const int i : live) errors[static_cast<size_t>(i)] = msg;
+    for (auto& item : batch) item->error = msg;
     return;
   }
   // Collect the states back into the sessions, on this thread (its GPU
   // stream has just run the batch; other threads' streams may be busy).
   const auto t_collect = Clock::now();
   for (int j = 0; j < nl; ++j) {
-    const int i = live[static_cast<size_t>(j)];
-    Session& s = *batch[static_cast<size_t>(i)];
-    IncrementRecord& rec = recs[static_cast<size_t>(i)];
+    Prepared& item = *batch[static_cast<size_t>(j)];
+    Session& s = *item.session;
     try {
-      Problem& problem = *problems[static_cast<size_t>(i)];
-      const SolveResult r = s.hip.Collect(problem, results[static_cast<size_t>(j)],
+      const SolveResult r = s.hip.Collect(*item.problem, results[static_cast<size_t>(j)],
                                           status[static_cast<size_t>(j)], hist[static_cast<size_t>(j)]);
-      s.estimated = problem.poses();
-      s.map = problem.map();
-      rec.iterations = r.iterations;
-      rec.initial_cost = r.initial_cost;
-      rec.final_cost = r.final_cost;
-      rec.solve_seconds = solve_s;
+      s.estimated = item.problem->poses();
+      s.map = item.problem->map();
+      item.record.iterations = r.iterations;
+      item.record.initial_cost = r.initial_cost;
+      item.record.final_cost = r.final_cost;
+      item.record.solve_seconds = solve_s;
     } catch (const std::exception& e) {
-      errors[static_cast<size_t>(i)] = e.what();
+      item.error = e.what();
     }
   }
   const double collect_s = Seconds(t_collect, Clock::now());
@@ -238,11 +205,20 @@ void Server::RunBatch(const std::vector<Session*>& batch, std::vector<Pending>& 
   stats_.increments += nl;
   stats_.graph_captures += info.graph_captures;
   stats_.epochs_rebuilt += info.epochs_rebuilt;
-  stats_.prepare_seconds += prepare_s;
   stats_.solve_seconds += solve_s;
   stats_.collect_seconds += collect_s;
   stats_.capture_seconds += info.capture_seconds;
   stats_.analysis_seconds += info.analysis_seconds;
+}
+
+void Server::Finish(Prepared& item, Clock::time_point done, int64_t device_bytes) {
+  Session& s = *item.session;
+  s.records.push_back(item.record);
+  s.latency.push_back(Seconds(item.arrival, done));
+  s.errors.push_back(item.error);
+  if (device_bytes > 0) s.device_bytes = device_bytes;
+  s.busy = false;
+  --in_flight_;
 }
 
 ServerStats Server::Stats() {
@@ -250,54 +226,68 @@ ServerStats Server::Stats() {
   return stats_;
 }
 
-void Server::SchedulerLoop() {
+void Server::PrepareLoop() {
   std::unique_lock<std::mutex> lock(mu_);
   while (true) {
-    std::vector<Session*> batch;
+    Session* pick = nullptr;
     work_cv_.wait(lock, [&] {
       if (stop_) return true;
+      pick = nullptr;
+      if (options_.max_in_flight > 0 && in_flight_ >= options_.max_in_flight) return false;
+      // The idle session with the oldest waiting frame.
       for (const auto& s : sessions_) {
-        if (!s->busy && !s->pending.empty()) return true;
+        if (!s->busy && !s->pending.empty() &&
+            (pick == nullptr || s->pending.front().arrival < pick->pending.front().arrival)) {
+          pick = s.get();
+        }
       }
-      return false;
+      return pick != nullptr;
     });
     if (stop_) return;
-    // Every idle session with a waiting frame, oldest frame first.
-    for (const auto& s : sessions_) {
-      if (!s->busy && !s->pending.empty()) batch.push_back(s.get());
+    pick->busy = true;
+    ++in_flight_;
+    Pending frame = std::move(pick->pending.front());
+    pick->pending.pop_front();
+    lock.unlock();
+    auto item = std::make_unique<Prepared>();
+    item->session = pick;
+    item->arrival = frame.arrival;
+    const auto t0 = Clock::now();
+    Prepare(*item, frame);
+    {
+      std::lock_guard<std::mutex> stats_lock(stats_mu_);
+      stats_.prepare_seconds += Seconds(t0, Clock::now());
     }
-    std::sort(batch.begin(), batch.end(), [](const Session* a, const Session* b) {
-      return a->pending.front().arrival < b->pending.front().arrival;
-    });
-    // With several pipelines, split the waiting sessions between them so
-    // that one batch is prepared on the host while another runs on the GPU.
-    const int share = (static_cast<int>(batch.size()) + options_.pipelines - 1) / options_.pipelines;
-    const int take = std::max(1, std::min(options_.max_batch, share));
-    if (static_cast<int>(batch.size()) > take) batch.resize(static_cast<size_t>(take));
-    std::vector<Pending> frames;
-    std::vector<Clock::time_point> arrivals;
-    for (Session* s : batch) {
-      s->busy = true;
-      arrivals.push_back(s->pending.front().arrival);
-      frames.push_back(std::move(s->pending.front()));
-      s->pending.pop_front();
+    lock.lock();
+    if (!item->error.empty()) {
+      Finish(*item, Clock::now(), 0);
+      done_cv_.notify_all();
+      work_cv_.notify_all();
+    } else {
+      prepared_.push_back(std::move(item));
+      solve_cv_.notify_one();
+    }
+  }
+}
+
+void Server::SolveLoop() {
+  std::unique_lock<std::mutex> lock(mu_);
+  while (true) {
+    solve_cv_.wait(lock, [&] { return stop_solvers_ || !prepared_.empty(); });
+    if (prepared_.empty()) return;  // stopping, nothing left to solve
+    // Every prepared increment, oldest first (preparation order), up to max_batch.
+    std::vector<std::unique_ptr<Prepared>> batch;
+    while (!prepared_.empty() && static_cast<int>(batch.size()) < options_.max_batch) {
+      batch.push_back(std::move(prepared_.front()));
+      prepared_.pop_front();
     }
     lock.unlock();
-    std::vector<IncrementRecord> recs(batch.size());
-    std::vector<std::string> errors(batch.size());
-    RunBatch(batch, frames, recs, errors);
+    SolveBatch(batch);
     const auto done = Clock::now();
     std::vector<int64_t> bytes(batch.size());
-    for (size_t i = 0; i < batch.size(); ++i) bytes[i] = batch[i]->hip.DeviceBytes();
+    for (size_t i = 0; i < batch.size(); ++i) bytes[i] = batch[i]->session->hip.DeviceBytes();
     lock.lock();
-    for (size_t i = 0; i < batch.size(); ++i) {
-      Session& s = *batch[i];
-      s.records.push_back(recs[i]);
-      s.latency.push_back(Seconds(arrivals[i], done));
-      s.errors.push_back(errors[i]);
-      if (bytes[i] > 0) s.device_bytes = bytes[i];
-      s.busy = false;
-    }
+    for (size_t i = 0; i < batch.size(); ++i) Finish(*batch[i], done, bytes[i]);
     done_cv_.notify_all();
     // Sessions of this batch may have more frames waiting.
     work_cv_.notify_all();
@@ -345,6 +335,7 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   so.workers = o.workers;
   so.max_batch = o.max_batch;
   so.pipelines = o.pipelines;
+  so.max_in_flight = o.max_in_flight;
   Server server(so);
   for (int i = 0; i < o.sessions; ++i) {
     const Workload& wl = wls[static_cast<size_t>(i)];
@@ -365,11 +356,20 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   }
   ServedResult r;
   std::vector<double> lat;
+  std::vector<double> lat_after_first;
+  double first_sum = 0.0;
+  int64_t first_count = 0;
   double build = 0.0;
   double solve = 0.0;
   for (const auto& [sid, ticket] : tickets) {
     const IncrementRecord rec = server.Wait(sid, ticket);
     lat.push_back(1e3 * server.Latency(sid, ticket));
+    if (ticket == 1) {
+      first_sum += lat.back();
+      ++first_count;
+    } else {
+      lat_after_first.push_back(lat.back());
+    }
     build += rec.build_seconds;
     solve += rec.solve_seconds;
     r.lm_iterations += rec.iterations;
@@ -380,6 +380,9 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   r.increments_per_second = r.increments / std::max(r.wall_seconds, 1e-9);
   r.latency_p50_ms = Percentile(lat, 0.50);
   r.latency_p99_ms = Percentile(lat, 0.99);
+  r.latency_p50_after_first_ms = Percentile(lat_after_first, 0.50);
+  r.latency_p99_after_first_ms = Percentile(lat_after_first, 0.99);
+  r.first_increment_mean_ms = first_count > 0 ? first_sum / static_cast<double>(first_count) : 0.0;
   double sum = 0.0;
   for (const double v : lat) sum += v;
   r.latency_mean_ms = lat.empty() ? 0.0 : sum / static_cast<double>(lat.size());

@@ -4,16 +4,18 @@
 // and runs one incremental session per process (src/pipeline/runner.cpp:
 // 226-285). The server keeps many incremental sessions on one GPU, each with
 // its own device context (memory reused from one increment to the next).
-// Continuous batching: a scheduler takes every idle session with a waiting
-// frame (oldest first, up to max_batch), rebuilds their problems and loads
-// their contexts on host worker threads, solves them together with
-// sdfslam_hip_batch_solve (every stage of an LM iteration is one launch over
-// all of them), and reads the states back. Frames arriving meanwhile join
-// the next batch; with several pipelines (scheduler threads) one batch is
-// prepared on the host while others run on the GPU, each pipeline on its own
-// stream. Each session's increments run one at a time in
-// submission order, and a session's results do not depend on what else is
-// being served.
+// Continuous batching in two stages. Preparer threads (`workers`) take the
+// idle session with the oldest waiting frame, rebuild its problem and load
+// its context (host work only: the context's uploads run at the start of its
+// next solve) and queue it as prepared, so a frame is prepared as soon as it
+// arrives and its session is idle. Solver threads (`pipelines`) take every
+// prepared increment (oldest first, up to max_batch), solve them together
+// with sdfslam_hip_batch_solve (every stage of an LM iteration is one launch
+// over all of them) on their own stream, and read the states back. While the
+// GPU is busy, prepared increments accumulate and the next batch is larger;
+// when it is idle, an increment is solved as soon as it is prepared. Each
+// session's increments run one at a time in submission order, and a
+// session's results do not depend on what else is being served.
 //
 // Admission: a session is opened only while the device memory of all open
 // sessions plus an estimate for the new one stays within the budget.
@@ -45,10 +47,12 @@ struct ServerStats {
 };
 
 struct ServerOptions {
-  int workers{12};                 // host threads preparing a batch
+  int workers{12};                 // host threads preparing increments
   int max_batch{64};               // sessions per batched solve (1: one at a time)
-  int pipelines{4};                // batches in flight (scheduler threads)
+  int pipelines{4};                // batched solves in flight (solver threads)
   int64_t device_budget_bytes{0};  // 0: 80 % of the device's memory
+  int max_in_flight{0};            // increments prepared or solving at once (0: no limit;
+                                   // 1: one increment at a time, the plain path)
 };
 
 class Server {
@@ -103,19 +107,39 @@ class Server {
     int64_t device_bytes{0};
   };
 
-  void SchedulerLoop();
-  /// Prepares, solves (one batched solve) and collects one increment of each
-  /// session; records and errors per session.
-  void RunBatch(const std::vector<Session*>& batch, std::vector<Pending>& frames,
-                std::vector<IncrementRecord>& recs, std::vector<std::string>& errors);
+  /// One increment between its preparation and its solve.
+  struct Prepared {
+    Session* session{nullptr};
+    std::chrono::steady_clock::time_point arrival;
+    std::unique_ptr<Problem> problem;
+    sdfslam_hip_context* ctx{nullptr};
+    IncrementRecord record;
+    std::string error;
+  };
+
+  void PrepareLoop();
+  void SolveLoop();
+  /// Appends a session's next frame to its history, rebuilds its problem
+  /// and loads its context (runner.cpp:240-260 for one appended frame).
+  void Prepare(Prepared& item, Pending& frame);
+  /// One batched solve of prepared increments and the read-back of their
+  /// states; fills each item's record or error.
+  void SolveBatch(std::vector<std::unique_ptr<Prepared>>& batch);
+  /// Records a finished increment (under mu_).
+  void Finish(Prepared& item, std::chrono::steady_clock::time_point done, int64_t device_bytes);
 
   ServerOptions options_;
   std::mutex mu_;
   std::condition_variable work_cv_;
+  std::condition_variable solve_cv_;
   std::condition_variable done_cv_;
   std::vector<std::unique_ptr<Session>> sessions_;
-  std::vector<std::thread> workers_;
+  std::deque<std::unique_ptr<Prepared>> prepared_;
+  std::vector<std::thread> preparers_;
+  std::vector<std::thread> solvers_;
   bool stop_{false};
+  bool stop_solvers_{false};
+  int in_flight_{0};
   std::mutex stats_mu_;
   ServerStats stats_;
 };
@@ -131,6 +155,7 @@ struct ServedBenchmark {
   int max_batch{64};
   int pipelines{4};
   uint64_t seed0{2000};
+  int max_in_flight{0};
 };
 struct ServedResult {
   int64_t increments{0};
@@ -145,12 +170,15 @@ struct ServedResult {
   int64_t device_bytes{0};
   int64_t batches{0};
   double mean_batch{0.0};
-  double prepare_batch_ms{0.0};
+  double prepare_batch_ms{0.0};  // host preparation of a batch's increments (overlaps other solves)
   double solve_batch_ms{0.0};
   double collect_batch_ms{0.0};
   double capture_batch_ms{0.0};
   double analysis_batch_ms{0.0};
   int64_t epochs_rebuilt{0};
+  double latency_p50_after_first_ms{0.0};  // each session's first increment left out
+  double latency_p99_after_first_ms{0.0};
+  double first_increment_mean_ms{0.0};
 };
 ServedResult RunServedBenchmark(const ServedBenchmark& options);
 

@@ -107,13 +107,18 @@ class IncrementRecord(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("workers", C.c_int32), ("max_batch", C.c_int32), ("pipelines", C.c_int32),
-                ("device_budget_bytes", C.c_int64)]
+                ("device_budget_bytes", C.c_int64), ("max_in_flight", C.c_int32)]
+
+
+class RunOptions(C.Structure):
+    _fields_ = [("incremental", C.c_int32), ("increment_size", C.c_int32), ("max_frames", C.c_int32),
+                ("snapshot_every", C.c_int32), ("init_from_scans", C.c_int32), ("init_signed", C.c_int32)]
 
 
 class ServedOptions(C.Structure):
     _fields_ = [("sessions", C.c_int32), ("rate_hz", C.c_double), ("duration_s", C.c_double),
                 ("workers", C.c_int32), ("max_batch", C.c_int32), ("pipelines", C.c_int32),
-                ("seed0", C.c_uint64)]
+                ("seed0", C.c_uint64), ("max_in_flight", C.c_int32)]
 
 
 class ServedResult(C.Structure):
@@ -125,7 +130,9 @@ class ServedResult(C.Structure):
                 ("batches", C.c_int64), ("mean_batch", C.c_double),
                 ("prepare_batch_ms", C.c_double), ("solve_batch_ms", C.c_double),
                 ("collect_batch_ms", C.c_double), ("capture_batch_ms", C.c_double),
-                ("analysis_batch_ms", C.c_double), ("epochs_rebuilt", C.c_int64)]
+                ("analysis_batch_ms", C.c_double), ("epochs_rebuilt", C.c_int64),
+                ("latency_p50_after_first_ms", C.c_double), ("latency_p99_after_first_ms", C.c_double),
+                ("first_increment_mean_ms", C.c_double)]
 
 
 class BatchInfo(C.Structure):
@@ -158,7 +165,7 @@ HIP_SYMBOLS = [
     "sdfslam_hip_normal_equations", "sdfslam_hip_linear_step", "sdfslam_hip_cost",
     "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
     "sdfslam_hip_context_update", "sdfslam_hip_device_memory", "sdfslam_hip_batch_solve",
-    "sdfslam_hip_pose_margins", "sdfslam_hip_node_normals",
+    "sdfslam_hip_pose_margins", "sdfslam_hip_node_normals", "sdfslam_hip_init_from_scans",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -170,7 +177,7 @@ HOST_SYMBOLS = [
     "sdfslam_host_dataset_frame", "sdfslam_server_create", "sdfslam_server_destroy",
     "sdfslam_server_open_session", "sdfslam_server_submit", "sdfslam_server_wait",
     "sdfslam_server_state", "sdfslam_host_served_benchmark", "sdfslam_host_batch_benchmark",
-    "sdfslam_host_node_normals",
+    "sdfslam_host_node_normals", "sdfslam_host_run", "sdfslam_host_init_from_scans",
 ]
 
 
@@ -209,6 +216,7 @@ hip.sdfslam_hip_context_update.argtypes = [C.c_void_p, C.POINTER(ProblemView), _
 hip.sdfslam_hip_pose_margins.argtypes = [C.c_void_p, _i32p, C.c_int32, _i32p]
 hip.sdfslam_hip_node_normals.argtypes = [_f64p, _f64p, C.c_int64, C.c_int32, C.POINTER(Grid), C.c_int32,
                                          C.c_double, _f64p, _f64p]
+hip.sdfslam_hip_init_from_scans.argtypes = [_f64p, _f64p, C.c_int64, C.POINTER(Grid), C.c_int32, _f64p]
 hip.sdfslam_hip_batch_solve.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(SolverOptions),
                                         C.POINTER(SolveResult), _i32p, C.POINTER(BatchInfo)]
 
@@ -230,6 +238,10 @@ host.sdfslam_host_solve.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(SolverOpti
 host.sdfslam_host_run_incremental.argtypes = [
     C.c_char_p, C.c_uint64, C.POINTER(SolverOptions), C.c_int32, C.c_int32, SOLVE_FN, C.c_void_p,
     C.POINTER(SolveResult), C.POINTER(IncrementRecord), C.c_int32, _i32p, _f64p, _f64p]
+host.sdfslam_host_run.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(RunOptions), C.POINTER(SolverOptions),
+                                  SOLVE_FN, C.c_void_p, C.c_char_p, C.POINTER(SolveResult)]
+host.sdfslam_host_init_from_scans.argtypes = [C.POINTER(Grid), _f64p, _f64p, _i64p, C.c_int32, _f64p,
+                                              C.c_int32, C.c_int32]
 host.sdfslam_host_dataset_create.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(C.c_void_p)]
 host.sdfslam_host_dataset_destroy.argtypes = [C.c_void_p]
 host.sdfslam_host_dataset_destroy.restype = None
@@ -589,6 +601,57 @@ def run_incremental(name, seed, options=None, increment_size=1, max_frames=0, so
                 map=map_values)
 
 
+def run(name, seed, output_dir=None, incremental=False, increment_size=1, max_frames=0, snapshot_every=0,
+        init_from_scans=False, init_signed=True, options=None, solve_fn=None):
+    """A whole run of a synthetic workload with the reference's output files
+    (sdfslam_host_run; Run, src/pipeline/runner.cpp:196-303). solve_fn as in
+    run_incremental. Returns the accumulated SolveResult as a dict."""
+    ro = RunOptions(int(incremental), increment_size, max_frames, snapshot_every, int(init_from_scans),
+                    int(init_signed))
+    frames = Problem.workload(name, seed).view.num_poses
+    iters = options.max_iterations if options is not None else 256
+    hist = np.zeros(frames * (iters + 1) + 1)
+    total = SolveResult()
+    total.cost_history = ptr(hist, _f64p)
+    total.cost_history_capacity = len(hist)
+    cb = SOLVE_FN(0)
+    if solve_fn is not None:
+        cb = SOLVE_FN(lambda user, v, o, m, p, r: solve_fn(v, o, m, p, r))
+    _check_host(host.sdfslam_host_run(name.encode(), seed, C.byref(ro),
+                                      C.byref(options) if options is not None else None, cb, None,
+                                      output_dir.encode() if output_dir is not None else None,
+                                      C.byref(total)))
+    return _result_dict(total, hist)
+
+
+def init_from_scans(grid, map_values, scans, poses, signed_distance=True, device=True):
+    """Map seeding from scans (InitializeFromScans, src/core/map_init.cpp:31-69).
+
+    scans: list of (n_i, 2) sensor-frame points; poses: (N, 3). With device
+    True the host mirror places the points and the device takes the distances
+    (sdfslam_host_init_from_scans); with device False the points must already
+    be in world coordinates as (points (n, 2), origins (n, 2)) and the device
+    entry point is called directly (sdfslam_hip_init_from_scans). Returns the
+    new map values (nx * ny)."""
+    out = np.ascontiguousarray(map_values, dtype=np.float64).copy()
+    if device:
+        ptrs = np.zeros(len(scans) + 1, dtype=np.int64)
+        ptrs[1:] = np.cumsum([len(np.reshape(sc, (-1, 2))) for sc in scans])
+        xy = (np.ascontiguousarray(np.concatenate([np.reshape(sc, (-1, 2)) for sc in scans]), dtype=np.float64)
+              if len(scans) else np.zeros((0, 2)))
+        ps = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 3)
+        _check_host(host.sdfslam_host_init_from_scans(C.byref(grid), ptr(out, _f64p), ptr(xy, _f64p),
+                                                      ptr(ptrs, _i64p), len(scans), ptr(ps, _f64p),
+                                                      len(ps), int(signed_distance)))
+    else:
+        pts, origins = scans
+        p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 2)
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 2)
+        _check_hip(hip.sdfslam_hip_init_from_scans(ptr(p, _f64p), ptr(o, _f64p), len(p), C.byref(grid),
+                                                   int(signed_distance), ptr(out, _f64p)))
+    return out
+
+
 class Dataset:
     """Frames of a synthetic workload (sdfslam_host_dataset_*)."""
 
@@ -625,9 +688,9 @@ class Dataset:
 class Server:
     """Served form: many incremental sessions on one GPU (sdfslam_server_*)."""
 
-    def __init__(self, workers=12, device_budget_bytes=0, max_batch=64, pipelines=4):
+    def __init__(self, workers=12, device_budget_bytes=0, max_batch=64, pipelines=4, max_in_flight=0):
         h = C.c_void_p()
-        o = ServerOptions(workers, max_batch, pipelines, device_budget_bytes)
+        o = ServerOptions(workers, max_batch, pipelines, device_budget_bytes, max_in_flight)
         _check_host(host.sdfslam_server_create(C.byref(o), C.byref(h)))
         self._h = h
         self._grids = {}
@@ -679,8 +742,8 @@ class Server:
 
 
 def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=12, max_batch=64,
-                     pipelines=4, seed0=2000):
-    o = ServedOptions(sessions, rate_hz, duration_s, workers, max_batch, pipelines, seed0)
+                     pipelines=4, seed0=2000, max_in_flight=0):
+    o = ServedOptions(sessions, rate_hz, duration_s, workers, max_batch, pipelines, seed0, max_in_flight)
     r = ServedResult()
     _check_host(host.sdfslam_host_served_benchmark(C.byref(o), C.byref(r)))
     return {k: getattr(r, k) for k, _ in ServedResult._fields_}

@@ -69,6 +69,7 @@ def _setup(view_type, grid_type, solver_options_type, solve_result_type):
     lib.oracle_solve.argtypes = [V, C.POINTER(solver_options_type), _f64p, _f64p,
                                  C.POINTER(solve_result_type)]
     lib.oracle_last_phase_seconds.argtypes = [_f64p]
+    lib.oracle_init_from_scans.argtypes = [G, _f64p, _f64p, _i64p, C.c_int32, _f64p, C.c_int32, C.c_int]
 
 
 _configured = False
@@ -164,6 +165,19 @@ def linear_step(view, map_values, poses, lam, marquardt=False):
     st = lib.oracle_linear_step(C.byref(view), _p(m, _f64p), _p(p, _f64p), lam,
                                 1 if marquardt else 0, _p(step, _f64p))
     return st, step
+
+
+def init_from_scans(grid, map_values, scans, poses, signed_distance=True):
+    """InitializeFromScans restated (exhaustive nearest point); returns the new map values."""
+    out = np.ascontiguousarray(map_values, dtype=np.float64).copy()
+    ptrs = np.zeros(len(scans) + 1, dtype=np.int64)
+    ptrs[1:] = np.cumsum([len(np.reshape(sc, (-1, 2))) for sc in scans])
+    xy = (np.ascontiguousarray(np.concatenate([np.reshape(sc, (-1, 2)) for sc in scans]), dtype=np.float64)
+          if len(scans) else np.zeros((0, 2)))
+    ps = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 3)
+    lib.oracle_init_from_scans(C.byref(grid), _p(out, _f64p), _p(xy, _f64p), _p(ptrs, _i64p), len(scans),
+                               _p(ps, _f64p), len(ps), int(signed_distance))
+    return out
 
 
 def solve(mod, view, options, map_values, poses):
