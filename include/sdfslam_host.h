@@ -205,6 +205,8 @@ typedef struct sdfslam_server_options {
   int64_t device_budget_bytes;
   int32_t max_in_flight;  /* increments prepared or solving at once; 0: no limit,
                              1: one increment at a time (the plain path) */
+  int32_t device_normals; /* 1: problem rebuilds take their node normals from the
+                             device (sdfslam_hip_node_normals, same values) */
 } sdfslam_server_options;
 int sdfslam_server_create(const sdfslam_server_options* options, sdfslam_server** out);
 /* Waits for the increments in flight, then frees every session. */
@@ -239,6 +241,7 @@ typedef struct sdfslam_served_options {
   int32_t pipelines;
   uint64_t seed0;
   int32_t max_in_flight;  /* as in sdfslam_server_options */
+  int32_t device_normals; /* as in sdfslam_server_options */
 } sdfslam_served_options;
 typedef struct sdfslam_served_result {
   int64_t increments;

@@ -747,6 +747,42 @@ extern "C" {
 
 double oracle_wrap_angle(double angle) { return Wrap(angle); }
 
+void oracle_eval_points_frozen(const sdfslam_grid* grid, const double* values, const double* poses,
+                               int64_t n, const int32_t* frame, const double* px, const double* py,
+                               const double* delta, const int32_t* cell_w, const int32_t* cell_h,
+                               double* residual, double* d_nodes, double* d_pose) {
+  for (int64_t k = 0; k < n; ++k) {
+    const double* q = poses + 3 * static_cast<int64_t>(frame[k]);
+    const double c = std::cos(q[2]);
+    const double s = std::sin(q[2]);
+    const double x = px[k];
+    const double y = py[k];
+    const double gx = c * x + (-s) * y + q[0];
+    const double gy = s * x + c * y + q[1];
+    const int w = cell_w[k];
+    const int h = cell_h[k];
+    const double ox = grid->min_x + w * grid->dx;  // GridMap::NodePosition
+    const double oy = grid->min_y + h * grid->dy;
+    const double alpha = (gx - ox) / grid->dx;
+    const double beta = (gy - oy) / grid->dy;
+    const double wt[4] = {(1.0 - alpha) * (1.0 - beta), alpha * (1.0 - beta), (1.0 - alpha) * beta,
+                          alpha * beta};
+    const int64_t base = static_cast<int64_t>(h) * grid->nx + w;
+    const double v[4] = {values[base], values[base + 1], values[base + grid->nx], values[base + grid->nx + 1]};
+    double r = -delta[k];
+    for (int i = 0; i < 4; ++i) r += wt[i] * v[i];
+    residual[k] = r;
+    for (int i = 0; i < 4; ++i) d_nodes[4 * k + i] = wt[i];
+    const double sx = ((1.0 - beta) * (v[1] - v[0]) + beta * (v[3] - v[2])) / grid->dx;
+    const double sy = ((1.0 - alpha) * (v[2] - v[0]) + alpha * (v[3] - v[1])) / grid->dy;
+    const double dx_dtheta = -s * x - c * y;
+    const double dy_dtheta = c * x - s * y;
+    d_pose[3 * k] = sx;
+    d_pose[3 * k + 1] = sy;
+    d_pose[3 * k + 2] = sx * dx_dtheta + sy * dy_dtheta;
+  }
+}
+
 void oracle_init_from_scans(const sdfslam_grid* grid, double* map_values, const double* scan_xy,
                             const int64_t* scan_ptr, int32_t num_scans, const double* poses,
                             int32_t num_poses, int signed_distance) {

@@ -21,6 +21,8 @@
  *   oracle_ldlt_solve                   dense solves (numpy), residual norms, singular -> error
  *   oracle_solve (LM control)           e2e thresholds (test_e2e.cpp:63-88) and the
  *                                       trust-region monotonicity (test_e2e.cpp:168-184)
+ *   oracle_eval_points_frozen           equals oracle_eval_point inside the cell
+ *                                       (test_jacobians.cpp:78-102) + central FD off-cell
  *   oracle_init_from_scans              tests/test_map_init.cpp known answers, plus a
  *                                       vectorised numpy nearest-point search
  */
@@ -107,6 +109,17 @@ int oracle_linear_step(const sdfslam_problem_view* problem, const double* map_va
 /* SolveLm (lm_solver.cpp:201-434): map_values/poses updated in place. */
 int oracle_solve(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
                  double* map_values, double* poses, sdfslam_solve_result* result);
+
+/* EvalPointResidualFrozenCell (residual_math.hpp:113-147) for n point rows:
+ * row k evaluates point (px[k], py[k]) of frame frame[k] (pose
+ * poses[3*frame[k]..]) against the cell (cell_w[k], cell_h[k]) with the map's
+ * values at that cell's four nodes: alpha, beta from the cell's corner node,
+ * so a point outside the cell extrapolates its bilinear patch. Outputs per
+ * row: residual, d_nodes[4k..4k+3], d_pose[3k..3k+2]. */
+void oracle_eval_points_frozen(const sdfslam_grid* grid, const double* values, const double* poses,
+                               int64_t n, const int32_t* frame, const double* px, const double* py,
+                               const double* delta, const int32_t* cell_w, const int32_t* cell_h,
+                               double* residual, double* d_nodes, double* d_pose);
 
 /* InitializeFromScans (map_init.cpp:31-69): scan i (points
  * scan_xy[2*scan_ptr[i] .. 2*scan_ptr[i+1]), sensor frame) placed by pose i;

@@ -619,6 +619,7 @@ int sdfslam_server_create(const sdfslam_server_options* options, sdfslam_server*
     o.pipelines = options->pipelines;
     o.device_budget_bytes = options->device_budget_bytes;
     o.max_in_flight = options->max_in_flight;
+    o.device_normals = options->device_normals != 0;
     auto s = std::make_unique<sdfslam_server>();
     s->server = std::make_unique<Server>(o);
     *out = s.release();
@@ -718,6 +719,7 @@ int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam
     b.pipelines = options->pipelines;
     b.seed0 = options->seed0;
     b.max_in_flight = options->max_in_flight;
+    b.device_normals = options->device_normals != 0;
     const ServedResult r = RunServedBenchmark(b);
     *out = {r.increments,       r.lm_iterations,    r.wall_seconds,     r.increments_per_second,
             r.latency_p50_ms,   r.latency_p99_ms,   r.latency_mean_ms,  r.build_mean_ms,

@@ -148,7 +148,9 @@ void Server::Prepare(Prepared& item, Pending& frame) {
     s.odometry.push_back(frame.odometry);
     rec.frames = static_cast<int>(s.scans.size());
     const auto t0 = Clock::now();
-    item.problem = std::make_unique<Problem>(s.map, s.estimated, s.scans, s.odometry, s.problem);
+    item.problem = std::make_unique<Problem>(
+        s.map, s.estimated, s.scans, s.odometry, s.problem, std::vector<RelationMeasurement>(),
+        options_.device_normals ? NodeNormalsFn(DeviceNodeNormals) : NodeNormalsFn());
     rec.build_seconds = Seconds(t0, Clock::now());
     item.ctx = s.hip.Load(*item.problem, s.step);
   } catch (const std::exception& e) {
@@ -336,6 +338,7 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   so.max_batch = o.max_batch;
   so.pipelines = o.pipelines;
   so.max_in_flight = o.max_in_flight;
+  so.device_normals = o.device_normals;
   Server server(so);
   for (int i = 0; i < o.sessions; ++i) {
     const Workload& wl = wls[static_cast<size_t>(i)];

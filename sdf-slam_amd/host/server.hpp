@@ -53,6 +53,7 @@ struct ServerOptions {
   int64_t device_budget_bytes{0};  // 0: 80 % of the device's memory
   int max_in_flight{0};            // increments prepared or solving at once (0: no limit;
                                    // 1: one increment at a time, the plain path)
+  bool device_normals{false};      // rebuilds take node normals from the device (same values)
 };
 
 class Server {
@@ -156,6 +157,7 @@ struct ServedBenchmark {
   int pipelines{4};
   uint64_t seed0{2000};
   int max_in_flight{0};
+  bool device_normals{false};
 };
 struct ServedResult {
   int64_t increments{0};

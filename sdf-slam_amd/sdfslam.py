@@ -107,7 +107,8 @@ class IncrementRecord(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("workers", C.c_int32), ("max_batch", C.c_int32), ("pipelines", C.c_int32),
-                ("device_budget_bytes", C.c_int64), ("max_in_flight", C.c_int32)]
+                ("device_budget_bytes", C.c_int64), ("max_in_flight", C.c_int32),
+                ("device_normals", C.c_int32)]
 
 
 class RunOptions(C.Structure):
@@ -118,7 +119,8 @@ class RunOptions(C.Structure):
 class ServedOptions(C.Structure):
     _fields_ = [("sessions", C.c_int32), ("rate_hz", C.c_double), ("duration_s", C.c_double),
                 ("workers", C.c_int32), ("max_batch", C.c_int32), ("pipelines", C.c_int32),
-                ("seed0", C.c_uint64), ("max_in_flight", C.c_int32)]
+                ("seed0", C.c_uint64), ("max_in_flight", C.c_int32),
+                ("device_normals", C.c_int32)]
 
 
 class ServedResult(C.Structure):
@@ -688,9 +690,11 @@ class Dataset:
 class Server:
     """Served form: many incremental sessions on one GPU (sdfslam_server_*)."""
 
-    def __init__(self, workers=12, device_budget_bytes=0, max_batch=64, pipelines=4, max_in_flight=0):
+    def __init__(self, workers=12, device_budget_bytes=0, max_batch=64, pipelines=4, max_in_flight=0,
+                 device_normals=False):
         h = C.c_void_p()
-        o = ServerOptions(workers, max_batch, pipelines, device_budget_bytes, max_in_flight)
+        o = ServerOptions(workers, max_batch, pipelines, device_budget_bytes, max_in_flight,
+                          int(device_normals))
         _check_host(host.sdfslam_server_create(C.byref(o), C.byref(h)))
         self._h = h
         self._grids = {}
@@ -742,8 +746,9 @@ class Server:
 
 
 def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=12, max_batch=64,
-                     pipelines=4, seed0=2000, max_in_flight=0):
-    o = ServedOptions(sessions, rate_hz, duration_s, workers, max_batch, pipelines, seed0, max_in_flight)
+                     pipelines=4, seed0=2000, max_in_flight=0, device_normals=False):
+    o = ServedOptions(sessions, rate_hz, duration_s, workers, max_batch, pipelines, seed0, max_in_flight,
+                      int(device_normals))
     r = ServedResult()
     _check_host(host.sdfslam_host_served_benchmark(C.byref(o), C.byref(r)))
     return {k: getattr(r, k) for k, _ in ServedResult._fields_}

@@ -69,6 +69,8 @@ def _setup(view_type, grid_type, solver_options_type, solve_result_type):
     lib.oracle_solve.argtypes = [V, C.POINTER(solver_options_type), _f64p, _f64p,
                                  C.POINTER(solve_result_type)]
     lib.oracle_last_phase_seconds.argtypes = [_f64p]
+    lib.oracle_eval_points_frozen.argtypes = [G, _f64p, _f64p, C.c_int64, _i32p, _f64p, _f64p, _f64p,
+                                              _i32p, _i32p, _f64p, _f64p, _f64p]
     lib.oracle_init_from_scans.argtypes = [G, _f64p, _f64p, _i64p, C.c_int32, _f64p, C.c_int32, C.c_int]
 
 
@@ -165,6 +167,21 @@ def linear_step(view, map_values, poses, lam, marquardt=False):
     st = lib.oracle_linear_step(C.byref(view), _p(m, _f64p), _p(p, _f64p), lam,
                                 1 if marquardt else 0, _p(step, _f64p))
     return st, step
+
+
+def eval_points_frozen(grid, values, poses, frame, px, py, delta, cell_w, cell_h):
+    """EvalPointResidualFrozenCell over arrays of rows; returns (residual (n,),
+    d_nodes (n, 4), d_pose (n, 3))."""
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    frame, cell_w, cell_h = i32(frame), i32(cell_w), i32(cell_h)
+    px, py, delta, v, ps = f64(px), f64(py), f64(delta), f64(values), f64(poses)
+    n = len(px)
+    r, dn, dp = np.zeros(n), np.zeros((n, 4)), np.zeros((n, 3))
+    lib.oracle_eval_points_frozen(C.byref(grid), _p(v, _f64p), _p(ps, _f64p), n, _p(frame, _i32p),
+                                  _p(px, _f64p), _p(py, _f64p), _p(delta, _f64p), _p(cell_w, _i32p),
+                                  _p(cell_h, _i32p), _p(r, _f64p), _p(dn, _f64p), _p(dp, _f64p))
+    return r, dn, dp
 
 
 def init_from_scans(grid, map_values, scans, poses, signed_distance=True):

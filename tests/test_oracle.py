@@ -324,3 +324,58 @@ def test_oracle_trust_region_is_monotone(sdfslam, oracle):
     res, _, _ = _recovered(sdfslam, oracle, prob, opts, GOLD["e2e_thresholds"])
     h = res["cost_history"]
     assert np.all(h[1:] <= h[:-1] + 1e-9)
+
+
+# ------------------------------------------------ frozen-cell point residual
+
+
+@pytest.mark.parametrize("seed", [7, 11])
+def test_frozen_cell_matches_regular_inside_cell(sdfslam, oracle, seed):
+    # test_jacobians.cpp:78-102: inside its build-time cell the frozen-cell
+    # evaluation equals the regular one to 1e-12.
+    rng = np.random.default_rng(seed)
+    checked = 0
+    for _ in range(30):
+        gr, vals = random_map(sdfslam, rng)
+        pose = np.array([rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(-1.5, 1.5)])
+        px, py, delta = rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1)
+        valid, r, ids, dn, dp = oracle.eval_point(gr, vals, pose, px, py, delta, smooth=0)
+        if not valid:
+            continue
+        w, h = ids[0] % gr.nx, ids[0] // gr.nx
+        fr, fdn, fdp = oracle.eval_points_frozen(gr, vals, pose, [0], [px], [py], [delta], [w], [h])
+        assert abs(fr[0] - r) <= 1e-12
+        assert np.abs(fdn[0] - dn).max() <= 1e-12 and np.abs(fdp[0] - dp).max() <= 1e-12
+        checked += 1
+    assert checked >= 10
+
+
+def test_frozen_cell_pose_jacobian_matches_fd_off_cell(sdfslam, oracle):
+    # A point that left its cell extrapolates the cell's bilinear patch, which
+    # is smooth in the pose: central differences pin d_pose everywhere.
+    rng = np.random.default_rng(3)
+    eps = GOLD["jacobian_fd"]["eps"]
+    tol = GOLD["jacobian_fd"]["tol"]
+    for _ in range(10):
+        gr, vals = random_map(sdfslam, rng)
+        pose = np.array([rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(-1.5, 1.5)])
+        px, py, delta = rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1)
+        w, h = int(rng.integers(0, gr.nx - 1)), int(rng.integers(0, gr.ny - 1))
+        _, _, dp = oracle.eval_points_frozen(gr, vals, pose, [0], [px], [py], [delta], [w], [h])
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = eps
+            rp = oracle.eval_points_frozen(gr, vals, pose + e, [0], [px], [py], [delta], [w], [h])[0][0]
+            rm = oracle.eval_points_frozen(gr, vals, pose - e, [0], [px], [py], [delta], [w], [h])[0][0]
+            assert abs(dp[0, k] - (rp - rm) / (2 * eps)) <= tol
+
+
+@pytest.mark.parametrize("name", ["room_small", "office_small"])
+@pytest.mark.parametrize("smooth", [0, 1])
+def test_oracle_point_rows_match_frozen_cell(sdfslam, oracle, name, smooth):
+    # A second pin of the assembly: the oracle's point rows equal the
+    # frozen-cell evaluation at each row's own cell.
+    from frozen_check import check_point_rows, oracle_rows, seeded_problem
+    prob, g, m, p = seeded_problem(sdfslam, name, smooth)
+    r, cols, vals = oracle_rows(oracle, prob, m, p)
+    assert check_point_rows(oracle, prob, g, m, p, r, cols, vals, smooth) > 0.5 * prob.view.num_points
