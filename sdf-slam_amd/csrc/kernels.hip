@@ -838,22 +838,36 @@ __device__ __forceinline__ double Recip(double d) {
 }
 
 // Steps K0 .. K0 + 15 of FactorDiagWave (compile-time indices keep `row` in
-// registers).
+// registers). On entry c_next = this lane's unscaled c_ik of the first step,
+// already broadcast in `col`, d_next its pivot and r_next = Recip(d_next).
 template <int K0>
-__device__ __forceinline__ void FactorSteps(double (&row)[kNB], int lane, double* col, bool& zero) {
+__device__ __forceinline__ void FactorSteps(double (&row)[kNB], int lane, double* col, bool& zero,
+                                            double& c_next, double& d_next, double& r_next) {
 #pragma unroll
   for (int kk = 0; kk < 16; ++kk) {
     constexpr int kBase = K0;
     const int k = kBase + kk;
-    const double c = row[k];
-    col[lane] = c;
-    const double d = ReadLane(c, k);
+    const double c = c_next;
+    const double d = d_next;
     zero = zero || d == 0.0;
     col[kNB + k] = d;
-    const double l = c * Recip(d);
+    const double l = c * r_next;
     row[k] = l;
+    double cj[kNB];
 #pragma unroll
-    for (int j = k + 1; j < kNB; ++j) row[j] = fma(-l, col[j], row[j]);
+    for (int j = k + 1; j < kNB; ++j) cj[j] = col[j];
+    if (k + 1 < kNB) {
+      // Look-ahead: the next step's column first, its broadcast (after this
+      // step's reads in program order) and its pivot's reciprocal issued
+      // before the rest of this step's update.
+      row[k + 1] = fma(-l, cj[k + 1], row[k + 1]);
+      c_next = row[k + 1];
+      col[lane] = c_next;
+      d_next = ReadLane(c_next, k + 1);
+      r_next = Recip(d_next);
+    }
+#pragma unroll
+    for (int j = k + 2; j < kNB; ++j) row[j] = fma(-l, cj[j], row[j]);
   }
 }
 
@@ -865,7 +879,10 @@ __device__ __forceinline__ void FactorSteps(double (&row)[kNB], int lane, double
 // interleave). Step k: every lane broadcasts its unscaled c_ik = row[k]
 // through the LDS vector `col` (kNB doubles) while the pivot d = c_kk comes
 // from lane k with v_readlane, so the broadcast overlaps the reciprocal;
-// then row[j] -= (c_ik / d) c_jk for j > k. Every lane stores its
+// then row[j] -= (c_ik / d) c_jk for j > k. Step k + 1's column is updated
+// first and its broadcast, pivot and reciprocal are issued before the rest
+// of step k's update, so the LDS round trip and the reciprocal of one step
+// overlap the row update of the previous one. Every lane stores its
 // multiplier in row[k] (on and above the diagonal that is scratch: a
 // per-step lane test would hold 64 masks live) and the pivots go to
 // col[kNB + k] (1 for the padding). LDS operations of one wavefront complete
@@ -875,10 +892,14 @@ __device__ __forceinline__ void FactorSteps(double (&row)[kNB], int lane, double
 __device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int nb, double* col) {
   bool zero = false;
   col[kNB + lane] = 1.0;
-  FactorSteps<0>(row, lane, col, zero);
-  if (nb > 16) FactorSteps<16>(row, lane, col, zero);
-  if (nb > 32) FactorSteps<32>(row, lane, col, zero);
-  if (nb > 48) FactorSteps<48>(row, lane, col, zero);
+  double c_next = row[0];
+  col[lane] = c_next;
+  double d_next = ReadLane(c_next, 0);
+  double r_next = Recip(d_next);
+  FactorSteps<0>(row, lane, col, zero, c_next, d_next, r_next);
+  if (nb > 16) FactorSteps<16>(row, lane, col, zero, c_next, d_next, r_next);
+  if (nb > 32) FactorSteps<32>(row, lane, col, zero, c_next, d_next, r_next);
+  if (nb > 48) FactorSteps<48>(row, lane, col, zero, c_next, d_next, r_next);
   return zero;
 }
 
