@@ -78,6 +78,28 @@ def measured_traffic(kernel):
     return None
 
 
+def profile_avg_launch_us(kernels):
+    """Average kernel duration (us) of the kernels' launches in an LM iteration
+    from the committed rocprofv3 summary (profiles/office200_kernel_stats.md,
+    per-iteration table), or None when a kernel has no row. The HIP-event
+    figures of this run bracket each launch and so also hold its dispatch."""
+    path = os.path.join(ROOT, "profiles", "office200_kernel_stats.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if "## Per LM iteration" not in text:
+        return None
+    rows = {}
+    for line in text.split("## Per LM iteration", 1)[1].splitlines():
+        parts = [x.strip() for x in line.strip().strip("|").split("|")]
+        if len(parts) == 5 and parts[0] in kernels:
+            rows[parts[0]] = (float(parts[1]), float(parts[2]))
+    if set(rows) != set(kernels):
+        return None
+    launches = sum(r[0] for r in rows.values())
+    return sum(r[1] for r in rows.values()) / launches if launches > 0 else None
+
+
 def cpu_baseline(problem, iterations):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import sdfslam as S
@@ -262,6 +284,7 @@ def main():
             "flops_per_launch": gemm_flops,
             "launches_per_iteration": info["gemm_launches"],
             "avg_launch_ms": gemm_ms,
+            "profile_avg_launch_us": profile_avg_launch_us(["KGemmLowerAhead", "KGemmUpdate", "KGemmLower"]),
             "note": "latency-bound: a level's launches are short chains of 64x64 tiles with K <= 64; "
                     "the top fronts' steps form a dependent chain",
         },
@@ -277,6 +300,7 @@ def main():
             "flops_per_launch": step_flops,
             "launches_per_iteration": info["panel_step_launches"],
             "avg_launch_ms": step_ms,
+            "profile_avg_launch_us": profile_avg_launch_us(["KPanelStep"]),
             "note": "latency-bound: at most a few thousand rows per step; flops are the TRSM's "
                     "(rows x nb^2, plus the diagonal block's 2/3 nb^3 on the first step), "
                     "not the product's",
@@ -291,6 +315,7 @@ def main():
             "traffic": measured_traffic("KAssemblePoints"),
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": kernel_ms,
+            "profile_avg_launch_us": profile_avg_launch_us(["KAssemblePoints"]),
         },
         "cpu_baseline": None,
     }
