@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -1538,9 +1539,12 @@ __global__ void KForward(Slots slots, Lists plan, int tri) {
       if (b0 + tid < b1) v[b0 + tid] = y;
     }
     __syncthreads();
-    // Rows below the block.
+    // Rows below the block. The load batch does not change the sums; the
+    // slot-table form (which keeps more of its slot in registers) takes
+    // half of it to stay within the 1024-thread register budget.
+    constexpr int kBatch = std::is_same<Slots, OneSlot>::value ? 16 : 8;
     const int rows_end = tri != 0 ? n : f;
-    for (int i = b1 + tid; i < rows_end; i += blockDim.x) v[i] = RowUpdate<8>(F, f, i, v, b0, b1, v[i]);
+    for (int i = b1 + tid; i < rows_end; i += blockDim.x) v[i] = RowUpdate<kBatch>(F, f, i, v, b0, b1, v[i]);
     __syncthreads();
   }
   double* out = W.vec + A.sn_idx_ptr[s];
@@ -1645,20 +1649,41 @@ __global__ void KBackward(Slots slots, Lists plan, int tri) {
     // The factorization's block columns, last first.
     const int b0 = kb * kNB;
     const int b1 = min(n, b0 + kNB);
-    for (int j = b0 + wave; j < b1; j += waves) {
-      const double* col = F + static_cast<int64_t>(j) * f;
-      double acc = 0.0;
+    // Columns j0 and j0 + waves of this wave side by side (their loads in
+    // flight together; two keep the 1024-thread kernel within its 128
+    // registers), each summed in the same order as on its own; a column past
+    // the block reads column b1 - 1 and is discarded.
+    constexpr int kCols = 2;
+    for (int j0 = b0 + wave; j0 < b1; j0 += kCols * waves) {
+      const double* col[kCols];
+#pragma unroll
+      for (int q = 0; q < kCols; ++q) col[q] = F + static_cast<int64_t>(min(j0 + q * waves, b1 - 1)) * f;
+      double acc[kCols];
+#pragma unroll
+      for (int q = 0; q < kCols; ++q) acc[q] = 0.0;
       int i = b1 + lane;
       for (; i + 192 < n; i += 256) {
-        const double f0 = col[i], f1 = col[i + 64], f2 = col[i + 128], f3 = col[i + 192];
-        acc = fma(f0, t[i], acc);
-        acc = fma(f1, t[i + 64], acc);
-        acc = fma(f2, t[i + 128], acc);
-        acc = fma(f3, t[i + 192], acc);
+        double fv[kCols][4];
+#pragma unroll
+        for (int q = 0; q < kCols; ++q) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) fv[q][u] = col[q][i + 64 * u];
+        }
+#pragma unroll
+        for (int q = 0; q < kCols; ++q) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[q] = fma(fv[q][u], t[i + 64 * u], acc[q]);
+        }
       }
-      for (; i < n; i += 64) acc = fma(col[i], t[i], acc);
-      acc = WaveSum(acc);
-      if (lane == 0) t[j] -= acc;
+      for (; i < n; i += 64) {
+#pragma unroll
+        for (int q = 0; q < kCols; ++q) acc[q] = fma(col[q][i], t[i], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < kCols; ++q) {
+        const double sum = WaveSum(acc[q]);
+        if (lane == 0 && j0 + q * waves < b1) t[j0 + q * waves] -= sum;
+      }
     }
     __syncthreads();
     if (tid < 64) {
