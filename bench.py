@@ -166,17 +166,24 @@ def served_leg(S, duration, plain_duration, workers):
             "saturated_30hz": r(sat)}
 
 
-def batched_leg(S, max_batch=8, workers=8):
+def batched_leg(S):
     """64 uneven batch instances, 30 LM iterations each: aggregate LM
-    iterations/s with batched solves of up to `max_batch` similar-sized
-    instances on `workers` concurrent streams, against one solve at a time."""
+    iterations/s with batched solves of similar-sized instances on concurrent
+    streams, the shape (instances per batched solve, solves in flight) picked
+    by the autotuner from trials of the same iteration count (shorter trials
+    over-weight the contexts' setup), against one solve at a time."""
+    best, trials = S.batch_autotune(instances=64, max_batches=(4, 8, 16), workers=(4, 8, 12),
+                                    trial_iterations=30)
+    max_batch, workers = best[0], best[1]
     full = S.batch_benchmark(instances=64, iterations=30, max_batch=max_batch, workers=workers)
     serial = S.batch_benchmark(instances=64, iterations=30, max_batch=1, workers=1)
     return {"instances": 64, "iterations": 30, "max_batch": max_batch, "workers": workers,
             "lm_iters_per_s": full["lm_iterations_per_second"],
             "serial_lm_iters_per_s": serial["lm_iterations_per_second"],
             "speedup_vs_serial": full["lm_iterations_per_second"] / serial["lm_iterations_per_second"],
-            "unknowns_total": full["unknowns"]}
+            "unknowns_total": full["unknowns"],
+            "autotune_trials": [{"max_batch": t[0], "workers": t[1], "lm_iters_per_s": round(t[2], 1)}
+                                for t in trials]}
 
 
 def main():

@@ -290,6 +290,23 @@ typedef struct sdfslam_batch_result {
 } sdfslam_batch_result;
 int sdfslam_host_batch_benchmark(const sdfslam_batch_options* options, sdfslam_batch_result* out);
 
+/* Autotuner of the batched form's shapes (no counterpart in the reference):
+ * builds the instances of `options` once, then times trial_iterations LM
+ * iterations for every pair of max_batches[i] (instances per batched solve)
+ * and workers[j] (batched solves in flight, each on its own stream), row
+ * major, after one untimed warm-up trial. trials receives
+ * num_max_batches * num_workers entries (may be NULL); *best the fastest.
+ * options->max_batch, ->workers and ->iterations are not used. */
+typedef struct sdfslam_batch_shape {
+  int32_t max_batch;
+  int32_t workers;
+  double lm_iterations_per_second;
+} sdfslam_batch_shape;
+int sdfslam_host_batch_autotune(const sdfslam_batch_options* options, const int32_t* max_batches,
+                                int32_t num_max_batches, const int32_t* workers, int32_t num_workers,
+                                int32_t trial_iterations, sdfslam_batch_shape* trials,
+                                sdfslam_batch_shape* best);
+
 /* Parses the `solver:` block of a run-config YAML file with the reference's
  * strictness (unknown keys are errors, config.cpp:13-26, 170-190). backend
  * receives "lm", "ceres" or "hip" (capacity >= 8). */

@@ -746,4 +746,27 @@ int sdfslam_host_batch_benchmark(const sdfslam_batch_options* options, sdfslam_b
   });
 }
 
+int sdfslam_host_batch_autotune(const sdfslam_batch_options* options, const int32_t* max_batches,
+                                int32_t num_max_batches, const int32_t* workers, int32_t num_workers,
+                                int32_t trial_iterations, sdfslam_batch_shape* trials,
+                                sdfslam_batch_shape* best) {
+  return Guard([&]() -> int {
+    if (options == nullptr || best == nullptr || num_max_batches < 1 || num_workers < 1 || max_batches == nullptr ||
+        workers == nullptr) {
+      throw std::invalid_argument("null argument or empty candidate list");
+    }
+    BatchBenchmark b;
+    b.instances = options->instances;
+    b.seed0 = options->seed0;
+    const std::vector<int> mb(max_batches, max_batches + num_max_batches);
+    const std::vector<int> w(workers, workers + num_workers);
+    const BatchTuneResult r = TuneBatchShapes(b, mb, w, trial_iterations);
+    for (size_t i = 0; trials != nullptr && i < r.trials.size(); ++i) {
+      trials[i] = {r.trials[i].max_batch, r.trials[i].workers, r.trials[i].lm_iterations_per_second};
+    }
+    *best = {r.best.max_batch, r.best.workers, r.best.lm_iterations_per_second};
+    return SDFSLAM_OK;
+  });
+}
+
 }  // extern "C"

@@ -405,29 +405,37 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   return r;
 }
 
-BatchResult RunBatchBenchmark(const BatchBenchmark& o) {
-  if (o.instances < 1 || o.max_batch < 1) {
-    throw std::invalid_argument("batch benchmark needs instances and max_batch >= 1");
-  }
+namespace {
+
+/// The batched benchmark's instances (built before any clock starts).
+std::vector<std::unique_ptr<Problem>> BuildBatchInstances(const BatchBenchmark& o) {
   std::vector<std::unique_ptr<Problem>> problems(static_cast<size_t>(o.instances));
-  {
-    std::vector<std::thread> gen;
-    std::atomic<int> next{0};
-    const int threads = std::max(1, std::min(o.instances, 8));
-    for (int t = 0; t < threads; ++t) {
-      gen.emplace_back([&] {
-        for (int i = next++; i < o.instances; i = next++) {
-          const Workload wl = MakeWorkload("batch" + std::to_string(i), o.seed0);
-          problems[static_cast<size_t>(i)] = std::make_unique<Problem>(
-              wl.map, wl.initial, wl.scans, wl.odometry, wl.problem, wl.relations);
-        }
-      });
-    }
-    for (std::thread& t : gen) t.join();
+  std::vector<std::thread> gen;
+  std::atomic<int> next{0};
+  const int threads = std::max(1, std::min(o.instances, 8));
+  for (int t = 0; t < threads; ++t) {
+    gen.emplace_back([&] {
+      for (int i = next++; i < o.instances; i = next++) {
+        const Workload wl = MakeWorkload("batch" + std::to_string(i), o.seed0);
+        problems[static_cast<size_t>(i)] =
+            std::make_unique<Problem>(wl.map, wl.initial, wl.scans, wl.odometry, wl.problem, wl.relations);
+      }
+    });
   }
+  for (std::thread& t : gen) t.join();
+  return problems;
+}
+
+/// Times `iterations` LM iterations of every instance: instances largest
+/// first, cut into batched solves of at most `max_batch`, which `workers`
+/// host threads run concurrently on their own streams. The clock covers
+/// device setup (one context per instance) and the iterations.
+BatchResult TimeBatches(const std::vector<std::unique_ptr<Problem>>& problems, int iterations, int max_batch,
+                        int workers) {
+  const auto instances = static_cast<int>(problems.size());
   SolverOptions opts;
   opts.backend = SolverBackend::kHip;
-  opts.max_iterations = o.iterations;
+  opts.max_iterations = iterations;
   opts.step_tolerance = 0.0;
   opts.lambda_init = 1.0;
   opts.lambda_factor = 1.0;
@@ -435,25 +443,25 @@ BatchResult RunBatchBenchmark(const BatchBenchmark& o) {
   BatchResult r;
   for (const auto& p : problems) r.unknowns += p->num_parameters();
   // Largest first, so each batch holds instances of similar size.
-  std::vector<int> order(static_cast<size_t>(o.instances));
-  for (int i = 0; i < o.instances; ++i) order[static_cast<size_t>(i)] = i;
+  std::vector<int> order(static_cast<size_t>(instances));
+  for (int i = 0; i < instances; ++i) order[static_cast<size_t>(i)] = i;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return problems[static_cast<size_t>(a)]->num_parameters() > problems[static_cast<size_t>(b)]->num_parameters();
   });
-  const int batches = (o.instances + o.max_batch - 1) / o.max_batch;
+  const int batches = (instances + max_batch - 1) / max_batch;
   std::atomic<int> next_batch{0};
   std::atomic<int64_t> iters{0};
   std::mutex err_mu;
   std::string error;
   // Contexts live until the clock stops (freeing device memory waits for the
   // whole device).
-  std::vector<std::unique_ptr<HipSession>> sessions(static_cast<size_t>(o.instances));
+  std::vector<std::unique_ptr<HipSession>> sessions(static_cast<size_t>(instances));
   const auto start = Clock::now();
   auto body = [&] {
     for (int b = next_batch++; b < batches; b = next_batch++) {
       try {
-        const int first = b * o.max_batch;
-        const int last = std::min(o.instances, first + o.max_batch);
+        const int first = b * max_batch;
+        const int last = std::min(instances, first + max_batch);
         const int nb = last - first;
         std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(nb));
         for (int q = first; q < last; ++q) {
@@ -479,7 +487,7 @@ BatchResult RunBatchBenchmark(const BatchBenchmark& o) {
     }
   };
   std::vector<std::thread> pool;
-  for (int w = 1; w < std::max(1, o.workers); ++w) pool.emplace_back(body);
+  for (int w = 1; w < std::max(1, workers); ++w) pool.emplace_back(body);
   body();
   for (std::thread& t : pool) t.join();
   r.wall_seconds = Seconds(start, Clock::now());
@@ -487,6 +495,43 @@ BatchResult RunBatchBenchmark(const BatchBenchmark& o) {
   r.lm_iterations = iters;
   r.lm_iterations_per_second = static_cast<double>(r.lm_iterations) / std::max(r.wall_seconds, 1e-9);
   return r;
+}
+
+}  // namespace
+
+BatchResult RunBatchBenchmark(const BatchBenchmark& o) {
+  if (o.instances < 1 || o.max_batch < 1) {
+    throw std::invalid_argument("batch benchmark needs instances and max_batch >= 1");
+  }
+  return TimeBatches(BuildBatchInstances(o), o.iterations, o.max_batch, o.workers);
+}
+
+BatchTuneResult TuneBatchShapes(const BatchBenchmark& o, const std::vector<int>& max_batches,
+                                const std::vector<int>& workers, int trial_iterations) {
+  if (o.instances < 1 || max_batches.empty() || workers.empty() || trial_iterations < 1) {
+    throw std::invalid_argument("autotune needs instances, candidate shapes and trial iterations >= 1");
+  }
+  for (const int mb : max_batches) {
+    if (mb < 1) throw std::invalid_argument("autotune: max_batch candidates must be >= 1");
+  }
+  for (const int w : workers) {
+    if (w < 1) throw std::invalid_argument("autotune: worker candidates must be >= 1");
+  }
+  const std::vector<std::unique_ptr<Problem>> problems = BuildBatchInstances(o);
+  BatchTuneResult out;
+  // One untimed trial first, so the process's first device setup is not
+  // charged to the first candidate.
+  (void)TimeBatches(problems, 1, max_batches.front(), workers.front());
+  for (const int mb : max_batches) {
+    for (const int w : workers) {
+      const BatchResult r = TimeBatches(problems, trial_iterations, mb, w);
+      out.trials.push_back({mb, w, r.lm_iterations_per_second});
+      if (r.lm_iterations_per_second > out.best.lm_iterations_per_second) {
+        out.best = {mb, w, r.lm_iterations_per_second};
+      }
+    }
+  }
+  return out;
 }
 
 }  // namespace sdf_slam

@@ -206,4 +206,22 @@ struct BatchResult {
 };
 BatchResult RunBatchBenchmark(const BatchBenchmark& options);
 
+/// Autotuner of the batched form's shapes: times `trial_iterations` LM
+/// iterations of the benchmark's instances for every (max_batch, workers)
+/// candidate pair (instances per batched solve, batched solves in flight on
+/// their own streams) and keeps the fastest. The instances are built once
+/// and shared by the trials; each trial creates its contexts afresh, as a
+/// run does.
+struct BatchShape {
+  int max_batch{0};
+  int workers{0};
+  double lm_iterations_per_second{0.0};
+};
+struct BatchTuneResult {
+  BatchShape best;
+  std::vector<BatchShape> trials;
+};
+BatchTuneResult TuneBatchShapes(const BatchBenchmark& options, const std::vector<int>& max_batches,
+                                const std::vector<int>& workers, int trial_iterations);
+
 }  // namespace sdf_slam

@@ -149,6 +149,10 @@ class BatchOptions(C.Structure):
                 ("workers", C.c_int32), ("seed0", C.c_uint64)]
 
 
+class BatchShape(C.Structure):
+    _fields_ = [("max_batch", C.c_int32), ("workers", C.c_int32), ("lm_iterations_per_second", C.c_double)]
+
+
 class BatchResult(C.Structure):
     _fields_ = [("lm_iterations", C.c_int64), ("unknowns", C.c_int64),
                 ("wall_seconds", C.c_double), ("lm_iterations_per_second", C.c_double)]
@@ -180,6 +184,7 @@ HOST_SYMBOLS = [
     "sdfslam_server_open_session", "sdfslam_server_submit", "sdfslam_server_wait",
     "sdfslam_server_state", "sdfslam_host_served_benchmark", "sdfslam_host_batch_benchmark",
     "sdfslam_host_node_normals", "sdfslam_host_run", "sdfslam_host_init_from_scans",
+    "sdfslam_host_batch_autotune",
 ]
 
 
@@ -263,6 +268,8 @@ host.sdfslam_server_wait.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.POINTER
 host.sdfslam_server_state.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, _f64p, _f64p]
 host.sdfslam_host_served_benchmark.argtypes = [C.POINTER(ServedOptions), C.POINTER(ServedResult)]
 host.sdfslam_host_batch_benchmark.argtypes = [C.POINTER(BatchOptions), C.POINTER(BatchResult)]
+host.sdfslam_host_batch_autotune.argtypes = [C.POINTER(BatchOptions), _i32p, C.c_int32, _i32p, C.c_int32, C.c_int32,
+                                             C.POINTER(BatchShape), C.POINTER(BatchShape)]
 host.sdfslam_host_node_normals.argtypes = [_f64p, _f64p, C.c_int64, C.c_int32, C.POINTER(Grid), C.c_int32,
                                            C.c_double, _f64p, _f64p]
 host.sdfslam_host_parse_solver_yaml.argtypes = [C.c_char_p, C.POINTER(SolverOptions), C.c_char_p,
@@ -773,6 +780,20 @@ def node_normals(points_xy, origins_xy, k_neighbors, grid, method="pca", corner_
             corner_threshold, ptr(nrm, _f64p), ptr(scale, _f64p))
     (_check_hip if device else _check_host)(st)
     return nrm, scale
+
+
+def batch_autotune(instances=64, max_batches=(4, 8, 16), workers=(4, 8), trial_iterations=10, seed0=1000):
+    """Autotuner of the batched form's shapes (sdfslam_host_batch_autotune):
+    returns (best (max_batch, workers, lm iters/s), trials as such tuples)."""
+    o = BatchOptions(instances, trial_iterations, 1, 1, seed0)
+    mb = np.ascontiguousarray(max_batches, dtype=np.int32)
+    w = np.ascontiguousarray(workers, dtype=np.int32)
+    trials = (BatchShape * (len(mb) * len(w)))()
+    best = BatchShape()
+    _check_host(host.sdfslam_host_batch_autotune(C.byref(o), ptr(mb, _i32p), len(mb), ptr(w, _i32p), len(w),
+                                                 trial_iterations, trials, C.byref(best)))
+    as_tuple = lambda s: (s.max_batch, s.workers, s.lm_iterations_per_second)
+    return as_tuple(best), [as_tuple(t) for t in trials]
 
 
 def batch_benchmark(instances=64, iterations=30, max_batch=8, workers=8, seed0=1000):

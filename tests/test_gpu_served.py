@@ -86,3 +86,11 @@ def test_batch_solve_matches_each_alone(sdfslam, tol):
         assert r["iterations"] == res["iterations"]
         assert np.array_equal(r["cost_history"], res["cost_history"])
         assert np.array_equal(bm, m) and np.array_equal(bp, p)
+
+
+def test_batch_autotune_picks_its_fastest_trial(sdfslam):
+    print("case batch autotune", flush=True)
+    best, trials = sdfslam.batch_autotune(instances=4, max_batches=(1, 2), workers=(1, 2), trial_iterations=2)
+    assert [(t[0], t[1]) for t in trials] == [(1, 1), (1, 2), (2, 1), (2, 2)]
+    assert all(t[2] > 0 for t in trials)
+    assert best == max(trials, key=lambda t: t[2])

@@ -231,3 +231,12 @@ def test_host_node_normals_closed_forms(sdfslam):
     # No scan points: default node normals (zero vector, scale 1).
     nrm, scale = sdfslam.node_normals(np.zeros((0, 2)), np.zeros((0, 2)), 7, g2, "pca", device=False)
     assert not nrm.any() and np.all(scale == 1.0)
+
+
+@pytest.mark.parametrize("max_batches,workers,iters", [((), (1,), 2), ((0,), (1,), 2), ((2,), (0,), 2),
+                                                        ((2,), (1,), 0)])
+def test_batch_autotune_rejects_bad_candidates(sdfslam, max_batches, workers, iters):
+    # Argument checks run before any instance is built or any device call;
+    # std::invalid_argument arrives as ValueError.
+    with pytest.raises(ValueError):
+        sdfslam.batch_autotune(instances=2, max_batches=max_batches, workers=workers, trial_iterations=iters)
