@@ -463,6 +463,15 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   int32_t* ustart = ucell + sort_size;
   __shared__ int32_t counts[kThreads + 1];
   __shared__ double red[9][kThreads];
+  // Until the diagonal block's reduction, `red` holds a bitmap of the
+  // frame's occupied cells (grids of up to 9 * kThreads * 64 cells), so
+  // footprint nodes with no occupied cell around them skip the searches.
+  uint32_t* occupied = reinterpret_cast<uint32_t*>(&red[0][0]);
+  const int occ_words = (P.num_cells + 31) / 32;
+  const bool use_occ = occ_words <= 9 * kThreads * 2;
+  if (use_occ) {
+    for (int i = tid; i < occ_words; i += kThreads) occupied[i] = 0u;
+  }
   const int64_t rb = A.pr_ptr[pose];
   const int m = static_cast<int>(A.pr_ptr[pose + 1] - rb);
   // This frame's sort size: its row count rounded up to a power of two
@@ -540,6 +549,13 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   __syncthreads();
   if (valid_end_local >= 0) ustart[num_u] = valid_end_local;
   if (num_u == 0 && tid == 0) ustart[0] = 0;
+  if (use_occ) {
+    // The bitmap was cleared before the sort's barriers.
+    for (int u = tid; u < num_u; u += kThreads) {
+      const int c = ucell[u];
+      atomicOr(&occupied[c >> 5], 1u << (c & 31));
+    }
+  }
   __syncthreads();
 
   // This frame's partial sums per occupied cell, in the cell-sorted order:
@@ -617,6 +633,7 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
       const int ch = chs[k];
       if (cw < 0 || ch < 0 || cw >= ncx || ch >= ncy) continue;
       const int cid = ch * ncx + cw;
+      if (use_occ && ((occupied[cid >> 5] >> (cid & 31)) & 1u) == 0u) continue;
       int lo2 = 0, hi2 = num_u;
       while (lo2 < hi2) {
         const int mid = (lo2 + hi2) >> 1;
@@ -634,7 +651,10 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
     W.panel[o + st] = v1;
     W.panel[o + 2 * st] = v2;
   }
-  // Diagonal block and -J^T r over the pose's rows (strided, then a fixed tree).
+  // Diagonal block and -J^T r over the pose's rows (strided, then a fixed
+  // tree); `red` is the occupied-cell bitmap until every thread is past the
+  // footprint entries.
+  __syncthreads();
   double acc[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k] = 0.0;
