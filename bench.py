@@ -142,7 +142,7 @@ def incremental_leg(S, frames):
 def served_leg(S, duration, plain_duration, workers):
     """32 incremental sessions, Poisson frame arrivals at 10 Hz each; the
     served form (continuous batching: `workers` host threads prepare each
-    increment on arrival, four solver threads batch what is prepared) against
+    increment on arrival, three solver threads batch what is prepared) against
     the plain path (one increment at a time, preparation and solve in turn)
     and one session alone; then the served form at 30 Hz per session for the
     throughput it sustains."""

@@ -49,7 +49,7 @@ struct ServerStats {
 struct ServerOptions {
   int workers{12};                 // host threads preparing increments
   int max_batch{64};               // sessions per batched solve (1: one at a time)
-  int pipelines{4};                // batched solves in flight (solver threads)
+  int pipelines{3};                // batched solves in flight (solver threads)
   int64_t device_budget_bytes{0};  // 0: 80 % of the device's memory
   int max_in_flight{0};            // increments prepared or solving at once (0: no limit;
                                    // 1: one increment at a time, the plain path)
@@ -154,7 +154,7 @@ struct ServedBenchmark {
   double duration_s{10.0};
   int workers{12};
   int max_batch{64};
-  int pipelines{4};
+  int pipelines{3};
   uint64_t seed0{2000};
   int max_in_flight{0};
   bool device_normals{false};

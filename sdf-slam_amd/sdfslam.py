@@ -697,7 +697,7 @@ class Dataset:
 class Server:
     """Served form: many incremental sessions on one GPU (sdfslam_server_*)."""
 
-    def __init__(self, workers=12, device_budget_bytes=0, max_batch=64, pipelines=4, max_in_flight=0,
+    def __init__(self, workers=12, device_budget_bytes=0, max_batch=64, pipelines=3, max_in_flight=0,
                  device_normals=False):
         h = C.c_void_p()
         o = ServerOptions(workers, max_batch, pipelines, device_budget_bytes, max_in_flight,
@@ -753,7 +753,7 @@ class Server:
 
 
 def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=12, max_batch=64,
-                     pipelines=4, seed0=2000, max_in_flight=0, device_normals=False):
+                     pipelines=3, seed0=2000, max_in_flight=0, device_normals=False):
     o = ServedOptions(sessions, rate_hz, duration_s, workers, max_batch, pipelines, seed0, max_in_flight,
                       int(device_normals))
     r = ServedResult()
