@@ -999,6 +999,14 @@ int oracle_linear_step(const sdfslam_problem_view* problem, const double* map_va
 // SolveLm (lm_solver.cpp:201-434)
 int oracle_solve(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
                  double* map_values, double* poses, sdfslam_solve_result* result) {
+  return oracle_solve_recorded(problem, options, map_values, poses, result, 0, nullptr, nullptr,
+                               nullptr);
+}
+
+int oracle_solve_recorded(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
+                          double* map_values, double* poses, sdfslam_solve_result* result,
+                          int32_t num_records, const int32_t* record_iterations, double* record_map,
+                          double* record_poses) {
   using clock = std::chrono::steady_clock;
   const auto start = clock::now();
   for (double& s : g_phase_seconds) {
@@ -1095,6 +1103,17 @@ int oracle_solve(const sdfslam_problem_view* problem, const sdfslam_solver_optio
     }
     history.push_back(current_cost);
     result->final_cost = current_cost;
+    // The state the iteration ends in, for every requested iteration count.
+    for (int32_t q = 0; q < num_records; ++q) {
+      if (record_iterations[q] == iter + 1) {
+        if (record_map != nullptr) {
+          std::copy(map_values, map_values + map_size, record_map + static_cast<size_t>(q) * map_size);
+        }
+        if (record_poses != nullptr) {
+          std::copy(poses, poses + pose_size, record_poses + static_cast<size_t>(q) * pose_size);
+        }
+      }
+    }
   }
   if (result->iterations == 0) {
     result->final_cost = result->initial_cost;

@@ -10,7 +10,7 @@
  * explicit std::fma calls the reference itself writes in its LDL^T row
  * kernel (src/solvers/parallel_ldlt.hpp:186, 196).
  *
- * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ * Parity pins (what checks each function, in tests/test_oracle.py):
  *   oracle_wrap_angle/compose/relative  tests/test_pose.cpp known answers
  *   oracle_locate/interpolate           tests/test_grid_map.cpp known answers
  *   oracle_eval_point (exact gradient)  central finite differences (test_jacobians.cpp:33-76)
@@ -19,8 +19,14 @@
  *   oracle_assemble                     brute-force per-row restatement + FD of the cost
  *   oracle_normal_matrix/neg_atb        dense J^T J and -J^T r computed with numpy
  *   oracle_ldlt_solve                   dense solves (numpy), residual norms, singular -> error
- *   oracle_solve (LM control)           e2e thresholds (test_e2e.cpp:63-88) and the
- *                                       trust-region monotonicity (test_e2e.cpp:168-184)
+ *   oracle_cost                         r^T r of the assembled residuals (Huber off);
+ *                                       hand-computed Huber cost on a 3-row affine toy
+ *   oracle_solve (LM control)           e2e thresholds (test_e2e.cpp:63-88), the
+ *                                       trust-region monotonicity (test_e2e.cpp:168-184),
+ *                                       and hand-computed fixed-factor lambda traces with
+ *                                       and without reject_worse_steps (each iteration's
+ *                                       state = ApplyStep(linear step at the traced lambda),
+ *                                       reverts bitwise)
  *   oracle_eval_points_frozen           equals oracle_eval_point inside the cell
  *                                       (test_jacobians.cpp:78-102) + central FD off-cell
  *   oracle_init_from_scans              tests/test_map_init.cpp known answers, plus a
@@ -109,6 +115,15 @@ int oracle_linear_step(const sdfslam_problem_view* problem, const double* map_va
 /* SolveLm (lm_solver.cpp:201-434): map_values/poses updated in place. */
 int oracle_solve(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
                  double* map_values, double* poses, sdfslam_solve_result* result);
+
+/* oracle_solve that also copies the state at the end of iteration
+ * record_iterations[q] (q < num_records) into record_map + q*nx*ny and
+ * record_poses + q*3*num_poses (either may be NULL). An iteration count the
+ * solve does not reach (convergence stop) is not written. */
+int oracle_solve_recorded(const sdfslam_problem_view* problem, const sdfslam_solver_options* options,
+                          double* map_values, double* poses, sdfslam_solve_result* result,
+                          int32_t num_records, const int32_t* record_iterations, double* record_map,
+                          double* record_poses);
 
 /* EvalPointResidualFrozenCell (residual_math.hpp:113-147) for n point rows:
  * row k evaluates point (px[k], py[k]) of frame frame[k] (pose

@@ -1260,13 +1260,29 @@ class BatchRun {
           status[static_cast<size_t>(b)] = SDFSLAM_ERR_NOT_SPD;
           continue;
         }
-        if (st.overflow == 0) continue;
+        if (st.overflow == 0) {
+          // The same speculative epochs as a solve of this context alone
+          // (RunIterations): its chunks end at the same iterations, so the
+          // pattern, and with it every rounding, follows the same sequence.
+          if (st.halt == 0 && st.iter < options.max_iterations) {
+            const int epochs = c.pattern_epochs;
+            const auto t_spec = Now();
+            Speculate(c);
+            if (c.pattern_epochs != epochs) {
+              info_.analysis_seconds += SecondsSince(t_spec);
+              ++info_.epochs_rebuilt;
+              changed = true;
+            }
+          }
+          continue;
+        }
         if (st.iter == last_overflow[static_cast<size_t>(b)] && repeats[static_cast<size_t>(b)]++ > 0) {
           throw Failure(SDFSLAM_ERR_INTERNAL, "pattern re-analysis did not cover the point cells");
         }
         c.short_epoch = last_overflow[static_cast<size_t>(b)] >= 0 &&
                         st.iter - last_overflow[static_cast<size_t>(b)] <= 2;
         last_overflow[static_cast<size_t>(b)] = st.iter;
+        DropSpeculation(c);
         const auto t_epoch = Now();
         BuildEpoch(c);
         info_.analysis_seconds += SecondsSince(t_epoch);
@@ -1279,6 +1295,8 @@ class BatchRun {
       }
       if (changed) Rebuild();
     }
+    // A re-analysis started for iterations this call does not run is dropped.
+    for (sdfslam_hip_context* c : ctxs_) DropSpeculation(*c);
   }
 
  private:

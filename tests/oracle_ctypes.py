@@ -68,6 +68,8 @@ def _setup(view_type, grid_type, solver_options_type, solve_result_type):
     lib.oracle_linear_step.argtypes = [V, _f64p, _f64p, C.c_double, C.c_int, _f64p]
     lib.oracle_solve.argtypes = [V, C.POINTER(solver_options_type), _f64p, _f64p,
                                  C.POINTER(solve_result_type)]
+    lib.oracle_solve_recorded.argtypes = [V, C.POINTER(solver_options_type), _f64p, _f64p,
+                                          C.POINTER(solve_result_type), C.c_int32, _i32p, _f64p, _f64p]
     lib.oracle_last_phase_seconds.argtypes = [_f64p]
     lib.oracle_eval_points_frozen.argtypes = [G, _f64p, _f64p, C.c_int64, _i32p, _f64p, _f64p, _f64p,
                                               _i32p, _i32p, _f64p, _f64p, _f64p]
@@ -159,6 +161,15 @@ def cost(view, map_values, poses):
     return lib.oracle_cost(C.byref(view), _p(m, _f64p), _p(p, _f64p))
 
 
+def apply_step(view, step, map_values, poses):
+    """Problem::ApplyStep restated; returns the new (map, poses)."""
+    m = np.ascontiguousarray(map_values, dtype=np.float64).copy()
+    p = np.ascontiguousarray(poses, dtype=np.float64).copy()
+    st = np.ascontiguousarray(step, dtype=np.float64)
+    lib.oracle_apply_step(C.byref(view), _p(st, _f64p), _p(m, _f64p), _p(p, _f64p))
+    return m, p
+
+
 def linear_step(view, map_values, poses, lam, marquardt=False):
     n = view.num_active_nodes + 3 * (view.num_poses - 1)
     step = np.zeros(n)
@@ -211,6 +222,27 @@ def solve(mod, view, options, map_values, poses):
                final_cost=r.final_cost, duration_seconds=r.duration_seconds,
                cost_history=hist[:n].copy())
     return st, res, m, p
+
+
+def solve_recorded(mod, view, options, map_values, poses, record_iterations):
+    """solve() that also returns the state after each of record_iterations:
+    (status, result, map, poses, maps (k, nx*ny), poses (k, 3*N))."""
+    m = np.ascontiguousarray(map_values, dtype=np.float64).copy()
+    p = np.ascontiguousarray(poses, dtype=np.float64).copy()
+    its = np.ascontiguousarray(record_iterations, dtype=np.int32)
+    rm = np.full((len(its), len(m)), np.nan)
+    rp = np.full((len(its), len(p)), np.nan)
+    hist = np.zeros(max(options.max_iterations, 0) + 1)
+    r = mod.SolveResult()
+    r.cost_history = _p(hist, _f64p)
+    r.cost_history_capacity = len(hist)
+    st = lib.oracle_solve_recorded(C.byref(view), C.byref(options), _p(m, _f64p), _p(p, _f64p),
+                                   C.byref(r), len(its), _p(its, _i32p), _p(rm, _f64p), _p(rp, _f64p))
+    n = min(r.cost_history_length, len(hist))
+    res = dict(converged=bool(r.converged), iterations=r.iterations, initial_cost=r.initial_cost,
+               final_cost=r.final_cost, duration_seconds=r.duration_seconds,
+               cost_history=hist[:n].copy())
+    return st, res, m, p, rm, rp
 
 
 def last_phase_seconds():

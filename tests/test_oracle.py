@@ -379,3 +379,151 @@ def test_oracle_point_rows_match_frozen_cell(sdfslam, oracle, name, smooth):
     prob, g, m, p = seeded_problem(sdfslam, name, smooth)
     r, cols, vals = oracle_rows(oracle, prob, m, p)
     assert check_point_rows(oracle, prob, g, m, p, r, cols, vals, smooth) > 0.5 * prob.view.num_points
+
+
+# ------------------------------------------------ cost and LM control pins
+
+
+@pytest.mark.parametrize("name", ["room_small", "office_small"])
+def test_cost_is_the_squared_norm_of_the_assembled_residuals(sdfslam, oracle, name):
+    # Problem::Cost (problem.cpp:132-156) with Huber off is r^T r of the
+    # residual vector the assembly writes (lm_solver.cpp:30-82, 275-296):
+    # the same rows, summed in another order.
+    prob = sdfslam.Problem.workload(name, 2)
+    m, p = prob.state()
+    rng = np.random.default_rng(5)
+    m = m + rng.normal(0, 0.1, size=m.shape)
+    p = p + np.r_[np.zeros(3), rng.normal(0, 0.02, size=len(p) - 3)]
+    assert prob.view.huber_delta == 0.0
+    r, _ = oracle.assemble(prob.view, m, p)
+    c = oracle.cost(prob.view, m, p)
+    assert abs(c - float(np.dot(r, r))) <= 1e-12 * c
+
+
+def huber_toy(sdfslam, delta):
+    """Three point rows on the field D(x, y) = x (bilinear interpolation is
+    exact on it), frame 0 at the origin: residuals 0.2, 0.5, 1.5 with unit
+    weights; eikonal rows carry weight 0."""
+    po = sdfslam.default_problem_options(hallucination_points_per_scan_point=0, weight_eikonal=0.0,
+                                         normals_k_neighbors=2, huber_delta=delta, smooth_gradient=0)
+    vals = np.array([float(w) for h in range(3) for w in range(3)])
+    scan = np.array([[0.2, 0.7], [0.5, 1.3], [1.5, 0.4]])
+    return sdfslam.Problem.build((3, 3, 0.0, 0.0, 1.0, 1.0), np.zeros((1, 3)), [scan], options=po,
+                                 map_values=vals)
+
+
+def test_huber_cost_and_irls_rows_closed_form(sdfslam, oracle):
+    # huber_delta 0.4 (problem.hpp:115-121, problem.cpp:140-146):
+    #   |0.2| <= 0.4: 0.2^2                = 0.04
+    #   |0.5|  > 0.4: 0.4 * (2*0.5 - 0.4)  = 0.24
+    #   |1.5|  > 0.4: 0.4 * (2*1.5 - 0.4)  = 1.04      cost 1.32
+    # IRLS rows (lm_solver.cpp:39-47): w = delta/|r| outside, row scaled by
+    # sqrt(w): 0.5 -> sqrt(0.2), 1.5 -> sqrt(0.6); node weights of (0.2, 0.7)
+    # in cell (0, 0): (0.8*0.3, 0.2*0.3, 0.8*0.7, 0.2*0.7).
+    prob = huber_toy(sdfslam, 0.4)
+    m, p = prob.state()
+    assert abs(oracle.cost(prob.view, m, p) - 1.32) <= 1e-14
+    r, jac = oracle.assemble(prob.view, m, p)
+    assert np.allclose(r[:3], [0.2, math.sqrt(0.2), math.sqrt(0.6)], rtol=0, atol=1e-15)
+    J = jac.to_scipy().toarray()
+    assert np.allclose(J[0, [0, 1, 3, 4]], [0.24, 0.06, 0.56, 0.14], rtol=0, atol=1e-15)
+    # (0.5, 1.3) in cell (0, 1): weights (0.35, 0.35, 0.15, 0.15) times sqrt(0.8).
+    assert np.allclose(J[1, [3, 4, 6, 7]], np.array([0.35, 0.35, 0.15, 0.15]) * math.sqrt(0.8),
+                       rtol=0, atol=1e-15)
+    # Huber off: the plain sum of squares 0.04 + 0.25 + 2.25.
+    prob0 = huber_toy(sdfslam, 0.0)
+    assert abs(oracle.cost(prob0.view, *prob0.state()) - 2.54) <= 1e-14
+
+
+def _step_from(oracle, view, m, p, lam):
+    st, step = oracle.linear_step(view, m, p, lam)
+    assert st == 0
+    return oracle.apply_step(view, step, m, p)
+
+
+def test_fixed_factor_lambda_trace_with_reject(sdfslam, oracle):
+    # room_small from its initial state with lambda_init 1e-2, factor 3 and
+    # reject_worse_steps: iteration 1 lowers the cost, iterations 2-6 raise it
+    # and are reverted, iteration 7 lowers it. By lm_solver.cpp:411-423 the
+    # lambdas are 1e-2, then 1e-2/3 (success divides), then times 3 per
+    # rejection; a rejection restores the backup bit for bit and repeats the
+    # previous cost in the history.
+    prob = sdfslam.Problem.workload("room_small", 1)
+    v = prob.view
+    m0, p0 = prob.state()
+    lam0, f = 1e-2, 3.0
+
+    def run(k, reject=1):
+        o = sdfslam.default_solver_options(max_iterations=k, step_tolerance=0.0, lambda_init=lam0,
+                                           lambda_factor=f, reject_worse_steps=reject)
+        st, res, m, p = oracle.solve(sdfslam, v, o, m0, p0)
+        assert st == 0 and res["iterations"] == k
+        return res["cost_history"], m, p
+
+    m1, p1 = _step_from(oracle, v, m0, p0, lam0)
+    h, m, p = run(1)
+    assert np.array_equal(m, m1) and np.array_equal(p, p1)
+    c1 = oracle.cost(v, m1, p1)
+    assert h[1] == c1 < h[0]
+    lam = lam0 / f
+    # Iteration 2's step (lambda0 / f) raises the cost; the solve reverts it.
+    m2, p2 = _step_from(oracle, v, m1, p1, lam)
+    assert oracle.cost(v, m2, p2) > c1
+    h, m, p = run(2)
+    assert np.array_equal(m, m1) and np.array_equal(p, p1) and h[2] == h[1]
+    for _ in range(4):  # iterations 3-6, all rejected
+        lam *= f
+    h, m, p = run(6)
+    assert np.array_equal(m, m1) and np.array_equal(p, p1) and np.all(h[2:] == h[1])
+    lam *= f  # iteration 7 runs at lambda0 / f * f^5 and is accepted
+    m7, p7 = _step_from(oracle, v, m1, p1, lam)
+    h, m, p = run(7)
+    assert np.array_equal(m, m7) and np.array_equal(p, p7)
+    assert h[7] == oracle.cost(v, m7, p7) < c1
+    # The lambdas a swapped update (multiply on success) would use instead.
+    m7x, _ = _step_from(oracle, v, m1, p1, lam0 * f ** 6)
+    assert not np.array_equal(m7x, m7)
+
+
+def test_fixed_factor_lambda_trace_without_reject(sdfslam, oracle):
+    # Same start without reject_worse_steps: the worse state of iteration 2
+    # is kept and becomes the previous cost (lm_solver.cpp:416-421), lambda
+    # goes 1e-2 -> 1e-2/3 -> (1e-2/3)*3 and iteration 3 starts from state 2.
+    prob = sdfslam.Problem.workload("room_small", 1)
+    v = prob.view
+    m0, p0 = prob.state()
+    lam0, f = 1e-2, 3.0
+    o = sdfslam.default_solver_options(max_iterations=3, step_tolerance=0.0, lambda_init=lam0,
+                                       lambda_factor=f, reject_worse_steps=0)
+    st, res, m, p = oracle.solve(sdfslam, v, o, m0, p0)
+    m1, p1 = _step_from(oracle, v, m0, p0, lam0)
+    m2, p2 = _step_from(oracle, v, m1, p1, lam0 / f)
+    m3, p3 = _step_from(oracle, v, m2, p2, lam0 / f * f)
+    c = [oracle.cost(v, m0, p0), oracle.cost(v, m1, p1), oracle.cost(v, m2, p2), oracle.cost(v, m3, p3)]
+    assert c[2] > c[1] and c[3] < c[2]
+    assert np.array_equal(m, m3) and np.array_equal(p, p3)
+    assert list(res["cost_history"]) == c
+
+
+# ------------------------------------------------ jitter ensembles
+
+
+def test_ensemble_summary_known_answer():
+    import ensemble
+    g = GOLD["stability_ensemble_summary"]
+    s = ensemble.summarize(g["scores"], g["divergence_factor"])
+    assert s["best"] == pytest.approx(g["best"]) and s["diverged"] == g["diverged"]
+    assert s["n"] == g["n"] and s["spread"] == pytest.approx(g["spread"])
+    j = GOLD["stability_ensemble_jitter"]
+    assert ensemble.jittered_lambda(j["lambda_init"], 0, j["jitter"]) == j["lambda_init"]
+    lam1 = ensemble.jittered_lambda(j["lambda_init"], 1, j["jitter"])
+    assert lam1 == j["lambda_init"] * (1.0 + 1e-9) and lam1 != j["lambda_init"]
+
+
+def test_mean_translation_error_closed_form():
+    import ensemble
+    gt = np.zeros((4, 3))
+    est = np.array([[0, 0, 0], [3, 4, 0.1], [0, 1, 0], [1, 0, -2.0]])
+    assert ensemble.mean_translation_error(est, gt) == pytest.approx((0 + 5 + 1 + 1) / 4)
+    with pytest.raises(ValueError):
+        ensemble.mean_translation_error(est[:1], gt[:1])
