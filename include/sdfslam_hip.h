@@ -288,6 +288,24 @@ int sdfslam_hip_node_normals(const double* points_xy, const double* origins_xy, 
 int sdfslam_hip_init_from_scans(const double* points_xy, const double* origins_xy, int64_t num_points,
                                 const sdfslam_grid* grid, int32_t signed_distance, double* map_values);
 
+/* sdfslam_hip_init_from_scans without any device allocation. No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results, but points_xy, origins_xy and
+ * map_values are DEVICE pointers, and the work is enqueued on `stream` (a
+ * hipStream_t; NULL = the null stream) without host synchronisation: the
+ * values are in map_values once the stream reaches that point. Every
+ * intermediate buffer lives in one file-scope device array per GPU
+ * (SDFSLAM_NO_ALLOC_SCRATCH_MIB, held until the process ends), so at most
+ * SDFSLAM_NO_ALLOC_MAX_POINTS points are accepted (more: INVALID_ARGUMENT,
+ * nothing enqueued). Calls that overlap on one GPU share that array and
+ * race: serialise them (one stream, or wait between calls). */
+#define SDFSLAM_NO_ALLOC_MAX_POINTS 524288
+#define SDFSLAM_NO_ALLOC_SCRATCH_MIB 24
+int sdfslam_hip_init_from_scans_noalloc(const double* points_xy, const double* origins_xy,
+                                        int64_t num_points, const sdfslam_grid* grid,
+                                        int32_t signed_distance, double* map_values, void* stream);
+
 /* Runs exactly `iterations` LM iterations from the current device state,
  * carrying the LM state (lambda, previous cost) in the context and ignoring
  * max_iterations; used to time iterations. With collect_phase_times != 0

@@ -1912,6 +1912,19 @@ int sdfslam_hip_init_from_scans(const double* points_xy, const double* origins_x
   });
 }
 
+int sdfslam_hip_init_from_scans_noalloc(const double* points_xy, const double* origins_xy,
+                                        int64_t num_points, const sdfslam_grid* grid,
+                                        int32_t signed_distance, double* map_values, void* stream) {
+  static_assert(kNoAllocMaxPoints == SDFSLAM_NO_ALLOC_MAX_POINTS, "header and kernels disagree");
+  return Guard([&]() -> int {
+    if (grid == nullptr || num_points < 0) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    const int st = InitFromScansNoAlloc(points_xy, origins_xy, num_points, *grid, signed_distance, map_values,
+                                        static_cast<hipStream_t>(stream));
+    if (st != SDFSLAM_OK) throw Failure(st, NormalsLastError());
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_hip_node_normals(const double* points_xy, const double* origins_xy, int64_t num_points,
                              int32_t k_neighbors, const sdfslam_grid* grid, int32_t method,
                              double corner_threshold, double* node_normal_xy, double* node_scale) {

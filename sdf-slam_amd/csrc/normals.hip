@@ -35,25 +35,41 @@ struct PointGridDev {
   int gx, gy;
 };
 
+// Where a kernel finds the point grid: by value (computed on the host from
+// host arrays) or in device memory (computed on the device by KPointGrid).
+struct GridByValue {
+  PointGridDev g;
+  __device__ PointGridDev Load() const { return g; }
+};
+struct GridInMemory {
+  const PointGridDev* p;
+  __device__ PointGridDev Load() const { return *p; }
+};
+
 __device__ __forceinline__ int CellCoord(double v, double v0, double cell, int n) {
   const double f = floor((v - v0) / cell);
   return f < 0.0 ? 0 : (f >= n ? n - 1 : static_cast<int>(f));
 }
 
+template <class GridSrc>
 __global__ void KCellCount(const double* __restrict__ px, const double* __restrict__ py, int64_t n,
-                           PointGridDev g, int* __restrict__ cell_of, int* __restrict__ count) {
+                           GridSrc src, int* __restrict__ cell_of, int* __restrict__ count) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
+  const PointGridDev g = src.Load();
   const int c = CellCoord(py[i], g.y0, g.cell, g.gy) * g.gx + CellCoord(px[i], g.x0, g.cell, g.gx);
   cell_of[i] = c;
   atomicAdd(&count[c], 1);
 }
 
-// Exclusive scan of count[0 .. cells) into start[0 .. cells] (one
-// workgroup of kScanThreads; serial over chunks).
+// Exclusive scan of count[0 .. cells) into start[0 .. cells], cells = gx * gy
+// (one workgroup of kScanThreads; serial over chunks).
 constexpr int kScanThreads = 1024;
-__global__ void __launch_bounds__(kScanThreads) KCellScan(const int* __restrict__ count, int cells,
+template <class GridSrc>
+__global__ void __launch_bounds__(kScanThreads) KCellScan(const int* __restrict__ count, GridSrc src,
                                                           int* __restrict__ start) {
+  const PointGridDev g = src.Load();
+  const int cells = g.gx * g.gy;
   __shared__ int part[kScanThreads];
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
@@ -258,13 +274,15 @@ __global__ void KNodeNormals(int nx, int ny, double min_x, double min_y, double 
 // node takes the distance to its nearest scan point, negated behind the
 // surface when signed (the nearest point's sensor origin decides: the node
 // is in front when it is no farther from that origin than the point).
+template <class GridSrc>
 __global__ void KInitFromScans(int nx, int ny, double min_x, double min_y, double dx, double dy,
                                const double* __restrict__ px, const double* __restrict__ py,
-                               const double* __restrict__ ox, const double* __restrict__ oy, PointGridDev g,
+                               const double* __restrict__ ox, const double* __restrict__ oy, GridSrc src,
                                const int* __restrict__ start, const int* __restrict__ sorted, int signed_distance,
                                double* __restrict__ out) {
   const int64_t node = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (node >= static_cast<int64_t>(nx) * ny) return;
+  const PointGridDev g = src.Load();
   const int w = static_cast<int>(node % nx);
   const int h = static_cast<int>(node / nx);
   const double qx = min_x + w * dx;
@@ -443,8 +461,8 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
   HIPN(hipMemsetAsync(count, 0, static_cast<size_t>(cells + 1) * sizeof(int), s));
   HIPN(hipMemsetAsync(fill, 0, static_cast<size_t>(cells + 1) * sizeof(int), s));
   const unsigned pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
-  KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, g, cell_of, count);
-  KCellScan<<<1, kScanThreads, 0, s>>>(count, static_cast<int>(cells), start);
+  KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridByValue{g}, cell_of, count);
+  KCellScan<<<1, kScanThreads, 0, s>>>(count, GridByValue{g}, start);
   KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
   KPointNormals<<<pb, kThreads, 0, s>>>(px, py, ox, oy, n, kk, g, start, sorted, pnx, pny, pcorner);
   const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
@@ -525,15 +543,184 @@ int InitFromScansOnDevice(const double* points_xy, const double* origins_xy, int
   HIPN(hipMemsetAsync(count, 0, static_cast<size_t>(cells + 1) * sizeof(int), s));
   HIPN(hipMemsetAsync(fill, 0, static_cast<size_t>(cells + 1) * sizeof(int), s));
   const unsigned pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
-  KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, g, cell_of, count);
-  KCellScan<<<1, kScanThreads, 0, s>>>(count, static_cast<int>(cells), start);
+  KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridByValue{g}, cell_of, count);
+  KCellScan<<<1, kScanThreads, 0, s>>>(count, GridByValue{g}, start);
   KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
   const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
   KInitFromScans<<<vb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, ox,
-                                         oy, g, start, sorted, signed_distance, out);
+                                         oy, GridByValue{g}, start, sorted, signed_distance, out);
   HIPN(hipGetLastError());
   HIPN(hipMemcpyAsync(map_values, out, static_cast<size_t>(nodes) * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPN(hipStreamSynchronize(s));
+  return SDFSLAM_OK;
+}
+
+// ---- Map seeding without allocation ---------------------------------------
+// InitFromScansNoAlloc keeps every intermediate buffer in `scratch`, one
+// file-scope device array per GPU (kNoAllocScratchBytes, held until the
+// process ends), so it calls no hipMalloc: for callers under graph capture
+// or in serving loops. Inputs and output are device memory of the caller;
+// the point grid is computed on the device. Every call on a GPU uses that
+// GPU's one array: calls that overlap on one GPU race.
+
+constexpr int64_t kNoAllocMaxCells = kNoAllocMaxPoints / 2 + 4097;
+constexpr size_t Align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+constexpr size_t kNoAllocPointBytes = Align256(sizeof(double) * static_cast<size_t>(kNoAllocMaxPoints));
+constexpr size_t kNoAllocIndexBytes = Align256(sizeof(int) * static_cast<size_t>(kNoAllocMaxPoints));
+constexpr size_t kNoAllocCellBytes = Align256(sizeof(int) * static_cast<size_t>(kNoAllocMaxCells + 1));
+constexpr size_t kNoAllocScratchBytes =
+    256 + 4 * kNoAllocPointBytes + 2 * kNoAllocIndexBytes + 3 * kNoAllocCellBytes;
+
+static_assert(kNoAllocScratchBytes <= (size_t{SDFSLAM_NO_ALLOC_SCRATCH_MIB} << 20),
+              "scratch beyond the size the header states");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// points_xy / origins_xy (x, y pairs) into separate coordinate arrays.
+__global__ void KSplitPoints(const double* __restrict__ points_xy, const double* __restrict__ origins_xy,
+                             int64_t n, double* __restrict__ px, double* __restrict__ py,
+                             double* __restrict__ ox, double* __restrict__ oy) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  px[i] = points_xy[2 * i];
+  py[i] = points_xy[2 * i + 1];
+  ox[i] = origins_xy[2 * i];
+  oy[i] = origins_xy[2 * i + 1];
+}
+
+// The point grid of MakePointGrid, on the device (one workgroup): the
+// bounding box by a min/max reduction, then the same cell size; the cell
+// grows while gx * gy exceeds max_cells (the bound the size allows).
+constexpr int kGridThreads = 1024;
+__global__ void __launch_bounds__(kGridThreads) KPointGrid(const double* __restrict__ px,
+                                                           const double* __restrict__ py, int64_t n,
+                                                           int64_t max_cells, PointGridDev* __restrict__ out) {
+  __shared__ double red[4][kGridThreads];
+  double x0 = px[0], x1 = px[0], y0 = py[0], y1 = py[0];
+  for (int64_t i = threadIdx.x; i < n; i += kGridThreads) {
+    x0 = fmin(x0, px[i]);
+    x1 = fmax(x1, px[i]);
+    y0 = fmin(y0, py[i]);
+    y1 = fmax(y1, py[i]);
+  }
+  red[0][threadIdx.x] = x0;
+  red[1][threadIdx.x] = x1;
+  red[2][threadIdx.x] = y0;
+  red[3][threadIdx.x] = y1;
+  __syncthreads();
+  for (int off = kGridThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(off)) {
+      red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][threadIdx.x + off]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + off]);
+      red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][threadIdx.x + off]);
+      red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  x0 = red[0][0];
+  x1 = red[1][0];
+  y0 = red[2][0];
+  y1 = red[3][0];
+  const double extent = fmax(fmax(x1 - x0, y1 - y0), 1e-6);
+  double cell = sqrt((x1 - x0 + 1e-6) * (y1 - y0 + 1e-6) * 2.0 / static_cast<double>(n));
+  cell = fmax(cell, extent / 2048.0);
+  int gx = static_cast<int>((x1 - x0) / cell) + 1;
+  int gy = static_cast<int>((y1 - y0) / cell) + 1;
+  for (int t = 0; t < 64 && static_cast<int64_t>(gx) * gy > max_cells; ++t) {
+    cell *= 2.0;
+    gx = static_cast<int>((x1 - x0) / cell) + 1;
+    gy = static_cast<int>((y1 - y0) / cell) + 1;
+  }
+  *out = PointGridDev{x0, y0, cell, gx, gy};
+}
+
+__global__ void KClearInts(int* __restrict__ a, int* __restrict__ b, int64_t n) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  a[i] = 0;
+  b[i] = 0;
+}
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxScratchDevices] = {};
+
+/// The device address of `scratch` on the current device, looked up once
+/// per device (with that device current).
+int ScratchOfCurrentDevice(unsigned char** out) {
+  int device = 0;
+  HIPN(hipGetDevice(&device));
+  if (device < 0 || device >= kMaxScratchDevices) {
+    t_error = "device index beyond the scratch table";
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device] == nullptr) {
+    void* ptr = nullptr;
+    HIPN(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_dev[device];
+  return SDFSLAM_OK;
+}
+
+}  // namespace
+
+int InitFromScansNoAlloc(const double* points_xy, const double* origins_xy, int64_t num_points,
+                         const sdfslam_grid& grid, int32_t signed_distance, double* map_values, hipStream_t s) {
+  if (num_points <= 0) return SDFSLAM_OK;  // an empty scan list leaves the map unchanged
+  const int64_t n = num_points;
+  const int64_t nodes = static_cast<int64_t>(grid.nx) * grid.ny;
+  if (n > kNoAllocMaxPoints) {
+    t_error = "num_points " + std::to_string(n) + " above the allocation-free maximum of " +
+              std::to_string(kNoAllocMaxPoints);
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  if (grid.nx < 2 || grid.ny < 2 || nodes > (int64_t{1} << 30)) {
+    t_error = "grid needs 2..2^30 nodes with nx, ny >= 2";
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  if (points_xy == nullptr || origins_xy == nullptr || map_values == nullptr) {
+    t_error = "null device pointer";
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  unsigned char* base = nullptr;
+  const int st = ScratchOfCurrentDevice(&base);
+  if (st != SDFSLAM_OK) return st;
+  unsigned char* p = base;
+  auto take = [&p](size_t b) {
+    unsigned char* q = p;
+    p += b;
+    return q;
+  };
+  auto* gp = reinterpret_cast<PointGridDev*>(take(256));
+  auto* px = reinterpret_cast<double*>(take(kNoAllocPointBytes));
+  auto* py = reinterpret_cast<double*>(take(kNoAllocPointBytes));
+  auto* ox = reinterpret_cast<double*>(take(kNoAllocPointBytes));
+  auto* oy = reinterpret_cast<double*>(take(kNoAllocPointBytes));
+  auto* cell_of = reinterpret_cast<int*>(take(kNoAllocIndexBytes));
+  auto* sorted = reinterpret_cast<int*>(take(kNoAllocIndexBytes));
+  auto* count = reinterpret_cast<int*>(take(kNoAllocCellBytes));
+  auto* start = reinterpret_cast<int*>(take(kNoAllocCellBytes));
+  auto* fill = reinterpret_cast<int*>(take(kNoAllocCellBytes));
+  // At most n/2 + 4097 cells for n points (MakePointGrid's cell size), so
+  // count and fill need n/2 + 4098 entries cleared.
+  const int64_t max_cells = n / 2 + 4097;
+  const unsigned pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
+  const unsigned cb = static_cast<unsigned>((max_cells + 1 + kThreads - 1) / kThreads);
+  const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
+  KSplitPoints<<<pb, kThreads, 0, s>>>(points_xy, origins_xy, n, px, py, ox, oy);
+  KPointGrid<<<1, kGridThreads, 0, s>>>(px, py, n, max_cells, gp);
+  KClearInts<<<cb, kThreads, 0, s>>>(count, fill, max_cells + 1);
+  KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridInMemory{gp}, cell_of, count);
+  KCellScan<<<1, kScanThreads, 0, s>>>(count, GridInMemory{gp}, start);
+  KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
+  KInitFromScans<<<vb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, ox,
+                                         oy, GridInMemory{gp}, start, sorted, signed_distance, map_values);
+  HIPN(hipGetLastError());
   return SDFSLAM_OK;
 }
 

@@ -1,6 +1,8 @@
 // Device normal estimation of the problem build (see normals.hip).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 #include "sdfslam_hip.h"
@@ -22,6 +24,14 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
 /// points the values are left unchanged.
 int InitFromScansOnDevice(const double* points_xy, const double* origins_xy, int64_t num_points,
                           const sdfslam_grid& grid, int32_t signed_distance, double* map_values);
+/// InitFromScansOnDevice without allocation: device pointers (points and
+/// origins as x, y pairs, map values node-major), work enqueued on `stream`
+/// with no host synchronisation; every intermediate buffer lives in one
+/// file-scope device array per GPU. At most kNoAllocMaxPoints points.
+constexpr int64_t kNoAllocMaxPoints = int64_t{1} << 19;
+int InitFromScansNoAlloc(const double* points_xy, const double* origins_xy, int64_t num_points,
+                         const sdfslam_grid& grid, int32_t signed_distance, double* map_values,
+                         hipStream_t stream);
 const char* NormalsLastError();
 
 }  // namespace sdfslam

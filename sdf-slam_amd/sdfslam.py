@@ -172,6 +172,7 @@ HIP_SYMBOLS = [
     "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
     "sdfslam_hip_context_update", "sdfslam_hip_device_memory", "sdfslam_hip_batch_solve",
     "sdfslam_hip_pose_margins", "sdfslam_hip_node_normals", "sdfslam_hip_init_from_scans",
+    "sdfslam_hip_init_from_scans_noalloc",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -224,6 +225,8 @@ hip.sdfslam_hip_pose_margins.argtypes = [C.c_void_p, _i32p, C.c_int32, _i32p]
 hip.sdfslam_hip_node_normals.argtypes = [_f64p, _f64p, C.c_int64, C.c_int32, C.POINTER(Grid), C.c_int32,
                                          C.c_double, _f64p, _f64p]
 hip.sdfslam_hip_init_from_scans.argtypes = [_f64p, _f64p, C.c_int64, C.POINTER(Grid), C.c_int32, _f64p]
+hip.sdfslam_hip_init_from_scans_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Grid),
+                                                    C.c_int32, C.c_void_p, C.c_void_p]
 hip.sdfslam_hip_batch_solve.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(SolverOptions),
                                         C.POINTER(SolveResult), _i32p, C.POINTER(BatchInfo)]
 
@@ -659,6 +662,20 @@ def init_from_scans(grid, map_values, scans, poses, signed_distance=True, device
         _check_hip(hip.sdfslam_hip_init_from_scans(ptr(p, _f64p), ptr(o, _f64p), len(p), C.byref(grid),
                                                    int(signed_distance), ptr(out, _f64p)))
     return out
+
+
+NO_ALLOC_MAX_POINTS = 524288  # SDFSLAM_NO_ALLOC_MAX_POINTS
+
+
+def init_from_scans_noalloc(grid, points_dev, origins_dev, num_points, map_dev, signed_distance=True,
+                            stream=None):
+    """sdfslam_hip_init_from_scans_noalloc: device addresses (ints) of the
+    points and origins (x, y pairs) and of the map values, work enqueued on
+    `stream` (a hipStream_t as an int, None = the null stream) with no host
+    synchronisation."""
+    _check_hip(hip.sdfslam_hip_init_from_scans_noalloc(C.c_void_p(points_dev), C.c_void_p(origins_dev),
+                                                       num_points, C.byref(grid), int(signed_distance),
+                                                       C.c_void_p(map_dev), C.c_void_p(stream)))
 
 
 class Dataset:
