@@ -146,6 +146,11 @@ int sdfslam_hip_context_create(const sdfslam_problem_view* problem,
                                const sdfslam_solver_options* options, const double* map_values,
                                const double* poses, sdfslam_hip_context** out);
 int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx);
+/* Frees the device and pinned host buffers a context outgrew (its pools keep
+ * them when a layout grows, because freeing device memory waits for the
+ * whole device). No counterpart in the reference; a server calls it when it
+ * needs the memory. Waits for the device; the context stays usable. */
+int sdfslam_hip_context_trim(sdfslam_hip_context* ctx);
 /* Replaces the context's problem and state with a new one (the next
  * increment of an incremental run: the problem rebuilt over more frames, as
  * runner.cpp:249-260 rebuilds it before each Solve), keeping the context's

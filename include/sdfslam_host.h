@@ -194,9 +194,13 @@ int sdfslam_host_dataset_frame(const sdfslam_host_dataset* dataset, int32_t fram
  * session's own device context. Continuous batching: a scheduler takes every
  * idle session with a waiting frame (up to max_batch), prepares them on
  * `workers` host threads and solves them in one sdfslam_hip_batch_solve;
- * `pipelines` batches are in flight at once. Sessions are admitted while the
- * device memory of all sessions stays within device_budget_bytes (0: 80 %
- * of the device). */
+ * `pipelines` batches are in flight at once; sessions with different step
+ * options are solved in different batches. A session is admitted while the
+ * device memory of the admitted sessions stays within device_budget_bytes
+ * (0: 80 % of the device): each session's exact context bytes once it has
+ * solved, a reservation of 64 MiB + 4 KiB per grid node before. A session
+ * that does not fit is queued, its frames accepted, and admitted in order
+ * as closed sessions free their memory. */
 typedef struct sdfslam_server sdfslam_server;
 typedef struct sdfslam_server_options {
   int32_t workers;
@@ -212,7 +216,9 @@ int sdfslam_server_create(const sdfslam_server_options* options, sdfslam_server*
 /* Waits for the increments in flight, then frees every session. */
 void sdfslam_server_destroy(sdfslam_server* server);
 /* Opens a session at its first frame. map_values: nx*ny initial values
- * (NULL = 0). Refused admission returns SDFSLAM_ERR_OOM. */
+ * (NULL = 0). The session is admitted at once or queued (see above); a
+ * session whose reservation alone exceeds the budget is refused with
+ * SDFSLAM_ERR_OOM. */
 int sdfslam_server_open_session(sdfslam_server* server, const sdfslam_grid* grid,
                                 const double* map_values, const sdfslam_problem_options* problem,
                                 const sdfslam_solver_options* step, const double* first_pose,
@@ -225,6 +231,17 @@ int sdfslam_server_submit(sdfslam_server* server, int32_t session, const double*
  * (may be NULL) receives completion minus submission time. */
 int sdfslam_server_wait(sdfslam_server* server, int32_t session, int64_t ticket,
                         sdfslam_increment_record* record, double* latency_seconds);
+/* Closes a session: waits for its submitted increments (a session still
+ * queued for admission drops them, and waiting on them fails), frees its
+ * device memory and admits queued sessions that now fit. Its state stays
+ * readable; later submits fail. */
+int sdfslam_server_close_session(sdfslam_server* server, int32_t session);
+/* Admission and progress of a session: admitted (0: queued), closed, the
+ * device bytes counted for it, increments submitted and finished. Any
+ * output may be NULL. */
+int sdfslam_server_session_info(sdfslam_server* server, int32_t session, int32_t* admitted,
+                                int32_t* closed, int64_t* device_bytes, int64_t* submitted,
+                                int64_t* finished);
 /* The session's estimate after its finished increments: *frames poses
  * (3 each, at most capacity frames) and nx*ny map values (may be NULL). */
 int sdfslam_server_state(sdfslam_server* server, int32_t session, int32_t capacity,

@@ -150,6 +150,15 @@ class EpochPool {
   }
   /// Device memory held, outgrown allocations included.
   size_t bytes() const { return cap_ + retired_bytes_; }
+  /// Frees the outgrown allocations (no work of the pool's owner may be in
+  /// flight; hipFree waits for the device).
+  void Trim() {
+    for (void* p : retired_dev_) (void)hipFree(p);
+    for (void* p : retired_host_) (void)hipHostFree(p);
+    retired_dev_.clear();
+    retired_host_.clear();
+    retired_bytes_ = 0;
+  }
   void Release() {
     if (dev_ != nullptr) (void)hipFree(dev_);
     if (host_ != nullptr) (void)hipHostFree(host_);
@@ -374,6 +383,18 @@ struct BatchSpace {
     plan_capacity = want;
   }
   std::vector<void*> retired_dev, retired_host;
+
+  /// Frees the outgrown buffers (no batch led by the owner in flight).
+  void Trim() {
+    for (void* p : retired_dev) {
+      if (p != nullptr) (void)hipFree(p);
+    }
+    for (void* p : retired_host) {
+      if (p != nullptr) (void)hipHostFree(p);
+    }
+    retired_dev.clear();
+    retired_host.clear();
+  }
 
   /// Frees everything; errors are ignored so that this can run from a
   /// destructor.
@@ -1549,6 +1570,18 @@ int sdfslam_hip_pose_margins(sdfslam_hip_context* ctx, int32_t* margins, int32_t
         margins[P] = static_cast<int>(pm.size()) == n ? pm[static_cast<size_t>(P)] : ctx->aopt.margin;
       }
     }
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_context_trim(sdfslam_hip_context* ctx) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null context");
+    Bind(*ctx);
+    HIPC(hipStreamSynchronize(ctx->stream));
+    ctx->epoch.Trim();
+    ctx->base.Trim();
+    ctx->batch.Trim();
     return SDFSLAM_OK;
   });
 }

@@ -686,6 +686,29 @@ int sdfslam_server_wait(sdfslam_server* server, int32_t session, int64_t ticket,
   });
 }
 
+int sdfslam_server_close_session(sdfslam_server* server, int32_t session) {
+  return Guard([&]() -> int {
+    if (server == nullptr) throw std::invalid_argument("null argument");
+    server->server->CloseSession(session);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_server_session_info(sdfslam_server* server, int32_t session, int32_t* admitted,
+                                int32_t* closed, int64_t* device_bytes, int64_t* submitted,
+                                int64_t* finished) {
+  return Guard([&]() -> int {
+    if (server == nullptr) throw std::invalid_argument("null argument");
+    const Server::SessionInfo i = server->server->Info(session);
+    if (admitted != nullptr) *admitted = i.admitted ? 1 : 0;
+    if (closed != nullptr) *closed = i.closed ? 1 : 0;
+    if (device_bytes != nullptr) *device_bytes = i.device_bytes;
+    if (submitted != nullptr) *submitted = i.submitted;
+    if (finished != nullptr) *finished = i.finished;
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_server_state(sdfslam_server* server, int32_t session, int32_t capacity,
                          int32_t* frames, double* poses, double* map_values) {
   return Guard([&]() -> int {

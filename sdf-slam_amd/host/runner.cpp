@@ -43,6 +43,15 @@ int64_t HipSession::DeviceBytes() const {
   return info.device_bytes;
 }
 
+void HipSession::Trim() {
+  if (ctx_ != nullptr) Rethrow(sdfslam_hip_context_trim(ctx_));
+}
+
+void HipSession::Release() {
+  if (ctx_ != nullptr) (void)sdfslam_hip_context_destroy(ctx_);
+  ctx_ = nullptr;
+}
+
 sdfslam_hip_context* HipSession::Load(const Problem& problem, const SolverOptions& options) {
   const sdfslam_problem_view view = problem.View();
   const sdfslam_solver_options copts = ToC(options);

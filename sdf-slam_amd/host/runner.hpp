@@ -33,6 +33,10 @@ class HipSession {
   SolveResult Solve(Problem& problem, const SolverOptions& options);
   /// Device memory the context holds (0 before the first Solve).
   int64_t DeviceBytes() const;
+  /// Frees the device buffers the context outgrew (waits for the device).
+  void Trim();
+  /// Destroys the context (a later Solve or Load creates a new one).
+  void Release();
 
   /// The two halves of Solve around a solve of the context, for callers that
   /// solve many sessions' contexts in one batch (sdfslam_hip_batch_solve):

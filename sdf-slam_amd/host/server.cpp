@@ -20,10 +20,20 @@ double Seconds(Clock::time_point a, Clock::time_point b) {
   return std::chrono::duration<double>(b - a).count();
 }
 
-/// Device memory a new session is assumed to need before its first solve:
-/// per-node grid arrays and a first epoch's factor storage.
+/// Device memory reserved for a session before its first solve (its
+/// context does not exist yet): per-node grid arrays and a first epoch's
+/// factor storage. After a solve the context's exact bytes count instead.
 int64_t AdmissionEstimate(const GridMap& map) {
   return (64LL << 20) + 4096LL * map.num_nodes();
+}
+
+/// Step options under which two sessions' increments give the same results
+/// as each alone when solved in one batch (every field SolveLm reads).
+bool SameStep(const SolverOptions& a, const SolverOptions& b) {
+  return a.max_iterations == b.max_iterations && a.step_tolerance == b.step_tolerance &&
+         a.lambda_init == b.lambda_init && a.lambda_factor == b.lambda_factor &&
+         a.reject_worse_steps == b.reject_worse_steps && a.marquardt_scaling == b.marquardt_scaling &&
+         a.trust_region == b.trust_region;
 }
 
 double Percentile(std::vector<double> v, double q) {
@@ -81,16 +91,110 @@ int Server::OpenSession(GridMap map, const ProblemOptions& problem, const Solver
   s->problem = problem;
   s->step = step;
   s->step.backend = SolverBackend::kHip;
-  std::lock_guard<std::mutex> lock(mu_);
-  int64_t used = 0;
-  for (const auto& o : sessions_) used += o->device_bytes;
-  if (used + estimate > options_.device_budget_bytes) {
-    throw std::runtime_error("admission refused: " + std::to_string(used + estimate) +
-                             " bytes of device memory would exceed the budget of " +
+  if (estimate > options_.device_budget_bytes) {
+    throw std::runtime_error("admission refused: the session's " + std::to_string(estimate) +
+                             " bytes of device memory exceed the budget of " +
                              std::to_string(options_.device_budget_bytes));
   }
+  std::unique_lock<std::mutex> lock(mu_);
+  if (waiting_.empty() && Used() + estimate > options_.device_budget_bytes) TrimIdle(lock);
+  Session* raw = s.get();
   sessions_.push_back(std::move(s));
-  return static_cast<int>(sessions_.size()) - 1;
+  const int id = static_cast<int>(sessions_.size()) - 1;
+  waiting_.push_back(raw);
+  AdmitWaiting();
+  return id;
+}
+
+int64_t Server::Used() const {
+  int64_t used = 0;
+  for (const auto& o : sessions_) {
+    if (o->admitted && !o->closed) used += o->device_bytes;
+  }
+  return used;
+}
+
+void Server::AdmitWaiting() {
+  bool any = false;
+  while (!waiting_.empty() && Used() + waiting_.front()->device_bytes <= options_.device_budget_bytes) {
+    waiting_.front()->admitted = true;
+    waiting_.pop_front();
+    any = true;
+  }
+  if (any) work_cv_.notify_all();
+}
+
+void Server::TrimIdle(std::unique_lock<std::mutex>& lock) {
+  std::vector<Session*> idle;
+  for (const auto& o : sessions_) {
+    if (o->admitted && !o->closed && !o->busy && o->device_bytes > 0) {
+      o->busy = true;  // keeps the scheduler off the session meanwhile
+      idle.push_back(o.get());
+    }
+  }
+  lock.unlock();
+  std::vector<int64_t> bytes(idle.size(), 0);
+  for (size_t i = 0; i < idle.size(); ++i) {
+    try {
+      idle[i]->hip.Trim();
+      bytes[i] = idle[i]->hip.DeviceBytes();
+    } catch (const std::exception&) {
+      bytes[i] = 0;  // keep the previous count
+    }
+  }
+  lock.lock();
+  for (size_t i = 0; i < idle.size(); ++i) {
+    if (bytes[i] > 0) idle[i]->device_bytes = bytes[i];
+    idle[i]->busy = false;
+  }
+  done_cv_.notify_all();
+  work_cv_.notify_all();
+}
+
+void Server::CloseSession(int session) {
+  std::unique_lock<std::mutex> lock(mu_);
+  Session& s = *sessions_.at(static_cast<size_t>(session));
+  if (s.closing) throw std::invalid_argument("session already closed");
+  s.closing = true;
+  if (!s.admitted) {
+    // Still queued: nothing ran; its frames are dropped.
+    waiting_.erase(std::remove(waiting_.begin(), waiting_.end(), &s), waiting_.end());
+    while (!s.pending.empty()) {
+      s.pending.pop_front();
+      s.records.push_back({});
+      s.latency.push_back(0.0);
+      s.errors.push_back("session closed before it was admitted");
+    }
+    s.closed = true;
+    done_cv_.notify_all();
+    return;
+  }
+  done_cv_.wait(lock, [&] { return !s.busy && s.pending.empty(); });
+  s.busy = true;
+  lock.unlock();
+  s.hip.Release();
+  lock.lock();
+  s.busy = false;
+  s.closed = true;
+  s.device_bytes = 0;
+  s.scans.clear();
+  s.scans.shrink_to_fit();
+  s.odometry.clear();
+  s.odometry.shrink_to_fit();
+  AdmitWaiting();
+  done_cv_.notify_all();
+}
+
+Server::SessionInfo Server::Info(int session) {
+  std::lock_guard<std::mutex> lock(mu_);
+  const Session& s = *sessions_.at(static_cast<size_t>(session));
+  SessionInfo i;
+  i.admitted = s.admitted;
+  i.closed = s.closed;
+  i.device_bytes = s.device_bytes;
+  i.submitted = s.submitted;
+  i.finished = static_cast<int64_t>(s.records.size());
+  return i;
 }
 
 int64_t Server::Submit(int session, const Scan& scan, const Pose2& odometry, Clock::time_point arrival) {
@@ -98,6 +202,7 @@ int64_t Server::Submit(int session, const Scan& scan, const Pose2& odometry, Clo
   {
     std::lock_guard<std::mutex> lock(mu_);
     Session& s = *sessions_.at(static_cast<size_t>(session));
+    if (s.closing) throw std::invalid_argument("session is closed");
     s.pending.push_back({scan, odometry, arrival});
     ticket = ++s.submitted;
   }
@@ -134,9 +239,7 @@ void Server::State(int session, std::vector<Pose2>& poses, GridMap& map) {
 
 int64_t Server::DeviceBytes() {
   std::lock_guard<std::mutex> lock(mu_);
-  int64_t used = 0;
-  for (const auto& s : sessions_) used += s->device_bytes;
-  return used;
+  return Used();
 }
 
 void Server::Prepare(Prepared& item, Pending& frame) {
@@ -160,7 +263,7 @@ void Server::Prepare(Prepared& item, Pending& frame) {
 
 void Server::SolveBatch(std::vector<std::unique_ptr<Prepared>>& batch) {
   const auto nl = static_cast<int>(batch.size());
-  // All sessions of a server share the step options.
+  // The sessions of a batch share their step options (SolveLoop).
   const SolverOptions& step = batch[0]->session->step;
   const sdfslam_solver_options copts = ToC(step);
   std::vector<sdfslam_hip_context*> ctxs(static_cast<size_t>(nl));
@@ -218,9 +321,10 @@ void Server::Finish(Prepared& item, Clock::time_point done, int64_t device_bytes
   s.records.push_back(item.record);
   s.latency.push_back(Seconds(item.arrival, done));
   s.errors.push_back(item.error);
-  if (device_bytes > 0) s.device_bytes = device_bytes;
+  if (device_bytes > 0) s.device_bytes = device_bytes;  // exact bytes replace the reservation
   s.busy = false;
   --in_flight_;
+  AdmitWaiting();
 }
 
 ServerStats Server::Stats() {
@@ -238,7 +342,7 @@ void Server::PrepareLoop() {
       if (options_.max_in_flight > 0 && in_flight_ >= options_.max_in_flight) return false;
       // The idle session with the oldest waiting frame.
       for (const auto& s : sessions_) {
-        if (!s->busy && !s->pending.empty() &&
+        if (s->admitted && !s->busy && !s->pending.empty() &&
             (pick == nullptr || s->pending.front().arrival < pick->pending.front().arrival)) {
           pick = s.get();
         }
@@ -277,12 +381,20 @@ void Server::SolveLoop() {
   while (true) {
     solve_cv_.wait(lock, [&] { return stop_solvers_ || !prepared_.empty(); });
     if (prepared_.empty()) return;  // stopping, nothing left to solve
-    // Every prepared increment, oldest first (preparation order), up to max_batch.
+    // The oldest prepared increment and every later one with the same step
+    // options, in preparation order, up to max_batch; the others wait for
+    // the next batch.
     std::vector<std::unique_ptr<Prepared>> batch;
-    while (!prepared_.empty() && static_cast<int>(batch.size()) < options_.max_batch) {
-      batch.push_back(std::move(prepared_.front()));
-      prepared_.pop_front();
+    for (auto it = prepared_.begin();
+         it != prepared_.end() && static_cast<int>(batch.size()) < options_.max_batch;) {
+      if (batch.empty() || SameStep(batch[0]->session->step, (*it)->session->step)) {
+        batch.push_back(std::move(*it));
+        it = prepared_.erase(it);
+      } else {
+        ++it;
+      }
     }
+    if (!prepared_.empty()) solve_cv_.notify_one();
     lock.unlock();
     SolveBatch(batch);
     const auto done = Clock::now();

@@ -17,8 +17,15 @@
 // session's increments run one at a time in submission order, and a
 // session's results do not depend on what else is being served.
 //
-// Admission: a session is opened only while the device memory of all open
-// sessions plus an estimate for the new one stays within the budget.
+// Admission: a session is admitted while the device memory of the admitted
+// sessions plus its own stays within the budget: a session's exact device
+// bytes (its context's pools) once it has solved, a per-node reservation
+// before. A session that does not fit waits in a queue (its frames are
+// accepted and wait with it) after the server has freed what the idle
+// sessions' contexts outgrew; queued sessions are admitted in order as
+// closed sessions free their memory. A session whose reservation alone
+// exceeds the budget is refused. Sessions with different step options are
+// never solved in the same batch.
 #pragma once
 
 #include <condition_variable>
@@ -63,11 +70,25 @@ class Server {
   Server(const Server&) = delete;
   Server& operator=(const Server&) = delete;
 
-  /// Opens a session at its first frame (pose and scan, no solve yet).
-  /// Throws std::runtime_error("admission refused: ...") when the device
-  /// memory budget would be exceeded.
+  /// Opens a session at its first frame (pose and scan, no solve yet),
+  /// admitted at once or queued until memory frees up. Throws
+  /// std::runtime_error("admission refused: ...") when the session alone
+  /// would exceed the device memory budget.
   int OpenSession(GridMap map, const ProblemOptions& problem, const SolverOptions& step,
                   const Pose2& first_pose, const Scan& first_scan);
+  /// Closes a session: waits for its submitted increments (a session still
+  /// queued for admission drops them; their Wait fails), frees its device
+  /// context and admits queued sessions that now fit. Its last estimate
+  /// stays readable with State; Submit fails from now on.
+  void CloseSession(int session);
+  struct SessionInfo {
+    bool admitted{false};
+    bool closed{false};
+    int64_t device_bytes{0};  // counted against the budget
+    int64_t submitted{0};
+    int64_t finished{0};
+  };
+  SessionInfo Info(int session);
   /// Queues the next frame of a session (scan and odometry relative to the
   /// previous frame); `arrival` is the steady-clock time latency counts from.
   /// Returns the increment's ticket (1, 2, ... per session).
@@ -106,6 +127,9 @@ class Server {
     std::vector<double> latency;
     std::vector<std::string> errors;
     int64_t device_bytes{0};
+    bool admitted{false};
+    bool closing{false};
+    bool closed{false};
   };
 
   /// One increment between its preparation and its solve.
@@ -128,6 +152,13 @@ class Server {
   void SolveBatch(std::vector<std::unique_ptr<Prepared>>& batch);
   /// Records a finished increment (under mu_).
   void Finish(Prepared& item, std::chrono::steady_clock::time_point done, int64_t device_bytes);
+  /// Device bytes counted for the admitted, open sessions (under mu_).
+  int64_t Used() const;
+  /// Admits queued sessions in order while they fit (under mu_).
+  void AdmitWaiting();
+  /// Frees what the idle admitted sessions' contexts outgrew; takes and
+  /// releases `lock` (on mu_) around the device work.
+  void TrimIdle(std::unique_lock<std::mutex>& lock);
 
   ServerOptions options_;
   std::mutex mu_;
@@ -135,6 +166,7 @@ class Server {
   std::condition_variable solve_cv_;
   std::condition_variable done_cv_;
   std::vector<std::unique_ptr<Session>> sessions_;
+  std::deque<Session*> waiting_;  // sessions queued for admission, in order
   std::deque<std::unique_ptr<Prepared>> prepared_;
   std::vector<std::thread> preparers_;
   std::vector<std::thread> solvers_;

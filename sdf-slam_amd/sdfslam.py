@@ -166,7 +166,8 @@ SOLVE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(ProblemView), C.POINTER(So
 
 HIP_SYMBOLS = [
     "sdfslam_hip_status_string", "sdfslam_hip_last_error", "sdfslam_hip_solve",
-    "sdfslam_hip_context_create", "sdfslam_hip_context_destroy", "sdfslam_hip_set_state",
+    "sdfslam_hip_context_create", "sdfslam_hip_context_destroy", "sdfslam_hip_context_trim",
+    "sdfslam_hip_set_state",
     "sdfslam_hip_get_state", "sdfslam_hip_analysis", "sdfslam_hip_assemble",
     "sdfslam_hip_normal_equations", "sdfslam_hip_linear_step", "sdfslam_hip_cost",
     "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
@@ -183,7 +184,8 @@ HOST_SYMBOLS = [
     "sdfslam_host_dataset_create", "sdfslam_host_dataset_destroy", "sdfslam_host_dataset_info",
     "sdfslam_host_dataset_frame", "sdfslam_server_create", "sdfslam_server_destroy",
     "sdfslam_server_open_session", "sdfslam_server_submit", "sdfslam_server_wait",
-    "sdfslam_server_state", "sdfslam_host_served_benchmark", "sdfslam_host_batch_benchmark",
+    "sdfslam_server_state", "sdfslam_server_close_session", "sdfslam_server_session_info",
+    "sdfslam_host_served_benchmark", "sdfslam_host_batch_benchmark",
     "sdfslam_host_node_normals", "sdfslam_host_run", "sdfslam_host_init_from_scans",
     "sdfslam_host_batch_autotune",
 ]
@@ -208,6 +210,7 @@ hip.sdfslam_hip_solve.argtypes = [C.POINTER(ProblemView), C.POINTER(SolverOption
 hip.sdfslam_hip_context_create.argtypes = [C.POINTER(ProblemView), C.POINTER(SolverOptions),
                                            _f64p, _f64p, C.POINTER(C.c_void_p)]
 hip.sdfslam_hip_context_destroy.argtypes = [C.c_void_p]
+hip.sdfslam_hip_context_trim.argtypes = [C.c_void_p]
 hip.sdfslam_hip_set_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_get_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_analysis.argtypes = [C.c_void_p, C.POINTER(AnalysisInfo)]
@@ -269,6 +272,8 @@ host.sdfslam_server_submit.argtypes = [C.c_void_p, C.c_int32, _f64p, C.c_int64, 
 host.sdfslam_server_wait.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.POINTER(IncrementRecord),
                                      _f64p]
 host.sdfslam_server_state.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, _f64p, _f64p]
+host.sdfslam_server_close_session.argtypes = [C.c_void_p, C.c_int32]
+host.sdfslam_server_session_info.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _i64p, _i64p, _i64p]
 host.sdfslam_host_served_benchmark.argtypes = [C.POINTER(ServedOptions), C.POINTER(ServedResult)]
 host.sdfslam_host_batch_benchmark.argtypes = [C.POINTER(BatchOptions), C.POINTER(BatchResult)]
 host.sdfslam_host_batch_autotune.argtypes = [C.POINTER(BatchOptions), _i32p, C.c_int32, _i32p, C.c_int32, C.c_int32,
@@ -757,6 +762,19 @@ class Server:
         out = {k: getattr(rec, k) for k, _ in IncrementRecord._fields_}
         out["latency_seconds"] = lat.value
         return out
+
+    def close_session(self, session):
+        """Waits for the session's increments, frees its device memory and
+        admits queued sessions that now fit (sdfslam_server_close_session)."""
+        _check_host(host.sdfslam_server_close_session(self._h, session))
+
+    def session_info(self, session):
+        a, c = C.c_int32(), C.c_int32()
+        b, sub, fin = C.c_int64(), C.c_int64(), C.c_int64()
+        _check_host(host.sdfslam_server_session_info(self._h, session, C.byref(a), C.byref(c), C.byref(b),
+                                                     C.byref(sub), C.byref(fin)))
+        return dict(admitted=bool(a.value), closed=bool(c.value), device_bytes=b.value,
+                    submitted=sub.value, finished=fin.value)
 
     def state(self, session):
         frames = C.c_int32()

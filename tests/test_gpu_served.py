@@ -94,3 +94,73 @@ def test_batch_autotune_picks_its_fastest_trial(sdfslam):
     assert [(t[0], t[1]) for t in trials] == [(1, 1), (1, 2), (2, 1), (2, 2)]
     assert all(t[2] > 0 for t in trials)
     assert best == max(trials, key=lambda t: t[2])
+
+
+def test_sessions_with_different_step_options(sdfslam):
+    # Two sessions with their own max_iterations and lambda_factor in one
+    # server: each ends as its own incremental run with its options (they
+    # are never batched together).
+    print("case served per-session options", flush=True)
+    frames = 6
+    d = [sdfslam.Dataset(f"served{i}", 2000) for i in (3, 4)]
+    opts = [d[0].step_options, d[1].step_options]
+    opts[1].max_iterations = 3
+    opts[1].lambda_factor = 2.0
+    server = sdfslam.Server(workers=4)
+    sids = []
+    for di, o in zip(d, opts):
+        xy, pose, _ = di.frame(0)
+        sids.append(server.open_session(di.grid, di.problem_options, o, pose, xy))
+    tickets = []
+    for k in range(1, frames):
+        for di, sid in zip(d, sids):
+            xy, _, odo = di.frame(k)
+            tickets.append((sid, server.submit(sid, xy, odo)))
+    recs = {sid: [] for sid in sids}
+    for sid, t in tickets:
+        recs[sid].append(server.wait(sid, t))
+    assert all(r["iterations"] <= 3 for r in recs[sids[1]])
+    for i, (sid, o) in enumerate(zip(sids, opts)):
+        p, m = server.state(sid)
+        run = sdfslam.run_incremental(f"served{(3, 4)[i]}", 2000, options=o, max_frames=frames)
+        assert np.array_equal(run["poses"], p) and np.array_equal(run["map"], m), i
+        assert [r["iterations"] for r in recs[sid]] == [r["iterations"] for r in run["records"]]
+    server.close()
+
+
+def test_queued_session_is_admitted_after_another_closes(sdfslam):
+    # A budget that holds one session: the second waits in the queue with
+    # its frames accepted, and is admitted and served once the first closes.
+    print("case served admission queue", flush=True)
+    d = [sdfslam.Dataset(f"served{i}", 2000) for i in (0, 1, 2)]
+    reserve = max((64 << 20) + 4096 * di.grid.nx * di.grid.ny for di in d)
+    server = sdfslam.Server(workers=2, device_budget_bytes=reserve + (1 << 20))
+    sids = []
+    for di in d:
+        xy, pose, _ = di.frame(0)
+        sids.append(server.open_session(di.grid, di.problem_options, di.step_options, pose, xy))
+    a, b, c = sids
+    assert server.session_info(a)["admitted"]
+    assert not server.session_info(b)["admitted"] and not server.session_info(c)["admitted"]
+    tb = [server.submit(b, *(lambda f: (f[0], f[2]))(d[1].frame(k))) for k in (1, 2, 3)]
+    tc = server.submit(c, *(lambda f: (f[0], f[2]))(d[2].frame(1)))
+    ta = [server.submit(a, *(lambda f: (f[0], f[2]))(d[0].frame(k))) for k in (1, 2)]
+    for t in ta:
+        server.wait(a, t)
+    info = server.session_info(b)
+    assert not info["admitted"] and info["finished"] == 0 and info["submitted"] == 3
+    # Closing the queued third session drops its frame.
+    server.close_session(c)
+    with pytest.raises(Exception, match="closed before it was admitted"):
+        server.wait(c, tc)
+    server.close_session(a)
+    assert server.session_info(a)["closed"] and server.session_info(a)["device_bytes"] == 0
+    assert server.session_info(b)["admitted"]
+    for t in tb:
+        server.wait(b, t)
+    p, m = server.state(b)
+    run = sdfslam.run_incremental("served1", 2000, max_frames=4)
+    assert np.array_equal(run["poses"], p) and np.array_equal(run["map"], m)
+    with pytest.raises(ValueError, match="closed"):
+        server.submit(a, *(lambda f: (f[0], f[2]))(d[0].frame(3)))
+    server.close()
