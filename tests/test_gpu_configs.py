@@ -114,7 +114,7 @@ def test_office150_jitter_ensemble_matches_cpu_replicates(sdfslam):
     full = sdfslam.Problem.workload(name, seed)
     o, _, per_increment, gt = full.workload_info()
     o.max_iterations = per_increment
-    errs = []
+    errs, same_run = [], []
     for i, lam in enumerate(fx["ensemble_lambda_init"]):
         assert lam == ensemble.jittered_lambda(fx["ensemble_lambda_init"][0], i, float(fx["jitter"]))
         o.lambda_init = float(lam)
@@ -122,12 +122,17 @@ def test_office150_jitter_ensemble_matches_cpu_replicates(sdfslam):
         errs.append(ensemble.mean_translation_error(run["poses"], gt[:frames]))
         print(f"  run {i}: mean translation error {errs[-1]:.5f} m (cpu {fx['ensemble_error'][i]:.5f})",
               flush=True)
+        same_run.append(_check_state(f"run {i} final poses", run["poses"], fx["ensemble_poses"][i]))
     dev = ensemble.summarize(errs)
     cpu = ensemble.summarize(fx["ensemble_error"])
     print(f"  device {dev}\n  cpu    {cpu}", flush=True)
+    # The distributions agree (how chaotic runs are judged) ...
     assert dev["diverged"] == cpu["diverged"]
     assert 0.8 * cpu["median"] <= dev["median"] <= 1.25 * cpu["median"]
     assert dev["worst"] <= 1.5 * cpu["worst"]
+    # ... and over these 300 frames the runs are not chaotic: each device run
+    # ends where its CPU replicate ends.
+    assert all(same_run), same_run
 
 
 def _batch_names():
