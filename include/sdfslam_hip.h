@@ -205,6 +205,18 @@ typedef struct sdfslam_hip_analysis_info {
 } sdfslam_hip_analysis_info;
 int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* info);
 
+/* The factorization's size without this backend's padding: the symbolic
+ * LDL^T of the reference's J^T J pattern at the current state (every pair of
+ * columns sharing a Jacobian row, plus the diagonal; lm_solver.cpp:113-172,
+ * 321-330) under the context's elimination order. factor_nnz: entries of L
+ * below the diagonal; factor_flops: sum over columns of c^2 + 3c (the
+ * count of the analysis' factor_flops, which includes the superset
+ * pattern's explicit zeros and the supernodes' dense blocks). */
+int sdfslam_hip_reference_factor_counts(sdfslam_hip_context* ctx, int64_t* factor_nnz, double* factor_flops);
+/* The context's current elimination order: order[k] = the column (in the
+ * reference's column layout) eliminated k-th; num_parameters entries. */
+int sdfslam_hip_elimination_order(sdfslam_hip_context* ctx, int32_t* order);
+
 /* Residual vector and Jacobian rows at the current state
  * (AssemblePointRows/AssembleEikonalRows/odometry tail, lm_solver.cpp:30-82,
  * 275-296). Rows are in reference order; each row has 7 slots of

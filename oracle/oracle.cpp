@@ -4,14 +4,39 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <limits>
 #include <cstring>
 #include <functional>
+#include <thread>
 #include <vector>
 
+#include "ordering.hpp"
+
 namespace {
+
+// Host threads of the threaded phases (assembly chunks, J^T J column ranges,
+// -J^T r, LDL^T subtrees); 1 runs everything serially. Every threaded phase
+// gives the serial result bit for bit (lm_solver.cpp:217-219, 237-274,
+// 84-105; parallel_ldlt.hpp:11-30).
+int g_threads = 1;
+
+// Runs fn(t) for t in [0, threads), t = 0 on the calling thread.
+template <typename F>
+void ForThreads(int threads, F&& fn) {
+  if (threads <= 1) {
+    fn(0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back([&fn, t] { fn(t); });
+  fn(0);
+  for (std::thread& th : pool) th.join();
+}
+
+int64_t ChunkBound(int64_t n, int chunks, int k) { return n * k / chunks; }
 
 struct Grid {
   int nx, ny;
@@ -315,14 +340,10 @@ double HuberWeight(double huber_delta, double weighted) {
   return abs_r <= huber_delta ? 1.0 : huber_delta / abs_r;
 }
 
-void Assemble(const sdfslam_problem_view* p, const double* map, const double* poses,
-              double* residuals, Csc& jac) {
-  const Grid g(p->grid);
-  std::vector<Triplet> trip;
-  trip.reserve(static_cast<size_t>(p->num_points) * 7 + static_cast<size_t>(p->num_eikonal) * 3 +
-               static_cast<size_t>(p->num_odom) * 18);
-  // AssemblePointRows (lm_solver.cpp:30-61).
-  for (int64_t s = 0; s < p->num_points; ++s) {
+// AssemblePointRows (lm_solver.cpp:30-61) for point rows [begin, end).
+void AssemblePointRows(const sdfslam_problem_view* p, const Grid& g, const double* map, const double* poses,
+                       int64_t begin, int64_t end, double* residuals, std::vector<Triplet>& trip) {
+  for (int64_t s = begin; s < end; ++s) {
     const int frame = p->point_frame[s];
     const PointEval e = EvalPoint(g, map, poses + 3 * frame, p->point_x[s], p->point_y[s],
                                   p->point_expected_sdf[s], p->smooth_gradient != 0,
@@ -348,8 +369,12 @@ void Assemble(const sdfslam_problem_view* p, const double* map, const double* po
       }
     }
   }
-  // AssembleEikonalRows (lm_solver.cpp:67-82).
-  for (int64_t e = 0; e < p->num_eikonal; ++e) {
+}
+
+// AssembleEikonalRows (lm_solver.cpp:67-82) for eikonal rows [begin, end).
+void AssembleEikonalRows(const sdfslam_problem_view* p, const Grid& g, const double* map, int64_t begin,
+                         int64_t end, double* residuals, std::vector<Triplet>& trip) {
+  for (int64_t e = begin; e < end; ++e) {
     double r;
     int ids[3];
     double d[3];
@@ -364,6 +389,31 @@ void Assemble(const sdfslam_problem_view* p, const double* map, const double* po
       }
     }
   }
+}
+
+// Residuals and J (lm_solver.cpp:232-313): point and eikonal rows in
+// g_threads contiguous chunks each (lm_solver.cpp:237-274), their triplets
+// concatenated in chunk order, then the odometry tail, serially.
+void Assemble(const sdfslam_problem_view* p, const double* map, const double* poses,
+              double* residuals, Csc& jac) {
+  const Grid g(p->grid);
+  const int chunks = std::max(1, g_threads);
+  std::vector<std::vector<Triplet>> parts(2 * static_cast<size_t>(chunks));
+  ForThreads(chunks, [&](int t) {
+    std::vector<Triplet>& pt = parts[static_cast<size_t>(t)];
+    const int64_t b = ChunkBound(p->num_points, chunks, t);
+    const int64_t e = ChunkBound(p->num_points, chunks, t + 1);
+    pt.reserve(static_cast<size_t>(e - b) * 7);
+    AssemblePointRows(p, g, map, poses, b, e, residuals, pt);
+    std::vector<Triplet>& et = parts[static_cast<size_t>(chunks + t)];
+    AssembleEikonalRows(p, g, map, ChunkBound(p->num_eikonal, chunks, t),
+                        ChunkBound(p->num_eikonal, chunks, t + 1), residuals, et);
+  });
+  size_t total = static_cast<size_t>(p->num_odom) * 18;
+  for (const auto& part : parts) total += part.size();
+  std::vector<Triplet> trip;
+  trip.reserve(total);
+  for (const auto& part : parts) trip.insert(trip.end(), part.begin(), part.end());
   // Odometry tail (lm_solver.cpp:275-296).
   for (int64_t o = 0; o < p->num_odom; ++o) {
     const int fi = p->odom_i[o];
@@ -390,18 +440,47 @@ void Assemble(const sdfslam_problem_view* p, const double* map, const double* po
   jac = FromTriplets(NumRows(p), NumParams(p), trip);
 }
 
+void NormalColumns(const Csc& j, const Csc& jt, int64_t begin, int64_t end, Csc& out);
+
 // ParallelAtA (lm_solver.cpp:113-172): column j accumulates rows k of
 // column j ascending, inner i ascending within row k; first product assigned.
+// Column ranges run on g_threads threads (lm_solver.cpp:84-105); each column
+// keeps the serial accumulation order.
 Csc NormalMatrix(const Csc& j) {
   const Csc jt = Transpose(j);
+  const int chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(g_threads, j.cols)));
+  std::vector<Csc> parts(static_cast<size_t>(chunks));
+  ForThreads(chunks, [&](int t) {
+    NormalColumns(j, jt, ChunkBound(j.cols, chunks, t), ChunkBound(j.cols, chunks, t + 1),
+                  parts[static_cast<size_t>(t)]);
+  });
   Csc out;
   out.rows = j.cols;
   out.cols = j.cols;
   out.col_ptr.assign(static_cast<size_t>(j.cols) + 1, 0);
+  size_t nnz = 0;
+  for (const Csc& part : parts) nnz += part.row_idx.size();
+  out.row_idx.reserve(nnz);
+  out.values.reserve(nnz);
+  int64_t col = 0;
+  for (const Csc& part : parts) {
+    const int64_t base = static_cast<int64_t>(out.row_idx.size());
+    for (size_t c = 0; c + 1 < part.col_ptr.size(); ++c) {
+      out.col_ptr[static_cast<size_t>(++col)] = base + part.col_ptr[c + 1];
+    }
+    out.row_idx.insert(out.row_idx.end(), part.row_idx.begin(), part.row_idx.end());
+    out.values.insert(out.values.end(), part.values.begin(), part.values.end());
+  }
+  return out;
+}
+
+// Columns [begin, end) of J^T J into `out` (col_ptr relative to the range).
+void NormalColumns(const Csc& j, const Csc& jt, int64_t begin, int64_t end, Csc& out) {
+  out.col_ptr.assign(static_cast<size_t>(end - begin) + 1, 0);
   std::vector<char> mask(static_cast<size_t>(j.cols), 0);
   std::vector<double> values(static_cast<size_t>(j.cols), 0.0);
   std::vector<int32_t> indices;
-  for (int64_t col = 0; col < j.cols; ++col) {
+  for (int64_t col = begin; col < end; ++col) {
     indices.clear();
     for (int64_t q = j.col_ptr[static_cast<size_t>(col)];
          q < j.col_ptr[static_cast<size_t>(col) + 1]; ++q) {
@@ -426,9 +505,8 @@ Csc NormalMatrix(const Csc& j) {
       out.values.push_back(values[static_cast<size_t>(i)]);
       mask[static_cast<size_t>(i)] = 0;
     }
-    out.col_ptr[static_cast<size_t>(col) + 1] = static_cast<int64_t>(out.row_idx.size());
+    out.col_ptr[static_cast<size_t>(col - begin) + 1] = static_cast<int64_t>(out.row_idx.size());
   }
-  return out;
 }
 
 // lm_solver.cpp:321-330: coeffRef inserts a missing diagonal entry.
@@ -473,9 +551,18 @@ void Damp(Csc& a, double lambda, bool marquardt) {
   a = std::move(out);
 }
 
+void NegAtbColumns(const Csc& j, const double* r, double* rhs, int64_t begin, int64_t end);
+
 // ParallelNegAtb (lm_solver.cpp:178-197): two alternating accumulators.
 void NegAtb(const Csc& j, const double* r, double* rhs) {
-  for (int64_t col = 0; col < j.cols; ++col) {
+  const int chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(g_threads, j.cols)));
+  ForThreads(chunks, [&](int t) {
+    NegAtbColumns(j, r, rhs, ChunkBound(j.cols, chunks, t), ChunkBound(j.cols, chunks, t + 1));
+  });
+}
+
+void NegAtbColumns(const Csc& j, const double* r, double* rhs, int64_t begin, int64_t end) {
+  for (int64_t col = begin; col < end; ++col) {
     double tmp_a = 0.0;
     double tmp_b = 0.0;
     int64_t p = j.col_ptr[static_cast<size_t>(col)];
@@ -529,6 +616,18 @@ void Order(const sdfslam_problem_view* p, int32_t* order) {
     order[k++] = static_cast<int32_t>(c);
   }
 }
+
+bool LdltRow(int k, int n, const std::vector<int64_t>& cp, const std::vector<int>& ci,
+             const std::vector<double>& cx, const std::vector<int>& parent, const std::vector<int64_t>& lp,
+             std::vector<int>& li, std::vector<double>& lx, std::vector<double>& diag, std::vector<int64_t>& nzc,
+             std::vector<double>& y, std::vector<int>& pattern, std::vector<int>& tags);
+void SubtreeSchedule(const std::vector<int>& parent, const std::vector<int64_t>& lnz, int threads,
+                     std::vector<std::vector<int>>& rows_of, std::vector<int>& trunk);
+
+// Statistics of the last factorization: nonzeros of L below the diagonal
+// and sum over columns of c^2 + 3c (c = a column's count below the diagonal).
+double g_last_factor_flops = 0.0;
+int64_t g_last_factor_nnz = 0;
 
 // LDL^T of P A P^T with the up-looking row kernel of parallel_ldlt.hpp:165-206
 // (Eigen's factorize_preordered), then Eigen's SimplicialLDLT::solve order.
@@ -605,15 +704,79 @@ int LdltSolve(const Csc& a, const int32_t* order, const double* b, double* x, in
   if (lnz_out != nullptr) {
     *lnz_out = lp[static_cast<size_t>(n)];
   }
+  {
+    double flops = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double c = static_cast<double>(lnz[static_cast<size_t>(k)]);
+      flops += c * c + 3.0 * c;
+    }
+    g_last_factor_flops = flops;
+    g_last_factor_nnz = lp[static_cast<size_t>(n)];
+  }
   std::vector<int> li(static_cast<size_t>(lp[static_cast<size_t>(n)]));
   std::vector<double> lx(li.size());
   std::vector<double> diag(static_cast<size_t>(n));
-  std::vector<double> y(static_cast<size_t>(n), 0.0);
-  std::vector<int> pattern(static_cast<size_t>(n), 0);
-  std::vector<int> tags(static_cast<size_t>(n), -1);
   std::vector<int64_t> nzc(static_cast<size_t>(n), 0);
+  // Rows over elimination-tree subtrees (parallel_ldlt.hpp:11-30, 223-239):
+  // row k reads and appends to columns that are its descendants, so disjoint
+  // subtrees run on separate threads and their common ancestors (the trunk)
+  // after them; every column receives its entries in ascending row order,
+  // as serially, so the factor is the serial one bit for bit.
+  std::vector<std::vector<int>> rows_of;
+  std::vector<int> trunk;
+  SubtreeSchedule(parent, lnz, g_threads, rows_of, trunk);
+  std::atomic<bool> ok{true};
+  auto run_rows = [&](const std::vector<int>& rows) {
+    std::vector<double> y(static_cast<size_t>(n), 0.0);
+    std::vector<int> pattern(static_cast<size_t>(n), 0);
+    std::vector<int> tags(static_cast<size_t>(n), -1);
+    for (const int k : rows) {
+      if (!ok.load(std::memory_order_relaxed)) return;
+      if (!LdltRow(k, n, cp, ci, cx, parent, lp, li, lx, diag, nzc, y, pattern, tags)) {
+        ok.store(false);
+        return;
+      }
+    }
+  };
+  ForThreads(static_cast<int>(rows_of.size()), [&](int t) { run_rows(rows_of[static_cast<size_t>(t)]); });
+  if (ok.load()) run_rows(trunk);
+  if (!ok.load()) return SDFSLAM_ERR_NOT_SPD;
+  // x = P^T L^-T D^-1 L^-1 P b
+  std::vector<double> t(static_cast<size_t>(n));
   for (int k = 0; k < n; ++k) {
-    // parallel_ldlt.hpp:165-206
+    t[static_cast<size_t>(k)] = b[perm[static_cast<size_t>(k)]];
+  }
+  for (int j = 0; j < n; ++j) {
+    const double tj = t[static_cast<size_t>(j)];
+    for (int64_t q = lp[static_cast<size_t>(j)]; q < lp[static_cast<size_t>(j) + 1]; ++q) {
+      t[static_cast<size_t>(li[static_cast<size_t>(q)])] -= lx[static_cast<size_t>(q)] * tj;
+    }
+  }
+  for (int j = 0; j < n; ++j) {
+    t[static_cast<size_t>(j)] /= diag[static_cast<size_t>(j)];
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    double s = t[static_cast<size_t>(j)];
+    for (int64_t q = lp[static_cast<size_t>(j)]; q < lp[static_cast<size_t>(j) + 1]; ++q) {
+      s -= lx[static_cast<size_t>(q)] * t[static_cast<size_t>(li[static_cast<size_t>(q)])];
+    }
+    t[static_cast<size_t>(j)] = s;
+  }
+  for (int k = 0; k < n; ++k) {
+    x[perm[static_cast<size_t>(k)]] = t[static_cast<size_t>(k)];
+  }
+  return SDFSLAM_OK;
+}
+
+// One row of the up-looking LDL^T (parallel_ldlt.hpp:165-206, Eigen's
+// factorize_preordered): the pattern of row k by elimination-tree reach,
+// d = fma(y_k, 1, 0), l_ki = y_i / d_i, fused y updates, unfused d -= l_ki y_i.
+// Returns false at a zero pivot.
+bool LdltRow(int k, int n, const std::vector<int64_t>& cp, const std::vector<int>& ci,
+             const std::vector<double>& cx, const std::vector<int>& parent, const std::vector<int64_t>& lp,
+             std::vector<int>& li, std::vector<double>& lx, std::vector<double>& diag, std::vector<int64_t>& nzc,
+             std::vector<double>& y, std::vector<int>& pattern, std::vector<int>& tags) {
+  {
     y[static_cast<size_t>(k)] = 0.0;
     int top = n;
     tags[static_cast<size_t>(k)] = k;
@@ -652,35 +815,87 @@ int LdltSolve(const Csc& a, const int32_t* order, const double* b, double* x, in
       ++nzc[static_cast<size_t>(i)];
     }
     diag[static_cast<size_t>(k)] = d;
-    if (d == 0.0) {
-      return SDFSLAM_ERR_NOT_SPD;
-    }
+    return d != 0.0;
   }
-  // x = P^T L^-T D^-1 L^-1 P b
-  std::vector<double> t(static_cast<size_t>(n));
+}
+
+// Thread schedule of the numeric LDL^T (the role of BuildSchedule,
+// parallel_ldlt.hpp:59-148, with a balance rule of its own): starting from
+// the roots, the subtree of largest work (sum over its rows of 1 + the
+// column count) is split, its root moving to the serial trunk, until no
+// subtree holds more than a quarter of a thread's share; the subtrees then
+// go to threads largest first, each to the least loaded. Rows ascend within
+// each list. Any such partition gives the same factor.
+void SubtreeSchedule(const std::vector<int>& parent, const std::vector<int64_t>& lnz, int threads,
+                     std::vector<std::vector<int>>& rows_of, std::vector<int>& trunk) {
+  const int n = static_cast<int>(parent.size());
+  rows_of.assign(1, {});
+  trunk.clear();
+  if (threads <= 1 || n < 256) {
+    rows_of[0].resize(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) rows_of[0][static_cast<size_t>(k)] = k;
+    return;
+  }
+  std::vector<std::vector<int>> children(static_cast<size_t>(n));
+  std::vector<double> work(static_cast<size_t>(n), 0.0);
+  std::vector<int> roots;
   for (int k = 0; k < n; ++k) {
-    t[static_cast<size_t>(k)] = b[perm[static_cast<size_t>(k)]];
-  }
-  for (int j = 0; j < n; ++j) {
-    const double tj = t[static_cast<size_t>(j)];
-    for (int64_t q = lp[static_cast<size_t>(j)]; q < lp[static_cast<size_t>(j) + 1]; ++q) {
-      t[static_cast<size_t>(li[static_cast<size_t>(q)])] -= lx[static_cast<size_t>(q)] * tj;
+    const double c = static_cast<double>(lnz[static_cast<size_t>(k)]);
+    work[static_cast<size_t>(k)] += 1.0 + c;
+    const int p = parent[static_cast<size_t>(k)];
+    if (p >= 0) {
+      children[static_cast<size_t>(p)].push_back(k);
+      work[static_cast<size_t>(p)] += work[static_cast<size_t>(k)];  // parent > k: k's total is final
+    } else {
+      roots.push_back(k);
     }
   }
-  for (int j = 0; j < n; ++j) {
-    t[static_cast<size_t>(j)] /= diag[static_cast<size_t>(j)];
-  }
-  for (int j = n - 1; j >= 0; --j) {
-    double s = t[static_cast<size_t>(j)];
-    for (int64_t q = lp[static_cast<size_t>(j)]; q < lp[static_cast<size_t>(j) + 1]; ++q) {
-      s -= lx[static_cast<size_t>(q)] * t[static_cast<size_t>(li[static_cast<size_t>(q)])];
+  double total = 0.0;
+  for (const int r : roots) total += work[static_cast<size_t>(r)];
+  const double limit = total / (4.0 * threads);
+  std::vector<int> parts = roots;
+  while (true) {
+    size_t big = 0;
+    for (size_t q = 1; q < parts.size(); ++q) {
+      if (work[static_cast<size_t>(parts[q])] > work[static_cast<size_t>(parts[big])]) big = q;
     }
-    t[static_cast<size_t>(j)] = s;
+    const int r = parts[big];
+    if (work[static_cast<size_t>(r)] <= limit || children[static_cast<size_t>(r)].empty() ||
+        parts.size() > 64 * static_cast<size_t>(threads)) {
+      break;
+    }
+    trunk.push_back(r);
+    parts.erase(parts.begin() + static_cast<std::ptrdiff_t>(big));
+    for (const int c : children[static_cast<size_t>(r)]) parts.push_back(c);
   }
-  for (int k = 0; k < n; ++k) {
-    x[perm[static_cast<size_t>(k)]] = t[static_cast<size_t>(k)];
+  std::sort(parts.begin(), parts.end(), [&](int a, int b) {
+    return work[static_cast<size_t>(a)] > work[static_cast<size_t>(b)] ||
+           (work[static_cast<size_t>(a)] == work[static_cast<size_t>(b)] && a < b);
+  });
+  rows_of.assign(static_cast<size_t>(threads), {});
+  std::vector<double> load(static_cast<size_t>(threads), 0.0);
+  std::vector<int> stack;
+  for (const int r : parts) {
+    const size_t t = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+    load[t] += work[static_cast<size_t>(r)];
+    stack.push_back(r);
+    while (!stack.empty()) {
+      const int v = stack.back();
+      stack.pop_back();
+      rows_of[t].push_back(v);
+      for (const int c : children[static_cast<size_t>(v)]) stack.push_back(c);
+    }
   }
-  return SDFSLAM_OK;
+  for (auto& rows : rows_of) std::sort(rows.begin(), rows.end());
+  std::sort(trunk.begin(), trunk.end());
+}
+
+// The fill-reducing order of the reference's LDL^T: approximate minimum
+// degree on the pattern of the damped normal matrix (AMDOrdering applied by
+// SimplicialLDLT::analyzePattern, parallel_ldlt.hpp:39-42), order[k] = the
+// column eliminated k-th.
+std::vector<int32_t> AmdOfMatrix(const Csc& a) {
+  return oracle::AmdOrder(static_cast<int>(a.cols), a.col_ptr, a.row_idx);
 }
 
 // Problem::Cost (problem.cpp:132-156)
@@ -991,8 +1206,7 @@ int oracle_linear_step(const sdfslam_problem_view* problem, const double* map_va
   Damp(a, lambda, marquardt_scaling != 0);
   std::vector<double> rhs(static_cast<size_t>(n));
   NegAtb(j, r.data(), rhs.data());
-  std::vector<int32_t> order(static_cast<size_t>(n));
-  Order(problem, order.data());
+  const std::vector<int32_t> order = AmdOfMatrix(a);
   return LdltSolve(a, order.data(), rhs.data(), step, nullptr);
 }
 
@@ -1024,8 +1238,11 @@ int oracle_solve_recorded(const sdfslam_problem_view* problem, const sdfslam_sol
   double lambda = options->lambda_init;
   double previous_cost = result->initial_cost;
   double nielsen_nu = 2.0;
-  std::vector<int32_t> order(static_cast<size_t>(n));
-  Order(problem, order.data());
+  // The ordering is recomputed when the pattern of the damped normal matrix
+  // changes (lm_solver.cpp:333-350).
+  std::vector<int32_t> order;
+  std::vector<int64_t> pattern_ptr;
+  std::vector<int32_t> pattern_idx;
   std::vector<double> r(static_cast<size_t>(rows));
   std::vector<double> rhs(static_cast<size_t>(n));
   std::vector<double> step(static_cast<size_t>(n));
@@ -1040,6 +1257,11 @@ int oracle_solve_recorded(const sdfslam_problem_view* problem, const sdfslam_sol
     Csc a = NormalMatrix(j);
     Damp(a, lambda, options->marquardt_scaling != 0);
     NegAtb(j, r.data(), rhs.data());
+    if (a.col_ptr != pattern_ptr || a.row_idx != pattern_idx) {
+      order = AmdOfMatrix(a);
+      pattern_ptr = a.col_ptr;
+      pattern_idx = a.row_idx;
+    }
     auto t2 = clock::now();
     status = LdltSolve(a, order.data(), rhs.data(), step.data(), nullptr);
     auto t3 = clock::now();
@@ -1125,6 +1347,20 @@ int oracle_solve_recorded(const sdfslam_problem_view* problem, const sdfslam_sol
   }
   result->duration_seconds = std::chrono::duration<double>(clock::now() - start).count();
   return status;
+}
+
+void oracle_set_threads(int threads) { g_threads = std::max(1, threads); }
+
+int oracle_threads(void) { return g_threads; }
+
+void oracle_last_factor_stats(int64_t* factor_nnz, double* factor_flops) {
+  if (factor_nnz != nullptr) *factor_nnz = g_last_factor_nnz;
+  if (factor_flops != nullptr) *factor_flops = g_last_factor_flops;
+}
+
+void oracle_amd_order(const oracle_matrix* a, int32_t* order) {
+  const std::vector<int32_t> o = AmdOfMatrix(a->m);
+  std::copy(o.begin(), o.end(), order);
 }
 
 void oracle_last_phase_seconds(double out[4]) {

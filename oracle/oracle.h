@@ -91,12 +91,13 @@ void oracle_damp(oracle_matrix* a, double lambda, int marquardt_scaling);
 /* ParallelNegAtb (lm_solver.cpp:178-197). */
 void oracle_neg_atb(const oracle_matrix* jacobian, const double* residuals, double* rhs);
 
-/* Fill-reducing order used by the oracle's LDL^T: active grid nodes by
- * recursive coordinate bisection, then the pose columns. order[k] = column
- * eliminated k-th. */
+/* A geometric order (active grid nodes by recursive coordinate bisection,
+ * then the pose columns; order[k] = column eliminated k-th). The solves use
+ * oracle_amd_order instead. */
 void oracle_order(const sdfslam_problem_view* problem, int32_t* order);
 /* Up-looking simplicial LDL^T with the row kernel of parallel_ldlt.hpp:165-206
- * and the solve P^T L^-T D^-1 L^-1 P b. order may be NULL (natural order).
+ * (rows over elimination subtrees on oracle_set_threads threads) and the
+ * solve P^T L^-T D^-1 L^-1 P b. order may be NULL (natural order).
  * Returns SDFSLAM_OK or SDFSLAM_ERR_NOT_SPD (zero pivot, parallel_ldlt.hpp:205). */
 int oracle_ldlt_solve(const oracle_matrix* a, const int32_t* order, const double* b, double* x,
                       int64_t* factor_nnz);
@@ -108,7 +109,8 @@ double oracle_cost(const sdfslam_problem_view* problem, const double* map_values
 void oracle_apply_step(const sdfslam_problem_view* problem, const double* step,
                        double* map_values, double* poses);
 
-/* One damped Gauss-Newton step at the given state (lm_solver.cpp:232-364). */
+/* One damped Gauss-Newton step at the given state (lm_solver.cpp:232-364),
+ * with the approximate minimum degree order of its matrix. */
 int oracle_linear_step(const sdfslam_problem_view* problem, const double* map_values,
                        const double* poses, double lambda, int marquardt_scaling, double* step);
 
@@ -145,6 +147,18 @@ void oracle_eval_points_frozen(const sdfslam_grid* grid, const double* values, c
 void oracle_init_from_scans(const sdfslam_grid* grid, double* map_values, const double* scan_xy,
                             const int64_t* scan_ptr, int32_t num_scans, const double* poses,
                             int32_t num_poses, int signed_distance);
+
+/* Host threads of the threaded phases (assembly chunks, J^T J column ranges,
+ * -J^T r, LDL^T elimination subtrees), default 1; the results do not depend
+ * on it (every phase gives the serial result bit for bit). */
+void oracle_set_threads(int threads);
+int oracle_threads(void);
+/* Nonzeros of L below the diagonal and sum over columns of c^2 + 3c of the
+ * last LDL^T factorization (any thread). */
+void oracle_last_factor_stats(int64_t* factor_nnz, double* factor_flops);
+/* The approximate minimum degree order the solves use for matrix a (order[k]
+ * = the column eliminated k-th), oracle/ordering.cpp. */
+void oracle_amd_order(const oracle_matrix* a, int32_t* order);
 
 /* Per-phase wall time (seconds) of the last oracle_solve on this thread:
  * assembly, normal equations, factorization+solve, step+cost. */

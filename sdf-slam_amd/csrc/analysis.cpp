@@ -909,6 +909,49 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   return a;
 }
 
+void ReferenceFactorCounts(const Analysis& a, int64_t rows, const int32_t* jcols, int64_t* nnz, double* flops) {
+  const int n = a.n;
+  // Lower neighbours (in elimination positions) of every column.
+  std::vector<std::vector<int32_t>> below(static_cast<size_t>(n));
+  for (int64_t r = 0; r < rows; ++r) {
+    int32_t q[7];
+    int m = 0;
+    for (int t = 0; t < 7; ++t) {
+      const int32_t c = jcols[7 * r + t];
+      if (c >= 0) q[m++] = a.perm[static_cast<size_t>(c)];
+    }
+    for (int x = 0; x < m; ++x) {
+      for (int y = 0; y < m; ++y) {
+        if (q[y] < q[x]) below[static_cast<size_t>(q[x])].push_back(q[y]);
+      }
+    }
+  }
+  // Elimination tree and column counts (row k's pattern is the union of the
+  // tree paths from its entries up to k).
+  std::vector<int> parent(static_cast<size_t>(n), -1);
+  std::vector<int> flag(static_cast<size_t>(n), -1);
+  std::vector<int64_t> count(static_cast<size_t>(n), 0);
+  for (int k = 0; k < n; ++k) {
+    flag[static_cast<size_t>(k)] = k;
+    for (int i : below[static_cast<size_t>(k)]) {
+      for (; flag[static_cast<size_t>(i)] != k; i = parent[static_cast<size_t>(i)]) {
+        if (parent[static_cast<size_t>(i)] == -1) parent[static_cast<size_t>(i)] = k;
+        ++count[static_cast<size_t>(i)];
+        flag[static_cast<size_t>(i)] = k;
+      }
+    }
+  }
+  int64_t total = 0;
+  double f = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double c = static_cast<double>(count[static_cast<size_t>(k)]);
+    total += count[static_cast<size_t>(k)];
+    f += c * c + 3.0 * c;
+  }
+  *nnz = total;
+  *flops = f;
+}
+
 std::vector<PatternEntry> LowerPattern(const Analysis& a, const sdfslam_problem_view& v) {
   std::vector<PatternEntry> out;
   out.reserve(static_cast<size_t>(a.pattern_nnz_lower));

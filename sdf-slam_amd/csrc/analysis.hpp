@@ -132,4 +132,13 @@ struct PatternEntry {
 };
 std::vector<PatternEntry> LowerPattern(const Analysis& a, const sdfslam_problem_view& view);
 
+/// Symbolic LDL^T of the reference pattern under the analysis's order:
+/// `jcols` holds 7 column slots per Jacobian row (-1 = empty; the device
+/// assembly's layout), the pattern is every pair of columns sharing a row
+/// plus the diagonal (lm_solver.cpp:113-172, 321-330), permuted by a.perm.
+/// Outputs the nonzeros of L below the diagonal and sum over columns of
+/// c^2 + 3c (c = a column's count below the diagonal): the factorization's
+/// flops without the superset pattern's explicit zeros or supernode padding.
+void ReferenceFactorCounts(const Analysis& a, int64_t rows, const int32_t* jcols, int64_t* nnz, double* flops);
+
 }  // namespace sdfslam

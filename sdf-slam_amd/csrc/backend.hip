@@ -1835,6 +1835,30 @@ int sdfslam_hip_normal_equations(sdfslam_hip_context* ctx, double lambda, int64_
   });
 }
 
+int sdfslam_hip_elimination_order(sdfslam_hip_context* ctx, int32_t* order) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || order == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    std::copy(ctx->an.iperm.begin(), ctx->an.iperm.end(), order);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_reference_factor_counts(sdfslam_hip_context* ctx, int64_t* factor_nnz, double* factor_flops) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || factor_nnz == nullptr || factor_flops == nullptr) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    }
+    const int64_t rows = ctx->hp->rows();
+    std::vector<double> r(static_cast<size_t>(rows));
+    std::vector<int32_t> jc(7 * static_cast<size_t>(rows));
+    std::vector<double> jv(7 * static_cast<size_t>(rows));
+    const int st = sdfslam_hip_assemble(ctx, r.data(), jc.data(), jv.data());
+    if (st != SDFSLAM_OK) throw Failure(st, g_last_error);
+    ReferenceFactorCounts(ctx->an, rows, jc.data(), factor_nnz, factor_flops);
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_hip_linear_step(sdfslam_hip_context* ctx, double lambda, double* step) {
   return Guard([&]() -> int {
     if (ctx == nullptr || step == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");

@@ -173,7 +173,8 @@ HIP_SYMBOLS = [
     "sdfslam_hip_context_solve", "sdfslam_hip_context_iterate", "sdfslam_hip_phase_times_get",
     "sdfslam_hip_context_update", "sdfslam_hip_device_memory", "sdfslam_hip_batch_solve",
     "sdfslam_hip_pose_margins", "sdfslam_hip_node_normals", "sdfslam_hip_init_from_scans",
-    "sdfslam_hip_init_from_scans_noalloc",
+    "sdfslam_hip_init_from_scans_noalloc", "sdfslam_hip_reference_factor_counts",
+    "sdfslam_hip_elimination_order",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -211,6 +212,8 @@ hip.sdfslam_hip_context_create.argtypes = [C.POINTER(ProblemView), C.POINTER(Sol
                                            _f64p, _f64p, C.POINTER(C.c_void_p)]
 hip.sdfslam_hip_context_destroy.argtypes = [C.c_void_p]
 hip.sdfslam_hip_context_trim.argtypes = [C.c_void_p]
+hip.sdfslam_hip_reference_factor_counts.argtypes = [C.c_void_p, _i64p, _f64p]
+hip.sdfslam_hip_elimination_order.argtypes = [C.c_void_p, _i32p]
 hip.sdfslam_hip_set_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_get_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_analysis.argtypes = [C.c_void_p, C.POINTER(AnalysisInfo)]
@@ -560,6 +563,19 @@ class Context:
         r.cost_history_capacity = len(hist)
         _check_hip(hip.sdfslam_hip_context_solve(self._h, C.byref(o), C.byref(r)))
         return _result_dict(r, hist)
+
+    def reference_factor_counts(self):
+        """(nonzeros of L, sum of c^2 + 3c) of the reference pattern under the
+        context's order (sdfslam_hip_reference_factor_counts)."""
+        nnz, flops = C.c_int64(), C.c_double()
+        _check_hip(hip.sdfslam_hip_reference_factor_counts(self._h, C.byref(nnz), C.byref(flops)))
+        return nnz.value, flops.value
+
+    def elimination_order(self):
+        """order[k] = the column eliminated k-th (sdfslam_hip_elimination_order)."""
+        o = np.zeros(self.n, dtype=np.int32)
+        _check_hip(hip.sdfslam_hip_elimination_order(self._h, ptr(o, _i32p)))
+        return o
 
     def pose_margins(self):
         """Footprint margin (cells) of every pose in the current pattern epoch."""

@@ -71,6 +71,10 @@ def _setup(view_type, grid_type, solver_options_type, solve_result_type):
     lib.oracle_solve_recorded.argtypes = [V, C.POINTER(solver_options_type), _f64p, _f64p,
                                           C.POINTER(solve_result_type), C.c_int32, _i32p, _f64p, _f64p]
     lib.oracle_last_phase_seconds.argtypes = [_f64p]
+    lib.oracle_set_threads.argtypes = [C.c_int]
+    lib.oracle_threads.restype = C.c_int
+    lib.oracle_last_factor_stats.argtypes = [_i64p, _f64p]
+    lib.oracle_amd_order.argtypes = [_vp, _i32p]
     lib.oracle_eval_points_frozen.argtypes = [G, _f64p, _f64p, C.c_int64, _i32p, _f64p, _f64p, _f64p,
                                               _i32p, _i32p, _f64p, _f64p, _f64p]
     lib.oracle_init_from_scans.argtypes = [G, _f64p, _f64p, _i64p, C.c_int32, _f64p, C.c_int32, C.c_int]
@@ -243,6 +247,27 @@ def solve_recorded(mod, view, options, map_values, poses, record_iterations):
                final_cost=r.final_cost, duration_seconds=r.duration_seconds,
                cost_history=hist[:n].copy())
     return st, res, m, p, rm, rp
+
+
+def set_threads(n):
+    lib.oracle_set_threads(int(n))
+
+
+def threads():
+    return lib.oracle_threads()
+
+
+def last_factor_stats():
+    """(nonzeros of L below the diagonal, sum of c^2 + 3c) of the last LDL^T."""
+    nnz, flops = C.c_int64(), C.c_double()
+    lib.oracle_last_factor_stats(C.byref(nnz), C.byref(flops))
+    return nnz.value, flops.value
+
+
+def amd_order(a: OracleMatrix, n):
+    o = np.zeros(n, dtype=np.int32)
+    lib.oracle_amd_order(a.h, _p(o, _i32p))
+    return o
 
 
 def last_phase_seconds():

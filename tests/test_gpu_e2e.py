@@ -87,3 +87,27 @@ def test_full_size_normal_equations_match_oracle(sdfslam, oracle):
     assert np.all(np.abs(diff.data) <= 1e-10 * np.sqrt(d[diff.row] * d[diff.col]))
     rhs_or = oracle.neg_atb(jac, r_or, n)
     assert np.all(np.abs(rhs - rhs_or) <= 1e-10 * np.sqrt(d) * np.linalg.norm(r_or))
+
+
+@pytest.mark.parametrize("name", ["office_small", "office200"])
+def test_reference_factor_counts_match_the_oracle_under_the_device_order(sdfslam, oracle, name):
+    # The unpadded factor size the roofline is priced on: the oracle's
+    # symbolic LDL^T of the reference pattern under the device's elimination
+    # order gives the same nonzeros and the same sum of c^2 + 3c; the
+    # device's own (padded) factor is at least that large.
+    print(f"case reference factor counts {name}", flush=True)
+    prob = sdfslam.Problem.workload(name, 1)
+    m, p = prob.state()
+    ctx = sdfslam.Context(prob, sdfslam.default_solver_options(), m, p)
+    nnz, flops = ctx.reference_factor_counts()
+    order = ctx.elimination_order()
+    n = prob.num_parameters
+    assert np.array_equal(np.sort(order), np.arange(n))
+    r, jac = oracle.assemble(prob.view, m, p)
+    a = oracle.normal_matrix(jac, 1.0, False)
+    st, _ = oracle.ldlt_solve(a, oracle.neg_atb(jac, r, n), order)
+    assert st == 0
+    o_nnz, o_flops = oracle.last_factor_stats()
+    assert (nnz, flops) == (o_nnz, o_flops)
+    info = ctx.analysis()
+    assert info["factor_flops"] >= flops

@@ -527,3 +527,65 @@ def test_mean_translation_error_closed_form():
     assert ensemble.mean_translation_error(est, gt) == pytest.approx((0 + 5 + 1 + 1) / 4)
     with pytest.raises(ValueError):
         ensemble.mean_translation_error(est[:1], gt[:1])
+
+
+# ------------------------------------------------ ordering and threads
+
+
+@pytest.mark.parametrize("name", ["room_small", "office_small"])
+def test_amd_order_is_a_permutation_with_less_fill(sdfslam, oracle, name):
+    # The approximate minimum degree order (the reference's default for
+    # SimplicialLDLT) is a permutation and fills less than the geometric
+    # order with the poses last; solving with either gives the same step.
+    prob = sdfslam.Problem.workload(name, 1)
+    m, p = prob.state()
+    n = prob.num_parameters
+    r, jac = oracle.assemble(prob.view, m, p)
+    a = oracle.normal_matrix(jac, 0.5, False)
+    amd = oracle.amd_order(a, n)
+    assert np.array_equal(np.sort(amd), np.arange(n))
+    rhs = oracle.neg_atb(jac, r, n)
+    st, x_amd = oracle.ldlt_solve(a, rhs, amd)
+    nnz_amd, flops_amd = oracle.last_factor_stats()
+    st2, x_geo = oracle.ldlt_solve(a, rhs, oracle.order(prob.view, n))
+    nnz_geo, flops_geo = oracle.last_factor_stats()
+    assert st == st2 == 0
+    assert nnz_amd < nnz_geo and flops_amd < flops_geo
+    assert np.linalg.norm(x_amd - x_geo) <= 1e-10 * np.linalg.norm(x_geo)
+    A = a.to_scipy()
+    assert np.linalg.norm(A @ x_amd - rhs) <= 1e-12 * np.linalg.norm(rhs)
+
+
+def test_amd_order_small_closed_forms(oracle):
+    # An arrow matrix (vertex 0 joined to all others): minimum degree
+    # eliminates the leaves first and the hub last, so L has no fill.
+    n = 20
+    rows = [0] * (n - 1) + list(range(1, n))
+    cols = list(range(1, n)) + [0] * (n - 1)
+    A = sp.csc_matrix((np.ones(2 * (n - 1)), (rows, cols)), shape=(n, n)) + sp.eye(n) * n
+    mat = oracle.from_csc(A)
+    o = oracle.amd_order(mat, n)
+    assert o[-1] == 0 and sorted(o) == list(range(n))
+    st, _ = oracle.ldlt_solve(mat, np.ones(n), o)
+    assert st == 0 and oracle.last_factor_stats()[0] == n - 1
+
+
+def test_threaded_oracle_is_bitwise_serial(sdfslam, oracle):
+    # Assembly chunks, J^T J column ranges, -J^T r and the LDL^T over
+    # elimination subtrees give the serial results bit for bit.
+    prob = sdfslam.Problem.workload("office_small", 1)
+    m0, p0 = prob.state()
+    o = sdfslam.default_solver_options(max_iterations=3, step_tolerance=0.0)
+    results = []
+    for t in (1, 4, 7):
+        oracle.set_threads(t)
+        try:
+            results.append(oracle.solve(sdfslam, prob.view, o, m0, p0))
+            results[-1] += (oracle.linear_step(prob.view, m0, p0, 0.3)[1],)
+        finally:
+            oracle.set_threads(1)
+    for st, res, m, p, x in results[1:]:
+        assert st == 0
+        assert np.array_equal(m, results[0][2]) and np.array_equal(p, results[0][3])
+        assert np.array_equal(res["cost_history"], results[0][1]["cost_history"])
+        assert np.array_equal(x, results[0][4])
