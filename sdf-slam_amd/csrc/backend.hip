@@ -736,14 +736,18 @@ void SetPlans(LaunchShape& sh, const ListRef* dev) {
 /// footprint in the ending epoch: by one cell (two when that epoch lasted at
 /// most two iterations), and at least to twice the pose's motion bound over
 /// the epoch (|dt| + reach * |dtheta|, in cells), since a pose that moved
-/// fast tends to keep moving; poses that came within a cell of leaving
-/// theirs grow by one. Up to kMaxMargin. Poses that stay put keep the
-/// initial margin, so the pattern grows only where the state moves.
-constexpr int kMaxMargin = 8;
+/// fast tends to keep moving. Poses that moved more than half their margin
+/// over the epoch grow the same way before their points leave (a warm
+/// start's transient moves a few poses by whole cells per iteration, and
+/// they would otherwise end the next epochs one after another). Up to
+/// kMaxMargin. Poses that stay put keep the initial margin, so the pattern
+/// grows only where the state moves.
+constexpr int kMaxMargin = 12;
 // Margin of the poses an incremental update appends; while a run has few
 // frames (its map still barely constrained, its poses moving most, its
-// pattern small) they start at the largest margin.
+// pattern small) they start at kEarlyMargin.
 constexpr int kFreshPoseMargin = 5;
+constexpr int kEarlyMargin = 8;
 constexpr int kEarlyFrames = 10;
 
 void GrowMargins(sdfslam_hip_context& c, const std::vector<double>& poses) {
@@ -771,8 +775,8 @@ void GrowMargins(sdfslam_hip_context& c, const std::vector<double>& poses) {
     }
     if (flags[static_cast<size_t>(P)] != 0) {
       m = std::min(std::max(m + step, static_cast<int>(std::ceil(2.0 * cells))), kMaxMargin);
-    } else if (cells > m - 1) {
-      m = std::min(m + 1, kMaxMargin);
+    } else if (cells > 0.5 * m) {
+      m = std::min(std::max(m + 1, static_cast<int>(std::ceil(2.0 * cells))), kMaxMargin);
     }
   }
   HIPC(hipMemsetAsync(c.dw.pose_overflow, 0, sizeof(int32_t) * flags.size(), c.stream));
@@ -1449,7 +1453,7 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
     margins = c.aopt.pose_margin;
     if (static_cast<int>(margins.size()) != old_poses) margins.assign(static_cast<size_t>(old_poses), c.aopt.margin);
     margins.resize(static_cast<size_t>(problem->num_poses),
-                   problem->num_poses <= kEarlyFrames ? kMaxMargin : kFreshPoseMargin);
+                   problem->num_poses <= kEarlyFrames ? kEarlyMargin : kFreshPoseMargin);
   }
   c.hp = std::move(hp);
   c.aopt.margin = AnalysisOptions{}.margin;
