@@ -687,8 +687,10 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     const int pn = a.sn_first[static_cast<size_t>(p) + 1] - a.sn_first[static_cast<size_t>(p)];
     int split = 0;
     int64_t q = a.rel_off[static_cast<size_t>(c)];
+    // Both lists ascend: one merge walk.
+    const int32_t* it = pb;
     for (const int i : R[static_cast<size_t>(c)]) {
-      const int32_t* it = std::lower_bound(pb, pe, i);
+      while (it != pe && *it < i) ++it;
       if (it == pe || *it != i) {
         throw std::logic_error("child front row missing from parent front");
       }
@@ -703,25 +705,53 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   a.nn_eik.assign(5 * static_cast<size_t>(num_nodes), 0.0);
   const int dws[5] = {0, 1, -1, 0, 1};
   const int dhs[5] = {0, 0, 1, 1, 1};
-  ParallelFor(num_nodes, threads, 1024, [&](int64_t gi) {
-    const int g = static_cast<int>(gi);
-    const int cu = v.node_to_active[g];
-    if (cu < 0) return;
-    const int w = g % a.nx;
-    const int h = g / a.nx;
-    for (int k = 0; k < 5; ++k) {
-      const int ww = w + dws[k];
-      const int hh = h + dhs[k];
-      if (ww < 0 || hh < 0 || ww >= a.nx || hh >= a.ny) continue;
-      const int cv = v.node_to_active[hh * a.nx + ww];
-      if (cv < 0) continue;
+  // By supernode of the slot's column (the smaller position): a dense map
+  // of the supernode's front positions, filled once per supernode, replaces
+  // a binary search per slot.
+  {
+    std::vector<std::vector<int32_t>> cols_of_sn(static_cast<size_t>(a.num_sn));
+    for (int g = 0; g < num_nodes; ++g) {
+      const int cu = v.node_to_active[g];
+      if (cu < 0) continue;
       const int pu = a.perm[static_cast<size_t>(cu)];
-      const int pv = a.perm[static_cast<size_t>(cv)];
-      const int64_t off = PanelOffset(a, std::max(pu, pv), std::min(pu, pv));
-      if (off < 0) throw std::logic_error("node-node slot missing from the factor");
-      a.nn_off[5 * static_cast<size_t>(g) + k] = off;
+      const int w = g % a.nx;
+      const int h = g / a.nx;
+      for (int k = 0; k < 5; ++k) {
+        const int ww = w + dws[k];
+        const int hh = h + dhs[k];
+        if (ww < 0 || hh < 0 || ww >= a.nx || hh >= a.ny) continue;
+        const int cv = v.node_to_active[hh * a.nx + ww];
+        if (cv < 0) continue;
+        const int pv = a.perm[static_cast<size_t>(cv)];
+        cols_of_sn[static_cast<size_t>(a.snode_of[static_cast<size_t>(std::min(pu, pv))])].push_back(5 * g + k);
+      }
     }
-  });
+    const int chunks = std::max(1, std::min(threads, a.num_sn / 64));
+    RunChunks(chunks, [&](int t) {
+      std::vector<int32_t> pos_in(static_cast<size_t>(a.n), -1);
+      for (int64_t sn = Bound(a.num_sn, chunks, t); sn < Bound(a.num_sn, chunks, t + 1); ++sn) {
+        const std::vector<int32_t>& slots = cols_of_sn[static_cast<size_t>(sn)];
+        if (slots.empty()) continue;
+        const int32_t* idx = a.sn_idx.data() + a.sn_idx_ptr[static_cast<size_t>(sn)];
+        const int f = a.sn_f[static_cast<size_t>(sn)];
+        for (int r = 0; r < f; ++r) pos_in[static_cast<size_t>(idx[r])] = r;
+        for (const int32_t q : slots) {
+          const int g = q / 5;
+          const int k = q % 5;
+          const int cu = v.node_to_active[g];
+          const int cv = v.node_to_active[(g / a.nx + dhs[k]) * a.nx + g % a.nx + dws[k]];
+          const int pu = a.perm[static_cast<size_t>(cu)];
+          const int pv = a.perm[static_cast<size_t>(cv)];
+          const int j = std::min(pu, pv);
+          const int32_t row = pos_in[static_cast<size_t>(std::max(pu, pv))];
+          if (row < 0) throw std::logic_error("node-node slot missing from the factor");
+          a.nn_off[static_cast<size_t>(q)] = a.panel_off[static_cast<size_t>(sn)] +
+                                             static_cast<int64_t>(j - a.sn_first[static_cast<size_t>(sn)]) * f + row;
+        }
+        for (int r = 0; r < f; ++r) pos_in[static_cast<size_t>(idx[r])] = -1;
+      }
+    });
+  }
   a.eik_at.assign(static_cast<size_t>(num_nodes), -1);
   for (int64_t e = 0; e < v.num_eikonal; ++e) {
     const int w = v.eik_w[e];
