@@ -151,6 +151,30 @@ int sdfslam_hip_context_destroy(sdfslam_hip_context* ctx);
  * whole device). No counterpart in the reference; a server calls it when it
  * needs the memory. Waits for the device; the context stays usable. */
 int sdfslam_hip_context_trim(sdfslam_hip_context* ctx);
+/* A context update whose problem's eikonal normals are computed on the
+ * device, at the start of the context's next call, instead of taken from
+ * the view: the per-increment normal estimation of the problem build
+ * (ComputeScanNormals over all scans and AssignNodeNormals,
+ * src/core/normals.cpp:128-220, called by Problem::Problem,
+ * src/problem/problem.cpp:72-94) without its host cost. The scan points are
+ * the view's point rows with expected SDF 0 (in row order), placed by the
+ * problem's poses with frame_cos_sin (2 per frame: cos and sin of the pose
+ * angle as the caller's own point placement computes them; NULL: both from
+ * one ::sincos call per frame, as the host library's Pose2 does); every
+ * eikonal spec takes
+ * its node's normal. Only the pca and hybrid methods, whose eikonal scale is
+ * 1 (the specs do not depend on the normals), are accepted; the values
+ * equal the host builder's bit for bit. No counterpart in the reference. */
+typedef struct sdfslam_normals_request {
+  int32_t k_neighbors;
+  int32_t method; /* SDFSLAM_NORMALS_PCA or SDFSLAM_NORMALS_HYBRID */
+  double corner_threshold;
+  const double* frame_cos_sin;
+} sdfslam_normals_request;
+int sdfslam_hip_context_update_device_normals(sdfslam_hip_context* ctx, const sdfslam_problem_view* problem,
+                                              const double* map_values, const double* poses,
+                                              const sdfslam_normals_request* normals);
+
 /* Replaces the context's problem and state with a new one (the next
  * increment of an incremental run: the problem rebuilt over more frames, as
  * runner.cpp:249-260 rebuilds it before each Solve), keeping the context's

@@ -700,9 +700,8 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     }
     a.rel_split[static_cast<size_t>(c)] = split;
   });
-  // Node-node slots and constant eikonal contributions.
+  // Node-node slots.
   a.nn_off.assign(5 * static_cast<size_t>(num_nodes), -1);
-  a.nn_eik.assign(5 * static_cast<size_t>(num_nodes), 0.0);
   const int dws[5] = {0, 1, -1, 0, 1};
   const int dhs[5] = {0, 0, 1, 1, 1};
   // By supernode of the slot's column (the smaller position): a dense map
@@ -764,21 +763,6 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
       throw std::invalid_argument("more than one eikonal spec at a grid node");
     }
     a.eik_at[static_cast<size_t>(g)] = static_cast<int32_t>(e);
-    const double sw = v.eik_sqrt_weight[e];
-    const double j0 = sw * (v.eik_nx[e] / v.grid.dx + v.eik_ny[e] / v.grid.dy);
-    const double j1 = sw * (-v.eik_nx[e] / v.grid.dx);
-    const double j2 = sw * (-v.eik_ny[e] / v.grid.dy);
-    const int g1 = g + 1;
-    const int g2 = g + a.nx;
-    const bool a0 = v.node_to_active[g] >= 0;
-    const bool a1 = v.node_to_active[g1] >= 0;
-    const bool a2 = v.node_to_active[g2] >= 0;
-    if (a0) a.nn_eik[5 * static_cast<size_t>(g)] += j0 * j0;
-    if (a1) a.nn_eik[5 * static_cast<size_t>(g1)] += j1 * j1;
-    if (a2) a.nn_eik[5 * static_cast<size_t>(g2)] += j2 * j2;
-    if (a0 && a1) a.nn_eik[5 * static_cast<size_t>(g) + 1] += j0 * j1;   // right
-    if (a0 && a2) a.nn_eik[5 * static_cast<size_t>(g) + 3] += j0 * j2;   // up
-    if (a1 && a2) a.nn_eik[5 * static_cast<size_t>(g1) + 2] += j1 * j2;  // up-left of g1
   }
   // Pose footprint slots.
   a.fp_ptr.assign(static_cast<size_t>(N) + 1, 0);

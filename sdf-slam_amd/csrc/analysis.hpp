@@ -59,9 +59,8 @@ struct Analysis {
   std::vector<int32_t> rel;
   std::vector<int32_t> rel_split;
   // Node-node slots: 5 per grid node (self, right, up-left, up, up-right);
-  // panel offset or -1. Constant eikonal J^T J contribution per slot.
+  // panel offset or -1.
   std::vector<int64_t> nn_off;
-  std::vector<double> nn_eik;
   // Eikonal spec at each grid node, or -1.
   std::vector<int32_t> eik_at;
   // Pose footprints: nodes (grid ids, ascending) and the panel offset of the

@@ -440,6 +440,7 @@ struct sdfslam_hip_context {
   double analysis_seconds{0.0};  // host time spent building pattern epochs
   EpochPool base;   // problem specs, state and per-row work arrays
   EpochPool epoch;  // symbolic analysis and factor storage
+  EpochPool normals;  // device eikonal normals: scan rows, cos/sin, work
   DeviceArena hist;  // cost history
   DeviceProblem dp{};
   DeviceAnalysis da{};
@@ -501,6 +502,7 @@ struct sdfslam_hip_context {
     for (hipEvent_t e : gemm_events) (void)hipEventDestroy(e);
     epoch.Release();
     base.Release();
+    normals.Release();
     hist.Clear();
   }
 };
@@ -916,7 +918,6 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     D.rel = E.Upload(an.rel);
     D.rel_split = E.Upload(an.rel_split);
     D.nn_off = E.Upload(an.nn_off);
-    D.nn_eik = E.Upload(an.nn_eik);
     D.eik_at = E.Upload(an.eik_at);
     D.fp_ptr = E.Upload(an.fp_ptr);
     D.fp_node = E.Upload(an.fp_node);
@@ -1432,8 +1433,76 @@ void CreateContext(const sdfslam_problem_view* problem, const sdfslam_solver_opt
 /// (pools grow only when the new problem needs more). Host work only: the
 /// uploads and the LM initialisation run at the start of the context's next
 /// call, on that call's stream.
+/// Enqueues (deferred, like the rest of an update) the device computation
+/// of the eikonal specs' normals of the context's problem from its scan
+/// points (the rows with expected SDF 0) at its poses.
+void StageDeviceNormals(sdfslam_hip_context& c, const sdfslam_normals_request& nr, const double* poses) {
+  const HostProblem& hp = *c.hp;
+  const sdfslam_problem_view& v = hp.view;
+  if (nr.method != SDFSLAM_NORMALS_PCA && nr.method != SDFSLAM_NORMALS_HYBRID) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "device eikonal normals need the pca or hybrid method");
+  }
+  std::vector<int32_t> rows;
+  for (size_t r = 0; r < hp.pt_delta.size(); ++r) {
+    if (hp.pt_delta[r] == 0.0) rows.push_back(static_cast<int32_t>(r));
+  }
+  if (rows.empty()) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "no scan points for the eikonal normals");
+  std::vector<double> cs(2 * static_cast<size_t>(v.num_poses));
+  if (nr.frame_cos_sin != nullptr) {
+    std::copy(nr.frame_cos_sin, nr.frame_cos_sin + cs.size(), cs.begin());
+  } else {
+    // One ::sincos per frame angle (the host's Pose2 rotation does the same).
+    for (int f = 0; f < v.num_poses; ++f) ::sincos(poses[3 * f + 2], &cs[2 * f + 1], &cs[2 * f]);
+  }
+  const int64_t nodes = static_cast<int64_t>(v.grid.nx) * v.grid.ny;
+  const size_t work_bytes = EikonalNormalsWorkBytes(static_cast<int64_t>(rows.size()), nodes);
+  EpochPool& E = c.normals;
+  int32_t* d_rows = nullptr;
+  double* d_cs = nullptr;
+  unsigned char* d_work = nullptr;
+  auto layout = [&]() {
+    d_rows = E.Upload(rows);
+    d_cs = E.Upload(cs);
+    d_work = E.Alloc<unsigned char>(work_bytes);
+  };
+  E.BeginMeasure();
+  layout();
+  E.BeginPlace();
+  layout();
+  const DeviceProblem& P = c.dp;
+  EikonalNormalsJob job{};
+  job.scan_rows = d_rows;
+  job.scan_points = static_cast<int64_t>(rows.size());
+  job.frame = P.pt_frame;
+  job.sensor_x = P.pt_x;
+  job.sensor_y = P.pt_y;
+  job.cos_sin = d_cs;
+  job.poses = c.dw.poses;
+  job.nx = v.grid.nx;
+  job.ny = v.grid.ny;
+  job.min_x = v.grid.min_x;
+  job.min_y = v.grid.min_y;
+  job.dx = v.grid.dx;
+  job.dy = v.grid.dy;
+  job.k = nr.k_neighbors;
+  job.method = nr.method;
+  job.corner_threshold = nr.corner_threshold;
+  job.eik_w = P.eik_w;
+  job.eik_h = P.eik_h;
+  job.num_eikonal = P.num_eikonal;
+  // The spec arrays live in the context's own base pool.
+  job.eik_nx = const_cast<double*>(P.eik_nx);
+  job.eik_ny = const_cast<double*>(P.eik_ny);
+  Enqueue(c, [&E, job, d_work](hipStream_t s) {
+    E.Copy(s);
+    const int st = EnqueueEikonalNormals(job, d_work, s);
+    if (st != SDFSLAM_OK) throw Failure(st, NormalsLastError());
+  });
+}
+
 void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
-                   const double* map_values, const double* poses) {
+                   const double* map_values, const double* poses,
+                   const sdfslam_normals_request* normals = nullptr) {
   if (problem == nullptr || map_values == nullptr || poses == nullptr) {
     throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
   }
@@ -1467,6 +1536,7 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
     ~Deferring() { c.deferring = false; }
   } deferring(c);
   AllocateBase(c, map_values, poses);
+  if (normals != nullptr) StageDeviceNormals(c, *normals, poses);
   BuildEpoch(c, map_values, poses);
   const double lambda = c.opts.lambda_init;
   Enqueue(c, [&c, lambda](hipStream_t) { InitLm(c, lambda); });
@@ -1562,6 +1632,16 @@ int sdfslam_hip_context_update(sdfslam_hip_context* ctx, const sdfslam_problem_v
   });
 }
 
+int sdfslam_hip_context_update_device_normals(sdfslam_hip_context* ctx, const sdfslam_problem_view* problem,
+                                              const double* map_values, const double* poses,
+                                              const sdfslam_normals_request* normals) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || normals == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    UpdateContext(*ctx, problem, map_values, poses, normals);
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_hip_pose_margins(sdfslam_hip_context* ctx, int32_t* margins, int32_t capacity,
                              int32_t* num_poses) {
   return Guard([&]() -> int {
@@ -1585,6 +1665,7 @@ int sdfslam_hip_context_trim(sdfslam_hip_context* ctx) {
     HIPC(hipStreamSynchronize(ctx->stream));
     ctx->epoch.Trim();
     ctx->base.Trim();
+    ctx->normals.Trim();
     ctx->batch.Trim();
     return SDFSLAM_OK;
   });
@@ -1642,7 +1723,8 @@ int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* in
     info->tree_levels = a.num_levels;
     info->pattern_epochs = ctx->pattern_epochs;
     info->analysis_seconds = ctx->analysis_seconds;
-    info->device_bytes = static_cast<int64_t>(ctx->base.bytes() + ctx->epoch.bytes() + ctx->hist.bytes());
+    info->device_bytes =
+        static_cast<int64_t>(ctx->base.bytes() + ctx->epoch.bytes() + ctx->normals.bytes() + ctx->hist.bytes());
     info->panel_step_launches = a.panel_step_launches;
     info->panel_step_flops = a.panel_step_flops;
     info->speculative_epochs = ctx->speculative_epochs;

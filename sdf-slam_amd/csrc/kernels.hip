@@ -371,6 +371,69 @@ __device__ __forceinline__ double Damp(double diag, double lambda, int marquardt
 
 // Node-node slots (9-point stencil, lower triangle owned by the node with the
 // smaller grid id) and the node entries of -J^T r. One thread per grid node.
+// The eikonal rows' J^T J terms at node g's five node-node slots (self,
+// right, up-left, up, up-right). The row of spec e at node (w, h) is
+// sw * (nx/dx + ny/dy, -nx/dx, -ny/dy) on nodes g, g + 1, g + nx (each where
+// active); a slot's terms come from the specs at g, g - 1 and g - nx and are
+// summed in spec order, as the reference's product accumulates them
+// (lm_solver.cpp:113-172).
+__device__ __forceinline__ void EikonalSlots(const DeviceProblem& P, const DeviceAnalysis& A, int g, int w, int h,
+                                             double (&out)[5]) {
+  const int nx = P.grid.nx;
+  const double dx = P.grid.dx;
+  const double dy = P.grid.dy;
+  auto active = [&](int node) { return P.node_to_active[node] >= 0; };
+  // Terms of slot 0 with their spec indices, sorted by spec before summing.
+  int te[3];
+  double tv[3];
+  int nt = 0;
+  for (int k = 0; k < 5; ++k) out[k] = 0.0;
+  if (h >= 1) {  // spec at g - nx: g is its third node
+    const int e = A.eik_at[g - nx];
+    if (e >= 0 && active(g)) {
+      const double j2 = P.eik_sw[e] * (-P.eik_ny[e] / dy);
+      te[nt] = e;
+      tv[nt++] = j2 * j2;
+    }
+  }
+  if (w >= 1) {  // spec at g - 1: g is its second node
+    const int e = A.eik_at[g - 1];
+    if (e >= 0 && active(g)) {
+      const double j1 = P.eik_sw[e] * (-P.eik_nx[e] / dx);
+      te[nt] = e;
+      tv[nt++] = j1 * j1;
+      if (active(g - 1 + nx)) {
+        const double j2 = P.eik_sw[e] * (-P.eik_ny[e] / dy);
+        out[2] = 0.0 + j1 * j2;
+      }
+    }
+  }
+  const int e = A.eik_at[g];
+  if (e >= 0) {  // spec at g: g is its first node
+    const double sw = P.eik_sw[e];
+    const double j0 = sw * (P.eik_nx[e] / dx + P.eik_ny[e] / dy);
+    const double j1 = sw * (-P.eik_nx[e] / dx);
+    const double j2 = sw * (-P.eik_ny[e] / dy);
+    if (active(g)) {
+      te[nt] = e;
+      tv[nt++] = j0 * j0;
+      if (active(g + 1)) out[1] = 0.0 + j0 * j1;
+      if (active(g + nx)) out[3] = 0.0 + j0 * j2;
+    }
+  }
+  for (int a = 1; a < nt; ++a) {
+    for (int b = a; b > 0 && te[b - 1] > te[b]; --b) {
+      const int ti = te[b];
+      te[b] = te[b - 1];
+      te[b - 1] = ti;
+      const double tvv = tv[b];
+      tv[b] = tv[b - 1];
+      tv[b - 1] = tvv;
+    }
+  }
+  for (int a = 0; a < nt; ++a) out[0] += tv[a];
+}
+
 template <class Slots>
 __global__ void KNodeSlots(Slots slots, int marquardt) {
   BIND_SLOT(blockIdx.y);
@@ -391,7 +454,8 @@ __global__ void KNodeSlots(Slots slots, int marquardt) {
   auto nn = [&](int cw, int ch, int t) { return W.nn[static_cast<int64_t>(t) * nc + ch * ncx + cw]; };
   auto rn = [&](int cw, int ch, int t) { return W.rn[static_cast<int64_t>(t) * nc + ch * ncx + cw]; };
   const int64_t* off = A.nn_off + 5 * static_cast<int64_t>(g);
-  const double* eik = A.nn_eik + 5 * static_cast<int64_t>(g);
+  double eik[5];
+  EikonalSlots(P, A, g, w, h, eik);
   // Upper-triangle index of corner pair (a <= b) in the 10 cell products.
   // (0,0)0 (0,1)1 (0,2)2 (0,3)3 (1,1)4 (1,2)5 (1,3)6 (2,2)7 (2,3)8 (3,3)9
   {

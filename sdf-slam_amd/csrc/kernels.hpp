@@ -129,7 +129,6 @@ struct DeviceAnalysis {
   const int32_t* rel;
   const int32_t* rel_split;
   const int64_t* nn_off;
-  const double* nn_eik;
   const int32_t* eik_at;
   const int64_t* fp_ptr;
   const int32_t* fp_node;

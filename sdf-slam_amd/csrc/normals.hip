@@ -158,12 +158,14 @@ __device__ int Knn(double qx, double qy, int k, const double* __restrict__ px, c
 // kNN PCA normal of every point (PcaNormal, normals.cpp:70-103): the
 // eigenvector of the smaller covariance eigenvalue (closed form), oriented
 // toward the point's origin, and the eigenvalue ratio.
+template <class GridSrc>
 __global__ void KPointNormals(const double* __restrict__ px, const double* __restrict__ py,
                               const double* __restrict__ ox, const double* __restrict__ oy, int64_t n, int k,
-                              PointGridDev g, const int* __restrict__ start, const int* __restrict__ sorted,
+                              GridSrc src, const int* __restrict__ start, const int* __restrict__ sorted,
                               double* __restrict__ nrm_x, double* __restrict__ nrm_y, double* __restrict__ corner) {
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= n) return;
+  const PointGridDev g = src.Load();
   int idx[kMaxK];
   double d2[kMaxK];
   const double qx = px[i];
@@ -221,15 +223,17 @@ __global__ void KPointNormals(const double* __restrict__ px, const double* __res
 
 // Normal and eikonal scale of every grid node from its nearest scan point
 // (AssignNodeNormals, normals.cpp:162-220).
+template <class GridSrc>
 __global__ void KNodeNormals(int nx, int ny, double min_x, double min_y, double dx, double dy,
                              const double* __restrict__ px, const double* __restrict__ py,
                              const double* __restrict__ pnx, const double* __restrict__ pny,
-                             const double* __restrict__ pcorner, PointGridDev g, const int* __restrict__ start,
+                             const double* __restrict__ pcorner, GridSrc src, const int* __restrict__ start,
                              const int* __restrict__ sorted, int method, double corner_threshold,
                              double* __restrict__ out_nx, double* __restrict__ out_ny,
                              double* __restrict__ out_scale) {
   const int64_t node = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (node >= static_cast<int64_t>(nx) * ny) return;
+  const PointGridDev g = src.Load();
   const int w = static_cast<int>(node % nx);
   const int h = static_cast<int>(node / nx);
   const double qx = min_x + w * dx;
@@ -464,11 +468,11 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
   KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridByValue{g}, cell_of, count);
   KCellScan<<<1, kScanThreads, 0, s>>>(count, GridByValue{g}, start);
   KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
-  KPointNormals<<<pb, kThreads, 0, s>>>(px, py, ox, oy, n, kk, g, start, sorted, pnx, pny, pcorner);
+  KPointNormals<<<pb, kThreads, 0, s>>>(px, py, ox, oy, n, kk, GridByValue{g}, start, sorted, pnx, pny, pcorner);
   const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
   KNodeNormals<<<vb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, pnx,
-                                       pny, pcorner, g, start, sorted, method, corner_threshold, vnx, vny,
-                                       vscale);
+                                       pny, pcorner, GridByValue{g}, start, sorted, method, corner_threshold, vnx,
+                                       vny, vscale);
   HIPN(hipGetLastError());
   std::vector<double> rx(static_cast<size_t>(nodes)), ry(static_cast<size_t>(nodes));
   const size_t vb_bytes = static_cast<size_t>(nodes) * sizeof(double);
@@ -720,6 +724,122 @@ int InitFromScansNoAlloc(const double* points_xy, const double* origins_xy, int6
   KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
   KInitFromScans<<<vb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, ox,
                                          oy, GridInMemory{gp}, start, sorted, signed_distance, map_values);
+  HIPN(hipGetLastError());
+  return SDFSLAM_OK;
+}
+
+
+// ---- Eikonal normals of a device-resident problem ---------------------------
+// The eikonal specs' normals of a problem whose specs already live on the
+// device, computed there from its scan points (the point rows with expected
+// SDF 0, in row order: ComputeScanNormals' order) at its poses: the
+// per-increment normal estimation of the problem build, without a host
+// round trip. World points use the host's per-frame cos and sin and its
+// Pose2::Apply arithmetic, so every value equals the host builder's.
+
+namespace {
+
+// Scan point q (row rows[q] of frame f) in world coordinates,
+// (c p.x - s p.y) + t.x and (s p.x + c p.y) + t.y as Pose2::Apply, and its
+// sensor origin t.
+__global__ void KScanWorld(const int32_t* __restrict__ rows, int64_t n, const int32_t* __restrict__ frame,
+                           const double* __restrict__ sx, const double* __restrict__ sy,
+                           const double* __restrict__ cos_sin, const double* __restrict__ poses,
+                           double* __restrict__ px, double* __restrict__ py, double* __restrict__ ox,
+                           double* __restrict__ oy) {
+  const int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (q >= n) return;
+  const int r = rows[q];
+  const int f = frame[r];
+  const double c = cos_sin[2 * f];
+  const double sn = cos_sin[2 * f + 1];
+  const double tx = poses[3 * f];
+  const double ty = poses[3 * f + 1];
+  px[q] = (c * sx[r] - sn * sy[r]) + tx;
+  py[q] = (sn * sx[r] + c * sy[r]) + ty;
+  ox[q] = tx;
+  oy[q] = ty;
+}
+
+// Eikonal spec e at node (w, h) takes that node's normal.
+__global__ void KEikonalNormals(const int32_t* __restrict__ eik_w, const int32_t* __restrict__ eik_h, int64_t m,
+                                int nx, const double* __restrict__ vnx, const double* __restrict__ vny,
+                                double* __restrict__ eik_nx, double* __restrict__ eik_ny) {
+  const int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (e >= m) return;
+  const int64_t node = static_cast<int64_t>(eik_h[e]) * nx + eik_w[e];
+  eik_nx[e] = vnx[node];
+  eik_ny[e] = vny[node];
+}
+
+}  // namespace
+
+size_t EikonalNormalsWorkBytes(int64_t scan_points, int64_t nodes) {
+  const int64_t max_cells = scan_points / 2 + 4097;
+  const size_t nd = Align256(sizeof(double) * static_cast<size_t>(std::max<int64_t>(scan_points, 1)));
+  const size_t ni = Align256(sizeof(int) * static_cast<size_t>(std::max<int64_t>(scan_points, 1)));
+  const size_t ci = Align256(sizeof(int) * static_cast<size_t>(max_cells + 1));
+  const size_t vd = Align256(sizeof(double) * static_cast<size_t>(nodes));
+  return 256 + 7 * nd + 2 * ni + 3 * ci + 3 * vd;
+}
+
+int EnqueueEikonalNormals(const EikonalNormalsJob& j, void* work, hipStream_t s) {
+  const int64_t n = j.scan_points;
+  const int64_t nodes = static_cast<int64_t>(j.nx) * j.ny;
+  if (n <= 0) {
+    t_error = "no scan points for the eikonal normals";
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  if (j.k > kMaxK || (j.method != SDFSLAM_NORMALS_PCA && j.method != SDFSLAM_NORMALS_HYBRID)) {
+    t_error = "device eikonal normals need the pca or hybrid method and k_neighbors <= 32";
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  const int64_t max_cells = n / 2 + 4097;
+  const size_t nd = Align256(sizeof(double) * static_cast<size_t>(n));
+  const size_t ni = Align256(sizeof(int) * static_cast<size_t>(n));
+  const size_t ci = Align256(sizeof(int) * static_cast<size_t>(max_cells + 1));
+  const size_t vd = Align256(sizeof(double) * static_cast<size_t>(nodes));
+  unsigned char* p = static_cast<unsigned char*>(work);
+  auto take = [&p](size_t b) {
+    unsigned char* q = p;
+    p += b;
+    return q;
+  };
+  auto* gp = reinterpret_cast<PointGridDev*>(take(256));
+  auto* px = reinterpret_cast<double*>(take(nd));
+  auto* py = reinterpret_cast<double*>(take(nd));
+  auto* ox = reinterpret_cast<double*>(take(nd));
+  auto* oy = reinterpret_cast<double*>(take(nd));
+  auto* pnx = reinterpret_cast<double*>(take(nd));
+  auto* pny = reinterpret_cast<double*>(take(nd));
+  auto* pcorner = reinterpret_cast<double*>(take(nd));
+  auto* cell_of = reinterpret_cast<int*>(take(ni));
+  auto* sorted = reinterpret_cast<int*>(take(ni));
+  auto* count = reinterpret_cast<int*>(take(ci));
+  auto* start = reinterpret_cast<int*>(take(ci));
+  auto* fill = reinterpret_cast<int*>(take(ci));
+  auto* vnx = reinterpret_cast<double*>(take(vd));
+  auto* vny = reinterpret_cast<double*>(take(vd));
+  auto* vscale = reinterpret_cast<double*>(take(vd));
+  const int kk = static_cast<int>(std::min<int64_t>(std::max(2, j.k), n));
+  const unsigned pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
+  const unsigned cb = static_cast<unsigned>((max_cells + 1 + kThreads - 1) / kThreads);
+  const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
+  KScanWorld<<<pb, kThreads, 0, s>>>(j.scan_rows, n, j.frame, j.sensor_x, j.sensor_y, j.cos_sin, j.poses, px, py, ox,
+                                      oy);
+  KPointGrid<<<1, kGridThreads, 0, s>>>(px, py, n, max_cells, gp);
+  KClearInts<<<cb, kThreads, 0, s>>>(count, fill, max_cells + 1);
+  KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridInMemory{gp}, cell_of, count);
+  KCellScan<<<1, kScanThreads, 0, s>>>(count, GridInMemory{gp}, start);
+  KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
+  KPointNormals<<<pb, kThreads, 0, s>>>(px, py, ox, oy, n, kk, GridInMemory{gp}, start, sorted, pnx, pny, pcorner);
+  KNodeNormals<<<vb, kThreads, 0, s>>>(j.nx, j.ny, j.min_x, j.min_y, j.dx, j.dy, px, py, pnx, pny, pcorner,
+                                       GridInMemory{gp}, start, sorted, j.method, j.corner_threshold, vnx, vny,
+                                       vscale);
+  if (j.num_eikonal > 0) {
+    const unsigned eb = static_cast<unsigned>((j.num_eikonal + kThreads - 1) / kThreads);
+    KEikonalNormals<<<eb, kThreads, 0, s>>>(j.eik_w, j.eik_h, j.num_eikonal, j.nx, vnx, vny, j.eik_nx, j.eik_ny);
+  }
   HIPN(hipGetLastError());
   return SDFSLAM_OK;
 }

@@ -30,6 +30,13 @@ struct Vec2 {
   [[nodiscard]] double squaredNorm() const { return x * x + y * y; }
 };
 
+/// cos and sin of an angle from one ::sincos call. The compiler turns a
+/// std::cos and a std::sin of the same angle into that call where it can, and
+/// ::sincos can differ from the separate functions in the last bit, so every
+/// rotation calls it explicitly: the device, fed these values, places points
+/// bit for bit where the host does.
+inline void CosSin(double angle, double& c, double& s) { ::sincos(angle, &s, &c); }
+
 /// atan2(sin a, cos a), pose.hpp:9.
 inline double WrapAngle(double angle) { return std::atan2(std::sin(angle), std::cos(angle)); }
 
@@ -42,14 +49,14 @@ struct Pose2 {
   [[nodiscard]] Vec2 Translation() const { return {x, y}; }
   /// R(theta) p (pose.hpp:19-25).
   [[nodiscard]] Vec2 Rotate(const Vec2& p) const {
-    const double c = std::cos(theta);
-    const double s = std::sin(theta);
+    double c, s;
+    CosSin(theta, c, s);
     return {c * p.x - s * p.y, s * p.x + c * p.y};
   }
   /// R(theta)^T p.
   [[nodiscard]] Vec2 RotateInverse(const Vec2& p) const {
-    const double c = std::cos(theta);
-    const double s = std::sin(theta);
+    double c, s;
+    CosSin(theta, c, s);
     return {c * p.x + s * p.y, -s * p.x + c * p.y};
   }
   /// R p + t (pose.hpp:27-30).

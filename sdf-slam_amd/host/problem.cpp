@@ -259,7 +259,12 @@ Problem::Problem(GridMap map, std::vector<Pose2> poses, const std::vector<Scan>&
   }
   // Eikonal rows (problem.cpp:72-94).
   std::vector<NodeNormal> node_normals;
-  if (node_normals_fn && !options.normals.per_scan) {
+  if (node_normals_fn && node_normals_fn.target<DeferredNodeNormals>() != nullptr) {
+    if (options.normals.per_scan || options.normals.method == NormalMethod::kWeighted) {
+      throw std::invalid_argument("deferred node normals need the all-scans pca or hybrid method");
+    }
+    node_normals = DeferredNodeNormals{}({}, {}, map_, options.normals);
+  } else if (node_normals_fn && !options.normals.per_scan) {
     // The all-scans neighbourhood's inputs (ComputeScanNormals' order).
     std::vector<Vec2> points;
     std::vector<Vec2> origins;
@@ -308,6 +313,11 @@ Problem::Problem(GridMap map, std::vector<Pose2> poses, const std::vector<Scan>&
     }
   }
   BuildSoa();
+}
+
+std::vector<NodeNormal> DeferredNodeNormals::operator()(const std::vector<Vec2>&, const std::vector<Vec2>&,
+                                                        const GridMap& map, const NormalOptions&) const {
+  return std::vector<NodeNormal>(static_cast<size_t>(map.num_nodes()), NodeNormal{Vec2{0.0, 0.0}, 1.0});
 }
 
 void Problem::BuildSoa() {

@@ -126,6 +126,16 @@ using NodeNormalsFn = std::function<std::vector<NodeNormal>(
     const std::vector<Vec2>& points, const std::vector<Vec2>& origins, const GridMap& map,
     const NormalOptions& options)>;
 
+/// A NodeNormalsFn marker: the node normals are computed later, on the
+/// device, for a context update (sdfslam_hip_context_update_device_normals).
+/// The problem gets a zero normal and eikonal scale 1 at every node, which
+/// for the pca and hybrid methods (scale 1 everywhere) gives the specs the
+/// real normals give, with placeholder normal values.
+struct DeferredNodeNormals {
+  std::vector<NodeNormal> operator()(const std::vector<Vec2>& points, const std::vector<Vec2>& origins,
+                                     const GridMap& map, const NormalOptions& options) const;
+};
+
 class Problem {
  public:
   /// Builds the specs exactly as problem.cpp:11-115: point rows per frame

@@ -52,17 +52,35 @@ void HipSession::Release() {
   ctx_ = nullptr;
 }
 
-sdfslam_hip_context* HipSession::Load(const Problem& problem, const SolverOptions& options) {
+bool DeviceNormalsApply(const ProblemOptions& options) {
+  return !options.normals.per_scan && options.normals.method != NormalMethod::kWeighted;
+}
+
+void HipSession::Place(const Problem& problem, const SolverOptions& options, const NormalOptions* device_normals) {
   const sdfslam_problem_view view = problem.View();
   const sdfslam_solver_options copts = ToC(options);
   std::vector<double> map_values;
   std::vector<double> poses;
   problem.GetState(map_values, poses);
   if (ctx_ == nullptr) {
+    if (device_normals != nullptr) throw std::logic_error("device normals need an existing context");
     Rethrow(sdfslam_hip_context_create(&view, &copts, map_values.data(), poses.data(), &ctx_));
+  } else if (device_normals != nullptr) {
+    // The frames' cos and sin come from ::sincos in the backend, as Pose2 computes them.
+    sdfslam_normals_request req{};
+    req.k_neighbors = device_normals->k_neighbors;
+    req.method = device_normals->method == NormalMethod::kPca ? SDFSLAM_NORMALS_PCA : SDFSLAM_NORMALS_HYBRID;
+    req.corner_threshold = device_normals->corner_threshold;
+    req.frame_cos_sin = nullptr;
+    Rethrow(sdfslam_hip_context_update_device_normals(ctx_, &view, map_values.data(), poses.data(), &req));
   } else {
     Rethrow(sdfslam_hip_context_update(ctx_, &view, map_values.data(), poses.data()));
   }
+}
+
+sdfslam_hip_context* HipSession::Load(const Problem& problem, const SolverOptions& options,
+                                      const NormalOptions* device_normals) {
+  Place(problem, options, device_normals);
   return ctx_;
 }
 
@@ -115,18 +133,13 @@ std::vector<NodeNormal> DeviceNodeNormals(const std::vector<Vec2>& points, const
   return out;
 }
 
-SolveResult HipSession::Solve(Problem& problem, const SolverOptions& options) {
+SolveResult HipSession::Solve(Problem& problem, const SolverOptions& options, const NormalOptions* device_normals) {
   const auto t0 = std::chrono::steady_clock::now();
-  const sdfslam_problem_view view = problem.View();
   const sdfslam_solver_options copts = ToC(options);
   std::vector<double> map_values;
   std::vector<double> poses;
   problem.GetState(map_values, poses);
-  if (ctx_ == nullptr) {
-    Rethrow(sdfslam_hip_context_create(&view, &copts, map_values.data(), poses.data(), &ctx_));
-  } else {
-    Rethrow(sdfslam_hip_context_update(ctx_, &view, map_values.data(), poses.data()));
-  }
+  Place(problem, options, device_normals);
   SolveResult result;
   result.cost_history.assign(static_cast<size_t>(std::max(0, options.max_iterations)) + 1, 0.0);
   sdfslam_solve_result cres{};
@@ -226,12 +239,18 @@ IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, i
                                          wl.scans.begin() + static_cast<std::ptrdiff_t>(next_frame));
     const std::vector<Pose2> odom_so_far(wl.odometry.begin(),
                                          wl.odometry.begin() + static_cast<std::ptrdiff_t>(next_frame - 1));
-    // The device runs build their node normals on the device too.
-    Problem problem(std::move(run.map), run.estimated, scans_so_far, odom_so_far, wl.problem,
-                    wl.relations, solve_fn ? NodeNormalsFn() : NodeNormalsFn(DeviceNodeNormals));
+    // The device runs build their node normals on the device too: the first
+    // increment through a device call of its own, the later ones inside the
+    // context update (computed at the start of the solve, no host round trip).
+    const bool deferred = !solve_fn && session.HasContext() && DeviceNormalsApply(wl.problem);
+    Problem problem(std::move(run.map), run.estimated, scans_so_far, odom_so_far, wl.problem, wl.relations,
+                    deferred  ? NodeNormalsFn(DeferredNodeNormals{})
+                    : solve_fn ? NodeNormalsFn()
+                               : NodeNormalsFn(DeviceNodeNormals));
     rec.build_seconds = Seconds(t0);
     t0 = std::chrono::steady_clock::now();
-    SolveResult step = solve_fn ? SolveWith(solve_fn, problem, solver) : session.Solve(problem, solver);
+    SolveResult step = solve_fn ? SolveWith(solve_fn, problem, solver)
+                                : session.Solve(problem, solver, deferred ? &wl.problem.normals : nullptr);
     rec.solve_seconds = Seconds(t0);
     step.duration_seconds = rec.solve_seconds;
     run.estimated = problem.poses();

@@ -29,8 +29,13 @@ class HipSession {
   ~HipSession();
   /// Solve(problem, options) on the device (same results and exceptions as
   /// SolveHip); the first call creates the context, later calls replace its
-  /// problem and keep its memory.
-  SolveResult Solve(Problem& problem, const SolverOptions& options);
+  /// problem and keep its memory. With `device_normals` the problem was built
+  /// with DeferredNodeNormals and its eikonal normals are computed on the
+  /// device with these options (the session must have a context already).
+  SolveResult Solve(Problem& problem, const SolverOptions& options,
+                    const NormalOptions* device_normals = nullptr);
+  /// Whether the session has a device context (made by its first Solve/Load).
+  bool HasContext() const { return ctx_ != nullptr; }
   /// Device memory the context holds (0 before the first Solve).
   int64_t DeviceBytes() const;
   /// Frees the device buffers the context outgrew (waits for the device).
@@ -43,13 +48,20 @@ class HipSession {
   /// Load puts the problem and its state into the context (created on first
   /// use); Collect reads the solved state back into the problem and returns
   /// the result, rethrowing `status` as Solve would.
-  sdfslam_hip_context* Load(const Problem& problem, const SolverOptions& options);
+  sdfslam_hip_context* Load(const Problem& problem, const SolverOptions& options,
+                            const NormalOptions* device_normals = nullptr);
   SolveResult Collect(Problem& problem, const sdfslam_solve_result& result, int status,
                       const std::vector<double>& history);
 
  private:
+  /// Creates the context or replaces its problem.
+  void Place(const Problem& problem, const SolverOptions& options, const NormalOptions* device_normals);
   sdfslam_hip_context* ctx_{nullptr};
 };
+
+/// Whether a problem with these options can take its eikonal normals from
+/// the device (DeferredNodeNormals): all-scans neighbourhood, pca or hybrid.
+bool DeviceNormalsApply(const ProblemOptions& options);
 
 /// Per-increment record.
 struct IncrementRecord {

@@ -251,11 +251,15 @@ void Server::Prepare(Prepared& item, Pending& frame) {
     s.odometry.push_back(frame.odometry);
     rec.frames = static_cast<int>(s.scans.size());
     const auto t0 = Clock::now();
-    item.problem = std::make_unique<Problem>(
-        s.map, s.estimated, s.scans, s.odometry, s.problem, std::vector<RelationMeasurement>(),
-        options_.device_normals ? NodeNormalsFn(DeviceNodeNormals) : NodeNormalsFn());
+    // With device normals, increments after the first take their eikonal
+    // normals from the device inside the batched solve (no host kNN, no GPU
+    // call from this thread).
+    const bool deferred = options_.device_normals && s.hip.HasContext() && DeviceNormalsApply(s.problem);
+    item.problem = std::make_unique<Problem>(s.map, s.estimated, s.scans, s.odometry, s.problem,
+                                             std::vector<RelationMeasurement>(),
+                                             deferred ? NodeNormalsFn(DeferredNodeNormals{}) : NodeNormalsFn());
     rec.build_seconds = Seconds(t0, Clock::now());
-    item.ctx = s.hip.Load(*item.problem, s.step);
+    item.ctx = s.hip.Load(*item.problem, s.step, deferred ? &s.problem.normals : nullptr);
   } catch (const std::exception& e) {
     item.error = e.what();
   }

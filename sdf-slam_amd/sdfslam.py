@@ -137,6 +137,11 @@ class ServedResult(C.Structure):
                 ("first_increment_mean_ms", C.c_double)]
 
 
+class NormalsRequest(C.Structure):
+    _fields_ = [("k_neighbors", C.c_int32), ("method", C.c_int32), ("corner_threshold", C.c_double),
+                ("frame_cos_sin", C.POINTER(C.c_double))]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("graph_captures", C.c_int32), ("graph_launches", C.c_int32),
                 ("epochs_rebuilt", C.c_int32), ("setup_seconds", C.c_double),
@@ -174,7 +179,7 @@ HIP_SYMBOLS = [
     "sdfslam_hip_context_update", "sdfslam_hip_device_memory", "sdfslam_hip_batch_solve",
     "sdfslam_hip_pose_margins", "sdfslam_hip_node_normals", "sdfslam_hip_init_from_scans",
     "sdfslam_hip_init_from_scans_noalloc", "sdfslam_hip_reference_factor_counts",
-    "sdfslam_hip_elimination_order",
+    "sdfslam_hip_elimination_order", "sdfslam_hip_context_update_device_normals",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -214,6 +219,8 @@ hip.sdfslam_hip_context_destroy.argtypes = [C.c_void_p]
 hip.sdfslam_hip_context_trim.argtypes = [C.c_void_p]
 hip.sdfslam_hip_reference_factor_counts.argtypes = [C.c_void_p, _i64p, _f64p]
 hip.sdfslam_hip_elimination_order.argtypes = [C.c_void_p, _i32p]
+hip.sdfslam_hip_context_update_device_normals.argtypes = [C.c_void_p, C.POINTER(ProblemView), _f64p, _f64p,
+                                                             C.POINTER(NormalsRequest)]
 hip.sdfslam_hip_set_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_get_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_analysis.argtypes = [C.c_void_p, C.POINTER(AnalysisInfo)]
@@ -585,13 +592,24 @@ class Context:
         _check_hip(hip.sdfslam_hip_pose_margins(self._h, ptr(out, _i32p), len(out), ptr(n, _i32p)))
         return out
 
-    def update(self, problem, map_values=None, poses=None):
-        """Replaces the context's problem (sdfslam_hip_context_update)."""
+    def update(self, problem, map_values=None, poses=None, device_normals=None):
+        """Replaces the context's problem (sdfslam_hip_context_update). With
+        device_normals = (k_neighbors, method name, corner_threshold) its
+        eikonal normals are computed on the device instead
+        (sdfslam_hip_context_update_device_normals)."""
         m0, p0 = problem.state()
         m = np.ascontiguousarray(m0 if map_values is None else map_values, dtype=np.float64)
         p = np.ascontiguousarray(p0 if poses is None else poses, dtype=np.float64)
-        _check_hip(hip.sdfslam_hip_context_update(self._h, C.byref(problem.view), ptr(m, _f64p),
-                                                  ptr(p, _f64p)))
+        if device_normals is None:
+            _check_hip(hip.sdfslam_hip_context_update(self._h, C.byref(problem.view), ptr(m, _f64p),
+                                                      ptr(p, _f64p)))
+        else:
+            # cos/sin of the frame angles from ::sincos in the library, as the
+            # host problem builder places the points.
+            req = NormalsRequest(int(device_normals[0]), NORMAL_METHODS[device_normals[1]],
+                                 float(device_normals[2]), None)
+            _check_hip(hip.sdfslam_hip_context_update_device_normals(
+                self._h, C.byref(problem.view), ptr(m, _f64p), ptr(p, _f64p), C.byref(req)))
         self.problem = problem
 
     def iterate(self, k, phase_times=False):

@@ -164,3 +164,28 @@ def test_queued_session_is_admitted_after_another_closes(sdfslam):
     with pytest.raises(ValueError, match="closed"):
         server.submit(a, *(lambda f: (f[0], f[2]))(d[0].frame(3)))
     server.close()
+
+
+def test_device_normals_match_host_normals(sdfslam):
+    # Increments after a session's first take their eikonal normals from the
+    # device inside the batched solve; every state equals the host-built
+    # sessions' and the incremental runner's bit for bit.
+    print("case served device normals", flush=True)
+    frames = 7
+    states = {}
+    for device_normals in (False, True):
+        server = sdfslam.Server(workers=4, device_normals=device_normals)
+        opened = [_open(sdfslam, server, name, frames) for name in ("served5", "served6")]
+        tickets = []
+        for k in range(frames - 1):
+            for _, sid, fr in opened:
+                xy, _, odo = fr[k]
+                tickets.append((sid, server.submit(sid, xy, odo)))
+        for sid, t in tickets:
+            server.wait(sid, t)
+        states[device_normals] = [server.state(sid) for _, sid, _ in opened]
+        server.close()
+    for (p0, m0), (p1, m1) in zip(states[False], states[True]):
+        assert np.array_equal(p0, p1) and np.array_equal(m0, m1)
+    run = sdfslam.run_incremental("served5", 2000, max_frames=frames)
+    assert np.array_equal(run["poses"], states[True][0][0]) and np.array_equal(run["map"], states[True][0][1])
