@@ -210,13 +210,16 @@ def batched_leg(S):
     streams, the shape (instances per batched solve, solves in flight) picked
     by the autotuner from trials of the same iteration count (shorter trials
     over-weight the contexts' setup), against one solve at a time."""
-    best, trials = S.batch_autotune(instances=64, max_batches=(4, 8, 16), workers=(4, 8, 12),
+    best, trials = S.batch_autotune(instances=64, max_batches=(4, 8, 16, 64), workers=(1, 4, 8, 12),
                                     trial_iterations=30)
     max_batch, workers = best[0], best[1]
     full = S.batch_benchmark(instances=64, iterations=30, max_batch=max_batch, workers=workers)
+    # Every stage of an iteration one launch per level over all 64 instances.
+    one = S.batch_benchmark(instances=64, iterations=30, max_batch=64, workers=1)
     serial = S.batch_benchmark(instances=64, iterations=30, max_batch=1, workers=1)
     return {"instances": 64, "iterations": 30, "max_batch": max_batch, "workers": workers,
             "lm_iters_per_s": full["lm_iterations_per_second"],
+            "one_batch_of_64_lm_iters_per_s": one["lm_iterations_per_second"],
             "serial_lm_iters_per_s": serial["lm_iterations_per_second"],
             "speedup_vs_serial": full["lm_iterations_per_second"] / serial["lm_iterations_per_second"],
             "unknowns_total": full["unknowns"],

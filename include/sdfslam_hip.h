@@ -213,17 +213,17 @@ typedef struct sdfslam_hip_analysis_info {
   int32_t pattern_epochs;
   double analysis_seconds;
   int64_t device_bytes; /* device memory the context holds */
-  /* Launches of the 64-column panel-step kernel per factorization and
-   * their algorithmic FP64 flops (nb^2 per TRSM row, plus 2/3 nb^3 for the
-   * diagonal-block LDL^T of a front's first step -- later steps' blocks are
-   * factored by the previous step's trailing update -- summed over fronts
-   * and steps). */
+  /* Launches of the first 64-column panel step per factorization (one per
+   * level; later steps are fused into the previous step's trailing update)
+   * and their algorithmic FP64 flops (the fronts' first diagonal blocks,
+   * 2/3 nb^3 each, and nb^2 per TRSM row below them). */
   int32_t panel_step_launches;
   double panel_step_flops;
   int32_t speculative_epochs;
   /* Launches of the trailing-update / update-matrix kernels (FP64 MFMA)
    * per factorization and their algorithmic flops (2 K per lower-triangle
-   * output entry). */
+   * output entry, plus the diagonal block and TRSM rows of the step that a
+   * trailing update prepares, counted once). */
   int32_t gemm_launches;
   double gemm_flops;
 } sdfslam_hip_analysis_info;

@@ -890,8 +890,8 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
         std::max(a.level_max_children[L], static_cast<int32_t>(a.children[static_cast<size_t>(s)].size()));
   }
   for (int L = 0; L < a.num_levels; ++L) {
-    a.panel_step_launches += (a.level_max_nb[static_cast<size_t>(L)] + 63) / 64;
     if (!a.level_sn[static_cast<size_t>(L)].empty()) {
+      a.panel_step_launches += 1;
       a.gemm_launches += (a.level_max_nb[static_cast<size_t>(L)] + 63) / 64 +
                          (a.level_max_m[static_cast<size_t>(L)] > 0 ? 1 : 0);
     }
@@ -902,9 +902,15 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
     for (int c0 = 0; c0 < ns; c0 += 64) {
       const double nb = std::min(64, ns - c0);
       const double rows = f - c0 - nb;
-      // The first step factors its diagonal block; later steps find theirs
-      // factored by the previous step's trailing update.
-      a.panel_step_flops += (c0 == 0 ? 2.0 / 3.0 * nb * nb * nb : 0.0) + rows * nb * nb;
+      // Diagonal block LDL^T and the TRSM of the rows below it: the first
+      // step's in the panel-step launch, every later step's in the previous
+      // step's trailing update (counted once; its copies are not work).
+      const double step = 2.0 / 3.0 * nb * nb * nb + rows * nb * nb;
+      if (c0 == 0) {
+        a.panel_step_flops += step;
+      } else {
+        a.gemm_flops += step;
+      }
       // Trailing update: the lower part of the rows and columns beyond the block.
       const double big_n = std::max(0.0, ns - c0 - nb);
       const double big_m = f - c0 - nb;

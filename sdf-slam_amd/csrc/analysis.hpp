@@ -100,16 +100,16 @@ struct Analysis {
   int64_t pattern_nnz_lower{0};
   int64_t factor_nnz{0};
   double factor_flops{0.0};
-  // Panel steps per factorization (launches of the 64-column step) and
-  // their algorithmic FP64 flops: per front and step, the TRSM of the rows
-  // below the diagonal block (nb^2 per row), and for the first step the
-  // block's LDL^T (2/3 nb^3; later blocks are factored by the previous
-  // step's trailing update).
+  // Launches of the first 64-column panel step per factorization (one per
+  // level) and their algorithmic FP64 flops: per front, the diagonal
+  // block's LDL^T (2/3 nb^3) and the TRSM of the rows below it (nb^2 per
+  // row).
   int32_t panel_step_launches{0};
   double panel_step_flops{0.0};
   // Trailing-update and update-matrix launches per factorization (one per
   // step and level, one per level with update matrices) and their flops:
-  // 2 K per lower-triangle output entry.
+  // 2 K per lower-triangle output entry, plus the next step's diagonal
+  // block and TRSM that a trailing update does (as above, counted once).
   int32_t gemm_launches{0};
   double gemm_flops{0.0};
 };

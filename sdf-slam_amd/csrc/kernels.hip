@@ -1157,14 +1157,16 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
   }
 }
 
-// One 64-column step of the blocked panel factorization of every front at
-// the level: the panel rows below the diagonal block, 64 per workgroup,
-// become L21 = A21 W with W = L_kk^-T D_k^-1, a 64 x 64 x 64 FP64 MFMA
-// product. W is read from the staging slot (KDiagBlock or the previous
-// step's look-ahead filled it) or, with kFused, formed by wave 0 of
-// every workgroup, which factors and inverts the block itself; workgroup 0
-// then stores the staging slot. The trailing-update kernel moves the factored block into the
-// panel, so no workgroup of this launch reads a partly written block.
+// The first 64-column step of the blocked panel factorization of every
+// front at the level (later steps are fused into the previous step's
+// trailing update, KGemmAhead): the panel rows below the diagonal block, 64
+// per workgroup, become L21 = A21 W with W = L_kk^-T D_k^-1, a
+// 64 x 64 x 64 FP64 MFMA product. W is read from the staging slot
+// (KDiagBlock filled it) or, with kFused, formed by wave 0 of every
+// workgroup, which factors and inverts the block itself; workgroup 0 then
+// stores the staging slot. The trailing-update kernel moves the factored
+// block into the panel, so no workgroup of this launch reads a partly
+// written block.
 template <bool kFused, class Slots, class Lists>
 __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, int kb) {
   BIND_SLOT(blockIdx.z);
@@ -1304,94 +1306,44 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
   }
 }
 
-// C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
-// mode 0: trailing panel update after block column kb (K = nb);
-// mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated.
-// With kLookAhead (mode 0), the workgroup of tile (0, 0) -- the next
-// block column's diagonal block, final once its update is stored -- then
-// factors and inverts that block and stages it for step kb + 1, whose panel
-// step then only multiplies rows by W (the other tiles' updates run beside
-// the factorization).
-// Tile (ti, tj) (tj <= ti) of front s in slot `slot`.
-template <bool kLookAhead, int kMode, class Slots>
-__device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int s, int ti, int tj, int kb) {
-  constexpr int mode = kMode;
-  BIND_SLOT(slot);
-  if (Halted(W.state)) return;
-  const int ns = A.sn_first[s + 1] - A.sn_first[s];
-  const int f = A.sn_f[s];
-  double* Fw = W.panel + A.panel_off[s];
-  const double* F = Fw;
-  const double* stage = StageSlot(W, s, kb);
-  if constexpr (mode == 0) {
-    const int c0 = kb * kNB;
-    if (c0 >= ns) return;
-    if (ti == 0 && tj == 0) {
-      // The step kernel's factored diagonal block goes into the panel here.
-      const int nb = min(kNB, ns - c0);
-      for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int i = idx % nb;
-        const int j = idx / nb;
-        if (i >= j) Fw[static_cast<int64_t>(c0 + j) * f + c0 + i] = stage[j * kNB + i];
-      }
-    }
-  }
-  int kbase, K, rbase, M, N;
-  double* C;
-  int64_t ldc;
-  if constexpr (mode == 0) {
-    const int c0 = kb * kNB;
-    if (c0 >= ns) return;
-    const int nb = min(kNB, ns - c0);
-    kbase = c0;
-    K = nb;
-    rbase = c0 + nb;  // panel row of tile row 0 and panel column of tile col 0
-    M = f - rbase;
-    N = ns - rbase;
-    C = W.panel + A.panel_off[s] + static_cast<int64_t>(rbase) * f + rbase;
-    ldc = f;
-  } else {
-    const int ms = A.sn_m[s];
-    kbase = 0;
-    K = ns;
-    rbase = ns;
-    M = ms;
-    N = ms;
-    C = W.upd + A.upd_off[s];
-    ldc = ms;
-  }
-  const int R0 = ti * kTile;
-  const int C0 = tj * kTile;
-  if (R0 >= M || C0 >= N) return;
-  // Rows of the staged chunks are padded by two doubles, so the two
-  // k rows an MFMA operand read touches fall in different LDS banks.
-  // One LDS array: the look-ahead factorization reuses it whole.
-  __shared__ double lds[2 * 2 * kTk * (kTile + 2)];
-  static_assert(2 * 2 * kTk * (kTile + 2) >= kNB * (kNB + 2), "look-ahead scratch");
-  auto As = reinterpret_cast<double (*)[kTk][kTile + 2]>(lds);
-  auto Bs = reinterpret_cast<double (*)[kTk][kTile + 2]>(lds + 2 * kTk * (kTile + 2));
+// acc[t] = A_t (B D)^T, k = 0 .. K - 1, on FP64 MFMA for kTiles 64x64 tiles
+// that share their B rows: A_t holds panel rows rbase + r0[t] .. + 63 and B
+// rows rbase + cb0 .. + 63 of panel columns kbase .. kbase + K - 1 (rows
+// from M, resp. N, on and k from K on are zero); D(k) = dk[k * dstride].
+// Each wavefront owns a 32x32 quadrant of every tile, four 16x16 blocks. The
+// MFMA's A operand is the B chunk (its 16 rows are tile columns) and its B
+// operand the A chunk (its 16 columns are tile rows), so that lane l's
+// result e is tile row (l & 15), column 4 e + (l >> 4) of its block: a
+// wavefront's C accesses are 16-row runs of 4 columns (coalesced in the
+// column-major panel / update matrix). K chunks of kTk are double-buffered
+// in `lds` (2 (kTiles + 1) kTk (kTile + 2) doubles; rows padded by two
+// doubles, so the two k rows an MFMA operand read touches fall in different
+// banks), the next chunk's loads in flight while one is multiplied. With
+// kRelease the last barrier comes after the last product, so the LDS is
+// free on return.
+template <int kTiles, bool kRelease>
+__device__ __forceinline__ void TileProducts(const double* F, int f, int kbase, int K, int rbase, int M, int N,
+                                             const int (&r0)[kTiles], int cb0, const double* dk,
+                                             int64_t dstride, double* lds, v4d (&acc)[kTiles][2][2]) {
+  auto As = reinterpret_cast<double (*)[kTiles][kTk][kTile + 2]>(lds);
+  auto Bs = reinterpret_cast<double (*)[kTk][kTile + 2]>(lds + 2 * kTiles * kTk * (kTile + 2));
   const int tid = threadIdx.x;
-  // FP64 MFMA (16x16x4): each wavefront owns a 32x32 quadrant of the tile,
-  // four 16x16 blocks. The MFMA's A operand is the B chunk (its 16 rows are
-  // tile columns) and its B operand the A chunk (its 16 columns are tile
-  // rows), so that lane l's result e is tile row (l & 15), column
-  // 4 e + (l >> 4) of its block: a wavefront's C accesses are 16-row runs of
-  // 4 columns (coalesced in the column-major panel / update matrix).
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wr = 32 * (wave & 1);
   const int wc = 32 * (wave >> 1);
-  v4d acc[2][2];  // [column block][row block]
 #pragma unroll
-  for (int cb = 0; cb < 2; ++cb)
+  for (int t = 0; t < kTiles; ++t)
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb) acc[cb][rb] = v4d{0.0, 0.0, 0.0, 0.0};
-  // Staging of one K chunk: each thread loads 4 A and 4 B entries (rows
-  // rr = idx & 63 of k row kk = idx >> 6); B is scaled by D_k. Every load
-  // is issued at an address clamped into the operands and the padding
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) acc[t][cb][rb] = v4d{0.0, 0.0, 0.0, 0.0};
+  // Staging of one K chunk: each thread loads 4 entries of every operand
+  // (rows rr = idx & 63 of k row kk = idx >> 6); B is scaled by D_k. Every
+  // load is issued at an address clamped into the operands and the padding
   // selected afterwards: a conditional load would become a branch that
   // waits for its data before the next load is issued.
-  double ra[4], rb[4];
+  double ra[kTiles][4], rb[4];
   auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -1401,19 +1353,22 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
       const bool kin = k0 + kk < K;
       const int kq = min(k0 + kk, K - 1);
       const int64_t colbase = static_cast<int64_t>(kbase + kq) * f;
-      const double* dk_at = mode == 0 ? stage + kq * kNB + kq : F + colbase + kbase + kq;
-      const double dk = *dk_at;
-      const double a = F[colbase + rbase + min(R0 + rr, M - 1)];
-      const double b = F[colbase + rbase + min(C0 + rr, N - 1)];
-      ra[e] = (kin && R0 + rr < M) ? a : 0.0;
-      rb[e] = (kin && C0 + rr < N) ? b * dk : 0.0;
+      const double d = dk[kq * dstride];
+#pragma unroll
+      for (int t = 0; t < kTiles; ++t) {
+        const double a = F[colbase + rbase + min(r0[t] + rr, M - 1)];
+        ra[t][e] = (kin && r0[t] + rr < M) ? a : 0.0;
+      }
+      const double b = F[colbase + rbase + min(cb0 + rr, N - 1)];
+      rb[e] = (kin && cb0 + rr < N) ? b * d : 0.0;
     }
   };
   auto store_chunk = [&](int buf) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int idx = tid + kThreads * e;
-      As[buf][idx >> 6][idx & 63] = ra[e];
+#pragma unroll
+      for (int t = 0; t < kTiles; ++t) As[buf][t][idx >> 6][idx & 63] = ra[t][e];
       Bs[buf][idx >> 6][idx & 63] = rb[e];
     }
   };
@@ -1427,17 +1382,21 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
 #pragma unroll
     for (int kk = 0; kk < kTk; kk += 4) {
       const int kr = kk + (lane >> 4);
-      double av[2], bv[2];
+      double av[kTiles][2], bv[2];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) av[rb] = As[buf][kr][wr + 16 * rb + (lane & 15)];
+      for (int t = 0; t < kTiles; ++t)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) av[t][rb] = As[buf][t][kr][wr + 16 * rb + (lane & 15)];
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb) bv[cb] = Bs[buf][kr][wc + 16 * cb + (lane & 15)];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb)
+      for (int t = 0; t < kTiles; ++t)
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-          acc[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[cb], av[rb], acc[cb][rb], 0, 0, 0);
-        }
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) {
+            acc[t][cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[cb], av[t][rb], acc[t][cb][rb], 0, 0, 0);
+          }
     }
     if (more) {
       store_chunk(buf ^ 1);
@@ -1445,15 +1404,28 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
       buf ^= 1;
     }
   }
-  // With look-ahead, tile (0, 0) is rows and columns c1 .. c1 + 63 of the
-  // panel: the next diagonal block, final once updated here. Its values
-  // also go to the LDS (column-major, kNB + 1 apart), once every wavefront is
-  // done with the staged chunks, for the factorization below.
-  const int c1 = (kb + 1) * kNB;
-  const bool ahead = kLookAhead && mode == 0 && ti == 0 && tj == 0 && c1 < ns;
-  if (ahead) __syncthreads();
-  // Mode 0 accumulates into C: its 16 entries are loaded together first
-  // (clamped addresses, as the chunks above), then the lower ones stored.
+  if (kRelease) __syncthreads();
+}
+
+// The factored diagonal block of step kb (staging slot: L below the
+// diagonal, D on it) goes into the panel, by the workgroup of tile (0, 0).
+__device__ __forceinline__ void MoveStagedBlock(double* F, int f, int c0, int nb, const double* stage) {
+  for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+    const int i = idx % nb;
+    const int j = idx / nb;
+    if (i >= j) F[static_cast<int64_t>(c0 + j) * f + c0 + i] = stage[j * kNB + i];
+  }
+}
+
+// C -= acc over the tile's lower entries (c <= r, inside M x N): the 16
+// entries of a lane are loaded together first (clamped addresses), then
+// the lower ones stored.
+__device__ __forceinline__ void SubtractLower(double* C, int64_t ldc, int R0, int C0, int M, int N,
+                                              const v4d (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wr = 32 * (wave & 1);
+  const int wc = 32 * (wave >> 1);
   double prev[2][4][2];
 #pragma unroll
   for (int cb = 0; cb < 2; ++cb) {
@@ -1463,7 +1435,7 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb) {
         const int r = min(R0 + wr + 16 * rb + (lane & 15), M - 1);
-        prev[cb][e][rb] = mode == 0 ? C[static_cast<int64_t>(c) * ldc + r] : 0.0;
+        prev[cb][e][rb] = C[static_cast<int64_t>(c) * ldc + r];
       }
     }
   }
@@ -1475,37 +1447,250 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb) {
         const int r = R0 + wr + 16 * rb + (lane & 15);
-        if (c < N && r < M && c <= r) {
-          const double v = prev[cb][e][rb] - acc[cb][rb][e];
-          C[static_cast<int64_t>(c) * ldc + r] = v;
-          if (ahead) lds[c * (kNB + 1) + r] = v;
+        if (c < N && r < M && c <= r) C[static_cast<int64_t>(c) * ldc + r] = prev[cb][e][rb] - acc[cb][rb][e];
+      }
+    }
+  }
+}
+
+// C -= A_blk (L_blk D)^T over the lower triangle, 64x64 tiles.
+// mode 0: trailing panel update after block column kb (K = nb), whose
+// factored diagonal block tile (0, 0)'s workgroup first moves into the
+// panel;
+// mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated.
+// Tile (ti, tj) (tj <= ti) of front s in slot `slot`.
+template <int kMode, class Slots>
+__device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int s, int ti, int tj, int kb) {
+  constexpr int mode = kMode;
+  BIND_SLOT(slot);
+  if (Halted(W.state)) return;
+  const int ns = A.sn_first[s + 1] - A.sn_first[s];
+  const int f = A.sn_f[s];
+  double* Fw = W.panel + A.panel_off[s];
+  int kbase, K, rbase, M, N;
+  double* C;
+  int64_t ldc;
+  const double* dk;
+  int64_t dstride;
+  if constexpr (mode == 0) {
+    const int c0 = kb * kNB;
+    if (c0 >= ns) return;
+    const int nb = min(kNB, ns - c0);
+    const double* stage = StageSlot(W, s, kb);
+    if (ti == 0 && tj == 0) MoveStagedBlock(Fw, f, c0, nb, stage);
+    kbase = c0;
+    K = nb;
+    rbase = c0 + nb;  // panel row of tile row 0 and panel column of tile col 0
+    M = f - rbase;
+    N = ns - rbase;
+    C = Fw + static_cast<int64_t>(rbase) * f + rbase;
+    ldc = f;
+    dk = stage;
+    dstride = kNB + 1;
+  } else {
+    const int ms = A.sn_m[s];
+    kbase = 0;
+    K = ns;
+    rbase = ns;
+    M = ms;
+    N = ms;
+    C = W.upd + A.upd_off[s];
+    ldc = ms;
+    dk = Fw;  // D on the factored panel's diagonal
+    dstride = static_cast<int64_t>(f) + 1;
+  }
+  const int R0 = ti * kTile;
+  const int C0 = tj * kTile;
+  if (R0 >= M || C0 >= N) return;
+  __shared__ double lds[2 * 2 * kTk * (kTile + 2)];
+  v4d acc[1][2][2];
+  const int r0[1] = {R0};
+  TileProducts<1, false>(Fw, f, kbase, K, rbase, M, N, r0, C0, dk, dstride, lds, acc);
+  if constexpr (mode == 0) {
+    SubtractLower(C, ldc, R0, C0, M, N, acc[0]);
+  } else {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wr = 32 * (wave & 1);
+    const int wc = 32 * (wave >> 1);
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = C0 + wc + 16 * cb + 4 * e + (lane >> 4);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          const int r = R0 + wr + 16 * rb + (lane & 15);
+          if (c < N && r < M && c <= r) C[static_cast<int64_t>(c) * ldc + r] = 0.0 - acc[0][cb][rb][e];
         }
       }
     }
   }
-  if constexpr (kLookAhead) {
-    if (!ahead) return;
-    __syncthreads();
-    // Wave 0 factors and inverts the block (the tile, then D, in the LDS);
-    // then the whole workgroup stores it.
-    double* dvec = lds + kNB * (kNB + 1);
-    if (tid < kNB) {
-      // Row tid of the block (lower triangle; identity pad beyond nb1). Every
-      // entry read was written above: nb1 <= N <= M.
-      const int nb1 = min(kNB, ns - c1);
-      double row[kNB];
+}
+
+// LDS of the fused step: the K chunks of two tiles, then the diagonal tile
+// (column-major, kNB + 1 apart), D, and the TRSM rows.
+constexpr int kAheadLd = kNB + 1;
+constexpr int kAheadLds = kNB * kAheadLd + kNB + kNB * kLdT;
+static_assert(kAheadLds >= 2 * 3 * kTk * (kTile + 2), "fused step staging");
+
+// Trailing update of step kb fused with step kb + 1's panel step. Tile
+// (ti, tj), tj <= ti, of front s. The tiles of column 0 are the next block
+// column. Each of their workgroups also updates the next diagonal block
+// (tile (0, 0)), factors and inverts it in wave 0 -- every copy computes
+// the same values from the same inputs, bit for bit -- and multiplies its
+// own updated rows by W = L^-T D^-1 (in tile (0, 0) the rows below the
+// block's nb1 columns), so step kb + 1 has no launch of its own. Nothing
+// here writes the diagonal tile into the panel (the copies read its values
+// before the update); the workgroup of tile (0, 0) stages the factored
+// block (and L^-1 for the solves) and sets the zero-pivot flag, and the
+// next launch moves the block into the panel. The other tiles are plain
+// trailing updates. A tile past the block column's rows or columns exits,
+// so from the products on N > 0: block kb is full and block kb + 1 starts
+// at panel column rbase.
+template <class Slots>
+__device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, int ti, int tj, int kb) {
+  BIND_SLOT(slot);
+  if (Halted(W.state)) return;
+  const int ns = A.sn_first[s + 1] - A.sn_first[s];
+  const int f = A.sn_f[s];
+  double* Fw = W.panel + A.panel_off[s];
+  const int c0 = kb * kNB;
+  if (c0 >= ns) return;
+  const int nb = min(kNB, ns - c0);
+  const double* stage = StageSlot(W, s, kb);
+  if (ti == 0 && tj == 0) MoveStagedBlock(Fw, f, c0, nb, stage);
+  const int rbase = c0 + nb;
+  const int M = f - rbase;
+  const int N = ns - rbase;
+  const int R0 = ti * kTile;
+  const int C0 = tj * kTile;
+  if (R0 >= M || C0 >= N) return;
+  double* C = Fw + static_cast<int64_t>(rbase) * f + rbase;
+  __shared__ double lds[kAheadLds];
+  if (tj > 0) {
+    v4d acc[1][2][2];
+    const int r0[1] = {R0};
+    TileProducts<1, false>(Fw, f, c0, nb, rbase, M, N, r0, C0, stage, kNB + 1, lds, acc);
+    SubtractLower(C, f, R0, C0, M, N, acc[0]);
+    return;
+  }
+  const int nb1 = min(kNB, N);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wr = 32 * (wave & 1);
+  const int wc = 32 * (wave >> 1);
+  double* lt = lds;                         // the diagonal tile, then its factor
+  double* dvec = lds + kNB * kAheadLd;      // D
+  auto at = reinterpret_cast<double (*)[kLdT]>(lds + kNB * kAheadLd + kNB);  // at[k][r]: TRSM rows
+  // The updated tile (0, 0) goes to lt (rows below nb1: to at, in tile
+  // (0, 0)'s own workgroup), tile (ti, 0) to at; everything else in at is
+  // zero (rows past M, columns past nb1).
+  auto put = [&](const v4d (&acc)[2][2], int t_r0, bool diag, bool rows) {
+    double prev[2][4][2];
 #pragma unroll
-      for (int j = 0; j < kNB; ++j) {
-        row[j] = (tid < nb1 && j < nb1 && j <= tid) ? lds[j * (kNB + 1) + tid] : (j == tid ? 1.0 : 0.0);
-      }
-      const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb1, lds, dvec);
-      if (tid == 0 && zero) {
-        W.state->not_spd = 1;
-        W.state->halt = 1;
+    for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = min(wc + 16 * cb + 4 * e + (lane >> 4), N - 1);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          const int r = min(t_r0 + wr + 16 * rb + (lane & 15), M - 1);
+          prev[cb][e][rb] = C[static_cast<int64_t>(c) * f + r];
+        }
       }
     }
-    __syncthreads();
-    StoreFactored<kNB + 1, kThreads>(lds, dvec, StageSlot(W, s, kb + 1), DiagInv(A, W, s, kb + 1), tid);
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = wc + 16 * cb + 4 * e + (lane >> 4);
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          const int r = wr + 16 * rb + (lane & 15);
+          const double v = prev[cb][e][rb] - acc[cb][rb][e];
+          if (diag && r < nb1) {
+            if (c <= r && c < N && r < M) lt[c * kAheadLd + r] = v;
+            if (rows) at[c][r] = 0.0;
+          } else if (rows) {
+            at[c][r] = (c < nb1 && t_r0 + r < M) ? v : 0.0;
+          }
+        }
+      }
+    }
+  };
+  if (ti == 0) {
+    v4d acc[1][2][2];
+    const int r0[1] = {0};
+    TileProducts<1, true>(Fw, f, c0, nb, rbase, M, N, r0, 0, stage, kNB + 1, lds, acc);
+    put(acc[0], 0, true, true);
+  } else {
+    v4d acc[2][2][2];
+    const int r0[2] = {0, R0};
+    TileProducts<2, true>(Fw, f, c0, nb, rbase, M, N, r0, 0, stage, kNB + 1, lds, acc);
+    put(acc[0], 0, true, false);
+    put(acc[1], R0, false, true);
+  }
+  __syncthreads();
+  if (tid < kNB) {
+    // Row tid of the block (lower triangle; identity pad beyond nb1).
+    double row[kNB];
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      row[j] = (tid < nb1 && j < nb1 && j <= tid) ? lt[j * kAheadLd + tid] : (j == tid ? 1.0 : 0.0);
+    }
+    const bool zero = FactorInvertWave<kAheadLd>(row, tid, nb1, lt, dvec);
+    if (ti == 0 && tid == 0 && zero) {
+      W.state->not_spd = 1;
+      W.state->halt = 1;
+    }
+  }
+  __syncthreads();
+  if (ti == 0) StoreFactored<kAheadLd, kThreads>(lt, dvec, StageSlot(W, s, kb + 1), DiagInv(A, W, s, kb + 1), tid);
+  // L21 = A21 W, W(k, j) = M(j, k) / D(j) above the diagonal and 1 / D(j)
+  // on it (lt holds M(j, k) at j * kAheadLd + k and 1 / D(j) at
+  // j * kAheadLd + j), the same 64 x 64 x 64 FP64 MFMA product as the
+  // panel step's.
+  v4d acc[2][2];
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) acc[cb][rb] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kNB; k += 4) {
+    const int kr = k + (lane >> 4);
+    double av[2], bv[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) av[rb] = at[kr][wr + 16 * rb + (lane & 15)];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      const int j = wc + 16 * cb + (lane & 15);
+      const double m = lt[j * kAheadLd + kr];
+      const double id = lt[j * kAheadLd + j];
+      bv[cb] = kr < j ? m * id : (kr == j ? m : 0.0);
+    }
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        acc[cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[cb], av[rb], acc[cb][rb], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int c = wc + 16 * cb + 4 * e + (lane >> 4);
+      if (c >= nb1) continue;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int r = wr + 16 * rb + (lane & 15);
+        if (R0 + r < M && (ti > 0 || r >= nb1)) {
+          Fw[static_cast<int64_t>(rbase + c) * f + rbase + R0 + r] = acc[cb][rb][e];
+        }
+      }
+    }
   }
 }
 
@@ -1516,20 +1701,19 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots, Lists pla
   const int slot = static_cast<int>(blockIdx.z) / fronts;
   const int s = PlanItem(plan, slot, static_cast<int>(blockIdx.z) % fronts);
   if (s < 0) return;
-  GemmLowerBody<false, 0>(slots, slot, s, blockIdx.x, blockIdx.y, kb);
+  GemmLowerBody<0>(slots, slot, s, blockIdx.x, blockIdx.y, kb);
 }
 
-// Trailing update with look-ahead (the diagonal factorization keeps a row of
-// 64 doubles and a straight-line schedule in registers: one workgroup per
-// CU).
+// Trailing update of step kb with step kb + 1's diagonal block and panel
+// rows (same grid; the diagonal factorization keeps a row of 64 doubles and
+// a straight-line schedule in registers).
 template <class Slots, class Lists>
-__global__ void __launch_bounds__(kThreads, 1) KGemmLowerAhead(Slots slots, Lists plan, int kb,
-                                                               int fronts) {
+__global__ void __launch_bounds__(kThreads, 1) KGemmAhead(Slots slots, Lists plan, int kb, int fronts) {
   if (blockIdx.y > blockIdx.x) return;
   const int slot = static_cast<int>(blockIdx.z) / fronts;
   const int s = PlanItem(plan, slot, static_cast<int>(blockIdx.z) % fronts);
   if (s < 0) return;
-  GemmLowerBody<true, 0>(slots, slot, s, blockIdx.x, blockIdx.y, kb);
+  AheadBody(slots, slot, s, blockIdx.x, blockIdx.y, kb);
 }
 
 // Update matrices of the level's fronts over a flat list of their lower
@@ -1543,7 +1727,7 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmUpdate(Slots slots, Lists ti
   if (t >= lr.count) return;
   const int s = lr.items[2 * t];
   const int packed = lr.items[2 * t + 1];
-  GemmLowerBody<false, 1>(slots, slot, s, packed >> 16, packed & 0xffff, 0);
+  GemmLowerBody<1>(slots, slot, s, packed >> 16, packed & 0xffff, 0);
 }
 
 // ---------------------------------------------------------------- solves
@@ -2197,27 +2381,29 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
       const auto plan = LevelLists<kOne>(sh, L);
       const int nblocks = (lv.max_nb + kNB - 1) / kNB;
       for (int kb = 0; kb < nblocks; ++kb) {
-        // Rows below the block: at most f - c0 - 1 (a front's block has at
-        // least one column); one 64-row workgroup each, at least one.
-        const int rows = lv.max_f - kb * kNB - 1;
-        const int row_blocks = rows > 0 ? (rows + kTile - 1) / kTile : 1;
-        // Step 0: with few fronts each workgroup factors the diagonal block
-        // itself (one launch on the critical path), with many the block is
-        // factored once per front first. Later steps find their block staged
-        // by the previous step's trailing update (look-ahead).
-        const int staged =
-            kb > 0 || static_cast<int64_t>(row_blocks) * count * sh.slots > kStagedMinBlocks ? 1 : 0;
-        const bool timed = step_events != nullptr && step_events->used < step_events->capacity;
-        if (timed) (void)hipEventRecord(step_events->events[2 * step_events->used], s);
-        if (kb == 0 && staged != 0) KDiagBlock<<<dim3(count, k), kNB, 0, s>>>(slots, plan, kb);
-        if (staged != 0) {
-          KPanelStep<false><<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb);
-        } else {
-          KPanelStep<true><<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb);
-        }
-        if (timed) {
-          (void)hipEventRecord(step_events->events[2 * step_events->used + 1], s);
-          ++step_events->used;
+        // The first step's panel: with few fronts each workgroup factors
+        // the diagonal block itself (one launch on the critical path), with
+        // many the block is factored once per front first. Every later
+        // step's diagonal block and panel rows are done by the previous
+        // step's trailing update (KGemmAhead).
+        if (kb == 0) {
+          // Rows below the block: at most f - 1 (a front's block has at
+          // least one column); one 64-row workgroup each, at least one.
+          const int rows = lv.max_f - 1;
+          const int row_blocks = rows > 0 ? (rows + kTile - 1) / kTile : 1;
+          const bool staged = static_cast<int64_t>(row_blocks) * count * sh.slots > kStagedMinBlocks;
+          const bool timed = step_events != nullptr && step_events->used < step_events->capacity;
+          if (timed) (void)hipEventRecord(step_events->events[2 * step_events->used], s);
+          if (staged) {
+            KDiagBlock<<<dim3(count, k), kNB, 0, s>>>(slots, plan, kb);
+            KPanelStep<false><<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb);
+          } else {
+            KPanelStep<true><<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb);
+          }
+          if (timed) {
+            (void)hipEventRecord(step_events->events[2 * step_events->used + 1], s);
+            ++step_events->used;
+          }
         }
         const int trail_rows = lv.max_f - (kb + 1) * kNB;
         const int trail_cols = lv.max_nb - (kb + 1) * kNB;
@@ -2229,7 +2415,7 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
         const bool gt = step_events != nullptr && step_events->gemm_used < step_events->gemm_capacity;
         if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used], s);
         if (kb + 1 < nblocks) {
-          KGemmLowerAhead<<<grid, kThreads, 0, s>>>(slots, plan, kb, count);
+          KGemmAhead<<<grid, kThreads, 0, s>>>(slots, plan, kb, count);
         } else {
           KGemmLower<<<grid, kThreads, 0, s>>>(slots, plan, kb, count);
         }
