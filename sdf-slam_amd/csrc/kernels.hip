@@ -922,179 +922,228 @@ __device__ __forceinline__ double Recip(double d) {
   return r;
 }
 
-// Steps K0 .. K0 + 15 of FactorDiagWave (compile-time indices keep `row` in
-// registers). On entry c_next = this lane's unscaled c_ik of the first step,
-// already broadcast in `col`, d_next its pivot and r_next = Recip(d_next).
-template <int K0>
-__device__ __forceinline__ void FactorSteps(double (&row)[kNB], int lane, double* col, bool& zero,
-                                            double& c_next, double& d_next, double& r_next) {
+// Pivots kC .. kC + 15 of the blocked LDL^T of a 64x64 block, one
+// wavefront, row `lane` of the panel in r[j] = T(lane, kC + j): right-looking
+// over the panel's 16 columns only. Step k: every lane broadcasts its
+// unscaled c = r[k] through the LDS vector `col` while the pivot d = c of
+// lane kC + k comes with v_readlane, so the broadcast overlaps the
+// reciprocal (v_rcp_f64 and two Newton steps); then r[j] -= (c / d) T(kC + j,
+// kC + k) for j > k. Step k + 1's column is updated first and its broadcast,
+// pivot and reciprocal issued before the rest of step k's update. LDS
+// operations of one wavefront complete in order, so no barrier is needed.
+// On return r[j] holds the multiplier L(lane, kC + j) (scratch where
+// lane <= kC + j) and dvec[kC + k] = D(kC + k).
+template <int kC>
+__device__ __forceinline__ void PanelPivots(double (&r)[16], int lane, double* col, double* dvec, bool& zero) {
+  double c_next = r[0];
+  col[lane] = c_next;
+  double d_next = ReadLane(c_next, kC);
+  double r_next = Recip(d_next);
 #pragma unroll
-  for (int kk = 0; kk < 16; ++kk) {
-    constexpr int kBase = K0;
-    const int k = kBase + kk;
+  for (int k = 0; k < 16; ++k) {
     const double c = c_next;
     const double d = d_next;
     zero = zero || d == 0.0;
-    col[kNB + k] = d;
+    dvec[kC + k] = d;
     const double l = c * r_next;
-    row[k] = l;
-    double cj[kNB];
+    r[k] = l;
+    double cj[16];
 #pragma unroll
-    for (int j = k + 1; j < kNB; ++j) cj[j] = col[j];
-    if (k + 1 < kNB) {
-      // Look-ahead: the next step's column first, its broadcast (after this
-      // step's reads in program order) and its pivot's reciprocal issued
-      // before the rest of this step's update.
-      row[k + 1] = fma(-l, cj[k + 1], row[k + 1]);
-      c_next = row[k + 1];
+    for (int j = k + 1; j < 16; ++j) cj[j] = col[kC + j];
+    if (k + 1 < 16) {
+      r[k + 1] = fma(-l, cj[k + 1], r[k + 1]);
+      c_next = r[k + 1];
       col[lane] = c_next;
-      d_next = ReadLane(c_next, k + 1);
+      d_next = ReadLane(c_next, kC + k + 1);
       r_next = Recip(d_next);
     }
 #pragma unroll
-    for (int j = k + 2; j < kNB; ++j) row[j] = fma(-l, cj[j], row[j]);
+    for (int j = k + 2; j < 16; ++j) r[j] = fma(-l, cj[j], r[j]);
   }
 }
 
-// Right-looking LDL^T of a 64x64 block held by one wavefront, row `lane` in
-// `row` (lower triangle and diagonal meaningful; the upper part is scratch;
-// rows and columns from nb on are an identity pad, whose steps would change
-// nothing in the lower triangle: they are skipped in groups of 16, and
-// within a group the steps form one straight-line block the compiler can
-// interleave). Step k: every lane broadcasts its unscaled c_ik = row[k]
-// through the LDS vector `col` (kNB doubles) while the pivot d = c_kk comes
-// from lane k with v_readlane, so the broadcast overlaps the reciprocal;
-// then row[j] -= (c_ik / d) c_jk for j > k. Step k + 1's column is updated
-// first and its broadcast, pivot and reciprocal are issued before the rest
-// of step k's update, so the LDS round trip and the reciprocal of one step
-// overlap the row update of the previous one. Every lane stores its
-// multiplier in row[k] (on and above the diagonal that is scratch: a
-// per-step lane test would hold 64 masks live) and the pivots go to
-// col[kNB + k] (1 for the padding). LDS operations of one wavefront complete
-// in order, so no barrier is needed. `col` holds 2 kNB doubles. On return
-// row[j < lane] = L(lane, j) and col[kNB + j] = D(j). Returns true when a
-// pivot is exactly zero.
-__device__ __forceinline__ bool FactorDiagWave(double (&row)[kNB], int lane, int nb, double* col) {
-  bool zero = false;
-  col[kNB + lane] = 1.0;
-  double c_next = row[0];
-  col[lane] = c_next;
-  double d_next = ReadLane(c_next, 0);
-  double r_next = Recip(d_next);
-  FactorSteps<0>(row, lane, col, zero, c_next, d_next, r_next);
-  if (nb > 16) FactorSteps<16>(row, lane, col, zero, c_next, d_next, r_next);
-  if (nb > 32) FactorSteps<32>(row, lane, col, zero, c_next, d_next, r_next);
-  if (nb > 48) FactorSteps<48>(row, lane, col, zero, c_next, d_next, r_next);
-  return zero;
+// Panel kP (columns 16 kP .. + 15) of the blocked LDL^T, wave 0: loads the
+// panel's rows (identity pad beyond nb), runs its pivots and writes the
+// multipliers back over the panel (every row: below the diagonal they are
+// L, on and above it scratch) and 1 / D on the diagonal.
+template <int kLd, int kP>
+__device__ __forceinline__ void FactorPanel(double* lt, double* dvec, double* col, int nb, int lane, bool& zero) {
+  constexpr int c = 16 * kP;
+  double r[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double v = lt[(c + j) * kLd + lane];
+    r[j] = (lane < nb && c + j < nb && c + j <= lane) ? v : (c + j == lane ? 1.0 : 0.0);
+  }
+  PanelPivots<c>(r, lane, col, dvec, zero);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) lt[(c + j) * kLd + lane] = r[j];
+  if (lane >= c && lane < c + 16) lt[lane * kLd + lane] = 1.0 / dvec[lane];
 }
 
-// M = L^-1 of the unit lower triangle of a factored diagonal block, one
-// wavefront, in LDS: `lt` holds L(j, t) at lt[t * kLd + j] (j > t) and
-// receives M(r, c) at lt[r * kLd + c] (r > c; M's diagonal is 1), the other
-// triangle of the same tile. Blocked in 16 x 16 blocks: the four diagonal
-// blocks are inverted by forward substitution, lane 16 p + c forming column
-// c of block p; then, by distance d = p - q from the diagonal, each block
-// M_pq = -M_pp (sum_{k = q}^{p - 1} L_pk M_kq) is two chains of FP64 MFMAs
-// (16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and
-// holds D[4 e + l / 16][l % 16] in element e, so the accumulator of the sum
-// is already the B operand of the product with M_pp).
+// Trailing update after panel kP: every lower 16x16 block (I, J),
+// kP < J <= I < np, loses L_I,kP D_kP L_J,kP^T, one FP64 MFMA chain of
+// four (16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and
+// holds D[4 e + l / 16][l % 16] in element e); the blocks go round-robin to
+// the wavefronts.
+template <int kLd, int kP>
+__device__ __forceinline__ void UpdatePanel(double* lt, const double* dvec, int np, int wave, int lane) {
+  constexpr int c = 16 * kP;
+  int b = 0;
+#pragma unroll
+  for (int I = kP + 1; I < 4; ++I) {
+#pragma unroll
+    for (int J = kP + 1; J <= I; ++J, ++b) {
+      if (I >= np || (b & (kWaves - 1)) != wave) continue;
+      v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kc = 0; kc < 16; kc += 4) {
+        const int t = c + kc + (lane >> 4);
+        const double a = lt[t * kLd + 16 * I + (lane & 15)];
+        const double bl = lt[t * kLd + 16 * J + (lane & 15)] * dvec[t];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bl, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        double* x = lt + (16 * J + (lane & 15)) * kLd + 16 * I + 4 * e + (lane >> 4);
+        *x = *x - acc[e];
+      }
+    }
+  }
+}
+
+// M_pp = L_pp^-1 of the 16x16 unit lower diagonal block at `base`, lanes
+// 0 .. 15 of a wavefront (lane c forms column c by forward substitution);
+// M(r, c) goes to lt[r * kLd + c] (r > c), the other triangle of the tile.
 template <int kLd>
-__device__ __forceinline__ void InvertUnitLowerWave(double* lt, int lane) {
-  {
-    const int base = 16 * (lane >> 4);
-    const int c = lane & 15;
-    double y[16];
+__device__ __forceinline__ void InvertDiag16(double* lt, int base, int c) {
+  double y[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) y[j] = j == c ? 1.0 : 0.0;
+  for (int j = 0; j < 16; ++j) y[j] = j == c ? 1.0 : 0.0;
 #pragma unroll
-    for (int t = 0; t < 15; ++t) {
+  for (int t = 0; t < 15; ++t) {
 #pragma unroll
-      for (int j = t + 1; j < 16; ++j) y[j] = fma(-lt[(base + t) * kLd + base + j], y[t], y[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      if (j > c) lt[(base + j) * kLd + base + c] = y[j];
-    }
+    for (int j = t + 1; j < 16; ++j) y[j] = fma(-lt[(base + t) * kLd + base + j], y[t], y[j]);
   }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (j > c) lt[(base + j) * kLd + base + c] = y[j];
+  }
+}
+
+// M(r, c) of a block on or below the diagonal of the inverse (unit
+// diagonal, zero above).
+template <int kLd>
+__device__ __forceinline__ double InvAt(const double* lt, int r, int c) {
+  const double v = lt[r * kLd + c];
+  return r > c ? v : (r == c ? 1.0 : 0.0);
+}
+
+// Off-diagonal block M_pq = -M_pp (sum_{k = q}^{p - 1} L_pk M_kq) of the
+// inverse, one wavefront, two FP64 MFMA chains: the accumulator of the sum
+// (lane l holds S[4 e + l / 16][l % 16] in element e) is already the B
+// operand of the product with M_pp. Needs M_pp and every M_kq, k < p.
+template <int kLd>
+__device__ __forceinline__ void InvertOffDiag(double* lt, int p, int q, int lane) {
   const int lr = lane & 15;  // A row / B column within a block
   const int lk = lane >> 4;  // k within a 4-wide chunk
-  // M(r, c) of a block on or below the diagonal (unit diagonal, zero above);
-  // the load is unconditional (every index is inside the tile).
-  auto m_at = [&](int r, int c) {
-    const double v = lt[r * kLd + c];
-    return r > c ? v : (r == c ? 1.0 : 0.0);
-  };
-  // Blocks at distance d are independent: their operands are loaded first,
-  // then their MFMA chains interleave.
+  v4d sum = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int k = q; k < p; ++k) {
+    double a[4], b[4];
 #pragma unroll
-  for (int d = 1; d < 4; ++d) {
-    double a[3][12], b[3][12], mp[3][4];
-#pragma unroll
-    for (int i = 0; i < 4 - d; ++i) {
-      const int p = d + i;
-      const int q = i;
-#pragma unroll
-      for (int k = q; k < p; ++k) {
-#pragma unroll
-        for (int kc = 0; kc < 16; kc += 4) {
-          const int x = 4 * (k - q) + kc / 4;
-          a[i][x] = lt[(16 * k + kc + lk) * kLd + 16 * p + lr];  // L_pk
-          b[i][x] = m_at(16 * k + kc + lk, 16 * q + lr);       // M_kq
-        }
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) mp[i][e] = m_at(16 * p + lr, 16 * p + 4 * e + lk);  // M_pp
-    }
-    v4d sum[3], prod[3];
-#pragma unroll
-    for (int i = 0; i < 4 - d; ++i) {
-      sum[i] = v4d{0.0, 0.0, 0.0, 0.0};
-      prod[i] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int x = 0; x < 4; ++x) {
+      a[x] = lt[(16 * k + 4 * x + lk) * kLd + 16 * p + lr];  // L_pk
+      b[x] = InvAt<kLd>(lt, 16 * k + 4 * x + lk, 16 * q + lr);  // M_kq
     }
 #pragma unroll
-    for (int x = 0; x < 4 * d; ++x) {
-#pragma unroll
-      for (int i = 0; i < 4 - d; ++i) sum[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][x], b[i][x], sum[i], 0, 0, 0);
-    }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-#pragma unroll
-      for (int i = 0; i < 4 - d; ++i) {
-        prod[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(mp[i][e], sum[i][e], prod[i], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 4 - d; ++i) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) lt[(16 * (d + i) + 4 * e + lk) * kLd + 16 * i + lr] = -prod[i][e];
-    }
+    for (int x = 0; x < 4; ++x) sum = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[x], sum, 0, 0, 0);
   }
+  double mp[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) mp[e] = InvAt<kLd>(lt, 16 * p + lr, 16 * p + 4 * e + lk);  // M_pp
+  v4d prod = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) prod = __builtin_amdgcn_mfma_f64_16x16x4f64(mp[e], sum[e], prod, 0, 0, 0);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) lt[(16 * p + 4 * e + lk) * kLd + 16 * q + lr] = -prod[e];
 }
 
-// Factors the diagonal block held in `row` (one wavefront, as
-// FactorDiagWave) and inverts its unit lower triangle in the LDS tile `lt`
-// (kNB rows of kLd doubles): afterwards lt[j * kLd + j] = 1 / D(j),
-// lt[j * kLd + i] = L(i, j) for i > j and lt[i * kLd + j] = M(i, j) for
-// i > j, M = L^-1 (so W = L^-T D^-1 has W(i, j) = M(j, i) / D(j) above the
-// diagonal and 1 / D(j) on it); dvec[j] = D(j). Returns true when a pivot is
-// exactly zero.
+// LDL^T of a 64x64 block and M = L^-1 of its unit lower triangle by a
+// whole workgroup (kThreads threads; contains barriers) in the LDS tile lt,
+// column-major kLd apart: on entry T(i, j) at lt[j * kLd + i] for
+// j <= i < nb (everything else is ignored; rows and columns from nb on are
+// an identity pad). Blocked by 16 columns: wave 0 runs a panel's pivots
+// (PanelPivots), then every wavefront takes lower 16x16 blocks of the
+// trailing update on FP64 MFMA. The inverse's diagonal block of a panel is
+// formed by wave 1 while wave 0 factors the next panel, and a block row's
+// off-diagonal blocks by wave 2 one panel later; after the last panel its
+// diagonal block (wave 0) and the row before (wave 1), then its own row
+// (one block per wavefront). Panels past nb are written as identity.
+// On return lt[j * kLd + j] = 1 / D(j), lt[j * kLd + i] = L(i, j) and
+// lt[i * kLd + j] = M(i, j) for i > j (so W = L^-T D^-1 has W(i, j) =
+// M(j, i) / D(j) above the diagonal and 1 / D(j) on it), dvec[j] = D(j).
+// `col` is kNB doubles of scratch. Returns, in wave 0, true when a pivot
+// is exactly zero. Ends with a barrier.
 template <int kLd>
-__device__ __forceinline__ bool FactorInvertWave(double (&row)[kNB], int lane, int nb, double* lt,
-                                                 double* dvec) {
-  const bool zero = FactorDiagWave(row, lane, nb, lt);
-  // LDS operations of one wavefront complete in order: D is read and the
-  // factorization's broadcasts are done before these writes land. Entries
-  // on and above the diagonal receive scratch here and 1 / D and M below.
-  const double dl = lt[kNB + lane];
-  dvec[lane] = dl;
-#pragma unroll
-  for (int t = 0; t < kNB; ++t) lt[t * kLd + lane] = row[t];
-  lt[lane * kLd + lane] = 1.0 / dl;
-  InvertUnitLowerWave<kLd>(lt, lane);
+__device__ __forceinline__ bool FactorInvertBlock(double* lt, double* dvec, double* col, int nb) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int np = (nb + 15) >> 4;  // panels with a column inside the block
+  bool zero = false;
+  for (int idx = tid; idx < kNB * kNB; idx += kThreads) {
+    const int i = idx & (kNB - 1);
+    const int j = idx / kNB;
+    if (j >= 16 * np) lt[j * kLd + i] = i == j ? 1.0 : 0.0;
+  }
+  if (tid >= 16 * np && tid < kNB) dvec[tid] = 1.0;
+  if (wave == 0) FactorPanel<kLd, 0>(lt, dvec, col, nb, lane, zero);
+  __syncthreads();
+  if (np > 1) {
+    UpdatePanel<kLd, 0>(lt, dvec, np, wave, lane);
+    __syncthreads();
+    if (wave == 0) {
+      FactorPanel<kLd, 1>(lt, dvec, col, nb, lane, zero);
+    } else if (wave == 1 && lane < 16) {
+      InvertDiag16<kLd>(lt, 0, lane);
+    }
+    __syncthreads();
+  }
+  if (np > 2) {
+    UpdatePanel<kLd, 1>(lt, dvec, np, wave, lane);
+    __syncthreads();
+    if (wave == 0) {
+      FactorPanel<kLd, 2>(lt, dvec, col, nb, lane, zero);
+    } else if (wave == 1 && lane < 16) {
+      InvertDiag16<kLd>(lt, 16, lane);
+    }
+    __syncthreads();
+  }
+  if (np > 3) {
+    UpdatePanel<kLd, 2>(lt, dvec, np, wave, lane);
+    __syncthreads();
+    if (wave == 0) {
+      FactorPanel<kLd, 3>(lt, dvec, col, nb, lane, zero);
+    } else if (wave == 1 && lane < 16) {
+      InvertDiag16<kLd>(lt, 32, lane);
+    } else if (wave == 2) {
+      InvertOffDiag<kLd>(lt, 1, 0, lane);
+    }
+    __syncthreads();
+  }
+  const int last = np - 1;
+  if (wave == 0 && lane < 16) {
+    InvertDiag16<kLd>(lt, 16 * last, lane);
+  } else if (wave == 1 && last >= 2) {
+    for (int q = 0; q < last - 1; ++q) InvertOffDiag<kLd>(lt, last - 1, q, lane);
+  }
+  __syncthreads();
+  if (wave < last) InvertOffDiag<kLd>(lt, last, wave, lane);
+  __syncthreads();
   return zero;
 }
 
-// Stores a block left by FactorInvertWave, kN threads cooperating (thread
+// Stores a block left by FactorInvertBlock, kN threads cooperating (thread
 // tid takes entries tid, tid + kN, ... in column-major order): the staging
 // slot (L below the diagonal, D on it, W above; identity padding beyond nb
 // in both) and, with `inv`, M itself (unit diagonal, zeros above).
@@ -1115,26 +1164,34 @@ __device__ __forceinline__ void StoreFactored(const double* lt, const double* dv
   }
 }
 
-// Row `lane` of the kb-th diagonal block of front s (lower triangle and
-// diagonal; rows and columns beyond nb are an identity pad). Every load is
-// issued (clamped into the block) and the pad selected afterwards, so the 64
-// loads are in flight together.
-__device__ __forceinline__ void LoadDiagRow(const double* F, int f, int c0, int nb, int lane,
-                                            double (&row)[kNB]) {
-  const int ti = min(lane, nb - 1);
+// The kb-th diagonal block of front s into the LDS tile lt (column-major,
+// kLd apart; entries past nb hold copies of the block's last row or
+// column, which FactorInvertBlock ignores), the whole workgroup; every load
+// is issued (clamped into the block), so they are in flight together.
+template <int kLd>
+__device__ __forceinline__ void LoadDiagBlock(const double* F, int f, int c0, int nb, double* lt) {
+  constexpr int kPer = kNB * kNB / kThreads;
+  double v[kPer];
 #pragma unroll
-  for (int j = 0; j < kNB; ++j) {
-    const double v = F[static_cast<int64_t>(c0 + min(j, nb - 1)) * f + c0 + ti];
-    row[j] = (lane < nb && j < nb && j <= lane) ? v : (j == lane ? 1.0 : 0.0);
+  for (int e = 0; e < kPer; ++e) {
+    const int idx = threadIdx.x + kThreads * e;
+    const int i = min(idx & (kNB - 1), nb - 1);
+    const int j = min(idx / kNB, nb - 1);
+    v[e] = F[static_cast<int64_t>(c0 + j) * f + c0 + i];
+  }
+#pragma unroll
+  for (int e = 0; e < kPer; ++e) {
+    const int idx = threadIdx.x + kThreads * e;
+    lt[(idx / kNB) * kLd + (idx & (kNB - 1))] = v[e];
   }
 }
 
-// The kb-th diagonal block of every front at the level, one wavefront per
-// front: factored and inverted in registers and stored in the front's
-// staging slot (with the zero-pivot flag), for KPanelStep's staged mode.
+// The kb-th diagonal block of every front at the level, one workgroup per
+// front: factored and inverted in the LDS (FactorInvertBlock) and stored in
+// the front's staging slot (with the zero-pivot flag), for KPanelStep's
+// staged mode.
 template <class Slots, class Lists>
-__global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
-                                                 Lists plan, int kb) {
+__global__ void __launch_bounds__(kThreads) KDiagBlock(Slots slots, Lists plan, int kb) {
   BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int s = PlanItem(plan, blockIdx.y, blockIdx.x);
@@ -1146,11 +1203,13 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
   const int f = A.sn_f[s];
   const double* F = W.panel + A.panel_off[s];
   const int tid = threadIdx.x;
-  __shared__ double lt[kNB * (kNB + 1) + kNB];  // the tile, then D
-  double row[kNB];
-  LoadDiagRow(F, f, c0, nb, tid, row);
-  const bool zero = FactorInvertWave<kNB + 1>(row, tid, nb, lt, lt + kNB * (kNB + 1));
-  StoreFactored<kNB + 1, kNB>(lt, lt + kNB * (kNB + 1), StageSlot(W, s, kb), DiagInv(A, W, s, kb), tid);
+  constexpr int kLd = kNB + 1;
+  __shared__ double lt[kNB * kLd + 2 * kNB];  // the tile, D, scratch
+  double* dvec = lt + kNB * kLd;
+  LoadDiagBlock<kLd>(F, f, c0, nb, lt);
+  __syncthreads();
+  const bool zero = FactorInvertBlock<kLd>(lt, dvec, dvec + kNB, nb);
+  StoreFactored<kLd, kThreads>(lt, dvec, StageSlot(W, s, kb), DiagInv(A, W, s, kb), tid);
   if (tid == 0 && zero) {
     W.state->not_spd = 1;
     W.state->halt = 1;
@@ -1162,9 +1221,9 @@ __global__ void __launch_bounds__(kNB) KDiagBlock(Slots slots,
 // trailing update, KGemmAhead): the panel rows below the diagonal block, 64
 // per workgroup, become L21 = A21 W with W = L_kk^-T D_k^-1, a
 // 64 x 64 x 64 FP64 MFMA product. W is read from the staging slot
-// (KDiagBlock filled it) or, with kFused, formed by wave 0 of every
-// workgroup, which factors and inverts the block itself; workgroup 0 then
-// stores the staging slot. The trailing-update kernel moves the factored
+// (KDiagBlock filled it) or, with kFused, formed by every workgroup,
+// which factors and inverts the block itself; workgroup 0 then stores the
+// staging slot. The trailing-update kernel moves the factored
 // block into the panel, so no workgroup of this launch reads a partly
 // written block.
 template <bool kFused, class Slots, class Lists>
@@ -1187,6 +1246,7 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
   __shared__ double wt[kNB][kLdT];  // wt[k][j] = W(k, j)
   __shared__ double at[kNB][kLdT];  // at[k][r] = F(r0 + r, c0 + k)
   __shared__ double dsh[kNB];  // D, with kFused
+  __shared__ double colscr[kFused ? kNB : 1];  // the factorization's scratch
   __shared__ int zero_pivot;
   const int tid = threadIdx.x;
   // A21: rows r0 .. r0 + 63 of the block column, zero beyond the front and
@@ -1234,14 +1294,11 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
       wt[i][j] = i < j ? v[e] : (i == j ? rd : 0.0);
     }
   } else {
-    if (tid < kNB) {
-      // at is the factorization's scratch until the rows are loaded.
-      double row[kNB];
-      LoadDiagRow(F, f, c0, nb, tid, row);
-      const bool zero = FactorInvertWave<kLdT>(row, tid, nb, &at[0][0], dsh);
-      if (tid == 0) zero_pivot = zero ? 1 : 0;
-    }
+    // at is the factorization's tile until the rows are loaded.
+    LoadDiagBlock<kLdT>(F, f, c0, nb, &at[0][0]);
     __syncthreads();
+    const bool zero = FactorInvertBlock<kLdT>(&at[0][0], dsh, colscr, nb);
+    if (tid == 0) zero_pivot = zero ? 1 : 0;
     // W from the tile: M(j, i) above the diagonal, 1 / D(j) on it.
 #pragma unroll 4
     for (int e = 0; e < kNB * kNB / kThreads; ++e) {
@@ -1529,16 +1586,17 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
 }
 
 // LDS of the fused step: the K chunks of two tiles, then the diagonal tile
-// (column-major, kNB + 1 apart), D, and the TRSM rows.
+// (column-major, kNB + 1 apart), D, the factorization's scratch and the
+// TRSM rows.
 constexpr int kAheadLd = kNB + 1;
-constexpr int kAheadLds = kNB * kAheadLd + kNB + kNB * kLdT;
+constexpr int kAheadLds = kNB * kAheadLd + 2 * kNB + kNB * kLdT;
 static_assert(kAheadLds >= 2 * 3 * kTk * (kTile + 2), "fused step staging");
 
 // Trailing update of step kb fused with step kb + 1's panel step. Tile
 // (ti, tj), tj <= ti, of front s. The tiles of column 0 are the next block
 // column. Each of their workgroups also updates the next diagonal block
-// (tile (0, 0)), factors and inverts it in wave 0 -- every copy computes
-// the same values from the same inputs, bit for bit -- and multiplies its
+// (tile (0, 0)), factors and inverts it (FactorInvertBlock) -- every copy
+// computes the same values from the same inputs, bit for bit -- and multiplies its
 // own updated rows by W = L^-T D^-1 (in tile (0, 0) the rows below the
 // block's nb1 columns), so step kb + 1 has no launch of its own. Nothing
 // here writes the diagonal tile into the panel (the copies read its values
@@ -1583,7 +1641,8 @@ __device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, i
   const int wc = 32 * (wave >> 1);
   double* lt = lds;                         // the diagonal tile, then its factor
   double* dvec = lds + kNB * kAheadLd;      // D
-  auto at = reinterpret_cast<double (*)[kLdT]>(lds + kNB * kAheadLd + kNB);  // at[k][r]: TRSM rows
+  double* colscr = dvec + kNB;              // the factorization's scratch
+  auto at = reinterpret_cast<double (*)[kLdT]>(colscr + kNB);  // at[k][r]: TRSM rows
   // The updated tile (0, 0) goes to lt (rows below nb1: to at, in tile
   // (0, 0)'s own workgroup), tile (ti, 0) to at; everything else in at is
   // zero (rows past M, columns past nb1).
@@ -1633,20 +1692,11 @@ __device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, i
     put(acc[1], R0, false, true);
   }
   __syncthreads();
-  if (tid < kNB) {
-    // Row tid of the block (lower triangle; identity pad beyond nb1).
-    double row[kNB];
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-      row[j] = (tid < nb1 && j < nb1 && j <= tid) ? lt[j * kAheadLd + tid] : (j == tid ? 1.0 : 0.0);
-    }
-    const bool zero = FactorInvertWave<kAheadLd>(row, tid, nb1, lt, dvec);
-    if (ti == 0 && tid == 0 && zero) {
-      W.state->not_spd = 1;
-      W.state->halt = 1;
-    }
+  const bool zero = FactorInvertBlock<kAheadLd>(lt, dvec, colscr, nb1);
+  if (ti == 0 && tid == 0 && zero) {
+    W.state->not_spd = 1;
+    W.state->halt = 1;
   }
-  __syncthreads();
   if (ti == 0) StoreFactored<kAheadLd, kThreads>(lt, dvec, StageSlot(W, s, kb + 1), DiagInv(A, W, s, kb + 1), tid);
   // L21 = A21 W, W(k, j) = M(j, k) / D(j) above the diagonal and 1 / D(j)
   // on it (lt holds M(j, k) at j * kAheadLd + k and 1 / D(j) at
@@ -1705,8 +1755,7 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmLower(Slots slots, Lists pla
 }
 
 // Trailing update of step kb with step kb + 1's diagonal block and panel
-// rows (same grid; the diagonal factorization keeps a row of 64 doubles and
-// a straight-line schedule in registers).
+// rows (same grid).
 template <class Slots, class Lists>
 __global__ void __launch_bounds__(kThreads, 1) KGemmAhead(Slots slots, Lists plan, int kb, int fronts) {
   if (blockIdx.y > blockIdx.x) return;
@@ -2395,7 +2444,7 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
           const bool timed = step_events != nullptr && step_events->used < step_events->capacity;
           if (timed) (void)hipEventRecord(step_events->events[2 * step_events->used], s);
           if (staged) {
-            KDiagBlock<<<dim3(count, k), kNB, 0, s>>>(slots, plan, kb);
+            KDiagBlock<<<dim3(count, k), kThreads, 0, s>>>(slots, plan, kb);
             KPanelStep<false><<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb);
           } else {
             KPanelStep<true><<<dim3(row_blocks, count, k), kThreads, 0, s>>>(slots, plan, kb);
