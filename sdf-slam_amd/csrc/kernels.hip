@@ -1396,41 +1396,43 @@ __device__ __forceinline__ void TileProducts(const double* F, int f, int kbase, 
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb) acc[t][cb][rb] = v4d{0.0, 0.0, 0.0, 0.0};
   // Staging of one K chunk: each thread loads 4 entries of every operand
-  // (rows rr = idx & 63 of k row kk = idx >> 6); B is scaled by D_k. Every
-  // load is issued at an address clamped into the operands and the padding
-  // selected afterwards: a conditional load would become a branch that
-  // waits for its data before the next load is issued.
-  double ra[kTiles][4], rb[4];
+  // (rows rr = idx & 63 of k row kk = idx >> 6) and D_k. Every load is
+  // issued at an address clamped into the operands, and the padding is
+  // selected (and B scaled by D_k) only when the chunk is stored to the LDS,
+  // through Opaque: a load feeding a select directly would become a branch
+  // that waits for its data before the next load is issued.
+  double ra[kTiles][4], rb[4], rd[4];
   auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int idx = tid + kThreads * e;
       const int rr = idx & 63;
-      const int kk = idx >> 6;
-      const bool kin = k0 + kk < K;
-      const int kq = min(k0 + kk, K - 1);
+      const int kq = min(k0 + (idx >> 6), K - 1);
       const int64_t colbase = static_cast<int64_t>(kbase + kq) * f;
-      const double d = dk[kq * dstride];
+      rd[e] = dk[kq * dstride];
 #pragma unroll
-      for (int t = 0; t < kTiles; ++t) {
-        const double a = F[colbase + rbase + min(r0[t] + rr, M - 1)];
-        ra[t][e] = (kin && r0[t] + rr < M) ? a : 0.0;
-      }
-      const double b = F[colbase + rbase + min(cb0 + rr, N - 1)];
-      rb[e] = (kin && cb0 + rr < N) ? b * d : 0.0;
+      for (int t = 0; t < kTiles; ++t) ra[t][e] = F[colbase + rbase + min(r0[t] + rr, M - 1)];
+      rb[e] = F[colbase + rbase + min(cb0 + rr, N - 1)];
     }
   };
-  auto store_chunk = [&](int buf) {
+  auto store_chunk = [&](int buf, int k0) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int idx = tid + kThreads * e;
+      const int rr = idx & 63;
+      const bool kin = k0 + (idx >> 6) < K;
 #pragma unroll
-      for (int t = 0; t < kTiles; ++t) As[buf][t][idx >> 6][idx & 63] = ra[t][e];
-      Bs[buf][idx >> 6][idx & 63] = rb[e];
+      for (int t = 0; t < kTiles; ++t) {
+        const double a = Opaque(ra[t][e]);
+        As[buf][t][idx >> 6][rr] = (kin && r0[t] + rr < M) ? a : 0.0;
+      }
+      const double b = Opaque(rb[e]);
+      const double d = Opaque(rd[e]);
+      Bs[buf][idx >> 6][rr] = (kin && cb0 + rr < N) ? b * d : 0.0;
     }
   };
   load_chunk(0);
-  store_chunk(0);
+  store_chunk(0, 0);
   __syncthreads();
   int buf = 0;
   for (int k0 = 0; k0 < K; k0 += kTk) {
@@ -1456,7 +1458,7 @@ __device__ __forceinline__ void TileProducts(const double* F, int f, int kbase, 
           }
     }
     if (more) {
-      store_chunk(buf ^ 1);
+      store_chunk(buf ^ 1, k0 + kTk);
       __syncthreads();
       buf ^= 1;
     }
