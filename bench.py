@@ -183,7 +183,8 @@ def served_leg(S, duration, plain_duration, workers):
     increment on arrival, three solver threads batch what is prepared) against
     the plain path (one increment at a time, preparation and solve in turn)
     and one session alone; then the served form at 30 Hz per session for the
-    throughput it sustains."""
+    throughput it sustains, and the 10 Hz load (and one session alone) on
+    sessions advanced to frame 150 first (steady state)."""
     full = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=duration, workers=workers)
     plain = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=plain_duration, workers=1,
                                max_batch=1, pipelines=1, max_in_flight=1)
@@ -192,6 +193,12 @@ def served_leg(S, duration, plain_duration, workers):
     # Saturating load: 32 sessions at 30 Hz each (960 frames/s offered), the
     # increments/s the served form sustains.
     sat = S.served_benchmark(sessions=32, rate_hz=30.0, duration_s=duration, workers=workers)
+    # Steady state: the same load on sessions already holding 150 frames
+    # (each increment rebuilds a problem over every frame so far).
+    steady = S.served_benchmark(sessions=32, rate_hz=10.0, duration_s=duration, workers=workers,
+                                start_frame=150)
+    steady_alone = S.served_benchmark(sessions=1, rate_hz=10.0, duration_s=duration, workers=1,
+                                      max_batch=1, pipelines=1, max_in_flight=1, start_frame=150)
     r = lambda d: {k: (round(v, 3) if isinstance(v, float) else v) for k, v in d.items()}
     return {"sessions": 32, "rate_hz_per_session": 10.0, "duration_s": duration, "workers": workers,
             "increments_per_s": full["increments_per_second"],
@@ -201,7 +208,12 @@ def served_leg(S, duration, plain_duration, workers):
             "saturated_increments_per_s": sat["increments_per_second"],
             "saturated_vs_plain": sat["increments_per_second"] / plain["increments_per_second"],
             "served": r(full), "plain_one_at_a_time": r(plain), "single_session": r(alone),
-            "saturated_30hz": r(sat)}
+            "saturated_30hz": r(sat),
+            "steady_state_from_frame_150": {
+                "increments_per_s": steady["increments_per_second"],
+                "latency_p50_ms": steady["latency_p50_ms"], "latency_p99_ms": steady["latency_p99_ms"],
+                "p99_vs_single_session": steady["latency_p99_ms"] / steady_alone["latency_p99_ms"],
+                "served": r(steady), "single_session": r(steady_alone)}}
 
 
 def batched_leg(S):

@@ -259,6 +259,10 @@ typedef struct sdfslam_served_options {
   uint64_t seed0;
   int32_t max_in_flight;  /* as in sdfslam_server_options */
   int32_t device_normals; /* as in sdfslam_server_options */
+  /* Frames 1 .. start_frame - 1 of every session are solved before the
+   * clock starts (steady state: the timed frames extend sessions that hold
+   * start_frame poses); 1 or less: timed from the first increment. */
+  int32_t start_frame;
 } sdfslam_served_options;
 typedef struct sdfslam_served_result {
   int64_t increments;

@@ -1,4 +1,5 @@
 // C API of the host library (include/sdfslam_host.h).
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <fstream>
@@ -743,6 +744,7 @@ int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam
     b.seed0 = options->seed0;
     b.max_in_flight = options->max_in_flight;
     b.device_normals = options->device_normals != 0;
+    b.start_frame = std::max(options->start_frame, 1);
     const ServedResult r = RunServedBenchmark(b);
     *out = {r.increments,       r.lm_iterations,    r.wall_seconds,     r.increments_per_second,
             r.latency_p50_ms,   r.latency_p99_ms,   r.latency_mean_ms,  r.build_mean_ms,

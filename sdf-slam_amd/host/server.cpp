@@ -331,6 +331,15 @@ void Server::Finish(Prepared& item, Clock::time_point done, int64_t device_bytes
   AdmitWaiting();
 }
 
+void Server::SetMaxInFlight(int max_in_flight) {
+  if (max_in_flight < 0) throw std::invalid_argument("max_in_flight must be >= 0");
+  {
+    std::lock_guard<std::mutex> lock(mu_);
+    options_.max_in_flight = max_in_flight;
+  }
+  work_cv_.notify_all();
+}
+
 ServerStats Server::Stats() {
   std::lock_guard<std::mutex> lock(stats_mu_);
   return stats_;
@@ -413,8 +422,8 @@ void Server::SolveLoop() {
 }
 
 ServedResult RunServedBenchmark(const ServedBenchmark& o) {
-  if (o.sessions < 1 || o.rate_hz <= 0.0 || o.duration_s <= 0.0) {
-    throw std::invalid_argument("served benchmark needs sessions, a rate and a duration");
+  if (o.sessions < 1 || o.rate_hz <= 0.0 || o.duration_s <= 0.0 || o.start_frame < 1) {
+    throw std::invalid_argument("served benchmark needs sessions, a rate, a duration and start_frame >= 1");
   }
   std::vector<Workload> wls(static_cast<size_t>(o.sessions));
   {
@@ -441,7 +450,7 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
     Rng rng(o.seed0 * 7919ULL + static_cast<uint64_t>(i));
     double t = 0.0;
     const auto frames = static_cast<int>(wls[static_cast<size_t>(i)].scans.size());
-    for (int f = 1; f < frames; ++f) {
+    for (int f = o.start_frame; f < frames; ++f) {
       t += -std::log(1.0 - rng.Uniform()) / o.rate_hz;
       if (t >= o.duration_s) break;
       arrivals.push_back({t, i, f});
@@ -461,6 +470,24 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
     SolverOptions step = wl.solver;
     step.max_iterations = wl.iterations_per_increment;
     server.OpenSession(wl.map, wl.problem, step, wl.dataset_poses[0], wl.scans[0]);
+  }
+  ServerStats before;
+  if (o.start_frame > 1) {
+    // Sessions advanced to start_frame before the clock starts.
+    server.SetMaxInFlight(0);
+    std::vector<std::pair<int, int64_t>> warm;
+    const auto now = Clock::now();
+    for (int f = 1; f < o.start_frame; ++f) {
+      for (int i = 0; i < o.sessions; ++i) {
+        const Workload& wl = wls[static_cast<size_t>(i)];
+        if (f >= static_cast<int>(wl.scans.size())) continue;
+        warm.emplace_back(i, server.Submit(i, wl.scans[static_cast<size_t>(f)],
+                                           wl.odometry[static_cast<size_t>(f) - 1], now));
+      }
+    }
+    for (const auto& [sid, ticket] : warm) (void)server.Wait(sid, ticket);
+    server.SetMaxInFlight(o.max_in_flight);
+    before = server.Stats();
   }
   std::vector<std::pair<int, int64_t>> tickets;
   tickets.reserve(arrivals.size());
@@ -508,7 +535,16 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   r.build_mean_ms = r.increments > 0 ? 1e3 * build / static_cast<double>(r.increments) : 0.0;
   r.solve_mean_ms = r.increments > 0 ? 1e3 * solve / static_cast<double>(r.increments) : 0.0;
   r.device_bytes = server.DeviceBytes();
-  const ServerStats st = server.Stats();
+  ServerStats st = server.Stats();
+  st.batches -= before.batches;
+  st.increments -= before.increments;
+  st.graph_captures -= before.graph_captures;
+  st.epochs_rebuilt -= before.epochs_rebuilt;
+  st.prepare_seconds -= before.prepare_seconds;
+  st.solve_seconds -= before.solve_seconds;
+  st.collect_seconds -= before.collect_seconds;
+  st.capture_seconds -= before.capture_seconds;
+  st.analysis_seconds -= before.analysis_seconds;
   const double nb = static_cast<double>(std::max<int64_t>(st.batches, 1));
   r.batches = st.batches;
   r.mean_batch = static_cast<double>(st.increments) / nb;

@@ -103,6 +103,9 @@ class Server {
   void State(int session, std::vector<Pose2>& poses, GridMap& map);
   /// Device memory held by all sessions.
   int64_t DeviceBytes();
+  /// Changes the limit on increments prepared or solving at once (as
+  /// ServerOptions::max_in_flight) for the increments picked from now on.
+  void SetMaxInFlight(int max_in_flight);
   /// Totals over the batches run so far.
   ServerStats Stats();
 
@@ -179,7 +182,9 @@ class Server {
 
 /// Served benchmark: `sessions` incremental sessions (synthetic floorplans,
 /// seeds seed0 + i), each receiving frames as a Poisson process of
-/// `rate_hz` for `duration_s` seconds of wall time, served by `workers`.
+/// `rate_hz` for `duration_s` seconds of wall time (from frame start_frame
+/// on, the earlier frames solved before the clock starts), served by
+/// `workers`.
 struct ServedBenchmark {
   int sessions{32};
   double rate_hz{10.0};
@@ -190,6 +195,11 @@ struct ServedBenchmark {
   uint64_t seed0{2000};
   int max_in_flight{0};
   bool device_normals{false};
+  // Frames 1 .. start_frame - 1 of every session are solved before the
+  // clock starts (all submitted at once, no limit on increments in flight),
+  // so the timed frames measure sessions that already hold start_frame
+  // poses; arrivals then continue from frame start_frame.
+  int start_frame{1};
 };
 struct ServedResult {
   int64_t increments{0};

@@ -120,7 +120,7 @@ class ServedOptions(C.Structure):
     _fields_ = [("sessions", C.c_int32), ("rate_hz", C.c_double), ("duration_s", C.c_double),
                 ("workers", C.c_int32), ("max_batch", C.c_int32), ("pipelines", C.c_int32),
                 ("seed0", C.c_uint64), ("max_in_flight", C.c_int32),
-                ("device_normals", C.c_int32)]
+                ("device_normals", C.c_int32), ("start_frame", C.c_int32)]
 
 
 class ServedResult(C.Structure):
@@ -822,9 +822,9 @@ class Server:
 
 
 def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=12, max_batch=64,
-                     pipelines=3, seed0=2000, max_in_flight=0, device_normals=False):
+                     pipelines=3, seed0=2000, max_in_flight=0, device_normals=False, start_frame=1):
     o = ServedOptions(sessions, rate_hz, duration_s, workers, max_batch, pipelines, seed0, max_in_flight,
-                      int(device_normals))
+                      int(device_normals), start_frame)
     r = ServedResult()
     _check_host(host.sdfslam_host_served_benchmark(C.byref(o), C.byref(r)))
     return {k: getattr(r, k) for k, _ in ServedResult._fields_}
