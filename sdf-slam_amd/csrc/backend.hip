@@ -334,6 +334,19 @@ class ThreadStreams {
   std::vector<std::pair<int, hipStream_t>> streams_;
 };
 thread_local ThreadStreams t_streams;
+// A second stream per thread and device: the forward solves of the lower
+// levels run on it beside the factorization of the upper ones.
+thread_local ThreadStreams t_aux_streams;
+
+/// `n` events without timing in `ev` (created on first use, kept).
+const hipEvent_t* OverlapEvents(std::vector<hipEvent_t>& ev, size_t n) {
+  while (ev.size() < n) {
+    hipEvent_t e = nullptr;
+    HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ev.push_back(e);
+  }
+  return ev.data();
+}
 
 using SteadyTime = std::chrono::steady_clock::time_point;
 static SteadyTime Now() { return std::chrono::steady_clock::now(); }
@@ -427,6 +440,8 @@ struct BatchSpace {
 struct sdfslam_hip_context {
   int device{0};
   hipStream_t stream{nullptr};  // the calling thread's stream, set at every call
+  hipStream_t aux_stream{nullptr};  // the calling thread's second stream
+  std::vector<hipEvent_t> overlap_events;  // level marks of the overlapped solves
   // Device work of a context update, deferred to the context's next call
   // (which runs it first, on its own stream): an update does host work only.
   std::vector<std::function<void(hipStream_t)>> pending;
@@ -500,6 +515,7 @@ struct sdfslam_hip_context {
     }
     for (hipEvent_t e : step_events) (void)hipEventDestroy(e);
     for (hipEvent_t e : gemm_events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : overlap_events) (void)hipEventDestroy(e);
     epoch.Release();
     base.Release();
     normals.Release();
@@ -536,6 +552,7 @@ void Enqueue(sdfslam_hip_context& c, std::function<void(hipStream_t)> work) {
 void Bind(sdfslam_hip_context& c) {
   HIPC(hipSetDevice(c.device));
   c.stream = t_streams.Get(c.device);
+  c.aux_stream = t_aux_streams.Get(c.device);
   std::vector<std::function<void(hipStream_t)>> work;
   work.swap(c.pending);
   for (auto& w : work) w(c.stream);
@@ -1003,7 +1020,14 @@ void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
     step_ev = {c.step_events.data(), static_cast<int>(c.step_events.size() / 2), 0,
                c.gemm_events.data(), static_cast<int>(c.gemm_events.size() / 2), 0};
   }
-  LaunchFactor(SlotArgs{c.slot_dev, &c.slot}, c.shape, s, timing ? &step_ev : nullptr);
+  if (!timing) {
+    // The lower levels' forward solves beside the upper levels' factorization.
+    HIPC(LaunchFactorSolveOverlapped(SlotArgs{c.slot_dev, &c.slot}, c.shape, s, c.aux_stream,
+                                     OverlapEvents(c.overlap_events, c.shape.levels.size() + 1)));
+    return;
+  }
+  // Timed phases: the factorization and the solves one after the other.
+  LaunchFactor(SlotArgs{c.slot_dev, &c.slot}, c.shape, s, &step_ev);
   c.step_events_used = step_ev.used;
   c.gemm_events_used = step_ev.gemm_used;
   Mark(c, timing, 3);
@@ -1359,8 +1383,8 @@ class BatchRun {
     HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
     LaunchAssemble(args, shape_, true, stream_);
     LaunchNormalSlots(args, shape_, o, stream_);
-    LaunchFactor(args, shape_, stream_);
-    LaunchSolve(args, shape_, stream_);
+    HIPC(LaunchFactorSolveOverlapped(args, shape_, stream_, ctxs_[0]->aux_stream,
+                                     OverlapEvents(ctxs_[0]->overlap_events, shape_.levels.size() + 1)));
     LaunchStepAndControl(args, shape_, o, stream_);
     HIPC(hipStreamEndCapture(stream_, &g));
     const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);

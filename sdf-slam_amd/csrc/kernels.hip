@@ -818,19 +818,23 @@ __global__ void KOdomPairs(Slots slots) {
 // stored for a < mc; a per-element loop would wait on two loads per element.
 __device__ __forceinline__ void ScatterAddColumn(double* dst, int64_t col, int off, const int32_t* rel,
                                                  const double* src, int b, int lane, int mc) {
-  for (int g = b; g < mc; g += 4 * 64) {
-    int r[4];
-    double u[4], d[4];
+  // Up to kPer entries per lane per round (1,024 rows of the column): the
+  // map and source loads of a round are all issued (clamped) before the
+  // destination loads, so a round costs two memory latencies.
+  constexpr int kPer = 4;
+  for (int g = b; g < mc; g += kPer * 64) {
+    int r[kPer];
+    double u[kPer], d[kPer];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kPer; ++q) {
       const int a = min(g + lane + 64 * q, mc - 1);
       r[q] = rel[a];
       u[q] = src[a];
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) d[q] = dst[col + r[q] - off];
+    for (int q = 0; q < kPer; ++q) d[q] = dst[col + r[q] - off];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kPer; ++q) {
       if (g + lane + 64 * q < mc) dst[col + r[q] - off] = d[q] + u[q];
     }
   }
@@ -2413,8 +2417,12 @@ void NormalSlotsImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& 
 // launch of their own instead of once per workgroup.
 constexpr int kStagedMinBlocks = 256;
 
-template <bool kOne>
-void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEvents* step_events) {
+// `panels_done(L)` runs once level L's fronts have their final panels and
+// kept inverses (before their update matrices), where a forward solve of the
+// level may start.
+template <bool kOne, class PanelsDone>
+void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEvents* step_events,
+                PanelsDone&& panels_done) {
   const auto slots = SlotsOf<kOne>(a);
   const unsigned k = static_cast<unsigned>(sh.slots);
   for (int L = 0; L < static_cast<int>(sh.levels.size()); ++L) {
@@ -2472,6 +2480,7 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
         }
         if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used++ + 1], s);
       }
+      panels_done(L);
       if (lv.tiles > 0) {
         const bool gt = step_events != nullptr && step_events->gemm_used < step_events->gemm_capacity;
         if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used], s);
@@ -2493,23 +2502,27 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
 // into launches of their own that spread them over many workgroups.
 constexpr int kSplitSolveFronts = 16;
 
+// Forward solve of level L's fronts (after their children's).
 template <bool kOne>
-void SolveImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s) {
+void ForwardLevel(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, int L) {
   const auto slots = SlotsOf<kOne>(a);
   const unsigned k = static_cast<unsigned>(sh.slots);
-  const int levels = static_cast<int>(sh.levels.size());
-  for (int L = 0; L < levels; ++L) {
-    const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
-    if (lv.count <= 0) continue;
-    // Few large fronts: one 1024-thread workgroup each for more loads in flight.
-    const int threads = lv.count <= 32 ? 1024 : kThreads;
-    const int tri = lv.count <= kSplitSolveFronts && lv.max_m > 0 ? 1 : 0;
-    const auto plan = LevelLists<kOne>(sh, L);
-    KForward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan,
-                                                                                               tri);
-    if (tri != 0) KForwardRest<<<dim3((lv.max_m + 63) / 64, lv.count, k), 64, 0, s>>>(slots, plan);
-  }
-  for (int L = levels - 1; L >= 0; --L) {
+  const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
+  if (lv.count <= 0) return;
+  // Few large fronts: one 1024-thread workgroup each for more loads in flight.
+  const int threads = lv.count <= 32 ? 1024 : kThreads;
+  const int tri = lv.count <= kSplitSolveFronts && lv.max_m > 0 ? 1 : 0;
+  const auto plan = LevelLists<kOne>(sh, L);
+  KForward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan, tri);
+  if (tri != 0) KForwardRest<<<dim3((lv.max_m + 63) / 64, lv.count, k), 64, 0, s>>>(slots, plan);
+}
+
+// Backward solves, top level first.
+template <bool kOne>
+void BackwardImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s) {
+  const auto slots = SlotsOf<kOne>(a);
+  const unsigned k = static_cast<unsigned>(sh.slots);
+  for (int L = static_cast<int>(sh.levels.size()) - 1; L >= 0; --L) {
     const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
     if (lv.count <= 0) continue;
     const int threads = lv.count <= 32 ? 1024 : kThreads;
@@ -2519,6 +2532,37 @@ void SolveImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s) {
     KBackward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan,
                                                                                                 tri);
   }
+}
+
+template <bool kOne>
+void SolveImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s) {
+  for (int L = 0; L < static_cast<int>(sh.levels.size()); ++L) ForwardLevel<kOne>(a, sh, s, L);
+  BackwardImpl<kOne>(a, sh, s);
+}
+
+// Factorization and solves with every level's forward solve on `aux`, as
+// soon as the level's panels are final, beside the factorization of the
+// levels above it on `s` (a few large fronts' dependent steps leave most of
+// the device free); the backward solves follow on `s` once the forward ones
+// are done. The same kernels as the sequential form, in the same order per
+// level, so the results are the same.
+template <bool kOne>
+hipError_t FactorSolveOverlapped(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, hipStream_t aux,
+                                 const hipEvent_t* events) {
+  hipError_t err = hipSuccess;
+  auto check = [&](hipError_t e) {
+    if (err == hipSuccess) err = e;
+  };
+  FactorImpl<kOne>(a, sh, s, nullptr, [&](int L) {
+    check(hipEventRecord(events[L], s));
+    check(hipStreamWaitEvent(aux, events[L], 0));
+    ForwardLevel<kOne>(a, sh, aux, L);
+  });
+  const hipEvent_t done = events[sh.levels.size()];
+  check(hipEventRecord(done, aux));
+  check(hipStreamWaitEvent(s, done, 0));
+  BackwardImpl<kOne>(a, sh, s);
+  return err;
 }
 
 template <bool kOne>
@@ -2565,10 +2609,16 @@ void LaunchNormalSlots(const SlotArgs& slots, const LaunchShape& sh, const LmOpt
 
 void LaunchFactor(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s, StepEvents* step_events) {
   if (OneOf(slots, sh)) {
-    FactorImpl<true>(slots, sh, s, step_events);
+    FactorImpl<true>(slots, sh, s, step_events, [](int) {});
   } else {
-    FactorImpl<false>(slots, sh, s, step_events);
+    FactorImpl<false>(slots, sh, s, step_events, [](int) {});
   }
+}
+
+hipError_t LaunchFactorSolveOverlapped(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s,
+                                       hipStream_t aux, const hipEvent_t* events) {
+  if (OneOf(slots, sh)) return FactorSolveOverlapped<true>(slots, sh, s, aux, events);
+  return FactorSolveOverlapped<false>(slots, sh, s, aux, events);
 }
 
 void LaunchSolve(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s) {

@@ -243,6 +243,13 @@ void LaunchFactor(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s,
                   StepEvents* step_events = nullptr);
 /// Forward and backward solves, level by level; the step in original order.
 void LaunchSolve(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s);
+/// LaunchFactor then LaunchSolve, with each level's forward solve on `aux`
+/// as soon as the level's panels are final, beside the factorization of
+/// the levels above it (same kernels, same results). `events` holds
+/// sh.levels.size() + 1 events (recorded and waited on here; they may be
+/// reused once this work is queued). Returns the first event error.
+hipError_t LaunchFactorSolveOverlapped(const SlotArgs& slots, const LaunchShape& sh, hipStream_t s,
+                                       hipStream_t aux, const hipEvent_t* events);
 
 /// Scaled eikonal Jacobian entries (3 per row) into j_eik (one slot).
 void LaunchAssembleEikonalJ(const SlotArgs& slots, const LaunchShape& sh, double* j_eik, hipStream_t s);
