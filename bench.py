@@ -78,11 +78,13 @@ def measured_traffic(kernel):
     return None
 
 
-def profile_avg_launch_us(kernels):
+def profile_avg_launch_us(kernels, counted=None):
     """Average kernel duration (us) of the kernels' launches in an LM iteration
     from the committed rocprofv3 summary (profiles/office200_kernel_stats.md,
-    per-iteration table), or None when a kernel has no row. The HIP-event
-    figures of this run bracket each launch and so also hold its dispatch."""
+    per-iteration table), or None when a kernel has no row: their time per
+    iteration over the launches of `counted` (default: all of them; a step
+    that is two kernels counts once). The HIP-event figures of this run
+    bracket each launch and so also hold its dispatch."""
     path = os.path.join(ROOT, "profiles", "office200_kernel_stats.md")
     if not os.path.exists(path):
         return None
@@ -96,7 +98,7 @@ def profile_avg_launch_us(kernels):
             rows[parts[0]] = (float(parts[1]), float(parts[2]))
     if set(rows) != set(kernels):
         return None
-    launches = sum(r[0] for r in rows.values())
+    launches = sum(r[0] for k, r in rows.items() if counted is None or k in counted)
     return sum(r[1] for r in rows.values()) / launches if launches > 0 else None
 
 
@@ -300,19 +302,26 @@ def main():
     bytes_launch = point_assembly_bytes(problem, info)
     kernel_ms = phases["point_assembly_kernel_ms"]
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
-    # Dominant kernel family: the trailing updates and update matrices of
-    # the supernodal LDL^T (KGemmLower / KGemmLowerAhead / KGemmUpdate,
-    # FP64 MFMA).
+    # Dominant kernel family: the trailing updates (fused with the next
+    # step's diagonal block and panel rows) and update matrices of the
+    # supernodal LDL^T (KGemmAhead / KGemmLower / KGemmUpdate, FP64 MFMA).
     gemm_flops_padded = info["gemm_flops"] / max(info["gemm_launches"], 1)
     gemm_flops = gemm_flops_padded * useful
     gemm_ms = phases["gemm_kernel_ms"]
     gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     gemm_tflops_padded = gemm_flops_padded / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
-    # The 64-column panel step (the rows below the diagonal block times its
-    # TRSM operator W = L_kk^-T D^-1, FP64 MFMA).
+    # The first 64-column panel step of each level (the fronts' first
+    # diagonal blocks, and the rows below them times W = L_kk^-T D^-1 on
+    # FP64 MFMA).
     step_flops = info["panel_step_flops"] / max(info["panel_step_launches"], 1) * useful
     step_ms = phases["panel_step_kernel_ms"]
     step_tflops = step_flops / (step_ms * 1e-3) / 1e12 if step_ms > 0 else 0.0
+    # Triangular solves: every factor entry (the supernodal panels as
+    # stored) is read once forward and once backward, the kept 64x64
+    # diagonal-block inverses once each way.
+    solve_ms = phases["solve_ms"]
+    solve_bytes = 2 * 8 * info["factor_nnz"] + 2 * 8 * 64 * 64 * info["diag_blocks"]
+    solve_gbs = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms > 0 else 0.0
     line = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -344,9 +353,10 @@ def main():
             "phase_ms": {k: round(v, 4) for k, v in phases.items()},
         },
         "roofline": {
-            "kernel": "KGemmLower + KGemmLowerAhead + KGemmUpdate: trailing updates (the look-ahead "
-                      "workgroup then factors and inverts the next diagonal block) and update matrices "
-                      "(flat lists of 64x64 tiles) of the supernodal LDL^T on FP64 MFMA",
+            "kernel": "KGemmAhead + KGemmLower + KGemmUpdate: trailing updates of the supernodal "
+                      "LDL^T on FP64 MFMA, each fused with the next step's diagonal block (factored "
+                      "and inverted by every tile-column-0 workgroup) and panel rows (times W), and "
+                      "update matrices (flat lists of 64x64 tiles)",
             "bound": "mfma",
             "achieved": gemm_tflops,
             "peak": FP64_PEAK_TFLOPS,
@@ -358,15 +368,16 @@ def main():
             "achieved_padded": gemm_tflops_padded,
             "launches_per_iteration": info["gemm_launches"],
             "avg_launch_ms": gemm_ms,
-            "profile_avg_launch_us": profile_avg_launch_us(["KGemmLowerAhead", "KGemmUpdate", "KGemmLower"]),
+            "profile_avg_launch_us": profile_avg_launch_us(["KGemmAhead", "KGemmUpdate", "KGemmLower"]),
             "note": "flops are the useful ones (the padded count of the supernodal superset pattern "
-                    "times reference_factor_gflop / factor_gflop); latency-bound: a level's launches are "
-                    "short chains of 64x64 tiles with K <= 64 and the top fronts' steps form a dependent "
-                    "chain",
+                    "times reference_factor_gflop / factor_gflop; the next step's diagonal block and "
+                    "TRSM counted once); latency-bound: the top fronts' steps form a dependent chain, "
+                    "each launch as long as its tile-column-0 workgroups' update, 64x64 LDL^T + inverse "
+                    "and TRSM",
         },
         "roofline_panel_step": {
-            "kernel": "KPanelStep (+KDiagBlock): 64-column panel step of the supernodal LDL^T, "
-                      "L21 = A21 W as a 64x64x64 FP64 MFMA product per 64 rows",
+            "kernel": "KPanelStep (+KDiagBlock): first 64-column panel step of each level of the "
+                      "supernodal LDL^T, L21 = A21 W as a 64x64x64 FP64 MFMA product per 64 rows",
             "bound": "mfma",
             "achieved": step_tflops,
             "peak": FP64_PEAK_TFLOPS,
@@ -376,10 +387,23 @@ def main():
             "flops_per_launch": step_flops,
             "launches_per_iteration": info["panel_step_launches"],
             "avg_launch_ms": step_ms,
-            "profile_avg_launch_us": profile_avg_launch_us(["KPanelStep"]),
-            "note": "latency-bound: at most a few thousand rows per step; flops are the TRSM's "
-                    "(rows x nb^2, plus the diagonal block's 2/3 nb^3 on the first step), "
-                    "not the product's, times the useful fraction",
+            "profile_avg_launch_us": profile_avg_launch_us(["KPanelStep", "KDiagBlock"], ["KPanelStep"]),
+            "note": "latency-bound: one step per level, the fronts' first diagonal blocks then "
+                    "their rows; flops are the TRSM's (rows x nb^2) and the block's 2/3 nb^3, not the "
+                    "product's, times the useful fraction",
+        },
+        "roofline_solves": {
+            "kernel": "KForward + KBackward (+ Rest): level-scheduled supernodal triangular solves",
+            "bound": "hbm",
+            "achieved": solve_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": solve_gbs / HBM_PEAK_GBS,
+            "bytes_per_solve": solve_bytes,
+            "solve_ms": solve_ms,
+            "note": "bytes: the stored factor read once each way and the diagonal-block inverses; "
+                    "time: both solves of an iteration, back to back; latency-bound: the top fronts' "
+                    "blocks are solved one after another by one workgroup each",
         },
         "roofline_assembly": {
             "kernel": "KAssemblePoints (point-row residual + Jacobian assembly)",

@@ -226,6 +226,9 @@ typedef struct sdfslam_hip_analysis_info {
    * trailing update prepares, counted once). */
   int32_t gemm_launches;
   double gemm_flops;
+  /* 64-column diagonal blocks of the factor (one kept 64 x 64 inverse
+   * each, read by the solves). */
+  int64_t diag_blocks;
 } sdfslam_hip_analysis_info;
 int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* info);
 

@@ -552,7 +552,7 @@ void Enqueue(sdfslam_hip_context& c, std::function<void(hipStream_t)> work) {
 void Bind(sdfslam_hip_context& c) {
   HIPC(hipSetDevice(c.device));
   c.stream = t_streams.Get(c.device);
-  c.aux_stream = t_aux_streams.Get(c.device);
+  c.aux_stream = nullptr;  // the calling thread's second stream, taken when needed
   std::vector<std::function<void(hipStream_t)>> work;
   work.swap(c.pending);
   for (auto& w : work) w(c.stream);
@@ -1000,6 +1000,14 @@ void EnqueueAssemblyAndNormal(sdfslam_hip_context& c, const LmOptions& o, bool t
   LaunchNormalSlots(SlotArgs{c.slot_dev, &c.slot}, c.shape, o, s);
 }
 
+// Factorizations of at least this many flops run their lower levels'
+// forward solves on the calling thread's second stream (a long chain of
+// dependent top-level steps leaves the device room for them). Smaller ones,
+// and batched solves, keep one stream: several pipelines with two streams
+// each share the device's few hardware queues, which then serializes
+// unrelated work (the served form at 10 Hz went from p50 20 ms to 110 ms).
+constexpr double kOverlapMinFlops = 1e9;
+
 void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
   hipStream_t s = c.stream;
   const Analysis& an = c.an;
@@ -1020,8 +1028,9 @@ void EnqueueFactorAndSolve(sdfslam_hip_context& c, bool timing) {
     step_ev = {c.step_events.data(), static_cast<int>(c.step_events.size() / 2), 0,
                c.gemm_events.data(), static_cast<int>(c.gemm_events.size() / 2), 0};
   }
-  if (!timing) {
+  if (!timing && an.factor_flops >= kOverlapMinFlops) {
     // The lower levels' forward solves beside the upper levels' factorization.
+    if (c.aux_stream == nullptr) c.aux_stream = t_aux_streams.Get(c.device);
     HIPC(LaunchFactorSolveOverlapped(SlotArgs{c.slot_dev, &c.slot}, c.shape, s, c.aux_stream,
                                      OverlapEvents(c.overlap_events, c.shape.levels.size() + 1)));
     return;
@@ -1383,8 +1392,8 @@ class BatchRun {
     HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
     LaunchAssemble(args, shape_, true, stream_);
     LaunchNormalSlots(args, shape_, o, stream_);
-    HIPC(LaunchFactorSolveOverlapped(args, shape_, stream_, ctxs_[0]->aux_stream,
-                                     OverlapEvents(ctxs_[0]->overlap_events, shape_.levels.size() + 1)));
+    LaunchFactor(args, shape_, stream_);
+    LaunchSolve(args, shape_, stream_);
     LaunchStepAndControl(args, shape_, o, stream_);
     HIPC(hipStreamEndCapture(stream_, &g));
     const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
@@ -1754,6 +1763,10 @@ int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* in
     info->speculative_epochs = ctx->speculative_epochs;
     info->gemm_launches = a.gemm_launches;
     info->gemm_flops = a.gemm_flops;
+    info->diag_blocks = 0;
+    for (int s = 0; s < a.num_sn; ++s) {
+      info->diag_blocks += (a.sn_first[static_cast<size_t>(s) + 1] - a.sn_first[static_cast<size_t>(s)] + 63) / 64;
+    }
     return SDFSLAM_OK;
   });
 }

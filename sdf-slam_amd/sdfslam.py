@@ -72,7 +72,8 @@ class AnalysisInfo(C.Structure):
                 ("pattern_epochs", C.c_int32), ("analysis_seconds", C.c_double),
                 ("device_bytes", C.c_int64), ("panel_step_launches", C.c_int32),
                 ("panel_step_flops", C.c_double), ("speculative_epochs", C.c_int32),
-                ("gemm_launches", C.c_int32), ("gemm_flops", C.c_double)]
+                ("gemm_launches", C.c_int32), ("gemm_flops", C.c_double),
+                ("diag_blocks", C.c_int64)]
 
 
 class PhaseTimes(C.Structure):
