@@ -441,6 +441,7 @@ struct sdfslam_hip_context {
   int device{0};
   hipStream_t stream{nullptr};  // the calling thread's stream, set at every call
   hipStream_t aux_stream{nullptr};  // the calling thread's second stream
+  size_t solve_sync_words{0};        // of dw.solve_sync
   std::vector<hipEvent_t> overlap_events;  // level marks of the overlapped solves
   // Device work of a context update, deferred to the context's next call
   // (which runs it first, on its own stream): an update does host work only.
@@ -964,6 +965,10 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
     c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 2 * 64 * 64);
     c.dw.diag_inv = E.Alloc<double>(static_cast<size_t>(inv_off.back()));
+    // Multi-workgroup solve words: 3 per supernode and one per 64-column
+    // block, uploaded as zeros.
+    c.solve_sync_words = static_cast<size_t>(inv_off.back() / (64 * 64) + 3 * an.num_sn);
+    c.dw.solve_sync = E.Upload(std::vector<int32_t>(c.solve_sync_words, 0));
     c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
     // The launch plans and the slot come last: they hold addresses placed
     // above (known in the placing pass).
@@ -1119,7 +1124,27 @@ void AccumulateTimes(sdfslam_hip_context& c) {
   ++c.timed_iterations;
 }
 
+/// Clears the multi-workgroup solve words (normally clear between launches;
+/// an aborted launch can leave them set).
+void ClearSolveSync(sdfslam_hip_context& c, hipStream_t s) {
+  if (c.dw.solve_sync != nullptr && c.solve_sync_words > 0) {
+    HIPC(hipMemsetAsync(c.dw.solve_sync, 0, sizeof(int32_t) * c.solve_sync_words, s));
+  }
+}
+
+/// The status of a stopped LM state: a zero pivot, or a multi-workgroup
+/// solve fault.
+int StateStatus(const LmState& st) {
+  if (st.not_spd == kSolveSyncFault) {
+    g_last_error = "triangular solve workgroups gave up waiting for each other";
+    return SDFSLAM_ERR_INTERNAL;
+  }
+  g_last_error = "sparse Cholesky factorization failed";
+  return SDFSLAM_ERR_NOT_SPD;
+}
+
 void InitLm(sdfslam_hip_context& c, double lambda_init) {
+  ClearSolveSync(c, c.stream);
   LaunchCost(SlotArgs{c.slot_dev, &c.slot}, c.shape, c.stream);
   LaunchLmInit(SlotArgs{c.slot_dev, &c.slot}, c.shape, lambda_init, c.stream);
   c.lm_ready = true;
@@ -1135,6 +1160,7 @@ void PrepareStage(sdfslam_hip_context& c, const double* lambda) {
   st.revert = 0;
   if (lambda != nullptr) st.lambda = *lambda;
   WriteState(c, st);
+  ClearSolveSync(c, c.stream);
 }
 
 /// Waits for a running speculative re-analysis and discards it.
@@ -1297,6 +1323,7 @@ class BatchRun {
     std::vector<int> repeats(static_cast<size_t>(k), 0);
     Rebuild();
     // LM initialisation of every slot: two launches over the table.
+    for (sdfslam_hip_context* c : ctxs_) ClearSolveSync(*c, stream_);
     const SlotArgs all{space_.table, nullptr};
     LaunchCost(all, shape_, stream_);
     LaunchLmInit(all, shape_, options.lambda_init, stream_);
@@ -1316,7 +1343,7 @@ class BatchRun {
         LmState& st = states[static_cast<size_t>(b)];
         sdfslam_hip_context& c = *ctxs_[static_cast<size_t>(b)];
         if (st.not_spd != 0) {
-          status[static_cast<size_t>(b)] = SDFSLAM_ERR_NOT_SPD;
+          status[static_cast<size_t>(b)] = StateStatus(st);
           continue;
         }
         if (st.overflow == 0) {
@@ -1602,10 +1629,7 @@ int SolveContext(sdfslam_hip_context& c, const sdfslam_solver_options& options,
   const LmOptions o = ToLm(options, c.history_capacity);
   InitLm(c, options.lambda_init);
   const LmState st = RunIterations(c, o, std::max(options.max_iterations, 0), false);
-  if (st.not_spd != 0) {
-    g_last_error = "sparse Cholesky factorization failed";
-    return SDFSLAM_ERR_NOT_SPD;
-  }
+  if (st.not_spd != 0) return StateStatus(st);
   if (result != nullptr) FillResult(c, st, result);
   return SDFSLAM_OK;
 }
@@ -1998,10 +2022,7 @@ int sdfslam_hip_linear_step(sdfslam_hip_context* ctx, double lambda, double* ste
       if (attempt == 1) throw Failure(SDFSLAM_ERR_INTERNAL, "pattern re-analysis failed");
       BuildEpoch(c);
     }
-    if (st.not_spd != 0) {
-      g_last_error = "sparse Cholesky factorization failed";
-      return SDFSLAM_ERR_NOT_SPD;
-    }
+    if (st.not_spd != 0) return StateStatus(st);
     HIPC(hipMemcpyAsync(step, c.dw.step, sizeof(double) * c.an.n, hipMemcpyDeviceToHost, c.stream));
     HIPC(hipStreamSynchronize(c.stream));
     return SDFSLAM_OK;
@@ -2144,10 +2165,7 @@ int sdfslam_hip_context_iterate(sdfslam_hip_context* ctx, int32_t iterations,
       WriteState(c, st);
     }
     st = RunIterations(c, o, iterations, collect_phase_times != 0);
-    if (st.not_spd != 0) {
-      g_last_error = "sparse Cholesky factorization failed";
-      return SDFSLAM_ERR_NOT_SPD;
-    }
+    if (st.not_spd != 0) return StateStatus(st);
     return SDFSLAM_OK;
   });
 }

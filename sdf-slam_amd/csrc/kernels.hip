@@ -2033,6 +2033,265 @@ __global__ void KBackward(Slots slots, Lists plan, int tri) {
   }
 }
 
+// ------------------------------------ multi-workgroup solves of wide fronts
+
+// Synchronization words of supernode s's multi-workgroup solves: a ticket
+// counter, a finished counter, an abort word, then one ready flag per
+// 64-column block. All are zero between launches (the last workgroup of a
+// front to finish clears them).
+__device__ __forceinline__ int32_t* SolveSync(const DeviceAnalysis& A, const DeviceWork& W, int s) {
+  return W.solve_sync + A.inv_off[s] / (kNB * kNB) + 3 * static_cast<int64_t>(s);
+}
+
+// Polls of a ready flag (with a short sleep between) before a wait gives up:
+// about a second, against microseconds in a working solve.
+constexpr int kSpinLimit = 1 << 20;
+
+// Thread 0 waits for a block's ready flag (acquire, device scope); false
+// when the front's solve aborted or the wait gave up (then it aborts it).
+__device__ bool WaitReady(int32_t* flag, int32_t* abort_word) {
+  for (int n = 0; n < kSpinLimit; ++n) {
+    if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != 0) return true;
+    if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+
+// The workgroup's ticket (the order in which the front's workgroups
+// started): a workgroup only waits for blocks of smaller tickets, whose
+// workgroups are already running, so the waits always end.
+__device__ __forceinline__ int TakeTicket(int32_t* sync, int* shared_ticket) {
+  if (threadIdx.x == 0) *shared_ticket = atomicAdd(&sync[0], 1);
+  __syncthreads();
+  return *shared_ticket;
+}
+
+// Publishes this workgroup's block (its stores are done by every thread
+// before the barrier), then the last of the front's nbk workgroups clears
+// the words for the next launch and reports an aborted solve.
+__device__ void PublishBlock(int32_t* sync, int kb, int nbk, LmState* state) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&sync[3 + kb], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (atomicAdd(&sync[1], 1) == nbk - 1) {
+      if (__hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+        state->not_spd = kSolveSyncFault;
+        state->halt = 1;
+      }
+      for (int q = 0; q < nbk; ++q) sync[3 + q] = 0;
+      sync[0] = 0;
+      sync[1] = 0;
+      sync[2] = 0;
+    }
+  }
+}
+
+// Forward solve L y = v over the own columns of the level's fronts, one
+// workgroup per 64-column block (taken in ticket order): the block's rows
+// start from the right-hand side and the children's update vectors, take
+// the row update of every earlier block as soon as its y is ready (the
+// same partial sums as RowUpdate: thread p of a row sums the columns
+// k0 + p, k0 + p + 4, ...), then y_b = M_b v_b with the kept inverse (the
+// same four partial sums as KForward's diagonal block). Bit for bit
+// KForward's result; the rows below the own columns are left as assembled
+// (ticket 0 writes them) for KForwardRest.
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads) KForwardBlocks(Slots slots, Lists plan) {
+  BIND_SLOT(blockIdx.z);
+  if (Halted(W.state)) return;
+  const int s = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (s < 0) return;
+  const int first = A.sn_first[s];
+  const int n = A.sn_first[s + 1] - first;
+  const int nbk = (n + kNB - 1) / kNB;
+  if (static_cast<int>(blockIdx.x) >= nbk) return;
+  const int f = A.sn_f[s];
+  const double* F = W.panel + A.panel_off[s];
+  double* out = W.vec + A.sn_idx_ptr[s];
+  int32_t* sync = SolveSync(A, W, s);
+  __shared__ int ticket;
+  __shared__ double v[kNB];   // the block's rows (to vend)
+  __shared__ double yq[kNB];  // y of the block being applied
+  const int tid = threadIdx.x;
+  const int kb = TakeTicket(sync, &ticket);
+  if (kb >= nbk) {  // words left over from an aborted launch: abort this one too
+    if (tid == 0) __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const int b0 = kb * kNB;
+  const int b1 = min(n, b0 + kNB);
+  const int vend = min(b0 + kNB, f);
+  const int r = tid >> 2;  // row b0 + r
+  const int p = tid & 3;   // columns k0 + p + 4 j
+  // The diagonal block's inverse, this thread's entries (t = p + 4 j of row r).
+  const double* Mb = DiagInv(A, W, s, kb);
+  double mv[kNB / 4];
+#pragma unroll
+  for (int j = 0; j < kNB / 4; ++j) mv[j] = Mb[(p + 4 * j) * kNB + r];
+  for (int i = tid; i < kNB; i += kThreads) v[i] = b0 + i < n ? W.rhs[A.iperm[first + b0 + i]] : 0.0;
+  __syncthreads();
+  for (int64_t ci = A.children_ptr[s]; ci < A.children_ptr[s + 1]; ++ci) {
+    const int c = A.children[ci];
+    const int mc = A.sn_m[c];
+    const int nc = A.sn_first[c + 1] - A.sn_first[c];
+    const int32_t* rel = A.rel + A.rel_off[c];
+    const double* u = W.vec + A.sn_idx_ptr[c] + nc;
+    for (int a = tid; a < mc; a += kThreads) {
+      const int i = rel[a];
+      if (i >= b0 && i < vend) v[i - b0] += u[a];
+    }
+    __syncthreads();
+  }
+  if (kb == 0) {
+    // The rows below the own columns, as assembled (for KForwardRest).
+    for (int i = n + tid; i < f; i += kThreads) out[i] = 0.0;
+    __syncthreads();
+    for (int64_t ci = A.children_ptr[s]; ci < A.children_ptr[s + 1]; ++ci) {
+      const int c = A.children[ci];
+      const int mc = A.sn_m[c];
+      const int nc = A.sn_first[c + 1] - A.sn_first[c];
+      const int32_t* rel = A.rel + A.rel_off[c];
+      const double* u = W.vec + A.sn_idx_ptr[c] + nc;
+      for (int a = tid; a < mc; a += kThreads) {
+        const int i = rel[a];
+        if (i >= n) out[i] += u[a];
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+  }
+  // Row updates of every earlier block, in block order.
+  const bool own_row = b0 + r < b1;
+  const int row = min(b0 + r, f - 1);
+  double run = p == 0 ? v[r] : 0.0;  // the row's running value (thread p = 0)
+  double fv[kNB / 4];
+  auto load_block = [&](int q) {
+#pragma unroll
+    for (int j = 0; j < kNB / 4; ++j) fv[j] = F[static_cast<int64_t>(q * kNB + p + 4 * j) * f + row];
+  };
+  if (kb > 0) load_block(0);
+  for (int q = 0; q < kb; ++q) {
+    if (tid == 0) (void)WaitReady(&sync[3 + q], &sync[2]);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (tid < kNB) yq[tid] = out[q * kNB + tid];
+    __syncthreads();
+    double acc = p == 0 ? run : 0.0;
+#pragma unroll
+    for (int j = 0; j < kNB / 4; ++j) acc = fma(-fv[j], yq[p + 4 * j], acc);
+    if (q + 1 < kb) load_block(q + 1);  // in flight during the next wait
+    const double a1 = __shfl_down(acc, 1, 4);
+    const double a2 = __shfl_down(acc, 2, 4);
+    const double a3 = __shfl_down(acc, 3, 4);
+    if (p == 0 && own_row) run = (acc + a1) + (a2 + a3);
+    __syncthreads();  // yq is rewritten next
+  }
+  if (p == 0 && own_row) v[r] = run;
+  __syncthreads();
+  // y_b = M_b v_b (identity padded; rows past vend read the last one).
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kNB / 4; ++j) acc = fma(mv[j], v[min(p + 4 * j, vend - 1 - b0)], acc);
+  const double a1 = __shfl_down(acc, 1, 4);
+  const double a2 = __shfl_down(acc, 2, 4);
+  const double a3 = __shfl_down(acc, 3, 4);
+  if (p == 0 && own_row) out[b0 + r] = (acc + a1) + (a2 + a3);
+  PublishBlock(sync, kb, nbk, W.state);
+}
+
+// Backward solve over the own columns of the level's fronts, one workgroup
+// per 64-column block, the last block first (ticket order): t = D^-1 y
+// (less L21^T x2 from KBackwardRest when the front has rows below), less
+// L_qb^T x_q for every later block q as soon as its x is ready (thread p of
+// a column sums rows 16 p .. 16 p + 15 of the block), then x_b = M_b^T t_b
+// with the kept inverse.
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads) KBackwardBlocks(Slots slots, Lists plan, int tri) {
+  BIND_SLOT(blockIdx.z);
+  if (Halted(W.state)) return;
+  const int s = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (s < 0) return;
+  const int first = A.sn_first[s];
+  const int n = A.sn_first[s + 1] - first;
+  const int nbk = (n + kNB - 1) / kNB;
+  if (static_cast<int>(blockIdx.x) >= nbk) return;
+  const int f = A.sn_f[s];
+  const double* F = W.panel + A.panel_off[s];
+  const double* y = W.vec + A.sn_idx_ptr[s];
+  const int32_t* idx = A.sn_idx + A.sn_idx_ptr[s];
+  int32_t* sync = SolveSync(A, W, s);
+  __shared__ int ticket;
+  __shared__ double t[kNB];   // the block's t, then x
+  __shared__ double xq[kNB];  // x of the block being applied
+  const int tid = threadIdx.x;
+  const int kb = nbk - 1 - TakeTicket(sync, &ticket);
+  if (kb < 0) {  // words left over from an aborted launch: abort this one too
+    if (tid == 0) __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  const int b0 = kb * kNB;
+  const int b1 = min(n, b0 + kNB);
+  const int c = tid >> 2;  // column b0 + c
+  const int p = tid & 3;   // rows 16 p .. 16 p + 15 of a block, or i = p + 4 j of the diagonal block
+  const int col = min(b0 + c, n - 1);
+  const double* Mc = DiagInv(A, W, s, kb) + c * kNB;
+  double mv[kNB / 4];
+#pragma unroll
+  for (int j = 0; j < kNB / 4; ++j) mv[j] = Mc[p + 4 * j];
+  for (int i = tid; i < kNB; i += kThreads) {
+    const int j = b0 + i;
+    if (j < b1) {
+      const double yd = y[j] / F[static_cast<int64_t>(j) * f + j];
+      t[i] = tri != 0 ? yd - W.solve_tmp[first + j] : yd;
+    } else {
+      t[i] = j < f ? W.xnew[idx[j]] : 0.0;  // past the own columns (meets the inverse's zero padding)
+    }
+  }
+  __syncthreads();
+  double run = p == 0 ? t[c] : 0.0;  // the column's running value (thread p = 0)
+  double fv[16];
+  auto load_block = [&](int q) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) fv[j] = F[static_cast<int64_t>(col) * f + min(q * kNB + 16 * p + j, f - 1)];
+  };
+  if (kb + 1 < nbk) load_block(nbk - 1);
+  for (int q = nbk - 1; q > kb; --q) {
+    if (tid == 0) (void)WaitReady(&sync[3 + q], &sync[2]);
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int q1 = min(n, q * kNB + kNB);
+    if (tid < kNB) xq[tid] = q * kNB + tid < q1 ? W.xnew[first + q * kNB + tid] : 0.0;
+    __syncthreads();
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc = fma(fv[j], xq[16 * p + j], acc);
+    if (q - 1 > kb) load_block(q - 1);  // in flight during the next wait
+    const double a1 = __shfl_down(acc, 1, 4);
+    const double a2 = __shfl_down(acc, 2, 4);
+    const double a3 = __shfl_down(acc, 3, 4);
+    if (p == 0) run -= (acc + a1) + (a2 + a3);
+    __syncthreads();  // xq is rewritten next
+  }
+  if (p == 0 && b0 + c < b1) t[c] = run;
+  __syncthreads();
+  // x_b = M_b^T t_b: column c of M (identity padded).
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < kNB / 4; ++j) acc = fma(mv[j], t[p + 4 * j], acc);
+  const double a1 = __shfl_down(acc, 1, 4);
+  const double a2 = __shfl_down(acc, 2, 4);
+  const double a3 = __shfl_down(acc, 3, 4);
+  if (p == 0 && b0 + c < b1) {
+    const double x = (acc + a1) + (a2 + a3);
+    W.xnew[first + b0 + c] = x;
+    W.step[A.iperm[first + b0 + c]] = x;
+  }
+  PublishBlock(sync, kb, nbk, W.state);
+}
+
 // ---------------------------------------------------------------- LM control
 
 __device__ double BlockSum(double v, double* red) {
@@ -2498,8 +2757,10 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
   }
 }
 
-// Levels of at most this many fronts move the L21 products of the solves
-// into launches of their own that spread them over many workgroups.
+// Levels of at most this many fronts solve each front with one workgroup per
+// 64-column block (KForwardBlocks, KBackwardBlocks) and move the L21
+// products into launches of their own that spread them over many
+// workgroups.
 constexpr int kSplitSolveFronts = 16;
 
 // Forward solve of level L's fronts (after their children's).
@@ -2509,12 +2770,18 @@ void ForwardLevel(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, int L
   const unsigned k = static_cast<unsigned>(sh.slots);
   const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
   if (lv.count <= 0) return;
+  const auto plan = LevelLists<kOne>(sh, L);
+  if (lv.count <= kSplitSolveFronts) {
+    // Few fronts: one workgroup per 64-column block of each, the rows
+    // below the own columns spread over many workgroups after.
+    const int blocks = (lv.max_nb + kNB - 1) / kNB;
+    KForwardBlocks<<<dim3(blocks, lv.count, k), kThreads, 0, s>>>(slots, plan);
+    if (lv.max_m > 0) KForwardRest<<<dim3((lv.max_m + 63) / 64, lv.count, k), 64, 0, s>>>(slots, plan);
+    return;
+  }
   // Few large fronts: one 1024-thread workgroup each for more loads in flight.
   const int threads = lv.count <= 32 ? 1024 : kThreads;
-  const int tri = lv.count <= kSplitSolveFronts && lv.max_m > 0 ? 1 : 0;
-  const auto plan = LevelLists<kOne>(sh, L);
-  KForward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan, tri);
-  if (tri != 0) KForwardRest<<<dim3((lv.max_m + 63) / 64, lv.count, k), 64, 0, s>>>(slots, plan);
+  KForward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan, 0);
 }
 
 // Backward solves, top level first.
@@ -2525,12 +2792,16 @@ void BackwardImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s) {
   for (int L = static_cast<int>(sh.levels.size()) - 1; L >= 0; --L) {
     const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
     if (lv.count <= 0) continue;
-    const int threads = lv.count <= 32 ? 1024 : kThreads;
-    const int tri = lv.count <= kSplitSolveFronts && lv.max_m > 0 ? 1 : 0;
     const auto plan = LevelLists<kOne>(sh, L);
-    if (tri != 0) KBackwardRest<<<dim3((lv.max_nb + 3) / 4, lv.count, k), kThreads, 0, s>>>(slots, plan);
-    KBackward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan,
-                                                                                                tri);
+    if (lv.count <= kSplitSolveFronts) {
+      const int tri = lv.max_m > 0 ? 1 : 0;
+      if (tri != 0) KBackwardRest<<<dim3((lv.max_nb + 3) / 4, lv.count, k), kThreads, 0, s>>>(slots, plan);
+      const int blocks = (lv.max_nb + kNB - 1) / kNB;
+      KBackwardBlocks<<<dim3(blocks, lv.count, k), kThreads, 0, s>>>(slots, plan, tri);
+      continue;
+    }
+    const int threads = lv.count <= 32 ? 1024 : kThreads;
+    KBackward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan, 0);
   }
 }
 

@@ -31,9 +31,13 @@ struct LmState {
   int halt;          // 1 once converged, out of iterations, or failed
   int converged;
   int overflow;      // a point left its pose's footprint: re-analyse
-  int not_spd;       // zero pivot
+  int not_spd;       // zero pivot (kSolveSyncFault: a multi-workgroup solve gave up waiting)
   int revert;        // restore the backup after this iteration
 };
+
+// not_spd value of a multi-workgroup triangular solve whose workgroups gave
+// up waiting for each other (an internal fault, not a pivot).
+constexpr int kSolveSyncFault = 2;
 
 struct LmOptions {
   int max_iterations;
@@ -110,6 +114,7 @@ struct DeviceWork {
   double* cost_history;
   int32_t* pose_overflow;  // per frame: 1 when one of its points left the footprint
   double* solve_tmp;       // L21^T x2 per column (elimination order), backward solve
+  int32_t* solve_sync;     // per supernode: ticket, finished, abort, then a flag per 64-column block
   LmState* state;
 };
 
