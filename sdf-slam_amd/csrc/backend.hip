@@ -853,6 +853,26 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     const int ns = an.sn_first[static_cast<size_t>(s) + 1] - an.sn_first[static_cast<size_t>(s)];
     inv_off[static_cast<size_t>(s) + 1] = inv_off[static_cast<size_t>(s)] + static_cast<int64_t>((ns + 63) / 64) * 64 * 64;
   }
+  // Per child whose entries reach its parent's update matrix: the map from
+  // the parent's update rows to the child's (rel minus the parent's own
+  // columns, inverted; -1 where the child has no row), for the gather in the
+  // parent's update-matrix kernel.
+  std::vector<int64_t> uinv_off(static_cast<size_t>(an.num_sn), -1);
+  std::vector<int32_t> uinv;
+  for (int ch = 0; ch < an.num_sn; ++ch) {
+    const int par = an.sn_parent[static_cast<size_t>(ch)];
+    const int mc = an.sn_m[static_cast<size_t>(ch)];
+    const int split = an.rel_split[static_cast<size_t>(ch)];
+    if (par < 0 || split >= mc) continue;
+    const int np = an.sn_first[static_cast<size_t>(par) + 1] - an.sn_first[static_cast<size_t>(par)];
+    const int mp = an.sn_m[static_cast<size_t>(par)];
+    uinv_off[static_cast<size_t>(ch)] = static_cast<int64_t>(uinv.size());
+    uinv.resize(uinv.size() + static_cast<size_t>(mp), -1);
+    const int32_t* rel = an.rel.data() + an.rel_off[static_cast<size_t>(ch)];
+    for (int a = split; a < mc; ++a) {
+      uinv[static_cast<size_t>(uinv_off[static_cast<size_t>(ch)] + rel[a] - np)] = a;
+    }
+  }
   // Level lists and, per level and child rank, the children to extend-add,
   // with the launch shape of this context.
   std::vector<int32_t> lvl;
@@ -887,7 +907,6 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
       int count = 0;
       int max_m = 0;
       int max_split = 0;
-      int max_rest = 0;
       for (const int s : an.level_sn[static_cast<size_t>(L)]) {
         const auto& ch = an.children[static_cast<size_t>(s)];
         if (static_cast<int>(ch.size()) > r) {
@@ -898,14 +917,12 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
           const int split = an.rel_split[static_cast<size_t>(child)];
           max_m = std::max(max_m, mc);
           max_split = std::max(max_split, split);
-          max_rest = std::max(max_rest, mc - split);
         }
       }
       rank_off.push_back(static_cast<int64_t>(ranks.size()));
       lv.rank_count.push_back(count);
       lv.rank_max_m.push_back(max_m);
       lv.rank_max_split.push_back(max_split);
-      lv.rank_max_rest.push_back(max_rest);
     }
     rank_first[static_cast<size_t>(L) + 1] = static_cast<int64_t>(rank_off.size()) - 1;
   }
@@ -935,6 +952,8 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     D.rel_off = E.Upload(an.rel_off);
     D.rel = E.Upload(an.rel);
     D.rel_split = E.Upload(an.rel_split);
+    D.uinv_off = E.Upload(uinv_off);
+    D.uinv = E.Upload(uinv);
     D.nn_off = E.Upload(an.nn_off);
     D.eik_at = E.Upload(an.eik_at);
     D.fp_ptr = E.Upload(an.fp_ptr);

@@ -840,10 +840,10 @@ __device__ __forceinline__ void ScatterAddColumn(double* dst, int64_t col, int o
   }
 }
 
-// Extend-add of a child's update matrix: columns [0, split) land in the
-// parent's panel, columns [split, m_c) in the parent's update matrix. One
-// wavefront per column, kWaves columns per workgroup; within one launch no
-// two children share a parent.
+// Extend-add of a child's update matrix, columns [0, split), into the
+// parent's panel (columns [split, m_c) are gathered by the parent's
+// update-matrix kernel). One wavefront per column, kWaves columns per
+// workgroup; within one launch no two children share a parent.
 template <class Slots, class Lists>
 __global__ void KExtendAddPanel(Slots slots, Lists plan) {
   BIND_SLOT(blockIdx.z);
@@ -862,24 +862,6 @@ __global__ void KExtendAddPanel(Slots slots, Lists plan) {
   ScatterAddColumn(fpanel, col, 0, rel, uc + static_cast<int64_t>(b) * mc, b, static_cast<int>(threadIdx.x & 63), mc);
 }
 
-template <class Slots, class Lists>
-__global__ void KExtendAddUpdate(Slots slots, Lists plan) {
-  BIND_SLOT(blockIdx.z);
-  if (Halted(W.state)) return;
-  const int c = PlanItem(plan, blockIdx.z, blockIdx.y);
-  if (c < 0) return;
-  const int mc = A.sn_m[c];
-  const int b = A.rel_split[c] + static_cast<int>(blockIdx.x) * kWaves + static_cast<int>(threadIdx.x >> 6);
-  if (b >= mc) return;
-  const int p = A.sn_parent[c];
-  const int np = A.sn_first[p + 1] - A.sn_first[p];
-  const int mp = A.sn_m[p];
-  const double* uc = W.upd + A.upd_off[c];
-  const int32_t* rel = A.rel + A.rel_off[c];
-  double* up = W.upd + A.upd_off[p];
-  const int64_t col = static_cast<int64_t>(rel[b] - np) * mp;
-  ScatterAddColumn(up, col, np, rel, uc + static_cast<int64_t>(b) * mc, b, static_cast<int>(threadIdx.x & 63), mc);
-}
 
 // Row stride (doubles) of the panel step's LDS tiles: two doubles of
 // padding put the four k rows an MFMA operand read touches in different
@@ -1572,10 +1554,55 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
   if constexpr (mode == 0) {
     SubtractLower(C, ldc, R0, C0, M, N, acc[0]);
   } else {
+    // U = -L21 D L21^T plus, child by child in ascending order, the entries
+    // of the children's update matrices that land in it (their columns past
+    // rel_split: the extend-add of the update part, gathered through each
+    // child's map from the parent's update rows to its own, -1 where it
+    // has none).
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wr = 32 * (wave & 1);
     const int wc = 32 * (wave >> 1);
+    double v[2][4][2];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) v[cb][e][rb] = 0.0 - acc[0][cb][rb][e];
+    for (int64_t ci = A.children_ptr[s]; ci < A.children_ptr[s + 1]; ++ci) {
+      const int ch = A.children[ci];
+      const int64_t uoff = A.uinv_off[ch];
+      if (uoff < 0) continue;
+      const int32_t* inv = A.uinv + uoff;
+      const int mc = A.sn_m[ch];
+      const double* uc = W.upd + A.upd_off[ch];
+      int crow[2], ccol[2][4];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) crow[rb] = inv[min(R0 + wr + 16 * rb + (lane & 15), M - 1)];
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ccol[cb][e] = inv[min(C0 + wc + 16 * cb + 4 * e + (lane >> 4), N - 1)];
+      double u[2][4][2];
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) {
+            u[cb][e][rb] = uc[static_cast<int64_t>(max(ccol[cb][e], 0)) * mc + max(crow[rb], 0)];
+          }
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) {
+            const double x = Opaque(u[cb][e][rb]);
+            if (crow[rb] >= 0 && ccol[cb][e] >= 0) v[cb][e][rb] = v[cb][e][rb] + x;
+          }
+    }
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
@@ -1584,7 +1611,7 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) {
           const int r = R0 + wr + 16 * rb + (lane & 15);
-          if (c < N && r < M && c <= r) C[static_cast<int64_t>(c) * ldc + r] = 0.0 - acc[0][cb][rb][e];
+          if (c < N && r < M && c <= r) C[static_cast<int64_t>(c) * ldc + r] = v[cb][e][rb];
         }
       }
     }
@@ -1789,8 +1816,8 @@ __global__ void __launch_bounds__(kThreads, 4) KGemmUpdate(Slots slots, Lists ti
 
 // acc - sum_k F(k, i) v[k] over one block column k in [k0, k1): column k
 // goes to partial sum (k - k0) % 4, in increasing k, whatever the batch of
-// loads issued together (kBatch, a multiple of 4), so the forward solve's
-// fused and split forms produce the same values.
+// loads issued together (kBatch, a multiple of 4); KForwardBlocks forms the
+// same partial sums with four threads per row.
 template <int kBatch>
 __device__ __forceinline__ double RowUpdate(const double* F, int f, int i, const double* v, int k0, int k1,
                                             double acc0) {
@@ -1874,8 +1901,9 @@ __global__ void KForward(Slots slots, Lists plan, int tri) {
   for (int i = tid; i < f; i += blockDim.x) out[i] = v[i];
 }
 
-// v2 -= L21 v1 for the fronts KForward solved with `tri`: one thread per
-// row, block column by block column as the fused kernel (RowUpdate).
+// v2 -= L21 v1 for the rows below the own columns (KForwardBlocks leaves
+// them as assembled): one thread per row, block column by block column
+// (RowUpdate).
 template <class Slots, class Lists>
 __global__ void __launch_bounds__(64) KForwardRest(Slots slots,
                                                    Lists plan) {
@@ -1895,8 +1923,8 @@ __global__ void __launch_bounds__(64) KForwardRest(Slots slots,
 }
 
 // L21^T x2 per column of the fronts at the level, one wavefront per column
-// (the same lane-strided sum and wave reduction as the fused backward
-// kernel), into tmp at the column's elimination index.
+// (lane-strided sum, wave reduction), into tmp at the column's elimination
+// index.
 template <class Slots, class Lists>
 __global__ void KBackwardRest(Slots slots, Lists plan) {
   BIND_SLOT(blockIdx.z);
@@ -2094,10 +2122,9 @@ __device__ void PublishBlock(int32_t* sync, int kb, int nbk, LmState* state) {
 // start from the right-hand side and the children's update vectors, take
 // the row update of every earlier block as soon as its y is ready (the
 // same partial sums as RowUpdate: thread p of a row sums the columns
-// k0 + p, k0 + p + 4, ...), then y_b = M_b v_b with the kept inverse (the
-// same four partial sums as KForward's diagonal block). Bit for bit
-// KForward's result; the rows below the own columns are left as assembled
-// (ticket 0 writes them) for KForwardRest.
+// k0 + p, k0 + p + 4, ...), then y_b = M_b v_b with the kept inverse (four
+// partial sums over t = p, p + 4, ...). The rows below the own columns are
+// left as assembled (ticket 0 writes them) for KForwardRest.
 template <class Slots, class Lists>
 __global__ void __launch_bounds__(kThreads) KForwardBlocks(Slots slots, Lists plan) {
   BIND_SLOT(blockIdx.z);
@@ -2580,13 +2607,11 @@ void LaunchShape::Merge(const LaunchShape& o) {
       a.rank_count.resize(b.rank_count.size(), 0);
       a.rank_max_m.resize(b.rank_max_m.size(), 0);
       a.rank_max_split.resize(b.rank_max_split.size(), 0);
-      a.rank_max_rest.resize(b.rank_max_rest.size(), 0);
     }
     for (size_t r = 0; r < b.rank_count.size(); ++r) {
       a.rank_count[r] = std::max(a.rank_count[r], b.rank_count[r]);
       a.rank_max_m[r] = std::max(a.rank_max_m[r], b.rank_max_m[r]);
       a.rank_max_split[r] = std::max(a.rank_max_split[r], b.rank_max_split[r]);
-      a.rank_max_rest[r] = std::max(a.rank_max_rest[r], b.rank_max_rest[r]);
     }
   }
 }
@@ -2747,23 +2772,17 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
         if (gt) (void)hipEventRecord(step_events->gemm_events[2 * step_events->gemm_used++ + 1], s);
       }
     }
-    for (int r = 0; r < ranks; ++r) {
-      const int cols = lv.rank_max_rest[static_cast<size_t>(r)];
-      if (lv.rank_count[static_cast<size_t>(r)] > 0 && cols > 0) {
-        KExtendAddUpdate<<<dim3((cols + kWaves - 1) / kWaves, lv.rank_count[static_cast<size_t>(r)], k), kThreads,
-                           0, s>>>(slots, RankLists<kOne>(sh, L, r));
-      }
-    }
   }
 }
 
-// Levels of at most this many fronts solve each front with one workgroup per
-// 64-column block (KForwardBlocks, KBackwardBlocks) and move the L21
-// products into launches of their own that spread them over many
-// workgroups.
-constexpr int kSplitSolveFronts = 16;
 
-// Forward solve of level L's fronts (after their children's).
+// Forward solve of level L's fronts (after their children's). Levels with
+// a front of more than one 64-column block: one workgroup per block of each
+// front for the own columns, then the rows below them spread over many
+// workgroups; other levels: one workgroup per front. For fronts of one
+// block both forms give the same values, and the choice follows the widest
+// front of the launch, so a front's result does not depend on the other
+// fronts or slots of its launch.
 template <bool kOne>
 void ForwardLevel(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, int L) {
   const auto slots = SlotsOf<kOne>(a);
@@ -2771,9 +2790,7 @@ void ForwardLevel(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, int L
   const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
   if (lv.count <= 0) return;
   const auto plan = LevelLists<kOne>(sh, L);
-  if (lv.count <= kSplitSolveFronts) {
-    // Few fronts: one workgroup per 64-column block of each, the rows
-    // below the own columns spread over many workgroups after.
+  if (lv.max_nb > kNB) {
     const int blocks = (lv.max_nb + kNB - 1) / kNB;
     KForwardBlocks<<<dim3(blocks, lv.count, k), kThreads, 0, s>>>(slots, plan);
     if (lv.max_m > 0) KForwardRest<<<dim3((lv.max_m + 63) / 64, lv.count, k), 64, 0, s>>>(slots, plan);
@@ -2784,7 +2801,11 @@ void ForwardLevel(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, int L
   KForward<<<dim3(lv.count, k), threads, static_cast<size_t>(lv.max_f) * sizeof(double), s>>>(slots, plan, 0);
 }
 
-// Backward solves, top level first.
+// Backward solves, top level first. Levels with a front of more than one
+// block: L21^T x2 over many workgroups, then one workgroup per 64-column
+// block of each front (its own summation order for the columns of the
+// front, which a front of one block does not have); other levels: one
+// workgroup per front.
 template <bool kOne>
 void BackwardImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s) {
   const auto slots = SlotsOf<kOne>(a);
@@ -2793,7 +2814,7 @@ void BackwardImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s) {
     const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
     if (lv.count <= 0) continue;
     const auto plan = LevelLists<kOne>(sh, L);
-    if (lv.count <= kSplitSolveFronts) {
+    if (lv.max_nb > kNB) {
       const int tri = lv.max_m > 0 ? 1 : 0;
       if (tri != 0) KBackwardRest<<<dim3((lv.max_nb + 3) / 4, lv.count, k), kThreads, 0, s>>>(slots, plan);
       const int blocks = (lv.max_nb + kNB - 1) / kNB;

@@ -133,6 +133,8 @@ struct DeviceAnalysis {
   const int64_t* rel_off;
   const int32_t* rel;
   const int32_t* rel_split;
+  const int64_t* uinv_off;  // per child: its map in uinv (-1: no entries in the parent's update matrix)
+  const int32_t* uinv;      // parent update row -> child update row, or -1
   const int64_t* nn_off;
   const int32_t* eik_at;
   const int64_t* fp_ptr;
@@ -198,7 +200,7 @@ struct LaunchShape {
     // Largest number of a rank's child columns landing in the parent's own
     // columns (the panel part of the extend-add) and beyond them (its update
     // matrix part).
-    std::vector<int> rank_max_split, rank_max_rest;
+    std::vector<int> rank_max_split;
     int rank_base{0};             // index of the level's first rank in rank_plan
   };
   std::vector<Level> levels;
