@@ -472,12 +472,11 @@ struct sdfslam_hip_context {
   Slot slot{};            // this context's problem as a launch slot
   Slot* slot_dev{nullptr};  // device copy (a table of one, in the epoch pool)
   LaunchShape shape;      // grid sizes of this context alone
-  // Level schedule: fronts per level and extend-add lists per (level, child
-  // rank), on the device with their host offsets, for launch plans.
+  // Level schedule: fronts and update-matrix tiles per level, on the device
+  // with their host offsets, for launch plans.
   const int32_t* level_sn_d{nullptr};
-  const int32_t* child_lists_d{nullptr};
   const int32_t* tiles_d{nullptr};  // update-matrix tiles, level by level (pairs)
-  std::vector<int64_t> level_off_h, rank_first_h, rank_off_h, tile_off_h;
+  std::vector<int64_t> level_off_h, tile_off_h;
 
   int max_f{0};
   hipEvent_t events[kPhases + 3]{};  // phase marks, then the point-row kernel pair
@@ -704,23 +703,15 @@ void SyncSlot(sdfslam_hip_context& c) {
 
 /// Launch plans of a launch over `ctxs` (slot b = ctxs[b]) for the merged
 /// shape `sh`: per level and slot the level's fronts and its update-matrix
-/// tiles, per (level, child rank) and slot the extend-add list; sets each
-/// level's rank_base. `all` receives the level, rank and tile plans in that
-/// order (the layout of their device copy, see SetPlans).
+/// tiles. `all` receives the level and tile plans in that order (the layout
+/// of their device copy, see SetPlans).
 void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape& sh, std::vector<ListRef>& all) {
   const size_t k = ctxs.size();
   const size_t levels = sh.levels.size();
   std::vector<ListRef>& level_plan = sh.level_plan_h;
-  std::vector<ListRef>& rank_plan = sh.rank_plan_h;
   std::vector<ListRef>& tile_plan = sh.tile_plan_h;
   level_plan.assign(levels * k, ListRef{nullptr, 0});
   tile_plan.assign(levels * k, ListRef{nullptr, 0});
-  int ranks = 0;
-  for (LaunchShape::Level& lv : sh.levels) {
-    lv.rank_base = ranks;
-    ranks += static_cast<int>(lv.rank_count.size());
-  }
-  rank_plan.assign(static_cast<size_t>(ranks) * k, ListRef{nullptr, 0});
   for (size_t b = 0; b < k; ++b) {
     const sdfslam_hip_context& c = *ctxs[b];
     const size_t own_levels = c.level_off_h.empty() ? 0 : c.level_off_h.size() - 1;
@@ -729,27 +720,16 @@ void BuildPlans(const std::vector<const sdfslam_hip_context*>& ctxs, LaunchShape
       level_plan[L * k + b] = {c.level_sn_d + l0, static_cast<int32_t>(c.level_off_h[L + 1] - l0)};
       const int64_t t0 = c.tile_off_h[L];
       tile_plan[L * k + b] = {c.tiles_d + 2 * t0, static_cast<int32_t>(c.tile_off_h[L + 1] - t0)};
-      const int64_t e0 = c.rank_first_h[L];
-      const int64_t e1 = c.rank_first_h[L + 1];
-      for (int64_t e = e0; e < e1; ++e) {
-        const size_t r = static_cast<size_t>(e - e0);
-        const size_t slot_index = (static_cast<size_t>(sh.levels[L].rank_base) + r) * k + b;
-        rank_plan[slot_index] = {c.child_lists_d + c.rank_off_h[static_cast<size_t>(e)],
-                                 static_cast<int32_t>(c.rank_off_h[static_cast<size_t>(e) + 1] -
-                                                      c.rank_off_h[static_cast<size_t>(e)])};
-      }
     }
   }
   all = level_plan;
-  all.insert(all.end(), rank_plan.begin(), rank_plan.end());
   all.insert(all.end(), tile_plan.begin(), tile_plan.end());
 }
 
 /// Points the shape's device plans into `dev`, a copy of BuildPlans' `all`.
 void SetPlans(LaunchShape& sh, const ListRef* dev) {
   sh.level_plan = dev;
-  sh.rank_plan = dev + sh.level_plan_h.size();
-  sh.tile_plan = sh.rank_plan + sh.rank_plan_h.size();
+  sh.tile_plan = dev + sh.level_plan_h.size();
 }
 
 /// Widens the footprint dilation of the poses whose points left their
@@ -873,13 +853,25 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
       uinv[static_cast<size_t>(uinv_off[static_cast<size_t>(ch)] + rel[a] - np)] = a;
     }
   }
-  // Level lists and, per level and child rank, the children to extend-add,
-  // with the launch shape of this context.
+  // Per child whose entries reach its parent's own columns: the map from the
+  // parent's own columns to the child's columns (rel below the split,
+  // inverted; -1 where no child column lands), for the panel extend-add.
+  std::vector<int64_t> pinv_off(static_cast<size_t>(an.num_sn), -1);
+  std::vector<int32_t> pinv;
+  for (int ch = 0; ch < an.num_sn; ++ch) {
+    const int par = an.sn_parent[static_cast<size_t>(ch)];
+    const int split = an.rel_split[static_cast<size_t>(ch)];
+    if (par < 0 || split <= 0) continue;
+    const int np = an.sn_first[static_cast<size_t>(par) + 1] - an.sn_first[static_cast<size_t>(par)];
+    pinv_off[static_cast<size_t>(ch)] = static_cast<int64_t>(pinv.size());
+    pinv.resize(pinv.size() + static_cast<size_t>(np), -1);
+    const int32_t* rel = an.rel.data() + an.rel_off[static_cast<size_t>(ch)];
+    for (int a = 0; a < split; ++a) pinv[static_cast<size_t>(pinv_off[static_cast<size_t>(ch)] + rel[a])] = a;
+  }
+  // Level lists and update-matrix tiles, with the launch shape of this
+  // context.
   std::vector<int32_t> lvl;
   std::vector<int64_t> level_off(static_cast<size_t>(an.num_levels) + 1, 0);
-  std::vector<int32_t> ranks;
-  std::vector<int64_t> rank_first(static_cast<size_t>(an.num_levels) + 1, 0);
-  std::vector<int64_t> rank_off{0};
   std::vector<int32_t> tiles;
   std::vector<int64_t> tile_off(static_cast<size_t>(an.num_levels) + 1, 0);
   LaunchShape& sh = c.shape;
@@ -892,6 +884,7 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     lv.max_nb = an.level_max_nb[static_cast<size_t>(L)];
     lv.max_f = an.level_max_f[static_cast<size_t>(L)];
     lv.max_m = an.level_max_m[static_cast<size_t>(L)];
+    lv.has_children = an.level_max_children[static_cast<size_t>(L)] > 0;
     for (const int s : an.level_sn[static_cast<size_t>(L)]) {
       const int t = (an.sn_m[static_cast<size_t>(s)] + 63) / 64;
       for (int ti = 0; ti < t; ++ti) {
@@ -903,38 +896,13 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     }
     tile_off[static_cast<size_t>(L) + 1] = static_cast<int64_t>(tiles.size() / 2);
     lv.tiles = static_cast<int>(tile_off[static_cast<size_t>(L) + 1] - tile_off[static_cast<size_t>(L)]);
-    for (int r = 0; r < an.level_max_children[static_cast<size_t>(L)]; ++r) {
-      int count = 0;
-      int max_m = 0;
-      int max_split = 0;
-      for (const int s : an.level_sn[static_cast<size_t>(L)]) {
-        const auto& ch = an.children[static_cast<size_t>(s)];
-        if (static_cast<int>(ch.size()) > r) {
-          const int child = ch[static_cast<size_t>(r)];
-          ranks.push_back(child);
-          ++count;
-          const int mc = an.sn_m[static_cast<size_t>(child)];
-          const int split = an.rel_split[static_cast<size_t>(child)];
-          max_m = std::max(max_m, mc);
-          max_split = std::max(max_split, split);
-        }
-      }
-      rank_off.push_back(static_cast<int64_t>(ranks.size()));
-      lv.rank_count.push_back(count);
-      lv.rank_max_m.push_back(max_m);
-      lv.rank_max_split.push_back(max_split);
-    }
-    rank_first[static_cast<size_t>(L) + 1] = static_cast<int64_t>(rank_off.size()) - 1;
   }
   sh.max_points_per_pose = std::max(1, an.max_points_per_pose);
   sh.panel_total = an.panel_total;
   sh.pairs = static_cast<int64_t>(an.op_pose.size() / 2);
   c.level_off_h = std::move(level_off);
-  c.rank_first_h = std::move(rank_first);
-  c.rank_off_h = std::move(rank_off);
   c.tile_off_h = std::move(tile_off);
-  size_t plan_entries = 2 * sh.levels.size();
-  for (const LaunchShape::Level& lv : sh.levels) plan_entries += lv.rank_count.size();
+  const size_t plan_entries = 2 * sh.levels.size();
   std::vector<ListRef> plans;
   EpochPool& E = c.epoch;
   DeviceAnalysis& D = c.da;
@@ -954,6 +922,8 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     D.rel_split = E.Upload(an.rel_split);
     D.uinv_off = E.Upload(uinv_off);
     D.uinv = E.Upload(uinv);
+    D.pinv_off = E.Upload(pinv_off);
+    D.pinv = E.Upload(pinv);
     D.nn_off = E.Upload(an.nn_off);
     D.eik_at = E.Upload(an.eik_at);
     D.fp_ptr = E.Upload(an.fp_ptr);
@@ -978,7 +948,6 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     D.children = E.Upload(clist);
     D.inv_off = E.Upload(inv_off);
     c.level_sn_d = E.Upload(lvl);
-    c.child_lists_d = E.Upload(ranks);
     c.tiles_d = E.Upload(tiles);
     c.dw.panel = E.Alloc<double>(static_cast<size_t>(an.panel_total));
     c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));

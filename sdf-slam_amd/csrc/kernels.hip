@@ -840,28 +840,36 @@ __device__ __forceinline__ void ScatterAddColumn(double* dst, int64_t col, int o
   }
 }
 
-// Extend-add of a child's update matrix, columns [0, split), into the
-// parent's panel (columns [split, m_c) are gathered by the parent's
-// update-matrix kernel). One wavefront per column, kWaves columns per
-// workgroup; within one launch no two children share a parent.
+// Extend-add into the panels of the level's fronts: the columns of their
+// children's update matrices that land in a front's own columns (each
+// child's columns [0, split); the rest are gathered by the front's
+// update-matrix kernel). One wavefront per own column of a front: it adds
+// the column of each child that lands there (at most one per child, found
+// through the child's map from the parent's own columns to its own), child
+// by child in ascending order, so no two wavefronts write the same entry.
 template <class Slots, class Lists>
 __global__ void KExtendAddPanel(Slots slots, Lists plan) {
   BIND_SLOT(blockIdx.z);
   if (Halted(W.state)) return;
-  const int c = PlanItem(plan, blockIdx.z, blockIdx.y);
-  if (c < 0) return;
-  const int b = static_cast<int>(blockIdx.x) * kWaves + static_cast<int>(threadIdx.x >> 6);
-  if (b >= A.rel_split[c]) return;
-  const int p = A.sn_parent[c];
-  const int mc = A.sn_m[c];
-  const double* uc = W.upd + A.upd_off[c];
-  const int32_t* rel = A.rel + A.rel_off[c];
+  const int p = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (p < 0) return;
+  const int j = static_cast<int>(blockIdx.x) * kWaves + static_cast<int>(threadIdx.x >> 6);
+  if (j >= A.sn_first[p + 1] - A.sn_first[p]) return;
   const int fp = A.sn_f[p];
   double* fpanel = W.panel + A.panel_off[p];
-  const int64_t col = static_cast<int64_t>(rel[b]) * fp;
-  ScatterAddColumn(fpanel, col, 0, rel, uc + static_cast<int64_t>(b) * mc, b, static_cast<int>(threadIdx.x & 63), mc);
+  const int64_t col = static_cast<int64_t>(j) * fp;
+  for (int64_t ci = A.children_ptr[p]; ci < A.children_ptr[p + 1]; ++ci) {
+    const int c = A.children[ci];
+    const int64_t poff = A.pinv_off[c];
+    if (poff < 0) continue;
+    const int b = A.pinv[poff + j];
+    if (b < 0) continue;
+    const int mc = A.sn_m[c];
+    ScatterAddColumn(fpanel, col, 0, A.rel + A.rel_off[c], W.upd + A.upd_off[c] + static_cast<int64_t>(b) * mc, b,
+                     static_cast<int>(threadIdx.x & 63), mc);
+    __threadfence_block();  // this child's stores before the next child's loads of the column
+  }
 }
-
 
 // Row stride (doubles) of the panel step's LDS tiles: two doubles of
 // padding put the four k rows an MFMA operand read touches in different
@@ -2603,16 +2611,7 @@ void LaunchShape::Merge(const LaunchShape& o) {
     a.max_f = std::max(a.max_f, b.max_f);
     a.max_m = std::max(a.max_m, b.max_m);
     a.tiles = std::max(a.tiles, b.tiles);
-    if (a.rank_count.size() < b.rank_count.size()) {
-      a.rank_count.resize(b.rank_count.size(), 0);
-      a.rank_max_m.resize(b.rank_max_m.size(), 0);
-      a.rank_max_split.resize(b.rank_max_split.size(), 0);
-    }
-    for (size_t r = 0; r < b.rank_count.size(); ++r) {
-      a.rank_count[r] = std::max(a.rank_count[r], b.rank_count[r]);
-      a.rank_max_m[r] = std::max(a.rank_max_m[r], b.rank_max_m[r]);
-      a.rank_max_split[r] = std::max(a.rank_max_split[r], b.rank_max_split[r]);
-    }
+    a.has_children = a.has_children || b.has_children;
   }
 }
 
@@ -2642,16 +2641,6 @@ auto TileLists(const LaunchShape& sh, int L) {
     return OneList{sh.tile_plan_h[static_cast<size_t>(L)]};
   } else {
     return ListTable{sh.tile_plan + static_cast<size_t>(L) * static_cast<size_t>(sh.slots)};
-  }
-}
-
-template <bool kOne>
-auto RankLists(const LaunchShape& sh, int L, int r) {
-  const size_t e = static_cast<size_t>(sh.levels[static_cast<size_t>(L)].rank_base + r);
-  if constexpr (kOne) {
-    return OneList{sh.rank_plan_h[e]};
-  } else {
-    return ListTable{sh.rank_plan + e * static_cast<size_t>(sh.slots)};
   }
 }
 
@@ -2711,15 +2700,11 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
   const unsigned k = static_cast<unsigned>(sh.slots);
   for (int L = 0; L < static_cast<int>(sh.levels.size()); ++L) {
     const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
-    const int ranks = static_cast<int>(lv.rank_count.size());
-    for (int r = 0; r < ranks; ++r) {
-      const int cols = lv.rank_max_split[static_cast<size_t>(r)];
-      if (lv.rank_count[static_cast<size_t>(r)] > 0 && cols > 0) {
-        KExtendAddPanel<<<dim3((cols + kWaves - 1) / kWaves, lv.rank_count[static_cast<size_t>(r)], k), kThreads, 0,
-                          s>>>(slots, RankLists<kOne>(sh, L, r));
-      }
-    }
     const int count = lv.count;
+    if (count > 0 && lv.has_children) {
+      KExtendAddPanel<<<dim3((lv.max_nb + kWaves - 1) / kWaves, count, k), kThreads, 0, s>>>(
+          slots, LevelLists<kOne>(sh, L));
+    }
     if (count > 0) {
       const auto plan = LevelLists<kOne>(sh, L);
       const int nblocks = (lv.max_nb + kNB - 1) / kNB;

@@ -135,6 +135,8 @@ struct DeviceAnalysis {
   const int32_t* rel_split;
   const int64_t* uinv_off;  // per child: its map in uinv (-1: no entries in the parent's update matrix)
   const int32_t* uinv;      // parent update row -> child update row, or -1
+  const int64_t* pinv_off;  // per child: its map in pinv (-1: no entries in the parent's own columns)
+  const int32_t* pinv;      // parent own column -> child column, or -1
   const int64_t* nn_off;
   const int32_t* eik_at;
   const int64_t* fp_ptr;
@@ -176,7 +178,7 @@ struct Slot {
 };
 
 /// The supernodes a level kernel works on in one slot: the fronts of a level,
-/// or one child rank's extend-add list (device pointer and length).
+/// or its update-matrix tiles (device pointer and length).
 struct ListRef {
   const int32_t* items;
   int32_t count;
@@ -195,25 +197,16 @@ struct LaunchShape {
   struct Level {
     int count{0}, max_nb{0}, max_f{0}, max_m{0};
     int tiles{0};  // 64 x 64 lower tiles of the fronts' update matrices
-    std::vector<int> rank_count;  // largest child-list length per child rank
-    std::vector<int> rank_max_m;  // largest child update size per child rank
-    // Largest number of a rank's child columns landing in the parent's own
-    // columns (the panel part of the extend-add) and beyond them (its update
-    // matrix part).
-    std::vector<int> rank_max_split;
-    int rank_base{0};             // index of the level's first rank in rank_plan
+    bool has_children{false};
   };
   std::vector<Level> levels;
   // Launch plans in device memory: level_plan[L * slots + b] = the fronts of
-  // level L in slot b; rank_plan[(rank_base + r) * slots + b] = the
-  // children of rank r at that level (count 0 where a slot has none);
-  // tile_plan[L * slots + b] = the update-matrix tiles of level L in slot b
+  // level L in slot b (count 0 where a slot has none); tile_plan[L * slots + b] = the update-matrix tiles of level L in slot b
   // (count pairs (front, ti << 16 | tj) at items).
   const ListRef* level_plan{nullptr};
-  const ListRef* rank_plan{nullptr};
   const ListRef* tile_plan{nullptr};
   // Host copies of the plans, for launches over a single slot.
-  std::vector<ListRef> level_plan_h, rank_plan_h, tile_plan_h;
+  std::vector<ListRef> level_plan_h, tile_plan_h;
   /// Grows this shape to cover `o` as well.
   void Merge(const LaunchShape& o);
 };
