@@ -186,6 +186,55 @@ int sdfslam_hip_context_update_device_normals(sdfslam_hip_context* ctx, const sd
  * the factorization's sums) can differ from a fresh context's. */
 int sdfslam_hip_context_update(sdfslam_hip_context* ctx, const sdfslam_problem_view* problem,
                                const double* map_values, const double* poses);
+/* Extends a context's problem by the next frames of an incremental run, as
+ * the reference's runner rebuilds the Problem over all frames so far before
+ * each Solve (runner.cpp:245-258 -> Problem::Problem,
+ * src/problem/problem.cpp:11-115), with the per-frame construction on the
+ * device and nothing of the earlier frames rebuilt or uploaded again:
+ *  - each frame's point rows (its scan points, then their hallucinated
+ *    samples, src/core/hallucination.cpp:5-28) are generated on the device
+ *    from the raw scan and appended to the rows the context holds;
+ *  - one odometry spec per frame joins it to the frame before;
+ *  - every eikonal spec's normal is recomputed on the device from all scan
+ *    points at the given poses (src/core/normals.cpp:128-220, as in
+ *    sdfslam_hip_context_update_device_normals);
+ *  - with the active region the active node set is recomputed on the device
+ *    from every point row at the given poses (problem.cpp:39-70) and read
+ *    back (the one device round trip of this call; without it the call
+ *    does host work only and defers its device work to the context's next
+ *    call, as an update does), and the eikonal specs follow it.
+ * The resulting problem equals the one Problem::Problem builds from all
+ * frames so far (the tests solve both and compare the results bit for bit).
+ * The context's problem must be such a session problem: consecutive
+ * odometry only (no relations), eikonal specs from the pca or hybrid
+ * normals, the same options as below. scan_xy holds the frames' sensor-frame
+ * points (x, y) one frame after another, points_per_frame their counts,
+ * odometry 3 values (dx, dy, dtheta) per frame; map_values and poses are
+ * the state the problem is built at (num_poses + num_frames poses: the
+ * earlier poses, then the new frames' initial poses). No counterpart in the
+ * reference (it rebuilds the whole problem on the host). */
+typedef struct sdfslam_frame_options {
+  int32_t hallucination_points;  /* per scan point (HallucinationOptions) */
+  double hallucination_step;     /* > 0 */
+  int32_t hallucination_both_directions;
+  double weight_scan;  /* Weights: sqrt taken as Problem::Problem does */
+  double weight_hallucination;
+  double weight_eikonal;
+  double weight_odometry;
+  int32_t active_region;
+  int32_t active_margin;
+  sdfslam_normals_request normals; /* frame_cos_sin must be NULL */
+} sdfslam_frame_options;
+int sdfslam_hip_context_append_frames(sdfslam_hip_context* ctx, int32_t num_frames,
+                                      const int64_t* points_per_frame, const double* scan_xy,
+                                      const double* odometry, const sdfslam_frame_options* options,
+                                      const double* map_values, const double* poses);
+/* The point rows the context holds, as the device holds them: the row
+ * count into *num_points (may be NULL) and the first min(capacity, count)
+ * rows into the arrays (any may be NULL). For checking appended rows. */
+int sdfslam_hip_context_point_rows(sdfslam_hip_context* ctx, int64_t capacity, int64_t* num_points,
+                                   int32_t* frame, double* x, double* y, double* expected_sdf,
+                                   double* sqrt_weight);
 /* Footprint margin (cells) of every pose in the current pattern epoch:
  * min(capacity, num_poses) entries into margins (may be NULL), the pose
  * count into *num_poses (may be NULL). */

@@ -143,6 +143,25 @@ class EpochPool {
   T* Upload(const std::vector<T>& v) {
     return Upload(v.data(), v.size());
   }
+  /// Room for `cap` elements of which the first `n` are uploaded.
+  template <typename T>
+  T* UploadPrefix(const T* data, size_t n, size_t cap) {
+    const size_t bytes = std::max<size_t>(cap, 1) * sizeof(T);
+    const size_t at = Take(bytes);
+    if (measuring_) {
+      upload_end_ = std::max(upload_end_, at + std::max<size_t>(n, 1) * sizeof(T));
+      return nullptr;
+    }
+    if (n > 0) {
+      std::memcpy(static_cast<char*>(host_) + at, data, n * sizeof(T));
+      if (!runs_.empty() && runs_.back().second == at) {
+        runs_.back().second = at + n * sizeof(T);
+      } else {
+        runs_.push_back({at, at + n * sizeof(T)});
+      }
+    }
+    return reinterpret_cast<T*>(static_cast<char*>(dev_) + at);
+  }
   template <typename T>
   T* Alloc(size_t n) {
     const size_t at = Take(std::max<size_t>(n, 1) * sizeof(T));
@@ -249,6 +268,13 @@ struct HostProblem {
       }
     }
     view = v;
+    Refresh();
+  }
+  /// Points the view at the arrays and sets its counts (after appending).
+  void Refresh() {
+    view.num_points = static_cast<int64_t>(pt_frame.size());
+    view.num_eikonal = static_cast<int64_t>(eik_w.size());
+    view.num_odom = static_cast<int64_t>(od_i.size());
     view.node_to_active = node_to_active.data();
     view.point_frame = pt_frame.data();
     view.point_x = pt_x.data();
@@ -456,7 +482,18 @@ struct sdfslam_hip_context {
   double analysis_seconds{0.0};  // host time spent building pattern epochs
   EpochPool base;   // problem specs, state and per-row work arrays
   EpochPool epoch;  // symbolic analysis and factor storage
-  EpochPool normals;  // device eikonal normals: scan rows, cos/sin, work
+  EpochPool normals;  // device eikonal normals: cos/sin, work
+  // Point rows (the point specs) with room for appended frames, and the
+  // list of rows that are scan points (for the device normals); rows_cap
+  // rows and list entries fit before the layout has to grow.
+  EpochPool rows;
+  int64_t rows_cap{0}, list_cap{0};
+  int32_t* pt_frame_d{nullptr};
+  double *pt_x_d{nullptr}, *pt_y_d{nullptr}, *pt_delta_d{nullptr}, *pt_sw_d{nullptr};
+  int32_t* scan_list_d{nullptr};
+  std::vector<int32_t> scan_list;  // host copy: rows with expected SDF 0 (appended frames: their scan rows)
+  // Staging of appended frames' raw scans (pinned) and device copies.
+  EpochPool frames;
   DeviceArena hist;  // cost history
   DeviceProblem dp{};
   DeviceAnalysis da{};
@@ -519,6 +556,8 @@ struct sdfslam_hip_context {
     epoch.Release();
     base.Release();
     normals.Release();
+    rows.Release();
+    frames.Release();
     hist.Clear();
   }
 };
@@ -558,6 +597,56 @@ void Bind(sdfslam_hip_context& c) {
   for (auto& w : work) w(c.stream);
 }
 
+/// How far each pose's points reach from it (metres), over rows [from, end)
+/// (from > 0 extends the current values).
+void ComputeReach(sdfslam_hip_context& c, size_t from) {
+  const HostProblem& hp = *c.hp;
+  if (from == 0) c.pose_reach.clear();
+  c.pose_reach.resize(static_cast<size_t>(hp.view.num_poses), 0.0);
+  for (size_t r = from; r < hp.pt_frame.size(); ++r) {
+    double& reach = c.pose_reach[static_cast<size_t>(hp.pt_frame[r])];
+    reach = std::max(reach, std::hypot(hp.pt_x[r], hp.pt_y[r]));
+  }
+}
+
+/// Lays out the row pool for the problem's rows with room to append (half
+/// again as many), uploading rows [0, upload_rows) and scan-list entries
+/// [0, upload_list) from the host copy; the rest is written on the device.
+void PlaceRows(sdfslam_hip_context& c, int64_t upload_rows, int64_t upload_list) {
+  const HostProblem& hp = *c.hp;
+  const auto np = static_cast<int64_t>(hp.pt_frame.size());
+  const auto nl = static_cast<int64_t>(c.scan_list.size());
+  c.rows_cap = np + np / 2 + 1024;
+  c.list_cap = nl + nl / 2 + 1024;
+  EpochPool& R = c.rows;
+  const auto up = static_cast<size_t>(upload_rows);
+  const auto cap = static_cast<size_t>(c.rows_cap);
+  auto layout = [&]() {
+    c.pt_frame_d = R.UploadPrefix(hp.pt_frame.data(), up, cap);
+    c.pt_x_d = R.UploadPrefix(hp.pt_x.data(), up, cap);
+    c.pt_y_d = R.UploadPrefix(hp.pt_y.data(), up, cap);
+    c.pt_delta_d = R.UploadPrefix(hp.pt_delta.data(), up, cap);
+    c.pt_sw_d = R.UploadPrefix(hp.pt_sw.data(), up, cap);
+    c.scan_list_d = R.UploadPrefix(c.scan_list.data(), static_cast<size_t>(upload_list),
+                                   static_cast<size_t>(c.list_cap));
+  };
+  R.BeginMeasure();
+  layout();
+  R.BeginPlace();
+  layout();
+  Enqueue(c, [&R](hipStream_t s) { R.Copy(s); });
+}
+
+/// The problem's scan rows for the device normals: the rows with expected
+/// SDF 0, in row order.
+void ComputeScanList(sdfslam_hip_context& c) {
+  c.scan_list.clear();
+  const HostProblem& hp = *c.hp;
+  for (size_t r = 0; r < hp.pt_delta.size(); ++r) {
+    if (hp.pt_delta[r] == 0.0) c.scan_list.push_back(static_cast<int32_t>(r));
+  }
+}
+
 void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double* poses) {
   const sdfslam_problem_view& v = c.hp->view;
   HostProblem& hp = *c.hp;
@@ -577,11 +666,6 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   P.huber_delta = v.huber_delta;
   P.smooth_gradient = v.smooth_gradient;
   P.smooth_gradient_step = v.smooth_gradient_step;
-  c.pose_reach.assign(static_cast<size_t>(v.num_poses), 0.0);
-  for (size_t r = 0; r < hp.pt_frame.size(); ++r) {
-    double& reach = c.pose_reach[static_cast<size_t>(hp.pt_frame[r])];
-    reach = std::max(reach, std::hypot(hp.pt_x[r], hp.pt_y[r]));
-  }
   EpochPool& A = c.base;
   DeviceWork& W = c.dw;
   const size_t nmap = static_cast<size_t>(v.grid.nx) * v.grid.ny;
@@ -590,11 +674,6 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   auto layout = [&]() {
     P.node_to_active = A.Upload(hp.node_to_active);
     P.active_to_node = A.Upload(active_to_node);
-    P.pt_frame = A.Upload(hp.pt_frame);
-    P.pt_x = A.Upload(hp.pt_x);
-    P.pt_y = A.Upload(hp.pt_y);
-    P.pt_delta = A.Upload(hp.pt_delta);
-    P.pt_sw = A.Upload(hp.pt_sw);
     P.eik_w = A.Upload(hp.eik_w);
     P.eik_h = A.Upload(hp.eik_h);
     P.eik_nx = A.Upload(hp.eik_nx);
@@ -638,6 +717,11 @@ void AllocateBase(sdfslam_hip_context& c, const double* map_values, const double
   layout();
   A.BeginPlace();
   layout();
+  P.pt_frame = c.pt_frame_d;
+  P.pt_x = c.pt_x_d;
+  P.pt_y = c.pt_y_d;
+  P.pt_delta = c.pt_delta_d;
+  P.pt_sw = c.pt_sw_d;
   int32_t* overflow = W.pose_overflow;
   LmState* state = W.state;
   const size_t poses_n = static_cast<size_t>(v.num_poses);
@@ -1470,6 +1554,9 @@ void CreateContext(const sdfslam_problem_view* problem, const sdfslam_solver_opt
   HIPC(hipGetDevice(&c->device));
   Bind(*c);
   for (hipEvent_t& e : c->events) HIPC(hipEventCreate(&e));
+  ComputeScanList(*c);
+  ComputeReach(*c, 0);
+  PlaceRows(*c, static_cast<int64_t>(c->hp->pt_frame.size()), static_cast<int64_t>(c->scan_list.size()));
   AllocateBase(*c, map_values, poses);
   BuildEpoch(*c, map_values, poses);
   InitLm(*c, options->lambda_init);
@@ -1490,11 +1577,7 @@ void StageDeviceNormals(sdfslam_hip_context& c, const sdfslam_normals_request& n
   if (nr.method != SDFSLAM_NORMALS_PCA && nr.method != SDFSLAM_NORMALS_HYBRID) {
     throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "device eikonal normals need the pca or hybrid method");
   }
-  std::vector<int32_t> rows;
-  for (size_t r = 0; r < hp.pt_delta.size(); ++r) {
-    if (hp.pt_delta[r] == 0.0) rows.push_back(static_cast<int32_t>(r));
-  }
-  if (rows.empty()) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "no scan points for the eikonal normals");
+  if (c.scan_list.empty()) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "no scan points for the eikonal normals");
   std::vector<double> cs(2 * static_cast<size_t>(v.num_poses));
   if (nr.frame_cos_sin != nullptr) {
     std::copy(nr.frame_cos_sin, nr.frame_cos_sin + cs.size(), cs.begin());
@@ -1503,13 +1586,11 @@ void StageDeviceNormals(sdfslam_hip_context& c, const sdfslam_normals_request& n
     for (int f = 0; f < v.num_poses; ++f) ::sincos(poses[3 * f + 2], &cs[2 * f + 1], &cs[2 * f]);
   }
   const int64_t nodes = static_cast<int64_t>(v.grid.nx) * v.grid.ny;
-  const size_t work_bytes = EikonalNormalsWorkBytes(static_cast<int64_t>(rows.size()), nodes);
+  const size_t work_bytes = EikonalNormalsWorkBytes(static_cast<int64_t>(c.scan_list.size()), nodes);
   EpochPool& E = c.normals;
-  int32_t* d_rows = nullptr;
   double* d_cs = nullptr;
   unsigned char* d_work = nullptr;
   auto layout = [&]() {
-    d_rows = E.Upload(rows);
     d_cs = E.Upload(cs);
     d_work = E.Alloc<unsigned char>(work_bytes);
   };
@@ -1519,8 +1600,8 @@ void StageDeviceNormals(sdfslam_hip_context& c, const sdfslam_normals_request& n
   layout();
   const DeviceProblem& P = c.dp;
   EikonalNormalsJob job{};
-  job.scan_rows = d_rows;
-  job.scan_points = static_cast<int64_t>(rows.size());
+  job.scan_rows = c.scan_list_d;
+  job.scan_points = static_cast<int64_t>(c.scan_list.size());
   job.frame = P.pt_frame;
   job.sensor_x = P.pt_x;
   job.sensor_y = P.pt_y;
@@ -1583,8 +1664,256 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
     explicit Deferring(sdfslam_hip_context& ctx) : c(ctx) { c.deferring = true; }
     ~Deferring() { c.deferring = false; }
   } deferring(c);
+  ComputeScanList(c);
+  ComputeReach(c, 0);
+  PlaceRows(c, static_cast<int64_t>(c.hp->pt_frame.size()), static_cast<int64_t>(c.scan_list.size()));
   AllocateBase(c, map_values, poses);
   if (normals != nullptr) StageDeviceNormals(c, *normals, poses);
+  BuildEpoch(c, map_values, poses);
+  const double lambda = c.opts.lambda_init;
+  Enqueue(c, [&c, lambda](hipStream_t) { InitLm(c, lambda); });
+  c.lm_ready = true;
+}
+
+/// Extends the context's problem by `frames` frames built on the device
+/// (sdfslam_hip_context_append_frames): the host copy gets the same rows (the
+/// symbolic analysis reads them), the device generates its own from the raw
+/// scans; the earlier rows stay where they are unless the row pool grows.
+void AppendFrames(sdfslam_hip_context& c, int32_t frames, const int64_t* counts, const double* scan_xy,
+                  const double* odometry, const sdfslam_frame_options& o, const double* map_values,
+                  const double* poses) {
+  if (frames < 1 || counts == nullptr || odometry == nullptr || map_values == nullptr || poses == nullptr) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "append needs frames, counts, odometry and the state");
+  }
+  if (o.normals.frame_cos_sin != nullptr) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "appended frames take cos and sin from ::sincos (frame_cos_sin NULL)");
+  }
+  if (o.hallucination_points > 0 && !(o.hallucination_step > 0.0)) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "appended frames need a positive hallucination step");
+  }
+  if (o.active_region != 0 && o.active_margin < 0) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "negative active margin");
+  }
+  HostProblem& hp = *c.hp;
+  const int old_poses = hp.view.num_poses;
+  for (int k = 0; k < static_cast<int>(hp.od_i.size()); ++k) {
+    if (hp.od_i[static_cast<size_t>(k)] != k || hp.od_j[static_cast<size_t>(k)] != k + 1) {
+      throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "appending frames needs consecutive odometry only (no relations)");
+    }
+  }
+  if (static_cast<int>(hp.od_i.size()) != old_poses - 1) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "appending frames needs one odometry spec per earlier frame");
+  }
+  int64_t total_points = 0;
+  for (int f = 0; f < frames; ++f) {
+    if (counts[f] < 0) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "negative point count");
+    total_points += counts[f];
+  }
+  if (total_points > 0 && scan_xy == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null scan points");
+  if (o.active_region == 0 && static_cast<int64_t>(hp.view.num_active_nodes) !=
+                                   static_cast<int64_t>(hp.view.grid.nx) * hp.view.grid.ny) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "a problem with an active region needs active_region set");
+  }
+  HIPC(hipSetDevice(c.device));
+  DropSpeculation(c);
+  // Deferred work of an earlier update that no call has run yet is
+  // superseded; the rows it would have written are uploaded again.
+  const bool rows_on_device = c.pending.empty();
+  c.pending.clear();
+  const auto old_rows = static_cast<int64_t>(hp.pt_frame.size());
+  const auto old_list = static_cast<int64_t>(c.scan_list.size());
+  // Host copy of the new rows (Problem::Problem, problem.cpp:26-37, with
+  // GenerateHallucinatedPoints, hallucination.cpp:5-28), the same
+  // arithmetic as the device's KFrameRows.
+  const double sw_scan = std::sqrt(o.weight_scan);
+  const double sw_hall = std::sqrt(o.weight_hallucination);
+  const int per_side = o.hallucination_points > 0
+                           ? o.hallucination_points / (o.hallucination_both_directions != 0 ? 2 : 1)
+                           : 0;
+  std::vector<int64_t> scan_off(static_cast<size_t>(frames) + 1, 0);
+  std::vector<int64_t> row_off(static_cast<size_t>(frames));
+  std::vector<int64_t> list_off(static_cast<size_t>(frames));
+  for (int f = 0; f < frames; ++f) {
+    const int frame = old_poses + f;
+    row_off[static_cast<size_t>(f)] = static_cast<int64_t>(hp.pt_frame.size());
+    list_off[static_cast<size_t>(f)] = static_cast<int64_t>(c.scan_list.size());
+    const double* xy = scan_xy + 2 * scan_off[static_cast<size_t>(f)];
+    const int64_t n = counts[f];
+    for (int64_t i = 0; i < n; ++i) {
+      c.scan_list.push_back(static_cast<int32_t>(hp.pt_frame.size()));
+      hp.pt_frame.push_back(frame);
+      hp.pt_x.push_back(xy[2 * i]);
+      hp.pt_y.push_back(xy[2 * i + 1]);
+      hp.pt_delta.push_back(0.0);
+      hp.pt_sw.push_back(sw_scan);
+    }
+    if (per_side > 0) {
+      for (int64_t i = 0; i < n; ++i) {
+        const double px = xy[2 * i];
+        const double py = xy[2 * i + 1];
+        const double norm = std::sqrt(px * px + py * py);
+        if (norm < 1e-9) continue;
+        const double tx = -px / norm * o.hallucination_step;
+        const double ty = -py / norm * o.hallucination_step;
+        for (int j = 1; j <= per_side; ++j) {
+          const double dj = static_cast<double>(j);
+          hp.pt_frame.push_back(frame);
+          hp.pt_x.push_back(px + tx * dj);
+          hp.pt_y.push_back(py + ty * dj);
+          hp.pt_delta.push_back(o.hallucination_step * dj);
+          hp.pt_sw.push_back(sw_hall);
+          if (o.hallucination_both_directions != 0) {
+            hp.pt_frame.push_back(frame);
+            hp.pt_x.push_back(px - tx * dj);
+            hp.pt_y.push_back(py - ty * dj);
+            hp.pt_delta.push_back(-o.hallucination_step * dj);
+            hp.pt_sw.push_back(sw_hall);
+          }
+        }
+      }
+    }
+    scan_off[static_cast<size_t>(f) + 1] = scan_off[static_cast<size_t>(f)] + n;
+  }
+  if (static_cast<int64_t>(hp.pt_frame.size()) > (int64_t{1} << 31) - 1) {
+    throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "too many point rows");
+  }
+  // Odometry (problem.cpp:96-100): frame k-1 -> k for every new frame.
+  const double sw_odo = std::sqrt(o.weight_odometry);
+  for (int f = 0; f < frames; ++f) {
+    const int j = old_poses + f;
+    hp.od_i.push_back(j - 1);
+    hp.od_j.push_back(j);
+    hp.od_dx.push_back(odometry[3 * f]);
+    hp.od_dy.push_back(odometry[3 * f + 1]);
+    hp.od_dth.push_back(odometry[3 * f + 2]);
+    hp.od_sw.push_back(sw_odo);
+  }
+  const int num_poses = old_poses + frames;
+  const sdfslam_grid g = hp.view.grid;
+  const int64_t nodes = static_cast<int64_t>(g.nx) * g.ny;
+  std::vector<double> cs(2 * static_cast<size_t>(num_poses));
+  for (int f = 0; f < num_poses; ++f) ::sincos(poses[3 * f + 2], &cs[2 * f + 1], &cs[2 * f]);
+  // Row pool: grow (uploading the earlier rows) or keep; the device writes
+  // the new rows either way.
+  const bool grow = static_cast<int64_t>(hp.pt_frame.size()) > c.rows_cap ||
+                    static_cast<int64_t>(c.scan_list.size()) > c.list_cap;
+  if (grow || !rows_on_device) {
+    struct Deferring {
+      sdfslam_hip_context& c;
+      explicit Deferring(sdfslam_hip_context& ctx) : c(ctx) { c.deferring = true; }
+      ~Deferring() { c.deferring = false; }
+    } deferring(c);
+    PlaceRows(c, old_rows, old_list);
+  }
+  // Raw scans and their offsets, staged for the frame-row kernel.
+  EpochPool& F = c.frames;
+  double* d_xy = nullptr;
+  int64_t* d_scan_off = nullptr;
+  int64_t* d_row_off = nullptr;
+  int64_t* d_list_off = nullptr;
+  double* d_cs = nullptr;
+  double* d_poses = nullptr;
+  uint8_t* d_marks = nullptr;
+  auto layout = [&]() {
+    d_xy = F.Upload(scan_xy, 2 * static_cast<size_t>(total_points));
+    d_scan_off = F.Upload(scan_off);
+    d_row_off = F.Upload(row_off);
+    d_list_off = F.Upload(list_off);
+    if (o.active_region != 0) {
+      d_cs = F.Upload(cs);
+      d_poses = F.Upload(poses, 3 * static_cast<size_t>(num_poses));
+      d_marks = F.Alloc<uint8_t>(static_cast<size_t>(nodes));
+    }
+  };
+  F.BeginMeasure();
+  layout();
+  F.BeginPlace();
+  layout();
+  FrameRowsJob job{};
+  job.frames = frames;
+  job.first_frame = old_poses;
+  job.scan_xy = d_xy;
+  job.scan_off = d_scan_off;
+  job.row_off = d_row_off;
+  job.list_off = d_list_off;
+  job.per_side = per_side;
+  job.both_directions = o.hallucination_both_directions;
+  job.step = o.hallucination_step;
+  job.sqrt_w_scan = sw_scan;
+  job.sqrt_w_hallucination = sw_hall;
+  job.frame = c.pt_frame_d;
+  job.x = c.pt_x_d;
+  job.y = c.pt_y_d;
+  job.expected_sdf = c.pt_delta_d;
+  job.sqrt_weight = c.pt_sw_d;
+  job.scan_list = c.scan_list_d;
+  auto rows_work = [&F, job](hipStream_t s) {
+    F.Copy(s);
+    const int st = EnqueueFrameRows(job, s);
+    if (st != SDFSLAM_OK) throw Failure(st, NormalsLastError());
+  };
+  hp.view.num_poses = num_poses;
+  if (o.active_region != 0) {
+    // The active set at the given poses from every row on the device, read
+    // back: the node columns and the eikonal specs follow it.
+    Bind(c);
+    rows_work(c.stream);
+    const int64_t np = static_cast<int64_t>(hp.pt_frame.size());
+    const int st = EnqueueActiveMarks(c.pt_frame_d, c.pt_x_d, c.pt_y_d, np, d_cs, d_poses, g, o.active_margin,
+                                      d_marks, c.stream);
+    if (st != SDFSLAM_OK) throw Failure(st, NormalsLastError());
+    std::vector<uint8_t> marks(static_cast<size_t>(nodes));
+    HIPC(hipMemcpyAsync(marks.data(), d_marks, marks.size(), hipMemcpyDeviceToHost, c.stream));
+    HIPC(hipStreamSynchronize(c.stream));
+    int active = 0;
+    for (int64_t v = 0; v < nodes; ++v) {
+      hp.node_to_active[static_cast<size_t>(v)] = marks[static_cast<size_t>(v)] != 0 ? active++ : -1;
+    }
+    hp.view.num_active_nodes = active;
+  } else {
+    // The dense problem keeps every node active; the device writes its rows
+    // at the start of the next call.
+    c.pending.push_back(rows_work);
+  }
+  // Eikonal specs (problem.cpp:72-94): every cell corner whose node and its
+  // right and upper neighbours are active, h-major, with weight
+  // sqrt(w_eikonal) (the pca and hybrid methods' eikonal scale is 1); their
+  // normals come from the device.
+  const double sw_eik = std::sqrt(o.weight_eikonal);
+  hp.eik_w.clear();
+  hp.eik_h.clear();
+  hp.eik_sw.clear();
+  for (int h = 0; h + 1 < g.ny; ++h) {
+    for (int w = 0; w + 1 < g.nx; ++w) {
+      const int64_t node = static_cast<int64_t>(h) * g.nx + w;
+      if (hp.node_to_active[static_cast<size_t>(node)] < 0 || hp.node_to_active[static_cast<size_t>(node) + 1] < 0 ||
+          hp.node_to_active[static_cast<size_t>(node + g.nx)] < 0) {
+        continue;
+      }
+      hp.eik_w.push_back(w);
+      hp.eik_h.push_back(h);
+      hp.eik_sw.push_back(sw_eik);
+    }
+  }
+  hp.eik_nx.assign(hp.eik_w.size(), 0.0);
+  hp.eik_ny.assign(hp.eik_w.size(), 0.0);
+  hp.Refresh();
+  // Margins: the earlier poses keep theirs, the new ones start wider.
+  std::vector<int32_t> margins = c.aopt.pose_margin;
+  if (static_cast<int>(margins.size()) != old_poses) margins.assign(static_cast<size_t>(old_poses), c.aopt.margin);
+  margins.resize(static_cast<size_t>(num_poses), num_poses <= kEarlyFrames ? kEarlyMargin : kFreshPoseMargin);
+  c.aopt.margin = AnalysisOptions{}.margin;
+  c.aopt.pose_margin = std::move(margins);
+  ComputeReach(c, static_cast<size_t>(old_rows));
+  struct Deferring {
+    sdfslam_hip_context& c;
+    explicit Deferring(sdfslam_hip_context& ctx) : c(ctx) { c.deferring = true; }
+    ~Deferring() { c.deferring = false; }
+  } deferring(c);
+  AllocateBase(c, map_values, poses);
+  sdfslam_normals_request nr = o.normals;
+  nr.frame_cos_sin = cs.data();
+  StageDeviceNormals(c, nr, poses);
   BuildEpoch(c, map_values, poses);
   const double lambda = c.opts.lambda_init;
   Enqueue(c, [&c, lambda](hipStream_t) { InitLm(c, lambda); });
@@ -1687,6 +2016,42 @@ int sdfslam_hip_context_update_device_normals(sdfslam_hip_context* ctx, const sd
   });
 }
 
+int sdfslam_hip_context_append_frames(sdfslam_hip_context* ctx, int32_t num_frames,
+                                      const int64_t* points_per_frame, const double* scan_xy,
+                                      const double* odometry, const sdfslam_frame_options* options,
+                                      const double* map_values, const double* poses) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || options == nullptr) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "null argument");
+    AppendFrames(*ctx, num_frames, points_per_frame, scan_xy, odometry, *options, map_values, poses);
+    return SDFSLAM_OK;
+  });
+}
+
+int sdfslam_hip_context_point_rows(sdfslam_hip_context* ctx, int64_t capacity, int64_t* num_points,
+                                   int32_t* frame, double* x, double* y, double* expected_sdf,
+                                   double* sqrt_weight) {
+  return Guard([&]() -> int {
+    if (ctx == nullptr || capacity < 0) throw Failure(SDFSLAM_ERR_INVALID_ARGUMENT, "bad argument");
+    sdfslam_hip_context& c = *ctx;
+    Bind(c);
+    if (num_points != nullptr) *num_points = c.hp->view.num_points;
+    const auto n = static_cast<size_t>(std::min<int64_t>(capacity, c.hp->view.num_points));
+    if (frame != nullptr) {
+      HIPC(hipMemcpyAsync(frame, c.pt_frame_d, n * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    }
+    if (x != nullptr) HIPC(hipMemcpyAsync(x, c.pt_x_d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (y != nullptr) HIPC(hipMemcpyAsync(y, c.pt_y_d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    if (expected_sdf != nullptr) {
+      HIPC(hipMemcpyAsync(expected_sdf, c.pt_delta_d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    }
+    if (sqrt_weight != nullptr) {
+      HIPC(hipMemcpyAsync(sqrt_weight, c.pt_sw_d, n * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    }
+    HIPC(hipStreamSynchronize(c.stream));
+    return SDFSLAM_OK;
+  });
+}
+
 int sdfslam_hip_pose_margins(sdfslam_hip_context* ctx, int32_t* margins, int32_t capacity,
                              int32_t* num_poses) {
   return Guard([&]() -> int {
@@ -1711,6 +2076,8 @@ int sdfslam_hip_context_trim(sdfslam_hip_context* ctx) {
     ctx->epoch.Trim();
     ctx->base.Trim();
     ctx->normals.Trim();
+    ctx->rows.Trim();
+    ctx->frames.Trim();
     ctx->batch.Trim();
     return SDFSLAM_OK;
   });
@@ -1769,7 +2136,8 @@ int sdfslam_hip_analysis(sdfslam_hip_context* ctx, sdfslam_hip_analysis_info* in
     info->pattern_epochs = ctx->pattern_epochs;
     info->analysis_seconds = ctx->analysis_seconds;
     info->device_bytes =
-        static_cast<int64_t>(ctx->base.bytes() + ctx->epoch.bytes() + ctx->normals.bytes() + ctx->hist.bytes());
+        static_cast<int64_t>(ctx->base.bytes() + ctx->epoch.bytes() + ctx->normals.bytes() + ctx->rows.bytes() +
+                             ctx->frames.bytes() + ctx->hist.bytes());
     info->panel_step_launches = a.panel_step_launches;
     info->panel_step_flops = a.panel_step_flops;
     info->speculative_epochs = ctx->speculative_epochs;

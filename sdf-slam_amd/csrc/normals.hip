@@ -155,9 +155,160 @@ __device__ int Knn(double qx, double qy, int k, const double* __restrict__ px, c
   return found;
 }
 
-// kNN PCA normal of every point (PcaNormal, normals.cpp:70-103): the
-// eigenvector of the smaller covariance eigenvalue (closed form), oriented
-// toward the point's origin, and the eigenvalue ratio.
+// The k best of a search held in registers: K is the compile-time k, so
+// every index into the arrays is a constant and nothing goes to scratch
+// memory. Entries are kept sorted by (squared distance, point index), the
+// order an exact search with ties broken by index returns; unused entries
+// hold +inf.
+template <int K>
+struct TopK {
+  double d[K];
+  int id[K];
+  int found;
+  __device__ void Init() {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      d[j] = __builtin_huge_val();
+      id[j] = 0x7fffffff;
+    }
+    found = 0;
+  }
+  static __device__ bool Less(double da, int ia, double db, int ib) { return da < db || (da == db && ia < ib); }
+  __device__ void Insert(int i, double dd) {
+    if (!Less(dd, i, d[K - 1], id[K - 1])) return;
+    // Entry j takes the old entry j - 1 if the new point sorts before it,
+    // else the new point if it sorts before the old entry j (descending j
+    // reads only old entries).
+#pragma unroll
+    for (int j = K - 1; j > 0; --j) {
+      const bool before_prev = Less(dd, i, d[j - 1], id[j - 1]);
+      const bool before_here = Less(dd, i, d[j], id[j]);
+      const double nd = before_prev ? d[j - 1] : (before_here ? dd : d[j]);
+      const int ni = before_prev ? id[j - 1] : (before_here ? i : id[j]);
+      d[j] = nd;
+      id[j] = ni;
+    }
+    if (Less(dd, i, d[0], id[0])) {
+      d[0] = dd;
+      id[0] = i;
+    }
+    found = min(found + 1, K);
+  }
+};
+
+// Knn with the k best in registers (same rings, same stopping rule).
+template <int K>
+__device__ void KnnTop(double qx, double qy, const double* __restrict__ px, const double* __restrict__ py,
+                       const PointGridDev& g, const int* __restrict__ start, const int* __restrict__ sorted,
+                       TopK<K>& best) {
+  best.Init();
+  const int cx = CellCoord(qx, g.x0, g.cell, g.gx);
+  const int cy = CellCoord(qy, g.y0, g.cell, g.gy);
+  const int rmax = max(max(cx, g.gx - 1 - cx), max(cy, g.gy - 1 - cy));
+  for (int r = 0; r <= rmax; ++r) {
+    for (int yy = max(cy - r, 0); yy <= min(cy + r, g.gy - 1); ++yy) {
+      const bool edge = yy == cy - r || yy == cy + r;
+      const int step = edge || r == 0 ? 1 : 2 * r;
+      for (int xx = cx - r; xx <= cx + r; xx += step) {
+        if (xx < 0 || xx >= g.gx) continue;
+        const int c = yy * g.gx + xx;
+        for (int t = start[c]; t < start[c + 1]; ++t) {
+          const int i = sorted[t];
+          const double dx = px[i] - qx;
+          const double dy = py[i] - qy;
+          best.Insert(i, dx * dx + dy * dy);
+        }
+      }
+    }
+    if (best.found == K && r >= 2) {
+      const double lim = static_cast<double>(r - 1) * g.cell;
+      if (best.d[K - 1] < lim * lim) break;
+    }
+  }
+}
+
+// PCA normal of point i from its k nearest (PcaNormal, normals.cpp:70-103):
+// the eigenvector of the smaller covariance eigenvalue (closed form),
+// oriented toward the point's origin, and the eigenvalue ratio.
+__device__ __forceinline__ void PcaFromSums(double a, double b, double c, double qx, double qy, double ox,
+                                            double oy, double& nx, double& ny, double& corner) {
+  const double half_tr = 0.5 * (a + c);
+  const double half_diff = 0.5 * (a - c);
+  const double rad = sqrt(half_diff * half_diff + b * b);
+  const double l_min = half_tr - rad;
+  const double l_max = half_tr + rad;
+  const double v1x = b;
+  const double v1y = l_min - a;
+  const double v2x = l_min - c;
+  const double v2y = b;
+  const double n1 = sqrt(v1x * v1x + v1y * v1y);
+  const double n2 = sqrt(v2x * v2x + v2y * v2y);
+  if (n1 < 1e-300 && n2 < 1e-300) {
+    nx = a <= c ? 1.0 : 0.0;
+    ny = a <= c ? 0.0 : 1.0;
+  } else if (n1 >= n2) {
+    nx = v1x / n1;
+    ny = v1y / n1;
+  } else {
+    nx = v2x / n2;
+    ny = v2y / n2;
+  }
+  if (nx * (ox - qx) + ny * (oy - qy) < 0.0) {
+    nx = -nx;
+    ny = -ny;
+  }
+  corner = fmax(l_min, 0.0) / fmax(l_max, 1e-12);
+}
+
+// kNN PCA normal of every point, k = K (at most the point count).
+template <int K, class GridSrc>
+__global__ void __launch_bounds__(kThreads) KPointNormalsK(const double* __restrict__ px, const double* __restrict__ py,
+                               const double* __restrict__ ox, const double* __restrict__ oy, int64_t n,
+                               GridSrc src, const int* __restrict__ start, const int* __restrict__ sorted,
+                               double* __restrict__ nrm_x, double* __restrict__ nrm_y, double* __restrict__ corner) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const PointGridDev g = src.Load();
+  const double qx = px[i];
+  const double qy = py[i];
+  TopK<K> best;
+  KnnTop<K>(qx, qy, px, py, g, start, sorted, best);
+  // K <= n, so all K entries are points; sums in (distance, index) order.
+  double nbx[K];
+  double nby[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    nbx[j] = px[best.id[j]];
+    nby[j] = py[best.id[j]];
+  }
+  double mx = 0.0;
+  double my = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    mx = mx + nbx[j];
+    my = my + nby[j];
+  }
+  mx = mx / static_cast<double>(K);
+  my = my / static_cast<double>(K);
+  double a = 0.0;
+  double b = 0.0;
+  double c = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const double dx = nbx[j] - mx;
+    const double dy = nby[j] - my;
+    a += dx * dx;
+    b += dx * dy;
+    c += dy * dy;
+  }
+  double nx, ny, cr;
+  PcaFromSums(a, b, c, qx, qy, ox[i], oy[i], nx, ny, cr);
+  nrm_x[i] = nx;
+  nrm_y[i] = ny;
+  corner[i] = cr;
+}
+
+// The same for any k up to kMaxK, with the k best in (scratch) arrays.
 template <class GridSrc>
 __global__ void KPointNormals(const double* __restrict__ px, const double* __restrict__ py,
                               const double* __restrict__ ox, const double* __restrict__ oy, int64_t n, int k,
@@ -189,36 +340,45 @@ __global__ void KPointNormals(const double* __restrict__ px, const double* __res
     b += dx * dy;
     c += dy * dy;
   }
-  const double half_tr = 0.5 * (a + c);
-  const double half_diff = 0.5 * (a - c);
-  const double rad = sqrt(half_diff * half_diff + b * b);
-  const double l_min = half_tr - rad;
-  const double l_max = half_tr + rad;
-  const double v1x = b;
-  const double v1y = l_min - a;
-  const double v2x = l_min - c;
-  const double v2y = b;
-  const double n1 = sqrt(v1x * v1x + v1y * v1y);
-  const double n2 = sqrt(v2x * v2x + v2y * v2y);
-  double nx;
-  double ny;
-  if (n1 < 1e-300 && n2 < 1e-300) {
-    nx = a <= c ? 1.0 : 0.0;
-    ny = a <= c ? 0.0 : 1.0;
-  } else if (n1 >= n2) {
-    nx = v1x / n1;
-    ny = v1y / n1;
-  } else {
-    nx = v2x / n2;
-    ny = v2y / n2;
-  }
-  if (nx * (ox[i] - qx) + ny * (oy[i] - qy) < 0.0) {
-    nx = -nx;
-    ny = -ny;
-  }
+  double nx, ny, cr;
+  PcaFromSums(a, b, c, qx, qy, ox[i], oy[i], nx, ny, cr);
   nrm_x[i] = nx;
   nrm_y[i] = ny;
-  corner[i] = fmax(l_min, 0.0) / fmax(l_max, 1e-12);
+  corner[i] = cr;
+}
+
+// Point normals with k = kk (2 <= kk <= the point count): the register
+// kernel for k up to 16, the general one above that.
+template <class GridSrc>
+void LaunchPointNormals(unsigned blocks, hipStream_t s, const double* px, const double* py, const double* ox,
+                        const double* oy, int64_t n, int kk, GridSrc src, const int* start, const int* sorted,
+                        double* nrm_x, double* nrm_y, double* corner) {
+#define SDFSLAM_POINT_NORMALS_K(K)                                                                        \
+  case K:                                                                                              \
+    KPointNormalsK<K, GridSrc><<<blocks, kThreads, 0, s>>>(px, py, ox, oy, n, src, start, sorted, nrm_x, \
+                                                            nrm_y, corner);                             \
+    return;
+  switch (kk) {
+    SDFSLAM_POINT_NORMALS_K(2)
+    SDFSLAM_POINT_NORMALS_K(3)
+    SDFSLAM_POINT_NORMALS_K(4)
+    SDFSLAM_POINT_NORMALS_K(5)
+    SDFSLAM_POINT_NORMALS_K(6)
+    SDFSLAM_POINT_NORMALS_K(7)
+    SDFSLAM_POINT_NORMALS_K(8)
+    SDFSLAM_POINT_NORMALS_K(9)
+    SDFSLAM_POINT_NORMALS_K(10)
+    SDFSLAM_POINT_NORMALS_K(11)
+    SDFSLAM_POINT_NORMALS_K(12)
+    SDFSLAM_POINT_NORMALS_K(13)
+    SDFSLAM_POINT_NORMALS_K(14)
+    SDFSLAM_POINT_NORMALS_K(15)
+    SDFSLAM_POINT_NORMALS_K(16)
+    default:
+      KPointNormals<GridSrc><<<blocks, kThreads, 0, s>>>(px, py, ox, oy, n, kk, src, start, sorted, nrm_x, nrm_y,
+                                                          corner);
+  }
+#undef SDFSLAM_POINT_NORMALS_K
 }
 
 // Normal and eikonal scale of every grid node from its nearest scan point
@@ -468,7 +628,7 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
   KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridByValue{g}, cell_of, count);
   KCellScan<<<1, kScanThreads, 0, s>>>(count, GridByValue{g}, start);
   KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
-  KPointNormals<<<pb, kThreads, 0, s>>>(px, py, ox, oy, n, kk, GridByValue{g}, start, sorted, pnx, pny, pcorner);
+  LaunchPointNormals(pb, s, px, py, ox, oy, n, kk, GridByValue{g}, start, sorted, pnx, pny, pcorner);
   const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
   KNodeNormals<<<vb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, pnx,
                                        pny, pcorner, GridByValue{g}, start, sorted, method, corner_threshold, vnx,
@@ -783,6 +943,140 @@ size_t EikonalNormalsWorkBytes(int64_t scan_points, int64_t nodes) {
   return 256 + 7 * nd + 2 * ni + 3 * ci + 3 * vd;
 }
 
+namespace {
+
+// Rows of one appended frame per workgroup (FrameRowsJob). The scan rows are
+// written directly; a hallucinated sample's row follows from the count of
+// points of length >= 1e-9 before its point (a workgroup scan per chunk).
+constexpr int kFrameThreads = 1024;
+__global__ void __launch_bounds__(kFrameThreads) KFrameRows(FrameRowsJob j) {
+  const int f = static_cast<int>(blockIdx.x);
+  const int64_t p0 = j.scan_off[f];
+  const int64_t n = j.scan_off[f + 1] - p0;
+  const int64_t r0 = j.row_off[f];
+  const int64_t l0 = j.list_off[f];
+  const int frame = j.first_frame + f;
+  const int per_point = j.per_side * (j.both_directions != 0 ? 2 : 1);
+  __shared__ int part[kFrameThreads];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += kFrameThreads) {
+    const int64_t i = base + threadIdx.x;
+    double px = 0.0;
+    double py = 0.0;
+    double norm = 0.0;
+    int valid = 0;
+    if (i < n) {
+      px = j.scan_xy[2 * (p0 + i)];
+      py = j.scan_xy[2 * (p0 + i) + 1];
+      const int64_t r = r0 + i;
+      j.frame[r] = frame;
+      j.x[r] = px;
+      j.y[r] = py;
+      j.expected_sdf[r] = 0.0;
+      j.sqrt_weight[r] = j.sqrt_w_scan;
+      j.scan_list[l0 + i] = static_cast<int32_t>(r);
+      norm = sqrt(px * px + py * py);
+      valid = (per_point > 0 && !(norm < 1e-9)) ? 1 : 0;
+    }
+    // Inclusive scan of the valid flags over the chunk.
+    part[threadIdx.x] = valid;
+    __syncthreads();
+    for (int off = 1; off < kFrameThreads; off <<= 1) {
+      const int t = threadIdx.x >= static_cast<unsigned>(off) ? part[threadIdx.x - off] : 0;
+      __syncthreads();
+      part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (valid != 0) {
+      const int64_t before = carry + part[threadIdx.x] - 1;
+      int64_t r = r0 + n + before * per_point;
+      // toward_origin = -point / norm * step (component by component).
+      const double tx = -px / norm * j.step;
+      const double ty = -py / norm * j.step;
+      for (int s = 1; s <= j.per_side; ++s) {
+        const double ds = static_cast<double>(s);
+        j.frame[r] = frame;
+        j.x[r] = px + tx * ds;
+        j.y[r] = py + ty * ds;
+        j.expected_sdf[r] = j.step * ds;
+        j.sqrt_weight[r] = j.sqrt_w_hallucination;
+        ++r;
+        if (j.both_directions != 0) {
+          j.frame[r] = frame;
+          j.x[r] = px - tx * ds;
+          j.y[r] = py - ty * ds;
+          j.expected_sdf[r] = -j.step * ds;
+          j.sqrt_weight[r] = j.sqrt_w_hallucination;
+          ++r;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == kFrameThreads - 1) carry += part[kFrameThreads - 1];
+    __syncthreads();
+  }
+}
+
+__global__ void KClearBytes(uint8_t* __restrict__ p, int64_t n) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// One thread per point row: the row placed at its frame's pose
+// (Pose2::Apply), located in the grid (GridMap::Locate, grid_map.hpp:58-72:
+// outside the grid marks nothing), marks the nodes of the dilated cell.
+// Every write stores 1, so concurrent writes to a node agree.
+__global__ void KActiveMarks(const int32_t* __restrict__ frame, const double* __restrict__ sx,
+                             const double* __restrict__ sy, int64_t n, const double* __restrict__ cos_sin,
+                             const double* __restrict__ poses, sdfslam_grid g, int margin,
+                             uint8_t* __restrict__ marks) {
+  const int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (r >= n) return;
+  const int f = frame[r];
+  const double c = cos_sin[2 * f];
+  const double s = cos_sin[2 * f + 1];
+  const double gx = (c * sx[r] - s * sy[r]) + poses[3 * f];
+  const double gy = (s * sx[r] + c * sy[r]) + poses[3 * f + 1];
+  const double fx = (gx - g.min_x) / g.dx;
+  const double fy = (gy - g.min_y) / g.dy;
+  if (fx < 0.0 || fy < 0.0 || fx > g.nx - 1 || fy > g.ny - 1) return;
+  const int cw = min(static_cast<int>(fx), g.nx - 2);
+  const int ch = min(static_cast<int>(fy), g.ny - 2);
+  for (int h = max(0, ch - margin); h <= min(g.ny - 1, ch + 1 + margin); ++h) {
+    for (int w = max(0, cw - margin); w <= min(g.nx - 1, cw + 1 + margin); ++w) {
+      marks[static_cast<int64_t>(h) * g.nx + w] = 1;
+    }
+  }
+}
+
+}  // namespace
+
+int EnqueueFrameRows(const FrameRowsJob& j, hipStream_t s) {
+  if (j.frames <= 0) return SDFSLAM_OK;
+  if (j.frames > 65535 || j.per_side < 0 || (j.per_side > 0 && !(j.step > 0.0))) {
+    t_error = "appended frames need at most 65535 frames and a positive hallucination step";
+    return SDFSLAM_ERR_INVALID_ARGUMENT;
+  }
+  KFrameRows<<<static_cast<unsigned>(j.frames), kFrameThreads, 0, s>>>(j);
+  HIPN(hipGetLastError());
+  return SDFSLAM_OK;
+}
+
+int EnqueueActiveMarks(const int32_t* frame, const double* x, const double* y, int64_t num_rows,
+                       const double* cos_sin, const double* poses, const sdfslam_grid& grid, int32_t margin,
+                       uint8_t* marks, hipStream_t s) {
+  const int64_t nodes = static_cast<int64_t>(grid.nx) * grid.ny;
+  KClearBytes<<<static_cast<unsigned>((nodes + kThreads - 1) / kThreads), kThreads, 0, s>>>(marks, nodes);
+  if (num_rows > 0) {
+    KActiveMarks<<<static_cast<unsigned>((num_rows + kThreads - 1) / kThreads), kThreads, 0, s>>>(
+        frame, x, y, num_rows, cos_sin, poses, grid, margin, marks);
+  }
+  HIPN(hipGetLastError());
+  return SDFSLAM_OK;
+}
+
 int EnqueueEikonalNormals(const EikonalNormalsJob& j, void* work, hipStream_t s) {
   const int64_t n = j.scan_points;
   const int64_t nodes = static_cast<int64_t>(j.nx) * j.ny;
@@ -832,7 +1126,7 @@ int EnqueueEikonalNormals(const EikonalNormalsJob& j, void* work, hipStream_t s)
   KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridInMemory{gp}, cell_of, count);
   KCellScan<<<1, kScanThreads, 0, s>>>(count, GridInMemory{gp}, start);
   KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
-  KPointNormals<<<pb, kThreads, 0, s>>>(px, py, ox, oy, n, kk, GridInMemory{gp}, start, sorted, pnx, pny, pcorner);
+  LaunchPointNormals(pb, s, px, py, ox, oy, n, kk, GridInMemory{gp}, start, sorted, pnx, pny, pcorner);
   KNodeNormals<<<vb, kThreads, 0, s>>>(j.nx, j.ny, j.min_x, j.min_y, j.dx, j.dy, px, py, pnx, pny, pcorner,
                                        GridInMemory{gp}, start, sorted, j.method, j.corner_threshold, vnx, vny,
                                        vscale);

@@ -63,6 +63,43 @@ size_t EikonalNormalsWorkBytes(int64_t scan_points, int64_t nodes);
 /// eikonal scale is 1, so the specs themselves do not change): the values
 /// AssignNodeNormals(ComputeScanNormals(...)) gives on the host, bit for bit.
 int EnqueueEikonalNormals(const EikonalNormalsJob& job, void* work, hipStream_t stream);
+/// Point rows of frames appended to a device-resident problem (device
+/// pointers): frame f (f = 0 .. frames-1, problem frame first_frame + f) has
+/// raw scan points scan_xy[2 * scan_off[f] ..] (x, y pairs, scan_off[f + 1] -
+/// scan_off[f] of them), its rows start at row_off[f] and its entries of the
+/// scan-row list at list_off[f]. Rows per frame (Problem::Problem,
+/// problem.cpp:26-37): the scan points in order (expected SDF 0, weight
+/// sqrt_w_scan), then the hallucinated samples of every point longer than
+/// 1e-9 (hallucination.cpp:5-28: for j = 1 .. per_side the free-space sample
+/// at +j * step toward the sensor, then, with both directions, the occluded
+/// one). At most 65,535 frames per job.
+struct FrameRowsJob {
+  int32_t frames;
+  int32_t first_frame;
+  const double* scan_xy;
+  const int64_t* scan_off;
+  const int64_t* row_off;
+  const int64_t* list_off;
+  int32_t per_side;
+  int32_t both_directions;
+  double step;
+  double sqrt_w_scan;
+  double sqrt_w_hallucination;
+  int32_t* frame;
+  double* x;
+  double* y;
+  double* expected_sdf;
+  double* sqrt_weight;
+  int32_t* scan_list;
+};
+int EnqueueFrameRows(const FrameRowsJob& job, hipStream_t stream);
+/// Active-region marks (problem.cpp:39-70): marks[node] = 1 for every node
+/// within `margin` cells (and the cell's own +1) of a cell holding a point
+/// row placed at its frame's pose (cos_sin as in EikonalNormalsJob); the
+/// marks are cleared first. Device pointers; num_rows point rows.
+int EnqueueActiveMarks(const int32_t* frame, const double* x, const double* y, int64_t num_rows,
+                       const double* cos_sin, const double* poses, const sdfslam_grid& grid, int32_t margin,
+                       uint8_t* marks, hipStream_t stream);
 const char* NormalsLastError();
 
 }  // namespace sdfslam

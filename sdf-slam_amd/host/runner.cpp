@@ -105,6 +105,86 @@ SolveResult HipSession::Collect(Problem& problem, const sdfslam_solve_result& r,
   return result;
 }
 
+bool AppendApplies(const ProblemOptions& options, bool has_relations) {
+  return DeviceNormalsApply(options) && !has_relations &&
+         (options.hallucination.points_per_scan_point <= 0 || options.hallucination.step_size > 0.0);
+}
+
+sdfslam_hip_context* HipSession::Append(const std::vector<const Scan*>& scans, const std::vector<Pose2>& odometry,
+                                        const ProblemOptions& options, const GridMap& map,
+                                        const std::vector<Pose2>& estimated) {
+  if (ctx_ == nullptr) throw std::logic_error("appending frames needs an existing context");
+  if (scans.size() != odometry.size() || scans.empty()) {
+    throw std::invalid_argument("append needs one odometry measurement per scan");
+  }
+  std::vector<int64_t> counts;
+  std::vector<double> xy;
+  std::vector<double> odo;
+  for (size_t f = 0; f < scans.size(); ++f) {
+    counts.push_back(static_cast<int64_t>(scans[f]->points.size()));
+    for (const Vec2& p : scans[f]->points) {
+      xy.push_back(p.x);
+      xy.push_back(p.y);
+    }
+    odo.push_back(odometry[f].x);
+    odo.push_back(odometry[f].y);
+    odo.push_back(odometry[f].theta);
+  }
+  sdfslam_frame_options fo{};
+  fo.hallucination_points = options.hallucination.points_per_scan_point;
+  fo.hallucination_step = options.hallucination.step_size;
+  fo.hallucination_both_directions = options.hallucination.both_directions ? 1 : 0;
+  fo.weight_scan = options.weights.scan;
+  fo.weight_hallucination = options.weights.hallucination;
+  fo.weight_eikonal = options.weights.eikonal;
+  fo.weight_odometry = options.weights.odometry;
+  fo.active_region = options.active_region ? 1 : 0;
+  fo.active_margin = options.active_margin;
+  fo.normals.k_neighbors = options.normals.k_neighbors;
+  fo.normals.method = options.normals.method == NormalMethod::kPca ? SDFSLAM_NORMALS_PCA : SDFSLAM_NORMALS_HYBRID;
+  fo.normals.corner_threshold = options.normals.corner_threshold;
+  fo.normals.frame_cos_sin = nullptr;
+  std::vector<double> poses(3 * estimated.size());
+  for (size_t i = 0; i < estimated.size(); ++i) {
+    poses[3 * i] = estimated[i].x;
+    poses[3 * i + 1] = estimated[i].y;
+    poses[3 * i + 2] = estimated[i].theta;
+  }
+  Rethrow(sdfslam_hip_context_append_frames(ctx_, static_cast<int32_t>(scans.size()), counts.data(), xy.data(),
+                                            odo.data(), &fo, map.values().data(), poses.data()));
+  return ctx_;
+}
+
+SolveResult HipSession::SolveLoaded(GridMap& map, std::vector<Pose2>& estimated, const SolverOptions& options) {
+  if (ctx_ == nullptr) throw std::logic_error("no context to solve");
+  const sdfslam_solver_options copts = ToC(options);
+  std::vector<double> history(static_cast<size_t>(std::max(0, options.max_iterations)) + 1, 0.0);
+  sdfslam_solve_result cres{};
+  cres.cost_history = history.data();
+  cres.cost_history_capacity = static_cast<int32_t>(history.size());
+  const int st = sdfslam_hip_context_solve(ctx_, &copts, &cres);
+  return CollectState(map, estimated, cres, st, history);
+}
+
+SolveResult HipSession::CollectState(GridMap& map, std::vector<Pose2>& estimated, const sdfslam_solve_result& r,
+                                     int status, const std::vector<double>& history) {
+  Rethrow(status);
+  std::vector<double> poses(3 * estimated.size());
+  Rethrow(sdfslam_hip_get_state(ctx_, map.values().data(), poses.data()));
+  for (size_t i = 0; i < estimated.size(); ++i) estimated[i] = {poses[3 * i], poses[3 * i + 1], poses[3 * i + 2]};
+  SolveResult result;
+  result.converged = r.converged != 0;
+  result.iterations = r.iterations;
+  result.initial_cost = r.initial_cost;
+  result.final_cost = r.final_cost;
+  result.duration_seconds = r.duration_seconds;
+  result.cost_history.assign(history.begin(),
+                             history.begin() + std::min<std::ptrdiff_t>(
+                                                   static_cast<std::ptrdiff_t>(history.size()),
+                                                   std::min(r.cost_history_length, r.cost_history_capacity)));
+  return result;
+}
+
 std::vector<NodeNormal> DeviceNodeNormals(const std::vector<Vec2>& points, const std::vector<Vec2>& origins,
                                           const GridMap& map, const NormalOptions& options) {
   const size_t n = points.size();
@@ -203,6 +283,22 @@ SolveResult SolveWith(const IncrementSolveFn& solve_fn, Problem& problem, const 
   return step;
 }
 
+/// One increment's record and the run's totals (runner.cpp:262-270).
+void Record(IncrementalRun& run, IncrementRecord& rec, const SolveResult& step) {
+  rec.iterations = step.iterations;
+  rec.initial_cost = step.initial_cost;
+  rec.final_cost = step.final_cost;
+  run.increments.push_back(rec);
+  if (run.total.initial_cost < 0.0) {
+    run.total.initial_cost = step.initial_cost;
+  }
+  run.total.iterations += step.iterations;
+  run.total.final_cost = step.final_cost;
+  run.total.duration_seconds += step.duration_seconds;
+  run.total.converged = step.converged;
+  run.total.cost_history.insert(run.total.cost_history.end(), step.cost_history.begin(), step.cost_history.end());
+}
+
 }  // namespace
 
 IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, int increment_size,
@@ -231,10 +327,32 @@ IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, i
     for (size_t i = next_frame; i < last; ++i) {
       run.estimated.push_back(run.estimated.back().Compose(wl.odometry[i - 1]));
     }
+    const size_t first_new = next_frame;
     next_frame = last;
     IncrementRecord rec;
     rec.frames = static_cast<int>(next_frame);
     auto t0 = std::chrono::steady_clock::now();
+    if (!solve_fn && hooks.append_frames && session.HasContext() &&
+        AppendApplies(wl.problem, !wl.relations.empty())) {
+      // The context extends its own problem by the new frames, built on the
+      // device (the same problem as the rebuild below).
+      std::vector<const Scan*> new_scans;
+      std::vector<Pose2> new_odometry;
+      for (size_t i = first_new; i < last; ++i) {
+        new_scans.push_back(&wl.scans[i]);
+        new_odometry.push_back(wl.odometry[i - 1]);
+      }
+      session.Append(new_scans, new_odometry, wl.problem, run.map, run.estimated);
+      rec.build_seconds = Seconds(t0);
+      t0 = std::chrono::steady_clock::now();
+      SolveResult step = session.SolveLoaded(run.map, run.estimated, solver);
+      rec.solve_seconds = Seconds(t0);
+      step.duration_seconds = rec.solve_seconds;
+      Record(run, rec, step);
+      ++increment;
+      if (hooks.after_increment) hooks.after_increment(next_frame, increment, run.estimated, run.map);
+      continue;
+    }
     const std::vector<Scan> scans_so_far(wl.scans.begin(),
                                          wl.scans.begin() + static_cast<std::ptrdiff_t>(next_frame));
     const std::vector<Pose2> odom_so_far(wl.odometry.begin(),
@@ -255,20 +373,7 @@ IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, i
     step.duration_seconds = rec.solve_seconds;
     run.estimated = problem.poses();
     run.map = problem.map();
-    rec.iterations = step.iterations;
-    rec.initial_cost = step.initial_cost;
-    rec.final_cost = step.final_cost;
-    run.increments.push_back(rec);
-    // Totals as runner.cpp:262-270.
-    if (run.total.initial_cost < 0.0) {
-      run.total.initial_cost = step.initial_cost;
-    }
-    run.total.iterations += step.iterations;
-    run.total.final_cost = step.final_cost;
-    run.total.duration_seconds += step.duration_seconds;
-    run.total.converged = step.converged;
-    run.total.cost_history.insert(run.total.cost_history.end(), step.cost_history.begin(),
-                                  step.cost_history.end());
+    Record(run, rec, step);
     ++increment;
     if (hooks.after_increment) hooks.after_increment(next_frame, increment, run.estimated, run.map);
   }

@@ -52,6 +52,21 @@ class HipSession {
                             const NormalOptions* device_normals = nullptr);
   SolveResult Collect(Problem& problem, const sdfslam_solve_result& result, int status,
                       const std::vector<double>& history);
+  /// Appends frames to the context's problem, built on the device
+  /// (sdfslam_hip_context_append_frames): the next increment of a session
+  /// whose context holds the problem over the earlier frames. `estimated`
+  /// holds every pose (the new frames' initial poses last) and `map` the
+  /// map the problem is built at.
+  sdfslam_hip_context* Append(const std::vector<const Scan*>& scans, const std::vector<Pose2>& odometry,
+                              const ProblemOptions& options, const GridMap& map,
+                              const std::vector<Pose2>& estimated);
+  /// Collect for an appended increment: the solved state into `map` and
+  /// `estimated`.
+  SolveResult CollectState(GridMap& map, std::vector<Pose2>& estimated, const sdfslam_solve_result& result,
+                           int status, const std::vector<double>& history);
+  /// Solves the context's current problem (after Append) and collects the
+  /// state into `map` and `estimated`.
+  SolveResult SolveLoaded(GridMap& map, std::vector<Pose2>& estimated, const SolverOptions& options);
 
  private:
   /// Creates the context or replaces its problem.
@@ -62,6 +77,11 @@ class HipSession {
 /// Whether a problem with these options can take its eikonal normals from
 /// the device (DeferredNodeNormals): all-scans neighbourhood, pca or hybrid.
 bool DeviceNormalsApply(const ProblemOptions& options);
+
+/// Whether a session with these options can append its frames on the
+/// device (HipSession::Append): device normals apply, no relations, and a
+/// positive hallucination step when it hallucinates.
+bool AppendApplies(const ProblemOptions& options, bool has_relations);
 
 /// Per-increment record.
 struct IncrementRecord {
@@ -108,6 +128,9 @@ void InitializeFromScans(GridMap& map, const std::vector<Scan>& scans, const std
 struct IncrementHooks {
   bool init_from_scans{false};
   bool init_signed{true};
+  // Device runs extend the context's problem frame by frame on the device
+  // (HipSession::Append) where that applies; false rebuilds every increment.
+  bool append_frames{true};
   std::function<void(size_t frames, size_t increment, const std::vector<Pose2>& estimated, const GridMap& map)>
       after_increment;
 };

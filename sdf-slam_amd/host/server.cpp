@@ -249,8 +249,20 @@ void Server::Prepare(Prepared& item, Pending& frame) {
     s.estimated.push_back(s.estimated.back().Compose(frame.odometry));
     s.scans.push_back(std::move(frame.scan));
     s.odometry.push_back(frame.odometry);
-    rec.frames = static_cast<int>(s.scans.size());
+    rec.frames = static_cast<int>(s.estimated.size());
     const auto t0 = Clock::now();
+    if (s.hip.HasContext() && AppendApplies(s.problem, false)) {
+      // The context extends its own problem by the frame (rows, odometry and
+      // normals built on the device). A failed append leaves the context's
+      // problem undefined: it is dropped and the problem rebuilt below.
+      try {
+        item.ctx = s.hip.Append({&s.scans.back()}, {frame.odometry}, s.problem, s.map, s.estimated);
+        rec.build_seconds = Seconds(t0, Clock::now());
+        return;
+      } catch (const std::exception&) {
+        s.hip.Release();
+      }
+    }
     // With device normals, increments after the first take their eikonal
     // normals from the device inside the batched solve (no host kNN, no GPU
     // call from this thread).
@@ -296,10 +308,16 @@ void Server::SolveBatch(std::vector<std::unique_ptr<Prepared>>& batch) {
     Prepared& item = *batch[static_cast<size_t>(j)];
     Session& s = *item.session;
     try {
-      const SolveResult r = s.hip.Collect(*item.problem, results[static_cast<size_t>(j)],
-                                          status[static_cast<size_t>(j)], hist[static_cast<size_t>(j)]);
-      s.estimated = item.problem->poses();
-      s.map = item.problem->map();
+      SolveResult r;
+      if (item.problem != nullptr) {
+        r = s.hip.Collect(*item.problem, results[static_cast<size_t>(j)], status[static_cast<size_t>(j)],
+                          hist[static_cast<size_t>(j)]);
+        s.estimated = item.problem->poses();
+        s.map = item.problem->map();
+      } else {
+        r = s.hip.CollectState(s.map, s.estimated, results[static_cast<size_t>(j)],
+                               status[static_cast<size_t>(j)], hist[static_cast<size_t>(j)]);
+      }
       item.record.iterations = r.iterations;
       item.record.initial_cost = r.initial_cost;
       item.record.final_cost = r.final_cost;

@@ -143,6 +143,23 @@ class NormalsRequest(C.Structure):
                 ("frame_cos_sin", C.POINTER(C.c_double))]
 
 
+class FrameOptions(C.Structure):
+    _fields_ = [("hallucination_points", C.c_int32), ("hallucination_step", C.c_double),
+                ("hallucination_both_directions", C.c_int32), ("weight_scan", C.c_double),
+                ("weight_hallucination", C.c_double), ("weight_eikonal", C.c_double),
+                ("weight_odometry", C.c_double), ("active_region", C.c_int32),
+                ("active_margin", C.c_int32), ("normals", NormalsRequest)]
+
+
+def frame_options(po):
+    """FrameOptions of a session with ProblemOptions po (pca or hybrid normals)."""
+    return FrameOptions(po.hallucination_points_per_scan_point, po.hallucination_step_size,
+                        po.hallucination_both_directions, po.weight_scan, po.weight_hallucination,
+                        po.weight_eikonal, po.weight_odometry, po.active_region, po.active_margin,
+                        NormalsRequest(po.normals_k_neighbors, po.normals_method,
+                                       po.normals_corner_threshold, None))
+
+
 class BatchInfo(C.Structure):
     _fields_ = [("graph_captures", C.c_int32), ("graph_launches", C.c_int32),
                 ("epochs_rebuilt", C.c_int32), ("setup_seconds", C.c_double),
@@ -181,6 +198,7 @@ HIP_SYMBOLS = [
     "sdfslam_hip_pose_margins", "sdfslam_hip_node_normals", "sdfslam_hip_init_from_scans",
     "sdfslam_hip_init_from_scans_noalloc", "sdfslam_hip_reference_factor_counts",
     "sdfslam_hip_elimination_order", "sdfslam_hip_context_update_device_normals",
+    "sdfslam_hip_context_append_frames", "sdfslam_hip_context_point_rows",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -222,6 +240,9 @@ hip.sdfslam_hip_reference_factor_counts.argtypes = [C.c_void_p, _i64p, _f64p]
 hip.sdfslam_hip_elimination_order.argtypes = [C.c_void_p, _i32p]
 hip.sdfslam_hip_context_update_device_normals.argtypes = [C.c_void_p, C.POINTER(ProblemView), _f64p, _f64p,
                                                              C.POINTER(NormalsRequest)]
+hip.sdfslam_hip_context_append_frames.argtypes = [C.c_void_p, C.c_int32, _i64p, _f64p, _f64p,
+                                                  C.POINTER(FrameOptions), _f64p, _f64p]
+hip.sdfslam_hip_context_point_rows.argtypes = [C.c_void_p, C.c_int64, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p]
 hip.sdfslam_hip_set_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_get_state.argtypes = [C.c_void_p, _f64p, _f64p]
 hip.sdfslam_hip_analysis.argtypes = [C.c_void_p, C.POINTER(AnalysisInfo)]
@@ -516,12 +537,43 @@ class Context:
         pv = None if poses is None else np.ascontiguousarray(poses, dtype=np.float64)
         _check_hip(hip.sdfslam_hip_set_state(self._h, ptr(mv, _f64p), ptr(pv, _f64p)))
 
+    @property
+    def num_poses(self):
+        return getattr(self, "_num_poses", None) or self.problem.view.num_poses
+
     def state(self):
         g = self.problem.view.grid
         m = np.zeros(g.nx * g.ny)
-        p = np.zeros(3 * self.problem.view.num_poses)
+        p = np.zeros(3 * self.num_poses)
         _check_hip(hip.sdfslam_hip_get_state(self._h, ptr(m, _f64p), ptr(p, _f64p)))
         return m, p
+
+    def append_frames(self, scans, odometry, options, map_values, poses):
+        """Extends the context's problem by frames built on the device
+        (sdfslam_hip_context_append_frames). scans: list of (k, 2) sensor-frame
+        points; odometry: (frames, 3); options: FrameOptions; map_values and
+        poses: the state (every pose, the new frames' last)."""
+        counts = np.array([len(s) for s in scans], dtype=np.int64)
+        xy = (np.ascontiguousarray(np.concatenate([np.asarray(s, float).reshape(-1, 2) for s in scans]))
+              if counts.sum() > 0 else np.zeros((1, 2)))
+        odo = np.ascontiguousarray(odometry, dtype=np.float64).reshape(-1, 3)
+        m = np.ascontiguousarray(map_values, dtype=np.float64)
+        p = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1)
+        _check_hip(hip.sdfslam_hip_context_append_frames(self._h, len(scans), ptr(counts, _i64p), ptr(xy, _f64p),
+                                                        ptr(odo, _f64p), C.byref(options), ptr(m, _f64p),
+                                                        ptr(p, _f64p)))
+        self._num_poses = len(p) // 3
+
+    def point_rows(self):
+        """(frame, x, y, expected_sdf, sqrt_weight) of the rows the device holds."""
+        cnt = np.zeros(1, dtype=np.int64)
+        _check_hip(hip.sdfslam_hip_context_point_rows(self._h, 0, ptr(cnt, _i64p), None, None, None, None, None))
+        n = int(cnt[0])
+        f = np.zeros(n, dtype=np.int32)
+        x, y, d, w = (np.zeros(n) for _ in range(4))
+        _check_hip(hip.sdfslam_hip_context_point_rows(self._h, n, None, ptr(f, _i32p), ptr(x, _f64p),
+                                                     ptr(y, _f64p), ptr(d, _f64p), ptr(w, _f64p)))
+        return f, x, y, d, w
 
     def assemble(self):
         rows = self.problem.num_residuals
