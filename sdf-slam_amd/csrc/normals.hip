@@ -101,6 +101,21 @@ __global__ void KCellScatter(const int* __restrict__ cell_of, int64_t n, const i
   sorted[start[c] + atomicAdd(&fill[c], 1)] = static_cast<int>(i);
 }
 
+// KCellScatter that also stores the points' coordinates in cell order, so a
+// range of cells is one contiguous range of coordinates.
+__global__ void KCellScatterXY(const int* __restrict__ cell_of, int64_t n, const int* __restrict__ start,
+                               int* __restrict__ fill, const double* __restrict__ px,
+                               const double* __restrict__ py, int* __restrict__ sorted, double* __restrict__ sx,
+                               double* __restrict__ sy) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_of[i];
+  const int t = start[c] + atomicAdd(&fill[c], 1);
+  sorted[t] = static_cast<int>(i);
+  sx[t] = px[i];
+  sy[t] = py[i];
+}
+
 // Inserts point i at squared distance dd into the k best, kept sorted by
 // (distance, index) (the host builder's rule).
 __device__ __forceinline__ void Insert(int i, double dd, int k, int& found, int* idx, double* d2) {
@@ -196,27 +211,81 @@ struct TopK {
   }
 };
 
-// Knn with the k best in registers (same rings, same stopping rule).
+// The k nearest points of one query searched by a whole wavefront: the
+// lanes take the points of a run of cells 64 at a time (cells sorted row
+// major, so a row of a ring is one contiguous run of sorted points), and the
+// points closer than the current k-th join the k best, which every lane
+// holds alike (candidates broadcast one by one from the lanes that found
+// them). The same rings and stopping rule as Knn, so the same k best.
+__device__ __forceinline__ int Lane() { return static_cast<int>(threadIdx.x & 63); }
+
 template <int K>
-__device__ void KnnTop(double qx, double qy, const double* __restrict__ px, const double* __restrict__ py,
-                       const PointGridDev& g, const int* __restrict__ start, const int* __restrict__ sorted,
-                       TopK<K>& best) {
+__device__ void WaveScanRange(int a, int b, double qx, double qy, const double* __restrict__ sx,
+                              const double* __restrict__ sy, const int* __restrict__ sid, TopK<K>& best) {
+  for (int base = a; base < b; base += 64) {
+    const int t = base + Lane();
+    double dd = __builtin_huge_val();
+    int id = 0x7fffffff;
+    if (t < b) {
+      const double dx = sx[t] - qx;
+      const double dy = sy[t] - qy;
+      dd = dx * dx + dy * dy;
+      id = sid[t];
+    }
+    uint64_t m = __ballot(t < b && TopK<K>::Less(dd, id, best.d[K - 1], best.id[K - 1]));
+    while (m != 0) {
+      const int src = __builtin_ctzll(m);
+      m &= m - 1;
+      const int lo = __builtin_amdgcn_readlane(static_cast<int>(__double_as_longlong(dd)), src);
+      const int hi = __builtin_amdgcn_readlane(static_cast<int>(__double_as_longlong(dd) >> 32), src);
+      const double cd =
+          __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+      const int ci = __builtin_amdgcn_readlane(id, src);
+      best.Insert(ci, cd);
+    }
+  }
+}
+
+template <int K>
+__device__ void WaveKnn(double qx, double qy, const PointGridDev& g, const int* __restrict__ start,
+                        const double* __restrict__ sx, const double* __restrict__ sy, const int* __restrict__ sid,
+                        TopK<K>& best) {
   best.Init();
   const int cx = CellCoord(qx, g.x0, g.cell, g.gx);
   const int cy = CellCoord(qy, g.y0, g.cell, g.gy);
   const int rmax = max(max(cx, g.gx - 1 - cx), max(cy, g.gy - 1 - cy));
   for (int r = 0; r <= rmax; ++r) {
-    for (int yy = max(cy - r, 0); yy <= min(cy + r, g.gy - 1); ++yy) {
-      const bool edge = yy == cy - r || yy == cy + r;
-      const int step = edge || r == 0 ? 1 : 2 * r;
-      for (int xx = cx - r; xx <= cx + r; xx += step) {
-        if (xx < 0 || xx >= g.gx) continue;
-        const int c = yy * g.gx + xx;
-        for (int t = start[c]; t < start[c + 1]; ++t) {
-          const int i = sorted[t];
-          const double dx = px[i] - qx;
-          const double dy = py[i] - qy;
-          best.Insert(i, dx * dx + dy * dy);
+    const int x0 = max(cx - r, 0);
+    const int x1 = min(cx + r, g.gx - 1);
+    // Rows cy - r and cy + r: whole runs of cells.
+    if (cy - r >= 0) {
+      WaveScanRange<K>(start[(cy - r) * g.gx + x0], start[(cy - r) * g.gx + x1 + 1], qx, qy, sx, sy, sid, best);
+    }
+    if (r > 0 && cy + r < g.gy) {
+      WaveScanRange<K>(start[(cy + r) * g.gx + x0], start[(cy + r) * g.gx + x1 + 1], qx, qy, sx, sy, sid, best);
+    }
+    // Columns cx - r and cx + r between them: the lanes read the cells'
+    // ranges side by side (lane j: row y0 + j / 2, left or right by j % 2)
+    // and the non-empty ones are scanned one after another.
+    if (r > 0) {
+      const int y0 = max(cy - r + 1, 0);
+      const int cells = 2 * (min(cy + r - 1, g.gy - 1) - y0 + 1);
+      for (int base = 0; base < cells; base += 64) {
+        const int j = base + Lane();
+        const int xx = (j & 1) != 0 ? cx + r : cx - r;
+        int a = 0;
+        int b = 0;
+        if (j < cells && xx >= 0 && xx < g.gx) {
+          const int c = (y0 + (j >> 1)) * g.gx + xx;
+          a = start[c];
+          b = start[c + 1];
+        }
+        uint64_t m = __ballot(b > a);
+        while (m != 0) {
+          const int src = __builtin_ctzll(m);
+          m &= m - 1;
+          WaveScanRange<K>(__builtin_amdgcn_readlane(a, src), __builtin_amdgcn_readlane(b, src), qx, qy, sx, sy,
+                           sid, best);
         }
       }
     }
@@ -260,52 +329,255 @@ __device__ __forceinline__ void PcaFromSums(double a, double b, double c, double
   corner = fmax(l_min, 0.0) / fmax(l_max, 1e-12);
 }
 
-// kNN PCA normal of every point, k = K (at most the point count).
-template <int K, class GridSrc>
-__global__ void __launch_bounds__(kThreads) KPointNormalsK(const double* __restrict__ px, const double* __restrict__ py,
-                               const double* __restrict__ ox, const double* __restrict__ oy, int64_t n,
-                               GridSrc src, const int* __restrict__ start, const int* __restrict__ sorted,
-                               double* __restrict__ nrm_x, double* __restrict__ nrm_y, double* __restrict__ corner) {
-  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (i >= n) return;
+// kNN PCA normals with one workgroup per cell of the point grid: the cell's
+// points are the queries (one per thread), and their candidates, the points
+// of the ring cells around the cell, are staged in LDS by the whole
+// workgroup and read by every thread alike: first rings 0 to 2 (the 5 x 5
+// cells every search visits), then, while some query is not settled by
+// Knn's stopping rule, one more ring at a time. Every query gets the k best
+// Knn gives. Outputs at each point's own index.
+constexpr int kCellThreads = 256;
+constexpr int kCellCandidates = 2048;  // candidates staged in LDS per pass
+constexpr int kCellRanges = 2 * kCellThreads + 8;  // ranges of one staged ring
+
+// The k best over the staged candidates first, first + L, ... below count
+// (a query's L threads share its candidates).
+template <int K, int L>
+__device__ __forceinline__ void TakeStaged(const double* cand_x, const double* cand_y, const int* cand_id, int first,
+                                           int count, double qx, double qy, TopK<K>& best) {
+  for (int k = first; k < count; k += L) {
+    const double dx = cand_x[k] - qx;
+    const double dy = cand_y[k] - qy;
+    best.Insert(cand_id[k], dx * dx + dy * dy);
+  }
+}
+
+// Merges the k best of a query's L threads (aligned groups of L lanes; each
+// point in at most one thread's list) into every thread of the group: k
+// rounds of a group minimum over the threads' heads, the winner's thread
+// dropping its head.
+template <int K, int L>
+__device__ __forceinline__ void MergeGroup(TopK<K>& best) {
+  double md[K];
+  int mid[K];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    double wd = best.d[0];
+    int wi = best.id[0];
+#pragma unroll
+    for (int m = 1; m < L; m <<= 1) {
+      const double od = __shfl_xor(wd, m, L);
+      const int oi = __shfl_xor(wi, m, L);
+      if (TopK<K>::Less(od, oi, wd, wi)) {
+        wd = od;
+        wi = oi;
+      }
+    }
+    md[r] = wd;
+    mid[r] = wi;
+    if (best.d[0] == wd && best.id[0] == wi) {
+#pragma unroll
+      for (int j = 0; j + 1 < K; ++j) {
+        best.d[j] = best.d[j + 1];
+        best.id[j] = best.id[j + 1];
+      }
+      best.d[K - 1] = __builtin_huge_val();
+      best.id[K - 1] = 0x7fffffff;
+    }
+  }
+  best.found = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    best.d[j] = md[j];
+    best.id[j] = mid[j];
+    best.found += md[j] < __builtin_huge_val() ? 1 : 0;
+  }
+}
+
+template <int K, int L, class GridSrc>
+__global__ void __launch_bounds__(kCellThreads)
+    KPointNormalsCell(const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ ox,
+                      const double* __restrict__ oy, GridSrc src, const int* __restrict__ start,
+                      const double* __restrict__ sx, const double* __restrict__ sy, const int* __restrict__ sid,
+                      double* __restrict__ nrm_x, double* __restrict__ nrm_y, double* __restrict__ corner) {
+  __shared__ double cand_x[kCellCandidates];
+  __shared__ double cand_y[kCellCandidates];
+  __shared__ int cand_id[kCellCandidates];
+  __shared__ int range_a[kCellRanges];
+  __shared__ int range_off[kCellRanges + 1];
+  __shared__ int num_ranges;
   const PointGridDev g = src.Load();
-  const double qx = px[i];
-  const double qy = py[i];
-  TopK<K> best;
-  KnnTop<K>(qx, qy, px, py, g, start, sorted, best);
-  // K <= n, so all K entries are points; sums in (distance, index) order.
-  double nbx[K];
-  double nby[K];
+  const int c = static_cast<int>(blockIdx.x);
+  if (c >= g.gx * g.gy) return;
+  const int q0 = start[c];
+  const int q1 = start[c + 1];
+  if (q1 == q0) return;
+  const int cx = c % g.gx;
+  const int cy = c / g.gx;
+  const int rmax = max(max(cx, g.gx - 1 - cx), max(cy, g.gy - 1 - cy));
+  const int tid = static_cast<int>(threadIdx.x);
+  // Lists the point ranges of rings r0 .. r1 into range_a / range_off, ring
+  // by ring (ring 0: the cell; ring r: its two rows as runs, then its
+  // 2 (2 r - 1) column cells), so that the nearest candidates come first
+  // and later ones rarely enter the k best; returns the candidate count.
+  auto ranges_before = [](int r) { return r == 0 ? 0 : 1 + 2 * r * (r - 1); };
+  auto list_rings = [&](int r0, int r1) -> int {
+    const int base0 = ranges_before(r0);
+    const int nr = ranges_before(r1 + 1) - base0;
+    __syncthreads();
+    for (int j = tid; j < nr; j += kCellThreads) {
+      int r = r0;
+      while (ranges_before(r + 1) - base0 <= j) ++r;
+      const int slot = j - (ranges_before(r) - base0);
+      int a = 0;
+      int b = 0;
+      if (r == 0) {
+        a = start[c];
+        b = start[c + 1];
+      } else if (slot < 2) {
+        const int row = slot == 0 ? cy - r : cy + r;
+        if (row >= 0 && row < g.gy) {
+          a = start[row * g.gx + max(cx - r, 0)];
+          b = start[row * g.gx + min(cx + r, g.gx - 1) + 1];
+        }
+      } else {
+        const int k = slot - 2;
+        const int yy = cy - r + 1 + (k >> 1);
+        const int xx = (k & 1) != 0 ? cx + r : cx - r;
+        if (yy >= 0 && yy < g.gy && xx >= 0 && xx < g.gx) {
+          a = start[yy * g.gx + xx];
+          b = start[yy * g.gx + xx + 1];
+        }
+      }
+      range_a[j] = a;
+      range_off[j + 1] = b - a;
+    }
+    if (tid == 0) num_ranges = nr;
+    __syncthreads();
+    if (tid == 0) {
+      range_off[0] = 0;
+      for (int k = 0; k < nr; ++k) range_off[k + 1] += range_off[k];
+    }
+    __syncthreads();
+    return range_off[nr];
+  };
+  // Stages candidates [pass, pass + count) of the listed ranges.
+  auto stage = [&](int pass, int count) {
+    __syncthreads();
+    const int nr = num_ranges;
+    for (int k = tid; k < count; k += kCellThreads) {
+      const int gi = pass + k;
+      int lo = 0;
+      int hi = nr - 1;
+      while (lo < hi) {  // the last range whose offset is <= gi
+        const int mid = (lo + hi + 1) >> 1;
+        if (range_off[mid] <= gi) {
+          lo = mid;
+        } else {
+          hi = mid - 1;
+        }
+      }
+      const int t = range_a[lo] + (gi - range_off[lo]);
+      cand_x[k] = sx[t];
+      cand_y[k] = sy[t];
+      cand_id[k] = sid[t];
+    }
+    __syncthreads();
+  };
+  constexpr int kQueries = kCellThreads / L;  // queries per pass, L threads each
+  const int sub = tid % L;
+  for (int qbase = q0; qbase < q1; qbase += kQueries) {
+    const int q = qbase + tid / L;
+    const bool active = q < q1;
+    const double qx = active ? sx[q] : 0.0;
+    const double qy = active ? sy[q] : 0.0;
+    TopK<K> best;
+    best.Init();
+    const int r_first = min(2, rmax);
+    int total = list_rings(0, r_first);
+    for (int pass = 0; pass < total; pass += kCellCandidates) {
+      const int count = min(kCellCandidates, total - pass);
+      stage(pass, count);
+      if (active) TakeStaged<K, L>(cand_x, cand_y, cand_id, sub, count, qx, qy, best);
+    }
+    MergeGroup<K, L>(best);
+    auto settled = [&](int r) {
+      if (r >= rmax) return true;  // every cell visited
+      if (best.found < K || r < 2) return false;
+      const double lim = static_cast<double>(r - 1) * g.cell;
+      return best.d[K - 1] < lim * lim;
+    };
+    bool done = !active || settled(r_first);
+    for (int r = r_first + 1; r <= rmax; ++r) {
+      if (__syncthreads_or(done ? 0 : 1) == 0) break;
+      if (4 * r > kCellRanges) {
+        // A ring wider than the range table: the unsettled queries finish
+        // on their own (first thread of each), ring by ring.
+        for (int rr = r; !done && sub == 0 && rr <= rmax; ++rr) {
+          for (int yy = max(cy - rr, 0); yy <= min(cy + rr, g.gy - 1); ++yy) {
+            const bool edge = yy == cy - rr || yy == cy + rr;
+            const int step = edge ? 1 : 2 * rr;
+            for (int xx = cx - rr; xx <= cx + rr; xx += step) {
+              if (xx < 0 || xx >= g.gx) continue;
+              const int cc = yy * g.gx + xx;
+              for (int t = start[cc]; t < start[cc + 1]; ++t) {
+                const double dx = sx[t] - qx;
+                const double dy = sy[t] - qy;
+                best.Insert(sid[t], dx * dx + dy * dy);
+              }
+            }
+          }
+          done = settled(rr);
+        }
+        break;
+      }
+      // The group's merged list stays with its first thread; the others
+      // collect the ring's candidates from empty lists.
+      if (sub != 0) best.Init();
+      total = list_rings(r, r);
+      for (int pass = 0; pass < total; pass += kCellCandidates) {
+        const int count = min(kCellCandidates, total - pass);
+        stage(pass, count);
+        if (!done) TakeStaged<K, L>(cand_x, cand_y, cand_id, sub, count, qx, qy, best);
+      }
+      MergeGroup<K, L>(best);
+      if (!done) done = settled(r);
+    }
+    if (!active || sub != 0) continue;
+    double nbx[K];
+    double nby[K];
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    nbx[j] = px[best.id[j]];
-    nby[j] = py[best.id[j]];
-  }
-  double mx = 0.0;
-  double my = 0.0;
+    for (int j = 0; j < K; ++j) {
+      nbx[j] = px[best.id[j]];
+      nby[j] = py[best.id[j]];
+    }
+    double mx = 0.0;
+    double my = 0.0;
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    mx = mx + nbx[j];
-    my = my + nby[j];
-  }
-  mx = mx / static_cast<double>(K);
-  my = my / static_cast<double>(K);
-  double a = 0.0;
-  double b = 0.0;
-  double c = 0.0;
+    for (int j = 0; j < K; ++j) {
+      mx = mx + nbx[j];
+      my = my + nby[j];
+    }
+    mx = mx / static_cast<double>(K);
+    my = my / static_cast<double>(K);
+    double a = 0.0;
+    double b = 0.0;
+    double cc = 0.0;
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    const double dx = nbx[j] - mx;
-    const double dy = nby[j] - my;
-    a += dx * dx;
-    b += dx * dy;
-    c += dy * dy;
+    for (int j = 0; j < K; ++j) {
+      const double dx = nbx[j] - mx;
+      const double dy = nby[j] - my;
+      a += dx * dx;
+      b += dx * dy;
+      cc += dy * dy;
+    }
+    const int i = sid[q];
+    double nx, ny, cr;
+    PcaFromSums(a, b, cc, qx, qy, ox[i], oy[i], nx, ny, cr);
+    nrm_x[i] = nx;
+    nrm_y[i] = ny;
+    corner[i] = cr;
   }
-  double nx, ny, cr;
-  PcaFromSums(a, b, c, qx, qy, ox[i], oy[i], nx, ny, cr);
-  nrm_x[i] = nx;
-  nrm_y[i] = ny;
-  corner[i] = cr;
 }
 
 // The same for any k up to kMaxK, with the k best in (scratch) arrays.
@@ -347,66 +619,59 @@ __global__ void KPointNormals(const double* __restrict__ px, const double* __res
   corner[i] = cr;
 }
 
-// Point normals with k = kk (2 <= kk <= the point count): the register
-// kernel for k up to 16, the general one above that.
+// Point normals with one workgroup per cell for k up to 16 (sx, sy, sid:
+// the points in cell order; `cells` workgroups, at least gx * gy), the
+// general kernel above that.
 template <class GridSrc>
-void LaunchPointNormals(unsigned blocks, hipStream_t s, const double* px, const double* py, const double* ox,
-                        const double* oy, int64_t n, int kk, GridSrc src, const int* start, const int* sorted,
-                        double* nrm_x, double* nrm_y, double* corner) {
-#define SDFSLAM_POINT_NORMALS_K(K)                                                                        \
-  case K:                                                                                              \
-    KPointNormalsK<K, GridSrc><<<blocks, kThreads, 0, s>>>(px, py, ox, oy, n, src, start, sorted, nrm_x, \
-                                                            nrm_y, corner);                             \
+void LaunchPointNormalsCells(hipStream_t s, const double* px, const double* py, const double* ox, const double* oy,
+                             int64_t n, int kk, GridSrc src, unsigned cells, const int* start, const int* sorted,
+                             const double* sx, const double* sy, double* nrm_x, double* nrm_y, double* corner) {
+#define SDFSLAM_POINT_NORMALS_CELL(K)                                                                       \
+  case K:                                                                                                \
+    KPointNormalsCell<K, 4, GridSrc><<<cells, kCellThreads, 0, s>>>(px, py, ox, oy, src, start, sx, sy, sorted, \
+                                                                 nrm_x, nrm_y, corner);                    \
     return;
   switch (kk) {
-    SDFSLAM_POINT_NORMALS_K(2)
-    SDFSLAM_POINT_NORMALS_K(3)
-    SDFSLAM_POINT_NORMALS_K(4)
-    SDFSLAM_POINT_NORMALS_K(5)
-    SDFSLAM_POINT_NORMALS_K(6)
-    SDFSLAM_POINT_NORMALS_K(7)
-    SDFSLAM_POINT_NORMALS_K(8)
-    SDFSLAM_POINT_NORMALS_K(9)
-    SDFSLAM_POINT_NORMALS_K(10)
-    SDFSLAM_POINT_NORMALS_K(11)
-    SDFSLAM_POINT_NORMALS_K(12)
-    SDFSLAM_POINT_NORMALS_K(13)
-    SDFSLAM_POINT_NORMALS_K(14)
-    SDFSLAM_POINT_NORMALS_K(15)
-    SDFSLAM_POINT_NORMALS_K(16)
-    default:
-      KPointNormals<GridSrc><<<blocks, kThreads, 0, s>>>(px, py, ox, oy, n, kk, src, start, sorted, nrm_x, nrm_y,
-                                                          corner);
+    SDFSLAM_POINT_NORMALS_CELL(2)
+    SDFSLAM_POINT_NORMALS_CELL(3)
+    SDFSLAM_POINT_NORMALS_CELL(4)
+    SDFSLAM_POINT_NORMALS_CELL(5)
+    SDFSLAM_POINT_NORMALS_CELL(6)
+    SDFSLAM_POINT_NORMALS_CELL(7)
+    SDFSLAM_POINT_NORMALS_CELL(8)
+    SDFSLAM_POINT_NORMALS_CELL(9)
+    SDFSLAM_POINT_NORMALS_CELL(10)
+    SDFSLAM_POINT_NORMALS_CELL(11)
+    SDFSLAM_POINT_NORMALS_CELL(12)
+    SDFSLAM_POINT_NORMALS_CELL(13)
+    SDFSLAM_POINT_NORMALS_CELL(14)
+    SDFSLAM_POINT_NORMALS_CELL(15)
+    SDFSLAM_POINT_NORMALS_CELL(16)
+    default: {
+      const auto pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
+      KPointNormals<GridSrc><<<pb, kThreads, 0, s>>>(px, py, ox, oy, n, kk, src, start, sorted, nrm_x, nrm_y,
+                                                      corner);
+    }
   }
-#undef SDFSLAM_POINT_NORMALS_K
+#undef SDFSLAM_POINT_NORMALS_CELL
 }
+
 
 // Normal and eikonal scale of every grid node from its nearest scan point
 // (AssignNodeNormals, normals.cpp:162-220).
-template <class GridSrc>
-__global__ void KNodeNormals(int nx, int ny, double min_x, double min_y, double dx, double dy,
-                             const double* __restrict__ px, const double* __restrict__ py,
-                             const double* __restrict__ pnx, const double* __restrict__ pny,
-                             const double* __restrict__ pcorner, GridSrc src, const int* __restrict__ start,
-                             const int* __restrict__ sorted, int method, double corner_threshold,
-                             double* __restrict__ out_nx, double* __restrict__ out_ny,
-                             double* __restrict__ out_scale) {
-  const int64_t node = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (node >= static_cast<int64_t>(nx) * ny) return;
-  const PointGridDev g = src.Load();
-  const int w = static_cast<int>(node % nx);
-  const int h = static_cast<int>(node / nx);
-  const double qx = min_x + w * dx;
-  const double qy = min_y + h * dy;
-  int nearest = 0;
-  double dist = 0.0;
-  Knn(qx, qy, 1, px, py, g, start, sorted, &nearest, &dist);
+// Normal and eikonal scale of a node at (qx, qy) from its nearest scan
+// point (AssignNodeNormals, normals.cpp:162-220).
+__device__ __forceinline__ void NodeNormalFrom(int nearest, double qx, double qy, const double* __restrict__ px,
+                                               const double* __restrict__ py, const double* __restrict__ pnx,
+                                               const double* __restrict__ pny, const double* __restrict__ pcorner,
+                                               int method, double corner_threshold, double& rx, double& ry,
+                                               double& scale) {
   const double n_x = pnx[nearest];
   const double n_y = pny[nearest];
   const double corner = fmin(fmax(pcorner[nearest] / corner_threshold, 0.0), 1.0);
-  double rx = n_x;
-  double ry = n_y;
-  double scale = 1.0;
+  rx = n_x;
+  ry = n_y;
+  scale = 1.0;
   if (method == SDFSLAM_NORMALS_HYBRID) {
     const double tx = px[nearest] - qx;
     const double ty = py[nearest] - qy;
@@ -429,9 +694,60 @@ __global__ void KNodeNormals(int nx, int ny, double min_x, double min_y, double 
   } else if (method == SDFSLAM_NORMALS_WEIGHTED) {
     scale = 1.0 - corner;
   }
+}
+
+// One thread per node.
+template <class GridSrc>
+__global__ void KNodeNormals(int nx, int ny, double min_x, double min_y, double dx, double dy,
+                             const double* __restrict__ px, const double* __restrict__ py,
+                             const double* __restrict__ pnx, const double* __restrict__ pny,
+                             const double* __restrict__ pcorner, GridSrc src, const int* __restrict__ start,
+                             const int* __restrict__ sorted, int method, double corner_threshold,
+                             double* __restrict__ out_nx, double* __restrict__ out_ny,
+                             double* __restrict__ out_scale) {
+  const int64_t node = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (node >= static_cast<int64_t>(nx) * ny) return;
+  const PointGridDev g = src.Load();
+  const int w = static_cast<int>(node % nx);
+  const int h = static_cast<int>(node / nx);
+  const double qx = min_x + w * dx;
+  const double qy = min_y + h * dy;
+  int nearest = 0;
+  double dist = 0.0;
+  Knn(qx, qy, 1, px, py, g, start, sorted, &nearest, &dist);
+  double rx, ry, scale;
+  NodeNormalFrom(nearest, qx, qy, px, py, pnx, pny, pcorner, method, corner_threshold, rx, ry, scale);
   out_nx[node] = rx;
   out_ny[node] = ry;
   out_scale[node] = scale;
+}
+
+// One wavefront per node (WaveKnn with k = 1; sx, sy, sid: the points in
+// cell order); lane 0 stores.
+template <class GridSrc>
+__global__ void __launch_bounds__(kThreads)
+    KNodeNormalsWave(int nx, int ny, double min_x, double min_y, double dx, double dy,
+                     const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pnx,
+                     const double* __restrict__ pny, const double* __restrict__ pcorner, GridSrc src,
+                     const int* __restrict__ start, const double* __restrict__ sx, const double* __restrict__ sy,
+                     const int* __restrict__ sid, int method, double corner_threshold, double* __restrict__ out_nx,
+                     double* __restrict__ out_ny, double* __restrict__ out_scale) {
+  const int64_t node = blockIdx.x * static_cast<int64_t>(kThreads / 64) + (threadIdx.x >> 6);
+  if (node >= static_cast<int64_t>(nx) * ny) return;
+  const PointGridDev g = src.Load();
+  const int w = static_cast<int>(node % nx);
+  const int h = static_cast<int>(node / nx);
+  const double qx = min_x + w * dx;
+  const double qy = min_y + h * dy;
+  TopK<1> best;
+  WaveKnn<1>(qx, qy, g, start, sx, sy, sid, best);
+  double rx, ry, scale;
+  NodeNormalFrom(best.id[0], qx, qy, px, py, pnx, pny, pcorner, method, corner_threshold, rx, ry, scale);
+  if (Lane() == 0) {
+    out_nx[node] = rx;
+    out_ny[node] = ry;
+    out_scale[node] = scale;
+  }
 }
 
 // Map seeding from scans (InitializeFromScans, map_init.cpp:31-69): every
@@ -574,7 +890,7 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
   const size_t ni = align(static_cast<size_t>(n) * sizeof(int));
   const size_t ci = align(static_cast<size_t>(cells + 1) * sizeof(int));
   const size_t vd = align(static_cast<size_t>(nodes) * sizeof(double));
-  const size_t need = 7 * nd + 2 * ni + 3 * ci + 3 * vd;
+  const size_t need = 9 * nd + 2 * ni + 3 * ci + 3 * vd;
   Lease lease;
   Workspace& ws = *lease.w;
   if (ws.stream == nullptr) HIPN(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
@@ -599,6 +915,8 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
   double* pnx = reinterpret_cast<double*>(take(nd));
   double* pny = reinterpret_cast<double*>(take(nd));
   double* pcorner = reinterpret_cast<double*>(take(nd));
+  double* sx = reinterpret_cast<double*>(take(nd));
+  double* sy = reinterpret_cast<double*>(take(nd));
   int* cell_of = reinterpret_cast<int*>(take(ni));
   int* sorted = reinterpret_cast<int*>(take(ni));
   int* count = reinterpret_cast<int*>(take(ci));
@@ -627,12 +945,13 @@ int NodeNormalsOnDevice(const double* points_xy, const double* origins_xy, int64
   const unsigned pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
   KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridByValue{g}, cell_of, count);
   KCellScan<<<1, kScanThreads, 0, s>>>(count, GridByValue{g}, start);
-  KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
-  LaunchPointNormals(pb, s, px, py, ox, oy, n, kk, GridByValue{g}, start, sorted, pnx, pny, pcorner);
-  const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
-  KNodeNormals<<<vb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, pnx,
-                                       pny, pcorner, GridByValue{g}, start, sorted, method, corner_threshold, vnx,
-                                       vny, vscale);
+  KCellScatterXY<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, px, py, sorted, sx, sy);
+  LaunchPointNormalsCells(s, px, py, ox, oy, n, kk, GridByValue{g}, static_cast<unsigned>(cells), start, sorted, sx,
+                         sy, pnx, pny, pcorner);
+  const auto wb = static_cast<unsigned>((nodes + kThreads / 64 - 1) / (kThreads / 64));
+  KNodeNormalsWave<<<wb, kThreads, 0, s>>>(grid.nx, grid.ny, grid.min_x, grid.min_y, grid.dx, grid.dy, px, py, pnx,
+                                           pny, pcorner, GridByValue{g}, start, sx, sy, sorted, method,
+                                           corner_threshold, vnx, vny, vscale);
   HIPN(hipGetLastError());
   std::vector<double> rx(static_cast<size_t>(nodes)), ry(static_cast<size_t>(nodes));
   const size_t vb_bytes = static_cast<size_t>(nodes) * sizeof(double);
@@ -940,7 +1259,7 @@ size_t EikonalNormalsWorkBytes(int64_t scan_points, int64_t nodes) {
   const size_t ni = Align256(sizeof(int) * static_cast<size_t>(std::max<int64_t>(scan_points, 1)));
   const size_t ci = Align256(sizeof(int) * static_cast<size_t>(max_cells + 1));
   const size_t vd = Align256(sizeof(double) * static_cast<size_t>(nodes));
-  return 256 + 7 * nd + 2 * ni + 3 * ci + 3 * vd;
+  return 256 + 9 * nd + 2 * ni + 3 * ci + 3 * vd;
 }
 
 namespace {
@@ -1107,6 +1426,8 @@ int EnqueueEikonalNormals(const EikonalNormalsJob& j, void* work, hipStream_t s)
   auto* pnx = reinterpret_cast<double*>(take(nd));
   auto* pny = reinterpret_cast<double*>(take(nd));
   auto* pcorner = reinterpret_cast<double*>(take(nd));
+  auto* sx = reinterpret_cast<double*>(take(nd));
+  auto* sy = reinterpret_cast<double*>(take(nd));
   auto* cell_of = reinterpret_cast<int*>(take(ni));
   auto* sorted = reinterpret_cast<int*>(take(ni));
   auto* count = reinterpret_cast<int*>(take(ci));
@@ -1118,18 +1439,20 @@ int EnqueueEikonalNormals(const EikonalNormalsJob& j, void* work, hipStream_t s)
   const int kk = static_cast<int>(std::min<int64_t>(std::max(2, j.k), n));
   const unsigned pb = static_cast<unsigned>((n + kThreads - 1) / kThreads);
   const unsigned cb = static_cast<unsigned>((max_cells + 1 + kThreads - 1) / kThreads);
-  const unsigned vb = static_cast<unsigned>((nodes + kThreads - 1) / kThreads);
   KScanWorld<<<pb, kThreads, 0, s>>>(j.scan_rows, n, j.frame, j.sensor_x, j.sensor_y, j.cos_sin, j.poses, px, py, ox,
                                       oy);
   KPointGrid<<<1, kGridThreads, 0, s>>>(px, py, n, max_cells, gp);
   KClearInts<<<cb, kThreads, 0, s>>>(count, fill, max_cells + 1);
   KCellCount<<<pb, kThreads, 0, s>>>(px, py, n, GridInMemory{gp}, cell_of, count);
   KCellScan<<<1, kScanThreads, 0, s>>>(count, GridInMemory{gp}, start);
-  KCellScatter<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, sorted);
-  LaunchPointNormals(pb, s, px, py, ox, oy, n, kk, GridInMemory{gp}, start, sorted, pnx, pny, pcorner);
-  KNodeNormals<<<vb, kThreads, 0, s>>>(j.nx, j.ny, j.min_x, j.min_y, j.dx, j.dy, px, py, pnx, pny, pcorner,
-                                       GridInMemory{gp}, start, sorted, j.method, j.corner_threshold, vnx, vny,
-                                       vscale);
+  KCellScatterXY<<<pb, kThreads, 0, s>>>(cell_of, n, start, fill, px, py, sorted, sx, sy);
+  // At most max_cells cells (KPointGrid): the workgroups past gx * gy exit.
+  LaunchPointNormalsCells(s, px, py, ox, oy, n, kk, GridInMemory{gp}, static_cast<unsigned>(max_cells), start, sorted,
+                         sx, sy, pnx, pny, pcorner);
+  const auto wb = static_cast<unsigned>((nodes + kThreads / 64 - 1) / (kThreads / 64));
+  KNodeNormalsWave<<<wb, kThreads, 0, s>>>(j.nx, j.ny, j.min_x, j.min_y, j.dx, j.dy, px, py, pnx, pny, pcorner,
+                                           GridInMemory{gp}, start, sx, sy, sorted, j.method, j.corner_threshold,
+                                           vnx, vny, vscale);
   if (j.num_eikonal > 0) {
     const unsigned eb = static_cast<unsigned>((j.num_eikonal + kThreads - 1) / kThreads);
     KEikonalNormals<<<eb, kThreads, 0, s>>>(j.eik_w, j.eik_h, j.num_eikonal, j.nx, vnx, vny, j.eik_nx, j.eik_ny);
