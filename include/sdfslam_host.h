@@ -289,6 +289,13 @@ typedef struct sdfslam_served_result {
   double latency_p50_after_first_ms;
   double latency_p99_after_first_ms;
   double first_increment_mean_ms;
+  /* 99th percentiles of the parts of an increment's latency: waiting for a
+   * preparer (the session's previous increment included), the preparation,
+   * waiting for a solver thread, and the batched solve call. */
+  double queue_p99_ms;
+  double prepare_p99_ms;
+  double solve_wait_p99_ms;
+  double solve_p99_ms;
 } sdfslam_served_result;
 int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam_served_result* out);
 

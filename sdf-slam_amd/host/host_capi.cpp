@@ -751,7 +751,8 @@ int sdfslam_host_served_benchmark(const sdfslam_served_options* options, sdfslam
             r.solve_mean_ms,    r.device_bytes,     r.batches,          r.mean_batch,
             r.prepare_batch_ms, r.solve_batch_ms,   r.collect_batch_ms, r.capture_batch_ms,
             r.analysis_batch_ms, r.epochs_rebuilt, r.latency_p50_after_first_ms,
-            r.latency_p99_after_first_ms, r.first_increment_mean_ms};
+            r.latency_p99_after_first_ms, r.first_increment_mean_ms, r.queue_p99_ms, r.prepare_p99_ms,
+            r.solve_wait_p99_ms, r.solve_p99_ms};
     return SDFSLAM_OK;
   });
 }

@@ -320,6 +320,12 @@ IncrementalRun RunIncremental(const Workload& wl, const SolverOptions& solver, i
     // runner.cpp:231-233: the first scan at the first dataset pose.
     InitializeFromScans(run.map, {wl.scans[0]}, {wl.dataset_poses[0]}, hooks.init_signed);
   }
+  if (!solve_fn && hooks.append_frames && AppendApplies(wl.problem, !wl.relations.empty())) {
+    // The device context is made over the first frame; every increment then
+    // appends its frames to it (HipSession::Append).
+    Problem first(run.map, run.estimated, {wl.scans[0]}, {}, wl.problem);
+    (void)session.Load(first, solver);
+  }
   size_t next_frame = 1;
   size_t increment = 0;
   while (next_frame < total_frames) {

@@ -91,6 +91,11 @@ struct IncrementRecord {
   double final_cost{0.0};
   double build_seconds{0.0};  // Problem construction
   double solve_seconds{0.0};  // the increment's Solve, transfers included
+  // Served increments: arrival to the start of the preparation, the
+  // preparation, and prepared to the start of the batched solve.
+  double queue_seconds{0.0};
+  double prepare_seconds{0.0};
+  double solve_wait_seconds{0.0};
 };
 
 struct IncrementalRun {

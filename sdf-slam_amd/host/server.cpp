@@ -103,6 +103,25 @@ int Server::OpenSession(GridMap map, const ProblemOptions& problem, const Solver
   const int id = static_cast<int>(sessions_.size()) - 1;
   waiting_.push_back(raw);
   AdmitWaiting();
+  if (raw->admitted && AppendApplies(raw->problem, false)) {
+    // The session's device context is made now, over its first frame, so
+    // that its increments append frames (no context setup on the first).
+    raw->busy = true;
+    lock.unlock();
+    int64_t bytes = 0;
+    try {
+      Problem first(raw->map, raw->estimated, raw->scans, {}, raw->problem);
+      (void)raw->hip.Load(first, raw->step);
+      bytes = raw->hip.DeviceBytes();
+    } catch (const std::exception&) {
+      raw->hip.Release();  // the first increment builds it instead
+    }
+    lock.lock();
+    raw->device_bytes = std::max(raw->device_bytes, bytes);
+    raw->busy = false;
+    work_cv_.notify_all();
+    done_cv_.notify_all();
+  }
   return id;
 }
 
@@ -293,6 +312,7 @@ void Server::SolveBatch(std::vector<std::unique_ptr<Prepared>>& batch) {
     results[static_cast<size_t>(j)].cost_history_capacity = static_cast<int32_t>(hist[static_cast<size_t>(j)].size());
   }
   const auto t0 = Clock::now();
+  for (auto& item : batch) item->record.solve_wait_seconds = Seconds(item->prepared, t0);
   sdfslam_hip_batch_info info{};
   const int st = sdfslam_hip_batch_solve(ctxs.data(), nl, &copts, results.data(), status.data(), &info);
   const double solve_s = Seconds(t0, Clock::now());
@@ -390,10 +410,13 @@ void Server::PrepareLoop() {
     item->session = pick;
     item->arrival = frame.arrival;
     const auto t0 = Clock::now();
+    item->record.queue_seconds = Seconds(frame.arrival, t0);
     Prepare(*item, frame);
+    item->prepared = Clock::now();
+    item->record.prepare_seconds = Seconds(t0, item->prepared);
     {
       std::lock_guard<std::mutex> stats_lock(stats_mu_);
-      stats_.prepare_seconds += Seconds(t0, Clock::now());
+      stats_.prepare_seconds += item->record.prepare_seconds;
     }
     lock.lock();
     if (!item->error.empty()) {
@@ -525,8 +548,13 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   int64_t first_count = 0;
   double build = 0.0;
   double solve = 0.0;
+  std::vector<double> queue_ms, prepare_ms, wait_ms, solve_ms;
   for (const auto& [sid, ticket] : tickets) {
     const IncrementRecord rec = server.Wait(sid, ticket);
+    queue_ms.push_back(1e3 * rec.queue_seconds);
+    prepare_ms.push_back(1e3 * rec.prepare_seconds);
+    wait_ms.push_back(1e3 * rec.solve_wait_seconds);
+    solve_ms.push_back(1e3 * rec.solve_seconds);
     lat.push_back(1e3 * server.Latency(sid, ticket));
     if (ticket == 1) {
       first_sum += lat.back();
@@ -544,6 +572,10 @@ ServedResult RunServedBenchmark(const ServedBenchmark& o) {
   r.increments_per_second = r.increments / std::max(r.wall_seconds, 1e-9);
   r.latency_p50_ms = Percentile(lat, 0.50);
   r.latency_p99_ms = Percentile(lat, 0.99);
+  r.queue_p99_ms = Percentile(queue_ms, 0.99);
+  r.prepare_p99_ms = Percentile(prepare_ms, 0.99);
+  r.solve_wait_p99_ms = Percentile(wait_ms, 0.99);
+  r.solve_p99_ms = Percentile(solve_ms, 0.99);
   r.latency_p50_after_first_ms = Percentile(lat_after_first, 0.50);
   r.latency_p99_after_first_ms = Percentile(lat_after_first, 0.99);
   r.first_increment_mean_ms = first_count > 0 ? first_sum / static_cast<double>(first_count) : 0.0;

@@ -139,6 +139,7 @@ class Server {
   struct Prepared {
     Session* session{nullptr};
     std::chrono::steady_clock::time_point arrival;
+    std::chrono::steady_clock::time_point prepared;
     std::unique_ptr<Problem> problem;
     sdfslam_hip_context* ctx{nullptr};
     IncrementRecord record;
@@ -223,6 +224,10 @@ struct ServedResult {
   double latency_p50_after_first_ms{0.0};  // each session's first increment left out
   double latency_p99_after_first_ms{0.0};
   double first_increment_mean_ms{0.0};
+  double queue_p99_ms{0.0};       // arrival to the start of its preparation
+  double prepare_p99_ms{0.0};     // the preparation
+  double solve_wait_p99_ms{0.0};  // prepared to the start of its batched solve
+  double solve_p99_ms{0.0};       // the batched solve call
 };
 ServedResult RunServedBenchmark(const ServedBenchmark& options);
 

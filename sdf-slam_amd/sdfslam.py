@@ -135,7 +135,8 @@ class ServedResult(C.Structure):
                 ("collect_batch_ms", C.c_double), ("capture_batch_ms", C.c_double),
                 ("analysis_batch_ms", C.c_double), ("epochs_rebuilt", C.c_int64),
                 ("latency_p50_after_first_ms", C.c_double), ("latency_p99_after_first_ms", C.c_double),
-                ("first_increment_mean_ms", C.c_double)]
+                ("first_increment_mean_ms", C.c_double), ("queue_p99_ms", C.c_double),
+                ("prepare_p99_ms", C.c_double), ("solve_wait_p99_ms", C.c_double), ("solve_p99_ms", C.c_double)]
 
 
 class NormalsRequest(C.Structure):
