@@ -255,6 +255,9 @@ def main():
     ap.add_argument("--plain-duration", type=float, default=2.0)
     ap.add_argument("--workers", type=int, default=12)
     ap.add_argument("--incremental-frames", type=int, default=150)
+    ap.add_argument("--profile-run", action="store_true",
+                    help="for kernel profiles: the iterations run as direct launches with per-launch "
+                         "event timing only (no graph replays, no other forms, no CPU baseline)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the incremental, served and batched legs")
     ap.add_argument("--calibration-copy-mb", type=int, default=0,
@@ -278,10 +281,12 @@ def main():
     t_build = time.perf_counter() - t_build
     print(f"[bench] built {args.workload}: n={problem.num_parameters} rows={problem.num_residuals} "
           f"in {t_build:.2f} s", file=sys.stderr, flush=True)
-    ctx.iterate(args.warmup)
+    if args.profile_run:
+        args.no_forms = args.no_cpu_baseline = True
+    ctx.iterate(args.warmup, phase_times=args.profile_run)
     info0 = ctx.analysis()
     t0 = time.perf_counter()
-    ctx.iterate(args.steps)
+    ctx.iterate(args.steps, phase_times=args.profile_run)
     elapsed = time.perf_counter() - t0
     info1 = ctx.analysis()
     reanalyses = info1["pattern_epochs"] - info0["pattern_epochs"]
