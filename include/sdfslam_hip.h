@@ -235,6 +235,11 @@ int sdfslam_hip_context_append_frames(sdfslam_hip_context* ctx, int32_t num_fram
 int sdfslam_hip_context_point_rows(sdfslam_hip_context* ctx, int64_t capacity, int64_t* num_points,
                                    int32_t* frame, double* x, double* y, double* expected_sdf,
                                    double* sqrt_weight);
+/* Process-wide: LM iterations as captured HIP graphs (enabled != 0, the
+ * default) or as direct launches of the same kernels in the same order
+ * (the same results; for kernel profilers and debuggers that do not follow
+ * graph launches). No counterpart in the reference. */
+int sdfslam_hip_set_graphs(int32_t enabled);
 /* Footprint margin (cells) of every pose in the current pattern epoch:
  * min(capacity, num_poses) entries into margins (may be NULL), the pose
  * count into *num_poses (may be NULL). */

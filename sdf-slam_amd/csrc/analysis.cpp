@@ -94,7 +94,8 @@ void ParallelFor(int64_t n, int threads, int64_t grain, F&& fn) {
 /// lists[src], builds ptr (num_tgt + 1) and out such that each target's
 /// sources are ascending. Only sources in [src_begin, num_src) take part.
 void InvertLists(const std::vector<std::vector<int32_t>>& lists, int src_begin, int num_tgt,
-                 int threads, std::vector<int64_t>& ptr, std::vector<int32_t>& out) {
+                 int threads, std::vector<int64_t>& ptr, std::vector<int32_t>& out,
+                 std::vector<int32_t>* pos = nullptr) {
   const int num_src = static_cast<int>(lists.size());
   const int span = std::max(num_src - src_begin, 0);
   const int chunks = std::max(1, std::min(threads, span / 16));
@@ -117,10 +118,16 @@ void InvertLists(const std::vector<std::vector<int32_t>>& lists, int src_begin, 
     ptr[static_cast<size_t>(t) + 1] = run;
   }
   out.resize(static_cast<size_t>(run));
+  if (pos != nullptr) pos->resize(static_cast<size_t>(run));
   RunChunks(chunks, [&](int k) {
     int64_t* c = cnt.data() + static_cast<size_t>(k) * static_cast<size_t>(num_tgt);
     for (int64_t s = src_begin + Bound(span, chunks, k); s < src_begin + Bound(span, chunks, k + 1); ++s) {
-      for (const int32_t t : lists[static_cast<size_t>(s)]) out[static_cast<size_t>(c[t]++)] = static_cast<int32_t>(s);
+      const std::vector<int32_t>& list = lists[static_cast<size_t>(s)];
+      for (size_t i = 0; i < list.size(); ++i) {
+        const int64_t at = c[list[i]]++;
+        out[static_cast<size_t>(at)] = static_cast<int32_t>(s);
+        if (pos != nullptr) (*pos)[static_cast<size_t>(at)] = static_cast<int32_t>(i);
+      }
     }
   });
 }
@@ -339,7 +346,21 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   }
   // Candidate frames per cell: every frame whose dilated cells contain it
   // (ascending), for the per-cell gather of the point-row partial sums.
-  InvertLists(dilated, 0, a.num_cells, threads, a.cp_ptr, a.cp_pose);
+  InvertLists(dilated, 0, a.num_cells, threads, a.cp_ptr, a.cp_pose, &a.cp_rank);
+  a.dl_ptr.assign(static_cast<size_t>(N) + 1, 0);
+  for (int P = 0; P < N; ++P) {
+    a.dl_ptr[static_cast<size_t>(P) + 1] = a.dl_ptr[static_cast<size_t>(P)] +
+                                           static_cast<int64_t>(dilated[static_cast<size_t>(P)].size());
+  }
+  a.fp_cellrank.resize(a.fp_cellbits.size());
+  ParallelFor(N, threads, 16, [&](int64_t P) {
+    const size_t w0 = static_cast<size_t>(P) * static_cast<size_t>(a.words_per_pose);
+    int32_t run = 0;
+    for (int w = 0; w < a.words_per_pose; ++w) {
+      a.fp_cellrank[w0 + static_cast<size_t>(w)] = run;
+      run += __builtin_popcount(a.fp_cellbits[w0 + static_cast<size_t>(w)]);
+    }
+  });
   // Inverse footprint: poses >= 1 per node (ascending).
   std::vector<int64_t> np_ptr;
   std::vector<int32_t> np_pose;

@@ -72,9 +72,15 @@ struct Analysis {
   // Dilated cell set of each pose as a bitmap (num_poses x words_per_pose).
   int words_per_pose{0};
   std::vector<uint32_t> fp_cellbits;
-  // Frames whose dilated cells contain each cell (ascending).
+  // Frames whose dilated cells contain each cell (ascending), and the
+  // cell's rank among that frame's dilated cells (ascending cell ids).
   std::vector<int64_t> cp_ptr;
   std::vector<int32_t> cp_pose;
+  std::vector<int32_t> cp_rank;
+  // Each frame's dilated cells as a range of ranks (dl_ptr[P] .. dl_ptr[P + 1])
+  // and, per word of its cell bitmap, the number of dilated cells before it.
+  std::vector<int64_t> dl_ptr;
+  std::vector<int32_t> fp_cellrank;
   // Point rows of each pose (ascending).
   std::vector<int64_t> pr_ptr;
   std::vector<int32_t> pr_row;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <climits>
@@ -363,6 +364,15 @@ thread_local ThreadStreams t_streams;
 // A second stream per thread and device: the forward solves of the lower
 // levels run on it beside the factorization of the upper ones.
 thread_local ThreadStreams t_aux_streams;
+// Two more streams per thread and device: a batched solve runs its
+// contexts' deferred work (uploads, device normals) on them and on the
+// thread's stream side by side.
+constexpr int kPendingStreams = 3;
+// Iterations as captured graphs (the default) or as direct launches
+// (sdfslam_hip_set_graphs; for kernel profilers).
+std::atomic<bool> g_graphs{true};
+thread_local ThreadStreams t_pending_streams[kPendingStreams - 1];
+thread_local std::vector<hipEvent_t> t_pending_events;
 
 /// `n` events without timing in `ev` (created on first use, kept).
 const hipEvent_t* OverlapEvents(std::vector<hipEvent_t>& ev, size_t n) {
@@ -528,6 +538,11 @@ struct sdfslam_hip_context {
   int64_t gemms_timed{0};
   int timed_iterations{0};
   bool lm_ready{false};
+  // An update's LM initialisation (InitLm with this lambda), run after its
+  // deferred work by the context's next call; a batched solve, which
+  // initialises every slot itself, drops it.
+  bool init_pending{false};
+  double init_lambda{0.0};
   BatchSpace batch;  // buffers of the batches this context leads
   // Speculative pattern epochs: the poses the current epoch's footprints
   // were computed at, how far each pose's points reach from it (metres),
@@ -588,13 +603,24 @@ void Enqueue(sdfslam_hip_context& c, std::function<void(hipStream_t)> work) {
 
 /// Makes the calling thread's stream the context's stream and runs the
 /// context's deferred work on it.
+void InitLm(sdfslam_hip_context& c, double lambda_init);
+
+/// Runs the context's deferred work on `s` (in order).
+void RunPending(sdfslam_hip_context& c, hipStream_t s) {
+  std::vector<std::function<void(hipStream_t)>> work;
+  work.swap(c.pending);
+  for (auto& w : work) w(s);
+}
+
 void Bind(sdfslam_hip_context& c) {
   HIPC(hipSetDevice(c.device));
   c.stream = t_streams.Get(c.device);
   c.aux_stream = nullptr;  // the calling thread's second stream, taken when needed
-  std::vector<std::function<void(hipStream_t)>> work;
-  work.swap(c.pending);
-  for (auto& w : work) w(c.stream);
+  RunPending(c, c.stream);
+  if (c.init_pending) {
+    c.init_pending = false;
+    InitLm(c, c.init_lambda);
+  }
 }
 
 /// How far each pose's points reach from it (metres), over rows [from, end)
@@ -1020,6 +1046,9 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     D.pr_row = E.Upload(an.pr_row);
     D.cp_ptr = E.Upload(an.cp_ptr);
     D.cp_pose = E.Upload(an.cp_pose);
+    D.cp_rank = E.Upload(an.cp_rank);
+    D.dl_ptr = E.Upload(an.dl_ptr);
+    D.fp_cellrank = E.Upload(an.fp_cellrank);
     D.pd_off = E.Upload(an.pd_off);
     D.po_ptr = E.Upload(an.po_ptr);
     D.po_spec = E.Upload(an.po_spec);
@@ -1042,6 +1071,7 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     c.solve_sync_words = static_cast<size_t>(inv_off.back() / (64 * 64) + 3 * an.num_sn);
     c.dw.solve_sync = E.Upload(std::vector<int32_t>(c.solve_sync_words, 0));
     c.dw.vec = E.Alloc<double>(static_cast<size_t>(an.sn_idx_ptr.back()));
+    c.dw.seg_map = E.Alloc<int32_t>(an.cp_pose.size());
     // The launch plans and the slot come last: they hold addresses placed
     // above (known in the placing pass).
     if (E.measuring()) {
@@ -1318,7 +1348,7 @@ LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations
         HIPC(hipStreamSynchronize(c.stream));
         AccumulateTimes(c);
       }
-    } else if (target - st.iter < kGraphMinIterations && !GraphValid(c, o)) {
+    } else if (!g_graphs.load() || (target - st.iter < kGraphMinIterations && !GraphValid(c, o))) {
       // Too few iterations left to repay a capture: launch directly.
       for (int k = 0; k < chunk; ++k) EnqueueIteration(c, o, false);
     } else {
@@ -1356,6 +1386,44 @@ LmState RunIterations(sdfslam_hip_context& c, const LmOptions& o, int iterations
 /// context keeps its own LM state, pattern epochs and results: the same as
 /// solving it alone. The table, plans and state buffers live in the lead
 /// context and are reused by its later batches.
+/// The calling thread's stream becomes every context's stream; their
+/// deferred work runs on it and on kPendingStreams - 1 more streams side by
+/// side (each context's in order on one stream; forked from and joined into
+/// the thread's stream with events). Their LM initialisation is dropped:
+/// the batched solve initialises every slot.
+void RunBatchPending(sdfslam_hip_context* const* ctxs, int count) {
+  const int dev = ctxs[0]->device;
+  HIPC(hipSetDevice(dev));
+  const hipStream_t main = t_streams.Get(dev);
+  std::vector<sdfslam_hip_context*> work;
+  for (int b = 0; b < count; ++b) {
+    sdfslam_hip_context& c = *ctxs[b];
+    c.stream = main;
+    c.aux_stream = nullptr;
+    c.init_pending = false;
+    DropSpeculation(c);
+    if (!c.pending.empty()) work.push_back(&c);
+  }
+  if (work.size() <= 1) {
+    for (sdfslam_hip_context* c : work) RunPending(*c, main);
+    return;
+  }
+  std::vector<hipEvent_t>& ev = t_pending_events;
+  OverlapEvents(ev, kPendingStreams);
+  HIPC(hipEventRecord(ev[0], main));
+  hipStream_t streams[kPendingStreams];
+  streams[0] = main;
+  for (int k = 1; k < kPendingStreams; ++k) {
+    streams[k] = t_pending_streams[k - 1].Get(dev);
+    HIPC(hipStreamWaitEvent(streams[k], ev[0], 0));
+  }
+  for (size_t i = 0; i < work.size(); ++i) RunPending(*work[i], streams[i % kPendingStreams]);
+  for (int k = 1; k < kPendingStreams; ++k) {
+    HIPC(hipEventRecord(ev[static_cast<size_t>(k)], streams[k]));
+    HIPC(hipStreamWaitEvent(main, ev[static_cast<size_t>(k)], 0));
+  }
+}
+
 class BatchRun {
  public:
   BatchRun(sdfslam_hip_context* const* ctxs, int count)
@@ -1407,7 +1475,13 @@ class BatchRun {
       }
       if (remaining <= 0) break;
       const int chunk = std::min(remaining, 8);
-      for (int i = 0; i < chunk; ++i) HIPC(hipGraphLaunch(graph_, stream_));
+      for (int i = 0; i < chunk; ++i) {
+        if (graph_ != nullptr) {
+          HIPC(hipGraphLaunch(graph_, stream_));
+        } else {
+          EnqueueIteration();
+        }
+      }
       info_.graph_launches += chunk;
       ReadAll(states);
       bool changed = false;
@@ -1458,6 +1532,17 @@ class BatchRun {
   }
 
  private:
+  /// One LM iteration of every slot, enqueued on the batch's stream.
+  void EnqueueIteration() {
+    const int k = static_cast<int>(ctxs_.size());
+    const SlotArgs args{space_.table, k == 1 ? &ctxs_[0]->slot : nullptr};
+    LaunchAssemble(args, shape_, true, stream_);
+    LaunchNormalSlots(args, shape_, options_, stream_);
+    LaunchFactor(args, shape_, stream_);
+    LaunchSolve(args, shape_, stream_);
+    LaunchStepAndControl(args, shape_, options_, stream_);
+  }
+
   /// Slot table, merged shape and the captured iteration for the current
   /// buffers of every context.
   void Rebuild() {
@@ -1485,15 +1570,13 @@ class BatchRun {
       HIPC(hipGraphExecDestroy(graph_));
       graph_ = nullptr;
     }
-    const LmOptions o = options_;
+    if (!g_graphs.load()) {
+      info_.capture_seconds += SecondsSince(t_capture);
+      return;
+    }
     hipGraph_t g = nullptr;
-    const SlotArgs args{space_.table, k == 1 ? &ctxs_[0]->slot : nullptr};
     HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
-    LaunchAssemble(args, shape_, true, stream_);
-    LaunchNormalSlots(args, shape_, o, stream_);
-    LaunchFactor(args, shape_, stream_);
-    LaunchSolve(args, shape_, stream_);
-    LaunchStepAndControl(args, shape_, o, stream_);
+    EnqueueIteration();
     HIPC(hipStreamEndCapture(stream_, &g));
     const hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1659,6 +1742,7 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
   // Work an earlier update deferred is superseded (the new layout uploads
   // everything again).
   c.pending.clear();
+  c.init_pending = false;
   struct Deferring {
     sdfslam_hip_context& c;
     explicit Deferring(sdfslam_hip_context& ctx) : c(ctx) { c.deferring = true; }
@@ -1670,8 +1754,8 @@ void UpdateContext(sdfslam_hip_context& c, const sdfslam_problem_view* problem,
   AllocateBase(c, map_values, poses);
   if (normals != nullptr) StageDeviceNormals(c, *normals, poses);
   BuildEpoch(c, map_values, poses);
-  const double lambda = c.opts.lambda_init;
-  Enqueue(c, [&c, lambda](hipStream_t) { InitLm(c, lambda); });
+  c.init_pending = true;
+  c.init_lambda = c.opts.lambda_init;
   c.lm_ready = true;
 }
 
@@ -1720,6 +1804,7 @@ void AppendFrames(sdfslam_hip_context& c, int32_t frames, const int64_t* counts,
   // superseded; the rows it would have written are uploaded again.
   const bool rows_on_device = c.pending.empty();
   c.pending.clear();
+  c.init_pending = false;
   const auto old_rows = static_cast<int64_t>(hp.pt_frame.size());
   const auto old_list = static_cast<int64_t>(c.scan_list.size());
   // Host copy of the new rows (Problem::Problem, problem.cpp:26-37, with
@@ -1915,8 +2000,8 @@ void AppendFrames(sdfslam_hip_context& c, int32_t frames, const int64_t* counts,
   nr.frame_cos_sin = cs.data();
   StageDeviceNormals(c, nr, poses);
   BuildEpoch(c, map_values, poses);
-  const double lambda = c.opts.lambda_init;
-  Enqueue(c, [&c, lambda](hipStream_t) { InitLm(c, lambda); });
+  c.init_pending = true;
+  c.init_lambda = c.opts.lambda_init;
   c.lm_ready = true;
 }
 
@@ -2050,6 +2135,11 @@ int sdfslam_hip_context_point_rows(sdfslam_hip_context* ctx, int64_t capacity, i
     HIPC(hipStreamSynchronize(c.stream));
     return SDFSLAM_OK;
   });
+}
+
+int sdfslam_hip_set_graphs(int32_t enabled) {
+  g_graphs.store(enabled != 0);
+  return SDFSLAM_OK;
 }
 
 int sdfslam_hip_pose_margins(sdfslam_hip_context* ctx, int32_t* margins, int32_t capacity,
@@ -2431,10 +2521,7 @@ int sdfslam_hip_batch_solve(sdfslam_hip_context* const* contexts, int32_t count,
       }
     }
     const auto t0 = std::chrono::steady_clock::now();
-    for (int b = 0; b < count; ++b) {
-      Bind(*contexts[b]);
-      DropSpeculation(*contexts[b]);
-    }
+    RunBatchPending(contexts, count);
     BatchRun run(contexts, count);
     std::vector<int> status;
     std::vector<LmState> states;

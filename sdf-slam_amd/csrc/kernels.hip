@@ -340,18 +340,15 @@ __global__ void KCellGather(Slots slots) {
   double acc[14];
 #pragma unroll
   for (int k = 0; k < 14; ++k) acc[k] = 0.0;
+  // Candidate frame q holds the cell at rank cp_rank[q] of its dilated
+  // cells; KPoseBlock mapped the rank to the frame's segment of the cell.
   for (int64_t q = A.cp_ptr[cell] + lane; q < A.cp_ptr[cell + 1]; q += 64) {
     const int pose = A.cp_pose[q];
-    const int64_t rb = A.pr_ptr[pose];
-    int lo = 0;
-    int hi = W.seg_count[pose];
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (W.seg_cell[rb + mid] < cell) lo = mid + 1; else hi = mid;
-    }
-    if (lo < W.seg_count[pose] && W.seg_cell[rb + lo] == cell) {
+    const int u = W.seg_map[A.dl_ptr[pose] + A.cp_rank[q]];
+    if (u >= 0) {
+      const int64_t rb = A.pr_ptr[pose];
 #pragma unroll
-      for (int k = 0; k < 14; ++k) acc[k] += W.seg_part[k * np + rb + lo];
+      for (int k = 0; k < 14; ++k) acc[k] += W.seg_part[k * np + rb + u];
     }
   }
 #pragma unroll
@@ -618,6 +615,25 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
     for (int u = tid; u < num_u; u += kThreads) {
       const int c = ucell[u];
       atomicOr(&occupied[c >> 5], 1u << (c & 31));
+    }
+  }
+  // The frame's dilated cells, by rank: their segment, or -1 where the frame
+  // has no row (KCellGather reads these). A cell outside the dilated cells
+  // (a point that left the footprint, which ends the pattern epoch) has no
+  // rank and is left out.
+  int32_t* smap = W.seg_map + A.dl_ptr[pose];
+  const int64_t nrank = A.dl_ptr[pose + 1] - A.dl_ptr[pose];
+  for (int64_t r = tid; r < nrank; r += kThreads) smap[r] = -1;
+  __syncthreads();
+  {
+    const uint32_t* bits = A.fp_cellbits + static_cast<int64_t>(pose) * A.words_per_pose;
+    const int32_t* wrank = A.fp_cellrank + static_cast<int64_t>(pose) * A.words_per_pose;
+    for (int u = tid; u < num_u; u += kThreads) {
+      const int c = ucell[u];
+      const uint32_t word = bits[c >> 5];
+      const uint32_t bit = 1u << (c & 31);
+      if ((word & bit) == 0u) continue;
+      smap[wrank[c >> 5] + __popc(word & (bit - 1u))] = u;
     }
   }
   __syncthreads();

@@ -100,6 +100,7 @@ struct DeviceWork {
   double* seg_part;    // 14 x num_points: per occupied cell, 10 weight products + 4 weight x residual
   double* seg_jp;      // 12 x num_points: per occupied cell, the 3 pose partials x each of the 4 weights
   int32_t* seg_count;  // occupied cells per frame
+  int32_t* seg_map;    // per frame and dilated-cell rank: the cell's segment (index in seg_cell) or -1
   double* nn;  // 10 x num_cells
   double* rn;  // 4 x num_cells
   double* rhs;  // -J^T r in original column order
@@ -149,6 +150,9 @@ struct DeviceAnalysis {
   const int32_t* pr_row;
   const int64_t* cp_ptr;  // candidate frames of each cell
   const int32_t* cp_pose;
+  const int32_t* cp_rank;      // the cell's rank among the candidate frame's dilated cells
+  const int64_t* dl_ptr;       // per frame: its range of dilated-cell ranks
+  const int32_t* fp_cellrank;  // per frame and bitmap word: dilated cells before the word
   const int64_t* pd_off;
   const int64_t* po_ptr;
   const int32_t* po_spec;

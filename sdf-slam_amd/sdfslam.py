@@ -199,7 +199,7 @@ HIP_SYMBOLS = [
     "sdfslam_hip_pose_margins", "sdfslam_hip_node_normals", "sdfslam_hip_init_from_scans",
     "sdfslam_hip_init_from_scans_noalloc", "sdfslam_hip_reference_factor_counts",
     "sdfslam_hip_elimination_order", "sdfslam_hip_context_update_device_normals",
-    "sdfslam_hip_context_append_frames", "sdfslam_hip_context_point_rows",
+    "sdfslam_hip_context_append_frames", "sdfslam_hip_context_point_rows", "sdfslam_hip_set_graphs",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -873,6 +873,12 @@ class Server:
         _check_host(host.sdfslam_server_state(self._h, session, frames.value, C.byref(frames),
                                               ptr(poses, _f64p), ptr(m, _f64p)))
         return poses.reshape(-1, 3), m
+
+
+def set_graphs(enabled):
+    """LM iterations as captured HIP graphs (default) or direct launches
+    (sdfslam_hip_set_graphs), process-wide."""
+    _check_hip(hip.sdfslam_hip_set_graphs(1 if enabled else 0))
 
 
 def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=12, max_batch=64,
