@@ -510,6 +510,28 @@ __global__ void KNodeSlots(Slots slots, int marquardt) {
 // and its -J^T r entries. One workgroup per pose; the pose's point rows are
 // sorted by cell in LDS (bitonic) so every entry sums its points in a fixed
 // order.
+// Exclusive scan of one value per thread over a kThreads-thread workgroup
+// (Hillis-Steele in `buf`, kThreads + 1 entries): buf[tid] receives the sum
+// over threads before tid; returns the total. Contains barriers.
+__device__ __forceinline__ int BlockExclusiveScan(int32_t* buf, int v) {
+  const int tid = static_cast<int>(threadIdx.x);
+  __syncthreads();
+  buf[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kThreads; off <<= 1) {
+    const int t = tid >= off ? buf[tid - off] : 0;
+    __syncthreads();
+    buf[tid] += t;
+    __syncthreads();
+  }
+  const int total = buf[kThreads - 1];
+  const int excl = buf[tid] - v;
+  __syncthreads();
+  buf[tid] = excl;
+  __syncthreads();
+  return total;
+}
+
 template <class Slots>
 __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   BIND_SLOT(blockIdx.y);
@@ -524,15 +546,13 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   int32_t* ustart = ucell + sort_size;
   __shared__ int32_t counts[kThreads + 1];
   __shared__ double red[9][kThreads];
-  // Until the diagonal block's reduction, `red` holds a bitmap of the
-  // frame's occupied cells (grids of up to 9 * kThreads * 64 cells), so
-  // footprint nodes with no occupied cell around them skip the searches.
+  // Until the diagonal block's reduction `red` is scratch: the counting
+  // sort's rank bitmap and fill counts, then a bitmap of the frame's
+  // occupied cells (grids of up to 9 * kThreads * 64 cells), so footprint
+  // nodes with no occupied cell around them skip the searches.
   uint32_t* occupied = reinterpret_cast<uint32_t*>(&red[0][0]);
   const int occ_words = (P.num_cells + 31) / 32;
   const bool use_occ = occ_words <= 9 * kThreads * 2;
-  if (use_occ) {
-    for (int i = tid; i < occ_words; i += kThreads) occupied[i] = 0u;
-  }
   const int64_t rb = A.pr_ptr[pose];
   const int m = static_cast<int>(A.pr_ptr[pose + 1] - rb);
   // This frame's sort size: its row count rounded up to a power of two
@@ -542,76 +562,205 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   sort_size = min(sort_size, ss);
   const int64_t np = P.num_points;
   const uint32_t invalid = static_cast<uint32_t>(P.num_cells);
-  for (int i = tid; i < sort_size; i += kThreads) {
-    if (i < m) {
+  // The frame's rows grouped by cell, cells ascending and rows in their
+  // frame order within a cell: order[t] is the local index (into rows) of
+  // the t-th grouped row, ucell[u] / ustart[u] the u-th occupied cell and
+  // its first position (ustart[num_u] = the valid rows). Rows without a
+  // Jacobian row (cell key num_cells) are left out.
+  int32_t* order = reinterpret_cast<int32_t*>(keys);
+  const int64_t nrank = A.dl_ptr[pose + 1] - A.dl_ptr[pose];
+  const int rwords = static_cast<int>((nrank + 31) / 32);
+  const int rgroups = (rwords + 7) / 8;  // groups of 8 bitmap words (256 ranks)
+  constexpr int kRedWords = 9 * kThreads * 2;
+  int num_u = 0;
+  if (rwords <= kRedWords && rgroups <= sort_size && sort_size <= kRedWords) {
+    // Counting sort over the occupied cells: every valid row's cell has a
+    // rank among the frame's dilated cells (a point outside them ends the
+    // pattern epoch, and the assembly then halts the iteration before this
+    // kernel); the occupied ranks, in a bitmap (in `red`), give each row a
+    // dense cell index in ascending cell order (a prefix count per group of
+    // 8 words, in order[sort_size ..]); rows go to their cell's range in
+    // frame order. ucell holds each row's cell index until the end.
+    uint32_t* rbits = reinterpret_cast<uint32_t*>(&red[0][0]);
+    int32_t* gpre = order + sort_size;
+    int32_t* rowu = ucell;
+    const uint32_t* fbits = A.fp_cellbits + static_cast<int64_t>(pose) * A.words_per_pose;
+    const int32_t* frank = A.fp_cellrank + static_cast<int64_t>(pose) * A.words_per_pose;
+    for (int w = tid; w < rwords; w += kThreads) rbits[w] = 0u;
+    __syncthreads();
+    for (int i = tid; i < m; i += kThreads) {
       const int32_t row = A.pr_row[rb + i];
       rows[i] = row;
-      keys[i] = (static_cast<uint64_t>(W.cellkey[row]) << 32) | static_cast<uint32_t>(i);
-    } else {
-      keys[i] = ~0ULL;
-    }
-  }
-  __syncthreads();
-  // Bitonic sort of (cell, local row) keys.
-  for (int k = 2; k <= sort_size; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = tid; i < sort_size; i += kThreads) {
-        const int l = i ^ j;
-        if (l > i) {
-          const uint64_t a = keys[i];
-          const uint64_t b = keys[l];
-          const bool up = (i & k) == 0;
-          if ((a > b) == up) {
-            keys[i] = b;
-            keys[l] = a;
-          }
+      const uint32_t c = W.cellkey[row];
+      int r = -1;
+      if (c < invalid) {
+        const uint32_t word = fbits[c >> 5];
+        const uint32_t bit = 1u << (c & 31);
+        if ((word & bit) != 0u) {
+          r = frank[c >> 5] + __popc(word & (bit - 1u));
+          atomicOr(&rbits[r >> 5], 1u << (r & 31));
         }
       }
+      rowu[i] = r;
+    }
+    __syncthreads();
+    // Exclusive prefix count per group of 8 words (a chunk of groups per
+    // thread, then the chunk totals in order).
+    const int per = (rgroups + kThreads - 1) / kThreads;
+    const int g0 = min(tid * per, rgroups);
+    const int g1 = min(g0 + per, rgroups);
+    auto group_count = [&](int g) {
+      int n = 0;
+      for (int w = 8 * g; w < min(8 * g + 8, rwords); ++w) n += __popc(rbits[w]);
+      return n;
+    };
+    int chunk = 0;
+    for (int g = g0; g < g1; ++g) chunk += group_count(g);
+    num_u = BlockExclusiveScan(counts, chunk);
+    int run = counts[tid];
+    for (int g = g0; g < g1; ++g) {
+      gpre[g] = run;
+      run += group_count(g);
+    }
+    for (int u = tid; u <= num_u; u += kThreads) ustart[u] = 0;
+    __syncthreads();
+    for (int i = tid; i < m; i += kThreads) {
+      const int r = rowu[i];
+      if (r < 0) continue;
+      int u = gpre[r >> 8];
+      for (int w = (r >> 8) * 8; w < (r >> 5); ++w) u += __popc(rbits[w]);
+      u += __popc(rbits[r >> 5] & ((1u << (r & 31)) - 1u));
+      rowu[i] = u;
+      atomicAdd(&ustart[u + 1], 1);
+    }
+    __syncthreads();
+    // Cell starts: the per-cell counts (moved to fill, which takes the
+    // bitmap's place) scanned, a chunk of cells per thread; fill is then
+    // cleared for the placement.
+    int32_t* fill = reinterpret_cast<int32_t*>(&red[0][0]);
+    for (int u = tid; u < num_u; u += kThreads) fill[u] = ustart[u + 1];
+    __syncthreads();
+    {
+      const int cper = (num_u + kThreads - 1) / kThreads;
+      const int u0 = min(tid * cper, num_u);
+      const int u1 = min(u0 + cper, num_u);
+      int csum = 0;
+      for (int u = u0; u < u1; ++u) csum += fill[u];
+      const int rows_valid = BlockExclusiveScan(counts, csum);
+      int at = counts[tid];
+      for (int u = u0; u < u1; ++u) {
+        ustart[u] = at;
+        at += fill[u];
+      }
+      if (tid == 0) ustart[num_u] = rows_valid;
+    }
+    __syncthreads();
+    for (int u = tid; u < num_u; u += kThreads) fill[u] = 0;
+    __syncthreads();
+    // Rows to their cells in frame order: a pass over the rows in chunks of
+    // kThreads, each row placed after the rows of its cell in earlier
+    // chunks and before the later rows of its own chunk.
+    int32_t* chunk_u = counts;  // the chunk's cells (kThreads entries)
+    for (int base = 0; base < m; base += kThreads) {
+      const int i = base + tid;
+      const int u = i < m ? rowu[i] : -1;
+      chunk_u[tid] = u;
+      __syncthreads();
+      if (u >= 0) {
+        int before = 0;
+        for (int j = 0; j < tid; ++j) before += chunk_u[j] == u ? 1 : 0;
+        order[ustart[u] + fill[u] + before] = i;
+      }
+      __syncthreads();
+      if (u >= 0) atomicAdd(&fill[u], 1);
+      __syncthreads();
+    }
+    for (int u = tid; u < num_u; u += kThreads) ucell[u] = static_cast<int32_t>(W.cellkey[rows[order[ustart[u]]]]);
+    __syncthreads();
+  } else {
+    // A frame with more dilated cells than the bitmap holds: bitonic sort
+    // of (cell, local row) keys.
+    for (int i = tid; i < sort_size; i += kThreads) {
+      if (i < m) {
+        const int32_t row = A.pr_row[rb + i];
+        rows[i] = row;
+        keys[i] = (static_cast<uint64_t>(W.cellkey[row]) << 32) | static_cast<uint32_t>(i);
+      } else {
+        keys[i] = ~0ULL;
+      }
+    }
+    __syncthreads();
+    for (int k = 2; k <= sort_size; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < sort_size; i += kThreads) {
+          const int l = i ^ j;
+          if (l > i) {
+            const uint64_t a = keys[i];
+            const uint64_t b = keys[l];
+            const bool up = (i & k) == 0;
+            if ((a > b) == up) {
+              keys[i] = b;
+              keys[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // Unique valid cells and their start positions (fixed-order compaction).
+    const int per = (sort_size + kThreads - 1) / kThreads;
+    const int lo = tid * per;
+    const int hi = min(lo + per, sort_size);
+    int cnt = 0;
+    for (int t = lo; t < hi; ++t) {
+      const uint32_t c = static_cast<uint32_t>(keys[t] >> 32);
+      if (c < invalid && (t == 0 || static_cast<uint32_t>(keys[t - 1] >> 32) != c)) ++cnt;
+    }
+    counts[tid] = cnt;
+    __syncthreads();
+    if (tid == 0) {
+      int run = 0;
+      for (int i = 0; i < kThreads; ++i) {
+        const int c = counts[i];
+        counts[i] = run;
+        run += c;
+      }
+      counts[kThreads] = run;
+    }
+    __syncthreads();
+    int pos = counts[tid];
+    int valid_end_local = -1;
+    for (int t = lo; t < hi; ++t) {
+      const uint32_t c = static_cast<uint32_t>(keys[t] >> 32);
+      if (c < invalid) {
+        if (t == 0 || static_cast<uint32_t>(keys[t - 1] >> 32) != c) {
+          ucell[pos] = static_cast<int32_t>(c);
+          ustart[pos] = t;
+          ++pos;
+        }
+        if (t + 1 == sort_size || static_cast<uint32_t>(keys[t + 1] >> 32) >= invalid) {
+          valid_end_local = t + 1;
+        }
+      }
+    }
+    num_u = counts[kThreads];
+    __syncthreads();
+    if (valid_end_local >= 0) ustart[num_u] = valid_end_local;
+    if (num_u == 0 && tid == 0) ustart[0] = 0;
+    // The keys' local indices as order[], a chunk of kRedWords at a time
+    // through `red` (order aliases keys).
+    int32_t* tmp = reinterpret_cast<int32_t*>(&red[0][0]);
+    for (int c0 = 0; c0 < sort_size; c0 += kRedWords) {
+      const int c1 = min(c0 + kRedWords, sort_size);
+      for (int t = c0 + tid; t < c1; t += kThreads) tmp[t - c0] = static_cast<int32_t>(keys[t] & 0xffffffffULL);
+      __syncthreads();
+      for (int t = c0 + tid; t < c1; t += kThreads) order[t] = tmp[t - c0];
       __syncthreads();
     }
   }
-  // Unique valid cells and their start positions (fixed-order compaction).
-  const int per = (sort_size + kThreads - 1) / kThreads;
-  const int lo = tid * per;
-  const int hi = min(lo + per, sort_size);
-  int cnt = 0;
-  for (int t = lo; t < hi; ++t) {
-    const uint32_t c = static_cast<uint32_t>(keys[t] >> 32);
-    if (c < invalid && (t == 0 || static_cast<uint32_t>(keys[t - 1] >> 32) != c)) ++cnt;
-  }
-  counts[tid] = cnt;
-  __syncthreads();
-  if (tid == 0) {
-    int run = 0;
-    for (int i = 0; i < kThreads; ++i) {
-      const int c = counts[i];
-      counts[i] = run;
-      run += c;
-    }
-    counts[kThreads] = run;
-  }
-  __syncthreads();
-  int pos = counts[tid];
-  int valid_end_local = -1;
-  for (int t = lo; t < hi; ++t) {
-    const uint32_t c = static_cast<uint32_t>(keys[t] >> 32);
-    if (c < invalid) {
-      if (t == 0 || static_cast<uint32_t>(keys[t - 1] >> 32) != c) {
-        ucell[pos] = static_cast<int32_t>(c);
-        ustart[pos] = t;
-        ++pos;
-      }
-      if (t + 1 == sort_size || static_cast<uint32_t>(keys[t + 1] >> 32) >= invalid) {
-        valid_end_local = t + 1;
-      }
-    }
-  }
-  const int num_u = counts[kThreads];
-  __syncthreads();
-  if (valid_end_local >= 0) ustart[num_u] = valid_end_local;
-  if (num_u == 0 && tid == 0) ustart[0] = 0;
   if (use_occ) {
-    // The bitmap was cleared before the sort's barriers.
+    for (int i = tid; i < occ_words; i += kThreads) occupied[i] = 0u;
+    __syncthreads();
     for (int u = tid; u < num_u; u += kThreads) {
       const int c = ucell[u];
       atomicOr(&occupied[c >> 5], 1u << (c & 31));
@@ -622,7 +771,6 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   // (a point that left the footprint, which ends the pattern epoch) has no
   // rank and is left out.
   int32_t* smap = W.seg_map + A.dl_ptr[pose];
-  const int64_t nrank = A.dl_ptr[pose + 1] - A.dl_ptr[pose];
   for (int64_t r = tid; r < nrank; r += kThreads) smap[r] = -1;
   __syncthreads();
   {
@@ -652,7 +800,7 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) accp[k] = 0.0;
     for (int t = ustart[u]; t < ustart[u + 1]; ++t) {
-      const int row = rows[static_cast<uint32_t>(keys[t] & 0xffffffffULL)];
+      const int row = rows[order[t]];
       const double w0 = W.jw[row];
       const double w1 = W.jw[np + row];
       const double w2 = W.jw[2 * np + row];
