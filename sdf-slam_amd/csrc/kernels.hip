@@ -1056,6 +1056,16 @@ __device__ __forceinline__ double Opaque(double v) {
   return v;
 }
 
+// base[byte_off / 8] for a byte offset below 4 GiB: a wave-uniform base and
+// a 32-bit per-lane offset, which the load takes as its scalar base and
+// vector offset (no 64-bit address per lane).
+__device__ __forceinline__ double LoadAt(const double* base, uint32_t byte_off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// The wavefront's index in the workgroup, as a wave-uniform value.
+__device__ __forceinline__ int WaveIndex() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // L_kk^-1 of supernode s's kb-th diagonal block (column-major 64 x 64,
 // identity beyond the block's nb columns), kept for the solves.
 __device__ __forceinline__ double* DiagInv(const DeviceAnalysis& A, const DeviceWork& W, int s, int kb) {
@@ -1530,13 +1540,15 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
 // operand the A chunk (its 16 columns are tile rows), so that lane l's
 // result e is tile row (l & 15), column 4 e + (l >> 4) of its block: a
 // wavefront's C accesses are 16-row runs of 4 columns (coalesced in the
-// column-major panel / update matrix). K chunks of kTk are double-buffered
-// in `lds` (2 (kTiles + 1) kTk (kTile + 2) doubles; rows padded by two
-// doubles, so the two k rows an MFMA operand read touches fall in different
-// banks), the next chunk's loads in flight while one is multiplied. With
-// kRelease the last barrier comes after the last product, so the LDS is
-// free on return.
-template <int kTiles, bool kRelease>
+// column-major panel / update matrix). K chunks of kTk go through a ring of
+// kDepth register sets, so the loads of up to kDepth chunks are in flight
+// together (a chunk's products take far less time than a load's round
+// trip), and through two LDS buffers (2 (kTiles + 1) kTk (kTile + 2)
+// doubles; rows padded by two doubles, so the two k rows an MFMA operand
+// read touches fall in different banks). The chunks' order, and so every
+// sum's, does not depend on kDepth. With kRelease the last barrier comes
+// after the last product, so the LDS is free on return.
+template <int kTiles, bool kRelease, int kDepth, bool kShortK>
 __device__ __forceinline__ void TileProducts(const double* F, int f, int kbase, int K, int rbase, int M, int N,
                                              const int (&r0)[kTiles], int cb0, const double* dk,
                                              int64_t dstride, double* lds, v4d (&acc)[kTiles][2][2]) {
@@ -1554,26 +1566,35 @@ __device__ __forceinline__ void TileProducts(const double* F, int f, int kbase, 
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb) acc[t][cb][rb] = v4d{0.0, 0.0, 0.0, 0.0};
   // Staging of one K chunk: each thread loads 4 entries of every operand
-  // (rows rr = idx & 63 of k row kk = idx >> 6) and D_k. Every load is
-  // issued at an address clamped into the operands, and the padding is
-  // selected (and B scaled by D_k) only when the chunk is stored to the LDS,
+  // (rows rr = idx & 63 of k row kk = idx >> 6) and one D_k (k = tid %
+  // kTk). Every load is issued at an address clamped into the operands, and
+  // the padding is selected only when the chunk is stored to the LDS,
   // through Opaque: a load feeding a select directly would become a branch
-  // that waits for its data before the next load is issued.
-  double ra[kTiles][4], rb[4], rd[4];
-  auto load_chunk = [&](int k0) {
+  // that waits for its data before the next load is issued. D_k goes into
+  // the first padding column of B's k row and scales B when it is read as
+  // an MFMA operand (the same product b D_k as a scaled copy would hold).
+  // Thread (wave w, lane l) takes row l of k rows w, w + 4, w + 8, w + 12
+  // of a chunk (entry idx = tid + kThreads e: rr = l, k = idx >> 6): a
+  // wave-uniform column and a per-lane row offset (fixed over the chunks).
+  double ra[kDepth][kTiles][4], rb[kDepth][4], rd[kDepth];
+  const int wq = WaveIndex();
+  uint32_t aoff[kTiles];
+#pragma unroll
+  for (int t = 0; t < kTiles; ++t) aoff[t] = 8u * static_cast<uint32_t>(rbase + min(r0[t] + lane, M - 1));
+  const uint32_t boff = 8u * static_cast<uint32_t>(rbase + min(cb0 + lane, N - 1));
+  const int dkk = tid & (kTk - 1);
+  auto load_chunk = [&](int q, int k0) {
+    rd[q] = dk[min(k0 + dkk, K - 1) * dstride];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int idx = tid + kThreads * e;
-      const int rr = idx & 63;
-      const int kq = min(k0 + (idx >> 6), K - 1);
-      const int64_t colbase = static_cast<int64_t>(kbase + kq) * f;
-      rd[e] = dk[kq * dstride];
+      const int kq = min(k0 + wq + 4 * e, K - 1);
+      const double* col = F + static_cast<int64_t>(kbase + kq) * f;
 #pragma unroll
-      for (int t = 0; t < kTiles; ++t) ra[t][e] = F[colbase + rbase + min(r0[t] + rr, M - 1)];
-      rb[e] = F[colbase + rbase + min(cb0 + rr, N - 1)];
+      for (int t = 0; t < kTiles; ++t) ra[q][t][e] = LoadAt(col, aoff[t]);
+      rb[q][e] = LoadAt(col, boff);
     }
   };
-  auto store_chunk = [&](int buf, int k0) {
+  auto store_chunk = [&](int q, int buf, int k0) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int idx = tid + kThreads * e;
@@ -1581,21 +1602,18 @@ __device__ __forceinline__ void TileProducts(const double* F, int f, int kbase, 
       const bool kin = k0 + (idx >> 6) < K;
 #pragma unroll
       for (int t = 0; t < kTiles; ++t) {
-        const double a = Opaque(ra[t][e]);
+        const double a = Opaque(ra[q][t][e]);
         As[buf][t][idx >> 6][rr] = (kin && r0[t] + rr < M) ? a : 0.0;
       }
-      const double b = Opaque(rb[e]);
-      const double d = Opaque(rd[e]);
-      Bs[buf][idx >> 6][rr] = (kin && cb0 + rr < N) ? b * d : 0.0;
+      const double b = Opaque(rb[q][e]);
+      Bs[buf][idx >> 6][rr] = (kin && cb0 + rr < N) ? b : 0.0;
+    }
+    if (tid < kTk) {
+      const double d = Opaque(rd[q]);
+      Bs[buf][tid][kTile] = k0 + tid < K ? d : 0.0;
     }
   };
-  load_chunk(0);
-  store_chunk(0, 0);
-  __syncthreads();
-  int buf = 0;
-  for (int k0 = 0; k0 < K; k0 += kTk) {
-    const bool more = k0 + kTk < K;
-    if (more) load_chunk(k0 + kTk);  // in flight while this chunk is used
+  auto products = [&](int buf) {
 #pragma unroll
     for (int kk = 0; kk < kTk; kk += 4) {
       const int kr = kk + (lane >> 4);
@@ -1604,8 +1622,9 @@ __device__ __forceinline__ void TileProducts(const double* F, int f, int kbase, 
       for (int t = 0; t < kTiles; ++t)
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb) av[t][rb] = As[buf][t][kr][wr + 16 * rb + (lane & 15)];
+      const double dkr = Bs[buf][kr][kTile];
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) bv[cb] = Bs[buf][kr][wc + 16 * cb + (lane & 15)];
+      for (int cb = 0; cb < 2; ++cb) bv[cb] = Bs[buf][kr][wc + 16 * cb + (lane & 15)] * dkr;
 #pragma unroll
       for (int t = 0; t < kTiles; ++t)
 #pragma unroll
@@ -1615,23 +1634,68 @@ __device__ __forceinline__ void TileProducts(const double* F, int f, int kbase, 
             acc[t][cb][rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[cb], av[t][rb], acc[t][cb][rb], 0, 0, 0);
           }
     }
-    if (more) {
-      store_chunk(buf ^ 1, k0 + kTk);
-      __syncthreads();
-      buf ^= 1;
+  };
+  const int chunks = (K + kTk - 1) / kTk;
+#pragma unroll
+  for (int q = 0; q < kDepth; ++q) load_chunk(q, q * kTk);
+  store_chunk(0, 0, 0);
+  __syncthreads();
+  auto step = [&](int q, int c) {
+    if (c < chunks) {
+      products(c & 1);
+      if (c + 1 < chunks) {
+        store_chunk((q + 1) % kDepth, (c + 1) & 1, (c + 1) * kTk);
+        __syncthreads();
+      }
     }
+  };
+  int g = 0;
+  if (!kShortK) {
+    for (; g + 2 * kDepth <= chunks; g += kDepth) {
+#pragma unroll
+      for (int q = 0; q < kDepth; ++q) {
+        const int c = g + q;
+        load_chunk(q, (c + kDepth) * kTk);  // set q's chunk c is in the LDS
+        products(c & 1);
+        store_chunk((q + 1) % kDepth, (c + 1) & 1, (c + 1) * kTk);
+        __syncthreads();
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kDepth; ++q) {
+      const int c = g + q;
+      if (c + kDepth < chunks) load_chunk(q, (c + kDepth) * kTk);
+      step(q, c);
+    }
+    g += kDepth;
   }
+#pragma unroll
+  for (int q = 0; q < kDepth; ++q) step(q, g + q);
   if (kRelease) __syncthreads();
 }
 
 // The factored diagonal block of step kb (staging slot: L below the
 // diagonal, D on it) goes into the panel, by the workgroup of tile (0, 0).
-__device__ __forceinline__ void MoveStagedBlock(double* F, int f, int c0, int nb, const double* stage) {
-  for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-    const int i = idx % nb;
-    const int j = idx / nb;
-    if (i >= j) F[static_cast<int64_t>(c0 + j) * f + c0 + i] = stage[j * kNB + i];
+// Split in two: each thread's loads (thread t takes entries t, t + kThreads,
+// ...), then its stores, which can come after other work of the workgroup.
+constexpr int kMovePer = kNB * kNB / kThreads;
+__device__ __forceinline__ void LoadStagedBlock(const double* stage, double (&v)[kMovePer]) {
+#pragma unroll
+  for (int e = 0; e < kMovePer; ++e) v[e] = stage[threadIdx.x + kThreads * e];
+}
+__device__ __forceinline__ void StoreStagedBlock(double* F, int f, int c0, int nb, const double (&v)[kMovePer]) {
+#pragma unroll
+  for (int e = 0; e < kMovePer; ++e) {
+    const int idx = threadIdx.x + kThreads * e;
+    const int i = idx & (kNB - 1);
+    const int j = idx / kNB;
+    if (i >= j && i < nb) F[static_cast<int64_t>(c0 + j) * f + c0 + i] = Opaque(v[e]);
   }
+}
+__device__ __forceinline__ void MoveStagedBlock(double* F, int f, int c0, int nb, const double* stage) {
+  double v[kMovePer];
+  LoadStagedBlock(stage, v);
+  StoreStagedBlock(F, f, c0, nb, v);
 }
 
 // C -= acc over the tile's lower entries (c <= r, inside M x N): the 16
@@ -1722,7 +1786,7 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
   __shared__ double lds[2 * 2 * kTk * (kTile + 2)];
   v4d acc[1][2][2];
   const int r0[1] = {R0};
-  TileProducts<1, false>(Fw, f, kbase, K, rbase, M, N, r0, C0, dk, dstride, lds, acc);
+  TileProducts<1, false, 2, false>(Fw, f, kbase, K, rbase, M, N, r0, C0, dk, dstride, lds, acc);
   if constexpr (mode == 0) {
     SubtractLower(C, ldc, R0, C0, M, N, acc[0]);
   } else {
@@ -1796,6 +1860,7 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
 constexpr int kAheadLd = kNB + 1;
 constexpr int kAheadLds = kNB * kAheadLd + 2 * kNB + kNB * kLdT;
 static_assert(kAheadLds >= 2 * 3 * kTk * (kTile + 2), "fused step staging");
+static_assert(4 * kTk >= kNB, "a block column's products load every chunk up front");
 
 // Trailing update of step kb fused with step kb + 1's panel step. Tile
 // (ti, tj), tj <= ti, of front s. The tiles of column 0 are the next block
@@ -1822,19 +1887,26 @@ __device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, i
   if (c0 >= ns) return;
   const int nb = min(kNB, ns - c0);
   const double* stage = StageSlot(W, s, kb);
-  if (ti == 0 && tj == 0) MoveStagedBlock(Fw, f, c0, nb, stage);
+  // Tile (0, 0)'s workgroup moves the block; its stores come last (the
+  // block is disjoint from everything the step reads or writes).
+  const bool mover = ti == 0 && tj == 0;
+  double mv[kMovePer];
+  if (mover) LoadStagedBlock(stage, mv);
   const int rbase = c0 + nb;
   const int M = f - rbase;
   const int N = ns - rbase;
   const int R0 = ti * kTile;
   const int C0 = tj * kTile;
-  if (R0 >= M || C0 >= N) return;
+  if (R0 >= M || C0 >= N) {
+    if (mover) StoreStagedBlock(Fw, f, c0, nb, mv);
+    return;
+  }
   double* C = Fw + static_cast<int64_t>(rbase) * f + rbase;
   __shared__ double lds[kAheadLds];
   if (tj > 0) {
     v4d acc[1][2][2];
     const int r0[1] = {R0};
-    TileProducts<1, false>(Fw, f, c0, nb, rbase, M, N, r0, C0, stage, kNB + 1, lds, acc);
+    TileProducts<1, false, 4, true>(Fw, f, c0, nb, rbase, M, N, r0, C0, stage, kNB + 1, lds, acc);
     SubtractLower(C, f, R0, C0, M, N, acc[0]);
     return;
   }
@@ -1850,9 +1922,10 @@ __device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, i
   auto at = reinterpret_cast<double (*)[kLdT]>(colscr + kNB);  // at[k][r]: TRSM rows
   // The updated tile (0, 0) goes to lt (rows below nb1: to at, in tile
   // (0, 0)'s own workgroup), tile (ti, 0) to at; everything else in at is
-  // zero (rows past M, columns past nb1).
-  auto put = [&](const v4d (&acc)[2][2], int t_r0, bool diag, bool rows) {
-    double prev[2][4][2];
+  // zero (rows past M, columns past nb1). The tiles' current values are
+  // loaded before the products, so that their loads are in flight with the
+  // products' operands.
+  auto load_prev = [&](double (&prev)[2][4][2], int t_r0) {
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
@@ -1865,6 +1938,8 @@ __device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, i
         }
       }
     }
+  };
+  auto put = [&](const v4d (&acc)[2][2], const double (&prev)[2][4][2], int t_r0, bool diag, bool rows) {
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
 #pragma unroll
@@ -1887,14 +1962,19 @@ __device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, i
   if (ti == 0) {
     v4d acc[1][2][2];
     const int r0[1] = {0};
-    TileProducts<1, true>(Fw, f, c0, nb, rbase, M, N, r0, 0, stage, kNB + 1, lds, acc);
-    put(acc[0], 0, true, true);
+    double prev[2][4][2];
+    load_prev(prev, 0);
+    TileProducts<1, true, 4, true>(Fw, f, c0, nb, rbase, M, N, r0, 0, stage, kNB + 1, lds, acc);
+    put(acc[0], prev, 0, true, true);
   } else {
     v4d acc[2][2][2];
     const int r0[2] = {0, R0};
-    TileProducts<2, true>(Fw, f, c0, nb, rbase, M, N, r0, 0, stage, kNB + 1, lds, acc);
-    put(acc[0], 0, true, false);
-    put(acc[1], R0, false, true);
+    double prev[2][2][4][2];
+    load_prev(prev[0], 0);
+    load_prev(prev[1], R0);
+    TileProducts<2, true, 4, true>(Fw, f, c0, nb, rbase, M, N, r0, 0, stage, kNB + 1, lds, acc);
+    put(acc[0], prev[0], 0, true, false);
+    put(acc[1], prev[1], R0, false, true);
   }
   __syncthreads();
   const bool zero = FactorInvertBlock<kAheadLd>(lt, dvec, colscr, nb1);
@@ -1947,6 +2027,7 @@ __device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, i
       }
     }
   }
+  if (mover) StoreStagedBlock(Fw, f, c0, nb, mv);
 }
 
 // Grid (tile rows, tile columns, fronts x slots) over the level's fronts.
