@@ -506,6 +506,36 @@ __global__ void KNodeSlots(Slots slots, int marquardt) {
   W.rhs[cu] = -gsum;
 }
 
+// The value itself, opaque to the optimizer. A load whose result feeds a
+// select on a wave-uniform condition is otherwise moved under a branch on
+// that condition, where its data is awaited before the next load issues;
+// loads feeding Opaque are issued where they stand.
+__device__ __forceinline__ double Opaque(double v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+
+// base[byte_off / 8] for a byte offset below 4 GiB: a wave-uniform base and
+// a 32-bit per-lane offset, which the load takes as its scalar base and
+// vector offset (no 64-bit address per lane).
+__device__ __forceinline__ double LoadAt(const double* base, uint32_t byte_off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// The wavefront's index in the workgroup, as a wave-uniform value.
+__device__ __forceinline__ int WaveIndex() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// Opaque for a 32-bit integer.
+__device__ __forceinline__ uint32_t OpaqueU(uint32_t v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+
+// Occupied cells of a frame whose pose sums KPoseBlock keeps in the LDS (12
+// per cell, in `red`).
+constexpr int kLdsCells = 192;
+static_assert(12 * kLdsCells <= 9 * kThreads, "pose sums fit the reduction buffer");
+
 // Pose rows: footprint (pose x node) entries, the pose's 3x3 diagonal block
 // and its -J^T r entries. One workgroup per pose; the pose's point rows are
 // sorted by cell in LDS (bitonic) so every entry sums its points in a fixed
@@ -588,20 +618,38 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
     const int32_t* frank = A.fp_cellrank + static_cast<int64_t>(pose) * A.words_per_pose;
     for (int w = tid; w < rwords; w += kThreads) rbits[w] = 0u;
     __syncthreads();
-    for (int i = tid; i < m; i += kThreads) {
-      const int32_t row = A.pr_row[rb + i];
-      rows[i] = row;
-      const uint32_t c = W.cellkey[row];
-      int r = -1;
-      if (c < invalid) {
-        const uint32_t word = fbits[c >> 5];
-        const uint32_t bit = 1u << (c & 31);
-        if ((word & bit) != 0u) {
-          r = frank[c >> 5] + __popc(word & (bit - 1u));
-          atomicOr(&rbits[r >> 5], 1u << (r & 31));
-        }
+    // Four rows per thread at a time: their row ids, then their cells, then
+    // their bitmap words are loaded together (clamped; rows past m unused).
+    for (int base = 0; base < m; base += 4 * kThreads) {
+      int32_t rw[4];
+      uint32_t ck[4], wd[4];
+      int32_t fr[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) rw[e] = A.pr_row[rb + min(base + tid + e * kThreads, m - 1)];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ck[e] = W.cellkey[rw[e]];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t cc = ck[e] < invalid ? ck[e] : 0u;
+        wd[e] = OpaqueU(fbits[cc >> 5]);
+        fr[e] = static_cast<int32_t>(OpaqueU(static_cast<uint32_t>(frank[cc >> 5])));
       }
-      rowu[i] = r;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = base + tid + e * kThreads;
+        if (i >= m) continue;
+        rows[i] = rw[e];
+        const uint32_t c = ck[e];
+        int r = -1;
+        if (c < invalid) {
+          const uint32_t bit = 1u << (c & 31);
+          if ((wd[e] & bit) != 0u) {
+            r = fr[e] + __popc(wd[e] & (bit - 1u));
+            atomicOr(&rbits[r >> 5], 1u << (r & 31));
+          }
+        }
+        rowu[i] = r;
+      }
     }
     __syncthreads();
     // Exclusive prefix count per group of 8 words (a chunk of groups per
@@ -657,24 +705,66 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
     __syncthreads();
     for (int u = tid; u < num_u; u += kThreads) fill[u] = 0;
     __syncthreads();
-    // Rows to their cells in frame order: a pass over the rows in chunks of
-    // kThreads, each row placed after the rows of its cell in earlier
-    // chunks and before the later rows of its own chunk.
-    int32_t* chunk_u = counts;  // the chunk's cells (kThreads entries)
-    for (int base = 0; base < m; base += kThreads) {
-      const int i = base + tid;
-      const int u = i < m ? rowu[i] : -1;
-      chunk_u[tid] = u;
+    // Rows to their cells in frame order (a row goes after the rows of its
+    // cell with smaller frame index). The rows are cut into four runs of
+    // 64-row chunks, one per wavefront; a wavefront walks its chunks in
+    // order and ranks each row among its cell's rows in the run (one ballot
+    // per distinct cell of a chunk, the run's count per cell in wcnt), then
+    // every row adds the counts of the earlier runs. The LDS operations of
+    // one wavefront complete in order, so a count is read before its
+    // increment lands. wcnt (4 num_u counts) follows fill in `red`; the
+    // ranks go where the group prefix counts were (order + sort_size).
+    int32_t* wcnt = fill + num_u;
+    int32_t* rrank = gpre;
+    const int nchunk = (m + 63) >> 6;
+    const int cper = (nchunk + kWaves - 1) / kWaves;
+    const bool waves4 = 5 * num_u <= kRedWords;
+    if (waves4) {
+      for (int x = tid; x < kWaves * num_u; x += kThreads) wcnt[x] = 0;
       __syncthreads();
-      if (u >= 0) {
-        int before = 0;
-        for (int j = 0; j < tid; ++j) before += chunk_u[j] == u ? 1 : 0;
-        order[ustart[u] + fill[u] + before] = i;
+      const int wv = tid >> 6;
+      const int ln = tid & 63;
+      int32_t* cnt = wcnt + wv * num_u;
+      for (int c = wv * cper; c < min(nchunk, (wv + 1) * cper); ++c) {
+        const int i = 64 * c + ln;
+        const int u = i < m ? rowu[i] : -1;
+        uint64_t pending = __ballot(u >= 0);
+        while (pending != 0ull) {
+          const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+          const int v = __builtin_amdgcn_readlane(u, leader);
+          const uint64_t same = __ballot(u == v);
+          if (u == v) rrank[i] = cnt[v] + __popcll(same & ((1ull << ln) - 1ull));
+          if (ln == leader) cnt[v] += __popcll(same);
+          pending &= ~same;
+        }
       }
       __syncthreads();
-      if (u >= 0) atomicAdd(&fill[u], 1);
-      __syncthreads();
+      for (int i = tid; i < m; i += kThreads) {
+        const int u = rowu[i];
+        if (u < 0) continue;
+        const int run = (i >> 6) / cper;
+        int at = ustart[u] + rrank[i];
+        for (int w2 = 0; w2 < run; ++w2) at += wcnt[w2 * num_u + u];
+        order[at] = i;
+      }
+    } else if (tid < 64) {
+      // More cells than the counts fit: wavefront 0 alone, fill as the
+      // running count.
+      for (int base = 0; base < m; base += 64) {
+        const int i = base + tid;
+        const int u = i < m ? rowu[i] : -1;
+        uint64_t pending = __ballot(u >= 0);
+        while (pending != 0ull) {
+          const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+          const int v = __builtin_amdgcn_readlane(u, leader);
+          const uint64_t same = __ballot(u == v);
+          if (u == v) order[ustart[v] + fill[v] + __popcll(same & ((1ull << tid) - 1ull))] = i;
+          if (tid == leader) fill[v] += __popcll(same);
+          pending &= ~same;
+        }
+      }
     }
+    __syncthreads();
     for (int u = tid; u < num_u; u += kThreads) ucell[u] = static_cast<int32_t>(W.cellkey[rows[order[ustart[u]]]]);
     __syncthreads();
   } else {
@@ -758,7 +848,29 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
       __syncthreads();
     }
   }
-  if (use_occ) {
+  // With at most kLdsCells occupied cells (and room for a half-empty table
+  // in the sort keys' second half, free after the sort) the cells' pose
+  // sums stay in the LDS (in `red`) and the footprint entries find a cell's
+  // segment in an open-addressing table (cell -> segment); otherwise the
+  // sums go through global memory and a node's cells are found by binary
+  // search behind a bitmap of the occupied cells (in `red`).
+  const int hslots = sort_size / 2;
+  const bool lds_fp = num_u <= kLdsCells && hslots >= 2 && 2 * num_u <= hslots;
+  int32_t* hkey = order + sort_size;
+  int32_t* hval = hkey + hslots;
+  const int hshift = 32 - __builtin_ctz(static_cast<unsigned>(max(hslots, 2)));
+  auto hslot = [&](int c) { return static_cast<int>((static_cast<uint32_t>(c) * 0x9E3779B1u) >> hshift); };
+  double* jp_lds = &red[0][0];
+  if (lds_fp) {
+    for (int h = tid; h < hslots; h += kThreads) hkey[h] = -1;
+    __syncthreads();
+    for (int u = tid; u < num_u; u += kThreads) {
+      const int c = ucell[u];
+      int h = hslot(c);
+      while (atomicCAS(&hkey[h], -1, c) != -1) h = (h + 1) & (hslots - 1);
+      hval[h] = u;
+    }
+  } else if (use_occ) {
     for (int i = tid; i < occ_words; i += kThreads) occupied[i] = 0u;
     __syncthreads();
     for (int u = tid; u < num_u; u += kThreads) {
@@ -786,56 +898,81 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   }
   __syncthreads();
 
-  // This frame's partial sums per occupied cell, in the cell-sorted order:
-  // the 14 node sums, and the 12 pose-partial x weight sums the footprint
-  // entries below add up over the node's cells.
   const double* jp0 = W.jp;
   const double* jp1 = W.jp + np;
   const double* jp2 = W.jp + 2 * np;
-  for (int u = tid; u < num_u; u += kThreads) {
-    double acc[14];
-    double accp[12];
+  // This frame's sums per occupied cell: the 14 node sums (10 weight
+  // products, 4 weight x residual) and the 12 pose-partial x weight sums
+  // the footprint entries below add up over the node's cells, all entries
+  // of the Gram matrix of the cell's rows x = (w0..w3, r, p0..p2), formed on
+  // FP64 MFMA: one 16x16x4 product takes four rows each of two cells (lane
+  // l supplies feature l % 8 of row l / 16 of cell 2 p + (l % 16) / 8 as
+  // both operands, so D(i, j) = sum_k x_i x_j within each cell's 8 x 8
+  // diagonal block; rows past a cell's last are zero). A wavefront takes
+  // cell pairs p = wave, wave + 4, ...; every sum is formed in the same
+  // order whatever the launch.
+  {
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int f16 = lane & 15;
+    const int feat = f16 & 7;
+    const int kr = lane >> 4;
+    const double* fbase = feat < 4 ? W.jw + static_cast<int64_t>(feat) * np
+                                   : (feat == 4 ? W.r_pt : W.jp + static_cast<int64_t>(feat - 5) * np);
+    for (int pair = wave; 2 * pair < num_u; pair += kWaves) {
+      const int ua = 2 * pair;
+      const bool has_b = ua + 1 < num_u;
+      const int my = f16 < 8 ? ua : (has_b ? ua + 1 : ua);
+      const int t0 = ustart[my];
+      const int t1 = (f16 < 8 || has_b) ? ustart[my + 1] : t0;
+      const int na = ustart[ua + 1] - ustart[ua];
+      const int nb = has_b ? ustart[ua + 2] - ustart[ua + 1] : 0;
+      const int groups = (max(na, nb) + 3) >> 2;
+      v4d g = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int q0 = 0; q0 < groups; q0 += 4) {
+        double x[4];
 #pragma unroll
-    for (int k = 0; k < 14; ++k) acc[k] = 0.0;
+        for (int q = 0; q < 4; ++q) {
+          const int t = t0 + 4 * (q0 + q) + kr;
+          const int tc = max(min(t, t1 - 1), 0);
+          x[q] = fbase[rows[order[tc]]];
+        }
 #pragma unroll
-    for (int k = 0; k < 12; ++k) accp[k] = 0.0;
-    for (int t = ustart[u]; t < ustart[u + 1]; ++t) {
-      const int row = rows[order[t]];
-      const double w0 = W.jw[row];
-      const double w1 = W.jw[np + row];
-      const double w2 = W.jw[2 * np + row];
-      const double w3 = W.jw[3 * np + row];
-      const double r = W.r_pt[row];
-      const double p0 = jp0[row];
-      const double p1 = jp1[row];
-      const double p2 = jp2[row];
-      const double wc[4] = {w0, w1, w2, w3};
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        accp[3 * c] += p0 * wc[c];
-        accp[3 * c + 1] += p1 * wc[c];
-        accp[3 * c + 2] += p2 * wc[c];
+        for (int q = 0; q < 4; ++q) {
+          const double v = Opaque(x[q]);
+          const double xv = t0 + 4 * (q0 + q) + kr < t1 ? v : 0.0;
+          if (q0 + q < groups) g = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, xv, g, 0, 0, 0);
+        }
       }
-      acc[0] += w0 * w0;
-      acc[1] += w0 * w1;
-      acc[2] += w0 * w2;
-      acc[3] += w0 * w3;
-      acc[4] += w1 * w1;
-      acc[5] += w1 * w2;
-      acc[6] += w1 * w3;
-      acc[7] += w2 * w2;
-      acc[8] += w2 * w3;
-      acc[9] += w3 * w3;
-      acc[10] += w0 * r;
-      acc[11] += w1 * r;
-      acc[12] += w2 * r;
-      acc[13] += w3 * r;
+      // Lane l holds D(4 e + l / 16, l % 16) in g[e]: cell a's block in
+      // e = 0, 1 of lanes with l % 16 < 8, cell b's in e = 2, 3 of the rest.
+      const bool lane_b = f16 >= 8;
+      if (!lane_b || has_b) {
+        const int u = lane_b ? ua + 1 : ua;
+        const int j = feat;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int i = 4 * h + kr;
+          const double v = lane_b ? g[2 + h] : g[h];
+          if (i <= j && j < 4) {
+            const int k = i * 4 - (i * (i - 1)) / 2 + (j - i);
+            W.seg_part[k * np + rb + u] = v;
+          } else if (j == 4 && i < 4) {
+            W.seg_part[(10 + i) * np + rb + u] = v;
+          } else if (i >= 5 && j < 4) {
+            const int k = 3 * j + (i - 5);
+            if (lds_fp) {
+              jp_lds[k * kLdsCells + u] = v;
+            } else {
+              W.seg_jp[k * np + rb + u] = v;
+            }
+          }
+        }
+        if (f16 == 0 || f16 == 8) {
+          if (kr == 0) W.seg_cell[rb + u] = ucell[u];
+        }
+      }
     }
-    W.seg_cell[rb + u] = ucell[u];
-#pragma unroll
-    for (int k = 0; k < 14; ++k) W.seg_part[k * np + rb + u] = acc[k];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) W.seg_jp[k * np + rb + u] = accp[k];
   }
   if (tid == 0) W.seg_count[pose] = num_u;
   if (pose == 0) return;  // frame 0 is fixed: no pose rows
@@ -847,11 +984,75 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   const int nx = P.grid.nx;
   const int ncx = nx - 1;
   const int ncy = P.grid.ny - 1;
-  for (int64_t q = A.fp_ptr[pose] + tid; q < A.fp_ptr[pose + 1]; q += kThreads) {
+  if (lds_fp) {
+    // Driven by the occupied cells: the nodes with an occupied cell around
+    // them are the corners of the occupied cells; item (u, cc) is corner cc
+    // of cell u, and the node goes to the item of its first occupied cell
+    // in the order below (cells (w-1,h-1), (w,h-1), (w-1,h), (w,h) of node
+    // (w, h)). That item sums the node's occupied cells in that order and
+    // finds the node's footprint entry by binary search over the pose's
+    // footprint nodes (ascending); an inactive node has none. Every other
+    // footprint entry keeps the zeros KZeroPanels stored.
+    const int64_t q_lo = A.fp_ptr[pose];
+    const int64_t q_hi = A.fp_ptr[pose + 1];
+    auto find_u = [&](int cw, int ch) {
+      if (cw < 0 || ch < 0 || cw >= ncx || ch >= ncy) return -1;
+      const int cid = ch * ncx + cw;
+      int hh = hslot(cid);
+      int key = hkey[hh];
+      while (key != cid && key != -1) {
+        hh = (hh + 1) & (hslots - 1);
+        key = hkey[hh];
+      }
+      return key == -1 ? -1 : hval[hh];
+    };
+    for (int x = tid; x < 4 * num_u; x += kThreads) {
+      const int cc = x & 3;
+      const int c = ucell[x >> 2];
+      const int w = c % ncx + (cc & 1);
+      const int h = c / ncx + (cc >> 1);
+      bool owner = true;
+      for (int k2 = 0; k2 < 3 - cc; ++k2) {
+        if (find_u(w - 1 + (k2 & 1), h - 1 + (k2 >> 1)) >= 0) {
+          owner = false;
+          break;
+        }
+      }
+      if (!owner) continue;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) {
+        const int uu = find_u(w - 1 + (k2 & 1), h - 1 + (k2 >> 1));
+        if (uu < 0) continue;
+        const double* sp = jp_lds + 3 * (3 - k2) * kLdsCells + uu;
+        v0 += sp[0];
+        v1 += sp[kLdsCells];
+        v2 += sp[2 * kLdsCells];
+      }
+      const int g = h * nx + w;
+      int64_t lo = q_lo, hi = q_hi;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (A.fp_node[mid] < g) {
+          lo = mid + 1;
+        } else {
+          hi = mid;
+        }
+      }
+      if (lo >= q_hi || A.fp_node[lo] != g) continue;
+      const int64_t o = A.fp_off[lo];
+      const int64_t st = A.fp_stride[lo];
+      W.panel[o] = v0;
+      W.panel[o + st] = v1;
+      W.panel[o + 2 * st] = v2;
+    }
+  }
+  for (int64_t q = A.fp_ptr[pose] + tid; !lds_fp && q < A.fp_ptr[pose + 1]; q += kThreads) {
     const int g = A.fp_node[q];
     const int w = g % nx;
     const int h = g / nx;
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+    bool any = false;
     const int cws[4] = {w - 1, w, w - 1, w};
     const int chs[4] = {h - 1, h - 1, h, h};
     const int corner[4] = {3, 2, 1, 0};
@@ -872,7 +1073,9 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
       v0 += sp[0];
       v1 += sp[np];
       v2 += sp[2 * np];
+      any = true;
     }
+    if (!any) continue;  // the zeros KZeroPanels stored
     const int64_t o = A.fp_off[q];
     const int64_t st = A.fp_stride[q];
     W.panel[o] = v0;
@@ -886,13 +1089,30 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   double acc[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k] = 0.0;
-  for (int i = tid; i < m; i += kThreads) {
-    const int row = rows[i];
-    if (W.cellkey[row] >= invalid) continue;
-    const double a0 = jp0[row];
-    const double a1 = jp1[row];
-    const double a2 = jp2[row];
-    const double r = W.r_pt[row];
+  for (int base = 0; base < m; base += 4 * kThreads) {
+    // Four rows per thread: every load issued first (clamped), then the
+    // sums in row order.
+    int rw[4];
+    uint32_t ck[4];
+    double la[4][3], lr[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) rw[e] = rows[min(base + tid + e * kThreads, m - 1)];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      ck[e] = W.cellkey[rw[e]];
+      la[e][0] = jp0[rw[e]];
+      la[e][1] = jp1[rw[e]];
+      la[e][2] = jp2[rw[e]];
+      lr[e] = W.r_pt[rw[e]];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+    const double a0 = Opaque(la[e][0]);
+    const double a1 = Opaque(la[e][1]);
+    const double a2 = Opaque(la[e][2]);
+    const double r = Opaque(lr[e]);
+    if (base + tid + e * kThreads >= m) break;
+    if (ck[e] >= invalid) continue;
     acc[0] += a0 * a0;
     acc[1] += a1 * a0;
     acc[2] += a1 * a1;
@@ -902,6 +1122,7 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
     acc[6] += a0 * r;
     acc[7] += a1 * r;
     acc[8] += a2 * r;
+    }
   }
 #pragma unroll
   for (int k = 0; k < 9; ++k) red[k][tid] = acc[k];
@@ -1047,25 +1268,6 @@ __device__ __forceinline__ double* StageSlot(const DeviceWork& W, int s, int kb)
   return W.diag_stage + (2 * static_cast<int64_t>(s) + (kb & 1)) * kNB * kNB;
 }
 
-// The value itself, opaque to the optimizer. A load whose result feeds a
-// select on a wave-uniform condition is otherwise moved under a branch on
-// that condition, where its data is awaited before the next load issues;
-// loads feeding Opaque are issued where they stand.
-__device__ __forceinline__ double Opaque(double v) {
-  __asm__ volatile("" : "+v"(v));
-  return v;
-}
-
-// base[byte_off / 8] for a byte offset below 4 GiB: a wave-uniform base and
-// a 32-bit per-lane offset, which the load takes as its scalar base and
-// vector offset (no 64-bit address per lane).
-__device__ __forceinline__ double LoadAt(const double* base, uint32_t byte_off) {
-  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
-}
-
-// The wavefront's index in the workgroup, as a wave-uniform value.
-__device__ __forceinline__ int WaveIndex() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
-
 // L_kk^-1 of supernode s's kb-th diagonal block (column-major 64 x 64,
 // identity beyond the block's nb columns), kept for the solves.
 __device__ __forceinline__ double* DiagInv(const DeviceAnalysis& A, const DeviceWork& W, int s, int kb) {
@@ -1147,6 +1349,85 @@ __device__ __forceinline__ void FactorPanel(double* lt, double* dvec, double* co
 #pragma unroll
   for (int j = 0; j < 16; ++j) lt[(c + j) * kLd + lane] = r[j];
   if (lane >= c && lane < c + 16) lt[lane * kLd + lane] = 1.0 / dvec[lane];
+}
+
+// The pivots of PanelPivots with the column broadcast by v_readlane
+// instead of the LDS: step k needs T(kC + j, kC + k), j > k, which lane
+// kC + j holds in r[k]; they are read into scalar registers as soon as
+// step k - 1 has updated column k (the first update of each step), so the
+// reads overlap the rest of the step. The arithmetic is PanelPivots', in
+// the same order. With kM the wavefront also forms M_pp = L_pp^-1 of the
+// panel's diagonal block: lane kC + i holds row i of M in m[] and, at step
+// k, rows below k lose L(i, k) times row k of M (final since step k - 1):
+// per entry the same FMAs, in the same order, as InvertDiag16's forward
+// substitution.
+template <int kC, bool kM>
+__device__ __forceinline__ void PanelPivotsRL(double (&r)[16], double (&m)[16], int lane, double* dvec,
+                                              bool& zero) {
+  double c_next = r[0];
+  double d_next = ReadLane(c_next, kC);
+  double r_next = Recip(d_next);
+  double cn[16];
+#pragma unroll
+  for (int j = 1; j < 16; ++j) cn[j] = ReadLane(c_next, kC + j);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const double c = c_next;
+    const double d = d_next;
+    zero = zero || d == 0.0;
+    dvec[kC + k] = d;
+    const double l = c * r_next;
+    r[k] = l;
+    double cj[16];
+#pragma unroll
+    for (int j = k + 1; j < 16; ++j) cj[j] = cn[j];
+    if (k + 1 < 16) {
+      r[k + 1] = fma(-l, cj[k + 1], r[k + 1]);
+      c_next = r[k + 1];
+      d_next = ReadLane(c_next, kC + k + 1);
+      r_next = Recip(d_next);
+#pragma unroll
+      for (int j = k + 2; j < 16; ++j) cn[j] = ReadLane(c_next, kC + j);
+    }
+#pragma unroll
+    for (int j = k + 2; j < 16; ++j) r[j] = fma(-l, cj[j], r[j]);
+    if constexpr (kM) {
+      if (k > 0) {
+        double mk[16];
+#pragma unroll
+        for (int q = 0; q < k; ++q) mk[q] = ReadLane(m[q], kC + k);
+        if (lane > kC + k) {
+#pragma unroll
+          for (int q = 0; q < k; ++q) m[q] = fma(-l, mk[q], m[q]);
+        }
+      }
+      if (lane > kC + k) m[k] = fma(-l, 1.0, m[k]);
+    }
+  }
+}
+
+// FactorPanel with PanelPivotsRL; with kM also M_pp, stored in the tile's
+// other triangle (as InvertDiag16 stores it), after the panel.
+template <int kLd, int kP, bool kM>
+__device__ __forceinline__ void FactorPanelRL(double* lt, double* dvec, int nb, int lane, bool& zero) {
+  constexpr int c = 16 * kP;
+  double r[16], m[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double v = lt[(c + j) * kLd + lane];
+    r[j] = (lane < nb && c + j < nb && c + j <= lane) ? v : (c + j == lane ? 1.0 : 0.0);
+    m[j] = c + j == lane ? 1.0 : 0.0;
+  }
+  PanelPivotsRL<c, kM>(r, m, lane, dvec, zero);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) lt[(c + j) * kLd + lane] = r[j];
+  if (lane >= c && lane < c + 16) lt[lane * kLd + lane] = 1.0 / dvec[lane];
+  if constexpr (kM) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      if (lane > c + q && lane < c + 16) lt[lane * kLd + c + q] = m[q];
+    }
+  }
 }
 
 // Trailing update after panel kP: every lower 16x16 block (I, J),
@@ -1296,6 +1577,94 @@ __device__ __forceinline__ bool FactorInvertBlock(double* lt, double* dvec, doub
       InvertDiag16<kLd>(lt, 32, lane);
     } else if (wave == 2) {
       InvertOffDiag<kLd>(lt, 1, 0, lane);
+    }
+    __syncthreads();
+  }
+  const int last = np - 1;
+  if (wave == 0 && lane < 16) {
+    InvertDiag16<kLd>(lt, 16 * last, lane);
+  } else if (wave == 1 && last >= 2) {
+    for (int q = 0; q < last - 1; ++q) InvertOffDiag<kLd>(lt, last - 1, q, lane);
+  }
+  __syncthreads();
+  if (wave < last) InvertOffDiag<kLd>(lt, last, wave, lane);
+  __syncthreads();
+  return zero;
+}
+
+// FactorInvertBlock with the same results (every entry of L, D, 1 / D and
+// M is formed by the same operations) on a shorter critical path: the
+// panels' pivots broadcast their columns by v_readlane (PanelPivotsRL),
+// and a full block (four panels) forms M sooner: M_00 and M_11 beside
+// panels 1 and 2 (wave 1), M_10 beside the trailing update after panel 2
+// (which has one block, for wave 0), M_33 by the last panel itself during
+// its pivots, M_22 beside it (waves 1 and 2 each form it, the same values,
+// then M_21 and M_20), so that only the last block row of M (one block per
+// wavefront) follows the last panel. A block with fewer panels keeps
+// FactorInvertBlock's schedule.
+template <int kLd>
+__device__ __forceinline__ bool FactorInvertBlockRL(double* lt, double* dvec, int nb) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int np = (nb + 15) >> 4;  // panels with a column inside the block
+  bool zero = false;
+  for (int idx = tid; idx < kNB * kNB; idx += kThreads) {
+    const int i = idx & (kNB - 1);
+    const int j = idx / kNB;
+    if (j >= 16 * np) lt[j * kLd + i] = i == j ? 1.0 : 0.0;
+  }
+  if (tid >= 16 * np && tid < kNB) dvec[tid] = 1.0;
+  if (wave == 0) FactorPanelRL<kLd, 0, false>(lt, dvec, nb, lane, zero);
+  __syncthreads();
+  if (np == 4) {
+    UpdatePanel<kLd, 0>(lt, dvec, np, wave, lane);
+    __syncthreads();
+    if (wave == 0) {
+      FactorPanelRL<kLd, 1, false>(lt, dvec, nb, lane, zero);
+    } else if (wave == 1 && lane < 16) {
+      InvertDiag16<kLd>(lt, 0, lane);
+    }
+    __syncthreads();
+    UpdatePanel<kLd, 1>(lt, dvec, np, wave, lane);
+    __syncthreads();
+    if (wave == 0) {
+      FactorPanelRL<kLd, 2, false>(lt, dvec, nb, lane, zero);
+    } else if (wave == 1 && lane < 16) {
+      InvertDiag16<kLd>(lt, 16, lane);
+    }
+    __syncthreads();
+    UpdatePanel<kLd, 2>(lt, dvec, np, wave, lane);
+    if (wave == 1) InvertOffDiag<kLd>(lt, 1, 0, lane);
+    __syncthreads();
+    if (wave == 0) {
+      FactorPanelRL<kLd, 3, true>(lt, dvec, nb, lane, zero);
+    } else if (wave <= 2) {
+      if (lane < 16) InvertDiag16<kLd>(lt, 32, lane);
+      InvertOffDiag<kLd>(lt, 2, 2 - wave, lane);
+    }
+    __syncthreads();
+    if (wave < 3) InvertOffDiag<kLd>(lt, 3, wave, lane);
+    __syncthreads();
+    return zero;
+  }
+  if (np > 1) {
+    UpdatePanel<kLd, 0>(lt, dvec, np, wave, lane);
+    __syncthreads();
+    if (wave == 0) {
+      FactorPanelRL<kLd, 1, false>(lt, dvec, nb, lane, zero);
+    } else if (wave == 1 && lane < 16) {
+      InvertDiag16<kLd>(lt, 0, lane);
+    }
+    __syncthreads();
+  }
+  if (np > 2) {
+    UpdatePanel<kLd, 1>(lt, dvec, np, wave, lane);
+    __syncthreads();
+    if (wave == 0) {
+      FactorPanelRL<kLd, 2, false>(lt, dvec, nb, lane, zero);
+    } else if (wave == 1 && lane < 16) {
+      InvertDiag16<kLd>(lt, 16, lane);
     }
     __syncthreads();
   }
