@@ -326,38 +326,63 @@ __global__ void KAssembleOdom(Slots slots) {
 
 // Per cell: the 10 products of its node weights and the 4 weight x residual
 // sums over the cell's points of every frame. Each frame's pose kernel has
-// summed its own points per occupied cell; here one wavefront per cell walks
-// the cell's candidate frames in ascending order (lanes in a fixed stride,
-// one binary search each, then a fixed butterfly).
+// summed its own points per occupied cell; here the cell's candidate frames
+// are added in a fixed order: slot s (0..63) sums the candidates s, s + 64,
+// ... in order, and the 64 slot sums meet in an xor butterfly (pairs 32
+// apart, then 16, 8, 4, 2, 1). Sixteen lanes take a cell (four cells per
+// wavefront): lane j holds slots j, j + 16, j + 32 and j + 48, forms the
+// butterfly's first two levels itself ((s_j + s_j+32) + (s_j+16 + s_j+48))
+// and the last four with its group; the result is the same sum as with a
+// wavefront per cell. The four candidates a lane takes per pass have their
+// loads issued together.
 template <class Slots>
-__global__ void KCellGather(Slots slots) {
+__global__ void __launch_bounds__(kThreads) KCellGather(Slots slots) {
   BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int lane = threadIdx.x & 63;
-  const int cell = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-  if (cell >= P.num_cells) return;
+  const int j = lane & 15;
+  const int cell = (blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)) * 4 + (lane >> 4);
+  const bool valid = cell < P.num_cells;
   const int64_t np = P.num_points;
-  double acc[14];
+  double acc[4][14];
 #pragma unroll
-  for (int k = 0; k < 14; ++k) acc[k] = 0.0;
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int k = 0; k < 14; ++k) acc[t][k] = 0.0;
   // Candidate frame q holds the cell at rank cp_rank[q] of its dilated
   // cells; KPoseBlock mapped the rank to the frame's segment of the cell.
-  for (int64_t q = A.cp_ptr[cell] + lane; q < A.cp_ptr[cell + 1]; q += 64) {
-    const int pose = A.cp_pose[q];
-    const int u = W.seg_map[A.dl_ptr[pose] + A.cp_rank[q]];
-    if (u >= 0) {
-      const int64_t rb = A.pr_ptr[pose];
+  const int64_t q_lo = valid ? A.cp_ptr[cell] : 0;
+  const int64_t q_hi = valid ? A.cp_ptr[cell + 1] : 0;
+  for (int64_t q0 = q_lo; q0 < q_hi; q0 += 64) {
+    int64_t base[4];
+    int u[4];
 #pragma unroll
-      for (int k = 0; k < 14; ++k) acc[k] += W.seg_part[k * np + rb + u];
+    for (int t = 0; t < 4; ++t) {
+      const int64_t q = min(q0 + j + 16 * t, q_hi - 1);
+      const int pose = A.cp_pose[q];
+      u[t] = W.seg_map[A.dl_ptr[pose] + A.cp_rank[q]];
+      base[t] = A.pr_ptr[pose];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (q0 + j + 16 * t >= q_hi || u[t] < 0) continue;
+#pragma unroll
+      for (int k = 0; k < 14; ++k) acc[t][k] += W.seg_part[k * np + base[t] + u[t]];
     }
   }
+  double sum[14];
 #pragma unroll
-  for (int k = 0; k < 14; ++k) acc[k] = WaveSum(acc[k]);
-  if (lane == 0) {
+  for (int k = 0; k < 14; ++k) {
+    double v = (acc[0][k] + acc[2][k]) + (acc[1][k] + acc[3][k]);
 #pragma unroll
-    for (int k = 0; k < 10; ++k) W.nn[static_cast<int64_t>(k) * P.num_cells + cell] = acc[k];
+    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    sum[k] = v;
+  }
+  if (valid && j == 0) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) W.rn[static_cast<int64_t>(k) * P.num_cells + cell] = acc[10 + k];
+    for (int k = 0; k < 10; ++k) W.nn[static_cast<int64_t>(k) * P.num_cells + cell] = sum[k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) W.rn[static_cast<int64_t>(k) * P.num_cells + cell] = sum[10 + k];
   }
 }
 
@@ -3293,7 +3318,7 @@ void NormalSlotsImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& 
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KPoseBlock<Slots>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
   KPoseBlock<<<dim3(sh.poses, k), kThreads, lds, s>>>(slots, o.marquardt_scaling, sort_size);
-  KCellGather<<<dim3(Blocks(sh.cells, kThreads / 64), k), kThreads, 0, s>>>(slots);
+  KCellGather<<<dim3(Blocks(sh.cells, kThreads / 16), k), kThreads, 0, s>>>(slots);
   KNodeSlots<<<dim3(Blocks(sh.nodes), k), kThreads, 0, s>>>(slots, o.marquardt_scaling);
   if (sh.pairs > 0) {
     KOdomPairs<<<dim3(Blocks(sh.pairs), k), kThreads, 0, s>>>(slots);
