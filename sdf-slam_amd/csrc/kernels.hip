@@ -1376,85 +1376,6 @@ __device__ __forceinline__ void FactorPanel(double* lt, double* dvec, double* co
   if (lane >= c && lane < c + 16) lt[lane * kLd + lane] = 1.0 / dvec[lane];
 }
 
-// The pivots of PanelPivots with the column broadcast by v_readlane
-// instead of the LDS: step k needs T(kC + j, kC + k), j > k, which lane
-// kC + j holds in r[k]; they are read into scalar registers as soon as
-// step k - 1 has updated column k (the first update of each step), so the
-// reads overlap the rest of the step. The arithmetic is PanelPivots', in
-// the same order. With kM the wavefront also forms M_pp = L_pp^-1 of the
-// panel's diagonal block: lane kC + i holds row i of M in m[] and, at step
-// k, rows below k lose L(i, k) times row k of M (final since step k - 1):
-// per entry the same FMAs, in the same order, as InvertDiag16's forward
-// substitution.
-template <int kC, bool kM>
-__device__ __forceinline__ void PanelPivotsRL(double (&r)[16], double (&m)[16], int lane, double* dvec,
-                                              bool& zero) {
-  double c_next = r[0];
-  double d_next = ReadLane(c_next, kC);
-  double r_next = Recip(d_next);
-  double cn[16];
-#pragma unroll
-  for (int j = 1; j < 16; ++j) cn[j] = ReadLane(c_next, kC + j);
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const double c = c_next;
-    const double d = d_next;
-    zero = zero || d == 0.0;
-    dvec[kC + k] = d;
-    const double l = c * r_next;
-    r[k] = l;
-    double cj[16];
-#pragma unroll
-    for (int j = k + 1; j < 16; ++j) cj[j] = cn[j];
-    if (k + 1 < 16) {
-      r[k + 1] = fma(-l, cj[k + 1], r[k + 1]);
-      c_next = r[k + 1];
-      d_next = ReadLane(c_next, kC + k + 1);
-      r_next = Recip(d_next);
-#pragma unroll
-      for (int j = k + 2; j < 16; ++j) cn[j] = ReadLane(c_next, kC + j);
-    }
-#pragma unroll
-    for (int j = k + 2; j < 16; ++j) r[j] = fma(-l, cj[j], r[j]);
-    if constexpr (kM) {
-      if (k > 0) {
-        double mk[16];
-#pragma unroll
-        for (int q = 0; q < k; ++q) mk[q] = ReadLane(m[q], kC + k);
-        if (lane > kC + k) {
-#pragma unroll
-          for (int q = 0; q < k; ++q) m[q] = fma(-l, mk[q], m[q]);
-        }
-      }
-      if (lane > kC + k) m[k] = fma(-l, 1.0, m[k]);
-    }
-  }
-}
-
-// FactorPanel with PanelPivotsRL; with kM also M_pp, stored in the tile's
-// other triangle (as InvertDiag16 stores it), after the panel.
-template <int kLd, int kP, bool kM>
-__device__ __forceinline__ void FactorPanelRL(double* lt, double* dvec, int nb, int lane, bool& zero) {
-  constexpr int c = 16 * kP;
-  double r[16], m[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const double v = lt[(c + j) * kLd + lane];
-    r[j] = (lane < nb && c + j < nb && c + j <= lane) ? v : (c + j == lane ? 1.0 : 0.0);
-    m[j] = c + j == lane ? 1.0 : 0.0;
-  }
-  PanelPivotsRL<c, kM>(r, m, lane, dvec, zero);
-#pragma unroll
-  for (int j = 0; j < 16; ++j) lt[(c + j) * kLd + lane] = r[j];
-  if (lane >= c && lane < c + 16) lt[lane * kLd + lane] = 1.0 / dvec[lane];
-  if constexpr (kM) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      if (lane > c + q && lane < c + 16) lt[lane * kLd + c + q] = m[q];
-    }
-  }
-}
-
 // Trailing update after panel kP: every lower 16x16 block (I, J),
 // kP < J <= I < np, loses L_I,kP D_kP L_J,kP^T, one FP64 MFMA chain of
 // four (16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and
@@ -1617,94 +1538,6 @@ __device__ __forceinline__ bool FactorInvertBlock(double* lt, double* dvec, doub
   return zero;
 }
 
-// FactorInvertBlock with the same results (every entry of L, D, 1 / D and
-// M is formed by the same operations) on a shorter critical path: the
-// panels' pivots broadcast their columns by v_readlane (PanelPivotsRL),
-// and a full block (four panels) forms M sooner: M_00 and M_11 beside
-// panels 1 and 2 (wave 1), M_10 beside the trailing update after panel 2
-// (which has one block, for wave 0), M_33 by the last panel itself during
-// its pivots, M_22 beside it (waves 1 and 2 each form it, the same values,
-// then M_21 and M_20), so that only the last block row of M (one block per
-// wavefront) follows the last panel. A block with fewer panels keeps
-// FactorInvertBlock's schedule.
-template <int kLd>
-__device__ __forceinline__ bool FactorInvertBlockRL(double* lt, double* dvec, int nb) {
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int np = (nb + 15) >> 4;  // panels with a column inside the block
-  bool zero = false;
-  for (int idx = tid; idx < kNB * kNB; idx += kThreads) {
-    const int i = idx & (kNB - 1);
-    const int j = idx / kNB;
-    if (j >= 16 * np) lt[j * kLd + i] = i == j ? 1.0 : 0.0;
-  }
-  if (tid >= 16 * np && tid < kNB) dvec[tid] = 1.0;
-  if (wave == 0) FactorPanelRL<kLd, 0, false>(lt, dvec, nb, lane, zero);
-  __syncthreads();
-  if (np == 4) {
-    UpdatePanel<kLd, 0>(lt, dvec, np, wave, lane);
-    __syncthreads();
-    if (wave == 0) {
-      FactorPanelRL<kLd, 1, false>(lt, dvec, nb, lane, zero);
-    } else if (wave == 1 && lane < 16) {
-      InvertDiag16<kLd>(lt, 0, lane);
-    }
-    __syncthreads();
-    UpdatePanel<kLd, 1>(lt, dvec, np, wave, lane);
-    __syncthreads();
-    if (wave == 0) {
-      FactorPanelRL<kLd, 2, false>(lt, dvec, nb, lane, zero);
-    } else if (wave == 1 && lane < 16) {
-      InvertDiag16<kLd>(lt, 16, lane);
-    }
-    __syncthreads();
-    UpdatePanel<kLd, 2>(lt, dvec, np, wave, lane);
-    if (wave == 1) InvertOffDiag<kLd>(lt, 1, 0, lane);
-    __syncthreads();
-    if (wave == 0) {
-      FactorPanelRL<kLd, 3, true>(lt, dvec, nb, lane, zero);
-    } else if (wave <= 2) {
-      if (lane < 16) InvertDiag16<kLd>(lt, 32, lane);
-      InvertOffDiag<kLd>(lt, 2, 2 - wave, lane);
-    }
-    __syncthreads();
-    if (wave < 3) InvertOffDiag<kLd>(lt, 3, wave, lane);
-    __syncthreads();
-    return zero;
-  }
-  if (np > 1) {
-    UpdatePanel<kLd, 0>(lt, dvec, np, wave, lane);
-    __syncthreads();
-    if (wave == 0) {
-      FactorPanelRL<kLd, 1, false>(lt, dvec, nb, lane, zero);
-    } else if (wave == 1 && lane < 16) {
-      InvertDiag16<kLd>(lt, 0, lane);
-    }
-    __syncthreads();
-  }
-  if (np > 2) {
-    UpdatePanel<kLd, 1>(lt, dvec, np, wave, lane);
-    __syncthreads();
-    if (wave == 0) {
-      FactorPanelRL<kLd, 2, false>(lt, dvec, nb, lane, zero);
-    } else if (wave == 1 && lane < 16) {
-      InvertDiag16<kLd>(lt, 16, lane);
-    }
-    __syncthreads();
-  }
-  const int last = np - 1;
-  if (wave == 0 && lane < 16) {
-    InvertDiag16<kLd>(lt, 16 * last, lane);
-  } else if (wave == 1 && last >= 2) {
-    for (int q = 0; q < last - 1; ++q) InvertOffDiag<kLd>(lt, last - 1, q, lane);
-  }
-  __syncthreads();
-  if (wave < last) InvertOffDiag<kLd>(lt, last, wave, lane);
-  __syncthreads();
-  return zero;
-}
-
 // Stores a block left by FactorInvertBlock, kN threads cooperating (thread
 // tid takes entries tid, tid + kN, ... in column-major order): the staging
 // slot (L below the diagonal, D on it, W above; identity padding beyond nb
@@ -1753,7 +1586,7 @@ __device__ __forceinline__ void LoadDiagBlock(const double* F, int f, int c0, in
 // the front's staging slot (with the zero-pivot flag), for KPanelStep's
 // staged mode.
 template <class Slots, class Lists>
-__global__ void __launch_bounds__(kThreads) KDiagBlock(Slots slots, Lists plan, int kb) {
+__global__ void __launch_bounds__(kThreads, 2) KDiagBlock(Slots slots, Lists plan, int kb) {
   BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int s = PlanItem(plan, blockIdx.y, blockIdx.x);
