@@ -1281,6 +1281,100 @@ __global__ void KExtendAddPanel(Slots slots, Lists plan) {
   }
 }
 
+// Fronts up to this many rows take the LDS form of the panel extend-add.
+constexpr int kExtendLdsRows = 2048;
+static_assert(kWaves * kExtendLdsRows * 8 <= 65536, "the LDS form fits the default LDS limit");
+
+// KExtendAddPanel with the parent's column staged in the LDS (one column of
+// at most ldf rows per wavefront): the column is read once (coalesced), the
+// children's maps from the parent's own columns are read together, then
+// each child's entries (rows b .. mc - 1 of its column b, in rounds of 1024:
+// the round's map and value loads of two children issued together) are
+// added in the LDS child by child in ascending order, and the column is
+// written back once. The same additions in the same order as the global
+// form; a column's wavefront has no other writer, so no barrier is needed
+// (the LDS operations of one wavefront complete in order).
+template <class Slots, class Lists>
+__global__ void __launch_bounds__(kThreads) KExtendAddPanelLds(Slots slots, Lists plan, int ldf) {
+  BIND_SLOT(blockIdx.z);
+  if (Halted(W.state)) return;
+  const int p = PlanItem(plan, blockIdx.z, blockIdx.y);
+  if (p < 0) return;
+  const int wave = static_cast<int>(threadIdx.x >> 6);
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const int j = static_cast<int>(blockIdx.x) * kWaves + wave;
+  if (j >= A.sn_first[p + 1] - A.sn_first[p]) return;
+  const int fp = A.sn_f[p];
+  double* fcol = W.panel + A.panel_off[p] + static_cast<int64_t>(j) * fp;
+  extern __shared__ double ext_lds[];
+  double* colv = ext_lds + wave * ldf;
+  constexpr int kPer = 16;  // entries per lane per round
+  for (int i0 = 0; i0 < fp; i0 += kPer * 64) {
+    double v[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) v[q] = fcol[min(i0 + lane + 64 * q, fp - 1)];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      if (i0 + lane + 64 * q < fp) colv[i0 + lane + 64 * q] = v[q];
+    }
+  }
+  const int64_t c_lo = A.children_ptr[p];
+  const int64_t c_hi = A.children_ptr[p + 1];
+  for (int64_t ci = c_lo; ci < c_hi; ci += 2) {
+    // Two children at a time: their columns (or none) and their entries.
+    int b[2], mc[2];
+    const int32_t* rel[2];
+    const double* src[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      b[h] = -1;
+      mc[h] = 0;
+      rel[h] = nullptr;
+      src[h] = nullptr;
+      if (ci + h < c_hi) {
+        const int c = A.children[ci + h];
+        const int64_t poff = A.pinv_off[c];
+        if (poff >= 0) {
+          b[h] = A.pinv[poff + j];
+          mc[h] = A.sn_m[c];
+          rel[h] = A.rel + A.rel_off[c];
+          src[h] = W.upd + A.upd_off[c] + static_cast<int64_t>(max(b[h], 0)) * mc[h];
+        }
+      }
+    }
+    const int span = max(b[0] >= 0 ? mc[0] - b[0] : 0, b[1] >= 0 ? mc[1] - b[1] : 0);
+    for (int g = 0; g < span; g += kPer * 64) {
+      int r[2][kPer];
+      double u[2][kPer];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (b[h] < 0) continue;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          const int a = min(b[h] + g + lane + 64 * q, mc[h] - 1);
+          r[h][q] = rel[h][a];
+          u[h][q] = src[h][a];
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (b[h] < 0) continue;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+          if (b[h] + g + lane + 64 * q < mc[h]) colv[r[h][q]] = colv[r[h][q]] + u[h][q];
+        }
+      }
+    }
+  }
+  for (int i0 = 0; i0 < fp; i0 += kPer * 64) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int i = i0 + lane + 64 * q;
+      if (i < fp) fcol[i] = colv[i];
+    }
+  }
+}
+
 // Row stride (doubles) of the panel step's LDS tiles: two doubles of
 // padding put the four k rows an MFMA operand read touches in different
 // banks.
@@ -3150,7 +3244,7 @@ void NormalSlotsImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& 
   const size_t lds = static_cast<size_t>(sort_size) * (8 + 4 + 4 + 4) + 4;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KPoseBlock<Slots>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-  KPoseBlock<<<dim3(sh.poses, k), kThreads, lds, s>>>(slots, o.marquardt_scaling, sort_size);
+  KPoseBlock<Slots><<<dim3(sh.poses, k), kThreads, lds, s>>>(slots, o.marquardt_scaling, sort_size);
   KCellGather<<<dim3(Blocks(sh.cells, kThreads / 16), k), kThreads, 0, s>>>(slots);
   KNodeSlots<<<dim3(Blocks(sh.nodes), k), kThreads, 0, s>>>(slots, o.marquardt_scaling);
   if (sh.pairs > 0) {
@@ -3174,8 +3268,14 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
     const LaunchShape::Level& lv = sh.levels[static_cast<size_t>(L)];
     const int count = lv.count;
     if (count > 0 && lv.has_children) {
-      KExtendAddPanel<<<dim3((lv.max_nb + kWaves - 1) / kWaves, count, k), kThreads, 0, s>>>(
-          slots, LevelLists<kOne>(sh, L));
+      const dim3 grid((lv.max_nb + kWaves - 1) / kWaves, count, k);
+      if (lv.max_f <= kExtendLdsRows) {
+        const int ldf = lv.max_f;  // at most 64 KiB of LDS: no attribute needed
+        const size_t lds = static_cast<size_t>(kWaves) * ldf * sizeof(double);
+        KExtendAddPanelLds<<<grid, kThreads, lds, s>>>(slots, LevelLists<kOne>(sh, L), ldf);
+      } else {
+        KExtendAddPanel<<<grid, kThreads, 0, s>>>(slots, LevelLists<kOne>(sh, L));
+      }
     }
     if (count > 0) {
       const auto plan = LevelLists<kOne>(sh, L);
