@@ -240,6 +240,13 @@ int sdfslam_hip_context_point_rows(sdfslam_hip_context* ctx, int64_t capacity, i
  * (the same results; for kernel profilers and debuggers that do not follow
  * graph launches). No counterpart in the reference. */
 int sdfslam_hip_set_graphs(int32_t enabled);
+/* Process-wide: the pose-block and panel extend-add kernels use their LDS
+ * forms (enabled, the default) or their global-memory forms, which give the
+ * same results bit for bit (the forms a frame with more than 192 occupied
+ * cells or a front of more than 2048 rows takes anyway). Launches made or
+ * captured after the call follow it. No counterpart in the reference (its
+ * CPU path has no kernels); for tests that exercise both forms. */
+int sdfslam_hip_set_lds_forms(int32_t enabled);
 /* Footprint margin (cells) of every pose in the current pattern epoch:
  * min(capacity, num_poses) entries into margins (may be NULL), the pose
  * count into *num_poses (may be NULL). */

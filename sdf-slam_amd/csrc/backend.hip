@@ -2142,6 +2142,11 @@ int sdfslam_hip_set_graphs(int32_t enabled) {
   return SDFSLAM_OK;
 }
 
+int sdfslam_hip_set_lds_forms(int32_t enabled) {
+  SetLdsForms(enabled != 0);
+  return SDFSLAM_OK;
+}
+
 int sdfslam_hip_pose_margins(sdfslam_hip_context* ctx, int32_t* margins, int32_t capacity,
                              int32_t* num_poses) {
   return Guard([&]() -> int {

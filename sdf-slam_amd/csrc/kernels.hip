@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <type_traits>
@@ -38,6 +39,9 @@ __device__ __forceinline__ double WaveSum(double v) {
   }
   return v;
 }
+
+// The pose-block and panel extend-add kernels' LDS forms (SetLdsForms).
+std::atomic<bool> g_lds_forms{true};
 
 // Where a kernel finds its slot: one slot passed by value as a kernel
 // argument (a single problem: no memory indirection before the kernel's
@@ -588,7 +592,7 @@ __device__ __forceinline__ int BlockExclusiveScan(int32_t* buf, int v) {
 }
 
 template <class Slots>
-__global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
+__global__ void KPoseBlock(Slots slots, int marquardt, int sort_size, int lds_ok) {
   BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int pose = blockIdx.x;
@@ -880,7 +884,7 @@ __global__ void KPoseBlock(Slots slots, int marquardt, int sort_size) {
   // sums go through global memory and a node's cells are found by binary
   // search behind a bitmap of the occupied cells (in `red`).
   const int hslots = sort_size / 2;
-  const bool lds_fp = num_u <= kLdsCells && hslots >= 2 && 2 * num_u <= hslots;
+  const bool lds_fp = lds_ok != 0 && num_u <= kLdsCells && hslots >= 2 && 2 * num_u <= hslots;
   int32_t* hkey = order + sort_size;
   int32_t* hval = hkey + hslots;
   const int hshift = 32 - __builtin_ctz(static_cast<unsigned>(max(hslots, 2)));
@@ -1504,15 +1508,19 @@ __device__ __forceinline__ void UpdatePanel(double* lt, const double* dvec, int 
 // M_pp = L_pp^-1 of the 16x16 unit lower diagonal block at `base`, lanes
 // 0 .. 15 of a wavefront (lane c forms column c by forward substitution);
 // M(r, c) goes to lt[r * kLd + c] (r > c), the other triangle of the tile.
+// Row j of the column is y_j = -sum_t L(j, t) y_t over t = 0 .. j - 1 in
+// ascending order (one FMA chain per row, rows in order), so only one row
+// of L is read at a time.
 template <int kLd>
 __device__ __forceinline__ void InvertDiag16(double* lt, int base, int c) {
   double y[16];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) y[j] = j == c ? 1.0 : 0.0;
+  for (int j = 0; j < 16; ++j) {
+    double acc = j == c ? 1.0 : 0.0;
 #pragma unroll
-  for (int t = 0; t < 15; ++t) {
-#pragma unroll
-    for (int j = t + 1; j < 16; ++j) y[j] = fma(-lt[(base + t) * kLd + base + j], y[t], y[j]);
+    for (int t = 0; t < j; ++t) acc = fma(-lt[(base + t) * kLd + base + j], y[t], acc);
+    y[j] = acc;
+    __asm__ volatile("" ::: "memory");
   }
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
@@ -1679,8 +1687,14 @@ __device__ __forceinline__ void LoadDiagBlock(const double* F, int f, int c0, in
 // front: factored and inverted in the LDS (FactorInvertBlock) and stored in
 // the front's staging slot (with the zero-pivot flag), for KPanelStep's
 // staged mode.
+// A single problem's kernel has its registers capped at two workgroups per
+// compute unit (its leaf level has hundreds of fronts); the slot-table form
+// keeps the larger budget it needs without spilling.
+template <class Slots>
+constexpr int kDiagBlockOccupancy = std::is_same<Slots, OneSlot>::value ? 2 : 1;
+
 template <class Slots, class Lists>
-__global__ void __launch_bounds__(kThreads, 2) KDiagBlock(Slots slots, Lists plan, int kb) {
+__global__ void __launch_bounds__(kThreads, kDiagBlockOccupancy<Slots>) KDiagBlock(Slots slots, Lists plan, int kb) {
   BIND_SLOT(blockIdx.y);
   if (Halted(W.state)) return;
   const int s = PlanItem(plan, blockIdx.y, blockIdx.x);
@@ -3244,7 +3258,8 @@ void NormalSlotsImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& 
   const size_t lds = static_cast<size_t>(sort_size) * (8 + 4 + 4 + 4) + 4;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&KPoseBlock<Slots>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-  KPoseBlock<Slots><<<dim3(sh.poses, k), kThreads, lds, s>>>(slots, o.marquardt_scaling, sort_size);
+  KPoseBlock<Slots><<<dim3(sh.poses, k), kThreads, lds, s>>>(slots, o.marquardt_scaling, sort_size,
+                                                             g_lds_forms.load() ? 1 : 0);
   KCellGather<<<dim3(Blocks(sh.cells, kThreads / 16), k), kThreads, 0, s>>>(slots);
   KNodeSlots<<<dim3(Blocks(sh.nodes), k), kThreads, 0, s>>>(slots, o.marquardt_scaling);
   if (sh.pairs > 0) {
@@ -3269,7 +3284,7 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
     const int count = lv.count;
     if (count > 0 && lv.has_children) {
       const dim3 grid((lv.max_nb + kWaves - 1) / kWaves, count, k);
-      if (lv.max_f <= kExtendLdsRows) {
+      if (g_lds_forms.load() && lv.max_f <= kExtendLdsRows) {
         const int ldf = lv.max_f;  // at most 64 KiB of LDS: no attribute needed
         const size_t lds = static_cast<size_t>(kWaves) * ldf * sizeof(double);
         KExtendAddPanelLds<<<grid, kThreads, lds, s>>>(slots, LevelLists<kOne>(sh, L), ldf);
@@ -3447,6 +3462,8 @@ void LaunchAssembleEikonalJ(const SlotArgs& slots, const LaunchShape& sh, double
     KAssembleEikonal<<<dim3(Blocks(sh.eikonal), 1), kThreads, 0, s>>>(SlotTable{slots.table}, j_eik);
   }
 }
+
+void SetLdsForms(bool enabled) { g_lds_forms.store(enabled); }
 
 void LaunchNormalSlots(const SlotArgs& slots, const LaunchShape& sh, const LmOptions& o, hipStream_t s) {
   if (OneOf(slots, sh)) {

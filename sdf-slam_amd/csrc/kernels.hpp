@@ -229,6 +229,10 @@ void LaunchAssemble(const SlotArgs& slots, const LaunchShape& sh, bool check_foo
                     hipEvent_t before = nullptr, hipEvent_t after = nullptr);
 /// Zeroes the factor panels and forms the damped J^T J and -J^T r in them.
 void LaunchNormalSlots(const SlotArgs& slots, const LaunchShape& sh, const LmOptions& o, hipStream_t s);
+/// Process-wide: the pose-block and panel extend-add kernels' LDS forms
+/// (default) or their global-memory forms, which give the same results (for
+/// tests); launches made or captured after the call follow it.
+void SetLdsForms(bool enabled);
 
 /// Optional timing of the panel steps: pairs (events[2k], events[2k+1])
 /// bracket the diagonal-block and panel-step kernels of one 64-column step; pairs

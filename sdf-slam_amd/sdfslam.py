@@ -200,6 +200,7 @@ HIP_SYMBOLS = [
     "sdfslam_hip_init_from_scans_noalloc", "sdfslam_hip_reference_factor_counts",
     "sdfslam_hip_elimination_order", "sdfslam_hip_context_update_device_normals",
     "sdfslam_hip_context_append_frames", "sdfslam_hip_context_point_rows", "sdfslam_hip_set_graphs",
+    "sdfslam_hip_set_lds_forms",
 ]
 HOST_SYMBOLS = [
     "sdfslam_host_default_problem_options", "sdfslam_host_default_solver_options",
@@ -879,6 +880,13 @@ def set_graphs(enabled):
     """LM iterations as captured HIP graphs (default) or direct launches
     (sdfslam_hip_set_graphs), process-wide."""
     _check_hip(hip.sdfslam_hip_set_graphs(1 if enabled else 0))
+
+
+def set_lds_forms(enabled):
+    """The pose-block and extend-add kernels' LDS forms (default) or their
+    global-memory forms, the same results (sdfslam_hip_set_lds_forms),
+    process-wide, for launches made or captured after the call."""
+    _check_hip(hip.sdfslam_hip_set_lds_forms(1 if enabled else 0))
 
 
 def served_benchmark(sessions=32, rate_hz=10.0, duration_s=10.0, workers=12, max_batch=64,
