@@ -997,6 +997,7 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     lv.has_children = an.level_max_children[static_cast<size_t>(L)] > 0;
     for (const int s : an.level_sn[static_cast<size_t>(L)]) {
       const int t = (an.sn_m[static_cast<size_t>(s)] + 63) / 64;
+      if (t == 0) lv.all_update = false;
       for (int ti = 0; ti < t; ++ti) {
         for (int tj = 0; tj <= ti; ++tj) {
           tiles.push_back(s);
@@ -1066,6 +1067,7 @@ void InstallEpoch(sdfslam_hip_context& c, Analysis&& analysis, const double* anc
     c.dw.upd = E.Alloc<double>(static_cast<size_t>(an.upd_total));
     c.dw.diag_stage = E.Alloc<double>(static_cast<size_t>(an.num_sn) * 2 * 64 * 64);
     c.dw.diag_inv = E.Alloc<double>(static_cast<size_t>(inv_off.back()));
+    c.dw.diag_d = E.Alloc<double>(static_cast<size_t>(inv_off.back() / 64));
     // Multi-workgroup solve words: 3 per supernode and one per 64-column
     // block, uploaded as zeros.
     c.solve_sync_words = static_cast<size_t>(inv_off.back() / (64 * 64) + 3 * an.num_sn);

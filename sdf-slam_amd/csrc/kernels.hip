@@ -1397,6 +1397,12 @@ __device__ __forceinline__ double* DiagInv(const DeviceAnalysis& A, const Device
   return W.diag_inv + A.inv_off[s] + static_cast<int64_t>(kb) * kNB * kNB;
 }
 
+// D of supernode s's kb-th diagonal block (64 entries, padded as the
+// inverses), kept for the update matrices.
+__device__ __forceinline__ double* DiagD(const DeviceAnalysis& A, const DeviceWork& W, int s, int kb) {
+  return W.diag_d + A.inv_off[s] / (kNB * kNB) * kNB + static_cast<int64_t>(kb) * kNB;
+}
+
 // Reads lane `lane`'s value of v (a wave-uniform lane index).
 __device__ __forceinline__ double ReadLane(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -1643,10 +1649,12 @@ __device__ __forceinline__ bool FactorInvertBlock(double* lt, double* dvec, doub
 // Stores a block left by FactorInvertBlock, kN threads cooperating (thread
 // tid takes entries tid, tid + kN, ... in column-major order): the staging
 // slot (L below the diagonal, D on it, W above; identity padding beyond nb
-// in both) and, with `inv`, M itself (unit diagonal, zeros above).
+// in both), D alone into `d`, and, with `inv`, M itself (unit diagonal,
+// zeros above).
 template <int kLd, int kN>
-__device__ __forceinline__ void StoreFactored(const double* lt, const double* dvec, double* stage, double* inv,
-                                              int tid) {
+__device__ __forceinline__ void StoreFactored(const double* lt, const double* dvec, double* stage, double* d,
+                                              double* inv, int tid) {
+  if (tid < kNB) d[tid] = dvec[tid];
 #pragma unroll 4
   for (int e = 0; e < kNB * kNB / kN; ++e) {
     const int idx = tid + kN * e;
@@ -1712,7 +1720,7 @@ __global__ void __launch_bounds__(kThreads, kDiagBlockOccupancy<Slots>) KDiagBlo
   LoadDiagBlock<kLd>(F, f, c0, nb, lt);
   __syncthreads();
   const bool zero = FactorInvertBlock<kLd>(lt, dvec, dvec + kNB, nb);
-  StoreFactored<kLd, kThreads>(lt, dvec, StageSlot(W, s, kb), DiagInv(A, W, s, kb), tid);
+  StoreFactored<kLd, kThreads>(lt, dvec, StageSlot(W, s, kb), DiagD(A, W, s, kb), DiagInv(A, W, s, kb), tid);
   if (tid == 0 && zero) {
     W.state->not_spd = 1;
     W.state->halt = 1;
@@ -1811,7 +1819,10 @@ __global__ void __launch_bounds__(kThreads) KPanelStep(Slots slots, Lists plan, 
       const double m = at[j][i];
       wt[i][j] = i < j ? m * at[j][j] : (i == j ? m : 0.0);
     }
-    if (owner) StoreFactored<kLdT, kThreads>(&at[0][0], dsh, StageSlot(W, s, kb), DiagInv(A, W, s, kb), tid);
+    if (owner) {
+      StoreFactored<kLdT, kThreads>(&at[0][0], dsh, StageSlot(W, s, kb), DiagD(A, W, s, kb), DiagInv(A, W, s, kb),
+                                    tid);
+    }
     __syncthreads();  // at is free again
     if (rows) {
       load_rows();
@@ -2073,7 +2084,12 @@ __device__ __forceinline__ void SubtractLower(double* C, int64_t ldc, int R0, in
 // mode 0: trailing panel update after block column kb (K = nb), whose
 // factored diagonal block tile (0, 0)'s workgroup first moves into the
 // panel;
-// mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated.
+// mode 1: update matrix U = -L21 D L21^T (K = n_s), written, not accumulated;
+// D comes from the copy of every block's D (the front's last factored
+// diagonal block may still be in its staging slot), and in single-slot
+// launches tile (0, 0) then moves that block into the panel (no other tile
+// reads or writes it; where a launch of mode 0 moved it already, the same
+// values again).
 // Tile (ti, tj) (tj <= ti) of front s in slot `slot`.
 template <int kMode, class Slots>
 __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int s, int ti, int tj, int kb) {
@@ -2112,8 +2128,8 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
     N = ms;
     C = W.upd + A.upd_off[s];
     ldc = ms;
-    dk = Fw;  // D on the factored panel's diagonal
-    dstride = static_cast<int64_t>(f) + 1;
+    dk = DiagD(A, W, s, 0);
+    dstride = 1;
   }
   const int R0 = ti * kTile;
   const int C0 = tj * kTile;
@@ -2185,6 +2201,13 @@ __device__ __forceinline__ void GemmLowerBody(const Slots& slots, int slot, int 
           if (c < N && r < M && c <= r) C[static_cast<int64_t>(c) * ldc + r] = v[cb][e][rb];
         }
       }
+    }
+    // Tile (0, 0) then moves the front's last factored diagonal block from
+    // its staging slot into the panel (single-slot launches; the slot-table
+    // form keeps the launch of mode 0 for it, within its register budget).
+    if (std::is_same<Slots, OneSlot>::value && ti == 0 && tj == 0) {
+      const int c_last = ((ns - 1) / kNB) * kNB;
+      MoveStagedBlock(Fw, f, c_last, ns - c_last, StageSlot(W, s, c_last / kNB));
     }
   }
 }
@@ -2317,7 +2340,10 @@ __device__ __forceinline__ void AheadBody(const Slots& slots, int slot, int s, i
     W.state->not_spd = 1;
     W.state->halt = 1;
   }
-  if (ti == 0) StoreFactored<kAheadLd, kThreads>(lt, dvec, StageSlot(W, s, kb + 1), DiagInv(A, W, s, kb + 1), tid);
+  if (ti == 0) {
+    StoreFactored<kAheadLd, kThreads>(lt, dvec, StageSlot(W, s, kb + 1), DiagD(A, W, s, kb + 1),
+                                      DiagInv(A, W, s, kb + 1), tid);
+  }
   // L21 = A21 W, W(k, j) = M(j, k) / D(j) above the diagonal and 1 / D(j)
   // on it (lt holds M(j, k) at j * kAheadLd + k and 1 / D(j) at
   // j * kAheadLd + j), the same 64 x 64 x 64 FP64 MFMA product as the
@@ -3187,6 +3213,7 @@ void LaunchShape::Merge(const LaunchShape& o) {
     Level& a = levels[L];
     const Level& b = o.levels[L];
     a.count = std::max(a.count, b.count);
+    a.all_update = a.all_update && b.all_update;
     a.max_nb = std::max(a.max_nb, b.max_nb);
     a.max_f = std::max(a.max_f, b.max_f);
     a.max_m = std::max(a.max_m, b.max_m);
@@ -3271,9 +3298,11 @@ void NormalSlotsImpl(const SlotArgs& a, const LaunchShape& sh, const LmOptions& 
 // launch of their own instead of once per workgroup.
 constexpr int kStagedMinBlocks = 256;
 
-// `panels_done(L)` runs once level L's fronts have their final panels and
-// kept inverses (before their update matrices), where a forward solve of the
-// level may start.
+// `panels_done(L)` runs once level L's fronts have their final panel rows
+// and kept inverses (before their update matrices, whose launch also moves
+// each front's last diagonal block into its panel: the forward solve reads
+// the kept inverses, not those blocks), where a forward solve of the level
+// may start.
 template <bool kOne, class PanelsDone>
 void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEvents* step_events,
                 PanelsDone&& panels_done) {
@@ -3320,6 +3349,10 @@ void FactorImpl(const SlotArgs& a, const LaunchShape& sh, hipStream_t s, StepEve
             ++step_events->used;
           }
         }
+        // The last step has no trailing columns: it only moves the factored
+        // diagonal blocks into the panels, which a single-slot update-matrix
+        // launch does where every front has an update matrix.
+        if (kOne && kb + 1 == nblocks && lv.all_update && lv.tiles > 0) break;
         const int trail_rows = lv.max_f - (kb + 1) * kNB;
         const int trail_cols = lv.max_nb - (kb + 1) * kNB;
         // With no columns beyond the block in any front only tile (0, 0)

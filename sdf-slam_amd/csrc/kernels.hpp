@@ -108,6 +108,7 @@ struct DeviceWork {
   double* panel;
   double* diag_stage;  // factored 64x64 diagonal blocks in flight, two per supernode
   double* diag_inv;    // L_kk^-1 of every 64x64 diagonal block (identity padded), for the solves
+  double* diag_d;      // D of every 64x64 diagonal block (64 per block), for the update matrices
   double* upd;
   double* vec;  // forward-solve vectors, sum of front sizes
   double* xnew;  // solution in elimination order
@@ -201,6 +202,10 @@ struct LaunchShape {
   struct Level {
     int count{0}, max_nb{0}, max_f{0}, max_m{0};
     int tiles{0};  // 64 x 64 lower tiles of the fronts' update matrices
+    // Every front has an update matrix (in single-slot launches its tile
+    // (0, 0) then moves the front's last factored diagonal block into the
+    // panel).
+    bool all_update{true};
     bool has_children{false};
   };
   std::vector<Level> levels;
