@@ -913,7 +913,12 @@ Analysis Analyze(const sdfslam_problem_view& v, const double* map_values, const 
   for (int L = 0; L < a.num_levels; ++L) {
     if (!a.level_sn[static_cast<size_t>(L)].empty()) {
       a.panel_step_launches += 1;
-      a.gemm_launches += (a.level_max_nb[static_cast<size_t>(L)] + 63) / 64 +
+      // The last step's launch only moves the factored diagonal blocks
+      // into the panels; the update-matrix launch does that instead where
+      // every front of the level has an update matrix.
+      bool all_update = true;
+      for (const int s : a.level_sn[static_cast<size_t>(L)]) all_update = all_update && a.sn_m[static_cast<size_t>(s)] > 0;
+      a.gemm_launches += (a.level_max_nb[static_cast<size_t>(L)] + 63) / 64 - (all_update ? 1 : 0) +
                          (a.level_max_m[static_cast<size_t>(L)] > 0 ? 1 : 0);
     }
   }

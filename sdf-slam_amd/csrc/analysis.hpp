@@ -113,7 +113,8 @@ struct Analysis {
   int32_t panel_step_launches{0};
   double panel_step_flops{0.0};
   // Trailing-update and update-matrix launches per factorization (one per
-  // step and level, one per level with update matrices) and their flops:
+  // step and level but the last step of a level whose fronts all have
+  // update matrices, one per level with update matrices) and their flops:
   // 2 K per lower-triangle output entry, plus the next step's diagonal
   // block and TRSM that a trailing update does (as above, counted once).
   int32_t gemm_launches{0};
